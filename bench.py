@@ -1,0 +1,220 @@
+"""Benchmark of the pressure-Poisson solve on one MI355X.
+
+Workload (one step = one solve from a zero initial guess to 1e-8 relative
+residual, inputs resident in HBM):
+    single block 256^3, dx=dy=dz=1/256, periodic in x and y, Neumann in z,
+    RHS = div(u*)/dt (dt = 1e-3) of seeded N(0,1) face velocities plus
+    5 sin(2 pi x) cos(2 pi z) on u, mean removed; 6 levels, red-black
+    Gauss-Seidel V(2,2).
+The headline `value` is the mixed-precision solver (FP32 V-cycle with an
+FP64 coarsest-level solve inside FP64 iterative refinement); the plain FP64
+solver (the reference's algorithm, bitwise) is timed beside it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Poisson solves/s to 1e-8 rel. residual (mixed vs FP64), HBM GB/s, p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_solves(S, f_dev, zero_dev, steps, warmup):
+    import torch
+    S.set_rhs(0, f_dev)
+    lat = []
+    cycles = None
+    for it in range(warmup + steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        S.set_p(0, zero_dev)
+        st, ncyc, hist = S.solve(raise_on_fail=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if it >= warmup:
+            lat.append(t1 - t0)
+        cycles = ncyc
+        final = hist[-1]
+    return np.array(lat), cycles, final
+
+
+def smoother_roofline(S, level_cells, word, passes=40):
+    """Average duration of one finest-level colour half-sweep (k_rb_pass),
+    timed with HIP events on the stream the kernels run on."""
+    import torch
+    stream = torch.cuda.Stream()
+    S.set_stream(stream.cuda_stream)
+    nsweep = passes // 2
+    S.dbg_relax(1, 2)  # warm
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    S.dbg_relax(1, nsweep)  # 2*nsweep passes, even sweep count: no copy-back
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / (2 * nsweep)
+    S.set_stream(None)
+    # one colour pass: read the other colour (N/2), rhs of this colour (N/2),
+    # write this colour (N/2)
+    bytes_per_launch = 1.5 * word * level_cells
+    achieved = bytes_per_launch / (ms * 1e-3) / 1e9
+    return ms, bytes_per_launch, achieved
+
+
+def cpu_baseline(c, fp64_cycles):
+    """Reference Fortran (if built) else the C oracle, on one core, on a
+    bounded sample: a few V-cycles of the 256^3 workload; solves/s is
+    1 / (seconds per cycle * cycles the FP64 solve needs)."""
+    from oracle import oracle as O
+    B = O.Block(c["n"], c["h"], c["bc"], c["nlev"])
+    ncyc = 3
+    if O.ref_available():
+        try:
+            t0 = time.perf_counter()
+            zero = np.zeros(B.interior_shape(1))
+            ni, nj, nk = B.n
+            nout = 1 + 2 * (ncyc + 1) + (ni + 2) * (nj + 2) * (nk + 2) + 2 * ni * nj * nk
+            O.run_ref(B, "newsolv", [c["u"], c["v"], c["w"], zero], nout, ncycles=ncyc, dt=c["dt"])
+            t_all = time.perf_counter() - t0
+            # subtract the cost of setup + the initial calvel/calmas: one run
+            # with 0 cycles
+            t0 = time.perf_counter()
+            nout0 = 1 + 2 + (ni + 2) * (nj + 2) * (nk + 2) + 2 * ni * nj * nk
+            O.run_ref(B, "newsolv", [c["u"], c["v"], c["w"], zero], nout0, ncycles=0, dt=c["dt"])
+            t_setup = time.perf_counter() - t0
+            per_cycle = max(t_all - t_setup, 1e-9) / ncyc
+            return dict(value=1.0 / (per_cycle * fp64_cycles), unit="solves/s", cores=1,
+                        kind="reference",
+                        sample=f"{ncyc} velocity-form cycles of the reference newsolv_mg loop "
+                               f"(mgkcyc+calvel+calmas) at 256^3 on 1 core, {per_cycle:.3f} s/cycle, "
+                               f"scaled by the {fp64_cycles} cycles an FP64 solve takes")
+        except Exception as e:  # fall through to the restatement
+            log("reference CPU baseline failed:", e)
+    t0 = time.perf_counter()
+    O.solve(B, c["f"], ncyc, 0.0)
+    per_cycle = (time.perf_counter() - t0) / ncyc
+    return dict(value=1.0 / (per_cycle * fp64_cycles), unit="solves/s", cores=1, kind="port",
+                sample=f"{ncyc} pressure-form cycles of the C restatement at 256^3 on 1 core, "
+                       f"{per_cycle:.3f} s/cycle, scaled by the {fp64_cycles} cycles an FP64 "
+                       f"solve takes")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the solver is single-GPU (independent blocks shard "
+              "across GPUs without communication, see DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from hydro3d_amd import MultigridSolver, configs
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    c = configs.single_256()
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    N = int(np.prod(c["n"]))
+    f_dev = torch.from_numpy(c["f"]).cuda()
+    zero_dev = torch.zeros_like(f_dev)
+
+    log("building solvers")
+    S64 = MultigridSolver([spec], tol=1e-8, precision="fp64", coarse="smooth")
+    Smx = MultigridSolver([spec], tol=1e-8, precision="mixed", coarse="exact")
+
+    log("timing fp64")
+    lat64, cyc64, fin64 = timed_solves(S64, f_dev, zero_dev, args.steps, args.warmup)
+    log(f"fp64: {cyc64} cycles, {lat64.mean()*1e3:.2f} ms/solve, final {fin64:.2e}")
+    log("timing mixed")
+    latmx, cycmx, finmx = timed_solves(Smx, f_dev, zero_dev, args.steps, args.warmup)
+    log(f"mixed: {cycmx} cycles, {latmx.mean()*1e3:.2f} ms/solve, final {finmx:.2e}")
+
+    # accuracy of the mixed pressure against the FP64 solve (mean removed)
+    p64 = torch.empty_like(f_dev)
+    pmx = torch.empty_like(f_dev)
+    S64.get_p(0, out=p64)
+    Smx.get_p(0, out=pmx)
+    p64 -= p64.mean()
+    pmx -= pmx.mean()
+    perr = float(torch.linalg.norm(pmx - p64) / torch.linalg.norm(p64))
+
+    ms_pass, bytes_pass, achieved = smoother_roofline(Smx, N, 4)
+    log(f"smoother pass (fp32, 256^3): {ms_pass*1e3:.1f} us, {achieved:.0f} GB/s")
+    ms64, bytes64, ach64 = smoother_roofline(S64, N, 8)
+    log(f"smoother pass (fp64, 256^3): {ms64*1e3:.1f} us, {ach64:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(c, cyc64)
+        log(cpu)
+
+    S64.close()
+    Smx.close()
+    out = {
+        "metric": METRIC,
+        "value": round(1.0 / latmx.mean(), 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(latmx.mean() * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": "single block 256^3 (periodic x/y, Neumann z), seeded div(u*)/dt RHS "
+                        "+ 5 sin(2pi x)cos(2pi z) mode, 6 levels, RB-GS V(2,2), to 1e-8 "
+                        "relative L2 residual from a zero guess",
+            "mixed": "FP32 V-cycle, FP64 exact coarsest solve, FP64 refinement",
+            "fp64": "reference algorithm (coarsest level smoothed), bitwise the oracle",
+        },
+        "fp64_solves_per_s": round(1.0 / lat64.mean(), 3),
+        "speedup_mixed_vs_fp64": round(lat64.mean() / latmx.mean(), 3),
+        "cycles": {"mixed": cycmx, "fp64": cyc64},
+        "final_rel_residual": {"mixed": finmx, "fp64": fin64},
+        "latency_ms": {"mixed_p50": round(float(np.percentile(latmx, 50)) * 1e3, 3),
+                       "mixed_p99": round(float(np.percentile(latmx, 99)) * 1e3, 3),
+                       "fp64_p50": round(float(np.percentile(lat64, 50)) * 1e3, 3),
+                       "fp64_p99": round(float(np.percentile(lat64, 99)) * 1e3, 3)},
+        "pressure_rel_err_mixed_vs_fp64": perr,
+        "roofline": {"bound": "hbm", "kernel": "k_rb_pass<float> (256^3 colour half-sweep)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_us": round(ms_pass * 1e3, 2),
+                     "algorithmic_bytes_per_launch": int(bytes_pass),
+                     "fp64_pass": {"achieved": round(ach64, 1), "avg_launch_us": round(ms64 * 1e3, 2),
+                                   "frac": round(ach64 / HBM_PEAK_GBS, 4)}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

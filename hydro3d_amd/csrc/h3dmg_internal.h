@@ -1,0 +1,107 @@
+// h3dmg_internal.h -- device data layout and kernel launchers (not part of
+// the C-ABI).
+//
+// Layout in HBM.  Every level of every block stores its cells COLOUR-SPLIT:
+// the red cells ((i+j+k) even, 0-based interior indices -- MOD(i+j+k,2)==0 in
+// the reference's 1-based padded indices, src_v70/mgsolver.for:556) and the
+// black cells live in two separate arrays.  Cell (i,j,k) of colour c sits at
+//     ((k*nj + j)*H + (i >> 1))            H = row pitch >= ceil(ni/2)
+// so one colour pass streams whole cache lines of one array (3 words per cell
+// per red+black sweep instead of 6 for interleaved colours).
+//
+// There is no ghost storage.  Every ghost value the reference would hold after
+// mgbound + exchangepp (mgsolver.for:614-615, exchangepp.for:2830-2832) is
+// the value of one interior cell, its IMAGE: the mirror (first interior) at a
+// wall and the wrap (opposite interior) at a periodic face.  Kernels read
+// images directly ("virtual ghosts").  The one place where the reference's
+// ghosts are stale -- the black half-sweep reads ghosts from before the red
+// half-sweep, because exchangepp runs once per full sweep -- is reproduced by
+// double buffering: sweep s reads buffer b and writes buffer 1-b, and ghost
+// reads always go to buffer b.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace h3dmg {
+
+constexpr int TILE_X = 64;  // one wavefront along the contiguous axis
+constexpr int TILE_Y = 4;   // four wavefronts per workgroup
+constexpr int NTHREADS = TILE_X * TILE_Y;
+
+struct LevelDev {
+  int ni, nj, nk;  // interior cells
+  int H;           // colour row pitch (elements)
+  int per;         // bit a set: axis a periodic (a = 0 x, 1 y, 2 z)
+  int blk;         // block index
+  int finer;       // descriptor index of level g-1 (or -1)
+  int coarser;     // descriptor index of level g+1 (or -1)
+  double cx, cy, cz;  // 1/h_g^2 (coeff, mgsolver.for:54-60,185-191)
+  void *phi[2][2];    // [buffer][colour]
+  void *rhs[2];       // [colour]
+};
+
+// Finest-level FP64 state of one block (pressure, right-hand side).
+struct FineDev {
+  int ni, nj, nk, H, per, blk;
+  double cx, cy, cz;
+  double *p[2];  // [colour]
+  double *f[2];  // [colour]
+};
+
+// A workgroup's work item: descriptor index and the tile origin.
+struct Tile {
+  int lev;
+  int x0, y0, z0;
+};
+
+struct RedPartial {
+  double ss;    // sum of squares
+  double amax;  // max |value|
+};
+
+// ---- launchers (h3dmg_kernels.hip) --------------------------------------
+// T = storage type of the level (double or float).
+template <class T>
+void launch_rb_pass(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                    const int *active, int colour, int src, int zero);
+template <class Tf, class Tc>
+void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                     const int *active, int src_f);
+template <class Tf, class Tc>
+void launch_prolong(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                    const int *active, int src_f, int src_c);
+template <class T>
+void launch_fill_zero(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                      const int *active, int buf);
+// tiles of the finest level: Tile::lev is the BLOCK index b; fine[b] and
+// levs[b] (level-1 descriptors come first) describe it.  use_p = 0 gives
+// r = f (the norm of the right-hand side).
+template <class T>
+void launch_residual(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,
+                     const LevelDev *levs, const int *active, RedPartial *partials, int use_p);
+template <class T>
+void launch_update(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,
+                   const LevelDev *levs, const int *active, int buf);
+void launch_finalize(hipStream_t s, const int *tile_begin, int nblocks, const RedPartial *partials,
+                     RedPartial *out);
+// one workgroup per tile; Tile::lev = coarsest descriptor, Tile::x0 = block
+void launch_coarse_exact(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                         const int *active, const double *const *eig_q,
+                         const double *const *eig_l, double *const *scratch, int buf);
+
+// layout conversion: dense interior (i fastest) <-> colour split
+void launch_pack_fine(hipStream_t s, const FineDev *fd, int which, const double *src, int ni,
+                      int nj, int nk);
+void launch_unpack_fine(hipStream_t s, const FineDev *fd, int which, double *dst, int ni, int nj,
+                        int nk);
+template <class T>
+void launch_pack_level(hipStream_t s, const LevelDev *ld, int which, int buf, const double *src,
+                       int ni, int nj, int nk);
+template <class T>
+void launch_unpack_level(hipStream_t s, const LevelDev *ld, int which, int buf, double *dst,
+                         int ni, int nj, int nk);
+void launch_remove_mean(hipStream_t s, const FineDev *fd, int ni, int nj, int nk, double *scratch);
+
+}  // namespace h3dmg

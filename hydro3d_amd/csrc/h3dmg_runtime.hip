@@ -1,0 +1,1139 @@
+// h3dmg_runtime.hip -- host runtime behind the C-ABI of include/h3dmg.h.
+//
+// Owns the device arena (every level of every block), the tile tables the
+// kernels read, the V-cycle schedule of src_v70/mgsolver.for:226-372, the
+// pressure-form outer loop of src_v70/newsolv_mg.for:19-59 and the hipGraph
+// that replays one cycle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/h3dmg.h"
+#include "h3dmg_internal.h"
+
+using namespace h3dmg;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+h3dmg_status fail(h3dmg_status st, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return st;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(H3DMG_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                  \
+  } while (0)
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct TileSet {
+  std::vector<Tile> h;
+  Tile *d = nullptr;
+  int n() const { return (int)h.size(); }
+};
+
+// Append the tiles covering an (nx, ny, nz) index space of descriptor `lev`.
+void add_tiles(TileSet &ts, int lev, int nx, int ny, int nz) {
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; y += TILE_Y)
+      for (int x = 0; x < nx; x += TILE_X) ts.h.push_back(Tile{lev, x, y, z});
+}
+
+struct BlockInfo {
+  h3dmg_block spec;
+  int L;
+  int per;               // periodic axes bitmask
+  std::vector<int> desc; // descriptor index of level g (1-based, desc[0] unused)
+  double fnorm = 0.0;    // ||f||_2
+  int cycles = 0;
+  double rel = 0.0;
+  double rmax = 0.0;
+};
+
+}  // namespace
+
+struct h3dmg_handle {
+  h3dmg_params prm{};
+  int nb = 0;
+  int Lmax = 0;
+  std::vector<BlockInfo> blk;
+
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+
+  char *arena = nullptr;
+  size_t arena_bytes = 0;
+
+  std::vector<LevelDev> levs;  // level-major: all blocks' level 1 first
+  std::vector<int> lev_is64;   // storage type of each descriptor
+  LevelDev *d_levs = nullptr;
+  std::vector<FineDev> fine;
+  FineDev *d_fine = nullptr;
+  std::vector<int> active;
+  int *d_active = nullptr;
+
+  // tile tables, indexed by level g (1..Lmax)
+  std::vector<TileSet> relax_nc, relax_c;   // colour space; L_b > g / L_b == g
+  std::vector<TileSet> restr_nc, restr_c;   // coarse natural at g; L_b > g / == g
+  std::vector<TileSet> prol_nc, prol_c;     // fine natural at g; L_b > g+1 / == g+1
+  std::vector<TileSet> nat_nc, nat_c;       // natural at g (zero fill)
+  std::vector<TileSet> coarse_list;         // blocks whose coarsest level is g
+  TileSet fine_tiles;                       // natural level 1, lev = block
+  std::vector<int> tile_begin;              // per block range in fine_tiles
+  int *d_tile_begin = nullptr;
+  RedPartial *d_partials = nullptr;
+  RedPartial *d_stats = nullptr;
+  RedPartial *h_stats = nullptr;  // pinned
+
+  // exact coarse solve
+  double **d_eig_q = nullptr, **d_eig_l = nullptr, **d_scratch = nullptr;
+
+  double *d_stage = nullptr;  // dense FP64 staging (largest padded level 1)
+  size_t stage_elems = 0;
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool rhs_ready = false;
+  int pp_buf = 0;  // level-1 buffer that holds pp after the last V-cycle
+  unsigned long long bytes = 0;
+};
+
+namespace {
+
+bool is_mixed(const h3dmg_handle *h) { return h->prm.precision == H3DMG_MIXED; }
+
+template <class T>
+void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool &zero) {
+  for (int s = 0; s < nsweep; ++s) {
+    launch_rb_pass<T>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, 0, buf, zero ? 1 : 0);
+    launch_rb_pass<T>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, 1, buf, zero ? 1 : 0);
+    buf ^= 1;
+    zero = false;
+  }
+}
+
+void relax_any(h3dmg_handle *h, bool is64, const TileSet &ts, int nsweep, int &buf, bool &zero) {
+  if (ts.n() == 0) {
+    // keep the bookkeeping identical for empty groups
+    if (nsweep > 0) {
+      buf ^= (nsweep & 1);
+      zero = false;
+    }
+    return;
+  }
+  if (is64)
+    relax_group<double>(h, ts, nsweep, buf, zero);
+  else
+    relax_group<float>(h, ts, nsweep, buf, zero);
+}
+
+void fill_any(h3dmg_handle *h, bool is64, const TileSet &ts, int buf) {
+  if (is64)
+    launch_fill_zero<double>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, buf);
+  else
+    launch_fill_zero<float>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, buf);
+}
+
+// Enqueue one V-cycle (mgkcyc, mgsolver.for:272-352) on the level-1 rhs that
+// is already in place.  Per level the schedule is relax(g, nu1) on the way
+// down and relax(g, nu2) after mgcorr on the way up; at a block's coarsest
+// level relax(L, nu1) + relax(L, nu2) (the reference) or the exact solve.
+// Sweep counts per level are nu1 + nu2 for every block, so the buffer that
+// holds a level at mgcorr time is the same for all blocks.
+void enqueue_vcycle(h3dmg_handle *h) {
+  const int nu1 = h->prm.nu1, nu2 = h->prm.nu2, Lm = h->Lmax;
+  const bool mixed = is_mixed(h);
+  const bool exact = h->prm.coarse == H3DMG_COARSE_EXACT;
+  std::vector<int> buf_nc(Lm + 2, 0), buf_c(Lm + 2, 0);
+  std::vector<char> z_nc(Lm + 2, 1), z_c(Lm + 2, 1);
+  auto nc_is64 = [&](int) { return !mixed; };
+
+  for (int g = 1; g <= Lm; ++g) {
+    if (g >= 2) {
+      // mgrestr(g): residual of level g-1 (non-coarsest there) into rhs_g
+      const bool f64 = nc_is64(g - 1);
+      if (z_nc[g - 1]) {
+        fill_any(h, f64, h->nat_nc[g - 1], buf_nc[g - 1]);
+        z_nc[g - 1] = 0;
+      }
+      if (h->restr_nc[g].n()) {
+        if (f64)
+          launch_restrict<double, double>(h->stream, h->restr_nc[g].d, h->restr_nc[g].n(),
+                                          h->d_levs, h->d_active, buf_nc[g - 1]);
+        else
+          launch_restrict<float, float>(h->stream, h->restr_nc[g].d, h->restr_nc[g].n(),
+                                        h->d_levs, h->d_active, buf_nc[g - 1]);
+      }
+      if (h->restr_c[g].n()) {
+        if (f64)
+          launch_restrict<double, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(), h->d_levs,
+                                          h->d_active, buf_nc[g - 1]);
+        else
+          launch_restrict<float, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(),
+                                         h->d_levs, h->d_active, buf_nc[g - 1]);
+      }
+    }
+    // blocks whose coarsest level is g
+    if (h->relax_c[g].n()) {
+      bool z = true;
+      int b = 0;
+      if (exact && g > 1) {
+        const int out = (nu1 + nu2) & 1;
+        launch_coarse_exact(h->stream, h->coarse_list[g].d, h->coarse_list[g].n(), h->d_levs,
+                            h->d_active, (const double *const *)h->d_eig_q,
+                            (const double *const *)h->d_eig_l, h->d_scratch, out);
+        b = out;
+        z = false;
+      } else {
+        relax_any(h, true, h->relax_c[g], nu1, b, z);
+        relax_any(h, true, h->relax_c[g], nu2, b, z);
+        if (z) {  // nu1 = nu2 = 0
+          fill_any(h, true, h->nat_c[g], b);
+          z = false;
+        }
+      }
+      buf_c[g] = b;
+      z_c[g] = 0;
+    }
+    // pre-smoothing of the blocks that go on coarsening
+    if (g < Lm && h->relax_nc[g].n()) {
+      bool z = z_nc[g];
+      relax_any(h, nc_is64(g), h->relax_nc[g], nu1, buf_nc[g], z);
+      z_nc[g] = z;
+    }
+  }
+  for (int g = Lm - 1; g >= 1; --g) {
+    // mgcorr(g) for blocks with L_b > g
+    if (h->prol_nc[g].n() || h->prol_c[g].n()) {
+      const bool f64 = nc_is64(g);
+      if (z_nc[g]) {
+        fill_any(h, f64, h->nat_nc[g], buf_nc[g]);
+        z_nc[g] = 0;
+      }
+      if (h->prol_nc[g].n()) {
+        if (z_nc[g + 1]) {
+          fill_any(h, nc_is64(g + 1), h->nat_nc[g + 1], buf_nc[g + 1]);
+          z_nc[g + 1] = 0;
+        }
+        if (f64)
+          launch_prolong<double, double>(h->stream, h->prol_nc[g].d, h->prol_nc[g].n(), h->d_levs,
+                                         h->d_active, buf_nc[g], buf_nc[g + 1]);
+        else
+          launch_prolong<float, float>(h->stream, h->prol_nc[g].d, h->prol_nc[g].n(), h->d_levs,
+                                       h->d_active, buf_nc[g], buf_nc[g + 1]);
+      }
+      if (h->prol_c[g].n()) {
+        if (f64)
+          launch_prolong<double, double>(h->stream, h->prol_c[g].d, h->prol_c[g].n(), h->d_levs,
+                                         h->d_active, buf_nc[g], buf_c[g + 1]);
+        else
+          launch_prolong<float, double>(h->stream, h->prol_c[g].d, h->prol_c[g].n(), h->d_levs,
+                                        h->d_active, buf_nc[g], buf_c[g + 1]);
+      }
+      bool z = false;
+      relax_any(h, f64, h->relax_nc[g], nu2, buf_nc[g], z);
+    }
+  }
+  // the level-1 correction pp: blocks with L_b == 1 hold it in buf_c[1]
+  h->pp_buf = (h->relax_c[1].n() && !h->relax_nc[1].n()) ? buf_c[1] : buf_nc[1];
+}
+
+// p += pp, r = f - A p, per-block ||r||, max|r| (one outer iteration tail)
+void enqueue_update_residual(h3dmg_handle *h, bool update) {
+  const bool f64 = !is_mixed(h);
+  const TileSet &ft = h->fine_tiles;
+  if (update) {
+    if (f64)
+      launch_update<double>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active, h->pp_buf);
+    else
+      launch_update<float>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active, h->pp_buf);
+  }
+  if (f64)
+    launch_residual<double>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+                            h->d_partials, 1);
+  else
+    launch_residual<float>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+                           h->d_partials, 1);
+  launch_finalize(h->stream, h->d_tile_begin, h->nb, h->d_partials, h->d_stats);
+}
+
+h3dmg_status read_stats(h3dmg_handle *h) {
+  HIP_TRY(hipMemcpyAsync(h->h_stats, h->d_stats, sizeof(RedPartial) * h->nb,
+                         hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+h3dmg_status push_active(h3dmg_handle *h) {
+  HIP_TRY(hipMemcpyAsync(h->d_active, h->active.data(), sizeof(int) * h->nb,
+                         hipMemcpyHostToDevice, h->stream));
+  return H3DMG_OK;
+}
+
+// Closed-form eigenpairs of the 1-D operator c*(second difference) with
+// Neumann (mirror ghost, zero wall coefficient) or periodic ends, index 0 =
+// the constant mode with eigenvalue exactly 0.  Q is row-major Q[x*n + m].
+void eig1d(int n, bool periodic, double c, std::vector<double> &Q, std::vector<double> &lam) {
+  Q.assign((size_t)n * n, 0.0);
+  lam.assign(n, 0.0);
+  const double pi = 3.14159265358979323846;
+  if (!periodic) {
+    for (int m = 0; m < n; ++m) {
+      const double s = std::sin(pi * m / (2.0 * n));
+      lam[m] = m == 0 ? 0.0 : -4.0 * c * s * s;
+      const double nrm = m == 0 ? std::sqrt(1.0 / n) : std::sqrt(2.0 / n);
+      for (int x = 0; x < n; ++x) Q[(size_t)x * n + m] = nrm * std::cos(pi * m * (x + 0.5) / n);
+    }
+  } else {
+    int col = 0;
+    for (int x = 0; x < n; ++x) Q[(size_t)x * n + 0] = std::sqrt(1.0 / n);
+    lam[0] = 0.0;
+    col = 1;
+    for (int m = 1; 2 * m < n; ++m) {
+      const double s = std::sin(pi * m / n);
+      const double l = -4.0 * c * s * s;
+      for (int x = 0; x < n; ++x) {
+        Q[(size_t)x * n + col] = std::sqrt(2.0 / n) * std::cos(2.0 * pi * m * x / n);
+        Q[(size_t)x * n + col + 1] = std::sqrt(2.0 / n) * std::sin(2.0 * pi * m * x / n);
+      }
+      lam[col] = l;
+      lam[col + 1] = l;
+      col += 2;
+    }
+    if (n % 2 == 0) {
+      for (int x = 0; x < n; ++x) Q[(size_t)x * n + col] = (x % 2 ? -1.0 : 1.0) * std::sqrt(1.0 / n);
+      lam[col] = -4.0 * c;
+      col += 1;
+    }
+  }
+}
+
+h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
+  h->nb = nblocks;
+  h->blk.resize(nblocks);
+  h->Lmax = 0;
+  for (int b = 0; b < nblocks; ++b) {
+    BlockInfo &B = h->blk[b];
+    B.spec = blocks[b];
+    const int n[3] = {blocks[b].ni, blocks[b].nj, blocks[b].nk};
+    B.L = blocks[b].nlevels;
+    if (B.L < 1 || B.L > 16) return fail(H3DMG_ERR_ARG, "block %d: nlevels %d out of range", b, B.L);
+    for (int a = 0; a < 3; ++a) {
+      if (n[a] < 1) return fail(H3DMG_ERR_ARG, "block %d: n[%d] = %d", b, a, n[a]);
+      if (n[a] % (1 << (B.L - 1)) != 0)
+        return fail(H3DMG_ERR_LEVELS,
+                    "block %d: %d cells along axis %d not divisible by 2^(nlevels-1) = %d", b,
+                    n[a], a, 1 << (B.L - 1));
+    }
+    B.per = 0;
+    for (int a = 0; a < 3; ++a) {
+      const int lo = blocks[b].bc[2 * a], hi = blocks[b].bc[2 * a + 1];
+      if ((lo == H3DMG_BC_PERIODIC) != (hi == H3DMG_BC_PERIODIC))
+        return fail(H3DMG_ERR_ARG, "block %d: axis %d periodic on one face only", b, a);
+      if (lo == H3DMG_BC_PERIODIC) B.per |= 1 << a;
+    }
+    if (!(blocks[b].dx > 0 && blocks[b].dy > 0 && blocks[b].dz > 0))
+      return fail(H3DMG_ERR_ARG, "block %d: non-positive spacing", b);
+    if (is_mixed(h) && B.L < 2)
+      return fail(H3DMG_ERR_ARG, "block %d: mixed precision needs nlevels >= 2", b);
+    h->Lmax = std::max(h->Lmax, B.L);
+  }
+  const bool exact = h->prm.coarse == H3DMG_COARSE_EXACT;
+
+  // ---- descriptors and arena layout (level-major) ----
+  size_t off = 0;
+  struct Pending {
+    int d;
+    size_t phi[2][2], rhs[2];
+  };
+  std::vector<Pending> pend;
+  for (int g = 1; g <= h->Lmax; ++g) {
+    for (int b = 0; b < nblocks; ++b) {
+      BlockInfo &B = h->blk[b];
+      if (B.desc.empty()) B.desc.assign(B.L + 1, -1);
+      if (g > B.L) continue;
+      LevelDev L{};
+      L.ni = B.spec.ni >> (g - 1);
+      L.nj = B.spec.nj >> (g - 1);
+      L.nk = B.spec.nk >> (g - 1);
+      L.H = (L.ni + 1) / 2;
+      L.per = B.per;
+      L.blk = b;
+      const double s = (double)(1 << (g - 1));
+      const double ndx = B.spec.dx * s, ndy = B.spec.dy * s, ndz = B.spec.dz * s;
+      L.cx = 1.0 / (ndx * ndx);
+      L.cy = 1.0 / (ndy * ndy);
+      L.cz = 1.0 / (ndz * ndz);
+      const bool is64 = !is_mixed(h) || g == B.L;
+      const size_t w = is64 ? 8 : 4;
+      const size_t half = (size_t)L.H * L.nj * L.nk * w;
+      Pending P{};
+      P.d = (int)h->levs.size();
+      for (int bb = 0; bb < 2; ++bb)
+        for (int c = 0; c < 2; ++c) {
+          P.phi[bb][c] = off;
+          off = align_up(off + half, 256);
+        }
+      for (int c = 0; c < 2; ++c) {
+        P.rhs[c] = off;
+        off = align_up(off + half, 256);
+      }
+      B.desc[g] = P.d;
+      h->levs.push_back(L);
+      h->lev_is64.push_back(is64 ? 1 : 0);
+      pend.push_back(P);
+    }
+  }
+  // level-1 descriptors must be indices 0..nb-1
+  for (int b = 0; b < nblocks; ++b)
+    if (h->blk[b].desc[1] != b) return fail(H3DMG_ERR_ARG, "internal: level-1 order");
+  for (int b = 0; b < nblocks; ++b) {
+    BlockInfo &B = h->blk[b];
+    for (int g = 1; g <= B.L; ++g) {
+      LevelDev &L = h->levs[B.desc[g]];
+      L.finer = g > 1 ? B.desc[g - 1] : -1;
+      L.coarser = g < B.L ? B.desc[g + 1] : -1;
+    }
+  }
+  // finest-level FP64 p and f
+  std::vector<size_t> foff(nblocks * 4);
+  for (int b = 0; b < nblocks; ++b) {
+    const LevelDev &L = h->levs[b];
+    const size_t half = (size_t)L.H * L.nj * L.nk * 8;
+    for (int q = 0; q < 4; ++q) {
+      foff[4 * b + q] = off;
+      off = align_up(off + half, 256);
+    }
+  }
+  // exact coarse solve tables
+  std::vector<size_t> eoff;  // per block: Q0,Q1,Q2,l0,l1,l2,S0,S1
+  std::vector<std::vector<double>> edata;
+  if (exact) {
+    eoff.resize((size_t)nblocks * 8);
+    for (int b = 0; b < nblocks; ++b) {
+      const LevelDev &L = h->levs[h->blk[b].desc[h->blk[b].L]];
+      const int nn[3] = {L.ni, L.nj, L.nk};
+      const double cc[3] = {L.cx, L.cy, L.cz};
+      for (int a = 0; a < 3; ++a) {
+        std::vector<double> Q, lam;
+        eig1d(nn[a], (L.per >> a) & 1, cc[a], Q, lam);
+        eoff[8 * b + a] = off;
+        off = align_up(off + Q.size() * 8, 256);
+        edata.push_back(Q);
+        eoff[8 * b + 3 + a] = off;
+        off = align_up(off + lam.size() * 8, 256);
+        edata.push_back(lam);
+      }
+      const size_t n3 = (size_t)nn[0] * nn[1] * nn[2];
+      eoff[8 * b + 6] = off;
+      off = align_up(off + n3 * 8, 256);
+      eoff[8 * b + 7] = off;
+      off = align_up(off + n3 * 8, 256);
+    }
+  }
+  // staging: largest padded level-1 array
+  size_t stage = 0;
+  for (int b = 0; b < nblocks; ++b)
+    stage = std::max(stage, (size_t)(h->blk[b].spec.ni + 2) * (h->blk[b].spec.nj + 2) *
+                                (h->blk[b].spec.nk + 2));
+  h->stage_elems = stage;
+  const size_t stage_off = off;
+  off = align_up(off + stage * 8, 256);
+
+  // ---- tile tables ----
+  const int Lm = h->Lmax;
+  h->relax_nc.assign(Lm + 2, {});
+  h->relax_c.assign(Lm + 2, {});
+  h->restr_nc.assign(Lm + 2, {});
+  h->restr_c.assign(Lm + 2, {});
+  h->prol_nc.assign(Lm + 2, {});
+  h->prol_c.assign(Lm + 2, {});
+  h->nat_nc.assign(Lm + 2, {});
+  h->nat_c.assign(Lm + 2, {});
+  h->coarse_list.assign(Lm + 2, {});
+  for (int b = 0; b < nblocks; ++b)
+    h->coarse_list[h->blk[b].L].h.push_back(Tile{h->blk[b].desc[h->blk[b].L], b, 0, 0});
+  for (int g = 1; g <= Lm; ++g)
+    for (int b = 0; b < nblocks; ++b) {
+      const BlockInfo &B = h->blk[b];
+      if (g > B.L) continue;
+      const int d = B.desc[g];
+      const LevelDev &L = h->levs[d];
+      if (g < B.L) {
+        add_tiles(h->relax_nc[g], d, L.H, L.nj, L.nk);
+        add_tiles(h->nat_nc[g], d, L.ni, L.nj, L.nk);
+        if (g + 1 < B.L)
+          add_tiles(h->prol_nc[g], d, L.ni, L.nj, L.nk);
+        else
+          add_tiles(h->prol_c[g], d, L.ni, L.nj, L.nk);
+        if (g >= 2) add_tiles(h->restr_nc[g], d, L.ni, L.nj, L.nk);
+      } else {
+        add_tiles(h->relax_c[g], d, L.H, L.nj, L.nk);
+        add_tiles(h->nat_c[g], d, L.ni, L.nj, L.nk);
+        if (g >= 2) add_tiles(h->restr_c[g], d, L.ni, L.nj, L.nk);
+      }
+    }
+  h->tile_begin.assign(nblocks + 1, 0);
+  for (int b = 0; b < nblocks; ++b) {
+    const LevelDev &L = h->levs[b];
+    h->tile_begin[b] = h->fine_tiles.n();
+    add_tiles(h->fine_tiles, b, L.ni, L.nj, L.nk);
+  }
+  h->tile_begin[nblocks] = h->fine_tiles.n();
+
+  std::vector<TileSet *> all;
+  for (int g = 1; g <= Lm; ++g)
+    for (auto *v : {&h->relax_nc, &h->relax_c, &h->restr_nc, &h->restr_c, &h->prol_nc, &h->prol_c,
+                    &h->nat_nc, &h->nat_c, &h->coarse_list})
+      all.push_back(&(*v)[g]);
+  all.push_back(&h->fine_tiles);
+  std::vector<size_t> toff(all.size());
+  for (size_t q = 0; q < all.size(); ++q) {
+    toff[q] = off;
+    off = align_up(off + all[q]->h.size() * sizeof(Tile), 256);
+  }
+  const size_t levs_off = off;
+  off = align_up(off + h->levs.size() * sizeof(LevelDev), 256);
+  const size_t fine_off = off;
+  off = align_up(off + nblocks * sizeof(FineDev), 256);
+  const size_t act_off = off;
+  off = align_up(off + nblocks * sizeof(int), 256);
+  const size_t tb_off = off;
+  off = align_up(off + (nblocks + 1) * sizeof(int), 256);
+  const size_t part_off = off;
+  off = align_up(off + std::max<size_t>(1, h->fine_tiles.h.size()) * sizeof(RedPartial), 256);
+  const size_t stats_off = off;
+  off = align_up(off + nblocks * sizeof(RedPartial), 256);
+  const size_t eptr_off = off;
+  off = align_up(off + (size_t)nblocks * 8 * sizeof(double *), 256);
+
+  h->arena_bytes = off;
+  hipError_t e = hipMalloc((void **)&h->arena, off);
+  if (e != hipSuccess)
+    return fail(H3DMG_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", off, hipGetErrorString(e));
+  h->bytes = off;
+  char *A = h->arena;
+
+  for (const Pending &P : pend) {
+    LevelDev &L = h->levs[P.d];
+    for (int bb = 0; bb < 2; ++bb)
+      for (int c = 0; c < 2; ++c) L.phi[bb][c] = A + P.phi[bb][c];
+    for (int c = 0; c < 2; ++c) L.rhs[c] = A + P.rhs[c];
+  }
+  h->fine.resize(nblocks);
+  for (int b = 0; b < nblocks; ++b) {
+    const LevelDev &L = h->levs[b];
+    FineDev F{};
+    F.ni = L.ni;
+    F.nj = L.nj;
+    F.nk = L.nk;
+    F.H = L.H;
+    F.per = L.per;
+    F.blk = b;
+    F.cx = L.cx;
+    F.cy = L.cy;
+    F.cz = L.cz;
+    F.p[0] = (double *)(A + foff[4 * b + 0]);
+    F.p[1] = (double *)(A + foff[4 * b + 1]);
+    F.f[0] = (double *)(A + foff[4 * b + 2]);
+    F.f[1] = (double *)(A + foff[4 * b + 3]);
+    h->fine[b] = F;
+  }
+  h->d_stage = (double *)(A + stage_off);
+  for (size_t q = 0; q < all.size(); ++q) {
+    all[q]->d = (Tile *)(A + toff[q]);
+    if (!all[q]->h.empty())
+      HIP_TRY(hipMemcpy(all[q]->d, all[q]->h.data(), all[q]->h.size() * sizeof(Tile),
+                        hipMemcpyHostToDevice));
+  }
+  h->d_levs = (LevelDev *)(A + levs_off);
+  HIP_TRY(hipMemcpy(h->d_levs, h->levs.data(), h->levs.size() * sizeof(LevelDev),
+                    hipMemcpyHostToDevice));
+  h->d_fine = (FineDev *)(A + fine_off);
+  HIP_TRY(hipMemcpy(h->d_fine, h->fine.data(), nblocks * sizeof(FineDev), hipMemcpyHostToDevice));
+  h->active.assign(nblocks, 1);
+  h->d_active = (int *)(A + act_off);
+  HIP_TRY(hipMemcpy(h->d_active, h->active.data(), nblocks * sizeof(int), hipMemcpyHostToDevice));
+  h->d_tile_begin = (int *)(A + tb_off);
+  HIP_TRY(hipMemcpy(h->d_tile_begin, h->tile_begin.data(), (nblocks + 1) * sizeof(int),
+                    hipMemcpyHostToDevice));
+  h->d_partials = (RedPartial *)(A + part_off);
+  h->d_stats = (RedPartial *)(A + stats_off);
+  HIP_TRY(hipHostMalloc((void **)&h->h_stats, nblocks * sizeof(RedPartial), hipHostMallocDefault));
+  if (exact) {
+    std::vector<double *> ptrs((size_t)nblocks * 8);
+    size_t ed = 0;
+    for (int b = 0; b < nblocks; ++b) {
+      for (int a = 0; a < 3; ++a) {
+        ptrs[8 * b + a] = (double *)(A + eoff[8 * b + a]);
+        HIP_TRY(hipMemcpy(ptrs[8 * b + a], edata[ed].data(), edata[ed].size() * 8,
+                          hipMemcpyHostToDevice));
+        ++ed;
+        ptrs[8 * b + 3 + a] = (double *)(A + eoff[8 * b + 3 + a]);
+        HIP_TRY(hipMemcpy(ptrs[8 * b + 3 + a], edata[ed].data(), edata[ed].size() * 8,
+                          hipMemcpyHostToDevice));
+        ++ed;
+      }
+      ptrs[8 * b + 6] = (double *)(A + eoff[8 * b + 6]);
+      ptrs[8 * b + 7] = (double *)(A + eoff[8 * b + 7]);
+    }
+    // the kernel indexes q/l tables with stride 3 and scratch with stride 2
+    std::vector<double *> q3(3 * nblocks), l3(3 * nblocks), s2(2 * nblocks);
+    for (int b = 0; b < nblocks; ++b) {
+      for (int a = 0; a < 3; ++a) {
+        q3[3 * b + a] = ptrs[8 * b + a];
+        l3[3 * b + a] = ptrs[8 * b + 3 + a];
+      }
+      s2[2 * b] = ptrs[8 * b + 6];
+      s2[2 * b + 1] = ptrs[8 * b + 7];
+    }
+    double **base = (double **)(A + eptr_off);
+    h->d_eig_q = base;
+    h->d_eig_l = base + 3 * nblocks;
+    h->d_scratch = base + 6 * nblocks;
+    HIP_TRY(hipMemcpy(h->d_eig_q, q3.data(), q3.size() * sizeof(double *), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->d_eig_l, l3.data(), l3.size() * sizeof(double *), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->d_scratch, s2.data(), s2.size() * sizeof(double *),
+                      hipMemcpyHostToDevice));
+  }
+  // p = 0, f = 0
+  for (int b = 0; b < nblocks; ++b) {
+    const LevelDev &L = h->levs[b];
+    const size_t half = (size_t)L.H * L.nj * L.nk * 8;
+    for (int q = 0; q < 4; ++q) HIP_TRY(hipMemset(A + foff[4 * b + q], 0, half));
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  return H3DMG_OK;
+}
+
+h3dmg_status check_block(const h3dmg_handle *h, int b) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  if (b < 0 || b >= h->nb) return fail(H3DMG_ERR_ARG, "block index %d out of range", b);
+  return H3DMG_OK;
+}
+
+void drop_graph(h3dmg_handle *h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+}
+
+// One outer iteration: V-cycle on the current level-1 rhs, p += pp, new
+// residual and per-block statistics.
+h3dmg_status run_cycle(h3dmg_handle *h) {
+  if (h->prm.use_graph) {
+    if (!h->gexec) {
+      HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_vcycle(h);
+      enqueue_update_residual(h, true);
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamEndCapture(h->stream, &g));
+      h->graph = g;
+      HIP_TRY(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(h->gexec, h->stream));
+  } else {
+    enqueue_vcycle(h);
+    enqueue_update_residual(h, true);
+  }
+  HIP_TRY(hipGetLastError());
+  return H3DMG_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+void h3dmg_default_params(h3dmg_params *p) {
+  if (!p) return;
+  p->smoother = H3DMG_SMOOTH_RBGS;
+  p->nu1 = 2;
+  p->nu2 = 2;
+  p->maxcy = 60;
+  p->criterion = H3DMG_CRIT_REL_L2;
+  p->tol = 1e-8;
+  p->dt = 1e-3;
+  p->precision = H3DMG_FP64;
+  p->coarse = H3DMG_COARSE_SMOOTH;
+  p->min_cycles = 1;
+  p->use_graph = 1;
+}
+
+const char *h3dmg_last_error(void) { return g_last_error.c_str(); }
+
+h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_params *prm,
+                          h3dmg_handle **out) {
+  if (!out) return fail(H3DMG_ERR_ARG, "null output handle pointer");
+  *out = nullptr;
+  if (!blocks || nblocks < 1) return fail(H3DMG_ERR_ARG, "no blocks");
+  h3dmg_handle *h = new h3dmg_handle();
+  if (prm)
+    h->prm = *prm;
+  else
+    h3dmg_default_params(&h->prm);
+  const h3dmg_params &P = h->prm;
+  if (P.smoother != H3DMG_SMOOTH_RBGS) {
+    delete h;
+    return fail(H3DMG_ERR_UNSUPPORTED, "device smoother %d not available (RBGS = 3 only)",
+                P.smoother);
+  }
+  if (P.nu1 < 0 || P.nu2 < 0 || P.nu1 + P.nu2 < 1 || P.maxcy < 0 ||
+      (P.precision != H3DMG_FP64 && P.precision != H3DMG_MIXED) ||
+      (P.coarse != H3DMG_COARSE_SMOOTH && P.coarse != H3DMG_COARSE_EXACT) ||
+      (P.criterion != H3DMG_CRIT_REL_L2 && P.criterion != H3DMG_CRIT_REF_RMAX)) {
+    delete h;
+    return fail(H3DMG_ERR_ARG, "invalid parameters");
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    delete h;
+    return fail(H3DMG_ERR_DEVICE, "no HIP device");
+  }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(H3DMG_ERR_DEVICE, "hipStreamCreate failed");
+  }
+  h->own_stream = true;
+  h3dmg_status st = build(h, blocks, nblocks);
+  if (st != H3DMG_OK) {
+    h3dmg_destroy(h);
+    return st;
+  }
+  *out = h;
+  return H3DMG_OK;
+}
+
+void h3dmg_destroy(h3dmg_handle *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graph(h);
+  if (h->h_stats) (void)hipHostFree(h->h_stats);
+  if (h->arena) (void)hipFree(h->arena);
+  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+unsigned long long h3dmg_device_bytes(const h3dmg_handle *h) { return h ? h->bytes : 0; }
+
+h3dmg_status h3dmg_set_stream(h3dmg_handle *h, void *stream) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  drop_graph(h);
+  if (h->own_stream) (void)hipStreamDestroy(h->stream);
+  if (stream) {
+    h->stream = (hipStream_t)stream;
+    h->own_stream = false;
+  } else {
+    HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->own_stream = true;
+  }
+  return H3DMG_OK;
+}
+
+// host Fortran array (i fastest, leading dims ld1, ld2, interior at off) ->
+// dense interior staging on the device
+static h3dmg_status host_to_stage(h3dmg_handle *h, int b, const double *src, int ld1, int ld2,
+                                  int off) {
+  const h3dmg_block &S = h->blk[b].spec;
+  if (!src || ld1 < S.ni + off || ld2 < S.nj + off || off < 0)
+    return fail(H3DMG_ERR_ARG, "bad host array description");
+  std::vector<double> dense((size_t)S.ni * S.nj * S.nk);
+  for (int k = 0; k < S.nk; ++k)
+    for (int j = 0; j < S.nj; ++j)
+      std::memcpy(&dense[((size_t)k * S.nj + j) * S.ni],
+                  src + (size_t)off + (size_t)ld1 * ((size_t)(j + off) + (size_t)ld2 * (k + off)),
+                  sizeof(double) * S.ni);
+  HIP_TRY(hipMemcpyAsync(h->d_stage, dense.data(), dense.size() * 8, hipMemcpyHostToDevice,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+// padded dense (n+2)^3 staging -> host Fortran array incl. the ghost layer
+static h3dmg_status stage_padded_to_host(h3dmg_handle *h, int b, double *dst, int ld1, int ld2,
+                                         int off) {
+  const h3dmg_block &S = h->blk[b].spec;
+  if (!dst || off < 1 || ld1 < S.ni + off + 1 || ld2 < S.nj + off + 1)
+    return fail(H3DMG_ERR_ARG, "bad host array description (needs one ghost layer)");
+  const size_t pi = S.ni + 2, pj = S.nj + 2, pk = S.nk + 2;
+  std::vector<double> dense(pi * pj * pk);
+  HIP_TRY(hipMemcpyAsync(dense.data(), h->d_stage, dense.size() * 8, hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (size_t k = 0; k < pk; ++k)
+    for (size_t j = 0; j < pj; ++j)
+      std::memcpy(dst + (size_t)(off - 1) +
+                      (size_t)ld1 * ((j + off - 1) + (size_t)ld2 * (k + off - 1)),
+                  &dense[(k * pj + j) * pi], sizeof(double) * pi);
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_set_rhs_device(h3dmg_handle *h, int b, const double *d_su) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!d_su) return fail(H3DMG_ERR_ARG, "null rhs");
+  const FineDev &F = h->fine[b];
+  launch_pack_fine(h->stream, h->d_fine + b, 1, d_su, F.ni, F.nj, F.nk);
+  // ||f||: residual with use_p = 0 over this block only
+  std::vector<int> act(h->nb, 0);
+  act[b] = 1;
+  HIP_TRY(hipMemcpyAsync(h->d_active, act.data(), sizeof(int) * h->nb, hipMemcpyHostToDevice,
+                         h->stream));
+  const TileSet &ft = h->fine_tiles;
+  if (is_mixed(h))
+    launch_residual<float>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+                           h->d_partials, 0);
+  else
+    launch_residual<double>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+                            h->d_partials, 0);
+  launch_finalize(h->stream, h->d_tile_begin, h->nb, h->d_partials, h->d_stats);
+  HIP_TRY(hipGetLastError());
+  st = read_stats(h);
+  if (st) return st;
+  h->blk[b].fnorm = std::sqrt(h->h_stats[b].ss);
+  st = push_active(h);
+  if (st) return st;
+  h->rhs_ready = true;
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_set_rhs(h3dmg_handle *h, int b, const double *su, int ld1, int ld2, int off) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  st = host_to_stage(h, b, su, ld1, ld2, off);
+  if (st) return st;
+  return h3dmg_set_rhs_device(h, b, h->d_stage);
+}
+
+h3dmg_status h3dmg_set_p_device(h3dmg_handle *h, int b, const double *d_p) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!d_p) return fail(H3DMG_ERR_ARG, "null p");
+  const FineDev &F = h->fine[b];
+  launch_pack_fine(h->stream, h->d_fine + b, 0, d_p, F.ni, F.nj, F.nk);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_get_p_device(h3dmg_handle *h, int b, double *d_p) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!d_p) return fail(H3DMG_ERR_ARG, "null p");
+  const FineDev &F = h->fine[b];
+  launch_unpack_fine(h->stream, h->d_fine + b, 0, d_p, F.ni, F.nj, F.nk);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_set_p(h3dmg_handle *h, int b, const double *p, int ld1, int ld2, int off) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  st = host_to_stage(h, b, p, ld1, ld2, off);
+  if (st) return st;
+  return h3dmg_set_p_device(h, b, h->d_stage);
+}
+
+h3dmg_status h3dmg_get_p(h3dmg_handle *h, int b, double *p, int ld1, int ld2, int off) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  // unpack interior, then build the ghost layer (images) on the host
+  const h3dmg_block &S = h->blk[b].spec;
+  const FineDev &F = h->fine[b];
+  launch_unpack_fine(h->stream, h->d_fine + b, 0, h->d_stage, F.ni, F.nj, F.nk);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> in((size_t)S.ni * S.nj * S.nk);
+  HIP_TRY(hipMemcpyAsync(in.data(), h->d_stage, in.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (!p || off < 1 || ld1 < S.ni + off + 1 || ld2 < S.nj + off + 1)
+    return fail(H3DMG_ERR_ARG, "bad host array description (needs one ghost layer)");
+  const int per = h->blk[b].per;
+  auto img = [](int x, int n, bool periodic) {
+    if (x < 0) return periodic ? n - 1 : 0;
+    if (x >= n) return periodic ? 0 : n - 1;
+    return x;
+  };
+  for (int k = -1; k <= S.nk; ++k)
+    for (int j = -1; j <= S.nj; ++j)
+      for (int i = -1; i <= S.ni; ++i) {
+        const int a = img(i, S.ni, per & 1), c = img(j, S.nj, per & 2), d = img(k, S.nk, per & 4);
+        p[(size_t)(i + off) + (size_t)ld1 * ((size_t)(j + off) + (size_t)ld2 * (size_t)(k + off))] =
+            in[(size_t)a + (size_t)S.ni * ((size_t)c + (size_t)S.nj * d)];
+      }
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_vcycle(h3dmg_handle *h) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  if (!h->rhs_ready) return fail(H3DMG_ERR_ARG, "rhs not set");
+  // level-1 rhs <- f (su), pp = 0
+  std::fill(h->active.begin(), h->active.end(), 1);
+  h3dmg_status st = push_active(h);
+  if (st) return st;
+  const TileSet &ft = h->fine_tiles;
+  if (is_mixed(h))
+    launch_residual<float>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+                           h->d_partials, 0);
+  else
+    launch_residual<double>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+                            h->d_partials, 0);
+  enqueue_vcycle(h);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_get_pp(h3dmg_handle *h, int b, double *pp, int ld1, int ld2, int off) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  const LevelDev &L = h->levs[b];
+  if (h->lev_is64[b])
+    launch_unpack_level<double>(h->stream, h->d_levs + b, 0, h->pp_buf, h->d_stage, L.ni, L.nj,
+                                L.nk);
+  else
+    launch_unpack_level<float>(h->stream, h->d_levs + b, 0, h->pp_buf, h->d_stage, L.ni, L.nj,
+                               L.nk);
+  HIP_TRY(hipGetLastError());
+  return stage_padded_to_host(h, b, pp, ld1, ld2, off);
+}
+
+h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycles,
+                         double *final_res) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  if (!h->rhs_ready) return fail(H3DMG_ERR_ARG, "rhs not set");
+  const h3dmg_params &P = h->prm;
+  const bool ref = P.criterion == H3DMG_CRIT_REF_RMAX;
+  const double rfac_base = P.dt;
+  std::fill(h->active.begin(), h->active.end(), 1);
+  h3dmg_status st = push_active(h);
+  if (st) return st;
+  for (auto &B : h->blk) {
+    B.cycles = 0;
+    B.rel = 0.0;
+  }
+  enqueue_update_residual(h, false);
+  HIP_TRY(hipGetLastError());
+  st = read_stats(h);
+  if (st) return st;
+
+  auto measure = [&](int it, double &worst, bool &all_done, bool &bad) {
+    worst = 0.0;
+    all_done = true;
+    bad = false;
+    double gmax = 0.0;
+    for (int b = 0; b < h->nb; ++b) {
+      BlockInfo &B = h->blk[b];
+      const double nrm = std::sqrt(h->h_stats[b].ss);
+      const double vol = B.spec.dx * B.spec.dy * B.spec.dz;
+      if (h->active[b]) {
+        B.rel = B.fnorm > 0 ? nrm / B.fnorm : nrm;
+        B.rmax = h->h_stats[b].amax * rfac_base * vol;
+        B.cycles = it;
+      }
+      if (!std::isfinite(nrm)) bad = true;
+      worst = std::max(worst, B.rel);
+      gmax = std::max(gmax, B.rmax);
+    }
+    for (int b = 0; b < h->nb; ++b) {
+      BlockInfo &B = h->blk[b];
+      bool done;
+      if (ref)
+        done = it >= std::max(P.min_cycles, 2) && gmax < P.tol;  // newsolv_mg.for:57, global
+      else
+        done = it >= P.min_cycles && B.rel <= P.tol;
+      if (done) h->active[b] = 0;
+      if (h->active[b]) all_done = false;
+    }
+    if (ref) worst = gmax;
+  };
+
+  double worst;
+  bool all_done, bad;
+  measure(0, worst, all_done, bad);
+  if (hist && hist_cap > 0) hist[0] = worst;
+  if (P.min_cycles > 0 || ref) {
+    std::fill(h->active.begin(), h->active.end(), 1);
+    all_done = false;
+  }
+  int it = 0;
+  while (!all_done && it < P.maxcy && !bad) {
+    st = push_active(h);
+    if (st) return st;
+    st = run_cycle(h);
+    if (st) return st;
+    st = read_stats(h);
+    if (st) return st;
+    ++it;
+    measure(it, worst, all_done, bad);
+    if (hist && it < hist_cap) hist[it] = worst;
+  }
+  if (ncycles) *ncycles = it;
+  if (final_res) *final_res = worst;
+  if (bad) return fail(H3DMG_ERR_DIVERGED, "non-finite residual after %d cycles", it);
+  if (ref && worst > 10.0) return fail(H3DMG_ERR_DIVERGED, "rmax %g > 10", worst);
+  if (!all_done) return fail(H3DMG_ERR_NOT_CONVERGED, "not converged in %d cycles (%g)", it, worst);
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_block_stats(const h3dmg_handle *h, int *cycles, double *rel_res) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  for (int b = 0; b < h->nb; ++b) {
+    if (cycles) cycles[b] = h->blk[b].cycles;
+    if (rel_res) rel_res[b] = h->blk[b].rel;
+  }
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_remove_mean(h3dmg_handle *h) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  for (int b = 0; b < h->nb; ++b) {
+    const FineDev &F = h->fine[b];
+    launch_remove_mean(h->stream, h->d_fine + b, F.ni, F.nj, F.nk, nullptr);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+// ---- test seams ----------------------------------------------------------
+h3dmg_status h3dmg_dbg_put(h3dmg_handle *h, int b, int level, int which, const double *d_src) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (level < 1 || level > h->blk[b].L || !d_src) return fail(H3DMG_ERR_ARG, "bad level/ptr");
+  const int d = h->blk[b].desc[level];
+  const LevelDev &L = h->levs[d];
+  if (h->lev_is64[d])
+    launch_pack_level<double>(h->stream, h->d_levs + d, which, 0, d_src, L.ni, L.nj, L.nk);
+  else
+    launch_pack_level<float>(h->stream, h->d_levs + d, which, 0, d_src, L.ni, L.nj, L.nk);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_dbg_get(h3dmg_handle *h, int b, int level, int which, double *d_dst) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (level < 1 || level > h->blk[b].L || !d_dst) return fail(H3DMG_ERR_ARG, "bad level/ptr");
+  const int d = h->blk[b].desc[level];
+  const LevelDev &L = h->levs[d];
+  if (h->lev_is64[d])
+    launch_unpack_level<double>(h->stream, h->d_levs + d, which, 0, d_dst, L.ni, L.nj, L.nk);
+  else
+    launch_unpack_level<float>(h->stream, h->d_levs + d, which, 0, d_dst, L.ni, L.nj, L.nk);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+// The seams keep every level's data in buffer 0: an odd sweep count is
+// followed by a copy back (a second pass of the smoother would change it).
+h3dmg_status h3dmg_dbg_relax(h3dmg_handle *h, int level, int nsweep) {
+  if (!h || level < 1 || level > h->Lmax || nsweep < 0) return fail(H3DMG_ERR_ARG, "bad level");
+  std::fill(h->active.begin(), h->active.end(), 1);
+  h3dmg_status st = push_active(h);
+  if (st) return st;
+  for (int grp = 0; grp < 2; ++grp) {
+    const TileSet &ts = grp == 0 ? h->relax_nc[level] : h->relax_c[level];
+    if (!ts.n()) continue;
+    const bool is64 = !is_mixed(h) || grp == 1;
+    int buf = 0;
+    bool z = false;
+    relax_any(h, is64, ts, nsweep, buf, z);
+    if (buf == 1) {
+      // copy buffer 1 -> 0 for every block of this group
+      for (int b = 0; b < h->nb; ++b) {
+        if (h->blk[b].L < level) continue;
+        if ((grp == 1) != (h->blk[b].L == level)) continue;
+        const int d = h->blk[b].desc[level];
+        const LevelDev &L = h->levs[d];
+        const size_t half = (size_t)L.H * L.nj * L.nk * (h->lev_is64[d] ? 8 : 4);
+        for (int c = 0; c < 2; ++c)
+          HIP_TRY(hipMemcpyAsync(L.phi[0][c], L.phi[1][c], half, hipMemcpyDeviceToDevice,
+                                 h->stream));
+      }
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_dbg_restrict(h3dmg_handle *h, int level) {
+  if (!h || level < 2 || level > h->Lmax) return fail(H3DMG_ERR_ARG, "bad level");
+  std::fill(h->active.begin(), h->active.end(), 1);
+  h3dmg_status st = push_active(h);
+  if (st) return st;
+  const bool f64 = !is_mixed(h);
+  if (h->restr_nc[level].n()) {
+    if (f64)
+      launch_restrict<double, double>(h->stream, h->restr_nc[level].d, h->restr_nc[level].n(),
+                                      h->d_levs, h->d_active, 0);
+    else
+      launch_restrict<float, float>(h->stream, h->restr_nc[level].d, h->restr_nc[level].n(),
+                                    h->d_levs, h->d_active, 0);
+  }
+  if (h->restr_c[level].n()) {
+    if (f64)
+      launch_restrict<double, double>(h->stream, h->restr_c[level].d, h->restr_c[level].n(),
+                                      h->d_levs, h->d_active, 0);
+    else
+      launch_restrict<float, double>(h->stream, h->restr_c[level].d, h->restr_c[level].n(),
+                                     h->d_levs, h->d_active, 0);
+  }
+  // mgrestr zeroes the coarse phi (mgsolver.for:655-662)
+  if (h->nat_nc[level].n()) fill_any(h, f64, h->nat_nc[level], 0);
+  if (h->nat_c[level].n()) fill_any(h, true, h->nat_c[level], 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_dbg_prolong(h3dmg_handle *h, int level) {
+  if (!h || level < 1 || level >= h->Lmax) return fail(H3DMG_ERR_ARG, "bad level");
+  std::fill(h->active.begin(), h->active.end(), 1);
+  h3dmg_status st = push_active(h);
+  if (st) return st;
+  const bool f64 = !is_mixed(h);
+  if (h->prol_nc[level].n()) {
+    if (f64)
+      launch_prolong<double, double>(h->stream, h->prol_nc[level].d, h->prol_nc[level].n(),
+                                     h->d_levs, h->d_active, 0, 0);
+    else
+      launch_prolong<float, float>(h->stream, h->prol_nc[level].d, h->prol_nc[level].n(),
+                                   h->d_levs, h->d_active, 0, 0);
+  }
+  if (h->prol_c[level].n()) {
+    if (f64)
+      launch_prolong<double, double>(h->stream, h->prol_c[level].d, h->prol_c[level].n(),
+                                     h->d_levs, h->d_active, 0, 0);
+    else
+      launch_prolong<float, double>(h->stream, h->prol_c[level].d, h->prol_c[level].n(),
+                                    h->d_levs, h->d_active, 0, 0);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,179 @@
+"""Python mirror of the solver interface (the C-ABI of include/h3dmg.h).
+
+The reference drives the solve through argument-less Fortran subroutines on
+module state (`newsolv_mg`, src_v70/newsolv_mg.for:2-93; `mgkcyc`,
+src_v70/mgsolver.for:226-372).  `MultigridSolver` holds the same state
+explicitly: one handle with every block's multigrid hierarchy resident in HBM.
+
+Array convention: numpy arrays are C-ordered (nk, nj, ni) -- the memory of the
+reference's Fortran (i, j, k) arrays.  Device tensors (torch, on the GPU) are
+passed by pointer and must be dense float64 of the same shape.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+_PREC = {"fp64": L.FP64, "mixed": L.MIXED}
+_COARSE = {"smooth": L.COARSE_SMOOTH, "exact": L.COARSE_EXACT}
+_CRIT = {"rel_l2": L.CRIT_REL_L2, "ref_rmax": L.CRIT_REF_RMAX}
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _is_device_tensor(x) -> bool:
+    return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
+
+
+class MultigridSolver:
+    """Independent blocks, each given as dict(n=(ni,nj,nk), h=(dx,dy,dz),
+    bc=(w,e,s,n,b,t) with 0 wall / 1 periodic, nlev=levels)."""
+
+    def __init__(self, blocks, nu1=2, nu2=2, maxcy=60, tol=1e-8, criterion="rel_l2",
+                 precision="fp64", coarse="smooth", min_cycles=1, use_graph=True, dt=1e-3):
+        self.lib = L.load()
+        nb = len(blocks)
+        arr = (L.H3dBlock * nb)()
+        self.blocks = []
+        for q, b in enumerate(blocks):
+            n = tuple(int(x) for x in b["n"])
+            arr[q].ni, arr[q].nj, arr[q].nk = n
+            arr[q].dx, arr[q].dy, arr[q].dz = (float(x) for x in b["h"])
+            arr[q].nlevels = int(b["nlev"])
+            arr[q].bc[:] = [int(x) for x in b["bc"]]
+            self.blocks.append(dict(n=n, h=tuple(b["h"]), bc=tuple(b["bc"]), nlev=int(b["nlev"])))
+        prm = L.H3dParams()
+        self.lib.h3dmg_default_params(ctypes.byref(prm))
+        prm.nu1, prm.nu2, prm.maxcy = nu1, nu2, maxcy
+        prm.tol, prm.dt = tol, dt
+        prm.criterion = _CRIT[criterion]
+        prm.precision = _PREC[precision]
+        prm.coarse = _COARSE[coarse]
+        prm.min_cycles = min_cycles
+        prm.use_graph = 1 if use_graph else 0
+        self.params = prm
+        h = ctypes.c_void_p()
+        L.check(self.lib.h3dmg_create(arr, nb, ctypes.byref(prm), ctypes.byref(h)))
+        self.h = h
+        self.maxcy = maxcy
+
+    # ---- lifetime
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.h3dmg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def device_bytes(self) -> int:
+        return int(self.lib.h3dmg_device_bytes(self.h))
+
+    def _shape(self, b, pad=0):
+        ni, nj, nk = self.blocks[b]["n"]
+        return (nk + 2 * pad, nj + 2 * pad, ni + 2 * pad)
+
+    # ---- data in / out
+    def set_rhs(self, b, f):
+        if _is_device_tensor(f):
+            assert tuple(f.shape) == self._shape(b) and f.is_contiguous()
+            L.check(self.lib.h3dmg_set_rhs_device(self.h, b, ctypes.c_void_p(f.data_ptr())))
+            return
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.shape == self._shape(b), (f.shape, self._shape(b))
+        ni, nj, _ = self.blocks[b]["n"]
+        L.check(self.lib.h3dmg_set_rhs(self.h, b, _dptr(f), ni, nj, 0))
+
+    def set_p(self, b, p):
+        if _is_device_tensor(p):
+            assert tuple(p.shape) == self._shape(b) and p.is_contiguous()
+            L.check(self.lib.h3dmg_set_p_device(self.h, b, ctypes.c_void_p(p.data_ptr())))
+            return
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        ni, nj, _ = self.blocks[b]["n"]
+        if p.shape == self._shape(b, 1):
+            L.check(self.lib.h3dmg_set_p(self.h, b, _dptr(p), ni + 2, nj + 2, 1))
+        else:
+            assert p.shape == self._shape(b)
+            L.check(self.lib.h3dmg_set_p(self.h, b, _dptr(p), ni, nj, 0))
+
+    def get_p(self, b, out=None):
+        """Pressure with its one-cell ghost layer, shape (nk+2, nj+2, ni+2);
+        with a device tensor `out` (nk, nj, ni): the interior, on the GPU."""
+        if out is not None and _is_device_tensor(out):
+            L.check(self.lib.h3dmg_get_p_device(self.h, b, ctypes.c_void_p(out.data_ptr())))
+            return out
+        ni, nj, _ = self.blocks[b]["n"]
+        p = np.zeros(self._shape(b, 1))
+        L.check(self.lib.h3dmg_get_p(self.h, b, _dptr(p), ni + 2, nj + 2, 1))
+        return p
+
+    def get_pp(self, b):
+        ni, nj, _ = self.blocks[b]["n"]
+        pp = np.zeros(self._shape(b, 1))
+        L.check(self.lib.h3dmg_get_pp(self.h, b, _dptr(pp), ni + 2, nj + 2, 1))
+        return pp
+
+    # ---- solver
+    def vcycle(self):
+        L.check(self.lib.h3dmg_vcycle(self.h))
+
+    def solve(self, raise_on_fail=False):
+        """Returns (status, ncycles, hist[0..ncycles]) -- hist is the largest
+        relative residual over the blocks (REL_L2) or the global rmax."""
+        hist = np.zeros(self.maxcy + 1)
+        nc = ctypes.c_int(0)
+        fr = ctypes.c_double(0.0)
+        st = self.lib.h3dmg_solve(self.h, _dptr(hist), hist.size, ctypes.byref(nc),
+                                  ctypes.byref(fr))
+        if raise_on_fail:
+            L.check(st)
+        else:
+            L.check(st, allow=(L.ERR_NOT_CONVERGED,))
+        return st, nc.value, hist[: nc.value + 1]
+
+    def block_stats(self):
+        nb = len(self.blocks)
+        cyc = (ctypes.c_int * nb)()
+        rel = (ctypes.c_double * nb)()
+        L.check(self.lib.h3dmg_block_stats(self.h, cyc, rel))
+        return list(cyc), list(rel)
+
+    def remove_mean(self):
+        L.check(self.lib.h3dmg_remove_mean(self.h))
+
+    def set_stream(self, stream_ptr: int | None):
+        L.check(self.lib.h3dmg_set_stream(self.h, ctypes.c_void_p(stream_ptr or 0)))
+
+    # ---- operator seams (device tensors in the oracle's level layout)
+    def dbg_put(self, b, level, which, t):
+        L.check(self.lib.h3dmg_dbg_put(self.h, b, level, which, ctypes.c_void_p(t.data_ptr())))
+
+    def dbg_get(self, b, level, which, t):
+        L.check(self.lib.h3dmg_dbg_get(self.h, b, level, which, ctypes.c_void_p(t.data_ptr())))
+        return t
+
+    def dbg_relax(self, level, nsweep):
+        L.check(self.lib.h3dmg_dbg_relax(self.h, level, nsweep))
+
+    def dbg_restrict(self, level):
+        L.check(self.lib.h3dmg_dbg_restrict(self.h, level))
+
+    def dbg_prolong(self, level):
+        L.check(self.lib.h3dmg_dbg_prolong(self.h, level))

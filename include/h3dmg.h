@@ -1,0 +1,177 @@
+/* h3dmg.h -- C-ABI of the MI355X pressure-Poisson multigrid for Hydro3D.
+ *
+ * The reference exposes the solve as argument-less Fortran subroutines that
+ * read and write module state (src_v70/module_vars.for, module_multidata.for):
+ *
+ *   newsolv_mg  src_v70/newsolv_mg.for:2-93   outer projection loop, called
+ *                                            from src_v70/flosol.for:215
+ *   coeff       src_v70/mgsolver.for:2-224    per-level 7-point coefficients
+ *   mgkcyc      src_v70/mgsolver.for:226-372  one V-cycle, pp <- V(su)
+ *
+ * This header replaces that seam with explicit handles, plain pointers and
+ * sizes (no Fortran module state, no torch types).  The Fortran binding is
+ * hydro3d_amd/fortran/h3dmg_mod.f90 (ISO_C_BINDING) and the drop-in
+ * `newsolv_mg` is hydro3d_amd/fortran/newsolv_mg_h3d.f90.
+ *
+ * Array conventions at this boundary:
+ *   host arrays are Fortran column-major (i fastest).  A "Fortran array" is
+ *   described by its leading dimensions ld1, ld2 and the 0-based offset `off`
+ *   of the first interior cell on every axis (off = pl in the reference's
+ *   p(ttc_i,ttc_j,ttc_k) arrays, src_v70/localparameters.for:106-143).
+ *   Device-pointer entry points take dense interior arrays n_i*n_j*n_k
+ *   (i fastest) that already live in HBM.
+ *
+ * Threading: one handle per process and GPU; calls on one handle must not
+ * overlap.  All calls are synchronous with respect to the host unless named
+ * *_async.
+ */
+#ifndef H3DMG_H
+#define H3DMG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  H3DMG_OK = 0,
+  H3DMG_ERR_ARG = 1,           /* bad argument (null pointer, bad size, ...) */
+  H3DMG_ERR_LEVELS = 2,        /* n not divisible by 2^(nlevels-1)
+                                  (src_v70/localparameters.for:202-208)     */
+  H3DMG_ERR_DEVICE = 3,        /* HIP runtime error                          */
+  H3DMG_ERR_NOMEM = 4,         /* device allocation refused                  */
+  H3DMG_ERR_NOT_CONVERGED = 5, /* maxcy cycles without meeting the criterion
+                                  (newsolv_mg.for:61-62 prints and goes on)  */
+  H3DMG_ERR_DIVERGED = 6,      /* residual non-finite or rmax > 10
+                                  (newsolv_mg.for:65-75 stops the run)       */
+  H3DMG_ERR_UNSUPPORTED = 7,   /* option not available on this build         */
+  H3DMG_ERR_BUSY = 8           /* serving: no capacity for the request now   */
+} h3dmg_status;
+
+/* boundary kind per face (west, east, south, north, bottom, top).  The
+ * reference treats bc code 5 as periodic and every other code as a Neumann
+ * wall for the pressure (src_v70/mgsolver.for:194-205). */
+enum { H3DMG_BC_WALL = 0, H3DMG_BC_PERIODIC = 1 };
+
+/* smoother (mg_itrsch, src_v70/control.cin line 10) */
+enum { H3DMG_SMOOTH_GS = 1, H3DMG_SMOOTH_TDMA = 2, H3DMG_SMOOTH_RBGS = 3 };
+
+/* stopping criterion */
+enum {
+  H3DMG_CRIT_REL_L2 = 0, /* ||f - A p||_2 / ||f||_2 <= tol                  */
+  H3DMG_CRIT_REF_RMAX = 1 /* reference: rmax < eps and iter > 1, with
+                             rmax = max|div(u)| dx dy dz (press.for:66-73)  */
+};
+
+/* arithmetic of the V-cycle */
+enum {
+  H3DMG_FP64 = 0, /* plain: every level in FP64 (the reference's arithmetic) */
+  H3DMG_MIXED = 1 /* FP32 smoothing and transfers inside FP64 iterative
+                     refinement; residual, norms, update and the coarsest
+                     level in FP64                                            */
+};
+
+/* treatment of the coarsest level */
+enum {
+  H3DMG_COARSE_SMOOTH = 0, /* reference: nu1 + nu2 sweeps, no exact solve
+                              (src_v70/mgsolver.for:305,335)                 */
+  H3DMG_COARSE_EXACT = 1   /* FP64 direct solve in the discrete eigenbasis,
+                              null-space (mean) mode removed                 */
+};
+
+typedef struct {
+  int ni, nj, nk;     /* interior cells at the finest level */
+  double dx, dy, dz;  /* spacing at the finest level */
+  int nlevels;        /* multigrid levels (ngrid) */
+  int bc[6];          /* H3DMG_BC_* per face; a periodic axis must have both
+                         faces periodic */
+} h3dmg_block;
+
+typedef struct {
+  int smoother;      /* H3DMG_SMOOTH_* (device: RBGS)              */
+  int nu1, nu2;      /* irestr, iproln (control.cin line 11)        */
+  int maxcy;         /* cycle cap (control.cin line 11)             */
+  int criterion;     /* H3DMG_CRIT_*                                */
+  double tol;        /* REL_L2 tolerance or reference eps           */
+  double dt;         /* time step, used by the REF_RMAX criterion   */
+  int precision;     /* H3DMG_FP64 | H3DMG_MIXED                    */
+  int coarse;        /* H3DMG_COARSE_*                              */
+  int min_cycles;    /* cycles always run (reference: 2)            */
+  int use_graph;     /* capture one V-cycle in a hipGraph           */
+} h3dmg_params;
+
+typedef struct h3dmg_handle h3dmg_handle;
+
+/* Fill `p` with the defaults: RBGS, V(2,2), maxcy 60, REL_L2 1e-8, FP64,
+ * reference coarsest level, min_cycles 1, graphs on. */
+void h3dmg_default_params(h3dmg_params *p);
+
+/* Build the hierarchy of every block in device memory (the role of coeff,
+ * mgsolver.for:2-224, and of the cof allocation, localparameters.for:225-239).
+ * Blocks are independent: no two blocks share a face (the reference's
+ * neighbour search, alloc_dom.for:524-690, finds none). */
+h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_params *prm,
+                          h3dmg_handle **out);
+void h3dmg_destroy(h3dmg_handle *h);
+const char *h3dmg_last_error(void);
+
+/* Device memory the handle holds, in bytes. */
+unsigned long long h3dmg_device_bytes(const h3dmg_handle *h);
+
+/* Right-hand side su of block `b` (dom(ib)%su, mgsolver.for:262) from a host
+ * Fortran array; interior cells only.  Also records ||f||_2. */
+h3dmg_status h3dmg_set_rhs(h3dmg_handle *h, int b, const double *su, int ld1, int ld2, int off);
+/* Same from a dense device interior array. */
+h3dmg_status h3dmg_set_rhs_device(h3dmg_handle *h, int b, const double *d_su);
+
+/* Pressure of block b (dom(ib)%p): warm start in, solution out.  Host
+ * Fortran arrays; `get` writes the interior and the one-cell ghost layer
+ * (images: mirror at walls, wrap at periodic faces). */
+h3dmg_status h3dmg_set_p(h3dmg_handle *h, int b, const double *p, int ld1, int ld2, int off);
+h3dmg_status h3dmg_get_p(h3dmg_handle *h, int b, double *p, int ld1, int ld2, int off);
+h3dmg_status h3dmg_set_p_device(h3dmg_handle *h, int b, const double *d_p);
+h3dmg_status h3dmg_get_p_device(h3dmg_handle *h, int b, double *d_p);
+
+/* One mgkcyc on the current rhs with pp = 0 (mgsolver.for:226-372); the
+ * correction is then read with h3dmg_get_pp (padded, ghosts included, as the
+ * reference copies cof(phi_1) back into pp at mgsolver.for:354-367). */
+h3dmg_status h3dmg_vcycle(h3dmg_handle *h);
+h3dmg_status h3dmg_get_pp(h3dmg_handle *h, int b, double *pp, int ld1, int ld2, int off);
+
+/* Pressure-form outer loop (newsolv_mg.for:19-59 with su = f - A p):
+ * cycles until every block meets the criterion (a converged block stops
+ * cycling) or maxcy.  hist (may be null) receives, per cycle k = 0..ncycles,
+ * the largest relative residual over the blocks; ncycles and final_res may be
+ * null.  Returns H3DMG_ERR_NOT_CONVERGED when maxcy is reached (p holds the
+ * last iterate, as the reference goes on). */
+h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycles,
+                         double *final_res);
+
+/* Per-block results of the last solve: cycles run and final relative
+ * residual (arrays of nblocks). */
+h3dmg_status h3dmg_block_stats(const h3dmg_handle *h, int *cycles, double *rel_res);
+
+/* Remove the mean of p over the interior of every block (FP64). */
+h3dmg_status h3dmg_remove_mean(h3dmg_handle *h);
+
+/* Stream the handle launches on (a hipStream_t); null = the handle's own. */
+h3dmg_status h3dmg_set_stream(h3dmg_handle *h, void *stream);
+
+/* ---- test seams: one operator at a time on level arrays ------------------
+ * Level arrays cross this seam as dense device arrays in the layout of the
+ * oracle: phi padded (n+2)^3 i-fastest, rhs interior.  Ghost cells of an
+ * input phi must already be images of the interior (the state every level is
+ * in after exchangepp); output ghosts are written as images. */
+enum { H3DMG_ARR_PHI = 0, H3DMG_ARR_RHS = 1 };
+h3dmg_status h3dmg_dbg_put(h3dmg_handle *h, int b, int level, int which, const double *d_src);
+h3dmg_status h3dmg_dbg_get(h3dmg_handle *h, int b, int level, int which, double *d_dst);
+/* mgrelax(level, nsweep) on every block */
+h3dmg_status h3dmg_dbg_relax(h3dmg_handle *h, int level, int nsweep);
+/* mgrestr(level): level is the coarse level */
+h3dmg_status h3dmg_dbg_restrict(h3dmg_handle *h, int level);
+/* mgcorr(level): level is the fine level */
+h3dmg_status h3dmg_dbg_prolong(h3dmg_handle *h, int level);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* H3DMG_H */

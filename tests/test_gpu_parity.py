@@ -1,0 +1,201 @@
+"""Parity of the HIP path (libh3dmg.so through its C-ABI) with the oracle and
+the reference fixtures.  Needs an MI355X.
+
+Bar: bitwise for every FP64 operator and for whole V-cycles and solves
+(the library is built with -ffp-contract=off and follows the reference's
+arithmetic order); residual norms to 1e-12 relative (reduction order
+differs); the mixed-precision path to the tolerances stated per test.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from h3d_util import GOLDEN, apply_np, image_ghosts, load_golden, mean_free, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+OPS = sorted(glob.glob(os.path.join(GOLDEN, "ops_*.npz")))
+
+
+def _blockspec(z):
+    return dict(n=tuple(int(x) for x in z["n"]), h=tuple(float(x) for x in z["h"]),
+                bc=tuple(int(x) for x in z["bc"]), nlev=int(z["nlev"]))
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def _host(t):
+    return t.cpu().numpy()
+
+
+def _solver(specs, **kw):
+    from hydro3d_amd import MultigridSolver
+    return MultigridSolver(specs if isinstance(specs, list) else [specs], **kw)
+
+
+@pytest.mark.parametrize("path", OPS, ids=[os.path.basename(p) for p in OPS])
+def test_gpu_operators_bitwise_vs_reference(gpu_lib, path):
+    import torch
+    from hydro3d_amd import _lib as L
+    z = load_golden(os.path.basename(path))
+    spec = _blockspec(z)
+    Lv = spec["nlev"]
+    print(f"case {os.path.basename(path)} {spec}", flush=True)
+    with _solver(spec, use_graph=False) as S:
+        for g in range(1, Lv + 1):
+            phi, rhs = z[f"relax{g}_s3_in_phi"], z[f"relax{g}_s3_in_rhs"]
+            S.dbg_put(0, g, L.ARR_PHI, _dev(phi))
+            S.dbg_put(0, g, L.ARR_RHS, _dev(rhs))
+            S.dbg_relax(g, 2)
+            out = _host(S.dbg_get(0, g, L.ARR_PHI, torch.empty(phi.shape, dtype=torch.float64,
+                                                                 device="cuda")))
+            assert np.array_equal(out, z[f"relax{g}_s3_out"]), f"relax level {g}"
+            if g > 1:
+                S.dbg_put(0, g - 1, L.ARR_PHI, _dev(z[f"restr{g}_in_phi"]))
+                S.dbg_put(0, g - 1, L.ARR_RHS, _dev(z[f"restr{g}_in_rhs"]))
+                S.dbg_restrict(g)
+                pc = _host(S.dbg_get(0, g, L.ARR_PHI, torch.empty(z[f"restr{g}_out_phi"].shape,
+                                                                    dtype=torch.float64,
+                                                                    device="cuda")))
+                rc = _host(S.dbg_get(0, g, L.ARR_RHS, torch.empty(z[f"restr{g}_out_rhs"].shape,
+                                                                    dtype=torch.float64,
+                                                                    device="cuda")))
+                assert np.array_equal(pc, z[f"restr{g}_out_phi"]), f"restr phi level {g}"
+                assert np.array_equal(rc, z[f"restr{g}_out_rhs"]), f"restr rhs level {g}"
+            if g < Lv:
+                S.dbg_put(0, g, L.ARR_PHI, _dev(z[f"corr{g}_in_fine"]))
+                S.dbg_put(0, g + 1, L.ARR_PHI, _dev(z[f"corr{g}_in_coarse"]))
+                S.dbg_prolong(g)
+                out = _host(S.dbg_get(0, g, L.ARR_PHI, torch.empty(z[f"corr{g}_out"].shape,
+                                                                     dtype=torch.float64,
+                                                                     device="cuda")))
+                assert np.array_equal(out, z[f"corr{g}_out"]), f"corr level {g}"
+        S.set_rhs(0, z["vcycle_s3_su"])
+        S.vcycle()
+        pp = S.get_pp(0)
+        assert np.array_equal(pp, z["vcycle_s3_pp"]), "vcycle"
+
+
+def test_gpu_solve_tracks_reference_velocity_form(gpu_lib):
+    from oracle import oracle as O
+    z = load_golden("newsolv_ppw32.npz")
+    spec = _blockspec(z)
+    nc = int(z["ncycles"])
+    f = z["su0"]
+    with _solver(spec, maxcy=nc, tol=0.0) as S:
+        S.set_rhs(0, f)
+        st, ran, hist = S.solve()
+        assert ran == nc
+        p = S.get_p(0)
+    fn = float(z["su_norm"][0])
+    # reference history (velocity form) per cycle
+    assert np.all(np.abs(hist * fn - z["su_norm"]) <= 1e-10 * fn)
+    assert rel_l2(mean_free(p[1:-1, 1:-1, 1:-1]), mean_free(z["p"][1:-1, 1:-1, 1:-1])) <= 1e-10
+    # and bitwise the oracle's pressure form
+    B = O.Block(spec["n"], spec["h"], spec["bc"], spec["nlev"])
+    po, ho = O.solve(B, f, nc, 0.0)
+    assert np.array_equal(p[1:-1, 1:-1, 1:-1], po[1:-1, 1:-1, 1:-1])
+    assert np.allclose(hist * fn, ho, rtol=1e-12, atol=0)
+
+
+def test_gpu_single_64_to_tolerance(gpu_lib):
+    """64^3, periodic x/y, Neumann z, 4 levels, V(2,2), to 1e-8: same cycle
+    count and bitwise the same pressure as the oracle."""
+    from hydro3d_amd import configs as C
+    from oracle import oracle as O
+    c = C.single_64()
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    with _solver(spec, tol=1e-8) as S:
+        S.set_rhs(0, c["f"])
+        st, ran, hist = S.solve(raise_on_fail=True)
+        p = S.get_p(0)
+    B = O.Block(c["n"], c["h"], c["bc"], c["nlev"])
+    po, ho = O.solve(B, c["f"], 60, 1e-8)
+    assert ran == len(ho) - 1
+    assert np.array_equal(p[1:-1, 1:-1, 1:-1], po[1:-1, 1:-1, 1:-1])
+    assert hist[-1] <= 1e-8
+
+
+def test_gpu_uneven_blocks_each_equal_alone(gpu_lib):
+    """Independent blocks of different shapes and level counts in one handle:
+    each block's pressure is bitwise the oracle's single-block solve."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(3)
+    W, P = 0, 1
+    specs = [dict(n=(32, 16, 24), h=(1 / 32,) * 3, bc=(W,) * 6, nlev=3),
+             dict(n=(16, 16, 16), h=(1 / 16,) * 3, bc=(P, P, P, P, W, W), nlev=4),
+             dict(n=(48, 8, 16), h=(0.02, 0.03, 0.04), bc=(P, P, W, W, P, P), nlev=2)]
+    fs = [mean_free(rng.standard_normal((s["n"][2], s["n"][1], s["n"][0]))) for s in specs]
+    with _solver(specs, tol=1e-9) as S:
+        for b, f in enumerate(fs):
+            S.set_rhs(b, f)
+        st, ran, hist = S.solve(raise_on_fail=True)
+        cyc, rel = S.block_stats()
+        ps = [S.get_p(b) for b in range(len(specs))]
+    for b, (s, f) in enumerate(zip(specs, fs)):
+        B = O.Block(s["n"], s["h"], s["bc"], s["nlev"])
+        po, ho = O.solve(B, f, 60, 1e-9)
+        assert cyc[b] == len(ho) - 1, (b, cyc[b], len(ho) - 1)
+        assert np.array_equal(ps[b][1:-1, 1:-1, 1:-1], po[1:-1, 1:-1, 1:-1]), b
+
+
+def test_gpu_graph_and_eager_identical(gpu_lib):
+    from hydro3d_amd import configs as C
+    c = C.cube(32, C.BC_PPN, 3, seed=9)
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    out = []
+    for graph in (True, False, True):
+        with _solver(spec, tol=1e-10, use_graph=graph) as S:
+            S.set_rhs(0, c["f"])
+            S.solve(raise_on_fail=True)
+            out.append(S.get_p(0))
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+@pytest.mark.parametrize("coarse", ["smooth", "exact"])
+def test_gpu_mixed_precision_64(gpu_lib, coarse):
+    """FP32 V-cycle inside FP64 refinement reaches 1e-8 and its pressure is
+    within 1e-7 (relative L2, mean removed) of the FP64 solution."""
+    from hydro3d_amd import configs as C
+    c = C.single_64()
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    with _solver(spec, tol=1e-8) as S:
+        S.set_rhs(0, c["f"])
+        S.solve(raise_on_fail=True)
+        p64 = S.get_p(0)
+    with _solver(spec, tol=1e-8, precision="mixed", coarse=coarse) as S:
+        S.set_rhs(0, c["f"])
+        st, ran, hist = S.solve(raise_on_fail=True)
+        pm = S.get_p(0)
+    print(f"mixed/{coarse}: {ran} cycles, final {hist[-1]:.3e}")
+    # independent residual check in numpy
+    r = c["f"] - apply_np(image_ghosts(pm, c["bc"]), c["h"], c["bc"])
+    assert np.linalg.norm(r) / np.linalg.norm(c["f"]) <= 1e-8 * 1.0001
+    assert rel_l2(mean_free(pm[1:-1, 1:-1, 1:-1]), mean_free(p64[1:-1, 1:-1, 1:-1])) <= 1e-7
+
+
+def test_gpu_256_fp64_and_mixed(gpu_lib):
+    """256^3 (6 levels): FP64 and mixed both reach 1e-8 (checked in numpy),
+    mixed within 1e-7 of FP64."""
+    from hydro3d_amd import configs as C
+    c = C.single_256()
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    fn = np.linalg.norm(c["f"])
+    res = {}
+    for prec, coarse in (("fp64", "smooth"), ("mixed", "exact")):
+        with _solver(spec, tol=1e-8, precision=prec, coarse=coarse) as S:
+            S.set_rhs(0, c["f"])
+            st, ran, hist = S.solve(raise_on_fail=True)
+            p = S.get_p(0)
+        r = c["f"] - apply_np(p, c["h"], c["bc"])
+        rel = np.linalg.norm(r) / fn
+        print(f"256^3 {prec}/{coarse}: {ran} cycles, device {hist[-1]:.3e}, numpy {rel:.3e}",
+              flush=True)
+        assert rel <= 1e-8 * 1.0001
+        res[prec] = p[1:-1, 1:-1, 1:-1]
+    assert rel_l2(mean_free(res["mixed"]), mean_free(res["fp64"])) <= 1e-7
