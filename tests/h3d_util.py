@@ -58,11 +58,11 @@ def spectral_solve(f, h, bc):
     mats = [op1d(ni, h[0], bc[0] == P), op1d(nj, h[1], bc[2] == P), op1d(nk, h[2], bc[4] == P)]
     ev = [np.linalg.eigh(A) for A in mats]
     (lx, Qx), (ly, Qy), (lz, Qz) = ev
-    fh = np.einsum("ia,jb,kc,kji->cba", Qx, Qy, Qz, f)
+    fh = np.einsum("ia,jb,kc,kji->cba", Qx, Qy, Qz, f, optimize=True)
     lam = lz[:, None, None] + ly[None, :, None] + lx[None, None, :]
     small = np.abs(lam) < 1e-9 * np.abs(lam).max()
     ph = np.where(small, 0.0, fh / np.where(small, 1.0, lam))
-    p = np.einsum("ia,jb,kc,cba->kji", Qx, Qy, Qz, ph)
+    p = np.einsum("ia,jb,kc,cba->kji", Qx, Qy, Qz, ph, optimize=True)
     return p - p.mean()
 
 

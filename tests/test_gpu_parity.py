@@ -12,7 +12,8 @@ import os
 import numpy as np
 import pytest
 
-from h3d_util import GOLDEN, apply_np, image_ghosts, load_golden, mean_free, rel_l2
+from h3d_util import (GOLDEN, apply_np, image_ghosts, load_golden, mean_free, rel_l2,
+                      spectral_solve)
 
 pytestmark = pytest.mark.gpu
 
@@ -127,9 +128,11 @@ def test_gpu_uneven_blocks_each_equal_alone(gpu_lib):
     from oracle import oracle as O
     rng = np.random.default_rng(3)
     W, P = 0, 1
-    specs = [dict(n=(32, 16, 24), h=(1 / 32,) * 3, bc=(W,) * 6, nlev=3),
-             dict(n=(16, 16, 16), h=(1 / 16,) * 3, bc=(P, P, P, P, W, W), nlev=4),
-             dict(n=(48, 8, 16), h=(0.02, 0.03, 0.04), bc=(P, P, W, W, P, P), nlev=2)]
+    # level counts 4, 3, 3: level 3 is the coarsest of two blocks and an
+    # intermediate level of the first (10, 17 and 21 cycles in the oracle)
+    specs = [dict(n=(32, 16, 24), h=(1 / 32,) * 3, bc=(W,) * 6, nlev=4),
+             dict(n=(16, 16, 16), h=(1 / 16,) * 3, bc=(P, P, P, P, W, W), nlev=3),
+             dict(n=(48, 8, 16), h=(0.02, 0.03, 0.04), bc=(P, P, W, W, P, P), nlev=3)]
     fs = [mean_free(rng.standard_normal((s["n"][2], s["n"][1], s["n"][0]))) for s in specs]
     with _solver(specs, tol=1e-9) as S:
         for b, f in enumerate(fs):
@@ -159,43 +162,60 @@ def test_gpu_graph_and_eager_identical(gpu_lib):
 
 @pytest.mark.parametrize("coarse", ["smooth", "exact"])
 def test_gpu_mixed_precision_64(gpu_lib, coarse):
-    """FP32 V-cycle inside FP64 refinement reaches 1e-8 and its pressure is
-    within 1e-7 (relative L2, mean removed) of the FP64 solution."""
+    """FP32 V-cycle inside FP64 refinement reaches 1e-8 (checked in numpy).
+    Pressure: with the reference's coarsest-level smoothing the mixed path
+    follows the FP64 iterates to 1e-7; with the exact coarsest solve it lands
+    within 1e-7 of the converged discrete solution (spectral solve), which the
+    FP64 reference schedule at 1e-8 does not (about 1.4e-5 here)."""
     from hydro3d_amd import configs as C
     c = C.single_64()
     spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
     with _solver(spec, tol=1e-8) as S:
         S.set_rhs(0, c["f"])
         S.solve(raise_on_fail=True)
-        p64 = S.get_p(0)
+        p64 = mean_free(S.get_p(0)[1:-1, 1:-1, 1:-1])
     with _solver(spec, tol=1e-8, precision="mixed", coarse=coarse) as S:
         S.set_rhs(0, c["f"])
         st, ran, hist = S.solve(raise_on_fail=True)
-        pm = S.get_p(0)
-    print(f"mixed/{coarse}: {ran} cycles, final {hist[-1]:.3e}")
-    # independent residual check in numpy
-    r = c["f"] - apply_np(image_ghosts(pm, c["bc"]), c["h"], c["bc"])
-    assert np.linalg.norm(r) / np.linalg.norm(c["f"]) <= 1e-8 * 1.0001
-    assert rel_l2(mean_free(pm[1:-1, 1:-1, 1:-1]), mean_free(p64[1:-1, 1:-1, 1:-1])) <= 1e-7
+        pm_pad = S.get_p(0)
+    pm = mean_free(pm_pad[1:-1, 1:-1, 1:-1])
+    pstar = spectral_solve(c["f"], c["h"], c["bc"])
+    r = c["f"] - apply_np(image_ghosts(pm_pad, c["bc"]), c["h"], c["bc"])
+    rel = float(np.linalg.norm(r) / np.linalg.norm(c["f"]))
+    e_m, e_64 = rel_l2(pm, pstar), rel_l2(p64, pstar)
+    e_m64 = rel_l2(pm, p64)
+    print(f"mixed/{coarse}: {ran} cycles, residual {rel:.3e}; pressure error vs converged: "
+          f"mixed {e_m:.3e}, fp64@1e-8 {e_64:.3e}; mixed vs fp64 {e_m64:.3e}")
+    assert rel <= 1e-8 * 1.0001
+    if coarse == "smooth":
+        assert e_m64 <= 1e-7
+    else:
+        assert e_m <= 1e-7
 
 
 def test_gpu_256_fp64_and_mixed(gpu_lib):
-    """256^3 (6 levels): FP64 and mixed both reach 1e-8 (checked in numpy),
-    mixed within 1e-7 of FP64."""
+    """256^3 (6 levels): FP64 and mixed both reach 1e-8 (checked in numpy);
+    the mixed pressure is within 1e-7 of the converged FP64 solution."""
     from hydro3d_amd import configs as C
     c = C.single_256()
     spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
     fn = np.linalg.norm(c["f"])
     res = {}
-    for prec, coarse in (("fp64", "smooth"), ("mixed", "exact")):
-        with _solver(spec, tol=1e-8, precision=prec, coarse=coarse) as S:
+    for prec, coarse, tol in (("fp64", "smooth", 1e-12), ("fp64", "smooth", 1e-8),
+                              ("mixed", "exact", 1e-8)):
+        with _solver(spec, tol=tol, precision=prec, coarse=coarse, maxcy=200) as S:
             S.set_rhs(0, c["f"])
             st, ran, hist = S.solve(raise_on_fail=True)
             p = S.get_p(0)
         r = c["f"] - apply_np(p, c["h"], c["bc"])
-        rel = np.linalg.norm(r) / fn
-        print(f"256^3 {prec}/{coarse}: {ran} cycles, device {hist[-1]:.3e}, numpy {rel:.3e}",
-              flush=True)
-        assert rel <= 1e-8 * 1.0001
-        res[prec] = p[1:-1, 1:-1, 1:-1]
-    assert rel_l2(mean_free(res["mixed"]), mean_free(res["fp64"])) <= 1e-7
+        rel = float(np.linalg.norm(r) / fn)
+        print(f"256^3 {prec}/{coarse} tol {tol:g}: {ran} cycles, device {hist[-1]:.3e}, "
+              f"numpy {rel:.3e}", flush=True)
+        assert rel <= tol * 1.0001
+        res[(prec, tol)] = mean_free(p[1:-1, 1:-1, 1:-1])
+        del p, r
+    pstar = res[("fp64", 1e-12)]
+    e_m = rel_l2(res[("mixed", 1e-8)], pstar)
+    e_64 = rel_l2(res[("fp64", 1e-8)], pstar)
+    print(f"pressure error vs converged FP64: mixed {e_m:.3e}, fp64@1e-8 {e_64:.3e}")
+    assert e_m <= 1e-7
