@@ -60,6 +60,8 @@ SIGNATURES = {
     "h3dmg_block_stats": (_i, [_vp, _ip, _dp]),
     "h3dmg_remove_mean": (_i, [_vp]),
     "h3dmg_set_stream": (_i, [_vp, _vp]),
+    "h3dmg_set_march": (_i, [_vp, _i, _i]),
+    "h3dmg_get_march": (_i, [_vp, _i]),
     "h3dmg_dbg_put": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_get": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_relax": (_i, [_vp, _i, _i]),

@@ -32,8 +32,9 @@ constexpr int NTHREADS = TILE_X * TILE_Y;
 
 struct LevelDev {
   int ni, nj, nk;  // interior cells
-  int H;           // colour row pitch (elements)
+  int H;           // colour row pitch (elements), even, >= ceil(ni/2)
   int per;         // bit a set: axis a periodic (a = 0 x, 1 y, 2 z)
+  int lpr;         // smoother lanes per row (power of two <= 64)
   int blk;         // block index
   int finer;       // descriptor index of level g-1 (or -1)
   int coarser;     // descriptor index of level g+1 (or -1)
@@ -45,15 +46,20 @@ struct LevelDev {
 // Finest-level FP64 state of one block (pressure, right-hand side).
 struct FineDev {
   int ni, nj, nk, H, per, blk;
+  int lpr;  // residual lanes per row (one colour-array entry per lane)
   double cx, cy, cz;
   double *p[2];  // [colour]
   double *f[2];  // [colour]
 };
 
-// A workgroup's work item: descriptor index and the tile origin.
+// A workgroup's work item: descriptor index, the tile origin, for the
+// plane-marching kernels the end plane z1 (exclusive), and the tile kind of
+// the smoother's tile lists.
 struct Tile {
   int lev;
   int x0, y0, z0;
+  int z1;
+  int kind;
 };
 
 struct RedPartial {
@@ -63,9 +69,11 @@ struct RedPartial {
 
 // ---- launchers (h3dmg_kernels.hip) --------------------------------------
 // T = storage type of the level (double or float).
+// one colour half-sweep of the red-black smoother over a tile list of
+// kinds 0 (interior march), 1 (boundary march) and 2 (x faces)
 template <class T>
-void launch_rb_pass(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                    const int *active, int colour, int src, int zero);
+void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                     const int *active, int colour, int src, int zero);
 template <class Tf, class Tc>
 void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                      const int *active, int src_f);

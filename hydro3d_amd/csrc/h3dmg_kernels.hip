@@ -82,29 +82,196 @@ __device__ __forceinline__ double fine_value(const FineDev &F, int x, int y, int
 }
 
 // ---------------------------------------------------------------------------
-// One colour half-sweep of the red-black smoother (mgsolver.for:553-605).
-// colour 0 = red, 1 = black.  Sweep reads buffer `src`, writes `1-src`:
+// One colour half-sweep of the red-black smoother (mgsolver.for:553-605):
+//   phi = (rhs - (aw*W + ae*E + as*S + an*N + ab*B + at*T)) / ap
+// colour 0 = red, 1 = black.  A sweep reads buffer `src` and writes `1-src`:
 //   red:   interior neighbours (black) from black[src]
 //   black: interior neighbours (red)   from red[1-src] (just written)
-//   ghost neighbours from buffer src (the pre-sweep state the reference's
-//   ghosts hold until exchangepp at the end of the sweep).
-// zero != 0: the level holds zeros in buffer src (first sweep after mgrestr
-// zeroed phi, mgsolver.for:655-662, or pp = 0 at the finest level).
+//   ghost neighbours from buffer src, resolved to their image (the
+//   pre-sweep state the reference's ghosts hold until exchangepp at the end
+//   of the sweep, mgsolver.for:614-615).
+// zero != 0: the level holds zeros in buffer src (the first sweep after
+// mgrestr zeroed phi, mgsolver.for:655-662, or pp = 0 at the finest level).
+//
+// Three tile kinds share a half-sweep (one launch, k_rb_sweep): march tiles
+// whose cells have all y/z neighbours inside the block (no ghost logic at
+// all), the other march tiles, and x-face tiles for the cells on the two x
+// faces (i = 0, ni-1), which the march tiles skip.
+//
+// Work layout of the march kernels.  A lane owns two adjacent entries m0,
+// m0+1 of one row of the colour arrays; L.lpr lanes cover a row segment and a
+// wavefront holds 64/L.lpr rows (coarse levels pack several short rows into
+// one wavefront).  Each lane walks the planes [t.z0, t.z1).  For a cell
+// i = 2m + s of colour c (s = (c+j+k)&1) entry m of the other colour in the
+// same row is its east (s = 0) or west (s = 1) neighbour and entry m of planes
+// k-1, k+1 its bottom and top neighbours: that "own column" pair streams once
+// through a register window; the remaining x-neighbour is the lane's other
+// entry or the adjacent lane's (DPP wavefront shift); y-neighbours are loaded.
 template <class T>
-__global__ void __launch_bounds__(NTHREADS)
-    k_rb_pass(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
-              const int *__restrict__ active, int colour, int src, int zero) {
-  const Tile t = tiles[blockIdx.x];
-  const LevelDev &L = levs[t.lev];
-  if (!active[L.blk]) return;
-  const int m = t.x0 + (int)threadIdx.x;
-  const int j = t.y0 + (int)threadIdx.y;
-  const int k = t.z0;
-  if (m >= L.H || j >= L.nj || k >= L.nk) return;
-  const int s = (colour + j + k) & 1;
-  const int i = 2 * m + s;
-  if (i >= L.ni) return;
+struct Vec2;
+template <>
+struct Vec2<float> {
+  typedef float __attribute__((ext_vector_type(2))) type;
+};
+template <>
+struct Vec2<double> {
+  typedef double __attribute__((ext_vector_type(2))) type;
+};
 
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8);
+// give each XCD a contiguous range of tiles so that neighbouring rows and
+// planes are read through the same L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  const int per = n >> 3, rem = n & 7;
+  const int x = b & 7, q = b >> 3;
+  return x * per + min(x, rem) + q;
+}
+
+// Division by ap: the FP64 path divides, as the reference does (bitwise
+// parity); the FP32 path multiplies by the reciprocal (one rounding more,
+// within the FP32 V-cycle's own error).
+__device__ __forceinline__ double div_ap(double num, double ap, double /*inv_ap*/) {
+  return num / ap;
+}
+__device__ __forceinline__ float div_ap(float num, float /*ap*/, float inv_ap) {
+  return num * inv_ap;
+}
+
+// DPP wavefront shifts by one lane (GFX9 wave_shr:1 = 0x138, wave_shl:1 =
+// 0x130): lane l receives lane l-1's (shr) or l+1's (shl) value; the end lane
+// receives 0.  Every lane of the wavefront must be active at the call.
+__device__ __forceinline__ int dpp_i32(int v, int ctrl_shr) {
+  return ctrl_shr ? __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false)
+                  : __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float wave_shift(float v, int shr) {
+  return __int_as_float(dpp_i32(__float_as_int(v), shr));
+}
+__device__ __forceinline__ double wave_shift(double v, int shr) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i32((int)(b & 0xffffffffll), shr);
+  const int hi = dpp_i32((int)(b >> 32), shr);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Global-address-space views of the level arrays (the descriptor holds
+// generic pointers; loads through these compile to global_load with a
+// scalar base and a 32-bit lane offset).
+template <class T>
+using gmem = __attribute__((address_space(1))) T;
+
+template <class T, bool INNER>
+__device__ __forceinline__ void march_tile(const Tile &t, const LevelDev &L, int colour, int src,
+                                           int zero) {
+  typedef typename Vec2<T>::type v2;
+  const int lpr = L.lpr;
+  const int lane = (int)threadIdx.x;
+  const int rl = lane & (lpr - 1);
+  const int m0 = t.x0 + 2 * rl;
+  const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
+  const int ni = L.ni, nj = L.nj, nk = L.nk, H = L.H;
+  const bool col_ok = m0 < H && j < nj;  // H even: m0 + 1 < H as well
+  const unsigned row = (unsigned)H, plane = (unsigned)H * (unsigned)nj;
+  const gmem<T> *O = (const gmem<T> *)(colour == 0 ? L.phi[src][1] : L.phi[1 - src][0]);
+  const gmem<T> *R = (const gmem<T> *)L.rhs[colour];
+  gmem<T> *D = (gmem<T> *)L.phi[1 - src][colour];
+  const bool zero_int = zero && colour == 0;  // uniform
+  const bool zero_ghost = zero != 0;
+  // lanes outside the level read a valid entry (clamped) and store nothing
+  const unsigned cbase = (unsigned)min(j, nj - 1) * row + (unsigned)min(m0, H - 2);
+  const v2 vz = {T(0), T(0)};
+  // tiles that do not start/end a row: the end lanes fetch entries m0-1 and
+  // m0+2 from the neighbouring tile's range (uniform test, then one lane)
+  const bool tile_left = t.x0 > 0, tile_right = t.x0 + 2 * lpr < H;
+  const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz;
+  const T ap_in = T(-1) * (cx + cx + cy + cy + cz + cz);
+  const T inv_in = T(1) / ap_in;
+
+  // byte offsets in 32 bits (arrays < 4 GiB): scalar base + lane offset
+  auto ld2 = [&](const gmem<T> *a, unsigned off) -> v2 {
+    return *(const gmem<v2> *)((const gmem<char> *)a + off * (unsigned)sizeof(T));
+  };
+  auto ld1 = [&](const gmem<T> *a, unsigned off) -> T {
+    return *(const gmem<T> *)((const gmem<char> *)a + off * (unsigned)sizeof(T));
+  };
+  auto own = [&](int k) -> v2 {
+    if (zero_int || (!INNER && (unsigned)k >= (unsigned)nk)) return vz;
+    return ld2(O, (unsigned)k * plane + cbase);
+  };
+  auto ghost = [&](int x, int y, int z) -> T {
+    return zero_ghost ? T(0) : level_value<T>(L, src, x, y, z);
+  };
+
+  int k = t.z0;
+  const int k1 = t.z1;
+  v2 o_m = own(k - 1), o_c = own(k);
+  for (; k < k1; ++k) {
+    const v2 o_p = own(k + 1);
+    const unsigned q = (unsigned)k * plane + cbase;
+    v2 S2 = vz, N2 = vz;
+    if (!zero_int) {
+      if (INNER || j > 0) S2 = ld2(O, q - row);
+      if (INNER || j + 1 < nj) N2 = ld2(O, q + row);
+    }
+    const v2 R2 = ld2(R, q);
+    T lf = wave_shift(o_c.y, 1);  // entry m0-1: lane-1's second entry
+    T rt = wave_shift(o_c.x, 0);  // entry m0+2: lane+1's first entry
+    if (tile_left && !zero_int && rl == 0) lf = ld1(O, q - 1);
+    if (tile_right && !zero_int && rl == lpr - 1) rt = ld1(O, q + 2);
+    const int s = (colour + j + k) & 1;
+    // x-neighbours of the cells i0 = 2*m0 + s and i1 = i0 + 2
+    const T W0 = s ? o_c.x : lf;
+    const T E0 = s ? o_c.y : o_c.x;
+    const T W1 = s ? o_c.y : o_c.x;
+    const T E1 = s ? rt : o_c.y;
+    const int i0 = 2 * m0 + s, i1 = i0 + 2;
+    T out0, out1;
+    if (INNER) {
+      const T sum0 = cx * W0 + cx * E0 + cy * S2.x + cy * N2.x + cz * o_m.x + cz * o_p.x;
+      const T sum1 = cx * W1 + cx * E1 + cy * S2.y + cy * N2.y + cz * o_m.y + cz * o_p.y;
+      out0 = div_ap(R2.x - sum0, ap_in, inv_in);
+      out1 = div_ap(R2.y - sum1, ap_in, inv_in);
+    } else {
+      const T Wv[2] = {W0, W1}, Ev[2] = {E0, E1};
+      T o[2];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int i = e ? i1 : i0;
+        const T S = (j > 0) ? (e ? S2.y : S2.x) : ghost(i, -1, k);
+        const T N = (j + 1 < nj) ? (e ? N2.y : N2.x) : ghost(i, nj, k);
+        const T B = (k > 0) ? (e ? o_m.y : o_m.x) : ghost(i, j, -1);
+        const T Tt = (k + 1 < nk) ? (e ? o_p.y : o_p.x) : ghost(i, j, nk);
+        const Coef<T> c = cell_coef<T>(L, i, j, k);
+        const T sum = c.aw * Wv[e] + c.ae * Ev[e] + c.as * S + c.an * N + c.ab * B + c.at * Tt;
+        o[e] = div_ap((e ? R2.y : R2.x) - sum, c.ap, T(1) / c.ap);
+      }
+      out0 = o[0];
+      out1 = o[1];
+    }
+    // cells on the x faces belong to the x-face tiles of the same launch
+    const bool w0 = col_ok && i0 > 0 && i0 < ni - 1, w1 = col_ok && i1 < ni - 1;
+    gmem<char> *dq = (gmem<char> *)D + q * (unsigned)sizeof(T);
+    if (w0 && w1) {
+      const v2 o = {out0, out1};
+      *(gmem<v2> *)dq = o;
+    } else {
+      if (w0) *(gmem<T> *)dq = out0;
+      if (w1) *(gmem<T> *)(dq + sizeof(T)) = out1;
+    }
+    o_m = o_c;
+    o_c = o_p;
+  }
+}
+
+// The cells of colour `colour` on the two x faces (i = 0 and i = ni-1), with
+// the full ghost treatment.  The tile covers 256 rows j = t.x0 + ... of plane
+// k = t.y0; one thread per row does both faces.
+template <class T>
+__device__ __forceinline__ void xface_tile(const Tile &t, const LevelDev &L, int colour, int src,
+                                           int zero) {
+  const int j = t.x0 + (int)threadIdx.x + TILE_X * (int)threadIdx.y;
+  const int k = t.y0;
+  if (j >= L.nj || k >= L.nk) return;
   const T *nb_int = colour == 0 ? (const T *)L.phi[src][1] : (const T *)L.phi[1 - src][0];
   const bool zero_int = zero && colour == 0;
   const bool zero_ghost = zero != 0;
@@ -114,14 +281,37 @@ __global__ void __launch_bounds__(NTHREADS)
     if (!ghost) return zero_int ? T(0) : nb_int[cidx(L.H, L.nj, x, y, z)];
     return zero_ghost ? T(0) : level_value<T>(L, src, x, y, z);
   };
-  const Coef<T> c = cell_coef<T>(L, i, j, k);
-  const T W = nb(i - 1, j, k), E = nb(i + 1, j, k);
-  const T S = nb(i, j - 1, k), N = nb(i, j + 1, k);
-  const T B = nb(i, j, k - 1), Tp = nb(i, j, k + 1);
-  const size_t q = cidx(L.H, L.nj, i, j, k);
-  const T r = ((const T *)L.rhs[colour])[q];
-  const T sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * Tp;
-  ((T *)L.phi[1 - src][colour])[q] = (r - sum) / c.ap;
+  for (int side = 0; side < 2; ++side) {
+    const int i = side ? L.ni - 1 : 0;
+    if (side && i == 0) break;
+    if (((i + j + k) & 1) != colour) continue;
+    const Coef<T> c = cell_coef<T>(L, i, j, k);
+    const T W = nb(i - 1, j, k), E = nb(i + 1, j, k);
+    const T S = nb(i, j - 1, k), N = nb(i, j + 1, k);
+    const T B = nb(i, j, k - 1), Tt = nb(i, j, k + 1);
+    const size_t q = cidx(L.H, L.nj, i, j, k);
+    const T r = ((const T *)L.rhs[colour])[q];
+    const T sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * Tt;
+    ((T *)L.phi[1 - src][colour])[q] = div_ap(r - sum, c.ap, T(1) / c.ap);
+  }
+}
+
+// One launch per colour half-sweep over a level: every tile of the list is
+// an interior march tile (kind 0), a boundary march tile (kind 1) or an
+// x-face tile (kind 2); the kinds write disjoint cells.
+template <class T>
+__global__ void __launch_bounds__(NTHREADS)
+    k_rb_sweep(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+               const int *__restrict__ active, int colour, int src, int zero) {
+  const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+  const LevelDev &L = levs[t.lev];
+  if (!active[L.blk]) return;
+  if (t.kind == 0)
+    march_tile<T, true>(t, L, colour, src, zero);
+  else if (t.kind == 1)
+    march_tile<T, false>(t, L, colour, src, zero);
+  else
+    xface_tile<T>(t, L, colour, src, zero);
 }
 
 // ---------------------------------------------------------------------------
@@ -173,35 +363,75 @@ __global__ void __launch_bounds__(NTHREADS)
 // weight of coarse corner (oi,oj,ok) is prod (3 if o==d else 1); terms are
 // summed in the order c, c+j, c+k, c+k+j, c+i, c+i+j, c+i+k, c+i+j+k and the
 // sum divided by 64.  Ghost corners are virtual (images of coarse interior).
-// One thread per fine interior cell; the fine level is updated in place.
+//
+// A thread owns fine entry l of both colours (cells 2l and 2l+1) in fine rows
+// 2m-1, 2m and planes 2n-1, 2n (coarse row m and plane n in padded numbering):
+// those eight fine cells share the twelve coarse corners x in {l-1, l, l+1},
+// y in {m-1, m}, z in {n-1, n} (interior numbering), which are loaded once.
+// Tile::x0 = first l, y0 = first m (TILE_Y per tile), z0 = n.  The fine
+// level is updated in place (each fine cell by exactly one thread).
 template <class Tf, class Tc>
 __global__ void __launch_bounds__(NTHREADS)
     k_prolong(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
               const int *__restrict__ active, int src_f, int src_c) {
-  const Tile t = tiles[blockIdx.x];
+  const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
   const LevelDev &F = levs[t.lev];
   if (!active[F.blk]) return;
   const LevelDev &C = levs[F.coarser];
-  const int i = t.x0 + (int)threadIdx.x;
-  const int j = t.y0 + (int)threadIdx.y;
-  const int k = t.z0;
-  if (i >= F.ni || j >= F.nj || k >= F.nk) return;
-  // padded fine (i+1) = 2*icp + di; coarse interior of padded icp is icp-1
-  const int di = (i + 1) & 1, dj = (j + 1) & 1, dk = (k + 1) & 1;
-  const int ci = ((i + 1) >> 1) - 1, cj = ((j + 1) >> 1) - 1, ck = ((k + 1) >> 1) - 1;
-  const int ord[8][3] = {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1},
-                         {1, 0, 0}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}};
-  Tc s = Tc(0);
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int oi = ord[q][0], oj = ord[q][1], ok = ord[q][2];
-    const Tc w = (oi == di ? Tc(3) : Tc(1)) * (oj == dj ? Tc(3) : Tc(1)) * (ok == dk ? Tc(3) : Tc(1));
-    const Tc v = level_value<Tc>(C, src_c, ci + oi, cj + oj, ck + ok);
-    s = (q == 0) ? w * v : s + w * v;
+  const int l = t.x0 + (int)threadIdx.x;
+  const int m = t.y0 + (int)threadIdx.y;
+  const int n = t.z0;
+  if (2 * l >= F.ni || m > C.nj || n > C.nk) return;
+  const bool px = C.per & 1, py = C.per & 2, pz = C.per & 4;
+  const gmem<Tc> *C0 = (const gmem<Tc> *)C.phi[src_c][0];
+  const gmem<Tc> *C1 = (const gmem<Tc> *)C.phi[src_c][1];
+  // coarse corner coordinates (interior numbering, ghosts resolved)
+  int xc[3], yc[2], zc[2];
+  for (int a = 0; a < 3; ++a) xc[a] = image(l - 1 + a, C.ni, px);
+  for (int a = 0; a < 2; ++a) {
+    yc[a] = image(m - 1 + a, C.nj, py);
+    zc[a] = image(n - 1 + a, C.nk, pz);
   }
-  const size_t fq = cidx(F.H, F.nj, i, j, k);
-  Tf *fa = (Tf *)F.phi[src_f][(i + j + k) & 1];
-  fa[fq] = fa[fq] + (Tf)(s / Tc(64));
+  Tc cv[3][2][2];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const unsigned off =
+            ((unsigned)zc[c] * (unsigned)C.nj + (unsigned)yc[b]) * (unsigned)C.H + (unsigned)(xc[a] >> 1);
+        cv[a][b][c] = ((xc[a] + yc[b] + zc[c]) & 1) ? C1[off] : C0[off];
+      }
+  gmem<Tf> *F0 = (gmem<Tf> *)F.phi[src_f][0];
+  gmem<Tf> *F1 = (gmem<Tf> *)F.phi[src_f][1];
+#pragma unroll
+  for (int dk = 0; dk < 2; ++dk) {
+    const int k = 2 * n - 1 + dk;
+    if (k < 0 || k >= F.nk) continue;
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj) {
+      const int j = 2 * m - 1 + dj;
+      if (j < 0 || j >= F.nj) continue;
+      const unsigned fq = ((unsigned)k * (unsigned)F.nj + (unsigned)j) * (unsigned)F.H + (unsigned)l;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        // cell 2l (e = 0): corners x = l-1, l and di = 1; cell 2l+1: x = l, l+1, di = 0
+        const int di = 1 - e;
+        Tc s;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int oi = q >> 2, oj = q & 1, ok = (q >> 1) & 1;  // c, c+j, c+k, c+k+j, c+i, ...
+          const Tc w = (oi == di ? Tc(3) : Tc(1)) * (oj == dj ? Tc(3) : Tc(1)) *
+                       (ok == dk ? Tc(3) : Tc(1));
+          const Tc v = cv[e + oi][oj][ok];
+          s = (q == 0) ? w * v : s + w * v;
+        }
+        gmem<Tf> *fa = ((2 * l + e + j + k) & 1) ? F1 : F0;
+        fa[fq] = fa[fq] + (Tf)(s / Tc(64));
+      }
+    }
+  }
 }
 
 template <class T>
@@ -246,8 +476,127 @@ __device__ __forceinline__ void block_reduce(double &ss, double &am) {
 // ---------------------------------------------------------------------------
 // Finest-level residual r = f - A p in FP64 (the rhs of the next V-cycle; in
 // the reference it is su = div(u* - dt grad p)/dt, press.for:2-144), stored
-// into the level-1 rhs in the level's precision, with per-workgroup partial
-// sum of r^2 and max |r| for the REL_L2 and REF_RMAX criteria.
+// into the level-1 rhs in the level's precision, with per-tile partial sum of
+// r^2 and max |r| for the REL_L2 and REF_RMAX criteria.
+//
+// A lane holds entry l of both colour arrays of one row: the two adjacent
+// cells a = 2l (colour ca = (j+k)&1) and b = 2l+1.  Planes k-1..k+1 of both
+// entries stream through a register window, x-neighbours come from the lane's
+// other cell or the adjacent lanes (DPP shift), rows j-1, j+1 are loaded.
+// F.lpr lanes cover a row (64/F.lpr rows per wavefront); the tile walks the
+// planes [t.z0, t.z1).  Tile kind 0: every row/plane of the tile has its y/z
+// neighbours inside the block; kind 1: the general path with ghost images.
+// use_p = 0 gives r = f (the norm of the right-hand side).
+template <class T, bool INNER>
+__device__ __forceinline__ void residual_tile(const Tile &t, const FineDev &F, const LevelDev &L,
+                                              int use_p, double &ss, double &am) {
+  const int lpr = F.lpr;
+  const int lane = (int)threadIdx.x;
+  const int rl = lane & (lpr - 1);
+  const int l = t.x0 + rl;
+  const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
+  const int ni = F.ni, nj = F.nj, nk = F.nk, H = F.H;
+  const bool ok = 2 * l < ni && j < nj;
+  const bool b_ok = 2 * l + 1 < ni && j < nj;
+  const unsigned row = (unsigned)H, plane = (unsigned)H * (unsigned)nj;
+  const unsigned cbase = (unsigned)min(j, nj - 1) * row + (unsigned)min(l, H - 1);
+  const gmem<double> *P0 = (const gmem<double> *)F.p[0], *P1 = (const gmem<double> *)F.p[1];
+  const gmem<double> *F0 = (const gmem<double> *)F.f[0], *F1 = (const gmem<double> *)F.f[1];
+  gmem<T> *R0 = (gmem<T> *)L.rhs[0], *R1 = (gmem<T> *)L.rhs[1];
+  const bool tile_left = l > 0 && rl == 0;             // entry l-1 in another tile
+  const bool tile_right = rl == lpr - 1 && l + 1 < H;  // entry l+1 in another tile
+  const bool px = F.per & 1;
+  const double cx = F.cx, cy = F.cy, cz = F.cz;
+  auto ldp = [&](int c, unsigned off) -> double { return c ? P1[off] : P0[off]; };
+  auto own = [&](int k, int c) -> double {
+    if (!INNER && (unsigned)k >= (unsigned)nk) return 0.0;
+    return ldp(c, (unsigned)k * plane + cbase);
+  };
+
+  const int k1 = t.z1;
+  int k = t.z0;
+  double w0m = 0, w1m = 0, w0c = 0, w1c = 0;
+  if (use_p) {
+    w0m = own(k - 1, 0);
+    w1m = own(k - 1, 1);
+    w0c = own(k, 0);
+    w1c = own(k, 1);
+  }
+  for (; k < k1; ++k) {
+    const unsigned q = (unsigned)k * plane + cbase;
+    const int ca = (j + k) & 1, cb = ca ^ 1;
+    double ra = ca ? F1[q] : F0[q];
+    double rb = cb ? F1[q] : F0[q];
+    double w0p = 0, w1p = 0;
+    if (use_p) {
+      w0p = own(k + 1, 0);
+      w1p = own(k + 1, 1);
+      const double Pa = ca ? w1c : w0c, Pb = ca ? w0c : w1c;
+      // x-neighbours: cell 2l-1 is lane-1's b, cell 2l+2 is lane+1's a
+      double lf = wave_shift(Pb, 1), rt = wave_shift(Pa, 0);
+      if (tile_left) lf = ldp(cb, q - 1);
+      if (tile_right) rt = ldp(ca, q + 1);
+      const int ia = 2 * l, ib = ia + 1;
+      const double Wa = ia == 0 ? fine_value(F, -1, j, k) : lf;
+      const double Ea = ib < ni ? Pb : fine_value(F, ni, j, k);
+      const double Wb = Pa;
+      const double Eb = ib + 1 < ni ? rt : fine_value(F, ni, j, k);
+      // y- and z-neighbours of a (colour cb there) and b (colour ca)
+      double Sa, Na, Sb, Nb;
+      if (INNER || j > 0) {
+        Sa = ldp(cb, q - row);
+        Sb = ldp(ca, q - row);
+      } else {
+        Sa = fine_value(F, ia, -1, k);
+        Sb = fine_value(F, ib, -1, k);
+      }
+      if (INNER || j + 1 < nj) {
+        Na = ldp(cb, q + row);
+        Nb = ldp(ca, q + row);
+      } else {
+        Na = fine_value(F, ia, nj, k);
+        Nb = fine_value(F, ib, nj, k);
+      }
+      const double Ba = (INNER || k > 0) ? (cb ? w1m : w0m) : fine_value(F, ia, j, -1);
+      const double Bb = (INNER || k > 0) ? (ca ? w1m : w0m) : fine_value(F, ib, j, -1);
+      const double Ta = (INNER || k + 1 < nk) ? (cb ? w1p : w0p) : fine_value(F, ia, j, nk);
+      const double Tb = (INNER || k + 1 < nk) ? (ca ? w1p : w0p) : fine_value(F, ib, j, nk);
+      Coef<double> c_a, c_b;
+      if (INNER) {
+        c_a.as = c_a.an = c_b.as = c_b.an = cy;
+        c_a.ab = c_a.at = c_b.ab = c_b.at = cz;
+        c_a.aw = (ia == 0 && !px) ? 0.0 : cx;
+        c_a.ae = (ia == ni - 1 && !px) ? 0.0 : cx;
+        c_b.aw = cx;
+        c_b.ae = (ib == ni - 1 && !px) ? 0.0 : cx;
+        c_a.ap = -1.0 * (c_a.aw + c_a.ae + c_a.as + c_a.an + c_a.ab + c_a.at);
+        c_b.ap = -1.0 * (c_b.aw + c_b.ae + c_b.as + c_b.an + c_b.ab + c_b.at);
+      } else {
+        c_a = fine_coef(F, ia, j, k);
+        c_b = fine_coef(F, ib, j, k);
+      }
+      ra = ra - (c_a.aw * Wa + c_a.ae * Ea + c_a.as * Sa + c_a.an * Na + c_a.ab * Ba +
+                 c_a.at * Ta + c_a.ap * Pa);
+      rb = rb - (c_b.aw * Wb + c_b.ae * Eb + c_b.as * Sb + c_b.an * Nb + c_b.ab * Bb +
+                 c_b.at * Tb + c_b.ap * Pb);
+    }
+    if (ok) {
+      (ca ? R1 : R0)[q] = (T)ra;
+      ss += ra * ra;
+      am = fmax(am, fabs(ra));
+    }
+    if (b_ok) {
+      (cb ? R1 : R0)[q] = (T)rb;
+      ss += rb * rb;
+      am = fmax(am, fabs(rb));
+    }
+    w0m = w0c;
+    w1m = w1c;
+    w0c = w0p;
+    w1c = w1p;
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(NTHREADS)
     k_residual(const Tile *__restrict__ tiles, const FineDev *__restrict__ fine,
@@ -256,48 +605,47 @@ __global__ void __launch_bounds__(NTHREADS)
   const Tile t = tiles[blockIdx.x];
   const FineDev &F = fine[t.lev];
   const LevelDev &L = levs[t.lev];
-  const int i = t.x0 + (int)threadIdx.x;
-  const int j = t.y0 + (int)threadIdx.y;
-  const int k = t.z0;
   double ss = 0.0, am = 0.0;
-  if (active[F.blk] && i < F.ni && j < F.nj && k < F.nk) {
-    const int col = (i + j + k) & 1;
-    const size_t q = cidx(F.H, F.nj, i, j, k);
-    double r = F.f[col][q];
-    if (use_p) {
-      const Coef<double> c = fine_coef(F, i, j, k);
-      const double W = fine_value(F, i - 1, j, k), E = fine_value(F, i + 1, j, k);
-      const double S = fine_value(F, i, j - 1, k), N = fine_value(F, i, j + 1, k);
-      const double B = fine_value(F, i, j, k - 1), Tt = fine_value(F, i, j, k + 1);
-      const double P = F.p[col][q];
-      r = r - (c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * Tt + c.ap * P);
-    }
-    ((T *)L.rhs[col])[q] = (T)r;
-    ss = r * r;
-    am = fabs(r);
+  if (active[F.blk]) {
+    if (t.kind == 0)
+      residual_tile<T, true>(t, F, L, use_p, ss, am);
+    else
+      residual_tile<T, false>(t, F, L, use_p, ss, am);
   }
   block_reduce(ss, am);
   if (threadIdx.x == 0 && threadIdx.y == 0) partials[blockIdx.x] = RedPartial{ss, am};
 }
 
-// p = p + (pp - ppref) with ppref = 0 (newsolv_mg.for:39-49); pp is the
-// level-1 correction in buffer `buf`.
+// p = p + (pp - ppref) with ppref = 0 (newsolv_mg.for:39-49), streamed over
+// the colour arrays of one block (pad entries included: they are never read
+// as cells); pp is the level-1 correction in buffer `buf`.  Tile::x0 is the
+// first entry, 4 entries per thread.
 template <class T>
 __global__ void __launch_bounds__(NTHREADS)
     k_update(const Tile *__restrict__ tiles, const FineDev *__restrict__ fine,
              const LevelDev *__restrict__ levs, const int *__restrict__ active, int buf) {
+  typedef typename Vec2<T>::type v2t;
+  typedef typename Vec2<double>::type v2d;
   const Tile t = tiles[blockIdx.x];
   const FineDev &F = fine[t.lev];
   const LevelDev &L = levs[t.lev];
   if (!active[F.blk]) return;
-  const int i = t.x0 + (int)threadIdx.x;
-  const int j = t.y0 + (int)threadIdx.y;
-  const int k = t.z0;
-  if (i >= F.ni || j >= F.nj || k >= F.nk) return;
-  const int col = (i + j + k) & 1;
-  const size_t q = cidx(F.H, F.nj, i, j, k);
-  const double pp = (double)((const T *)L.phi[buf][col])[q];
-  F.p[col][q] = F.p[col][q] + (pp - 0.0);
+  const unsigned n = (unsigned)F.H * (unsigned)F.nj * (unsigned)F.nk;
+  const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
+  for (int c = 0; c < 2; ++c) {
+    gmem<double> *P = (gmem<double> *)F.p[c];
+    const gmem<T> *Q = (const gmem<T> *)L.phi[buf][c];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const unsigned e = (unsigned)t.x0 + 2u * (unsigned)(tid + h * NTHREADS);
+      if (e >= n) continue;
+      const v2t pp = *(const gmem<v2t> *)(Q + e);
+      v2d p = *(gmem<v2d> *)(P + e);
+      p.x = p.x + ((double)pp.x - 0.0);
+      p.y = p.y + ((double)pp.y - 0.0);
+      *(gmem<v2d> *)(P + e) = p;
+    }
+  }
 }
 
 // per-block reduction of the partials of k_residual, fixed order
@@ -502,10 +850,10 @@ inline dim3 tile_block() { return dim3(TILE_X, TILE_Y, 1); }
 
 // ---- launchers -----------------------------------------------------------
 template <class T>
-void launch_rb_pass(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                    const int *active, int colour, int src, int zero) {
+void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                     const int *active, int colour, int src, int zero) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_rb_pass<T>, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour,
+  hipLaunchKernelGGL(k_rb_sweep<T>, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour,
                      src, zero);
 }
 
@@ -597,10 +945,10 @@ void launch_remove_mean(hipStream_t s, const FineDev *fd, int ni, int nj, int nk
 }
 
 // explicit instantiations
-template void launch_rb_pass<double>(hipStream_t, const Tile *, int, const LevelDev *, const int *,
+template void launch_rb_sweep<double>(hipStream_t, const Tile *, int, const LevelDev *,
+                                      const int *, int, int, int);
+template void launch_rb_sweep<float>(hipStream_t, const Tile *, int, const LevelDev *, const int *,
                                      int, int, int);
-template void launch_rb_pass<float>(hipStream_t, const Tile *, int, const LevelDev *, const int *,
-                                    int, int, int);
 template void launch_restrict<double, double>(hipStream_t, const Tile *, int, const LevelDev *,
                                               const int *, int);
 template void launch_restrict<float, float>(hipStream_t, const Tile *, int, const LevelDev *,

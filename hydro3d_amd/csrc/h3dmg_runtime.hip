@@ -46,6 +46,7 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 struct TileSet {
   std::vector<Tile> h;
   Tile *d = nullptr;
+  Tile *owned = nullptr;  // smoother sets: separately allocated (re-tunable)
   int n() const { return (int)h.size(); }
 };
 
@@ -53,7 +54,57 @@ struct TileSet {
 void add_tiles(TileSet &ts, int lev, int nx, int ny, int nz) {
   for (int z = 0; z < nz; ++z)
     for (int y = 0; y < ny; y += TILE_Y)
-      for (int x = 0; x < nx; x += TILE_X) ts.h.push_back(Tile{lev, x, y, z});
+      for (int x = 0; x < nx; x += TILE_X) ts.h.push_back(Tile{lev, x, y, z, z + 1});
+}
+
+// Smoother lanes per row: two entries per lane, a power of two so that a
+// wavefront holds 64/lpr whole rows when a row is short.
+int lanes_per_row(int H) {
+  int l = 1;
+  while (l < 64 && 2 * l < H) l *= 2;
+  return l;
+}
+
+// Plane-marching tiles: each workgroup walks kb planes.  kb is chosen so
+// that a level still spreads over >= ~4096 workgroups (16 per CU) when it
+// can; the autotuner may override it per level (h3dmg_set_march).
+int march_depth(int tiles_xy, int nz) {
+  long kb = (long)tiles_xy * nz / 4096;
+  if (kb < 1) kb = 1;
+  if (kb > 8) kb = 8;
+  return (int)kb;
+}
+
+// Smoother tiles of level descriptor `lev`, in plane-chunk order (the XCD
+// remap then hands every XCD a contiguous slab): per chunk of kb planes the
+// x-face tiles of those planes (kind 2, 256 rows each) and the march tiles
+// (2*lpr entries x 4*(64/lpr) rows x kb planes), kind 0 when every y/z
+// neighbour of the tile is inside the block, kind 1 otherwise.
+void add_tiles_march(TileSet &ts, int lev, const LevelDev &L, int kb_req = 0) {
+  const int xs = 2 * L.lpr, ys = TILE_Y * (64 / L.lpr);
+  const int nx = L.H, ny = L.nj, nz = L.nk;
+  const int txy = ((nx + xs - 1) / xs) * ((ny + ys - 1) / ys);
+  const int kb = kb_req > 0 ? std::min(kb_req, nz) : march_depth(txy, nz);
+  for (int z = 0; z < nz; z += kb) {
+    const int z1 = std::min(z + kb, nz);
+    for (int zz = z; zz < z1; ++zz)
+      for (int y = 0; y < ny; y += NTHREADS) ts.h.push_back(Tile{lev, y, zz, 0, 0, 2});
+    for (int y = 0; y < ny; y += ys)
+      for (int x = 0; x < nx; x += xs) {
+        const bool inner = L.ni >= 4 && y >= 1 && y + ys <= ny - 1 && z >= 1 && z1 <= nz - 1;
+        ts.h.push_back(Tile{lev, x, y, z, z1, inner ? 0 : 1});
+      }
+  }
+}
+
+// Prolongation tiles of fine level descriptor `lev` (k_prolong): fine entries
+// x0.. (64 per tile) x coarse row pairs m = y0.. (TILE_Y per tile) x one
+// coarse plane pair n = z0, with m in [0, nj/2] and n in [0, nk/2].
+void add_tiles_prolong(TileSet &ts, int lev, const LevelDev &F) {
+  const int ncj = F.nj / 2, nck = F.nk / 2, nfx = F.ni / 2;
+  for (int n = 0; n <= nck; ++n)
+    for (int m = 0; m <= ncj; m += TILE_Y)
+      for (int x = 0; x < nfx; x += TILE_X) ts.h.push_back(Tile{lev, x, m, n, n + 1, 0});
 }
 
 struct BlockInfo {
@@ -95,7 +146,8 @@ struct h3dmg_handle {
   std::vector<TileSet> prol_nc, prol_c;     // fine natural at g; L_b > g+1 / == g+1
   std::vector<TileSet> nat_nc, nat_c;       // natural at g (zero fill)
   std::vector<TileSet> coarse_list;         // blocks whose coarsest level is g
-  TileSet fine_tiles;                       // natural level 1, lev = block
+  TileSet fine_tiles;                       // residual tiles of level 1, lev = block
+  TileSet update_tiles;                     // p += pp tiles, lev = block
   std::vector<int> tile_begin;              // per block range in fine_tiles
   int *d_tile_begin = nullptr;
   RedPartial *d_partials = nullptr;
@@ -107,6 +159,8 @@ struct h3dmg_handle {
 
   double *d_stage = nullptr;  // dense FP64 staging (largest padded level 1)
   size_t stage_elems = 0;
+
+  std::vector<int> march_kb;  // planes per smoother workgroup per level (0 = auto)
 
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -122,12 +176,16 @@ bool is_mixed(const h3dmg_handle *h) { return h->prm.precision == H3DMG_MIXED; }
 template <class T>
 void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool &zero) {
   for (int s = 0; s < nsweep; ++s) {
-    launch_rb_pass<T>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, 0, buf, zero ? 1 : 0);
-    launch_rb_pass<T>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, 1, buf, zero ? 1 : 0);
+    for (int colour = 0; colour < 2; ++colour)
+      launch_rb_sweep<T>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, colour, buf,
+                         zero ? 1 : 0);
     buf ^= 1;
     zero = false;
   }
 }
+
+// (Re)build and upload the smoother tiles of level g for both block groups.
+h3dmg_status upload_relax(h3dmg_handle *h, int g);
 
 void relax_any(h3dmg_handle *h, bool is64, const TileSet &ts, int nsweep, int &buf, bool &zero) {
   if (ts.n() == 0) {
@@ -258,12 +316,12 @@ void enqueue_vcycle(h3dmg_handle *h) {
 // p += pp, r = f - A p, per-block ||r||, max|r| (one outer iteration tail)
 void enqueue_update_residual(h3dmg_handle *h, bool update) {
   const bool f64 = !is_mixed(h);
-  const TileSet &ft = h->fine_tiles;
+  const TileSet &ft = h->fine_tiles, &ut = h->update_tiles;
   if (update) {
     if (f64)
-      launch_update<double>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active, h->pp_buf);
+      launch_update<double>(h->stream, ut.d, ut.n(), h->d_fine, h->d_levs, h->d_active, h->pp_buf);
     else
-      launch_update<float>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active, h->pp_buf);
+      launch_update<float>(h->stream, ut.d, ut.n(), h->d_fine, h->d_levs, h->d_active, h->pp_buf);
   }
   if (f64)
     launch_residual<double>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
@@ -373,7 +431,8 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       L.ni = B.spec.ni >> (g - 1);
       L.nj = B.spec.nj >> (g - 1);
       L.nk = B.spec.nk >> (g - 1);
-      L.H = (L.ni + 1) / 2;
+      L.H = (((L.ni + 1) / 2) + 1) & ~1;  // even: aligned 2-entry accesses
+      L.lpr = lanes_per_row(L.H);
       L.per = B.per;
       L.blk = b;
       const double s = (double)(1 << (g - 1));
@@ -477,33 +536,46 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       const int d = B.desc[g];
       const LevelDev &L = h->levs[d];
       if (g < B.L) {
-        add_tiles(h->relax_nc[g], d, L.H, L.nj, L.nk);
         add_tiles(h->nat_nc[g], d, L.ni, L.nj, L.nk);
-        if (g + 1 < B.L)
-          add_tiles(h->prol_nc[g], d, L.ni, L.nj, L.nk);
-        else
-          add_tiles(h->prol_c[g], d, L.ni, L.nj, L.nk);
+        add_tiles_prolong(g + 1 < B.L ? h->prol_nc[g] : h->prol_c[g], d, L);
         if (g >= 2) add_tiles(h->restr_nc[g], d, L.ni, L.nj, L.nk);
       } else {
-        add_tiles(h->relax_c[g], d, L.H, L.nj, L.nk);
         add_tiles(h->nat_c[g], d, L.ni, L.nj, L.nk);
         if (g >= 2) add_tiles(h->restr_c[g], d, L.ni, L.nj, L.nk);
       }
     }
+  // finest level: residual tiles (one colour-array entry of both colours per
+  // lane, lpr lanes per row, kb planes; kind 0 = y/z interior) and p += pp
+  // tiles (1024 entries each)
   h->tile_begin.assign(nblocks + 1, 0);
   for (int b = 0; b < nblocks; ++b) {
     const LevelDev &L = h->levs[b];
     h->tile_begin[b] = h->fine_tiles.n();
-    add_tiles(h->fine_tiles, b, L.ni, L.nj, L.nk);
+    const int lpr = lanes_per_row(2 * L.H);
+    const int xs = lpr, ys = TILE_Y * (64 / lpr);
+    const int txy = ((L.H + xs - 1) / xs) * ((L.nj + ys - 1) / ys);
+    const int kb = march_depth(txy, L.nk);
+    for (int z = 0; z < L.nk; z += kb) {
+      const int z1 = std::min(z + kb, L.nk);
+      for (int y = 0; y < L.nj; y += ys)
+        for (int x = 0; x < L.H; x += xs) {
+          const bool inner = y >= 1 && y + ys <= L.nj - 1 && z >= 1 && z1 <= L.nk - 1;
+          h->fine_tiles.h.push_back(Tile{b, x, y, z, z1, inner ? 0 : 1});
+        }
+    }
+    const long entries = (long)L.H * L.nj * L.nk;
+    for (long e = 0; e < entries; e += 4 * NTHREADS)
+      h->update_tiles.h.push_back(Tile{b, (int)e, 0, 0, 0, 0});
   }
   h->tile_begin[nblocks] = h->fine_tiles.n();
 
   std::vector<TileSet *> all;
   for (int g = 1; g <= Lm; ++g)
-    for (auto *v : {&h->relax_nc, &h->relax_c, &h->restr_nc, &h->restr_c, &h->prol_nc, &h->prol_c,
-                    &h->nat_nc, &h->nat_c, &h->coarse_list})
+    for (auto *v : {&h->restr_nc, &h->restr_c, &h->prol_nc, &h->prol_c, &h->nat_nc, &h->nat_c,
+                    &h->coarse_list})
       all.push_back(&(*v)[g]);
   all.push_back(&h->fine_tiles);
+  all.push_back(&h->update_tiles);
   std::vector<size_t> toff(all.size());
   for (size_t q = 0; q < all.size(); ++q) {
     toff[q] = off;
@@ -546,6 +618,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     F.nk = L.nk;
     F.H = L.H;
     F.per = L.per;
+    F.lpr = lanes_per_row(2 * L.H);
     F.blk = b;
     F.cx = L.cx;
     F.cy = L.cy;
@@ -619,7 +692,34 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     const size_t half = (size_t)L.H * L.nj * L.nk * 8;
     for (int q = 0; q < 4; ++q) HIP_TRY(hipMemset(A + foff[4 * b + q], 0, half));
   }
+  h->march_kb.assign(Lm + 1, 0);
+  for (int g = 1; g <= Lm; ++g) {
+    h3dmg_status st = upload_relax(h, g);
+    if (st != H3DMG_OK) return st;
+  }
   HIP_TRY(hipDeviceSynchronize());
+  return H3DMG_OK;
+}
+
+h3dmg_status upload_relax(h3dmg_handle *h, int g) {
+  for (int grp = 0; grp < 2; ++grp) {
+    TileSet &ts = grp == 0 ? h->relax_nc[g] : h->relax_c[g];
+    ts.h.clear();
+    for (int b = 0; b < h->nb; ++b) {
+      const BlockInfo &B = h->blk[b];
+      if (B.L < g || (grp == 1) != (B.L == g)) continue;
+      const LevelDev &L = h->levs[B.desc[g]];
+      add_tiles_march(ts, B.desc[g], L, h->march_kb[g]);
+    }
+    Tile *d = nullptr;
+    if (!ts.h.empty()) {
+      HIP_TRY(hipMalloc((void **)&d, ts.h.size() * sizeof(Tile)));
+      HIP_TRY(hipMemcpy(d, ts.h.data(), ts.h.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    }
+    if (ts.owned) HIP_TRY(hipFree(ts.owned));
+    ts.owned = d;
+    ts.d = d;
+  }
   return H3DMG_OK;
 }
 
@@ -726,6 +826,9 @@ void h3dmg_destroy(h3dmg_handle *h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graph(h);
+  for (auto *v : {&h->relax_nc, &h->relax_c})
+    for (TileSet &ts : *v)
+      if (ts.owned) (void)hipFree(ts.owned);
   if (h->h_stats) (void)hipHostFree(h->h_stats);
   if (h->arena) (void)hipFree(h->arena);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
@@ -1134,6 +1237,22 @@ h3dmg_status h3dmg_dbg_prolong(h3dmg_handle *h, int level) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
   return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes) {
+  if (!h || level < 1 || level > h->Lmax) return fail(H3DMG_ERR_ARG, "bad level");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  drop_graph(h);
+  h->march_kb[level] = planes > 0 ? planes : 0;
+  return upload_relax(h, level);
+}
+
+int h3dmg_get_march(const h3dmg_handle *h, int level) {
+  if (!h || level < 1 || level > h->Lmax) return -1;
+  for (const TileSet *ts : {&h->relax_nc[level], &h->relax_c[level]})
+    for (const Tile &t : ts->h)
+      if (t.kind != 2) return t.z1 - t.z0;
+  return 0;
 }
 
 }  // extern "C"

@@ -161,6 +161,17 @@ class MultigridSolver:
     def set_stream(self, stream_ptr: int | None):
         L.check(self.lib.h3dmg_set_stream(self.h, ctypes.c_void_p(stream_ptr or 0)))
 
+    # ---- tuning
+    def set_march(self, level, planes):
+        L.check(self.lib.h3dmg_set_march(self.h, level, planes))
+
+    def get_march(self, level):
+        return int(self.lib.h3dmg_get_march(self.h, level))
+
+    @property
+    def nlevels_max(self):
+        return max(b["nlev"] for b in self.blocks)
+
     # ---- operator seams (device tensors in the oracle's level layout)
     def dbg_put(self, b, level, which, t):
         L.check(self.lib.h3dmg_dbg_put(self.h, b, level, which, ctypes.c_void_p(t.data_ptr())))

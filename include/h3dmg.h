@@ -156,6 +156,13 @@ h3dmg_status h3dmg_remove_mean(h3dmg_handle *h);
 /* Stream the handle launches on (a hipStream_t); null = the handle's own. */
 h3dmg_status h3dmg_set_stream(h3dmg_handle *h, void *stream);
 
+/* ---- tuning (used by the autotuner, hydro3d_amd/autotune.py) -------------
+ * Planes each smoother workgroup walks at `level` (every block); 0 restores
+ * the built-in choice.  Drops a captured graph.  h3dmg_get_march returns the
+ * depth in use (or -1 for a bad level). */
+h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes);
+int h3dmg_get_march(const h3dmg_handle *h, int level);
+
 /* ---- test seams: one operator at a time on level arrays ------------------
  * Level arrays cross this seam as dense device arrays in the layout of the
  * oracle: phi padded (n+2)^3 i-fastest, rhs interior.  Ghost cells of an
