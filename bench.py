@@ -57,8 +57,9 @@ def timed_solves(S, f_dev, zero_dev, steps, warmup):
 
 
 def smoother_roofline(S, level_cells, word, passes=40):
-    """Average duration of one finest-level colour half-sweep (k_rb_pass),
-    timed with HIP events on the stream the kernels run on."""
+    """Average duration of one finest-level colour half-sweep (k_rb_sweep,
+    the kernel the solve spends most of its time in), timed with HIP events
+    on the stream the kernels run on."""
     import torch
     stream = torch.cuda.Stream()
     S.set_stream(stream.cuda_stream)
@@ -78,6 +79,36 @@ def smoother_roofline(S, level_cells, word, passes=40):
     bytes_per_launch = 1.5 * word * level_cells
     achieved = bytes_per_launch / (ms * 1e-3) / 1e9
     return ms, bytes_per_launch, achieved
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rb_sweep_f32_256cube_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per level-1 FP32 half-sweep from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (calibrated against a plain copy), or
+    None when that summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(nbytes=1 << 28, reps=5):
+    """A plain device copy of known size, for calibrating the FETCH_SIZE and
+    WRITE_SIZE counters (bytes they report per byte actually moved)."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
+                      "reps": reps}), flush=True)
 
 
 def cpu_baseline(c, fp64_cycles):
@@ -124,6 +155,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver is single-GPU (independent blocks shard "
@@ -136,6 +169,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        copy_probe()
+        return
     c = configs.single_256()
     spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
     N = int(np.prod(c["n"]))
@@ -214,9 +250,9 @@ def main():
                        "fp64_p50": round(float(np.percentile(lat64, 50)) * 1e3, 3),
                        "fp64_p99": round(float(np.percentile(lat64, 99)) * 1e3, 3)},
         "pressure_rel_err_vs_converged_fp64": {"mixed": perr, "fp64_at_1e-8": perr64},
-        "roofline": {"bound": "hbm", "kernel": "k_rb_pass<float> (256^3 colour half-sweep)",
+        "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<float> (256^3 colour half-sweep)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "avg_launch_us": round(ms_pass * 1e3, 2),
                      "algorithmic_bytes_per_launch": int(bytes_pass),
                      "fp64_pass": {"achieved": round(ach64, 1), "avg_launch_us": round(ms64 * 1e3, 2),

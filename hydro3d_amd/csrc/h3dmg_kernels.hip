@@ -318,43 +318,174 @@ __global__ void __launch_bounds__(NTHREADS)
 // mgrestr (mgsolver.for:693-756): fine residual
 //   resf = rhs - (aw*W + ae*E + as*S + an*N + ab*B + at*T + ap*P)
 // and coarse rhs = 0.125 * (sum of the 8 children, i fastest, then j, k).
-// One thread per coarse cell; the tile's `lev` is the coarse descriptor.
 // The coarse phi is not touched: the next relax of the coarse level runs with
 // zero = 1, which is the zeroing of mgsolver.for:655-662.
+//
+// Work layout.  The x-children 2l, 2l+1 of coarse cell l are entry l of the
+// two fine colour arrays ("a" = cell 2l, "b" = cell 2l+1; in row j, plane k,
+// a has colour (j+k)&1).  A lane owns coarse cell (l, jc) of a coarse column:
+// entry l of fine rows 2jc, 2jc+1, and walks the fine planes 2*z0 .. 2*z1-1,
+// keeping planes k-1, k, k+1 of its two rows in a register window; rows 2jc-1
+// and 2jc+2 are loaded, x-neighbours come from the adjacent lanes (DPP
+// shift).  The eight residuals arrive in the reference's summation order
+// (a, b of row 2jc, a, b of row 2jc+1; plane 2kc, then 2kc+1).  C.lpr1 lanes
+// cover a coarse row; the tile's `lev` is the coarse descriptor.  Kind 0:
+// every fine row/plane the tile reads is inside the block (no ghost logic in
+// y and z).
+template <class Tf, class Tc, bool INNER>
+__device__ __forceinline__ void restrict_tile(const Tile &t, const LevelDev &C, const LevelDev &F,
+                                              int src_f) {
+  const int lpr = C.lpr1;
+  const int lane = (int)threadIdx.x;
+  const int rl = lane & (lpr - 1);
+  const int l = t.x0 + rl;
+  const int jc = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
+  const int ni = F.ni, nj = F.nj, nk = F.nk;
+  const bool ok = l < C.ni && jc < C.nj;
+  const unsigned row = (unsigned)F.H, plane = (unsigned)F.H * (unsigned)nj;
+  const int j0 = 2 * min(jc, C.nj - 1);  // even: colours of a row pair are uniform
+  const unsigned cbase = (unsigned)j0 * row + (unsigned)min(l, C.ni - 1);
+  const gmem<Tf> *P0 = (const gmem<Tf> *)F.phi[src_f][0];
+  const gmem<Tf> *P1 = (const gmem<Tf> *)F.phi[src_f][1];
+  const gmem<Tf> *R0 = (const gmem<Tf> *)F.rhs[0];
+  const gmem<Tf> *R1 = (const gmem<Tf> *)F.rhs[1];
+  const bool tile_left = l > 0 && rl == 0;
+  const bool tile_right = rl == lpr - 1 && l + 1 < C.ni;
+  const bool px = F.per & 1;
+  const Tf cx = (Tf)F.cx, cy = (Tf)F.cy, cz = (Tf)F.cz;
+  const int ia = 2 * l, ib = ia + 1;
+  auto ldc = [&](int c, unsigned off) -> Tf { return c ? P1[off] : P0[off]; };
+  auto ghost = [&](int x, int y, int z) -> Tf { return level_value<Tf>(F, src_f, x, y, z); };
+  // a and b of row j0 + r at plane k (zeros outside the block: never used)
+  auto own = [&](int r, int k, Tf &a, Tf &b) {
+    if (!INNER && (unsigned)k >= (unsigned)nk) {
+      a = b = Tf(0);
+      return;
+    }
+    const unsigned q = (unsigned)k * plane + cbase + (unsigned)r * row;
+    const int ca = (r + k) & 1;
+    a = ldc(ca, q);
+    b = ldc(ca ^ 1, q);
+  };
+  // x-direction coefficients of a and b (only walls at the x faces)
+  const Tf aw_a = (ia == 0 && !px) ? Tf(0) : cx;
+  const Tf ae_a = (ia == ni - 1 && !px) ? Tf(0) : cx;
+  const Tf ae_b = (ib == ni - 1 && !px) ? Tf(0) : cx;
+
+  const int k0 = 2 * t.z0, k1 = 2 * t.z1;
+  Tf am[2], bm[2], ac[2], bc[2], ap[2], bp[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    own(r, k0 - 1, am[r], bm[r]);
+    own(r, k0, ac[r], bc[r]);
+  }
+  Tc acc = Tc(0);
+  for (int k = k0; k < k1; ++k) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) own(r, k + 1, ap[r], bp[r]);
+    const unsigned q0 = (unsigned)k * plane + cbase;
+    const int c0 = k & 1;  // colour of a in row j0 (j0 even)
+    // row j0-1 (colour of a there: c0^1) and row j0+2 (colour c0)
+    Tf sa, sb, na, nb;
+    if (INNER || j0 > 0) {
+      sa = ldc(c0 ^ 1, q0 - row);
+      sb = ldc(c0, q0 - row);
+    } else {
+      sa = ghost(ia, -1, k);
+      sb = ghost(ib, -1, k);
+    }
+    if (INNER || j0 + 2 < nj) {
+      na = ldc(c0, q0 + 2 * row);
+      nb = ldc(c0 ^ 1, q0 + 2 * row);
+    } else {
+      na = ghost(ia, nj, k);
+      nb = ghost(ib, nj, k);
+    }
+    Tf fa[2], fb[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int ca = (r + k) & 1;
+      const unsigned q = q0 + (unsigned)r * row;
+      fa[r] = ca ? R1[q] : R0[q];
+      fb[r] = ca ? R0[q] : R1[q];
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int j = j0 + r;
+      const unsigned q = q0 + (unsigned)r * row;
+      const int ca = (r + k) & 1;
+      // cell 2l-1 is lane l-1's b, cell 2l+2 is lane l+1's a
+      Tf lf = wave_shift(bc[r], 1), rt = wave_shift(ac[r], 0);
+      if (tile_left) lf = ldc(ca ^ 1, q - 1);
+      if (tile_right) rt = ldc(ca, q + 1);
+      const Tf Wa = ia == 0 ? ghost(-1, j, k) : lf;
+      const Tf Ea = bc[r];
+      const Tf Wb = ac[r];
+      const Tf Eb = ib + 1 < ni ? rt : ghost(ni, j, k);
+      const Tf Sa = r ? ac[0] : sa, Sb = r ? bc[0] : sb;
+      const Tf Na = r ? na : ac[1], Nb = r ? nb : bc[1];
+      Tf Ba = am[r], Bb = bm[r], Ta = ap[r], Tb = bp[r];
+      Coef<Tf> c_a, c_b;
+      if (INNER) {
+        c_a.as = c_a.an = c_b.as = c_b.an = cy;
+        c_a.ab = c_a.at = c_b.ab = c_b.at = cz;
+        c_a.aw = aw_a;
+        c_a.ae = ae_a;
+        c_b.aw = cx;
+        c_b.ae = ae_b;
+        c_a.ap = Tf(-1) * (c_a.aw + c_a.ae + c_a.as + c_a.an + c_a.ab + c_a.at);
+        c_b.ap = Tf(-1) * (c_b.aw + c_b.ae + c_b.as + c_b.an + c_b.ab + c_b.at);
+      } else {
+        c_a = cell_coef<Tf>(F, ia, j, k);
+        c_b = cell_coef<Tf>(F, ib, j, k);
+        if (k == 0) {
+          Ba = ghost(ia, j, -1);
+          Bb = ghost(ib, j, -1);
+        }
+        if (k + 1 == nk) {
+          Ta = ghost(ia, j, nk);
+          Tb = ghost(ib, j, nk);
+        }
+      }
+      const Tf Pa = ac[r], Pb = bc[r];
+      const Tf ra = fa[r] - (c_a.aw * Wa + c_a.ae * Ea + c_a.as * Sa + c_a.an * Na +
+                             c_a.ab * Ba + c_a.at * Ta + c_a.ap * Pa);
+      const Tf rb = fb[r] - (c_b.aw * Wb + c_b.ae * Eb + c_b.as * Sb + c_b.an * Nb +
+                             c_b.ab * Bb + c_b.at * Tb + c_b.ap * Pb);
+      acc = (r == 0 && (k & 1) == 0) ? (Tc)ra : acc + (Tc)ra;
+      acc = acc + (Tc)rb;
+      (void)ca;
+    }
+    if (k & 1) {
+      const int kc = k >> 1;
+      if (ok) {
+        gmem<Tc> *D = (gmem<Tc> *)C.rhs[(l + jc + kc) & 1];
+        D[((unsigned)kc * (unsigned)C.nj + (unsigned)jc) * (unsigned)C.H + (unsigned)(l >> 1)] =
+            Tc(0.125) * acc;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      am[r] = ac[r];
+      bm[r] = bc[r];
+      ac[r] = ap[r];
+      bc[r] = bp[r];
+    }
+  }
+}
+
 template <class Tf, class Tc>
 __global__ void __launch_bounds__(NTHREADS)
     k_restrict(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
                const int *__restrict__ active, int src_f) {
-  const Tile t = tiles[blockIdx.x];
+  const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
   const LevelDev &C = levs[t.lev];
   if (!active[C.blk]) return;
   const LevelDev &F = levs[C.finer];
-  const int ic = t.x0 + (int)threadIdx.x;
-  const int jc = t.y0 + (int)threadIdx.y;
-  const int kc = t.z0;
-  if (ic >= C.ni || jc >= C.nj || kc >= C.nk) return;
-
-  Tf res[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int i = 2 * ic + (q & 1), j = 2 * jc + ((q >> 1) & 1), k = 2 * kc + (q >> 2);
-    const Coef<Tf> c = cell_coef<Tf>(F, i, j, k);
-    const Tf W = level_value<Tf>(F, src_f, i - 1, j, k);
-    const Tf E = level_value<Tf>(F, src_f, i + 1, j, k);
-    const Tf S = level_value<Tf>(F, src_f, i, j - 1, k);
-    const Tf N = level_value<Tf>(F, src_f, i, j + 1, k);
-    const Tf B = level_value<Tf>(F, src_f, i, j, k - 1);
-    const Tf Tt = level_value<Tf>(F, src_f, i, j, k + 1);
-    const size_t fq = cidx(F.H, F.nj, i, j, k);
-    const int fc = (i + j + k) & 1;
-    const Tf P = ((const Tf *)F.phi[src_f][fc])[fq];
-    const Tf r = ((const Tf *)F.rhs[fc])[fq];
-    res[q] = r - (c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * Tt + c.ap * P);
-  }
-  Tc acc = (Tc)res[0];
-#pragma unroll
-  for (int q = 1; q < 8; ++q) acc = acc + (Tc)res[q];
-  ((Tc *)C.rhs[(ic + jc + kc) & 1])[cidx(C.H, C.nj, ic, jc, kc)] = Tc(0.125) * acc;
+  if (t.kind == 0)
+    restrict_tile<Tf, Tc, true>(t, C, F, src_f);
+  else
+    restrict_tile<Tf, Tc, false>(t, C, F, src_f);
 }
 
 // ---------------------------------------------------------------------------

@@ -68,8 +68,8 @@ int lanes_per_row(int H) {
 // Plane-marching tiles: each workgroup walks kb planes.  kb is chosen so
 // that a level still spreads over >= ~4096 workgroups (16 per CU) when it
 // can; the autotuner may override it per level (h3dmg_set_march).
-int march_depth(int tiles_xy, int nz) {
-  long kb = (long)tiles_xy * nz / 4096;
+int march_depth(int tiles_xy, int nz, long target = 4096) {
+  long kb = (long)tiles_xy * nz / target;
   if (kb < 1) kb = 1;
   if (kb > 8) kb = 8;
   return (int)kb;
@@ -92,6 +92,25 @@ void add_tiles_march(TileSet &ts, int lev, const LevelDev &L, int kb_req = 0) {
     for (int y = 0; y < ny; y += ys)
       for (int x = 0; x < nx; x += xs) {
         const bool inner = L.ni >= 4 && y >= 1 && y + ys <= ny - 1 && z >= 1 && z1 <= nz - 1;
+        ts.h.push_back(Tile{lev, x, y, z, z1, inner ? 0 : 1});
+      }
+  }
+}
+
+// Restriction tiles of coarse level descriptor `lev` (k_restrict): lpr1
+// coarse cells x 4*(64/lpr1) coarse rows x kb coarse planes; kind 0 when every
+// fine row and plane the tile reads is inside the block.  Deeper columns than
+// the smoother's (~2048 workgroups): a column re-reads two fine planes.
+void add_tiles_restrict(TileSet &ts, int lev, const LevelDev &C) {
+  const int xs = C.lpr1, ys = TILE_Y * (64 / C.lpr1);
+  const int nx = C.ni, ny = C.nj, nz = C.nk;
+  const int txy = ((nx + xs - 1) / xs) * ((ny + ys - 1) / ys);
+  const int kb = march_depth(txy, nz, 2048);
+  for (int z = 0; z < nz; z += kb) {
+    const int z1 = std::min(z + kb, nz);
+    for (int y = 0; y < ny; y += ys)
+      for (int x = 0; x < nx; x += xs) {
+        const bool inner = y >= 1 && y + ys <= ny - 1 && z >= 1 && z1 <= nz - 1;
         ts.h.push_back(Tile{lev, x, y, z, z1, inner ? 0 : 1});
       }
   }
@@ -433,6 +452,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       L.nk = B.spec.nk >> (g - 1);
       L.H = (((L.ni + 1) / 2) + 1) & ~1;  // even: aligned 2-entry accesses
       L.lpr = lanes_per_row(L.H);
+      L.lpr1 = lanes_per_row(2 * L.ni);
       L.per = B.per;
       L.blk = b;
       const double s = (double)(1 << (g - 1));
@@ -538,10 +558,10 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       if (g < B.L) {
         add_tiles(h->nat_nc[g], d, L.ni, L.nj, L.nk);
         add_tiles_prolong(g + 1 < B.L ? h->prol_nc[g] : h->prol_c[g], d, L);
-        if (g >= 2) add_tiles(h->restr_nc[g], d, L.ni, L.nj, L.nk);
+        if (g >= 2) add_tiles_restrict(h->restr_nc[g], d, L);
       } else {
         add_tiles(h->nat_c[g], d, L.ni, L.nj, L.nk);
-        if (g >= 2) add_tiles(h->restr_c[g], d, L.ni, L.nj, L.nk);
+        if (g >= 2) add_tiles_restrict(h->restr_c[g], d, L);
       }
     }
   // finest level: residual tiles (one colour-array entry of both colours per
