@@ -64,6 +64,14 @@ def build_fortran() -> str | None:
     out = os.path.join(fdir, "_build")
     os.makedirs(out, exist_ok=True)
     _run([FLANG, "-O2", "-module-dir", out, "-c", src, "-o", os.path.join(out, "h3dmg_mod.o")])
+    # the drop-in newsolv_mg_h3d needs the reference's modules (vars, mpi,
+    # multidata): compile-checked where the reference objects were built
+    ref_mods = os.path.join(os.path.dirname(PKG), "oracle", "_ref", "obj")
+    if os.path.exists(os.path.join(ref_mods, "multidata.mod")):
+        _run([FLANG, "-O2", "-ffp-contract=off", "-fdefault-real-8", "-fdefault-double-8",
+              "-I", ref_mods, "-I", out, "-I/opt/conda/include", "-c",
+              os.path.join(fdir, "newsolv_mg_h3d.f90"), "-o",
+              os.path.join(out, "newsolv_mg_h3d.o")])
     prog = os.path.join(fdir, "h3dmg_fortran_demo.f90")
     if os.path.exists(prog):
         exe = os.path.join(fdir, "h3dmg_fortran_demo")
