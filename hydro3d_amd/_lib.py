@@ -42,6 +42,7 @@ _vp = ctypes.c_void_p
 _dp = _P(ctypes.c_double)
 _ip = _P(ctypes.c_int)
 _i = ctypes.c_int
+_d = ctypes.c_double
 SIGNATURES = {
     "h3dmg_default_params": (None, [_P(H3dParams)]),
     "h3dmg_create": (_i, [_P(H3dBlock), _i, _P(H3dParams), _P(_vp)]),
@@ -62,6 +63,10 @@ SIGNATURES = {
     "h3dmg_set_stream": (_i, [_vp, _vp]),
     "h3dmg_set_march": (_i, [_vp, _i, _i]),
     "h3dmg_get_march": (_i, [_vp, _i]),
+    "h3dmg_set_tail": (_i, [_vp, _i]),
+    "h3dmg_set_block_tol": (_i, [_vp, _i, _d]),
+    "h3dmg_set_block_enabled": (_i, [_vp, _i, _i]),
+    "h3dmg_get_tail": (_i, [_vp]),
     "h3dmg_dbg_put": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_get": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_relax": (_i, [_vp, _i, _i]),

@@ -29,6 +29,9 @@ namespace h3dmg {
 constexpr int TILE_X = 64;  // one wavefront along the contiguous axis
 constexpr int TILE_Y = 4;   // four wavefronts per workgroup
 constexpr int NTHREADS = TILE_X * TILE_Y;
+constexpr int TAIL_THREADS = 512;  // V-cycle tail: one workgroup per block
+constexpr int TAIL_U = 4;           // cells per thread with loads in flight together
+constexpr int TAIL_MAX_LEVELS = 16;
 
 struct LevelDev {
   int ni, nj, nk;  // interior cells
@@ -99,6 +102,18 @@ void launch_finalize(hipStream_t s, const int *tile_begin, int nblocks, const Re
 void launch_coarse_exact(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                          const int *active, const double *const *eig_q,
                          const double *const *eig_l, double *const *scratch, int buf);
+
+// levels >= the tail level of every block in one launch, one workgroup per
+// block, every level in LDS (Tile::lev = the block's descriptor at the tail
+// level); every level ends in buffer (nu1+nu2)&1.  vtail_lds_bytes gives the
+// LDS a block needs from descriptor `top` down (host copies of the
+// descriptors).
+void launch_vtail(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                  const int *active, const double *const *eig_q, const double *const *eig_l,
+                  int nu1, int nu2, int exact, int mixed, size_t lds_bytes);
+size_t vtail_lds_bytes(const LevelDev *levs_host, int top, bool mixed, bool exact);
+// permit `lds_bytes` of dynamic LDS for the tail kernel
+hipError_t vtail_allow_lds(size_t lds_bytes);
 
 // layout conversion: dense interior (i fastest) <-> colour split
 void launch_pack_fine(hipStream_t s, const FineDev *fd, int which, const double *src, int ni,

@@ -263,6 +263,96 @@ __device__ __forceinline__ void march_tile(const Tile &t, const LevelDev &L, int
   }
 }
 
+// Interior march tile of exactly KB planes with every load issued up front:
+// the own-column window (KB+2 planes), rows j-1 and j+1 and the rhs of all KB
+// planes, then the KB updates.  One memory round trip per tile instead of
+// one or more per plane; the arithmetic is march_tile<T, true>'s.
+template <class T, int KB>
+__device__ __forceinline__ void march_tile_inner(const Tile &t, const LevelDev &L, int colour,
+                                                 int src, int zero) {
+  typedef typename Vec2<T>::type v2;
+  const int lpr = L.lpr;
+  const int lane = (int)threadIdx.x;
+  const int rl = lane & (lpr - 1);
+  const int m0 = t.x0 + 2 * rl;
+  const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
+  const int ni = L.ni, nj = L.nj, H = L.H;
+  const bool col_ok = m0 < H && j < nj;
+  const unsigned row = (unsigned)H, plane = (unsigned)H * (unsigned)nj;
+  const gmem<T> *O = (const gmem<T> *)(colour == 0 ? L.phi[src][1] : L.phi[1 - src][0]);
+  const gmem<T> *R = (const gmem<T> *)L.rhs[colour];
+  gmem<T> *D = (gmem<T> *)L.phi[1 - src][colour];
+  const bool zero_int = zero && colour == 0;  // uniform
+  const unsigned cbase = (unsigned)min(j, nj - 1) * row + (unsigned)min(m0, H - 2);
+  const v2 vz = {T(0), T(0)};
+  const bool tile_left = t.x0 > 0, tile_right = t.x0 + 2 * lpr < H;
+  const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz;
+  const T ap_in = T(-1) * (cx + cx + cy + cy + cz + cz);
+  const T inv_in = T(1) / ap_in;
+  auto ld2 = [&](const gmem<T> *a, unsigned off) -> v2 {
+    return *(const gmem<v2> *)((const gmem<char> *)a + off * (unsigned)sizeof(T));
+  };
+  auto ld1 = [&](const gmem<T> *a, unsigned off) -> T {
+    return *(const gmem<T> *)((const gmem<char> *)a + off * (unsigned)sizeof(T));
+  };
+  const int k0 = t.z0;
+  const unsigned q0 = (unsigned)k0 * plane + cbase;
+  v2 own[KB + 2], S2[KB], N2[KB], R2[KB];
+  T lf_e[KB], rt_e[KB];
+  if (!zero_int) {
+#pragma unroll
+    for (int p = 0; p < KB + 2; ++p) own[p] = ld2(O, q0 + (unsigned)(p - 1) * plane);
+#pragma unroll
+    for (int p = 0; p < KB; ++p) {
+      S2[p] = ld2(O, q0 + (unsigned)p * plane - row);
+      N2[p] = ld2(O, q0 + (unsigned)p * plane + row);
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < KB + 2; ++p) own[p] = vz;
+#pragma unroll
+    for (int p = 0; p < KB; ++p) S2[p] = N2[p] = vz;
+  }
+#pragma unroll
+  for (int p = 0; p < KB; ++p) R2[p] = ld2(R, q0 + (unsigned)p * plane);
+  // entries m0-1 and m0+2 of tiles that do not start/end a row
+  const bool need_l = tile_left && !zero_int && rl == 0;
+  const bool need_r = tile_right && !zero_int && rl == lpr - 1;
+#pragma unroll
+  for (int p = 0; p < KB; ++p) {
+    lf_e[p] = need_l ? ld1(O, q0 + (unsigned)p * plane - 1) : T(0);
+    rt_e[p] = need_r ? ld1(O, q0 + (unsigned)p * plane + 2) : T(0);
+  }
+#pragma unroll
+  for (int p = 0; p < KB; ++p) {
+    const int k = k0 + p;
+    const v2 o_m = own[p], o_c = own[p + 1], o_p = own[p + 2];
+    T lf = wave_shift(o_c.y, 1);
+    T rt = wave_shift(o_c.x, 0);
+    if (need_l) lf = lf_e[p];
+    if (need_r) rt = rt_e[p];
+    const int s = (colour + j + k) & 1;
+    const T W0 = s ? o_c.x : lf;
+    const T E0 = s ? o_c.y : o_c.x;
+    const T W1 = s ? o_c.y : o_c.x;
+    const T E1 = s ? rt : o_c.y;
+    const int i0 = 2 * m0 + s, i1 = i0 + 2;
+    const T sum0 = cx * W0 + cx * E0 + cy * S2[p].x + cy * N2[p].x + cz * o_m.x + cz * o_p.x;
+    const T sum1 = cx * W1 + cx * E1 + cy * S2[p].y + cy * N2[p].y + cz * o_m.y + cz * o_p.y;
+    const T out0 = div_ap(R2[p].x - sum0, ap_in, inv_in);
+    const T out1 = div_ap(R2[p].y - sum1, ap_in, inv_in);
+    const bool w0 = col_ok && i0 > 0 && i0 < ni - 1, w1 = col_ok && i1 < ni - 1;
+    gmem<char> *dq = (gmem<char> *)D + (q0 + (unsigned)p * plane) * (unsigned)sizeof(T);
+    if (w0 && w1) {
+      const v2 o = {out0, out1};
+      *(gmem<v2> *)dq = o;
+    } else {
+      if (w0) *(gmem<T> *)dq = out0;
+      if (w1) *(gmem<T> *)(dq + sizeof(T)) = out1;
+    }
+  }
+}
+
 // The cells of colour `colour` on the two x faces (i = 0 and i = ni-1), with
 // the full ghost treatment.  The tile covers 256 rows j = t.x0 + ... of plane
 // k = t.y0; one thread per row does both faces.
@@ -306,9 +396,15 @@ __global__ void __launch_bounds__(NTHREADS)
   const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
   const LevelDev &L = levs[t.lev];
   if (!active[L.blk]) return;
-  if (t.kind == 0)
-    march_tile<T, true>(t, L, colour, src, zero);
-  else if (t.kind == 1)
+  if (t.kind == 0) {
+    switch (t.z1 - t.z0) {
+      case 1: march_tile_inner<T, 1>(t, L, colour, src, zero); break;
+      case 2: march_tile_inner<T, 2>(t, L, colour, src, zero); break;
+      case 4: march_tile_inner<T, 4>(t, L, colour, src, zero); break;
+      case 8: march_tile_inner<T, 8>(t, L, colour, src, zero); break;
+      default: march_tile<T, true>(t, L, colour, src, zero);
+    }
+  } else if (t.kind == 1)
     march_tile<T, false>(t, L, colour, src, zero);
   else
     xface_tile<T>(t, L, colour, src, zero);
@@ -810,30 +906,23 @@ __global__ void __launch_bounds__(256)
 // periodic second difference; with Q_a the orthonormal eigenvectors of D_a
 // (columns) and l_a its eigenvalues, u = Q (Q^T f ./ (lx+ly+lz)) with the
 // zero (constant) mode dropped -- the null-space projection.  One workgroup
-// per block; scratch holds two n^3 arrays.
-__global__ void __launch_bounds__(256)
-    k_coarse_exact(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
-                   const int *__restrict__ active, const double *const *__restrict__ eig_q,
-                   const double *const *__restrict__ eig_l, double *const *__restrict__ scratch,
-                   int buf) {
-  const Tile t = tiles[blockIdx.x];  // lev = coarsest descriptor, x0 = block
-  const int b = t.x0;
-  if (!active[b]) return;
-  const LevelDev &L = levs[t.lev];
+// per block; scratch holds two n^3 arrays.  The body is shared by k_coarse_exact (256 threads) and
+// the V-cycle tail kernel (TAIL_THREADS): every thread of the workgroup calls it.
+__device__ void coarse_exact_body(const LevelDev &L, const double *Q0, const double *Q1,
+                                  const double *Q2, const double *l0, const double *l1,
+                                  const double *l2, double *A, double *Bs, int buf) {
   const int n0 = L.ni, n1 = L.nj, n2 = L.nk;
   const size_t n = (size_t)n0 * n1 * n2;
-  const double *Q0 = eig_q[3 * b + 0], *Q1 = eig_q[3 * b + 1], *Q2 = eig_q[3 * b + 2];
-  const double *l0 = eig_l[3 * b + 0], *l1 = eig_l[3 * b + 1], *l2 = eig_l[3 * b + 2];
-  double *A = scratch[2 * b], *Bs = scratch[2 * b + 1];
-  const int tid = threadIdx.x;
+  const int tid = (int)threadIdx.x;
+  const int nthr = (int)blockDim.x;
   // gather rhs (colour split) into natural layout A[i + n0*(j + n1*k)]
-  for (size_t q = tid; q < n; q += 256) {
+  for (size_t q = tid; q < n; q += nthr) {
     const int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / ((size_t)n0 * n1));
     A[q] = ((const double *)L.rhs[(i + j + k) & 1])[cidx(L.H, L.nj, i, j, k)];
   }
   __syncthreads();
   // forward transforms: B = Q0^T along i, A = Q1^T along j, B = Q2^T along k
-  for (size_t q = tid; q < n; q += 256) {
+  for (size_t q = tid; q < n; q += nthr) {
     const int i = (int)(q % n0);
     const size_t base = q - i;
     double s = 0.0;
@@ -841,16 +930,15 @@ __global__ void __launch_bounds__(256)
     Bs[q] = s;
   }
   __syncthreads();
-  for (size_t q = tid; q < n; q += 256) {
-    const int i = (int)(q % n0), j = (int)((q / n0) % n1);
+  for (size_t q = tid; q < n; q += nthr) {
+    const int j = (int)((q / n0) % n1);
     const size_t base = q - (size_t)j * n0;
     double s = 0.0;
     for (int y = 0; y < n1; ++y) s += Q1[(size_t)y * n1 + j] * Bs[base + (size_t)y * n0];
     A[q] = s;
-    (void)i;
   }
   __syncthreads();
-  for (size_t q = tid; q < n; q += 256) {
+  for (size_t q = tid; q < n; q += nthr) {
     const size_t plane = (size_t)n0 * n1;
     const int k = (int)(q / plane);
     const size_t base = q - (size_t)k * plane;
@@ -863,7 +951,7 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   // inverse transforms: A = Q2 along k, B = Q1 along j, phi = Q0 along i
-  for (size_t q = tid; q < n; q += 256) {
+  for (size_t q = tid; q < n; q += nthr) {
     const size_t plane = (size_t)n0 * n1;
     const int k = (int)(q / plane);
     const size_t base = q - (size_t)k * plane;
@@ -872,7 +960,7 @@ __global__ void __launch_bounds__(256)
     A[q] = s;
   }
   __syncthreads();
-  for (size_t q = tid; q < n; q += 256) {
+  for (size_t q = tid; q < n; q += nthr) {
     const int j = (int)((q / n0) % n1);
     const size_t base = q - (size_t)j * n0;
     double s = 0.0;
@@ -880,12 +968,326 @@ __global__ void __launch_bounds__(256)
     Bs[q] = s;
   }
   __syncthreads();
-  for (size_t q = tid; q < n; q += 256) {
+  for (size_t q = tid; q < n; q += nthr) {
     const int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / ((size_t)n0 * n1));
     const size_t base = q - i;
     double s = 0.0;
     for (int x = 0; x < n0; ++x) s += Q0[(size_t)i * n0 + x] * Bs[base + x];
     ((double *)L.phi[buf][(i + j + k) & 1])[cidx(L.H, L.nj, i, j, k)] = s;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256)
+    k_coarse_exact(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+                   const int *__restrict__ active, const double *const *__restrict__ eig_q,
+                   const double *const *__restrict__ eig_l, double *const *__restrict__ scratch,
+                   int buf) {
+  const Tile t = tiles[blockIdx.x];  // lev = coarsest descriptor, x0 = block
+  if (!active[t.x0]) return;
+  const int b = t.x0;
+  coarse_exact_body(levs[t.lev], eig_q[3 * b], eig_q[3 * b + 1], eig_q[3 * b + 2], eig_l[3 * b],
+                    eig_l[3 * b + 1], eig_l[3 * b + 2], scratch[2 * b], scratch[2 * b + 1], buf);
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle tail: one workgroup per block runs every level from the tail level
+// Lt down to the block's coarsest and back up (mgkcyc, mgsolver.for:272-352,
+// restricted to levels >= Lt) entirely in LDS: pre-smoothing, restriction,
+// the coarsest treatment, prolongation and post-smoothing, separated by
+// workgroup barriers.  Small levels are latency-bound: one launch, LDS
+// latency and on-chip barriers replace ~3 full-chip launches per level.
+//
+// The tail levels are copied into LDS with their global layout (colour
+// split, pitch H): descriptors whose pointers point into LDS feed the same
+// per-cell functions, so every cell is computed with the same expression,
+// in the same order, as by the per-level kernels (bitwise-identical
+// results).  In: the top level's rhs (global).  Out: the top level's phi in
+// buffer (nu1+nu2)&1 (global), the buffer every level ends in on the host
+// side.  The tile's `lev` is the block's level-Lt descriptor.  Levels are
+// float in a mixed-precision handle except the coarsest (double).
+
+// neighbour (x,y,z) of a cell of colour `colour` in a half-sweep reading
+// buffer `src`: interior neighbours from the other colour's array, ghosts
+// from their image in buffer src.  Branch-free: one load from a selected
+// array, then the zero-state select.
+template <class T>
+__device__ __forceinline__ T sweep_nb(const LevelDev &L, const T *nb_int, int src, bool zero_int,
+                                      bool zero_ghost, int x, int y, int z) {
+  const bool ghost =
+      (unsigned)x >= (unsigned)L.ni || (unsigned)y >= (unsigned)L.nj || (unsigned)z >= (unsigned)L.nk;
+  const int xi = image(x, L.ni, L.per & 1), yi = image(y, L.nj, L.per & 2),
+            zi = image(z, L.nk, L.per & 4);
+  const T *a = ghost ? (const T *)L.phi[src][(xi + yi + zi) & 1] : nb_int;
+  const T v = a[cidx(L.H, L.nj, xi, yi, zi)];
+  return (ghost ? zero_ghost : zero_int) ? T(0) : v;
+}
+
+// One colour half-sweep over a whole level, cell-parallel; the loads of
+// TAIL_U cells per thread are issued before any of their stores.
+template <class T>
+__device__ __forceinline__ void tail_half_sweep(const LevelDev &L, int colour, int src, int zero) {
+  const int H = L.H, nj = L.nj, nk = L.nk, ni = L.ni;
+  const int n = H * nj * nk;
+  const T *nb_int = colour == 0 ? (const T *)L.phi[src][1] : (const T *)L.phi[1 - src][0];
+  const T *R = (const T *)L.rhs[colour];
+  T *D = (T *)L.phi[1 - src][colour];
+  const bool zero_int = zero && colour == 0, zero_ghost = zero != 0;
+  for (int base = (int)threadIdx.x; base < n; base += TAIL_THREADS * TAIL_U) {
+    T v[TAIL_U][7];
+    int ci[TAIL_U], cj[TAIL_U], ck[TAIL_U];
+    bool ok[TAIL_U];
+#pragma unroll
+    for (int u = 0; u < TAIL_U; ++u) {
+      const int e0 = base + u * TAIL_THREADS;
+      const int e = min(e0, n - 1);  // lanes past the end read a valid entry
+      const int m = e % H, r = e / H, j = r % nj, k = r / nj;
+      const int i = min(2 * m + ((colour + j + k) & 1), ni - 1);
+      ok[u] = e0 < n && 2 * m + ((colour + j + k) & 1) < ni;
+      ci[u] = i;
+      cj[u] = j;
+      ck[u] = k;
+      v[u][0] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i - 1, j, k);
+      v[u][1] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i + 1, j, k);
+      v[u][2] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i, j - 1, k);
+      v[u][3] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i, j + 1, k);
+      v[u][4] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i, j, k - 1);
+      v[u][5] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i, j, k + 1);
+      v[u][6] = R[e];
+    }
+#pragma unroll
+    for (int u = 0; u < TAIL_U; ++u) {
+      if (!ok[u]) continue;
+      const Coef<T> c = cell_coef<T>(L, ci[u], cj[u], ck[u]);
+      const T sum = c.aw * v[u][0] + c.ae * v[u][1] + c.as * v[u][2] + c.an * v[u][3] +
+                    c.ab * v[u][4] + c.at * v[u][5];
+      D[base + u * TAIL_THREADS] = div_ap(v[u][6] - sum, c.ap, T(1) / c.ap);
+    }
+  }
+  __syncthreads();
+}
+
+template <class T>
+__device__ __forceinline__ void tail_fill_zero(const LevelDev &L, int buf) {
+  const int n = L.H * L.nj * L.nk;
+  T *a = (T *)L.phi[buf][0], *b = (T *)L.phi[buf][1];
+  for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
+    a[e] = T(0);
+    b[e] = T(0);
+  }
+  __syncthreads();
+}
+
+// nsweep sweeps from buffer `buf` (flag `zero`: buffer holds zeros)
+template <class T>
+__device__ __forceinline__ void tail_relax(const LevelDev &L, int nsweep, int &buf, bool &zero) {
+  for (int sw = 0; sw < nsweep; ++sw) {
+    tail_half_sweep<T>(L, 0, buf, zero ? 1 : 0);
+    tail_half_sweep<T>(L, 1, buf, zero ? 1 : 0);
+    buf ^= 1;
+    zero = false;
+  }
+}
+
+// mgrestr for one coarse cell per thread (the arithmetic of restrict_tile)
+template <class Tf, class Tc>
+__device__ __forceinline__ void tail_restrict(const LevelDev &F, const LevelDev &C, int src_f) {
+  const int n = C.ni * C.nj * C.nk;
+  for (int q0 = (int)threadIdx.x; q0 < n; q0 += TAIL_THREADS) {
+    const int ic = q0 % C.ni, jc = (q0 / C.ni) % C.nj, kc = q0 / (C.ni * C.nj);
+    Tc acc = Tc(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int i = 2 * ic + (q & 1), j = 2 * jc + ((q >> 1) & 1), k = 2 * kc + (q >> 2);
+      const Coef<Tf> c = cell_coef<Tf>(F, i, j, k);
+      const Tf W = level_value<Tf>(F, src_f, i - 1, j, k);
+      const Tf E = level_value<Tf>(F, src_f, i + 1, j, k);
+      const Tf S = level_value<Tf>(F, src_f, i, j - 1, k);
+      const Tf N = level_value<Tf>(F, src_f, i, j + 1, k);
+      const Tf B = level_value<Tf>(F, src_f, i, j, k - 1);
+      const Tf Tt = level_value<Tf>(F, src_f, i, j, k + 1);
+      const size_t fq = cidx(F.H, F.nj, i, j, k);
+      const int fc = (i + j + k) & 1;
+      const Tf P = ((const Tf *)F.phi[src_f][fc])[fq];
+      const Tf r = ((const Tf *)F.rhs[fc])[fq];
+      const Tf res =
+          r - (c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * Tt + c.ap * P);
+      acc = q == 0 ? (Tc)res : acc + (Tc)res;
+    }
+    ((Tc *)C.rhs[(ic + jc + kc) & 1])[cidx(C.H, C.nj, ic, jc, kc)] = Tc(0.125) * acc;
+  }
+  __syncthreads();
+}
+
+// mgcorr for one fine cell per thread (the arithmetic of k_prolong): fine
+// cell (i,j,k) is cell e = i&1 of entry l = i>>1, coarse row/plane pair
+// m = (j+1)>>1, n = (k+1)>>1 with offsets dj = (j+1)&1, dk = (k+1)&1
+template <class Tf, class Tc>
+__device__ __forceinline__ void tail_prolong(const LevelDev &F, const LevelDev &C, int src_f,
+                                             int src_c) {
+  const int n = F.ni * F.nj * F.nk;
+  const bool px = C.per & 1, py = C.per & 2, pz = C.per & 4;
+  for (int q0 = (int)threadIdx.x; q0 < n; q0 += TAIL_THREADS) {
+    const int i = q0 % F.ni, j = (q0 / F.ni) % F.nj, k = q0 / (F.ni * F.nj);
+    const int l = i >> 1, e = i & 1, di = 1 - e;
+    const int m = (j + 1) >> 1, dj = (j + 1) & 1;
+    const int nn = (k + 1) >> 1, dk = (k + 1) & 1;
+    Tc s = Tc(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int oi = q >> 2, oj = q & 1, ok = (q >> 1) & 1;
+      const int x = image(l - 1 + e + oi, C.ni, px);
+      const int y = image(m - 1 + oj, C.nj, py);
+      const int z = image(nn - 1 + ok, C.nk, pz);
+      const Tc w = (oi == di ? Tc(3) : Tc(1)) * (oj == dj ? Tc(3) : Tc(1)) *
+                   (ok == dk ? Tc(3) : Tc(1));
+      const Tc v = ((const Tc *)C.phi[src_c][(x + y + z) & 1])[cidx(C.H, C.nj, x, y, z)];
+      s = (q == 0) ? w * v : s + w * v;
+    }
+    Tf *fa = (Tf *)F.phi[src_f][(i + j + k) & 1];
+    const size_t fq = cidx(F.H, F.nj, i, j, k);
+    fa[fq] = fa[fq] + (Tf)(s / Tc(64));
+  }
+  __syncthreads();
+}
+
+template <class T>
+__device__ __forceinline__ void tail_copy(T *dst, const T *src, int n) {
+  for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) dst[e] = src[e];
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(TAIL_THREADS)
+    k_vtail(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+            const int *__restrict__ active, const double *const *__restrict__ eig_q,
+            const double *const *__restrict__ eig_l, int nu1, int nu2, int exact) {
+  extern __shared__ __align__(16) char smem[];
+  // descriptors of the tail's levels with their arrays in LDS: level q of the
+  // tail is lv[q] (written by thread 0, read by all)
+  __shared__ LevelDev lv[TAIL_MAX_LEVELS];
+  __shared__ int s_nl;
+  __shared__ size_t s_off;
+  const Tile t = tiles[blockIdx.x];
+  const int top = t.lev;
+  const int blk = levs[top].blk;
+  if (!active[blk]) return;
+  const int fin = (nu1 + nu2) & 1;
+  if (threadIdx.x == 0) {
+    int nl = 0;
+    size_t off = 0;
+    for (int d = top; d >= 0 && nl < TAIL_MAX_LEVELS; d = levs[d].coarser) {
+      LevelDev L = levs[d];
+      const bool w8 = !MIXED || L.coarser < 0;
+      const size_t half = (size_t)L.H * L.nj * L.nk * (w8 ? 8 : 4);
+      for (int bb = 0; bb < 2; ++bb)
+        for (int c = 0; c < 2; ++c) {
+          L.phi[bb][c] = smem + off;
+          off += (half + 15) & ~(size_t)15;
+        }
+      for (int c = 0; c < 2; ++c) {
+        L.rhs[c] = smem + off;
+        off += (half + 15) & ~(size_t)15;
+      }
+      lv[nl++] = L;
+    }
+    s_nl = nl;
+    s_off = off;
+  }
+  __syncthreads();
+  const int last = s_nl - 1;
+  size_t off = s_off;
+  auto is64 = [&](int q) { return !MIXED || lv[q].coarser < 0; };
+  const bool do_exact = exact && lv[last].finer >= 0;  // as the host: not for a 1-level block
+  // the exact solve's tables and scratch follow the levels
+  const double *Q[3], *lam[3];
+  double *A = nullptr, *Bs = nullptr;
+  if (do_exact) {
+    const LevelDev &L = lv[last];
+    const int na[3] = {L.ni, L.nj, L.nk};
+    for (int a = 0; a < 3; ++a) {
+      double *q = (double *)(smem + off);
+      off += (size_t)na[a] * na[a] * 8;
+      double *l = (double *)(smem + off);
+      off += ((size_t)na[a] * 8 + 15) & ~(size_t)15;
+      tail_copy<double>(q, eig_q[3 * blk + a], na[a] * na[a]);
+      tail_copy<double>(l, eig_l[3 * blk + a], na[a]);
+      Q[a] = q;
+      lam[a] = l;
+    }
+    const size_t n3 = (size_t)na[0] * na[1] * na[2];
+    A = (double *)(smem + off);
+    off += n3 * 8;
+    Bs = (double *)(smem + off);
+  }
+  // top level's rhs into LDS
+  {
+    const LevelDev &G = levs[top];
+    const int n = G.H * G.nj * G.nk;
+    for (int c = 0; c < 2; ++c) {
+      if (is64(0))
+        tail_copy<double>((double *)lv[0].rhs[c], (const double *)G.rhs[c], n);
+      else
+        tail_copy<float>((float *)lv[0].rhs[c], (const float *)G.rhs[c], n);
+    }
+    __syncthreads();
+  }
+  // down: pre-smoothing and restriction
+  for (int q = 0; q < last; ++q) {
+    const LevelDev &L = lv[q];
+    const LevelDev &C = lv[q + 1];
+    int buf = 0;
+    bool zero = true;
+    if (is64(q)) {
+      tail_relax<double>(L, nu1, buf, zero);
+      if (zero) tail_fill_zero<double>(L, buf);
+      tail_restrict<double, double>(L, C, buf);
+    } else {
+      tail_relax<float>(L, nu1, buf, zero);
+      if (zero) tail_fill_zero<float>(L, buf);
+      if (is64(q + 1))
+        tail_restrict<float, double>(L, C, buf);
+      else
+        tail_restrict<float, float>(L, C, buf);
+    }
+  }
+  // coarsest level of the block (always FP64)
+  if (do_exact) {
+    coarse_exact_body(lv[last], Q[0], Q[1], Q[2], lam[0], lam[1], lam[2], A, Bs, fin);
+  } else {
+    int buf = 0;
+    bool zero = true;
+    tail_relax<double>(lv[last], nu1, buf, zero);
+    tail_relax<double>(lv[last], nu2, buf, zero);
+    if (zero) tail_fill_zero<double>(lv[last], buf);
+  }
+  // up: prolongation and post-smoothing
+  for (int q = last - 1; q >= 0; --q) {
+    const LevelDev &F = lv[q];
+    const LevelDev &C = lv[q + 1];
+    const int fb = nu1 & 1;
+    int buf = fb;
+    bool zero = false;
+    if (is64(q)) {
+      tail_prolong<double, double>(F, C, fb, fin);
+      tail_relax<double>(F, nu2, buf, zero);
+    } else {
+      if (is64(q + 1))
+        tail_prolong<float, double>(F, C, fb, fin);
+      else
+        tail_prolong<float, float>(F, C, fb, fin);
+      tail_relax<float>(F, nu2, buf, zero);
+    }
+  }
+  // top level's phi (buffer fin) back to global
+  {
+    const LevelDev &G = levs[top];
+    const int n = G.H * G.nj * G.nk;
+    for (int c = 0; c < 2; ++c) {
+      if (is64(0))
+        tail_copy<double>((double *)G.phi[fin][c], (const double *)lv[0].phi[fin][c], n);
+      else
+        tail_copy<float>((float *)G.phi[fin][c], (const float *)lv[0].phi[fin][c], n);
+    }
   }
 }
 
@@ -1038,6 +1440,45 @@ void launch_coarse_exact(hipStream_t s, const Tile *tiles, int ntiles, const Lev
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_coarse_exact, dim3(ntiles), dim3(256), 0, s, tiles, levs, active, eig_q,
                      eig_l, scratch, buf);
+}
+
+void launch_vtail(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                  const int *active, const double *const *eig_q, const double *const *eig_l,
+                  int nu1, int nu2, int exact, int mixed, size_t lds_bytes) {
+  if (ntiles <= 0) return;
+  if (mixed)
+    hipLaunchKernelGGL(k_vtail<true>, dim3(ntiles), dim3(TAIL_THREADS), lds_bytes, s, tiles, levs,
+                       active, eig_q, eig_l, nu1, nu2, exact);
+  else
+    hipLaunchKernelGGL(k_vtail<false>, dim3(ntiles), dim3(TAIL_THREADS), lds_bytes, s, tiles, levs,
+                       active, eig_q, eig_l, nu1, nu2, exact);
+}
+
+hipError_t vtail_allow_lds(size_t lds_bytes) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_vtail<true>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *)k_vtail<false>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+
+size_t vtail_lds_bytes(const LevelDev *levs_host, int top, bool mixed, bool exact) {
+  size_t off = 0;
+  int last = top, nl = 0;
+  for (int d = top; d >= 0 && nl < TAIL_MAX_LEVELS; d = levs_host[d].coarser, ++nl) {
+    const LevelDev &L = levs_host[d];
+    const bool is64 = !mixed || L.coarser < 0;
+    const size_t half = (size_t)L.H * L.nj * L.nk * (is64 ? 8 : 4);
+    off += 6 * ((half + 15) & ~(size_t)15);
+    last = d;
+  }
+  const LevelDev &C = levs_host[last];
+  if (exact && C.finer >= 0) {
+    const int na[3] = {C.ni, C.nj, C.nk};
+    for (int a = 0; a < 3; ++a) off += (size_t)na[a] * na[a] * 8 + (((size_t)na[a] * 8 + 15) & ~(size_t)15);
+    off += 2 * (size_t)na[0] * na[1] * na[2] * 8;
+  }
+  return off;
 }
 
 void launch_pack_fine(hipStream_t s, const FineDev *fd, int which, const double *src, int ni,

@@ -135,6 +135,8 @@ struct BlockInfo {
   int cycles = 0;
   double rel = 0.0;
   double rmax = 0.0;
+  double tol = 0.0;      // REL_L2 tolerance of this block (0: the handle's)
+  bool enabled = true;   // disabled blocks keep their state and are skipped
 };
 
 }  // namespace
@@ -180,6 +182,11 @@ struct h3dmg_handle {
   size_t stage_elems = 0;
 
   std::vector<int> march_kb;  // planes per smoother workgroup per level (0 = auto)
+  int tail_req = 0;           // requested tail level (0 = auto)
+  int tail_lt = 1 << 30;      // levels >= tail_lt run in the single-workgroup tail kernel
+  TileSet tail_tiles;         // one tile per block with L_b >= tail_lt
+  size_t tail_lds = 0;        // dynamic LDS of the tail launch
+  size_t lds_budget = 0;      // dynamic LDS a workgroup may use on this device
 
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -205,6 +212,8 @@ void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool 
 
 // (Re)build and upload the smoother tiles of level g for both block groups.
 h3dmg_status upload_relax(h3dmg_handle *h, int g);
+// (Re)choose the tail level and upload its tile list.
+h3dmg_status upload_tail(h3dmg_handle *h);
 
 void relax_any(h3dmg_handle *h, bool is64, const TileSet &ts, int nsweep, int &buf, bool &zero) {
   if (ts.n() == 0) {
@@ -242,7 +251,10 @@ void enqueue_vcycle(h3dmg_handle *h) {
   std::vector<char> z_nc(Lm + 2, 1), z_c(Lm + 2, 1);
   auto nc_is64 = [&](int) { return !mixed; };
 
-  for (int g = 1; g <= Lm; ++g) {
+  // levels >= Lt of the blocks that reach them run in one tail launch
+  const int Lt = h->tail_tiles.n() ? h->tail_lt : Lm + 1;
+  const int gdown = std::min(Lm, Lt);
+  for (int g = 1; g <= gdown; ++g) {
     if (g >= 2) {
       // mgrestr(g): residual of level g-1 (non-coarsest there) into rhs_g
       const bool f64 = nc_is64(g - 1);
@@ -266,6 +278,14 @@ void enqueue_vcycle(h3dmg_handle *h) {
           launch_restrict<float, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(),
                                          h->d_levs, h->d_active, buf_nc[g - 1]);
       }
+    }
+    if (g == Lt) {
+      launch_vtail(h->stream, h->tail_tiles.d, h->tail_tiles.n(), h->d_levs, h->d_active,
+                   (const double *const *)h->d_eig_q, (const double *const *)h->d_eig_l, nu1,
+                   nu2, exact ? 1 : 0, mixed ? 1 : 0, h->tail_lds);
+      buf_nc[g] = buf_c[g] = (nu1 + nu2) & 1;
+      z_nc[g] = z_c[g] = 0;
+      break;
     }
     // blocks whose coarsest level is g
     if (h->relax_c[g].n()) {
@@ -296,7 +316,7 @@ void enqueue_vcycle(h3dmg_handle *h) {
       z_nc[g] = z;
     }
   }
-  for (int g = Lm - 1; g >= 1; --g) {
+  for (int g = gdown - 1; g >= 1; --g) {
     // mgcorr(g) for blocks with L_b > g
     if (h->prol_nc[g].n() || h->prol_c[g].n()) {
       const bool f64 = nc_is64(g);
@@ -356,6 +376,11 @@ h3dmg_status read_stats(h3dmg_handle *h) {
                          hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return H3DMG_OK;
+}
+
+// every enabled block active
+void reset_active(h3dmg_handle *h) {
+  for (int b = 0; b < h->nb; ++b) h->active[b] = h->blk[b].enabled ? 1 : 0;
 }
 
 h3dmg_status push_active(h3dmg_handle *h) {
@@ -717,6 +742,10 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     h3dmg_status st = upload_relax(h, g);
     if (st != H3DMG_OK) return st;
   }
+  {
+    h3dmg_status st = upload_tail(h);
+    if (st != H3DMG_OK) return st;
+  }
   HIP_TRY(hipDeviceSynchronize());
   return H3DMG_OK;
 }
@@ -740,6 +769,57 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
     ts.owned = d;
     ts.d = d;
   }
+  return H3DMG_OK;
+}
+
+// LDS the tail needs when it starts at level g (max over the blocks that
+// reach g; 0 when none does)
+size_t tail_lds_at(const h3dmg_handle *h, int g) {
+  size_t need = 0;
+  for (const BlockInfo &B : h->blk)
+    if (B.L >= g)
+      need = std::max(need, vtail_lds_bytes(h->levs.data(), B.desc[g], is_mixed(h),
+                                            h->prm.coarse == H3DMG_COARSE_EXACT));
+  return need;
+}
+
+// Automatic tail level: the first level from which every block's remaining
+// levels fit in one workgroup's LDS; none when no level does.
+int auto_tail_level(const h3dmg_handle *h) {
+  for (int g = 1; g <= h->Lmax; ++g) {
+    const size_t need = tail_lds_at(h, g);
+    if (need > 0 && need <= h->lds_budget) return g;
+  }
+  return h->Lmax + 1;
+}
+
+h3dmg_status upload_tail(h3dmg_handle *h) {
+  h->tail_lt = h->tail_req > 0 ? h->tail_req : auto_tail_level(h);
+  h->tail_lds = h->tail_lt <= h->Lmax ? tail_lds_at(h, h->tail_lt) : 0;
+  if (h->tail_lds > h->lds_budget) {
+    const int lt = h->tail_lt;
+    h->tail_req = 0;
+    h->tail_lt = h->Lmax + 1;
+    h->tail_lds = 0;
+    return fail(H3DMG_ERR_ARG, "tail level %d needs %zu bytes of LDS (at most %zu here)", lt,
+                tail_lds_at(h, lt), h->lds_budget);
+  }
+  // the permitted LDS is a process-wide attribute of the kernel: always the
+  // device's whole budget, so that handles of different tail sizes coexist
+  if (h->tail_lds > 65536) HIP_TRY(vtail_allow_lds(h->lds_budget));
+  TileSet &ts = h->tail_tiles;
+  ts.h.clear();
+  if (h->tail_lt <= h->Lmax)
+    for (int b = 0; b < h->nb; ++b)
+      if (h->blk[b].L >= h->tail_lt) ts.h.push_back(Tile{h->blk[b].desc[h->tail_lt], b, 0, 0, 0, 0});
+  Tile *d = nullptr;
+  if (!ts.h.empty()) {
+    HIP_TRY(hipMalloc((void **)&d, ts.h.size() * sizeof(Tile)));
+    HIP_TRY(hipMemcpy(d, ts.h.data(), ts.h.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  }
+  if (ts.owned) HIP_TRY(hipFree(ts.owned));
+  ts.owned = d;
+  ts.d = d;
   return H3DMG_OK;
 }
 
@@ -828,6 +908,11 @@ h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_pa
     delete h;
     return fail(H3DMG_ERR_DEVICE, "no HIP device");
   }
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    lds = 65536;
+  // the tail kernel's static LDS (its level descriptors) comes on top
+  h->lds_budget = lds > 8192 ? (size_t)lds - 8192 : 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return fail(H3DMG_ERR_DEVICE, "hipStreamCreate failed");
@@ -849,6 +934,7 @@ void h3dmg_destroy(h3dmg_handle *h) {
   for (auto *v : {&h->relax_nc, &h->relax_c})
     for (TileSet &ts : *v)
       if (ts.owned) (void)hipFree(ts.owned);
+  if (h->tail_tiles.owned) (void)hipFree(h->tail_tiles.owned);
   if (h->h_stats) (void)hipHostFree(h->h_stats);
   if (h->arena) (void)hipFree(h->arena);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
@@ -1010,7 +1096,7 @@ h3dmg_status h3dmg_vcycle(h3dmg_handle *h) {
   if (!h) return fail(H3DMG_ERR_ARG, "null handle");
   if (!h->rhs_ready) return fail(H3DMG_ERR_ARG, "rhs not set");
   // level-1 rhs <- f (su), pp = 0
-  std::fill(h->active.begin(), h->active.end(), 1);
+  reset_active(h);
   h3dmg_status st = push_active(h);
   if (st) return st;
   const TileSet &ft = h->fine_tiles;
@@ -1047,7 +1133,7 @@ h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycl
   const h3dmg_params &P = h->prm;
   const bool ref = P.criterion == H3DMG_CRIT_REF_RMAX;
   const double rfac_base = P.dt;
-  std::fill(h->active.begin(), h->active.end(), 1);
+  reset_active(h);
   h3dmg_status st = push_active(h);
   if (st) return st;
   for (auto &B : h->blk) {
@@ -1066,6 +1152,7 @@ h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycl
     double gmax = 0.0;
     for (int b = 0; b < h->nb; ++b) {
       BlockInfo &B = h->blk[b];
+      if (!B.enabled) continue;
       const double nrm = std::sqrt(h->h_stats[b].ss);
       const double vol = B.spec.dx * B.spec.dy * B.spec.dz;
       if (h->active[b]) {
@@ -1080,10 +1167,12 @@ h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycl
     for (int b = 0; b < h->nb; ++b) {
       BlockInfo &B = h->blk[b];
       bool done;
-      if (ref)
+      if (!B.enabled)
+        done = true;
+      else if (ref)
         done = it >= std::max(P.min_cycles, 2) && gmax < P.tol;  // newsolv_mg.for:57, global
       else
-        done = it >= P.min_cycles && B.rel <= P.tol;
+        done = it >= P.min_cycles && B.rel <= (B.tol > 0 ? B.tol : P.tol);
       if (done) h->active[b] = 0;
       if (h->active[b]) all_done = false;
     }
@@ -1095,8 +1184,8 @@ h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycl
   measure(0, worst, all_done, bad);
   if (hist && hist_cap > 0) hist[0] = worst;
   if (P.min_cycles > 0 || ref) {
-    std::fill(h->active.begin(), h->active.end(), 1);
-    all_done = false;
+    reset_active(h);
+    all_done = std::none_of(h->active.begin(), h->active.end(), [](int a) { return a != 0; });
   }
   int it = 0;
   while (!all_done && it < P.maxcy && !bad) {
@@ -1115,6 +1204,21 @@ h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycl
   if (bad) return fail(H3DMG_ERR_DIVERGED, "non-finite residual after %d cycles", it);
   if (ref && worst > 10.0) return fail(H3DMG_ERR_DIVERGED, "rmax %g > 10", worst);
   if (!all_done) return fail(H3DMG_ERR_NOT_CONVERGED, "not converged in %d cycles (%g)", it, worst);
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_set_block_tol(h3dmg_handle *h, int b, double tol) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!(tol >= 0.0)) return fail(H3DMG_ERR_ARG, "negative tolerance");
+  h->blk[b].tol = tol;
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_set_block_enabled(h3dmg_handle *h, int b, int enabled) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  h->blk[b].enabled = enabled != 0;
   return H3DMG_OK;
 }
 
@@ -1265,6 +1369,19 @@ h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes) {
   drop_graph(h);
   h->march_kb[level] = planes > 0 ? planes : 0;
   return upload_relax(h, level);
+}
+
+h3dmg_status h3dmg_set_tail(h3dmg_handle *h, int level) {
+  if (!h || level < 0) return fail(H3DMG_ERR_ARG, "bad tail level");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  drop_graph(h);
+  h->tail_req = level;
+  return upload_tail(h);
+}
+
+int h3dmg_get_tail(const h3dmg_handle *h) {
+  if (!h) return -1;
+  return h->tail_tiles.n() ? h->tail_lt : h->Lmax + 1;
 }
 
 int h3dmg_get_march(const h3dmg_handle *h, int level) {

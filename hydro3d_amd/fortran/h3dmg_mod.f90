@@ -39,7 +39,8 @@ module h3dmg_mod
   public :: h3dmg_default_params, h3dmg_create, h3dmg_destroy, h3dmg_device_bytes, &
        h3dmg_set_rhs, h3dmg_set_p, h3dmg_get_p, h3dmg_vcycle, h3dmg_get_pp, &
        h3dmg_solve, h3dmg_block_stats, h3dmg_remove_mean, h3dmg_set_stream, &
-       h3dmg_set_march, h3dmg_get_march, h3dmg_error_message
+       h3dmg_set_march, h3dmg_get_march, h3dmg_set_tail, h3dmg_get_tail, &
+       h3dmg_set_block_tol, h3dmg_set_block_enabled, h3dmg_error_message
 
   interface
      subroutine h3dmg_default_params(p) bind(C, name="h3dmg_default_params")
@@ -125,6 +126,32 @@ module h3dmg_mod
        integer(c_int), intent(out) :: cycles(*)
        real(c_double), intent(out) :: rel_res(*)
      end function h3dmg_block_stats
+
+     integer(c_int) function h3dmg_set_block_tol(handle, b, tol) &
+          bind(C, name="h3dmg_set_block_tol")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: handle
+       integer(c_int), value :: b
+       real(c_double), value :: tol
+     end function h3dmg_set_block_tol
+
+     integer(c_int) function h3dmg_set_block_enabled(handle, b, enabled) &
+          bind(C, name="h3dmg_set_block_enabled")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: handle
+       integer(c_int), value :: b, enabled
+     end function h3dmg_set_block_enabled
+
+     integer(c_int) function h3dmg_set_tail(handle, level) bind(C, name="h3dmg_set_tail")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: handle
+       integer(c_int), value :: level
+     end function h3dmg_set_tail
+
+     integer(c_int) function h3dmg_get_tail(handle) bind(C, name="h3dmg_get_tail")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: handle
+     end function h3dmg_get_tail
 
      integer(c_int) function h3dmg_remove_mean(handle) bind(C, name="h3dmg_remove_mean")
        import :: c_ptr, c_int

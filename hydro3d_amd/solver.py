@@ -124,6 +124,12 @@ class MultigridSolver:
         L.check(self.lib.h3dmg_get_p(self.h, b, _dptr(p), ni + 2, nj + 2, 1))
         return p
 
+    def result_p(self, b):
+        """The pressure of block b as a new device tensor (nk, nj, ni)."""
+        import torch
+        out = torch.empty(self._shape(b), dtype=torch.float64, device="cuda")
+        return self.get_p(b, out=out)
+
     def get_pp(self, b):
         ni, nj, _ = self.blocks[b]["n"]
         pp = np.zeros(self._shape(b, 1))
@@ -155,6 +161,14 @@ class MultigridSolver:
         L.check(self.lib.h3dmg_block_stats(self.h, cyc, rel))
         return list(cyc), list(rel)
 
+    def set_block_tol(self, b, tol):
+        """REL_L2 tolerance of block b (0: the solver's)."""
+        L.check(self.lib.h3dmg_set_block_tol(self.h, b, float(tol)))
+
+    def set_block_enabled(self, b, enabled):
+        """A disabled block is skipped by solve/vcycle; its state stays."""
+        L.check(self.lib.h3dmg_set_block_enabled(self.h, b, 1 if enabled else 0))
+
     def remove_mean(self):
         L.check(self.lib.h3dmg_remove_mean(self.h))
 
@@ -167,6 +181,14 @@ class MultigridSolver:
 
     def get_march(self, level):
         return int(self.lib.h3dmg_get_march(self.h, level))
+
+    def set_tail(self, level):
+        """Levels >= level run in the single-workgroup V-cycle tail (0 = auto,
+        above the coarsest = off); results are identical either way."""
+        L.check(self.lib.h3dmg_set_tail(self.h, level))
+
+    def get_tail(self):
+        return int(self.lib.h3dmg_get_tail(self.h))
 
     @property
     def nlevels_max(self):

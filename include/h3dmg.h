@@ -150,6 +150,15 @@ h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycl
  * residual (arrays of nblocks). */
 h3dmg_status h3dmg_block_stats(const h3dmg_handle *h, int *cycles, double *rel_res);
 
+/* Per-block controls for batches of unrelated solves (no reference
+ * counterpart; the serving layer's batches):
+ *   set_block_tol      REL_L2 tolerance of block b (0: the handle's tol);
+ *   set_block_enabled  0 removes block b from h3dmg_solve / h3dmg_vcycle
+ *                      (its hierarchy, rhs and p stay resident), 1 restores it.
+ * A solve over enabled blocks gives each of them the result it gets alone. */
+h3dmg_status h3dmg_set_block_tol(h3dmg_handle *h, int b, double tol);
+h3dmg_status h3dmg_set_block_enabled(h3dmg_handle *h, int b, int enabled);
+
 /* Remove the mean of p over the interior of every block (FP64). */
 h3dmg_status h3dmg_remove_mean(h3dmg_handle *h);
 
@@ -162,6 +171,15 @@ h3dmg_status h3dmg_set_stream(h3dmg_handle *h, void *stream);
  * depth in use (or -1 for a bad level). */
 h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes);
 int h3dmg_get_march(const h3dmg_handle *h, int level);
+/* Levels >= `level` of every block run as one workgroup per block in a
+ * single launch with every level in LDS (the V-cycle tail); 0 restores the
+ * automatic choice (the first level from which each block's remaining levels
+ * fit in one workgroup's LDS), a level above every block's coarsest turns the
+ * tail off; a level that does not fit returns H3DMG_ERR_ARG.  Results are
+ * identical either way.  Drops a captured graph.  h3dmg_get_tail returns the
+ * level in use (nlevels_max + 1 when off). */
+h3dmg_status h3dmg_set_tail(h3dmg_handle *h, int level);
+int h3dmg_get_tail(const h3dmg_handle *h);
 
 /* ---- test seams: one operator at a time on level arrays ------------------
  * Level arrays cross this seam as dense device arrays in the layout of the

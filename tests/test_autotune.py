@@ -1,0 +1,61 @@
+"""Autotuner: cache keys and files on the CPU; on the GPU the tuned launch
+shapes leave every result bitwise unchanged and the cache round-trips."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from hydro3d_amd import autotune as AT
+
+
+class _FakeParams:
+    precision, coarse, nu1, nu2 = 1, 1, 2, 2
+
+
+class _FakeSolver:
+    params = _FakeParams()
+
+    def __init__(self, blocks):
+        self.blocks = blocks
+
+
+def test_config_key_ignores_block_order_and_spacing():
+    a = dict(n=(32, 16, 24), h=(0.1,) * 3, bc=(0,) * 6, nlev=4)
+    b = dict(n=(16, 16, 16), h=(0.2,) * 3, bc=(1, 1, 1, 1, 0, 0), nlev=3)
+    k1 = AT.config_key(_FakeSolver([a, b]), device="gfx950")
+    k2 = AT.config_key(_FakeSolver([b, dict(a, h=(0.3,) * 3)]), device="gfx950")
+    assert k1 == k2
+    assert k1 != AT.config_key(_FakeSolver([a]), device="gfx950")
+    assert json.loads(k1)["blocks"][0][0] == [16, 16, 16]
+
+
+def test_cache_round_trip(tmp_path):
+    path = str(tmp_path / "sub" / "tune.json")
+    assert AT.load_cache(path) == {}
+    AT.save_cache({"k": {"march": {"1": 4}, "tail": 3}}, path)
+    assert AT.load_cache(path) == {"k": {"march": {"1": 4}, "tail": 3}}
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_results_and_caches(tmp_path):
+    import torch
+    from hydro3d_amd import MultigridSolver, configs
+    print("case: autotune 64^3 mixed", flush=True)
+    c = configs.single_64()
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    f = torch.from_numpy(c["f"]).cuda()
+    with MultigridSolver([spec], precision="mixed", coarse="exact") as S:
+        S.set_rhs(0, f)
+        S.solve(raise_on_fail=True)
+        p_ref = S.get_p(0)
+    path = str(tmp_path / "tune.json")
+    with MultigridSolver([spec], precision="mixed", coarse="exact") as S:
+        S.set_rhs(0, f)
+        choice = AT.autotune(S, path, reps=2)
+        S.set_p(0, torch.zeros_like(f))
+        S.solve(raise_on_fail=True)
+        p_tuned = S.get_p(0)
+        assert AT.apply_cached(S, path) == choice
+    assert np.array_equal(p_ref, p_tuned)
+    assert os.path.exists(path) and len(AT.load_cache(path)) == 1

@@ -147,6 +147,45 @@ def test_gpu_uneven_blocks_each_equal_alone(gpu_lib):
         assert np.array_equal(ps[b][1:-1, 1:-1, 1:-1], po[1:-1, 1:-1, 1:-1]), b
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_gpu_tail_level_does_not_change_results(gpu_lib, prec):
+    """The single-workgroup V-cycle tail computes every cell with the same
+    arithmetic as the per-level kernels: the tail level (auto, every level,
+    off) changes neither the pressure (bitwise) nor the cycle counts."""
+    rng = np.random.default_rng(5)
+    W, P = 0, 1
+    specs = [dict(n=(32, 16, 24), h=(1 / 32,) * 3, bc=(W,) * 6, nlev=4),
+             dict(n=(16, 16, 16), h=(1 / 16,) * 3, bc=(P, P, P, P, W, W), nlev=3),
+             dict(n=(48, 8, 16), h=(0.02, 0.03, 0.04), bc=(P, P, W, W, P, P), nlev=3)]
+    fs = [mean_free(rng.standard_normal((s["n"][2], s["n"][1], s["n"][0]))) for s in specs]
+    kw = dict(tol=1e-9) if prec == "fp64" else dict(tol=1e-9, precision="mixed", coarse="exact")
+    from hydro3d_amd import _lib as L
+    results = {}
+    for tail in (0, 1, 2, 3, 4, 5):
+        print(f"case: tail level {tail} ({prec})", flush=True)
+        with _solver(specs, **kw) as S:
+            try:
+                S.set_tail(tail)
+            except L.H3dError as e:  # the levels from `tail` down do not fit in LDS
+                assert e.status == L.ERR_ARG and tail > 0
+                continue
+            used = S.get_tail()
+            for b, f in enumerate(fs):
+                S.set_rhs(b, f)
+            S.solve(raise_on_fail=True)
+            cyc, _ = S.block_stats()
+            results[tail] = (used, list(cyc), [S.get_p(b) for b in range(len(specs))])
+    # auto picks the first level whose remaining levels fit in LDS (2 here);
+    # 5 is above every coarsest level: off
+    assert results[0][0] == 2 and results[5][0] == 5
+    assert {2, 3, 4} <= set(results), sorted(results)
+    ref = results[5]
+    for tail, (used, cyc, ps) in results.items():
+        assert cyc == ref[1], (tail, cyc, ref[1])
+        for b in range(len(specs)):
+            assert np.array_equal(ps[b], ref[2][b]), (tail, b)
+
+
 def test_gpu_graph_and_eager_identical(gpu_lib):
     from hydro3d_amd import configs as C
     c = C.cube(32, C.BC_PPN, 3, seed=9)

@@ -1,0 +1,167 @@
+"""Serving layer: scheduling on the CPU with a host model of the solver
+(admission by footprint, batching, residency and warm reuse, LRU eviction);
+on the GPU, every served result equals the same request solved alone."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from hydro3d_amd import serving as SV
+
+
+class FakeSolver:
+    """Host stand-in with the solver's control surface; 'solves' take a
+    little time and return f/2 as the pressure."""
+    created = []
+    lock = threading.Lock()
+
+    def __init__(self, specs, tol=1e-8, **kw):
+        self.blocks = [dict(s) for s in specs]
+        self.enabled = [True] * len(specs)
+        self.f = [None] * len(specs)
+        self.tol = [tol] * len(specs)
+        self.p = [None] * len(specs)
+        self.closed = False
+        with FakeSolver.lock:
+            FakeSolver.created.append(self)
+
+    def set_block_enabled(self, b, on):
+        self.enabled[b] = bool(on)
+
+    def set_rhs(self, b, f):
+        self.f[b] = np.asarray(f)
+
+    def set_block_tol(self, b, tol):
+        self.tol[b] = tol
+
+    def solve(self):
+        time.sleep(0.01)
+        for b, on in enumerate(self.enabled):
+            if on:
+                warm = self.p[b] is not None
+                self.p[b] = self.f[b] / 2 + (1 if warm else 0)
+        return 0, 3, None
+
+    def block_stats(self):
+        return [3] * len(self.blocks), [t / 2 for t in self.tol]
+
+    def result_p(self, b):
+        return self.p[b]
+
+    def close(self):
+        self.closed = True
+
+
+def _req(n, key=None, tol=1e-8, val=1.0, rid=0):
+    return SV.Request(n=n, h=tuple(1.0 / x for x in n), bc=(0,) * 6,
+                      f=np.full(n[::-1], val), tol=tol, key=key, rid=rid)
+
+
+@pytest.fixture(autouse=True)
+def _reset():
+    FakeSolver.created = []
+
+
+def test_footprint_grows_with_blocks_and_precision():
+    a = dict(n=(64, 64, 64), h=(1,) * 3, bc=(0,) * 6, nlev=4)
+    b = dict(n=(128, 64, 32), h=(1,) * 3, bc=(0,) * 6, nlev=4)
+    f1 = SV.footprint_bytes([a])
+    f2 = SV.footprint_bytes([b])
+    # the staging array is shared by the blocks of a handle
+    assert max(f1, f2) < SV.footprint_bytes([a, b]) <= f1 + f2
+    assert SV.footprint_bytes([a], "fp64", "smooth") > f1
+    # 256^3 mixed: the arena is ~0.64 GB
+    big = dict(n=(256,) * 3, h=(1,) * 3, bc=(0,) * 6, nlev=6)
+    assert 0.6e9 < SV.footprint_bytes([big]) < 0.8e9
+
+
+def test_requests_waiting_together_share_one_batch():
+    srv = SV.PoissonServer(max_batches=1, solver_factory=FakeSolver)
+    try:
+        futs = [srv.submit(_req((32, 32, 32), key=i, rid=i)) for i in range(6)]
+        res = [f.result(timeout=10) for f in futs]
+    finally:
+        srv.close()
+    # the first batch starts at once; the other five wait and go together
+    assert sum(r.batch_blocks for r in res) >= 6
+    assert max(r.batch_blocks for r in res) >= 2
+    assert len(FakeSolver.created) < 6
+    assert all(np.all(r.p == 0.5) for r in res)
+
+
+def test_reuse_is_warm_and_on_the_resident_handle():
+    srv = SV.PoissonServer(solver_factory=FakeSolver)
+    try:
+        r1 = srv.submit(_req((32, 16, 16), key="sim-a", val=2.0)).result(timeout=10)
+        r2 = srv.submit(_req((32, 16, 16), key="sim-a", val=4.0)).result(timeout=10)
+    finally:
+        srv.close()
+    assert not r1.warm and r2.warm
+    assert len(FakeSolver.created) == 1
+    assert np.all(r2.p == 4.0 / 2 + 1)  # warm: the handle kept its pressure
+
+
+def test_budget_evicts_idle_hierarchies_lru():
+    one = SV.footprint_bytes([dict(n=(64,) * 3, h=(1,) * 3, bc=(0,) * 6, nlev=4)])
+    srv = SV.PoissonServer(hbm_budget=int(2.5 * one), max_batches=1, solver_factory=FakeSolver)
+    try:
+        for i in range(4):
+            srv.submit(_req((64, 64, 64), key=i, rid=i)).result(timeout=10)
+        assert srv.resident_bytes <= int(2.5 * one)
+        assert srv.stats["evicted"] >= 2
+        assert FakeSolver.created[0].closed and not FakeSolver.created[-1].closed
+    finally:
+        srv.close()
+
+
+def test_request_larger_than_budget_fails_cleanly():
+    srv = SV.PoissonServer(hbm_budget=1 << 20, solver_factory=FakeSolver)
+    try:
+        fut = srv.submit(_req((64, 64, 64)))
+        with pytest.raises(MemoryError):
+            fut.result(timeout=10)
+    finally:
+        srv.close()
+
+
+def test_request_stream_follows_the_recipe():
+    reqs = SV.request_stream(400, rate=200.0, seed=4)
+    t = np.array([r["t"] for r in reqs])
+    assert np.all(np.diff(t) > 0) and 1.5 < t[-1] < 2.5  # ~400 / 200 s
+    sizes = np.array([r["n"] for r in reqs])
+    assert sizes.min() >= 32 and sizes.max() <= 256 and np.all(sizes % 16 == 0)
+    reuse = sum(1 for r in reqs if r["step"] > 0)
+    assert 0.12 < reuse / len(reqs) < 0.28
+    assert {r["tol"] for r in reqs} == {1e-6, 1e-8}
+
+
+@pytest.mark.gpu
+def test_served_results_equal_solving_alone():
+    """Requests of different sizes, boundaries and tolerances served
+    concurrently (batched, overlapping, one warm reuse) give bitwise the
+    pressure each gets from a fresh solver alone."""
+    import torch
+    from hydro3d_amd import MultigridSolver
+    torch.cuda.set_device(0)
+    reqs = SV.request_stream(24, rate=2000.0, seed=11, hi=96)
+    print("case: 24 served requests vs alone", flush=True)
+    with SV.PoissonServer(max_batches=3) as srv:
+        results, _ = SV.run_stream(srv, reqs)
+    by_key = {}
+    for d, r in zip(reqs, results):
+        assert r.status == 0, (d["rid"], r.status)
+        f = SV.device_rhs(d)
+        spec = SV.Request(n=d["n"], h=d["h"], bc=d["bc"], f=None).spec()
+        if r.warm:
+            # alone: the same warm start (the key's previous pressure)
+            p0 = by_key[d["key"]]
+        else:
+            p0 = torch.zeros_like(f)
+        with MultigridSolver([spec], tol=d["tol"], precision="mixed", coarse="exact") as S:
+            S.set_rhs(0, f)
+            S.set_p(0, p0)
+            S.solve()
+            alone = S.result_p(0)
+        assert torch.equal(alone, r.p), d["rid"]
+        by_key[d["key"]] = r.p
