@@ -111,6 +111,90 @@ def copy_probe(nbytes=1 << 28, reps=5):
                       "reps": reps}), flush=True)
 
 
+def projected_divergence(c, p, with_ustar=True):
+    """Post-projection divergence for pressure p (interior, (nk,nj,ni)):
+    u = u* - dt grad p on every face that calvel corrects (press.for:77-144:
+    interior faces and the periodic wrap; wall-normal faces keep u* = 0),
+    then div(u) per cell as calmas forms it (press.for:48-63).  Returns the
+    reference's rmax = max|div u| dx dy dz and the L2 norm of div u."""
+    dx, dy, dz = c["h"]
+    dt = c["dt"]
+    bc = c["bc"]
+    div = np.zeros_like(p)
+    for ax, (hh, vel) in enumerate(((dx, c["u"]), (dy, c["v"]), (dz, c["w"]))):
+        a = 2 - ax  # numpy axis of this direction in (k, j, i)
+        corr = vel.copy() if with_ustar else np.zeros_like(vel)
+        dp = np.diff(p, axis=a) / hh
+        inner = [slice(None)] * 3
+        inner[a] = slice(1, -1)
+        corr[tuple(inner)] -= dt * dp
+        if bc[2 * ax] == 1:  # periodic: faces 0 and n are the wrap face
+            first = [slice(None)] * 3
+            last = [slice(None)] * 3
+            first[a], last[a] = 0, -1
+            wrap = (np.take(p, [0], axis=a) - np.take(p, [p.shape[a] - 1], axis=a)) / hh
+            corr[tuple(first)] -= dt * np.squeeze(wrap, axis=a)
+            corr[tuple(last)] = corr[tuple(first)]
+        div += np.diff(corr, axis=a) / hh
+    return float(np.abs(div).max() * dx * dy * dz), float(np.linalg.norm(div))
+
+
+def batched_blocks(steps, warmup, tune_cache):
+    """48 independent uneven blocks (sizes from {32,48,64,96,128} per axis,
+    Neumann walls, 4 levels) solved together: one launch per level and
+    operator covers every block, the V-cycle is a hipGraph, launch shapes
+    come from the autotuner.  Mixed precision to 1e-8 per block."""
+    import torch
+    from hydro3d_amd import MultigridSolver, configs
+    from hydro3d_amd import autotune as AT
+    cs = configs.uneven_blocks()
+    specs = [dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"]) for c in cs]
+    cells = int(sum(np.prod(c["n"]) for c in cs))
+    fs = [torch.from_numpy(c["f"]).cuda() for c in cs]
+    zs = [torch.zeros_like(f) for f in fs]
+    with MultigridSolver(specs, tol=1e-8, precision="mixed", coarse="exact") as S:
+        for b, f in enumerate(fs):
+            S.set_rhs(b, f)
+        choice = AT.autotune(S, tune_cache, reps=3)
+        lat = []
+        for it in range(warmup + steps):
+            for b, z in enumerate(zs):
+                S.set_p(b, z)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, ncyc, hist = S.solve(raise_on_fail=True)
+            torch.cuda.synchronize()
+            if it >= warmup:
+                lat.append(time.perf_counter() - t0)
+        cyc, rel = S.block_stats()
+    lat = np.array(lat)
+    return {"workload": "48 uneven blocks (seed 2), sizes {32,48,64,96,128}^3 per axis, "
+                        "Neumann walls, 4 levels, mixed to 1e-8 per block",
+            "cells": cells, "batch_solves_per_s": round(1.0 / lat.mean(), 3),
+            "ms_per_batch": round(lat.mean() * 1e3, 3),
+            "mcell_solves_per_s": round(cells / lat.mean() / 1e6, 1),
+            "cycles_max": int(max(cyc)), "cycles_min": int(min(cyc)),
+            "worst_rel_residual": float(max(rel)), "tuning": choice}
+
+
+def serving(nreq):
+    """The serving stream (seeded Poisson arrivals at 200/s, sizes 32..256,
+    tolerances 1e-6/1e-8, 20% warm reuse): continuous-batching mixed server
+    against sequential FP64 solves of the same stream."""
+    from hydro3d_amd import serving as SV
+    reqs = SV.request_stream(nreq, 200.0, 4)
+    out = {}
+    with SV.PoissonServer(hbm_budget=64 << 30, max_batches=4) as srv:
+        res, wall = SV.run_stream(srv, reqs)
+        out["served_mixed"] = SV.summarize(res, wall)
+    with SV.PoissonServer(hbm_budget=64 << 30, max_batches=1, max_batch_blocks=1,
+                          precision="fp64", coarse="smooth", keep_resident=False) as srv:
+        res, wall = SV.run_stream(srv, reqs)
+        out["sequential_fp64"] = SV.summarize(res, wall)
+    out["workload"] = f"{nreq} requests of the seeded stream (seed 4, 200/s)"
+    return out
+
+
 def cpu_baseline(c, fp64_cycles):
     """Reference Fortran (if built) else the C oracle, on one core, on a
     bounded sample: a few V-cycles of the 256^3 workload; solves/s is
@@ -157,6 +241,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and exit")
+    ap.add_argument("--serve-requests", type=int, default=400,
+                    help="requests of the serving stream to replay (0: skip)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 48-block batch")
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "hydro3d_amd", "tuned.json"))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver is single-GPU (independent blocks shard "
@@ -179,8 +267,12 @@ def main():
     zero_dev = torch.zeros_like(f_dev)
 
     log("building solvers")
+    from hydro3d_amd import autotune as AT
     S64 = MultigridSolver([spec], tol=1e-8, precision="fp64", coarse="smooth")
     Smx = MultigridSolver([spec], tol=1e-8, precision="mixed", coarse="exact")
+    # launch shapes chosen by the autotuner earlier (committed table), if any
+    tuned = {"fp64": AT.apply_cached(S64, args.tune_cache),
+             "mixed": AT.apply_cached(Smx, args.tune_cache)}
 
     log("timing fp64")
     lat64, cyc64, fin64 = timed_solves(S64, f_dev, zero_dev, args.steps, args.warmup)
@@ -207,11 +299,33 @@ def main():
     perr64 = float(torch.linalg.norm(p64 - pst) / nst)
     log(f"pressure error vs converged FP64 ({cyc_t} cycles, {hist_t[-1]:.1e}): "
         f"mixed {perr:.2e}, fp64@1e-8 {perr64:.2e}")
+    # post-projection divergence (the reference's rmax and the L2 norm)
+    div0 = float(np.linalg.norm(c["f"])) * c["dt"]
+    dv = {}
+    for name, t in (("mixed", pmx), ("fp64_at_1e-8", p64), ("fp64_converged", pst)):
+        rmax, l2 = projected_divergence(c, t.cpu().numpy())
+        dv[name] = {"rmax": rmax, "rel_l2": l2 / div0}
+    # div(u_mixed) - div(u_converged): the projection of the pressure
+    # difference alone (u* cancels)
+    dv["mixed_minus_converged_rel_l2"] = float(
+        projected_divergence(c, (pmx - pst).cpu().numpy(), with_ustar=False)[1] / div0)
+    log(f"post-projection divergence: {dv}")
 
     ms_pass, bytes_pass, achieved = smoother_roofline(Smx, N, 4)
     log(f"smoother pass (fp32, 256^3): {ms_pass*1e3:.1f} us, {achieved:.0f} GB/s")
     ms64, bytes64, ach64 = smoother_roofline(S64, N, 8)
     log(f"smoother pass (fp64, 256^3): {ms64*1e3:.1f} us, {ach64:.0f} GB/s")
+
+    batched = None
+    if not args.no_batched:
+        log("batched 48 blocks")
+        batched = batched_blocks(max(3, args.steps // 2), 1, args.tune_cache)
+        log(batched)
+    serve = None
+    if args.serve_requests > 0:
+        log("serving stream")
+        serve = serving(args.serve_requests)
+        log(serve)
 
     cpu = None
     if not args.no_cpu:
@@ -244,12 +358,14 @@ def main():
         "fp64_solves_per_s": round(1.0 / lat64.mean(), 3),
         "speedup_mixed_vs_fp64": round(lat64.mean() / latmx.mean(), 3),
         "cycles": {"mixed": cycmx, "fp64": cyc64},
+        "tuning": tuned,
         "final_rel_residual": {"mixed": finmx, "fp64": fin64},
         "latency_ms": {"mixed_p50": round(float(np.percentile(latmx, 50)) * 1e3, 3),
                        "mixed_p99": round(float(np.percentile(latmx, 99)) * 1e3, 3),
                        "fp64_p50": round(float(np.percentile(lat64, 50)) * 1e3, 3),
                        "fp64_p99": round(float(np.percentile(lat64, 99)) * 1e3, 3)},
         "pressure_rel_err_vs_converged_fp64": {"mixed": perr, "fp64_at_1e-8": perr64},
+        "post_projection_divergence": dv,
         "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<float> (256^3 colour half-sweep)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
@@ -258,6 +374,8 @@ def main():
                      "fp64_pass": {"achieved": round(ach64, 1), "avg_launch_us": round(ms64 * 1e3, 2),
                                    "frac": round(ach64 / HBM_PEAK_GBS, 4)}},
         "cpu_baseline": cpu,
+        "batched_uneven_48": batched,
+        "serving": serve,
     }
     print(json.dumps(out), flush=True)
 

@@ -824,6 +824,109 @@ __device__ __forceinline__ void residual_tile(const Tile &t, const FineDev &F, c
   }
 }
 
+// Interior residual tile of exactly KB planes, every load issued up front
+// (no branches between a load and its use): the own-column window of both
+// colours (KB+2 planes), f, rows j-1 and j+1, and one "edge" value per side
+// and plane -- the x-ghost image at a row end, the neighbouring tile's entry
+// at a tile edge, the lane's own entry (unused) elsewhere.  The arithmetic is
+// residual_tile<T, true>'s.
+template <class T, int KB>
+__device__ __forceinline__ void residual_tile_inner(const Tile &t, const FineDev &F,
+                                                    const LevelDev &L, double &ss, double &am) {
+  const int lpr = F.lpr;
+  const int lane = (int)threadIdx.x;
+  const int rl = lane & (lpr - 1);
+  const int l = t.x0 + rl;
+  const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
+  const int ni = F.ni, nj = F.nj, H = F.H;
+  const bool ok = 2 * l < ni && j < nj;
+  const bool b_ok = 2 * l + 1 < ni && j < nj;
+  const unsigned row = (unsigned)H, plane = (unsigned)H * (unsigned)nj;
+  const int jc = min(j, nj - 1);
+  const unsigned rbase = (unsigned)jc * row;
+  const unsigned cbase = rbase + (unsigned)min(l, H - 1);
+  const gmem<double> *P[2] = {(const gmem<double> *)F.p[0], (const gmem<double> *)F.p[1]};
+  const gmem<double> *Fv[2] = {(const gmem<double> *)F.f[0], (const gmem<double> *)F.f[1]};
+  gmem<T> *R[2] = {(gmem<T> *)L.rhs[0], (gmem<T> *)L.rhs[1]};
+  const bool px = F.per & 1;
+  const double cx = F.cx, cy = F.cy, cz = F.cz;
+  const int ia = 2 * l, ib = ia + 1;
+  // west neighbour of a (cell ia-1) and east neighbour of b (cell ib+1) when
+  // they are not in the adjacent lane: x-ghost image or the next tile's entry
+  const bool w_ghost = ia == 0, e_ghost = ib + 1 >= ni;
+  const bool w_edge = w_ghost || (rl == 0 && l > 0);
+  const bool e_edge = e_ghost || (rl == lpr - 1 && l + 1 < H);
+  const int xw = image(ia - 1, ni, px), xe = image(ib + 1, ni, px);
+  // entry and colour parity of those cells relative to (j + k)
+  const unsigned w_ent = rbase + (unsigned)(w_edge ? (xw >> 1) : min(l, H - 1));
+  const unsigned e_ent = rbase + (unsigned)(e_edge ? (xe >> 1) : min(l, H - 1));
+  const int w_par = w_edge ? (xw & 1) : 0, e_par = e_edge ? (xe & 1) : 0;
+  auto ldp = [&](int c, unsigned off) -> double { return P[c][off]; };
+
+  const int k0 = t.z0;
+  double A[KB + 2], B[KB + 2];  // cell a / b of the own row, planes k0-1 .. k0+KB
+  double FA[KB], FB[KB], SA[KB], SB[KB], NA[KB], NB[KB], WE[KB], EE[KB];
+#pragma unroll
+  for (int p = 0; p < KB + 2; ++p) {
+    const int k = k0 - 1 + p;
+    const int ca = (jc + k) & 1;
+    const unsigned q = (unsigned)k * plane + cbase;
+    A[p] = ldp(ca, q);
+    B[p] = ldp(ca ^ 1, q);
+  }
+#pragma unroll
+  for (int p = 0; p < KB; ++p) {
+    const int k = k0 + p;
+    const int ca = (jc + k) & 1, cb = ca ^ 1;
+    const unsigned q = (unsigned)k * plane + cbase;
+    FA[p] = Fv[ca][q];
+    FB[p] = Fv[cb][q];
+    SA[p] = ldp(cb, q - row);
+    SB[p] = ldp(ca, q - row);
+    NA[p] = ldp(cb, q + row);
+    NB[p] = ldp(ca, q + row);
+    WE[p] = ldp((w_par + jc + k) & 1, (unsigned)k * plane + w_ent);
+    EE[p] = ldp((e_par + jc + k) & 1, (unsigned)k * plane + e_ent);
+  }
+  Coef<double> c_a, c_b;
+  c_a.as = c_a.an = c_b.as = c_b.an = cy;
+  c_a.ab = c_a.at = c_b.ab = c_b.at = cz;
+  c_a.aw = (ia == 0 && !px) ? 0.0 : cx;
+  c_a.ae = (ia == ni - 1 && !px) ? 0.0 : cx;
+  c_b.aw = cx;
+  c_b.ae = (ib == ni - 1 && !px) ? 0.0 : cx;
+  c_a.ap = -1.0 * (c_a.aw + c_a.ae + c_a.as + c_a.an + c_a.ab + c_a.at);
+  c_b.ap = -1.0 * (c_b.aw + c_b.ae + c_b.as + c_b.an + c_b.ab + c_b.at);
+#pragma unroll
+  for (int p = 0; p < KB; ++p) {
+    const int k = k0 + p;
+    const int ca = (jc + k) & 1, cb = ca ^ 1;
+    const unsigned q = (unsigned)k * plane + cbase;
+    const double Pa = A[p + 1], Pb = B[p + 1];
+    // cell 2l-1 is lane-1's b, cell 2l+2 is lane+1's a
+    const double lf = wave_shift(Pb, 1), rt = wave_shift(Pa, 0);
+    const double Wa = w_edge ? WE[p] : lf;
+    const double Ea = ib < ni ? Pb : EE[p];
+    const double Wb = Pa;
+    const double Eb = e_edge ? EE[p] : rt;
+    double ra = FA[p], rb = FB[p];
+    ra = ra - (c_a.aw * Wa + c_a.ae * Ea + c_a.as * SA[p] + c_a.an * NA[p] + c_a.ab * A[p] +
+               c_a.at * A[p + 2] + c_a.ap * Pa);
+    rb = rb - (c_b.aw * Wb + c_b.ae * Eb + c_b.as * SB[p] + c_b.an * NB[p] + c_b.ab * B[p] +
+               c_b.at * B[p + 2] + c_b.ap * Pb);
+    if (ok) {
+      R[ca][q] = (T)ra;
+      ss += ra * ra;
+      am = fmax(am, fabs(ra));
+    }
+    if (b_ok) {
+      R[cb][q] = (T)rb;
+      ss += rb * rb;
+      am = fmax(am, fabs(rb));
+    }
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(NTHREADS)
     k_residual(const Tile *__restrict__ tiles, const FineDev *__restrict__ fine,
@@ -834,7 +937,13 @@ __global__ void __launch_bounds__(NTHREADS)
   const LevelDev &L = levs[t.lev];
   double ss = 0.0, am = 0.0;
   if (active[F.blk]) {
-    if (t.kind == 0)
+    if (t.kind == 0 && use_p && t.z1 - t.z0 == 4)
+      residual_tile_inner<T, 4>(t, F, L, ss, am);
+    else if (t.kind == 0 && use_p && t.z1 - t.z0 == 2)
+      residual_tile_inner<T, 2>(t, F, L, ss, am);
+    else if (t.kind == 0 && use_p && t.z1 - t.z0 == 1)
+      residual_tile_inner<T, 1>(t, F, L, ss, am);
+    else if (t.kind == 0)
       residual_tile<T, true>(t, F, L, use_p, ss, am);
     else
       residual_tile<T, false>(t, F, L, use_p, ss, am);

@@ -599,7 +599,10 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     const int lpr = lanes_per_row(2 * L.H);
     const int xs = lpr, ys = TILE_Y * (64 / lpr);
     const int txy = ((L.H + xs - 1) / xs) * ((L.nj + ys - 1) / ys);
-    const int kb = march_depth(txy, L.nk);
+    // at most 4 planes (a power of two): the interior tiles' loads all go
+    // out at once, which bounds the depth by the registers they occupy
+    int kb = std::min(4, march_depth(txy, L.nk, 2048));
+    if (kb == 3) kb = 2;
     for (int z = 0; z < L.nk; z += kb) {
       const int z1 = std::min(z + kb, L.nk);
       for (int y = 0; y < L.nj; y += ys)

@@ -165,3 +165,20 @@ def test_served_results_equal_solving_alone():
             alone = S.result_p(0)
         assert torch.equal(alone, r.p), d["rid"]
         by_key[d["key"]] = r.p
+
+
+@pytest.mark.gpu
+def test_footprint_covers_the_device_arena():
+    """Admission uses footprint_bytes: it must not undercount what a handle
+    allocates."""
+    from hydro3d_amd import MultigridSolver
+    print("case: footprint vs device bytes", flush=True)
+    shapes = [[dict(n=(64, 64, 64), h=(1 / 64,) * 3, bc=(1, 1, 1, 1, 0, 0), nlev=4)],
+              [dict(n=(96, 48, 32), h=(0.01,) * 3, bc=(0,) * 6, nlev=4),
+               dict(n=(32, 32, 32), h=(0.03,) * 3, bc=(1,) * 6, nlev=3)]]
+    for specs in shapes:
+        for prec, coarse in (("mixed", "exact"), ("fp64", "smooth")):
+            with MultigridSolver(specs, precision=prec, coarse=coarse) as S:
+                dev = S.device_bytes()
+            est = SV.footprint_bytes(specs, prec, coarse)
+            assert dev <= est <= 1.5 * dev + (4 << 20), (specs, prec, dev, est)
