@@ -66,6 +66,7 @@ SIGNATURES = {
     "h3dmg_set_tail": (_i, [_vp, _i]),
     "h3dmg_set_block_tol": (_i, [_vp, _i, _d]),
     "h3dmg_set_block_enabled": (_i, [_vp, _i, _i]),
+    "h3dmg_project_device": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _dp]),
     "h3dmg_get_tail": (_i, [_vp]),
     "h3dmg_dbg_put": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_get": (_i, [_vp, _i, _i, _i, _vp]),

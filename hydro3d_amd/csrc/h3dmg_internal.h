@@ -115,6 +115,15 @@ size_t vtail_lds_bytes(const LevelDev *levs_host, int top, bool mixed, bool exac
 // permit `lds_bytes` of dynamic LDS for the tail kernel
 hipError_t vtail_allow_lds(size_t lds_bytes);
 
+// velocity projection (calvel per axis over `faces` faces; calmas writes
+// su (optional) and one partial max per workgroup, returns the group count)
+constexpr int CALMAS_MAX_GROUPS = 1024;
+void launch_calvel(hipStream_t s, const FineDev *fd, int axis, const double *us, double *u,
+                   double dt_alfa, double h, size_t faces);
+int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const double *v,
+                  const double *w, double *su, double dt, double dx, double dy, double dz,
+                  RedPartial *partials, size_t cells);
+
 // layout conversion: dense interior (i fastest) <-> colour split
 void launch_pack_fine(hipStream_t s, const FineDev *fd, int which, const double *src, int ni,
                       int nj, int nk);

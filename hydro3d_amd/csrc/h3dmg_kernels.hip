@@ -9,6 +9,7 @@
 // is built with -ffp-contract=off, so the FP64 path is bitwise the reference.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "h3dmg_internal.h"
@@ -267,9 +268,9 @@ __device__ __forceinline__ void march_tile(const Tile &t, const LevelDev &L, int
 // the own-column window (KB+2 planes), rows j-1 and j+1 and the rhs of all KB
 // planes, then the KB updates.  One memory round trip per tile instead of
 // one or more per plane; the arithmetic is march_tile<T, true>'s.
-template <class T, int KB>
+template <class T, int KB, bool ZERO_INT>
 __device__ __forceinline__ void march_tile_inner(const Tile &t, const LevelDev &L, int colour,
-                                                 int src, int zero) {
+                                                 int src) {
   typedef typename Vec2<T>::type v2;
   const int lpr = L.lpr;
   const int lane = (int)threadIdx.x;
@@ -282,7 +283,6 @@ __device__ __forceinline__ void march_tile_inner(const Tile &t, const LevelDev &
   const gmem<T> *O = (const gmem<T> *)(colour == 0 ? L.phi[src][1] : L.phi[1 - src][0]);
   const gmem<T> *R = (const gmem<T> *)L.rhs[colour];
   gmem<T> *D = (gmem<T> *)L.phi[1 - src][colour];
-  const bool zero_int = zero && colour == 0;  // uniform
   const unsigned cbase = (unsigned)min(j, nj - 1) * row + (unsigned)min(m0, H - 2);
   const v2 vz = {T(0), T(0)};
   const bool tile_left = t.x0 > 0, tile_right = t.x0 + 2 * lpr < H;
@@ -299,30 +299,32 @@ __device__ __forceinline__ void march_tile_inner(const Tile &t, const LevelDev &
   const unsigned q0 = (unsigned)k0 * plane + cbase;
   v2 own[KB + 2], S2[KB], N2[KB], R2[KB];
   T lf_e[KB], rt_e[KB];
-  if (!zero_int) {
+  // entries m0-1 and m0+2 of tiles that do not start/end a row; the other
+  // lanes load their own entry instead (no branch between loads)
+  const bool need_l = tile_left && rl == 0;
+  const bool need_r = tile_right && rl == lpr - 1;
+  const unsigned off_l = need_l ? q0 - 1 : q0, off_r = need_r ? q0 + 2 : q0;
+  if (!ZERO_INT) {
 #pragma unroll
     for (int p = 0; p < KB + 2; ++p) own[p] = ld2(O, q0 + (unsigned)(p - 1) * plane);
 #pragma unroll
     for (int p = 0; p < KB; ++p) {
       S2[p] = ld2(O, q0 + (unsigned)p * plane - row);
       N2[p] = ld2(O, q0 + (unsigned)p * plane + row);
+      lf_e[p] = ld1(O, off_l + (unsigned)p * plane);
+      rt_e[p] = ld1(O, off_r + (unsigned)p * plane);
     }
   } else {
 #pragma unroll
     for (int p = 0; p < KB + 2; ++p) own[p] = vz;
 #pragma unroll
-    for (int p = 0; p < KB; ++p) S2[p] = N2[p] = vz;
+    for (int p = 0; p < KB; ++p) {
+      S2[p] = N2[p] = vz;
+      lf_e[p] = rt_e[p] = T(0);
+    }
   }
 #pragma unroll
   for (int p = 0; p < KB; ++p) R2[p] = ld2(R, q0 + (unsigned)p * plane);
-  // entries m0-1 and m0+2 of tiles that do not start/end a row
-  const bool need_l = tile_left && !zero_int && rl == 0;
-  const bool need_r = tile_right && !zero_int && rl == lpr - 1;
-#pragma unroll
-  for (int p = 0; p < KB; ++p) {
-    lf_e[p] = need_l ? ld1(O, q0 + (unsigned)p * plane - 1) : T(0);
-    rt_e[p] = need_r ? ld1(O, q0 + (unsigned)p * plane + 2) : T(0);
-  }
 #pragma unroll
   for (int p = 0; p < KB; ++p) {
     const int k = k0 + p;
@@ -336,19 +338,23 @@ __device__ __forceinline__ void march_tile_inner(const Tile &t, const LevelDev &
     const T E0 = s ? o_c.y : o_c.x;
     const T W1 = s ? o_c.y : o_c.x;
     const T E1 = s ? rt : o_c.y;
-    const int i0 = 2 * m0 + s, i1 = i0 + 2;
     const T sum0 = cx * W0 + cx * E0 + cy * S2[p].x + cy * N2[p].x + cz * o_m.x + cz * o_p.x;
     const T sum1 = cx * W1 + cx * E1 + cy * S2[p].y + cy * N2[p].y + cz * o_m.y + cz * o_p.y;
-    const T out0 = div_ap(R2[p].x - sum0, ap_in, inv_in);
-    const T out1 = div_ap(R2[p].y - sum1, ap_in, inv_in);
+    // the updates replace the rhs values (all stores after all loads)
+    R2[p].x = div_ap(R2[p].x - sum0, ap_in, inv_in);
+    R2[p].y = div_ap(R2[p].y - sum1, ap_in, inv_in);
+  }
+#pragma unroll
+  for (int p = 0; p < KB; ++p) {
+    const int s = (colour + j + k0 + p) & 1;
+    const int i0 = 2 * m0 + s, i1 = i0 + 2;
     const bool w0 = col_ok && i0 > 0 && i0 < ni - 1, w1 = col_ok && i1 < ni - 1;
     gmem<char> *dq = (gmem<char> *)D + (q0 + (unsigned)p * plane) * (unsigned)sizeof(T);
     if (w0 && w1) {
-      const v2 o = {out0, out1};
-      *(gmem<v2> *)dq = o;
+      *(gmem<v2> *)dq = R2[p];
     } else {
-      if (w0) *(gmem<T> *)dq = out0;
-      if (w1) *(gmem<T> *)(dq + sizeof(T)) = out1;
+      if (w0) *(gmem<T> *)dq = R2[p].x;
+      if (w1) *(gmem<T> *)(dq + sizeof(T)) = R2[p].y;
     }
   }
 }
@@ -397,11 +403,21 @@ __global__ void __launch_bounds__(NTHREADS)
   const LevelDev &L = levs[t.lev];
   if (!active[L.blk]) return;
   if (t.kind == 0) {
+    // the interior neighbours are zeros in the red half of a zero-state sweep
+    const bool zint = zero && colour == 0;
     switch (t.z1 - t.z0) {
-      case 1: march_tile_inner<T, 1>(t, L, colour, src, zero); break;
-      case 2: march_tile_inner<T, 2>(t, L, colour, src, zero); break;
-      case 4: march_tile_inner<T, 4>(t, L, colour, src, zero); break;
-      case 8: march_tile_inner<T, 8>(t, L, colour, src, zero); break;
+      case 1:
+        zint ? march_tile_inner<T, 1, true>(t, L, colour, src)
+             : march_tile_inner<T, 1, false>(t, L, colour, src);
+        break;
+      case 2:
+        zint ? march_tile_inner<T, 2, true>(t, L, colour, src)
+             : march_tile_inner<T, 2, false>(t, L, colour, src);
+        break;
+      case 4:
+        zint ? march_tile_inner<T, 4, true>(t, L, colour, src)
+             : march_tile_inner<T, 4, false>(t, L, colour, src);
+        break;
       default: march_tile<T, true>(t, L, colour, src, zero);
     }
   } else if (t.kind == 1)
@@ -899,9 +915,6 @@ __device__ __forceinline__ void residual_tile_inner(const Tile &t, const FineDev
   c_b.ap = -1.0 * (c_b.aw + c_b.ae + c_b.as + c_b.an + c_b.ab + c_b.at);
 #pragma unroll
   for (int p = 0; p < KB; ++p) {
-    const int k = k0 + p;
-    const int ca = (jc + k) & 1, cb = ca ^ 1;
-    const unsigned q = (unsigned)k * plane + cbase;
     const double Pa = A[p + 1], Pb = B[p + 1];
     // cell 2l-1 is lane-1's b, cell 2l+2 is lane+1's a
     const double lf = wave_shift(Pb, 1), rt = wave_shift(Pa, 0);
@@ -914,6 +927,15 @@ __device__ __forceinline__ void residual_tile_inner(const Tile &t, const FineDev
                c_a.at * A[p + 2] + c_a.ap * Pa);
     rb = rb - (c_b.aw * Wb + c_b.ae * Eb + c_b.as * SB[p] + c_b.an * NB[p] + c_b.ab * B[p] +
                c_b.at * B[p + 2] + c_b.ap * Pb);
+    FA[p] = ra;  // the residuals replace f (all stores after all loads)
+    FB[p] = rb;
+  }
+#pragma unroll
+  for (int p = 0; p < KB; ++p) {
+    const int k = k0 + p;
+    const int ca = (jc + k) & 1, cb = ca ^ 1;
+    const unsigned q = (unsigned)k * plane + cbase;
+    const double ra = FA[p], rb = FB[p];
     if (ok) {
       R[ca][q] = (T)ra;
       ss += ra * ra;
@@ -1400,6 +1422,75 @@ __global__ void __launch_bounds__(TAIL_THREADS)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Velocity projection of the outer loop (press.for:77-144 calvel and 2-75
+// calmas, 2nd-order branch) on the device.  Face arrays are dense, i
+// fastest: u (nk, nj, ni+1) with u[.., f] the face between cells f-1 and f,
+// v (nk, nj+1, ni), w (nk+1, nj, ni); cell arrays (nk, nj, ni).
+//
+// calvel: face f of axis a between cells f-1 and f,
+//     u = ustar - dt*alfapr*(p(f) - p(f-1))/h
+// on the faces the reference updates (isu..ieu, localparameters.for:
+// 106-192): the interior faces and, on a periodic axis, the wrap face (its
+// second copy f = 0 is what exchange(1) gives it); wall-normal faces keep
+// ustar.  Pressure ghosts are images, as after pbound/exchange(4).
+__global__ void __launch_bounds__(256)
+    k_calvel(const FineDev *__restrict__ fd, int axis, const double *__restrict__ us,
+             double *__restrict__ u, double dt_alfa, double h) {
+  const FineDev &F = *fd;
+  const int n[3] = {F.ni, F.nj, F.nk};
+  int dim[3] = {F.ni, F.nj, F.nk};
+  dim[axis] += 1;  // faces along `axis`
+  const size_t total = (size_t)dim[0] * dim[1] * dim[2];
+  const bool per = (F.per >> axis) & 1;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    int c[3];
+    c[0] = (int)(q % dim[0]);
+    c[1] = (int)((q / dim[0]) % dim[1]);
+    c[2] = (int)(q / ((size_t)dim[0] * dim[1]));
+    const int f = c[axis];
+    const bool upd = (f > 0 && f < n[axis]) || per;
+    double v = us[q];
+    if (upd) {
+      int hi[3] = {c[0], c[1], c[2]}, lo[3] = {c[0], c[1], c[2]};
+      hi[axis] = f == n[axis] ? 0 : f;        // wrap: cell 0 (the ghost image)
+      lo[axis] = f == 0 ? n[axis] - 1 : f - 1;
+      const double ph = F.p[(hi[0] + hi[1] + hi[2]) & 1][cidx(F.H, F.nj, hi[0], hi[1], hi[2])];
+      const double pl = F.p[(lo[0] + lo[1] + lo[2]) & 1][cidx(F.H, F.nj, lo[0], lo[1], lo[2])];
+      v = v - dt_alfa * (ph - pl) / h;
+    }
+    u[q] = v;
+  }
+}
+
+// calmas: su = ((du/dx + dv/dy) + dw/dz)/dt per cell, and per workgroup the
+// max of |fact*dx*dy*dz| (rmax before its MPI_ALLREDUCE, press.for:59-73)
+__global__ void __launch_bounds__(256)
+    k_calmas(const FineDev *__restrict__ fd, const double *__restrict__ u,
+             const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ su,
+             double dt, double dx, double dy, double dz, RedPartial *__restrict__ partials) {
+  const FineDev &F = *fd;
+  const int ni = F.ni, nj = F.nj, nk = F.nk;
+  const size_t total = (size_t)ni * nj * nk;
+  double am = 0.0, ss = 0.0;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % ni), j = (int)((q / ni) % nj), k = (int)(q / ((size_t)ni * nj));
+    const size_t qu = ((size_t)k * nj + j) * (ni + 1) + i;
+    const size_t qv = ((size_t)k * (nj + 1) + j) * ni + i;
+    const size_t qw = ((size_t)k * nj + j) * ni + i;
+    const double fact = (u[qu + 1] - u[qu]) / dx + (v[qv + ni] - v[qv]) / dy +
+                        (w[qw + (size_t)ni * nj] - w[qw]) / dz;
+    if (su) su[q] = fact / dt;
+    const double fact1 = fact * dx * dy * dz;
+    am = fmax(am, fabs(fact1));
+  }
+  // fixed-order workgroup reduction (block_reduce needs 256 = NTHREADS threads)
+  block_reduce(ss, am);
+  if (threadIdx.x == 0) partials[blockIdx.x] = RedPartial{0.0, am};
+}
+
 // ---- layout conversion ---------------------------------------------------
 __global__ void k_pack_fine(const FineDev *__restrict__ fd, int which,
                             const double *__restrict__ src, int ni, int nj, int nk) {
@@ -1588,6 +1679,21 @@ size_t vtail_lds_bytes(const LevelDev *levs_host, int top, bool mixed, bool exac
     off += 2 * (size_t)na[0] * na[1] * na[2] * 8;
   }
   return off;
+}
+
+void launch_calvel(hipStream_t s, const FineDev *fd, int axis, const double *us, double *u,
+                   double dt_alfa, double h, size_t faces) {
+  const unsigned grid = (unsigned)std::min<size_t>((faces + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_calvel, dim3(grid), dim3(256), 0, s, fd, axis, us, u, dt_alfa, h);
+}
+
+int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const double *v,
+                  const double *w, double *su, double dt, double dx, double dy, double dz,
+                  RedPartial *partials, size_t cells) {
+  const int grid = (int)std::min<size_t>((cells + 255) / 256, CALMAS_MAX_GROUPS);
+  hipLaunchKernelGGL(k_calmas, dim3(grid), dim3(256), 0, s, fd, u, v, w, su, dt, dx, dy, dz,
+                     partials);
+  return grid;
 }
 
 void launch_pack_fine(hipStream_t s, const FineDev *fd, int which, const double *src, int ni,

@@ -174,6 +174,7 @@ struct h3dmg_handle {
   RedPartial *d_partials = nullptr;
   RedPartial *d_stats = nullptr;
   RedPartial *h_stats = nullptr;  // pinned
+  RedPartial *d_proj = nullptr;   // calmas partial maxima
 
   // exact coarse solve
   double **d_eig_q = nullptr, **d_eig_l = nullptr, **d_scratch = nullptr;
@@ -641,6 +642,8 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   off = align_up(off + std::max<size_t>(1, h->fine_tiles.h.size()) * sizeof(RedPartial), 256);
   const size_t stats_off = off;
   off = align_up(off + nblocks * sizeof(RedPartial), 256);
+  const size_t proj_off = off;
+  off = align_up(off + CALMAS_MAX_GROUPS * sizeof(RedPartial), 256);
   const size_t eptr_off = off;
   off = align_up(off + (size_t)nblocks * 8 * sizeof(double *), 256);
 
@@ -697,6 +700,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
                     hipMemcpyHostToDevice));
   h->d_partials = (RedPartial *)(A + part_off);
   h->d_stats = (RedPartial *)(A + stats_off);
+  h->d_proj = (RedPartial *)(A + proj_off);
   HIP_TRY(hipHostMalloc((void **)&h->h_stats, nblocks * sizeof(RedPartial), hipHostMallocDefault));
   if (exact) {
     std::vector<double *> ptrs((size_t)nblocks * 8);
@@ -1222,6 +1226,40 @@ h3dmg_status h3dmg_set_block_enabled(h3dmg_handle *h, int b, int enabled) {
   h3dmg_status st = check_block(h, b);
   if (st) return st;
   h->blk[b].enabled = enabled != 0;
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_project_device(h3dmg_handle *h, int b, const double *d_ustar,
+                                  const double *d_vstar, const double *d_wstar, double *d_u,
+                                  double *d_v, double *d_w, double *d_su, double alfapr,
+                                  double *rmax) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!d_ustar || !d_vstar || !d_wstar || !d_u || !d_v || !d_w)
+    return fail(H3DMG_ERR_ARG, "null velocity array");
+  const FineDev &F = h->fine[b];
+  const BlockInfo &B = h->blk[b];
+  const double dt = h->prm.dt;
+  if (!(dt > 0)) return fail(H3DMG_ERR_ARG, "projection needs params.dt > 0");
+  const double dta = dt * alfapr;
+  const size_t ni = F.ni, nj = F.nj, nk = F.nk;
+  launch_calvel(h->stream, h->d_fine + b, 0, d_ustar, d_u, dta, B.spec.dx, (ni + 1) * nj * nk);
+  launch_calvel(h->stream, h->d_fine + b, 1, d_vstar, d_v, dta, B.spec.dy, ni * (nj + 1) * nk);
+  launch_calvel(h->stream, h->d_fine + b, 2, d_wstar, d_w, dta, B.spec.dz, ni * nj * (nk + 1));
+  const int groups = launch_calmas(h->stream, h->d_fine + b, d_u, d_v, d_w, d_su, dt, B.spec.dx,
+                                   B.spec.dy, B.spec.dz, h->d_proj, ni * nj * nk);
+  HIP_TRY(hipGetLastError());
+  if (rmax) {
+    std::vector<RedPartial> part(groups);
+    HIP_TRY(hipMemcpyAsync(part.data(), h->d_proj, groups * sizeof(RedPartial),
+                           hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    double m = 0.0;
+    for (const RedPartial &r : part) m = std::max(m, r.amax);
+    *rmax = m;
+  } else {
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
   return H3DMG_OK;
 }
 

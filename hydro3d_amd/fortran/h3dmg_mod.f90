@@ -40,7 +40,8 @@ module h3dmg_mod
        h3dmg_set_rhs, h3dmg_set_p, h3dmg_get_p, h3dmg_vcycle, h3dmg_get_pp, &
        h3dmg_solve, h3dmg_block_stats, h3dmg_remove_mean, h3dmg_set_stream, &
        h3dmg_set_march, h3dmg_get_march, h3dmg_set_tail, h3dmg_get_tail, &
-       h3dmg_set_block_tol, h3dmg_set_block_enabled, h3dmg_error_message
+       h3dmg_set_block_tol, h3dmg_set_block_enabled, h3dmg_project_device, &
+       h3dmg_error_message
 
   interface
      subroutine h3dmg_default_params(p) bind(C, name="h3dmg_default_params")
@@ -141,6 +142,16 @@ module h3dmg_mod
        type(c_ptr), value :: handle
        integer(c_int), value :: b, enabled
      end function h3dmg_set_block_enabled
+
+     integer(c_int) function h3dmg_project_device(handle, b, ustar, vstar, wstar, u, v, w, &
+          su, alfapr, rmax) bind(C, name="h3dmg_project_device")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: handle
+       integer(c_int), value :: b
+       type(c_ptr), value :: ustar, vstar, wstar, u, v, w, su
+       real(c_double), value :: alfapr
+       real(c_double), intent(out) :: rmax
+     end function h3dmg_project_device
 
      integer(c_int) function h3dmg_set_tail(handle, level) bind(C, name="h3dmg_set_tail")
        import :: c_ptr, c_int

@@ -161,6 +161,22 @@ class MultigridSolver:
         L.check(self.lib.h3dmg_block_stats(self.h, cyc, rel))
         return list(cyc), list(rel)
 
+    def project(self, b, ustar, vstar, wstar, alfapr=1.0, want_su=True):
+        """calvel + calmas of block b with its current pressure, on the GPU.
+        ustar (nk, nj, ni+1), vstar (nk, nj+1, ni), wstar (nk+1, nj, ni):
+        device float64 tensors.  Returns (u, v, w, su or None, rmax)."""
+        import torch
+        for t in (ustar, vstar, wstar):
+            assert _is_device_tensor(t) and t.dtype == torch.float64 and t.is_contiguous()
+        u, v, w = torch.empty_like(ustar), torch.empty_like(vstar), torch.empty_like(wstar)
+        su = torch.empty(self._shape(b), dtype=torch.float64, device=ustar.device) if want_su else None
+        rmax = ctypes.c_double(0.0)
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr() if t is not None else 0)  # noqa: E731
+        L.check(self.lib.h3dmg_project_device(self.h, b, ptr(ustar), ptr(vstar), ptr(wstar),
+                                              ptr(u), ptr(v), ptr(w), ptr(su), float(alfapr),
+                                              ctypes.byref(rmax)))
+        return u, v, w, su, rmax.value
+
     def set_block_tol(self, b, tol):
         """REL_L2 tolerance of block b (0: the solver's)."""
         L.check(self.lib.h3dmg_set_block_tol(self.h, b, float(tol)))

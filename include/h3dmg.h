@@ -146,6 +146,19 @@ h3dmg_status h3dmg_get_pp(h3dmg_handle *h, int b, double *pp, int ld1, int ld2, 
 h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycles,
                          double *final_res);
 
+/* Velocity projection of block b with its current pressure, on the device
+ * (calvel, src_v70/press.for:77-144, then calmas, press.for:2-75, 2nd-order
+ * branch): u = u* - dt*alfapr*grad p on the faces the reference corrects
+ * (interior faces and periodic wrap faces; wall-normal faces keep u*),
+ * su = div(u)/dt, and *rmax = max|div(u)| dx dy dz (may be null).  Dense
+ * device arrays, i fastest: u*, u (nk, nj, ni+1) with face f between cells
+ * f-1 and f; v*, v (nk, nj+1, ni); w*, w (nk+1, nj, ni); su (nk, nj, ni) or
+ * null.  dt is params.dt.  Bitwise the reference's arithmetic. */
+h3dmg_status h3dmg_project_device(h3dmg_handle *h, int b, const double *d_ustar,
+                                  const double *d_vstar, const double *d_wstar, double *d_u,
+                                  double *d_v, double *d_w, double *d_su, double alfapr,
+                                  double *rmax);
+
 /* Per-block results of the last solve: cycles run and final relative
  * residual (arrays of nblocks). */
 h3dmg_status h3dmg_block_stats(const h3dmg_handle *h, int *cycles, double *rel_res);

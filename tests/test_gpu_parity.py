@@ -186,6 +186,31 @@ def test_gpu_tail_level_does_not_change_results(gpu_lib, prec):
             assert np.array_equal(ps[b], ref[2][b]), (tail, b)
 
 
+def test_gpu_projection_bitwise_vs_reference(gpu_lib):
+    """calvel + calmas on the device (h3dmg_project_device) reproduce the
+    reference's su bitwise: su0 from p = 0 and the final su from the
+    reference's final pressure (newsolv_ppw32 fixture), and its rmax."""
+    import torch
+    from hydro3d_amd import configs as C
+    z = load_golden("newsolv_ppw32.npz")
+    c = C.cube(32, C.BC_PPN, 3, seed=0)
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    us, vs, ws = _dev(c["u"]), _dev(c["v"]), _dev(c["w"])
+    with _solver(spec, dt=float(z["dt"])) as S:
+        S.set_p(0, np.zeros(c["f"].shape))
+        _, _, _, su0, rmax0 = S.project(0, us, vs, ws)
+        S.set_p(0, z["p"])
+        u, v, w, su, rmax = S.project(0, us, vs, ws)
+    assert np.array_equal(_host(su0), z["su0"])
+    assert np.array_equal(_host(su), z["final_su"])
+    # the driver runs all ncycles cycles: the final p is the last cycle's
+    assert rmax == z["rmax"][-1], (rmax, z["rmax"])
+    assert rmax0 == z["rmax"][0]
+    # periodic x: the wrap face's two copies agree; walls in z keep w* = 0
+    assert torch.equal(u[:, :, 0], u[:, :, -1])
+    assert float(w[0].abs().max()) == 0.0 and float(w[-1].abs().max()) == 0.0
+
+
 def test_gpu_graph_and_eager_identical(gpu_lib):
     from hydro3d_amd import configs as C
     c = C.cube(32, C.BC_PPN, 3, seed=9)
