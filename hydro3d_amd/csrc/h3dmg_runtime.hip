@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,10 @@ using namespace h3dmg;
 namespace {
 
 thread_local std::string g_last_error;
+// Serialises graph capture with the calls that allocate, free or copy
+// synchronously (handle creation and destruction, tile uploads): handles may
+// be driven from several host threads at once.
+std::mutex g_device_mutex;
 
 h3dmg_status fail(h3dmg_status st, const char *fmt, ...) {
   char buf[512];
@@ -848,7 +853,8 @@ void drop_graph(h3dmg_handle *h) {
 h3dmg_status run_cycle(h3dmg_handle *h) {
   if (h->prm.use_graph) {
     if (!h->gexec) {
-      HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      std::lock_guard<std::mutex> lock(g_device_mutex);
+      HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
       enqueue_vcycle(h);
       enqueue_update_residual(h, true);
       hipGraph_t g = nullptr;
@@ -889,6 +895,7 @@ const char *h3dmg_last_error(void) { return g_last_error.c_str(); }
 
 h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_params *prm,
                           h3dmg_handle **out) {
+  std::lock_guard<std::mutex> lock(g_device_mutex);
   if (!out) return fail(H3DMG_ERR_ARG, "null output handle pointer");
   *out = nullptr;
   if (!blocks || nblocks < 1) return fail(H3DMG_ERR_ARG, "no blocks");
@@ -936,6 +943,7 @@ h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_pa
 
 void h3dmg_destroy(h3dmg_handle *h) {
   if (!h) return;
+  std::lock_guard<std::mutex> lock(g_device_mutex);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graph(h);
   for (auto *v : {&h->relax_nc, &h->relax_c})
@@ -1406,6 +1414,7 @@ h3dmg_status h3dmg_dbg_prolong(h3dmg_handle *h, int level) {
 
 h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes) {
   if (!h || level < 1 || level > h->Lmax) return fail(H3DMG_ERR_ARG, "bad level");
+  std::lock_guard<std::mutex> lock(g_device_mutex);
   HIP_TRY(hipStreamSynchronize(h->stream));
   drop_graph(h);
   h->march_kb[level] = planes > 0 ? planes : 0;
@@ -1414,6 +1423,7 @@ h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes) {
 
 h3dmg_status h3dmg_set_tail(h3dmg_handle *h, int level) {
   if (!h || level < 0) return fail(H3DMG_ERR_ARG, "bad tail level");
+  std::lock_guard<std::mutex> lock(g_device_mutex);
   HIP_TRY(hipStreamSynchronize(h->stream));
   drop_graph(h);
   h->tail_req = level;

@@ -29,7 +29,6 @@ import argparse
 import collections
 import json
 import math
-import queue
 import threading
 import time
 from concurrent.futures import Future
@@ -118,12 +117,16 @@ class PoissonServer:
 
     def __init__(self, hbm_budget=64 << 30, max_batches=4, max_batch_blocks=16,
                  precision="mixed", coarse="exact", nu1=2, nu2=2, maxcy=60,
-                 solver_factory=None, keep_resident=True):
+                 solver_factory=None, keep_resident=True, use_graph=False):
         if solver_factory is None:
             from .solver import MultigridSolver as solver_factory  # noqa: N813
         self._factory = solver_factory
         self.precision, self.coarse = precision, coarse
-        self._kw = dict(nu1=nu1, nu2=nu2, maxcy=maxcy, precision=precision, coarse=coarse)
+        # Handles run on several host threads; by default they launch eagerly
+        # (each on its own stream), since a handle's first solve would
+        # otherwise capture its graph while other threads create handles.
+        self._kw = dict(nu1=nu1, nu2=nu2, maxcy=maxcy, precision=precision, coarse=coarse,
+                        use_graph=use_graph)
         self.budget = int(hbm_budget)
         self.max_batches = max_batches
         self.max_batch_blocks = max_batch_blocks
