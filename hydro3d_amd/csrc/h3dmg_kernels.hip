@@ -1037,74 +1037,118 @@ __global__ void __launch_bounds__(256)
 // periodic second difference; with Q_a the orthonormal eigenvectors of D_a
 // (columns) and l_a its eigenvalues, u = Q (Q^T f ./ (lx+ly+lz)) with the
 // zero (constant) mode dropped -- the null-space projection.  One workgroup
-// per block; scratch holds two n^3 arrays.  The body is shared by k_coarse_exact (256 threads) and
-// the V-cycle tail kernel (TAIL_THREADS): every thread of the workgroup calls it.
-__device__ void coarse_exact_body(const LevelDev &L, const double *Q0, const double *Q1,
-                                  const double *Q2, const double *l0, const double *l1,
-                                  const double *l2, double *A, double *Bs, int buf) {
+// per block; scratch holds two n^3 arrays.  The body is shared by
+// k_coarse_exact (arrays in global memory) and the V-cycle tail kernel
+// (arrays in LDS, S = lds address space): the same arithmetic in the same
+// order.  Every thread of the workgroup calls it.
+template <class T>
+using lds = __attribute__((address_space(3))) T;
+
+template <bool IN_LDS>
+struct Space;
+template <>
+struct Space<false> {
+  template <class T>
+  using ptr = T *;
+  template <class T>
+  static __device__ __forceinline__ T *cast(void *p) {
+    return (T *)p;
+  }
+};
+template <>
+struct Space<true> {
+  template <class T>
+  using ptr = lds<T> *;
+  template <class T>
+  static __device__ __forceinline__ lds<T> *cast(void *p) {
+    return (lds<T> *)p;
+  }
+};
+
+__device__ __forceinline__ int cidx32(int H, int nj, int i, int j, int k) {
+  return (k * nj + j) * H + (i >> 1);
+}
+
+template <bool IN_LDS>
+__device__ void coarse_exact_body(const LevelDev &L, const double *Q0g, const double *Q1g,
+                                  const double *Q2g, const double *l0g, const double *l1g,
+                                  const double *l2g, double *Ag, double *Bg, int buf) {
+  typedef Space<IN_LDS> S;
+  typedef typename S::template ptr<double> dp;
+  const dp Q0 = S::template cast<double>((void *)Q0g), Q1 = S::template cast<double>((void *)Q1g),
+           Q2 = S::template cast<double>((void *)Q2g);
+  const dp l0 = S::template cast<double>((void *)l0g), l1 = S::template cast<double>((void *)l1g),
+           l2 = S::template cast<double>((void *)l2g);
+  const dp A = S::template cast<double>(Ag), Bs = S::template cast<double>(Bg);
+  const dp R0 = S::template cast<double>(L.rhs[0]), R1 = S::template cast<double>(L.rhs[1]);
+  const dp P0 = S::template cast<double>(L.phi[buf][0]), P1 = S::template cast<double>(L.phi[buf][1]);
   const int n0 = L.ni, n1 = L.nj, n2 = L.nk;
-  const size_t n = (size_t)n0 * n1 * n2;
+  const int plane = n0 * n1;
+  const int n = plane * n2;
   const int tid = (int)threadIdx.x;
   const int nthr = (int)blockDim.x;
   // gather rhs (colour split) into natural layout A[i + n0*(j + n1*k)]
-  for (size_t q = tid; q < n; q += nthr) {
-    const int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / ((size_t)n0 * n1));
-    A[q] = ((const double *)L.rhs[(i + j + k) & 1])[cidx(L.H, L.nj, i, j, k)];
+  for (int q = tid; q < n; q += nthr) {
+    const int i = q % n0, j = (q / n0) % n1, k = q / plane;
+    const int e = cidx32(L.H, L.nj, i, j, k);
+    A[q] = ((i + j + k) & 1) ? R1[e] : R0[e];
   }
   __syncthreads();
   // forward transforms: B = Q0^T along i, A = Q1^T along j, B = Q2^T along k
-  for (size_t q = tid; q < n; q += nthr) {
-    const int i = (int)(q % n0);
-    const size_t base = q - i;
+  for (int q = tid; q < n; q += nthr) {
+    const int i = q % n0;
+    const int base = q - i;
     double s = 0.0;
-    for (int x = 0; x < n0; ++x) s += Q0[(size_t)x * n0 + i] * A[base + x];
+    for (int x = 0; x < n0; ++x) s += Q0[x * n0 + i] * A[base + x];
     Bs[q] = s;
   }
   __syncthreads();
-  for (size_t q = tid; q < n; q += nthr) {
-    const int j = (int)((q / n0) % n1);
-    const size_t base = q - (size_t)j * n0;
+  for (int q = tid; q < n; q += nthr) {
+    const int j = (q / n0) % n1;
+    const int base = q - j * n0;
     double s = 0.0;
-    for (int y = 0; y < n1; ++y) s += Q1[(size_t)y * n1 + j] * Bs[base + (size_t)y * n0];
+    for (int y = 0; y < n1; ++y) s += Q1[y * n1 + j] * Bs[base + y * n0];
     A[q] = s;
   }
   __syncthreads();
-  for (size_t q = tid; q < n; q += nthr) {
-    const size_t plane = (size_t)n0 * n1;
-    const int k = (int)(q / plane);
-    const size_t base = q - (size_t)k * plane;
+  for (int q = tid; q < n; q += nthr) {
+    const int k = q / plane;
+    const int base = q - k * plane;
     double s = 0.0;
-    for (int z = 0; z < n2; ++z) s += Q2[(size_t)z * n2 + k] * A[base + (size_t)z * plane];
-    const int i = (int)(q % n0), j = (int)((q / n0) % n1);
+    for (int z = 0; z < n2; ++z) s += Q2[z * n2 + k] * A[base + z * plane];
+    const int i = q % n0, j = (q / n0) % n1;
     const double lam = l0[i] + l1[j] + l2[k];
     // mode (0,0,0) is the constant (null-space) mode of every axis
     Bs[q] = ((i | j | k) == 0) ? 0.0 : s / lam;
   }
   __syncthreads();
   // inverse transforms: A = Q2 along k, B = Q1 along j, phi = Q0 along i
-  for (size_t q = tid; q < n; q += nthr) {
-    const size_t plane = (size_t)n0 * n1;
-    const int k = (int)(q / plane);
-    const size_t base = q - (size_t)k * plane;
+  for (int q = tid; q < n; q += nthr) {
+    const int k = q / plane;
+    const int base = q - k * plane;
     double s = 0.0;
-    for (int z = 0; z < n2; ++z) s += Q2[(size_t)k * n2 + z] * Bs[base + (size_t)z * plane];
+    for (int z = 0; z < n2; ++z) s += Q2[k * n2 + z] * Bs[base + z * plane];
     A[q] = s;
   }
   __syncthreads();
-  for (size_t q = tid; q < n; q += nthr) {
-    const int j = (int)((q / n0) % n1);
-    const size_t base = q - (size_t)j * n0;
+  for (int q = tid; q < n; q += nthr) {
+    const int j = (q / n0) % n1;
+    const int base = q - j * n0;
     double s = 0.0;
-    for (int y = 0; y < n1; ++y) s += Q1[(size_t)j * n1 + y] * A[base + (size_t)y * n0];
+    for (int y = 0; y < n1; ++y) s += Q1[j * n1 + y] * A[base + y * n0];
     Bs[q] = s;
   }
   __syncthreads();
-  for (size_t q = tid; q < n; q += nthr) {
-    const int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / ((size_t)n0 * n1));
-    const size_t base = q - i;
+  for (int q = tid; q < n; q += nthr) {
+    const int i = q % n0, j = (q / n0) % n1, k = q / plane;
+    const int base = q - i;
     double s = 0.0;
-    for (int x = 0; x < n0; ++x) s += Q0[(size_t)i * n0 + x] * Bs[base + x];
-    ((double *)L.phi[buf][(i + j + k) & 1])[cidx(L.H, L.nj, i, j, k)] = s;
+    for (int x = 0; x < n0; ++x) s += Q0[i * n0 + x] * Bs[base + x];
+    const int e = cidx32(L.H, L.nj, i, j, k);
+    if ((i + j + k) & 1)
+      P1[e] = s;
+    else
+      P0[e] = s;
   }
   __syncthreads();
 }
@@ -1117,8 +1161,9 @@ __global__ void __launch_bounds__(256)
   const Tile t = tiles[blockIdx.x];  // lev = coarsest descriptor, x0 = block
   if (!active[t.x0]) return;
   const int b = t.x0;
-  coarse_exact_body(levs[t.lev], eig_q[3 * b], eig_q[3 * b + 1], eig_q[3 * b + 2], eig_l[3 * b],
-                    eig_l[3 * b + 1], eig_l[3 * b + 2], scratch[2 * b], scratch[2 * b + 1], buf);
+  coarse_exact_body<false>(levs[t.lev], eig_q[3 * b], eig_q[3 * b + 1], eig_q[3 * b + 2],
+                           eig_l[3 * b], eig_l[3 * b + 1], eig_l[3 * b + 2], scratch[2 * b],
+                           scratch[2 * b + 1], buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -1130,27 +1175,39 @@ __global__ void __launch_bounds__(256)
 // latency and on-chip barriers replace ~3 full-chip launches per level.
 //
 // The tail levels are copied into LDS with their global layout (colour
-// split, pitch H): descriptors whose pointers point into LDS feed the same
-// per-cell functions, so every cell is computed with the same expression,
-// in the same order, as by the per-level kernels (bitwise-identical
-// results).  In: the top level's rhs (global).  Out: the top level's phi in
-// buffer (nu1+nu2)&1 (global), the buffer every level ends in on the host
-// side.  The tile's `lev` is the block's level-Lt descriptor.  Levels are
-// float in a mixed-precision handle except the coarsest (double).
+// split, pitch H); descriptors whose pointers point into LDS feed per-cell
+// functions that compute every cell with the same expression, in the same
+// order, as the per-level kernels (bitwise-identical results).  In: the top
+// level's rhs (global).  Out: the top level's phi in buffer (nu1+nu2)&1
+// (global), the buffer every level ends in on the host side.  The tile's
+// `lev` is the block's level-Lt descriptor.  Levels are float in a
+// mixed-precision handle except the coarsest (double).
+
+// value of cell (x,y,z) in [-1, n] of an LDS-resident level, buffer `buf`,
+// ghosts resolved to their image
+template <class T>
+__device__ __forceinline__ T lds_value(const LevelDev &L, int buf, int x, int y, int z) {
+  x = image(x, L.ni, L.per & 1);
+  y = image(y, L.nj, L.per & 2);
+  z = image(z, L.nk, L.per & 4);
+  const lds<T> *a = (const lds<T> *)L.phi[buf][(x + y + z) & 1];
+  return a[cidx32(L.H, L.nj, x, y, z)];
+}
 
 // neighbour (x,y,z) of a cell of colour `colour` in a half-sweep reading
-// buffer `src`: interior neighbours from the other colour's array, ghosts
-// from their image in buffer src.  Branch-free: one load from a selected
-// array, then the zero-state select.
+// buffer `src`: interior neighbours from the other colour's array `nb_int`,
+// ghosts from their image in buffer src.  One load from a selected array,
+// then the zero-state select.
 template <class T>
-__device__ __forceinline__ T sweep_nb(const LevelDev &L, const T *nb_int, int src, bool zero_int,
-                                      bool zero_ghost, int x, int y, int z) {
+__device__ __forceinline__ T sweep_nb(const LevelDev &L, const lds<T> *nb_int, const lds<T> *g0,
+                                      const lds<T> *g1, bool zero_int, bool zero_ghost, int x,
+                                      int y, int z) {
   const bool ghost =
       (unsigned)x >= (unsigned)L.ni || (unsigned)y >= (unsigned)L.nj || (unsigned)z >= (unsigned)L.nk;
   const int xi = image(x, L.ni, L.per & 1), yi = image(y, L.nj, L.per & 2),
             zi = image(z, L.nk, L.per & 4);
-  const T *a = ghost ? (const T *)L.phi[src][(xi + yi + zi) & 1] : nb_int;
-  const T v = a[cidx(L.H, L.nj, xi, yi, zi)];
+  const lds<T> *a = ghost ? (((xi + yi + zi) & 1) ? g1 : g0) : nb_int;
+  const T v = a[cidx32(L.H, L.nj, xi, yi, zi)];
   return (ghost ? zero_ghost : zero_int) ? T(0) : v;
 }
 
@@ -1160,9 +1217,11 @@ template <class T>
 __device__ __forceinline__ void tail_half_sweep(const LevelDev &L, int colour, int src, int zero) {
   const int H = L.H, nj = L.nj, nk = L.nk, ni = L.ni;
   const int n = H * nj * nk;
-  const T *nb_int = colour == 0 ? (const T *)L.phi[src][1] : (const T *)L.phi[1 - src][0];
-  const T *R = (const T *)L.rhs[colour];
-  T *D = (T *)L.phi[1 - src][colour];
+  const lds<T> *nb_int =
+      colour == 0 ? (const lds<T> *)L.phi[src][1] : (const lds<T> *)L.phi[1 - src][0];
+  const lds<T> *g0 = (const lds<T> *)L.phi[src][0], *g1 = (const lds<T> *)L.phi[src][1];
+  const lds<T> *R = (const lds<T> *)L.rhs[colour];
+  lds<T> *D = (lds<T> *)L.phi[1 - src][colour];
   const bool zero_int = zero && colour == 0, zero_ghost = zero != 0;
   for (int base = (int)threadIdx.x; base < n; base += TAIL_THREADS * TAIL_U) {
     T v[TAIL_U][7];
@@ -1173,17 +1232,18 @@ __device__ __forceinline__ void tail_half_sweep(const LevelDev &L, int colour, i
       const int e0 = base + u * TAIL_THREADS;
       const int e = min(e0, n - 1);  // lanes past the end read a valid entry
       const int m = e % H, r = e / H, j = r % nj, k = r / nj;
-      const int i = min(2 * m + ((colour + j + k) & 1), ni - 1);
-      ok[u] = e0 < n && 2 * m + ((colour + j + k) & 1) < ni;
+      const int i0 = 2 * m + ((colour + j + k) & 1);
+      const int i = min(i0, ni - 1);
+      ok[u] = e0 < n && i0 < ni;
       ci[u] = i;
       cj[u] = j;
       ck[u] = k;
-      v[u][0] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i - 1, j, k);
-      v[u][1] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i + 1, j, k);
-      v[u][2] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i, j - 1, k);
-      v[u][3] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i, j + 1, k);
-      v[u][4] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i, j, k - 1);
-      v[u][5] = sweep_nb<T>(L, nb_int, src, zero_int, zero_ghost, i, j, k + 1);
+      v[u][0] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i - 1, j, k);
+      v[u][1] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i + 1, j, k);
+      v[u][2] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i, j - 1, k);
+      v[u][3] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i, j + 1, k);
+      v[u][4] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i, j, k - 1);
+      v[u][5] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i, j, k + 1);
       v[u][6] = R[e];
     }
 #pragma unroll
@@ -1201,7 +1261,7 @@ __device__ __forceinline__ void tail_half_sweep(const LevelDev &L, int colour, i
 template <class T>
 __device__ __forceinline__ void tail_fill_zero(const LevelDev &L, int buf) {
   const int n = L.H * L.nj * L.nk;
-  T *a = (T *)L.phi[buf][0], *b = (T *)L.phi[buf][1];
+  lds<T> *a = (lds<T> *)L.phi[buf][0], *b = (lds<T> *)L.phi[buf][1];
   for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) {
     a[e] = T(0);
     b[e] = T(0);
@@ -1224,6 +1284,9 @@ __device__ __forceinline__ void tail_relax(const LevelDev &L, int nsweep, int &b
 template <class Tf, class Tc>
 __device__ __forceinline__ void tail_restrict(const LevelDev &F, const LevelDev &C, int src_f) {
   const int n = C.ni * C.nj * C.nk;
+  const lds<Tf> *P0 = (const lds<Tf> *)F.phi[src_f][0], *P1 = (const lds<Tf> *)F.phi[src_f][1];
+  const lds<Tf> *F0 = (const lds<Tf> *)F.rhs[0], *F1 = (const lds<Tf> *)F.rhs[1];
+  lds<Tc> *C0 = (lds<Tc> *)C.rhs[0], *C1 = (lds<Tc> *)C.rhs[1];
   for (int q0 = (int)threadIdx.x; q0 < n; q0 += TAIL_THREADS) {
     const int ic = q0 % C.ni, jc = (q0 / C.ni) % C.nj, kc = q0 / (C.ni * C.nj);
     Tc acc = Tc(0);
@@ -1231,21 +1294,25 @@ __device__ __forceinline__ void tail_restrict(const LevelDev &F, const LevelDev 
     for (int q = 0; q < 8; ++q) {
       const int i = 2 * ic + (q & 1), j = 2 * jc + ((q >> 1) & 1), k = 2 * kc + (q >> 2);
       const Coef<Tf> c = cell_coef<Tf>(F, i, j, k);
-      const Tf W = level_value<Tf>(F, src_f, i - 1, j, k);
-      const Tf E = level_value<Tf>(F, src_f, i + 1, j, k);
-      const Tf S = level_value<Tf>(F, src_f, i, j - 1, k);
-      const Tf N = level_value<Tf>(F, src_f, i, j + 1, k);
-      const Tf B = level_value<Tf>(F, src_f, i, j, k - 1);
-      const Tf Tt = level_value<Tf>(F, src_f, i, j, k + 1);
-      const size_t fq = cidx(F.H, F.nj, i, j, k);
-      const int fc = (i + j + k) & 1;
-      const Tf P = ((const Tf *)F.phi[src_f][fc])[fq];
-      const Tf r = ((const Tf *)F.rhs[fc])[fq];
+      const Tf W = lds_value<Tf>(F, src_f, i - 1, j, k);
+      const Tf E = lds_value<Tf>(F, src_f, i + 1, j, k);
+      const Tf S = lds_value<Tf>(F, src_f, i, j - 1, k);
+      const Tf N = lds_value<Tf>(F, src_f, i, j + 1, k);
+      const Tf B = lds_value<Tf>(F, src_f, i, j, k - 1);
+      const Tf Tt = lds_value<Tf>(F, src_f, i, j, k + 1);
+      const int fq = cidx32(F.H, F.nj, i, j, k);
+      const bool fc = (i + j + k) & 1;
+      const Tf P = fc ? P1[fq] : P0[fq];
+      const Tf r = fc ? F1[fq] : F0[fq];
       const Tf res =
           r - (c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * Tt + c.ap * P);
       acc = q == 0 ? (Tc)res : acc + (Tc)res;
     }
-    ((Tc *)C.rhs[(ic + jc + kc) & 1])[cidx(C.H, C.nj, ic, jc, kc)] = Tc(0.125) * acc;
+    const int cq = cidx32(C.H, C.nj, ic, jc, kc);
+    if ((ic + jc + kc) & 1)
+      C1[cq] = Tc(0.125) * acc;
+    else
+      C0[cq] = Tc(0.125) * acc;
   }
   __syncthreads();
 }
@@ -1258,6 +1325,8 @@ __device__ __forceinline__ void tail_prolong(const LevelDev &F, const LevelDev &
                                              int src_c) {
   const int n = F.ni * F.nj * F.nk;
   const bool px = C.per & 1, py = C.per & 2, pz = C.per & 4;
+  const lds<Tc> *V0 = (const lds<Tc> *)C.phi[src_c][0], *V1 = (const lds<Tc> *)C.phi[src_c][1];
+  lds<Tf> *D0 = (lds<Tf> *)F.phi[src_f][0], *D1 = (lds<Tf> *)F.phi[src_f][1];
   for (int q0 = (int)threadIdx.x; q0 < n; q0 += TAIL_THREADS) {
     const int i = q0 % F.ni, j = (q0 / F.ni) % F.nj, k = q0 / (F.ni * F.nj);
     const int l = i >> 1, e = i & 1, di = 1 - e;
@@ -1272,18 +1341,19 @@ __device__ __forceinline__ void tail_prolong(const LevelDev &F, const LevelDev &
       const int z = image(nn - 1 + ok, C.nk, pz);
       const Tc w = (oi == di ? Tc(3) : Tc(1)) * (oj == dj ? Tc(3) : Tc(1)) *
                    (ok == dk ? Tc(3) : Tc(1));
-      const Tc v = ((const Tc *)C.phi[src_c][(x + y + z) & 1])[cidx(C.H, C.nj, x, y, z)];
+      const int cq = cidx32(C.H, C.nj, x, y, z);
+      const Tc v = ((x + y + z) & 1) ? V1[cq] : V0[cq];
       s = (q == 0) ? w * v : s + w * v;
     }
-    Tf *fa = (Tf *)F.phi[src_f][(i + j + k) & 1];
-    const size_t fq = cidx(F.H, F.nj, i, j, k);
+    lds<Tf> *fa = ((i + j + k) & 1) ? D1 : D0;
+    const int fq = cidx32(F.H, F.nj, i, j, k);
     fa[fq] = fa[fq] + (Tf)(s / Tc(64));
   }
   __syncthreads();
 }
 
-template <class T>
-__device__ __forceinline__ void tail_copy(T *dst, const T *src, int n) {
+template <class D, class S>
+__device__ __forceinline__ void tail_copy(D *dst, const S *src, int n) {
   for (int e = (int)threadIdx.x; e < n; e += TAIL_THREADS) dst[e] = src[e];
 }
 
@@ -1340,8 +1410,8 @@ __global__ void __launch_bounds__(TAIL_THREADS)
       off += (size_t)na[a] * na[a] * 8;
       double *l = (double *)(smem + off);
       off += ((size_t)na[a] * 8 + 15) & ~(size_t)15;
-      tail_copy<double>(q, eig_q[3 * blk + a], na[a] * na[a]);
-      tail_copy<double>(l, eig_l[3 * blk + a], na[a]);
+      tail_copy((lds<double> *)q, eig_q[3 * blk + a], na[a] * na[a]);
+      tail_copy((lds<double> *)l, eig_l[3 * blk + a], na[a]);
       Q[a] = q;
       lam[a] = l;
     }
@@ -1356,9 +1426,9 @@ __global__ void __launch_bounds__(TAIL_THREADS)
     const int n = G.H * G.nj * G.nk;
     for (int c = 0; c < 2; ++c) {
       if (is64(0))
-        tail_copy<double>((double *)lv[0].rhs[c], (const double *)G.rhs[c], n);
+        tail_copy((lds<double> *)lv[0].rhs[c], (const double *)G.rhs[c], n);
       else
-        tail_copy<float>((float *)lv[0].rhs[c], (const float *)G.rhs[c], n);
+        tail_copy((lds<float> *)lv[0].rhs[c], (const float *)G.rhs[c], n);
     }
     __syncthreads();
   }
@@ -1383,7 +1453,7 @@ __global__ void __launch_bounds__(TAIL_THREADS)
   }
   // coarsest level of the block (always FP64)
   if (do_exact) {
-    coarse_exact_body(lv[last], Q[0], Q[1], Q[2], lam[0], lam[1], lam[2], A, Bs, fin);
+    coarse_exact_body<true>(lv[last], Q[0], Q[1], Q[2], lam[0], lam[1], lam[2], A, Bs, fin);
   } else {
     int buf = 0;
     bool zero = true;
@@ -1415,9 +1485,9 @@ __global__ void __launch_bounds__(TAIL_THREADS)
     const int n = G.H * G.nj * G.nk;
     for (int c = 0; c < 2; ++c) {
       if (is64(0))
-        tail_copy<double>((double *)G.phi[fin][c], (const double *)lv[0].phi[fin][c], n);
+        tail_copy((double *)G.phi[fin][c], (const lds<double> *)lv[0].phi[fin][c], n);
       else
-        tail_copy<float>((float *)G.phi[fin][c], (const float *)lv[0].phi[fin][c], n);
+        tail_copy((float *)G.phi[fin][c], (const lds<float> *)lv[0].phi[fin][c], n);
     }
   }
 }
