@@ -187,8 +187,11 @@ def serving(nreq):
     with SV.PoissonServer(hbm_budget=64 << 30, max_batches=4) as srv:
         res, wall = SV.run_stream(srv, reqs)
         out["served_mixed"] = SV.summarize(res, wall)
+    # the reference schedule (coarsest level smoothed) needs many cycles when
+    # the level count is capped by the power of two dividing the size
     with SV.PoissonServer(hbm_budget=64 << 30, max_batches=1, max_batch_blocks=1,
-                          precision="fp64", coarse="smooth", keep_resident=False) as srv:
+                          precision="fp64", coarse="smooth", keep_resident=False,
+                          maxcy=400) as srv:
         res, wall = SV.run_stream(srv, reqs)
         out["sequential_fp64"] = SV.summarize(res, wall)
     out["workload"] = f"{nreq} requests of the seeded stream (seed 4, 200/s)"

@@ -311,8 +311,12 @@ __device__ __forceinline__ void march_tile_inner(const Tile &t, const LevelDev &
     for (int p = 0; p < KB; ++p) {
       S2[p] = ld2(O, q0 + (unsigned)p * plane - row);
       N2[p] = ld2(O, q0 + (unsigned)p * plane + row);
-      lf_e[p] = ld1(O, off_l + (unsigned)p * plane);
-      rt_e[p] = ld1(O, off_r + (unsigned)p * plane);
+    }
+    // tile-uniform: tiles spanning whole rows need no edge entries
+#pragma unroll
+    for (int p = 0; p < KB; ++p) {
+      lf_e[p] = tile_left ? ld1(O, off_l + (unsigned)p * plane) : T(0);
+      rt_e[p] = tile_right ? ld1(O, off_r + (unsigned)p * plane) : T(0);
     }
   } else {
 #pragma unroll
@@ -954,7 +958,9 @@ __global__ void __launch_bounds__(NTHREADS)
     k_residual(const Tile *__restrict__ tiles, const FineDev *__restrict__ fine,
                const LevelDev *__restrict__ levs, const int *__restrict__ active,
                RedPartial *__restrict__ partials, int use_p) {
-  const Tile t = tiles[blockIdx.x];
+  // XCD-contiguous tile ranges: rows j-1, j+1 of a tile are in the same L2
+  const int ti = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const Tile t = tiles[ti];
   const FineDev &F = fine[t.lev];
   const LevelDev &L = levs[t.lev];
   double ss = 0.0, am = 0.0;
@@ -971,7 +977,7 @@ __global__ void __launch_bounds__(NTHREADS)
       residual_tile<T, false>(t, F, L, use_p, ss, am);
   }
   block_reduce(ss, am);
-  if (threadIdx.x == 0 && threadIdx.y == 0) partials[blockIdx.x] = RedPartial{ss, am};
+  if (threadIdx.x == 0 && threadIdx.y == 0) partials[ti] = RedPartial{ss, am};
 }
 
 // p = p + (pp - ppref) with ppref = 0 (newsolv_mg.for:39-49), streamed over

@@ -324,7 +324,9 @@ class PoissonServer:
 def request_stream(nreq, rate=200.0, seed=4, reuse_frac=0.2, lo=32, hi=256, quantum=16):
     """Arrival offsets (s) and request descriptors: exponential gaps of mean
     1/rate; per-axis sizes exp(U(ln lo, ln hi)) rounded to a multiple of
-    `quantum`; per-axis Neumann or periodic by seed; tolerance 1e-6 or 1e-8;
+    `quantum`, cubic cells of side 1/64 (the spacing of the single-block and
+    batched workloads); per-axis Neumann or periodic by seed; tolerance 1e-6
+    or 1e-8;
     a fraction `reuse_frac` repeats an earlier request's key (same geometry,
     next right-hand side of that simulation, warm start)."""
     rng = np.random.default_rng(seed)
@@ -341,7 +343,7 @@ def request_stream(nreq, rate=200.0, seed=4, reuse_frac=0.2, lo=32, hi=256, quan
                           * quantum)) for _ in range(3))
         per = [bool(rng.random() < 0.5) for _ in range(3)]
         bc = tuple(x for a in range(3) for x in ((1, 1) if per[a] else (0, 0)))
-        out.append(dict(rid=i, t=float(t[i]), n=n, bc=bc, h=tuple(1.0 / x for x in n),
+        out.append(dict(rid=i, t=float(t[i]), n=n, bc=bc, h=(1.0 / 64,) * 3,
                         tol=float(rng.choice([1e-6, 1e-8])), key=i, step=0,
                         seed=int(rng.integers(1 << 31))))
     return out
@@ -399,7 +401,9 @@ def run_stream(server, reqs, realtime=True, progress=None):
 def summarize(results, wall):
     lat = np.array([r.latency_s for r in results])
     ok = sum(1 for r in results if r.status == 0)
-    return {"requests": len(results), "converged": ok, "wall_s": round(wall, 3),
+    status = collections.Counter(int(r.status) for r in results)
+    return {"requests": len(results), "converged": ok, "status_counts": dict(status),
+            "wall_s": round(wall, 3),
             "solves_per_s": round(len(results) / wall, 2),
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                            "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
@@ -428,7 +432,8 @@ def main(argv=None) -> int:
         out["served_mixed"]["server"] = dict(srv.stats)
     if not a.no_baseline:
         with PoissonServer(hbm_budget=int(a.budget_gb * 2**30), max_batches=1, max_batch_blocks=1,
-                           precision="fp64", coarse="smooth", keep_resident=False) as srv:
+                           precision="fp64", coarse="smooth", keep_resident=False,
+                           maxcy=400) as srv:
             res, wall = run_stream(srv, reqs, progress=500)
             out["sequential_fp64"] = summarize(res, wall)
     print(json.dumps(out))

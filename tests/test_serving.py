@@ -179,6 +179,6 @@ def test_footprint_covers_the_device_arena():
     for specs in shapes:
         for prec, coarse in (("mixed", "exact"), ("fp64", "smooth")):
             with MultigridSolver(specs, precision=prec, coarse=coarse) as S:
-                dev = S.device_bytes()
+                dev = S.device_bytes
             est = SV.footprint_bytes(specs, prec, coarse)
             assert dev <= est <= 1.5 * dev + (4 << 20), (specs, prec, dev, est)
