@@ -905,8 +905,13 @@ __device__ __forceinline__ void residual_tile_inner(const Tile &t, const FineDev
     SB[p] = ldp(ca, q - row);
     NA[p] = ldp(cb, q + row);
     NB[p] = ldp(ca, q + row);
-    WE[p] = ldp((w_par + jc + k) & 1, (unsigned)k * plane + w_ent);
-    EE[p] = ldp((e_par + jc + k) & 1, (unsigned)k * plane + e_ent);
+  }
+  // edge values: only the (at most two) lanes of a row that need them load
+#pragma unroll
+  for (int p = 0; p < KB; ++p) {
+    const int k = k0 + p;
+    WE[p] = w_edge ? ldp((w_par + jc + k) & 1, (unsigned)k * plane + w_ent) : 0.0;
+    EE[p] = e_edge ? ldp((e_par + jc + k) & 1, (unsigned)k * plane + e_ent) : 0.0;
   }
   Coef<double> c_a, c_b;
   c_a.as = c_a.an = c_b.as = c_b.an = cy;
@@ -965,12 +970,8 @@ __global__ void __launch_bounds__(NTHREADS)
   const LevelDev &L = levs[t.lev];
   double ss = 0.0, am = 0.0;
   if (active[F.blk]) {
-    if (t.kind == 0 && use_p && t.z1 - t.z0 == 4)
-      residual_tile_inner<T, 4>(t, F, L, ss, am);
-    else if (t.kind == 0 && use_p && t.z1 - t.z0 == 2)
+    if (t.kind == 0 && use_p && t.z1 - t.z0 == 2)
       residual_tile_inner<T, 2>(t, F, L, ss, am);
-    else if (t.kind == 0 && use_p && t.z1 - t.z0 == 1)
-      residual_tile_inner<T, 1>(t, F, L, ss, am);
     else if (t.kind == 0)
       residual_tile<T, true>(t, F, L, use_p, ss, am);
     else
@@ -1013,21 +1014,22 @@ __global__ void __launch_bounds__(NTHREADS)
 }
 
 // per-block reduction of the partials of k_residual, fixed order
-__global__ void __launch_bounds__(256)
+constexpr int FIN_THREADS = 1024;
+__global__ void __launch_bounds__(FIN_THREADS)
     k_finalize(const int *__restrict__ tile_begin, const RedPartial *__restrict__ partials,
                RedPartial *__restrict__ out) {
   const int b = blockIdx.x;
   const int t0 = tile_begin[b], t1 = tile_begin[b + 1];
   double ss = 0.0, am = 0.0;
-  for (int q = t0 + (int)threadIdx.x; q < t1; q += 256) {
+  for (int q = t0 + (int)threadIdx.x; q < t1; q += FIN_THREADS) {
     ss += partials[q].ss;
     am = fmax(am, partials[q].amax);
   }
-  __shared__ double s_ss[256], s_am[256];
+  __shared__ double s_ss[FIN_THREADS], s_am[FIN_THREADS];
   s_ss[threadIdx.x] = ss;
   s_am[threadIdx.x] = am;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = FIN_THREADS / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       s_ss[threadIdx.x] += s_ss[threadIdx.x + w];
       s_am[threadIdx.x] = fmax(s_am[threadIdx.x], s_am[threadIdx.x + w]);
@@ -1705,9 +1707,9 @@ void launch_update(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *
                      buf);
 }
 
-void launch_finalize(hipStream_t s, const int *tile_begin, int nblocks,
-                     const RedPartial *partials, RedPartial *out) {
-  hipLaunchKernelGGL(k_finalize, dim3(nblocks), dim3(256), 0, s, tile_begin, partials, out);
+void launch_finalize(hipStream_t s, const int *tile_begin, int nblocks, const RedPartial *partials,
+                     RedPartial *out) {
+  hipLaunchKernelGGL(k_finalize, dim3(nblocks), dim3(FIN_THREADS), 0, s, tile_begin, partials, out);
 }
 
 void launch_coarse_exact(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,

@@ -605,10 +605,10 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     const int lpr = lanes_per_row(2 * L.H);
     const int xs = lpr, ys = TILE_Y * (64 / lpr);
     const int txy = ((L.H + xs - 1) / xs) * ((L.nj + ys - 1) / ys);
-    // at most 4 planes (a power of two): the interior tiles' loads all go
-    // out at once, which bounds the depth by the registers they occupy
-    int kb = std::min(4, march_depth(txy, L.nk, 2048));
-    if (kb == 3) kb = 2;
+    // 2-plane tiles: the interior tiles issue all their loads at once; at
+    // two planes the kernel stays at 5 waves per SIMD (deeper tiles cost
+    // occupancy, single planes re-read the plane window too often)
+    const int kb = std::min(2, march_depth(txy, L.nk, 2048));
     for (int z = 0; z < L.nk; z += kb) {
       const int z1 = std::min(z + kb, L.nk);
       for (int y = 0; y < L.nj; y += ys)
@@ -816,9 +816,16 @@ h3dmg_status upload_tail(h3dmg_handle *h) {
     return fail(H3DMG_ERR_ARG, "tail level %d needs %zu bytes of LDS (at most %zu here)", lt,
                 tail_lds_at(h, lt), h->lds_budget);
   }
-  // the permitted LDS is a process-wide attribute of the kernel: always the
-  // device's whole budget, so that handles of different tail sizes coexist
-  if (h->tail_lds > 65536) HIP_TRY(vtail_allow_lds(h->lds_budget));
+  // The permitted LDS is a process-wide attribute of the kernel: set once,
+  // to the device's whole budget, so that handles of different tail sizes
+  // coexist and no later handle changes it while another thread launches.
+  if (h->tail_lds > 65536) {
+    static std::once_flag once;
+    static hipError_t once_err = hipSuccess;
+    const size_t budget = h->lds_budget;
+    std::call_once(once, [budget]() { once_err = vtail_allow_lds(budget); });
+    HIP_TRY(once_err);
+  }
   TileSet &ts = h->tail_tiles;
   ts.h.clear();
   if (h->tail_lt <= h->Lmax)
