@@ -177,6 +177,26 @@ def batched_blocks(steps, warmup, tune_cache):
             "worst_rel_residual": float(max(rel)), "tuning": choice}
 
 
+def single_64(steps, warmup):
+    """The 64^3 single block (periodic x/y, Neumann z, 4 levels, seed 0):
+    latency-bound at this size (39 MB per V-cycle fits the Infinity Cache),
+    so it is reported as latency for both arithmetics."""
+    import torch
+    from hydro3d_amd import MultigridSolver, configs
+    c = configs.single_64()
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    f = torch.from_numpy(c["f"]).cuda()
+    z = torch.zeros_like(f)
+    out = {"workload": "single block 64^3 (periodic x/y, Neumann z), 4 levels, seed 0, to 1e-8"}
+    for name, prec, coarse in (("fp64", "fp64", "smooth"), ("mixed", "mixed", "exact")):
+        with MultigridSolver([spec], tol=1e-8, precision=prec, coarse=coarse) as S:
+            lat, cyc, fin = timed_solves(S, f, z, steps, warmup)
+        out[name] = {"ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                     "ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                     "cycles": cyc, "final_rel_residual": fin}
+    return out
+
+
 def serving(nreq):
     """The serving stream (seeded Poisson arrivals at 200/s, sizes 32..256,
     tolerances 1e-6/1e-8, 20% warm reuse): continuous-batching mixed server
@@ -319,6 +339,9 @@ def main():
     ms64, bytes64, ach64 = smoother_roofline(S64, N, 8)
     log(f"smoother pass (fp64, 256^3): {ms64*1e3:.1f} us, {ach64:.0f} GB/s")
 
+    log("single 64^3")
+    small = single_64(args.steps, args.warmup)
+    log(small)
     batched = None
     if not args.no_batched:
         log("batched 48 blocks")
@@ -377,6 +400,7 @@ def main():
                      "fp64_pass": {"achieved": round(ach64, 1), "avg_launch_us": round(ms64 * 1e3, 2),
                                    "frac": round(ach64 / HBM_PEAK_GBS, 4)}},
         "cpu_baseline": cpu,
+        "single_64": small,
         "batched_uneven_48": batched,
         "serving": serve,
     }

@@ -104,6 +104,31 @@ def test_gpu_solve_tracks_reference_velocity_form(gpu_lib):
     assert np.allclose(hist * fn, ho, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("name", ["newsolv_ppw32.npz", "newsolv_ppw64.npz"])
+def test_gpu_reference_stopping_rule(gpu_lib, name):
+    """H3DMG_CRIT_REF_RMAX is the reference's rule (rmax < eps and iter > 1,
+    newsolv_mg.for:57, rmax = max|div u| dx dy dz from calmas): from the
+    reference's su0 the device stops at the cycle the reference stops at, and
+    its rmax per cycle is the reference's to 1e-10 relative."""
+    from hydro3d_amd import configs as C
+    z = load_golden(name)
+    spec = _blockspec(z)
+    if "su0" in z:
+        f = z["su0"]
+    else:  # rebuild the 64^3 workload from its recipe (no mean removal: su0)
+        f = C.divergence_rhs(*C.face_velocity(spec["n"], spec["bc"], 0, 0.0, spec["h"]),
+                             spec["h"], float(z["dt"]), remove_mean=False)
+    eps = 1e-6  # the eps the fixtures were made with
+    stop = int(z["stop_at"])
+    with _solver(spec, maxcy=int(z["ncycles"]), tol=eps, criterion="ref_rmax",
+                 dt=float(z["dt"]), min_cycles=2) as S:
+        S.set_rhs(0, f)
+        st, ran, hist = S.solve(raise_on_fail=True)
+    assert ran == stop, (ran, stop)
+    ref = z["rmax"][:stop + 1]
+    assert np.all(np.abs(hist - ref) <= 1e-10 * ref), (hist, ref)
+
+
 def test_gpu_single_64_to_tolerance(gpu_lib):
     """64^3, periodic x/y, Neumann z, 4 levels, V(2,2), to 1e-8: same cycle
     count and bitwise the same pressure as the oracle."""
