@@ -4,8 +4,10 @@ Two knobs change only how the work is launched, never the results (every
 cell is computed by the same arithmetic whatever the choice):
 
 * march depth per level -- planes each smoother workgroup walks
-  (h3dmg_set_march): deeper columns reuse more of the register window,
-  shallower ones spread a small level over more workgroups;
+  (h3dmg_set_march: 1, 2 or 4): deeper tiles reuse more of the register
+  window and keep more loads in flight per wave, shallower ones need fewer
+  registers (more waves per SIMD) and spread a small level over more
+  workgroups;
 * tail level -- the level from which one workgroup per block runs the rest
   of the V-cycle in a single launch (h3dmg_set_tail): below some size a level
   is launch-latency-bound and on-chip barriers beat full-chip launches.
@@ -30,7 +32,7 @@ from . import _lib as L
 DEFAULT_CACHE = os.environ.get("H3DMG_TUNE_CACHE",
                                os.path.join(os.path.expanduser("~"), ".cache", "h3dmg",
                                             "autotune.json"))
-MARCH_CANDIDATES = (1, 2, 4, 8, 16)
+MARCH_CANDIDATES = (1, 2, 4)
 TAIL_MAX_CELLS = 64 ** 3  # levels larger than this never go to the one-workgroup tail
 
 

@@ -73,10 +73,12 @@ struct RedPartial {
 
 // ---- launchers (h3dmg_kernels.hip) --------------------------------------
 // T = storage type of the level (double or float).
-// one colour half-sweep of the red-black smoother over a tile list of
-// kinds 0 (interior march), 1 (boundary march) and 2 (x faces)
+// one colour half-sweep of the red-black smoother over a tile list whose
+// tiles are all `depth` (1, 2 or 4) planes deep: kinds 0 (march tile, y/z
+// interior), 3 (march tile touching a y/z face), or, on levels one cell wide
+// in x (depth 1 only), 1 (general boundary march) and 2 (x faces)
 template <class T>
-void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
                      const int *active, int colour, int src, int zero);
 template <class Tf, class Tc>
 void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,

@@ -264,101 +264,193 @@ __device__ __forceinline__ void march_tile(const Tile &t, const LevelDev &L, int
   }
 }
 
-// Interior march tile of exactly KB planes with every load issued up front:
-// the own-column window (KB+2 planes), rows j-1 and j+1 and the rhs of all KB
-// planes, then the KB updates.  One memory round trip per tile instead of
-// one or more per plane; the arithmetic is march_tile<T, true>'s.
-template <class T, int KB, bool ZERO_INT>
-__device__ __forceinline__ void march_tile_inner(const Tile &t, const LevelDev &L, int colour,
-                                                 int src) {
+// March tile of KB planes with every load issued up front (one memory round
+// trip per tile): the own-column window (KB+2 planes), rows j-1 and j+1, the
+// rhs and one x-slot entry per plane, then the KB updates, then the stores.
+//
+// The tile also updates the cells on its block's faces.  A ghost neighbour
+// takes the pre-sweep (buffer src) value of its image -- the cell on the
+// opposite face across a periodic face, the cell itself across a wall (whose
+// coefficient is 0) -- which is the exchange/boundary state the reference's
+// ghosts hold during a sweep.  Every array of a level lies in one arena
+// allocation, so each load is the level's base (scalar) plus a 32-bit lane
+// offset that may select any of the arrays: the image rows and entries of
+// face lanes cost no branch.
+//   x faces: lane 0 of a row segment (entry m0 = 0) holds cell 0, the lane
+//     with m0 = H-2 holds cell ni-1; their x-slot loads the x-ghost image.
+//   y/z faces (BND tiles only): rows 0 and nj-1 read their image rows;
+//     chunks at plane 0 or nk-1 read the image plane into the window.
+// The arithmetic of every cell is cell_coef's and the sum of mgsolver.for's
+// relaxation, as in march_tile.
+template <class T, int KB, bool ZERO_INT, bool BND>
+__device__ __forceinline__ void march_tile_fast(const Tile &t, const LevelDev &L, int colour,
+                                                int src, int zero) {
   typedef typename Vec2<T>::type v2;
+  constexpr unsigned SZ = sizeof(T);
   const int lpr = L.lpr;
   const int lane = (int)threadIdx.x;
   const int rl = lane & (lpr - 1);
   const int m0 = t.x0 + 2 * rl;
   const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
-  const int ni = L.ni, nj = L.nj, H = L.H;
+  const int ni = L.ni, nj = L.nj, nk = L.nk, H = L.H;
   const bool col_ok = m0 < H && j < nj;
-  const unsigned row = (unsigned)H, plane = (unsigned)H * (unsigned)nj;
-  const gmem<T> *O = (const gmem<T> *)(colour == 0 ? L.phi[src][1] : L.phi[1 - src][0]);
-  const gmem<T> *R = (const gmem<T> *)L.rhs[colour];
-  gmem<T> *D = (gmem<T> *)L.phi[1 - src][colour];
-  const unsigned cbase = (unsigned)min(j, nj - 1) * row + (unsigned)min(m0, H - 2);
-  const v2 vz = {T(0), T(0)};
+  const int jc = min(j, nj - 1);
+  const unsigned row = (unsigned)H * SZ, plane = row * (unsigned)nj;  // bytes
+  // byte offsets of the level's arrays from its arena base (phi[0][0])
+  const char *base = (const char *)L.phi[0][0];
+  auto aoff = [&](const void *a) -> unsigned { return (unsigned)((const char *)a - base); };
+  const unsigned aO = aoff(colour == 0 ? L.phi[src][1] : L.phi[1 - src][0]);
+  const unsigned aOs = aoff(L.phi[src][1 - colour]);  // other colour, pre-sweep
+  const unsigned aPs = aoff(L.phi[src][colour]);      // own colour, pre-sweep
+  const unsigned aR = aoff(L.rhs[colour]);
+  const unsigned aD = aoff(L.phi[1 - src][colour]);
+  const gmem<char> *Bs = (const gmem<char> *)base;
+  gmem<char> *Bd = (gmem<char> *)base;
+  const bool px = L.per & 1, py = L.per & 2, pz = L.per & 4;
+  const bool zg = zero != 0;  // ghosts read the all-zero pre-sweep state
+  // the image of a ghost across a periodic face of extent n has the other
+  // colour when n is even; a wall's image (the cell itself) has its own
+  const unsigned aGX = (px && !(ni & 1)) ? aOs : aPs;
+  const unsigned aGY = (py && !(nj & 1)) ? aOs : aPs;
+  const unsigned aGZ = (pz && !(nk & 1)) ? aOs : aPs;
+  const unsigned jrow = (unsigned)jc * row;
+  const unsigned cell = jrow + (unsigned)min(m0, H - 2) * SZ;
+  const int me = (ni - 1) >> 1;  // entry of cell ni-1; its pair starts at H-2
   const bool tile_left = t.x0 > 0, tile_right = t.x0 + 2 * lpr < H;
+
+  // x slots: entry m0-1 for lane 0 of a row segment (neighbouring tile or the
+  // west ghost image), entry m0+2 for the last lane (neighbouring tile) or
+  // the east ghost image for the lane of cell ni-1; one load serves both
+  // unless a lane has both roles (one-lane rows, a last tile of one pair)
+  const bool role_r = tile_right ? rl == lpr - 1 : m0 == H - 2;
+  const unsigned off_l = tile_left ? aO + cell - SZ : aGX + jrow + (unsigned)(px ? me : 0) * SZ;
+  const unsigned off_r = tile_right ? aO + cell + 2 * SZ : aGX + jrow + (unsigned)(px ? 0 : me) * SZ;
+  const bool dual = lpr == 1 || (tile_left && t.x0 == H - 2);
+  const unsigned off_x = rl == 0 ? off_l : (role_r ? off_r : aO + cell);
+  const unsigned off_x2 = role_r ? off_r : aO + cell;
+  const bool gx_l = rl == 0 && !tile_left, gx_r = role_r && !tile_right;
+  const bool zx_1 = zg && (rl == 0 ? gx_l : gx_r), zx_2 = zg && gx_r;
+
+  // y rows: image rows on the y faces of BND tiles
+  unsigned off_s = aO + cell - row, off_n = aO + cell + row;
+  bool zs = false, zn = false;
+  T as = (T)L.cy, an = (T)L.cy;
+  if (BND) {
+    if (jc == 0) {
+      off_s = py ? aGY + cell + (unsigned)(nj - 1) * row : aPs + cell;
+      zs = zg;
+      if (!py) as = T(0);
+    }
+    if (jc == nj - 1) {
+      off_n = py ? aGY + cell - (unsigned)(nj - 1) * row : aPs + cell;
+      zn = zg;
+      if (!py) an = T(0);
+    }
+  }
   const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz;
   const T ap_in = T(-1) * (cx + cx + cy + cy + cz + cz);
   const T inv_in = T(1) / ap_in;
-  auto ld2 = [&](const gmem<T> *a, unsigned off) -> v2 {
-    return *(const gmem<v2> *)((const gmem<char> *)a + off * (unsigned)sizeof(T));
-  };
-  auto ld1 = [&](const gmem<T> *a, unsigned off) -> T {
-    return *(const gmem<T> *)((const gmem<char> *)a + off * (unsigned)sizeof(T));
-  };
-  const int k0 = t.z0;
-  const unsigned q0 = (unsigned)k0 * plane + cbase;
-  v2 own[KB + 2], S2[KB], N2[KB], R2[KB];
-  T lf_e[KB], rt_e[KB];
-  // entries m0-1 and m0+2 of tiles that do not start/end a row; the other
-  // lanes load their own entry instead (no branch between loads)
-  const bool need_l = tile_left && rl == 0;
-  const bool need_r = tile_right && rl == lpr - 1;
-  const unsigned off_l = need_l ? q0 - 1 : q0, off_r = need_r ? q0 + 2 : q0;
-  if (!ZERO_INT) {
+  const T ap_x1 = T(-1) * (cx + cy + cy + cz + cz);  // one x coefficient 0 (wall face)
+  const T inv_x1 = T(1) / ap_x1;
+  const v2 vz = {T(0), T(0)};
+
+  auto ld2 = [&](unsigned off) -> v2 { return *(const gmem<v2> *)(Bs + off); };
+  auto ld1 = [&](unsigned off) -> T { return *(const gmem<T> *)(Bs + off); };
+
+  {
+    const int k0 = t.z0;
+    const unsigned p0 = (unsigned)k0 * plane;
+    v2 own[KB + 2], S2[KB], N2[KB], R2[KB];
+    T x1[KB], x2[KB];
+    if (!ZERO_INT) {
 #pragma unroll
-    for (int p = 0; p < KB + 2; ++p) own[p] = ld2(O, q0 + (unsigned)(p - 1) * plane);
+      for (int p = 0; p < KB + 2; ++p) {
+        const int k = k0 - 1 + p;
+        if (!BND || (k >= 0 && k < nk))
+          own[p] = ld2(aO + cell + (unsigned)k * plane);
+        else if (zg)
+          own[p] = vz;
+        else if (k < 0)
+          own[p] = ld2((pz ? aGZ + (unsigned)(nk - 1) * plane : aPs) + cell);
+        else
+          own[p] = ld2((pz ? aGZ : aPs + (unsigned)(nk - 1) * plane) + cell);
+      }
 #pragma unroll
-    for (int p = 0; p < KB; ++p) {
-      S2[p] = ld2(O, q0 + (unsigned)p * plane - row);
-      N2[p] = ld2(O, q0 + (unsigned)p * plane + row);
-    }
-    // tile-uniform: tiles spanning whole rows need no edge entries
+      for (int p = 0; p < KB; ++p) {
+        const unsigned pp = p0 + (unsigned)p * plane;
+        S2[p] = ld2(off_s + pp);
+        N2[p] = ld2(off_n + pp);
+        x1[p] = ld1(off_x + pp);
+        x2[p] = dual ? ld1(off_x2 + pp) : T(0);
+      }
 #pragma unroll
-    for (int p = 0; p < KB; ++p) {
-      lf_e[p] = tile_left ? ld1(O, off_l + (unsigned)p * plane) : T(0);
-      rt_e[p] = tile_right ? ld1(O, off_r + (unsigned)p * plane) : T(0);
-    }
-  } else {
-#pragma unroll
-    for (int p = 0; p < KB + 2; ++p) own[p] = vz;
-#pragma unroll
-    for (int p = 0; p < KB; ++p) {
-      S2[p] = N2[p] = vz;
-      lf_e[p] = rt_e[p] = T(0);
-    }
-  }
-#pragma unroll
-  for (int p = 0; p < KB; ++p) R2[p] = ld2(R, q0 + (unsigned)p * plane);
-#pragma unroll
-  for (int p = 0; p < KB; ++p) {
-    const int k = k0 + p;
-    const v2 o_m = own[p], o_c = own[p + 1], o_p = own[p + 2];
-    T lf = wave_shift(o_c.y, 1);
-    T rt = wave_shift(o_c.x, 0);
-    if (need_l) lf = lf_e[p];
-    if (need_r) rt = rt_e[p];
-    const int s = (colour + j + k) & 1;
-    const T W0 = s ? o_c.x : lf;
-    const T E0 = s ? o_c.y : o_c.x;
-    const T W1 = s ? o_c.y : o_c.x;
-    const T E1 = s ? rt : o_c.y;
-    const T sum0 = cx * W0 + cx * E0 + cy * S2[p].x + cy * N2[p].x + cz * o_m.x + cz * o_p.x;
-    const T sum1 = cx * W1 + cx * E1 + cy * S2[p].y + cy * N2[p].y + cz * o_m.y + cz * o_p.y;
-    // the updates replace the rhs values (all stores after all loads)
-    R2[p].x = div_ap(R2[p].x - sum0, ap_in, inv_in);
-    R2[p].y = div_ap(R2[p].y - sum1, ap_in, inv_in);
-  }
-#pragma unroll
-  for (int p = 0; p < KB; ++p) {
-    const int s = (colour + j + k0 + p) & 1;
-    const int i0 = 2 * m0 + s, i1 = i0 + 2;
-    const bool w0 = col_ok && i0 > 0 && i0 < ni - 1, w1 = col_ok && i1 < ni - 1;
-    gmem<char> *dq = (gmem<char> *)D + (q0 + (unsigned)p * plane) * (unsigned)sizeof(T);
-    if (w0 && w1) {
-      *(gmem<v2> *)dq = R2[p];
+      for (int p = 0; p < KB; ++p) {
+        if (BND && zs) S2[p] = vz;
+        if (BND && zn) N2[p] = vz;
+        if (zx_1) x1[p] = T(0);
+        if (zx_2) x2[p] = T(0);
+      }
     } else {
-      if (w0) *(gmem<T> *)dq = R2[p].x;
-      if (w1) *(gmem<T> *)(dq + sizeof(T)) = R2[p].y;
+#pragma unroll
+      for (int p = 0; p < KB + 2; ++p) own[p] = vz;
+#pragma unroll
+      for (int p = 0; p < KB; ++p) {
+        S2[p] = N2[p] = vz;
+        x1[p] = x2[p] = T(0);
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < KB; ++p) R2[p] = ld2(aR + cell + p0 + (unsigned)p * plane);
+#pragma unroll
+    for (int p = 0; p < KB; ++p) {
+      const int k = k0 + p;
+      const v2 o_m = own[p], o_c = own[p + 1], o_p = own[p + 2];
+      T lf = wave_shift(o_c.y, 1);  // entry m0-1: lane-1's second entry
+      T rt = wave_shift(o_c.x, 0);  // entry m0+2: lane+1's first entry
+      const T gr = dual && rl == 0 ? x2[p] : x1[p];
+      if (rl == 0) lf = x1[p];
+      if (role_r) rt = gr;
+      const int s = (colour + j + k) & 1;
+      const int i0 = 2 * m0 + s, i1 = i0 + 2;
+      const bool fw0 = m0 == 0 && s == 0;  // cell 0
+      const bool fe0 = i0 == ni - 1, fe1 = i1 == ni - 1;
+      const T W0 = s ? o_c.x : lf;
+      const T E0 = fe0 ? gr : (s ? o_c.y : o_c.x);
+      const T W1 = s ? o_c.y : o_c.x;
+      const T E1 = fe1 ? gr : (s ? rt : o_c.y);
+      const T aw0 = (!px && fw0) ? T(0) : cx;
+      const T ae0 = (!px && fe0) ? T(0) : cx;
+      const T ae1 = (!px && fe1) ? T(0) : cx;
+      T ab = cz, at = cz;
+      if (BND) {
+        if (!pz && k == 0) ab = T(0);
+        if (!pz && k == nk - 1) at = T(0);
+      }
+      const T sum0 = aw0 * W0 + ae0 * E0 + as * S2[p].x + an * N2[p].x + ab * o_m.x + at * o_p.x;
+      const T sum1 = cx * W1 + ae1 * E1 + as * S2[p].y + an * N2[p].y + ab * o_m.y + at * o_p.y;
+      if (BND) {
+        const T ap0 = T(-1) * (aw0 + ae0 + as + an + ab + at);
+        const T ap1 = T(-1) * (cx + ae1 + as + an + ab + at);
+        R2[p].x = div_ap(R2[p].x - sum0, ap0, T(1) / ap0);
+        R2[p].y = div_ap(R2[p].y - sum1, ap1, T(1) / ap1);
+      } else {
+        const bool x0f = !px && (fw0 || fe0), x1f = !px && fe1;
+        R2[p].x = div_ap(R2[p].x - sum0, x0f ? ap_x1 : ap_in, x0f ? inv_x1 : inv_in);
+        R2[p].y = div_ap(R2[p].y - sum1, x1f ? ap_x1 : ap_in, x1f ? inv_x1 : inv_in);
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < KB; ++p) {
+      const int s = (colour + j + k0 + p) & 1;
+      const int i0 = 2 * m0 + s, i1 = i0 + 2;
+      const bool w0 = col_ok && i0 < ni, w1 = col_ok && i1 < ni;
+      gmem<char> *dq = Bd + aD + cell + p0 + (unsigned)p * plane;
+      if (w0 && w1) {
+        *(gmem<v2> *)dq = R2[p];
+      } else {
+        if (w0) *(gmem<T> *)dq = R2[p].x;
+        if (w1) *(gmem<T> *)(dq + SZ) = R2[p].y;
+      }
     }
   }
 }
@@ -396,38 +488,33 @@ __device__ __forceinline__ void xface_tile(const Tile &t, const LevelDev &L, int
   }
 }
 
-// One launch per colour half-sweep over a level: every tile of the list is
-// an interior march tile (kind 0), a boundary march tile (kind 1) or an
-// x-face tile (kind 2); the kinds write disjoint cells.
-template <class T>
+// One launch per colour half-sweep over a level.  Every tile of the list is
+// KB planes deep: a march tile (kind 0: y/z interior, kind 3: touching a y/z
+// face) or, on levels one cell wide in x (KB = 1 launches only), a general
+// boundary march tile (kind 1) or an x-face tile (kind 2); the tiles write
+// disjoint cells.  One instantiation per depth keeps each one's register
+// count to what its own tiles need.
+template <class T, int KB>
 __global__ void __launch_bounds__(NTHREADS)
     k_rb_sweep(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
                const int *__restrict__ active, int colour, int src, int zero) {
   const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
   const LevelDev &L = levs[t.lev];
   if (!active[L.blk]) return;
+  // the interior neighbours are zeros in the red half of a zero-state sweep
+  const bool zint = zero && colour == 0;
   if (t.kind == 0) {
-    // the interior neighbours are zeros in the red half of a zero-state sweep
-    const bool zint = zero && colour == 0;
-    switch (t.z1 - t.z0) {
-      case 1:
-        zint ? march_tile_inner<T, 1, true>(t, L, colour, src)
-             : march_tile_inner<T, 1, false>(t, L, colour, src);
-        break;
-      case 2:
-        zint ? march_tile_inner<T, 2, true>(t, L, colour, src)
-             : march_tile_inner<T, 2, false>(t, L, colour, src);
-        break;
-      case 4:
-        zint ? march_tile_inner<T, 4, true>(t, L, colour, src)
-             : march_tile_inner<T, 4, false>(t, L, colour, src);
-        break;
-      default: march_tile<T, true>(t, L, colour, src, zero);
-    }
-  } else if (t.kind == 1)
-    march_tile<T, false>(t, L, colour, src, zero);
-  else
-    xface_tile<T>(t, L, colour, src, zero);
+    zint ? march_tile_fast<T, KB, true, false>(t, L, colour, src, zero)
+         : march_tile_fast<T, KB, false, false>(t, L, colour, src, zero);
+  } else if (t.kind == 3) {
+    zint ? march_tile_fast<T, KB, true, true>(t, L, colour, src, zero)
+         : march_tile_fast<T, KB, false, true>(t, L, colour, src, zero);
+  } else if (KB == 1) {
+    if (t.kind == 1)
+      march_tile<T, false>(t, L, colour, src, zero);
+    else
+      xface_tile<T>(t, L, colour, src, zero);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -958,6 +1045,116 @@ __device__ __forceinline__ void residual_tile_inner(const Tile &t, const FineDev
   }
 }
 
+// Interior residual tile of KB planes, software-pipelined: a rolling window
+// of planes k-1, k, k+1 of both colours, and the loads of plane k+1 (its
+// f, rows j-1/j+1, edge values and window plane k+2) issued before plane k
+// is computed.  The arithmetic is residual_tile<T, true>'s.
+template <class T, int KB>
+__device__ __forceinline__ void residual_tile_pipe(const Tile &t, const FineDev &F,
+                                                   const LevelDev &L, double &ss, double &am) {
+  const int lpr = F.lpr;
+  const int lane = (int)threadIdx.x;
+  const int rl = lane & (lpr - 1);
+  const int l = t.x0 + rl;
+  const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
+  const int ni = F.ni, nj = F.nj, H = F.H;
+  const bool ok = 2 * l < ni && j < nj;
+  const bool b_ok = 2 * l + 1 < ni && j < nj;
+  const unsigned row = (unsigned)H, plane = (unsigned)H * (unsigned)nj;
+  const int jc = min(j, nj - 1);
+  const unsigned rbase = (unsigned)jc * row;
+  const unsigned cbase = rbase + (unsigned)min(l, H - 1);
+  const gmem<double> *P[2] = {(const gmem<double> *)F.p[0], (const gmem<double> *)F.p[1]};
+  const gmem<double> *Fv[2] = {(const gmem<double> *)F.f[0], (const gmem<double> *)F.f[1]};
+  gmem<T> *R[2] = {(gmem<T> *)L.rhs[0], (gmem<T> *)L.rhs[1]};
+  const bool px = F.per & 1;
+  const double cx = F.cx, cy = F.cy, cz = F.cz;
+  const int ia = 2 * l, ib = ia + 1;
+  const bool w_ghost = ia == 0, e_ghost = ib + 1 >= ni;
+  const bool w_edge = w_ghost || (rl == 0 && l > 0);
+  const bool e_edge = e_ghost || (rl == lpr - 1 && l + 1 < H);
+  const int xw = image(ia - 1, ni, px), xe = image(ib + 1, ni, px);
+  const unsigned w_ent = rbase + (unsigned)(w_edge ? (xw >> 1) : min(l, H - 1));
+  const unsigned e_ent = rbase + (unsigned)(e_edge ? (xe >> 1) : min(l, H - 1));
+  const int w_par = w_edge ? (xw & 1) : 0, e_par = e_edge ? (xe & 1) : 0;
+  auto ldp = [&](int c, unsigned off) -> double { return P[c][off]; };
+  Coef<double> c_a, c_b;
+  c_a.as = c_a.an = c_b.as = c_b.an = cy;
+  c_a.ab = c_a.at = c_b.ab = c_b.at = cz;
+  c_a.aw = (ia == 0 && !px) ? 0.0 : cx;
+  c_a.ae = (ia == ni - 1 && !px) ? 0.0 : cx;
+  c_b.aw = cx;
+  c_b.ae = (ib == ni - 1 && !px) ? 0.0 : cx;
+  c_a.ap = -1.0 * (c_a.aw + c_a.ae + c_a.as + c_a.an + c_a.ab + c_a.at);
+  c_b.ap = -1.0 * (c_b.aw + c_b.ae + c_b.as + c_b.an + c_b.ab + c_b.at);
+
+  struct PlaneSet {
+    double fa, fb, sa, sb, na, nb, we, ee, ap1, bp1;  // ap1/bp1: window plane k+1
+  };
+  auto load_set = [&](int k) -> PlaneSet {
+    PlaneSet q;
+    const int ca = (jc + k) & 1, cb = ca ^ 1;
+    const unsigned o = (unsigned)k * plane + cbase;
+    q.fa = Fv[ca][o];
+    q.fb = Fv[cb][o];
+    q.sa = ldp(cb, o - row);
+    q.sb = ldp(ca, o - row);
+    q.na = ldp(cb, o + row);
+    q.nb = ldp(ca, o + row);
+    q.ap1 = ldp(cb, o + plane);  // cell a of plane k+1 has colour cb
+    q.bp1 = ldp(ca, o + plane);
+    q.we = w_edge ? ldp((w_par + jc + k) & 1, (unsigned)k * plane + w_ent) : 0.0;
+    q.ee = e_edge ? ldp((e_par + jc + k) & 1, (unsigned)k * plane + e_ent) : 0.0;
+    return q;
+  };
+  const int k0 = t.z0;
+  // window: a/b of planes k-1 and k
+  double am1, bm1, a0, b0;
+  {
+    const int ca = (jc + k0 - 1) & 1;
+    const unsigned o = (unsigned)(k0 - 1) * plane + cbase;
+    am1 = ldp(ca, o);
+    bm1 = ldp(ca ^ 1, o);
+    a0 = ldp(ca ^ 1, o + plane);
+    b0 = ldp(ca, o + plane);
+  }
+  PlaneSet cur = load_set(k0);
+#pragma unroll
+  for (int p = 0; p < KB; ++p) {
+    const int k = k0 + p;
+    PlaneSet nxt;
+    if (p + 1 < KB) nxt = load_set(k + 1);
+    const double Pa = a0, Pb = b0;
+    const double lf = wave_shift(Pb, 1), rt = wave_shift(Pa, 0);
+    const double Wa = w_edge ? cur.we : lf;
+    const double Ea = ib < ni ? Pb : cur.ee;
+    const double Wb = Pa;
+    const double Eb = e_edge ? cur.ee : rt;
+    double ra = cur.fa, rb = cur.fb;
+    ra = ra - (c_a.aw * Wa + c_a.ae * Ea + c_a.as * cur.sa + c_a.an * cur.na + c_a.ab * am1 +
+               c_a.at * cur.ap1 + c_a.ap * Pa);
+    rb = rb - (c_b.aw * Wb + c_b.ae * Eb + c_b.as * cur.sb + c_b.an * cur.nb + c_b.ab * bm1 +
+               c_b.at * cur.bp1 + c_b.ap * Pb);
+    const int ca = (jc + k) & 1, cb = ca ^ 1;
+    const unsigned q = (unsigned)k * plane + cbase;
+    if (ok) {
+      R[ca][q] = (T)ra;
+      ss += ra * ra;
+      am = fmax(am, fabs(ra));
+    }
+    if (b_ok) {
+      R[cb][q] = (T)rb;
+      ss += rb * rb;
+      am = fmax(am, fabs(rb));
+    }
+    am1 = a0;
+    bm1 = b0;
+    a0 = cur.ap1;
+    b0 = cur.bp1;
+    if (p + 1 < KB) cur = nxt;
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(NTHREADS)
     k_residual(const Tile *__restrict__ tiles, const FineDev *__restrict__ fine,
@@ -970,7 +1167,9 @@ __global__ void __launch_bounds__(NTHREADS)
   const LevelDev &L = levs[t.lev];
   double ss = 0.0, am = 0.0;
   if (active[F.blk]) {
-    if (t.kind == 0 && use_p && t.z1 - t.z0 == 2)
+    if (t.kind == 0 && use_p && t.z1 - t.z0 == 8)
+      residual_tile_pipe<T, 8>(t, F, L, ss, am);
+    else if (t.kind == 0 && use_p && t.z1 - t.z0 == 2)
       residual_tile_inner<T, 2>(t, F, L, ss, am);
     else if (t.kind == 0)
       residual_tile<T, true>(t, F, L, use_p, ss, am);
@@ -1661,11 +1860,11 @@ inline dim3 tile_block() { return dim3(TILE_X, TILE_Y, 1); }
 
 // ---- launchers -----------------------------------------------------------
 template <class T>
-void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
                      const int *active, int colour, int src, int zero) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_rb_sweep<T>, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour,
-                     src, zero);
+  auto k = depth == 4 ? k_rb_sweep<T, 4> : depth == 2 ? k_rb_sweep<T, 2> : k_rb_sweep<T, 1>;
+  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour, src, zero);
 }
 
 template <class Tf, class Tc>
@@ -1810,10 +2009,10 @@ void launch_remove_mean(hipStream_t s, const FineDev *fd, int ni, int nj, int nk
 }
 
 // explicit instantiations
-template void launch_rb_sweep<double>(hipStream_t, const Tile *, int, const LevelDev *,
+template void launch_rb_sweep<double>(hipStream_t, const Tile *, int, int, const LevelDev *,
                                       const int *, int, int, int);
-template void launch_rb_sweep<float>(hipStream_t, const Tile *, int, const LevelDev *, const int *,
-                                     int, int, int);
+template void launch_rb_sweep<float>(hipStream_t, const Tile *, int, int, const LevelDev *,
+                                     const int *, int, int, int);
 template void launch_restrict<double, double>(hipStream_t, const Tile *, int, const LevelDev *,
                                               const int *, int);
 template void launch_restrict<float, float>(hipStream_t, const Tile *, int, const LevelDev *,

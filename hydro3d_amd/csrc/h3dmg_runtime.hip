@@ -52,6 +52,7 @@ struct TileSet {
   std::vector<Tile> h;
   Tile *d = nullptr;
   Tile *owned = nullptr;  // smoother sets: separately allocated (re-tunable)
+  int depth = 1;          // smoother sets: planes of every tile (1, 2 or 4)
   int n() const { return (int)h.size(); }
 };
 
@@ -82,22 +83,31 @@ int march_depth(int tiles_xy, int nz, long target = 4096) {
 
 // Smoother tiles of level descriptor `lev`, in plane-chunk order (the XCD
 // remap then hands every XCD a contiguous slab): per chunk of kb planes the
-// x-face tiles of those planes (kind 2, 256 rows each) and the march tiles
-// (2*lpr entries x 4*(64/lpr) rows x kb planes), kind 0 when every y/z
-// neighbour of the tile is inside the block, kind 1 otherwise.
-void add_tiles_march(TileSet &ts, int lev, const LevelDev &L, int kb_req = 0) {
+// march tiles (2*lpr entries x 4*(64/lpr) rows x kb planes), kind 0 when
+// every y/z neighbour of the tile is inside the block, kind 3 otherwise; both
+// update the x-face cells of their rows.  Levels one cell wide in x use the
+// general boundary tiles (kind 1) and separate x-face tiles (kind 2, 256
+// rows of one plane each).
+int march_tiles_xy(const LevelDev &L) {
+  const int xs = 2 * L.lpr, ys = TILE_Y * (64 / L.lpr);
+  return ((L.H + xs - 1) / xs) * ((L.nj + ys - 1) / ys);
+}
+
+// kb planes per tile; kb divides L.nk
+void add_tiles_march(TileSet &ts, int lev, const LevelDev &L, int kb) {
   const int xs = 2 * L.lpr, ys = TILE_Y * (64 / L.lpr);
   const int nx = L.H, ny = L.nj, nz = L.nk;
-  const int txy = ((nx + xs - 1) / xs) * ((ny + ys - 1) / ys);
-  const int kb = kb_req > 0 ? std::min(kb_req, nz) : march_depth(txy, nz);
+  const bool faces_in_tiles = L.ni >= 2;
   for (int z = 0; z < nz; z += kb) {
     const int z1 = std::min(z + kb, nz);
-    for (int zz = z; zz < z1; ++zz)
-      for (int y = 0; y < ny; y += NTHREADS) ts.h.push_back(Tile{lev, y, zz, 0, 0, 2});
+    if (!faces_in_tiles)
+      for (int zz = z; zz < z1; ++zz)
+        for (int y = 0; y < ny; y += NTHREADS) ts.h.push_back(Tile{lev, y, zz, 0, 0, 2});
     for (int y = 0; y < ny; y += ys)
       for (int x = 0; x < nx; x += xs) {
-        const bool inner = L.ni >= 4 && y >= 1 && y + ys <= ny - 1 && z >= 1 && z1 <= nz - 1;
-        ts.h.push_back(Tile{lev, x, y, z, z1, inner ? 0 : 1});
+        const bool inner = y >= 1 && y + ys <= ny - 1 && z >= 1 && z1 <= nz - 1;
+        const int kind = faces_in_tiles ? (inner ? 0 : 3) : 1;
+        ts.h.push_back(Tile{lev, x, y, z, z1, kind});
       }
   }
 }
@@ -209,7 +219,7 @@ template <class T>
 void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool &zero) {
   for (int s = 0; s < nsweep; ++s) {
     for (int colour = 0; colour < 2; ++colour)
-      launch_rb_sweep<T>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, colour, buf,
+      launch_rb_sweep<T>(h->stream, ts.d, ts.n(), ts.depth, h->d_levs, h->d_active, colour, buf,
                          zero ? 1 : 0);
     buf ^= 1;
     zero = false;
@@ -494,6 +504,12 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       const bool is64 = !is_mixed(h) || g == B.L;
       const size_t w = is64 ? 8 : 4;
       const size_t half = (size_t)L.H * L.nj * L.nk * w;
+      // kernels address a level's six arrays as 32-bit byte offsets from
+      // its first one
+      if (6 * align_up(half, 256) >= ((size_t)1 << 32))
+        return fail(H3DMG_ERR_UNSUPPORTED,
+                    "block %d level %d: %d x %d x %d cells exceed the 4 GiB level span", b, g,
+                    L.ni, L.nj, L.nk);
       Pending P{};
       P.d = (int)h->levs.size();
       for (int bb = 0; bb < 2; ++bb)
@@ -605,10 +621,10 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     const int lpr = lanes_per_row(2 * L.H);
     const int xs = lpr, ys = TILE_Y * (64 / lpr);
     const int txy = ((L.H + xs - 1) / xs) * ((L.nj + ys - 1) / ys);
-    // 2-plane tiles: the interior tiles issue all their loads at once; at
-    // two planes the kernel stays at 5 waves per SIMD (deeper tiles cost
-    // occupancy, single planes re-read the plane window too often)
-    const int kb = std::min(2, march_depth(txy, L.nk, 2048));
+    // up to 8-plane tiles: interior tiles of 8 planes are software-pipelined
+    // (the loads of plane k+1 are in flight while plane k is computed), so
+    // depth costs no occupancy and the plane window is re-read 1.25x, not 2x
+    const int kb = std::min(8, march_depth(txy, L.nk, 2048));
     for (int z = 0; z < L.nk; z += kb) {
       const int z1 = std::min(z + kb, L.nk);
       for (int y = 0; y < L.nj; y += ys)
@@ -766,12 +782,33 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
   for (int grp = 0; grp < 2; ++grp) {
     TileSet &ts = grp == 0 ? h->relax_nc[g] : h->relax_c[g];
     ts.h.clear();
+    // one depth for the whole launch: the requested planes, or enough for
+    // ~4096 workgroups over the group's blocks; at most 4, a power of two
+    // dividing every block's nk; 1 where a level is one cell wide in x
+    auto member = [&](const BlockInfo &B) { return B.L >= g && (grp == 1) == (B.L == g); };
+    long cols = 0;
+    int want = h->march_kb[g];
+    bool general = false;
+    for (const BlockInfo &B : h->blk)
+      if (member(B)) {
+        const LevelDev &L = h->levs[B.desc[g]];
+        cols += (long)march_tiles_xy(L) * L.nk;
+        general = general || L.ni < 2;
+      }
+    if (want <= 0) want = (int)std::min<long>(4, std::max<long>(1, cols / 4096));
+    int kb = want >= 4 ? 4 : want >= 2 ? 2 : 1;
+    for (const BlockInfo &B : h->blk)
+      if (member(B))
+        while (h->levs[B.desc[g]].nk % kb) kb >>= 1;
+    if (general) kb = 1;
+    ts.depth = kb;
     for (int b = 0; b < h->nb; ++b) {
       const BlockInfo &B = h->blk[b];
-      if (B.L < g || (grp == 1) != (B.L == g)) continue;
-      const LevelDev &L = h->levs[B.desc[g]];
-      add_tiles_march(ts, B.desc[g], L, h->march_kb[g]);
+      if (!member(B)) continue;
+      add_tiles_march(ts, B.desc[g], h->levs[B.desc[g]], kb);
     }
+    for (const Tile &t : ts.h)
+      if (t.kind != 2 && t.z1 - t.z0 != kb) return fail(H3DMG_ERR_ARG, "internal: tile depth");
     Tile *d = nullptr;
     if (!ts.h.empty()) {
       HIP_TRY(hipMalloc((void **)&d, ts.h.size() * sizeof(Tile)));
@@ -900,6 +937,20 @@ void h3dmg_default_params(h3dmg_params *p) {
 
 const char *h3dmg_last_error(void) { return g_last_error.c_str(); }
 
+// Frees a handle; the caller holds g_device_mutex.
+static void destroy_locked(h3dmg_handle *h) {
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graph(h);
+  for (auto *v : {&h->relax_nc, &h->relax_c})
+    for (TileSet &ts : *v)
+      if (ts.owned) (void)hipFree(ts.owned);
+  if (h->tail_tiles.owned) (void)hipFree(h->tail_tiles.owned);
+  if (h->h_stats) (void)hipHostFree(h->h_stats);
+  if (h->arena) (void)hipFree(h->arena);
+  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
 h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_params *prm,
                           h3dmg_handle **out) {
   std::lock_guard<std::mutex> lock(g_device_mutex);
@@ -941,7 +992,7 @@ h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_pa
   h->own_stream = true;
   h3dmg_status st = build(h, blocks, nblocks);
   if (st != H3DMG_OK) {
-    h3dmg_destroy(h);
+    destroy_locked(h);  // the device mutex is held here
     return st;
   }
   *out = h;
@@ -951,18 +1002,8 @@ h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_pa
 void h3dmg_destroy(h3dmg_handle *h) {
   if (!h) return;
   std::lock_guard<std::mutex> lock(g_device_mutex);
-  if (h->stream) (void)hipStreamSynchronize(h->stream);
-  drop_graph(h);
-  for (auto *v : {&h->relax_nc, &h->relax_c})
-    for (TileSet &ts : *v)
-      if (ts.owned) (void)hipFree(ts.owned);
-  if (h->tail_tiles.owned) (void)hipFree(h->tail_tiles.owned);
-  if (h->h_stats) (void)hipHostFree(h->h_stats);
-  if (h->arena) (void)hipFree(h->arena);
-  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
-  delete h;
+  destroy_locked(h);
 }
-
 unsigned long long h3dmg_device_bytes(const h3dmg_handle *h) { return h ? h->bytes : 0; }
 
 h3dmg_status h3dmg_set_stream(h3dmg_handle *h, void *stream) {

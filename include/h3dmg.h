@@ -179,9 +179,11 @@ h3dmg_status h3dmg_remove_mean(h3dmg_handle *h);
 h3dmg_status h3dmg_set_stream(h3dmg_handle *h, void *stream);
 
 /* ---- tuning (used by the autotuner, hydro3d_amd/autotune.py) -------------
- * Planes each smoother workgroup walks at `level` (every block); 0 restores
- * the built-in choice.  Drops a captured graph.  h3dmg_get_march returns the
- * depth in use (or -1 for a bad level). */
+ * Planes each smoother workgroup walks at `level` (every block): 1, 2 or 4
+ * (larger requests use 4; the depth in use is the largest power of two not
+ * above the request that divides every block's nk at that level); 0 restores
+ * the built-in choice.  Results do not depend on it.  Drops a captured graph.
+ * h3dmg_get_march returns the depth in use (or -1 for a bad level). */
 h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes);
 int h3dmg_get_march(const h3dmg_handle *h, int level);
 /* Levels >= `level` of every block run as one workgroup per block in a

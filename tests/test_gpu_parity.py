@@ -211,6 +211,66 @@ def test_gpu_tail_level_does_not_change_results(gpu_lib, prec):
             assert np.array_equal(ps[b], ref[2][b]), (tail, b)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_gpu_march_depth_does_not_change_results(gpu_lib, prec):
+    """Smoother and residual tiles of any depth (1..16 planes, including the
+    software-pipelined 8-plane interior tiles) compute every cell with the
+    same arithmetic: the pressure is bitwise that of the automatic choice."""
+    rng = np.random.default_rng(11)
+    W, P = 0, 1
+    specs = [dict(n=(64, 64, 64), h=(1 / 64,) * 3, bc=(P, P, P, P, W, W), nlev=4),
+             dict(n=(32, 48, 40), h=(0.02, 0.03, 0.025), bc=(W,) * 6, nlev=3)]
+    fs = [mean_free(rng.standard_normal((s["n"][2], s["n"][1], s["n"][0]))) for s in specs]
+    kw = dict(tol=1e-9, maxcy=12)
+    if prec == "mixed":
+        kw.update(precision="mixed", coarse="exact")
+    results = {}
+    for depth in (0, 1, 2, 4, 8, 16):
+        print(f"case: march depth {depth} ({prec})", flush=True)
+        with _solver(specs, **kw) as S:
+            S.set_tail(S.nlevels_max + 1)  # every level on the per-level kernels
+            for g in range(1, S.nlevels_max + 1):
+                S.set_march(g, depth)
+            for b, f in enumerate(fs):
+                S.set_rhs(b, f)
+            S.solve()  # 12 cycles whether or not they reach 1e-9
+            cyc, _ = S.block_stats()
+            results[depth] = (list(cyc), [S.get_p(b) for b in range(len(specs))])
+    ref = results[0]
+    for depth, (cyc, ps) in results.items():
+        assert cyc == ref[0], (depth, cyc, ref[0])
+        for b in range(len(specs)):
+            assert np.array_equal(ps[b], ref[1][b]), (depth, b)
+
+
+@pytest.mark.parametrize("n", [(64, 64, 64), (128, 128, 128), (128, 256, 256)],
+                         ids=["64", "128", "128x256x256"])
+def test_gpu_residual_norm_matches_numpy(gpu_lib, n):
+    """The fine-level residual kernel at shapes whose tiles are 1, 2 and 8
+    planes deep (8 = the software-pipelined interior tiles): the relative
+    residual norm of a random pressure, hist[0] of a solve, equals the numpy
+    norm of f - A p to 1e-12 (only the reduction order differs)."""
+    rng = np.random.default_rng(3)
+    P, W = 1, 0
+    bc = (P, P, P, P, W, W)
+    h = tuple(1.0 / x for x in n)
+    spec = dict(n=n, h=h, bc=bc, nlev=3)
+    f = mean_free(rng.standard_normal((n[2], n[1], n[0])))
+    p = rng.standard_normal((n[2], n[1], n[0]))
+    pad = np.zeros((n[2] + 2, n[1] + 2, n[0] + 2))
+    pad[1:-1, 1:-1, 1:-1] = p
+    r = f - apply_np(image_ghosts(pad, bc), h, bc)
+    want = float(np.linalg.norm(r) / np.linalg.norm(f))
+    for prec in ("fp64", "mixed"):
+        print(f"case: residual {n} {prec}", flush=True)
+        kw = dict(precision="mixed", coarse="exact") if prec == "mixed" else {}
+        with _solver(spec, tol=1e-30, maxcy=1, **kw) as S:
+            S.set_rhs(0, f)
+            S.set_p(0, p)
+            _, _, hist = S.solve()
+        assert abs(hist[0] - want) <= 1e-12 * want, (prec, hist[0], want)
+
+
 def test_gpu_projection_bitwise_vs_reference(gpu_lib):
     """calvel + calmas on the device (h3dmg_project_device) reproduce the
     reference's su bitwise: su0 from p = 0 and the final su from the
@@ -234,6 +294,16 @@ def test_gpu_projection_bitwise_vs_reference(gpu_lib):
     # periodic x: the wrap face's two copies agree; walls in z keep w* = 0
     assert torch.equal(u[:, :, 0], u[:, :, -1])
     assert float(w[0].abs().max()) == 0.0 and float(w[-1].abs().max()) == 0.0
+
+
+def test_gpu_block_beyond_level_span_is_refused(gpu_lib):
+    """Kernels address a level's arrays as 32-bit offsets: a block whose
+    level-1 arrays span 4 GiB or more is refused before any allocation."""
+    from hydro3d_amd import _lib as L
+    n = (1024, 1024, 1024)
+    with pytest.raises(L.H3dError) as e:
+        _solver(dict(n=n, h=(1 / 1024,) * 3, bc=(1, 1, 1, 1, 0, 0), nlev=6))
+    assert e.value.status == L.ERR_UNSUPPORTED, e.value
 
 
 def test_gpu_graph_and_eager_identical(gpu_lib):
