@@ -89,11 +89,13 @@ void launch_prolong(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev
 template <class T>
 void launch_fill_zero(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                       const int *active, int buf);
-// tiles of the finest level: Tile::lev is the BLOCK index b; fine[b] and
-// levs[b] (level-1 descriptors come first) describe it.  use_p = 0 gives
-// r = f (the norm of the right-hand side).
+// tiles of the finest level, all `depth` (1, 2, 4 or 8) planes deep, or
+// depth 0 for sets that hold general (kind 1) tiles:
+// Tile::lev is the BLOCK index b; fine[b] and levs[b] (level-1 descriptors
+// come first) describe it.  use_p = 0 gives r = f (the norm of the
+// right-hand side).
 template <class T>
-void launch_residual(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,
+void launch_residual(hipStream_t s, const Tile *tiles, int ntiles, int depth, const FineDev *fine,
                      const LevelDev *levs, const int *active, RedPartial *partials, int use_p);
 template <class T>
 void launch_update(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,

@@ -931,192 +931,92 @@ __device__ __forceinline__ void residual_tile(const Tile &t, const FineDev &F, c
   }
 }
 
-// Interior residual tile of exactly KB planes, every load issued up front
-// (no branches between a load and its use): the own-column window of both
-// colours (KB+2 planes), f, rows j-1 and j+1, and one "edge" value per side
-// and plane -- the x-ghost image at a row end, the neighbouring tile's entry
-// at a tile edge, the lane's own entry (unused) elsewhere.  The arithmetic is
-// residual_tile<T, true>'s.
-template <class T, int KB>
-__device__ __forceinline__ void residual_tile_inner(const Tile &t, const FineDev &F,
-                                                    const LevelDev &L, double &ss, double &am) {
-  const int lpr = F.lpr;
-  const int lane = (int)threadIdx.x;
-  const int rl = lane & (lpr - 1);
-  const int l = t.x0 + rl;
-  const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
-  const int ni = F.ni, nj = F.nj, H = F.H;
-  const bool ok = 2 * l < ni && j < nj;
-  const bool b_ok = 2 * l + 1 < ni && j < nj;
-  const unsigned row = (unsigned)H, plane = (unsigned)H * (unsigned)nj;
-  const int jc = min(j, nj - 1);
-  const unsigned rbase = (unsigned)jc * row;
-  const unsigned cbase = rbase + (unsigned)min(l, H - 1);
-  const gmem<double> *P[2] = {(const gmem<double> *)F.p[0], (const gmem<double> *)F.p[1]};
-  const gmem<double> *Fv[2] = {(const gmem<double> *)F.f[0], (const gmem<double> *)F.f[1]};
-  gmem<T> *R[2] = {(gmem<T> *)L.rhs[0], (gmem<T> *)L.rhs[1]};
-  const bool px = F.per & 1;
-  const double cx = F.cx, cy = F.cy, cz = F.cz;
-  const int ia = 2 * l, ib = ia + 1;
-  // west neighbour of a (cell ia-1) and east neighbour of b (cell ib+1) when
-  // they are not in the adjacent lane: x-ghost image or the next tile's entry
-  const bool w_ghost = ia == 0, e_ghost = ib + 1 >= ni;
-  const bool w_edge = w_ghost || (rl == 0 && l > 0);
-  const bool e_edge = e_ghost || (rl == lpr - 1 && l + 1 < H);
-  const int xw = image(ia - 1, ni, px), xe = image(ib + 1, ni, px);
-  // entry and colour parity of those cells relative to (j + k)
-  const unsigned w_ent = rbase + (unsigned)(w_edge ? (xw >> 1) : min(l, H - 1));
-  const unsigned e_ent = rbase + (unsigned)(e_edge ? (xe >> 1) : min(l, H - 1));
-  const int w_par = w_edge ? (xw & 1) : 0, e_par = e_edge ? (xe & 1) : 0;
-  auto ldp = [&](int c, unsigned off) -> double { return P[c][off]; };
-
-  const int k0 = t.z0;
-  double A[KB + 2], B[KB + 2];  // cell a / b of the own row, planes k0-1 .. k0+KB
-  double FA[KB], FB[KB], SA[KB], SB[KB], NA[KB], NB[KB], WE[KB], EE[KB];
-#pragma unroll
-  for (int p = 0; p < KB + 2; ++p) {
-    const int k = k0 - 1 + p;
-    const int ca = (jc + k) & 1;
-    const unsigned q = (unsigned)k * plane + cbase;
-    A[p] = ldp(ca, q);
-    B[p] = ldp(ca ^ 1, q);
-  }
-#pragma unroll
-  for (int p = 0; p < KB; ++p) {
-    const int k = k0 + p;
-    const int ca = (jc + k) & 1, cb = ca ^ 1;
-    const unsigned q = (unsigned)k * plane + cbase;
-    FA[p] = Fv[ca][q];
-    FB[p] = Fv[cb][q];
-    SA[p] = ldp(cb, q - row);
-    SB[p] = ldp(ca, q - row);
-    NA[p] = ldp(cb, q + row);
-    NB[p] = ldp(ca, q + row);
-  }
-  // edge values: only the (at most two) lanes of a row that need them load
-#pragma unroll
-  for (int p = 0; p < KB; ++p) {
-    const int k = k0 + p;
-    WE[p] = w_edge ? ldp((w_par + jc + k) & 1, (unsigned)k * plane + w_ent) : 0.0;
-    EE[p] = e_edge ? ldp((e_par + jc + k) & 1, (unsigned)k * plane + e_ent) : 0.0;
-  }
-  Coef<double> c_a, c_b;
-  c_a.as = c_a.an = c_b.as = c_b.an = cy;
-  c_a.ab = c_a.at = c_b.ab = c_b.at = cz;
-  c_a.aw = (ia == 0 && !px) ? 0.0 : cx;
-  c_a.ae = (ia == ni - 1 && !px) ? 0.0 : cx;
-  c_b.aw = cx;
-  c_b.ae = (ib == ni - 1 && !px) ? 0.0 : cx;
-  c_a.ap = -1.0 * (c_a.aw + c_a.ae + c_a.as + c_a.an + c_a.ab + c_a.at);
-  c_b.ap = -1.0 * (c_b.aw + c_b.ae + c_b.as + c_b.an + c_b.ab + c_b.at);
-#pragma unroll
-  for (int p = 0; p < KB; ++p) {
-    const double Pa = A[p + 1], Pb = B[p + 1];
-    // cell 2l-1 is lane-1's b, cell 2l+2 is lane+1's a
-    const double lf = wave_shift(Pb, 1), rt = wave_shift(Pa, 0);
-    const double Wa = w_edge ? WE[p] : lf;
-    const double Ea = ib < ni ? Pb : EE[p];
-    const double Wb = Pa;
-    const double Eb = e_edge ? EE[p] : rt;
-    double ra = FA[p], rb = FB[p];
-    ra = ra - (c_a.aw * Wa + c_a.ae * Ea + c_a.as * SA[p] + c_a.an * NA[p] + c_a.ab * A[p] +
-               c_a.at * A[p + 2] + c_a.ap * Pa);
-    rb = rb - (c_b.aw * Wb + c_b.ae * Eb + c_b.as * SB[p] + c_b.an * NB[p] + c_b.ab * B[p] +
-               c_b.at * B[p + 2] + c_b.ap * Pb);
-    FA[p] = ra;  // the residuals replace f (all stores after all loads)
-    FB[p] = rb;
-  }
-#pragma unroll
-  for (int p = 0; p < KB; ++p) {
-    const int k = k0 + p;
-    const int ca = (jc + k) & 1, cb = ca ^ 1;
-    const unsigned q = (unsigned)k * plane + cbase;
-    const double ra = FA[p], rb = FB[p];
-    if (ok) {
-      R[ca][q] = (T)ra;
-      ss += ra * ra;
-      am = fmax(am, fabs(ra));
-    }
-    if (b_ok) {
-      R[cb][q] = (T)rb;
-      ss += rb * rb;
-      am = fmax(am, fabs(rb));
-    }
-  }
-}
-
-// Interior residual tile of KB planes, software-pipelined: a rolling window
-// of planes k-1, k, k+1 of both colours, and the loads of plane k+1 (its
-// f, rows j-1/j+1, edge values and window plane k+2) issued before plane k
-// is computed.  The arithmetic is residual_tile<T, true>'s.
-template <class T, int KB>
-__device__ __forceinline__ void residual_tile_pipe(const Tile &t, const FineDev &F,
+// Finest-level residual tile of KB planes, software-pipelined: a rolling
+// window of planes k-1, k, k+1 of both colours, with the loads of plane k+1
+// (its f, rows j-1/j+1, x-edge values and window plane k+2) issued before
+// plane k is computed.  Every neighbour is addressed by the coordinates of
+// its image (a ghost row, plane or x position resolved once per lane or per
+// plane), as a 32-bit byte offset from the block's first fine array, so
+// boundary tiles (BND) run the same code as interior ones; only their
+// coefficients differ (cell_coef's zeros towards walls).  The arithmetic of
+// every cell is residual_tile's.
+template <class T, int KB, bool BND>
+__device__ __forceinline__ void residual_tile_fast(const Tile &t, const FineDev &F,
                                                    const LevelDev &L, double &ss, double &am) {
+  constexpr unsigned SZ = sizeof(double);
   const int lpr = F.lpr;
   const int lane = (int)threadIdx.x;
   const int rl = lane & (lpr - 1);
   const int l = t.x0 + rl;
   const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
-  const int ni = F.ni, nj = F.nj, H = F.H;
+  const int ni = F.ni, nj = F.nj, nk = F.nk, H = F.H;
   const bool ok = 2 * l < ni && j < nj;
   const bool b_ok = 2 * l + 1 < ni && j < nj;
-  const unsigned row = (unsigned)H, plane = (unsigned)H * (unsigned)nj;
   const int jc = min(j, nj - 1);
-  const unsigned rbase = (unsigned)jc * row;
-  const unsigned cbase = rbase + (unsigned)min(l, H - 1);
-  const gmem<double> *P[2] = {(const gmem<double> *)F.p[0], (const gmem<double> *)F.p[1]};
-  const gmem<double> *Fv[2] = {(const gmem<double> *)F.f[0], (const gmem<double> *)F.f[1]};
+  const bool px = F.per & 1, py = F.per & 2, pz = F.per & 4;
+  const unsigned row = (unsigned)H * SZ, plane = row * (unsigned)nj;  // bytes
+  const char *base = (const char *)F.p[0];
+  const unsigned aP1 = (unsigned)((const char *)F.p[1] - base);
+  const unsigned aF0 = (unsigned)((const char *)F.f[0] - base);
+  const unsigned aF1 = (unsigned)((const char *)F.f[1] - base);
+  const gmem<char> *Bs = (const gmem<char> *)base;
   gmem<T> *R[2] = {(gmem<T> *)L.rhs[0], (gmem<T> *)L.rhs[1]};
-  const bool px = F.per & 1;
-  const double cx = F.cx, cy = F.cy, cz = F.cz;
+  auto ld = [&](unsigned off) -> double { return *(const gmem<double> *)(Bs + off); };
+  // p of the cell at (x, y, k) with x even/odd given by the parity argument:
+  // colour array (x + y + k) & 1
+  auto pcol = [&](int par) -> unsigned { return (par & 1) ? aP1 : 0u; };
   const int ia = 2 * l, ib = ia + 1;
-  const bool w_ghost = ia == 0, e_ghost = ib + 1 >= ni;
-  const bool w_edge = w_ghost || (rl == 0 && l > 0);
-  const bool e_edge = e_ghost || (rl == lpr - 1 && l + 1 < H);
+  const unsigned ent = (unsigned)min(l, H - 1) * SZ;
+  // rows of the y neighbours, ghosts resolved to their image row
+  const int ys = BND ? image(jc - 1, nj, py) : jc - 1;
+  const int yn = BND ? image(jc + 1, nj, py) : jc + 1;
+  const unsigned o_c = (unsigned)jc * row + ent;
+  const unsigned o_s = (unsigned)ys * row + ent, o_n = (unsigned)yn * row + ent;
+  // x neighbours not held by the adjacent lane: x-ghost image or the next
+  // tile's entry
+  const bool w_edge = ia == 0 || (rl == 0 && l > 0);
+  const bool e_edge = ib + 1 >= ni || (rl == lpr - 1 && l + 1 < H);
   const int xw = image(ia - 1, ni, px), xe = image(ib + 1, ni, px);
-  const unsigned w_ent = rbase + (unsigned)(w_edge ? (xw >> 1) : min(l, H - 1));
-  const unsigned e_ent = rbase + (unsigned)(e_edge ? (xe >> 1) : min(l, H - 1));
+  const unsigned w_ent = (unsigned)jc * row + (w_edge ? (unsigned)(xw >> 1) * SZ : ent);
+  const unsigned e_ent = (unsigned)jc * row + (e_edge ? (unsigned)(xe >> 1) * SZ : ent);
   const int w_par = w_edge ? (xw & 1) : 0, e_par = e_edge ? (xe & 1) : 0;
-  auto ldp = [&](int c, unsigned off) -> double { return P[c][off]; };
-  Coef<double> c_a, c_b;
-  c_a.as = c_a.an = c_b.as = c_b.an = cy;
-  c_a.ab = c_a.at = c_b.ab = c_b.at = cz;
-  c_a.aw = (ia == 0 && !px) ? 0.0 : cx;
-  c_a.ae = (ia == ni - 1 && !px) ? 0.0 : cx;
-  c_b.aw = cx;
-  c_b.ae = (ib == ni - 1 && !px) ? 0.0 : cx;
-  c_a.ap = -1.0 * (c_a.aw + c_a.ae + c_a.as + c_a.an + c_a.ab + c_a.at);
-  c_b.ap = -1.0 * (c_b.aw + c_b.ae + c_b.as + c_b.an + c_b.ab + c_b.at);
+  const double cx = F.cx, cy = F.cy, cz = F.cz;
+  const double aw_a = (ia == 0 && !px) ? 0.0 : cx, ae_a = (ia == ni - 1 && !px) ? 0.0 : cx;
+  const double aw_b = cx, ae_b = (ib == ni - 1 && !px) ? 0.0 : cx;
+  const double as = (BND && !py && jc == 0) ? 0.0 : cy;
+  const double an = (BND && !py && jc == nj - 1) ? 0.0 : cy;
 
   struct PlaneSet {
     double fa, fb, sa, sb, na, nb, we, ee, ap1, bp1;  // ap1/bp1: window plane k+1
   };
+  // the window plane after k: plane k+1 or its image
+  auto next_plane = [&](int k) -> int { return BND ? image(k + 1, nk, pz) : k + 1; };
   auto load_set = [&](int k) -> PlaneSet {
     PlaneSet q;
-    const int ca = (jc + k) & 1, cb = ca ^ 1;
-    const unsigned o = (unsigned)k * plane + cbase;
-    q.fa = Fv[ca][o];
-    q.fb = Fv[cb][o];
-    q.sa = ldp(cb, o - row);
-    q.sb = ldp(ca, o - row);
-    q.na = ldp(cb, o + row);
-    q.nb = ldp(ca, o + row);
-    q.ap1 = ldp(cb, o + plane);  // cell a of plane k+1 has colour cb
-    q.bp1 = ldp(ca, o + plane);
-    q.we = w_edge ? ldp((w_par + jc + k) & 1, (unsigned)k * plane + w_ent) : 0.0;
-    q.ee = e_edge ? ldp((e_par + jc + k) & 1, (unsigned)k * plane + e_ent) : 0.0;
+    const unsigned o = (unsigned)k * plane;
+    const int ca = (jc + k) & 1;
+    q.fa = ld((ca ? aF1 : aF0) + o + o_c);
+    q.fb = ld((ca ? aF0 : aF1) + o + o_c);
+    q.sa = ld(pcol(ys + k) + o + o_s);
+    q.sb = ld(pcol(ys + k + 1) + o + o_s);
+    q.na = ld(pcol(yn + k) + o + o_n);
+    q.nb = ld(pcol(yn + k + 1) + o + o_n);
+    const int kp = next_plane(k);
+    q.ap1 = ld(pcol(jc + kp) + (unsigned)kp * plane + o_c);
+    q.bp1 = ld(pcol(jc + kp + 1) + (unsigned)kp * plane + o_c);
+    q.we = ld(pcol(w_par + jc + k) + o + w_ent);
+    q.ee = ld(pcol(e_par + jc + k) + o + e_ent);
     return q;
   };
   const int k0 = t.z0;
-  // window: a/b of planes k-1 and k
+  // window: cells a/b of planes k0-1 and k0
   double am1, bm1, a0, b0;
   {
-    const int ca = (jc + k0 - 1) & 1;
-    const unsigned o = (unsigned)(k0 - 1) * plane + cbase;
-    am1 = ldp(ca, o);
-    bm1 = ldp(ca ^ 1, o);
-    a0 = ldp(ca ^ 1, o + plane);
-    b0 = ldp(ca, o + plane);
+    const int km = BND ? image(k0 - 1, nk, pz) : k0 - 1;
+    am1 = ld(pcol(jc + km) + (unsigned)km * plane + o_c);
+    bm1 = ld(pcol(jc + km + 1) + (unsigned)km * plane + o_c);
+    a0 = ld(pcol(jc + k0) + (unsigned)k0 * plane + o_c);
+    b0 = ld(pcol(jc + k0 + 1) + (unsigned)k0 * plane + o_c);
   }
   PlaneSet cur = load_set(k0);
 #pragma unroll
@@ -1125,18 +1025,22 @@ __device__ __forceinline__ void residual_tile_pipe(const Tile &t, const FineDev 
     PlaneSet nxt;
     if (p + 1 < KB) nxt = load_set(k + 1);
     const double Pa = a0, Pb = b0;
+    // cell 2l-1 is lane-1's b, cell 2l+2 is lane+1's a
     const double lf = wave_shift(Pb, 1), rt = wave_shift(Pa, 0);
     const double Wa = w_edge ? cur.we : lf;
     const double Ea = ib < ni ? Pb : cur.ee;
     const double Wb = Pa;
     const double Eb = e_edge ? cur.ee : rt;
-    double ra = cur.fa, rb = cur.fb;
-    ra = ra - (c_a.aw * Wa + c_a.ae * Ea + c_a.as * cur.sa + c_a.an * cur.na + c_a.ab * am1 +
-               c_a.at * cur.ap1 + c_a.ap * Pa);
-    rb = rb - (c_b.aw * Wb + c_b.ae * Eb + c_b.as * cur.sb + c_b.an * cur.nb + c_b.ab * bm1 +
-               c_b.at * cur.bp1 + c_b.ap * Pb);
+    const double ab = (BND && !pz && k == 0) ? 0.0 : cz;
+    const double at = (BND && !pz && k == nk - 1) ? 0.0 : cz;
+    const double ap_a = -1.0 * (aw_a + ae_a + as + an + ab + at);
+    const double ap_b = -1.0 * (aw_b + ae_b + as + an + ab + at);
+    const double ra = cur.fa - (aw_a * Wa + ae_a * Ea + as * cur.sa + an * cur.na + ab * am1 +
+                                at * cur.ap1 + ap_a * Pa);
+    const double rb = cur.fb - (aw_b * Wb + ae_b * Eb + as * cur.sb + an * cur.nb + ab * bm1 +
+                                at * cur.bp1 + ap_b * Pb);
     const int ca = (jc + k) & 1, cb = ca ^ 1;
-    const unsigned q = (unsigned)k * plane + cbase;
+    const unsigned q = ((unsigned)k * plane + o_c) / SZ;
     if (ok) {
       R[ca][q] = (T)ra;
       ss += ra * ra;
@@ -1155,11 +1059,16 @@ __device__ __forceinline__ void residual_tile_pipe(const Tile &t, const FineDev 
   }
 }
 
-template <class T>
+// One launch over the finest level of every block.  k_residual: tiles of KB
+// planes (kind 0 y/z interior, kind 3 touching a y/z face); k_residual_gen:
+// the general path for r = f (use_p = 0) and for blocks one cell wide in x
+// (kind 1 tiles), in a kernel of its own so that its registers do not cost
+// the main kernel occupancy.
+template <class T, int KB>
 __global__ void __launch_bounds__(NTHREADS)
     k_residual(const Tile *__restrict__ tiles, const FineDev *__restrict__ fine,
                const LevelDev *__restrict__ levs, const int *__restrict__ active,
-               RedPartial *__restrict__ partials, int use_p) {
+               RedPartial *__restrict__ partials) {
   // XCD-contiguous tile ranges: rows j-1, j+1 of a tile are in the same L2
   const int ti = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const Tile t = tiles[ti];
@@ -1167,11 +1076,27 @@ __global__ void __launch_bounds__(NTHREADS)
   const LevelDev &L = levs[t.lev];
   double ss = 0.0, am = 0.0;
   if (active[F.blk]) {
-    if (t.kind == 0 && use_p && t.z1 - t.z0 == 8)
-      residual_tile_pipe<T, 8>(t, F, L, ss, am);
-    else if (t.kind == 0 && use_p && t.z1 - t.z0 == 2)
-      residual_tile_inner<T, 2>(t, F, L, ss, am);
-    else if (t.kind == 0)
+    if (t.kind == 0)
+      residual_tile_fast<T, KB, false>(t, F, L, ss, am);
+    else
+      residual_tile_fast<T, KB, true>(t, F, L, ss, am);
+  }
+  block_reduce(ss, am);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partials[ti] = RedPartial{ss, am};
+}
+
+template <class T>
+__global__ void __launch_bounds__(NTHREADS)
+    k_residual_gen(const Tile *__restrict__ tiles, const FineDev *__restrict__ fine,
+                   const LevelDev *__restrict__ levs, const int *__restrict__ active,
+                   RedPartial *__restrict__ partials, int use_p) {
+  const int ti = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const Tile t = tiles[ti];
+  const FineDev &F = fine[t.lev];
+  const LevelDev &L = levs[t.lev];
+  double ss = 0.0, am = 0.0;
+  if (active[F.blk]) {
+    if (t.kind == 0)
       residual_tile<T, true>(t, F, L, use_p, ss, am);
     else
       residual_tile<T, false>(t, F, L, use_p, ss, am);
@@ -1891,11 +1816,19 @@ void launch_fill_zero(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
 }
 
 template <class T>
-void launch_residual(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,
+void launch_residual(hipStream_t s, const Tile *tiles, int ntiles, int depth, const FineDev *fine,
                      const LevelDev *levs, const int *active, RedPartial *partials, int use_p) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_residual<T>, dim3(ntiles), tile_block(), 0, s, tiles, fine, levs, active,
-                     partials, use_p);
+  if (!use_p || depth <= 0) {
+    hipLaunchKernelGGL(k_residual_gen<T>, dim3(ntiles), tile_block(), 0, s, tiles, fine, levs,
+                       active, partials, use_p);
+    return;
+  }
+  auto k = depth == 8   ? k_residual<T, 8>
+           : depth == 4 ? k_residual<T, 4>
+           : depth == 2 ? k_residual<T, 2>
+                        : k_residual<T, 1>;
+  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, fine, levs, active, partials);
 }
 
 template <class T>
@@ -2029,9 +1962,9 @@ template void launch_fill_zero<double>(hipStream_t, const Tile *, int, const Lev
                                        const int *, int);
 template void launch_fill_zero<float>(hipStream_t, const Tile *, int, const LevelDev *,
                                       const int *, int);
-template void launch_residual<double>(hipStream_t, const Tile *, int, const FineDev *,
+template void launch_residual<double>(hipStream_t, const Tile *, int, int, const FineDev *,
                                       const LevelDev *, const int *, RedPartial *, int);
-template void launch_residual<float>(hipStream_t, const Tile *, int, const FineDev *,
+template void launch_residual<float>(hipStream_t, const Tile *, int, int, const FineDev *,
                                      const LevelDev *, const int *, RedPartial *, int);
 template void launch_update<double>(hipStream_t, const Tile *, int, const FineDev *,
                                     const LevelDev *, const int *, int);

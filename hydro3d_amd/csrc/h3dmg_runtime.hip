@@ -379,10 +379,10 @@ void enqueue_update_residual(h3dmg_handle *h, bool update) {
       launch_update<float>(h->stream, ut.d, ut.n(), h->d_fine, h->d_levs, h->d_active, h->pp_buf);
   }
   if (f64)
-    launch_residual<double>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+    launch_residual<double>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
                             h->d_partials, 1);
   else
-    launch_residual<float>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+    launch_residual<float>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
                            h->d_partials, 1);
   launch_finalize(h->stream, h->d_tile_begin, h->nb, h->d_partials, h->d_stats);
 }
@@ -614,23 +614,48 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   // finest level: residual tiles (one colour-array entry of both colours per
   // lane, lpr lanes per row, kb planes; kind 0 = y/z interior) and p += pp
   // tiles (1024 entries each)
+  bool fine_general = false;  // a block one cell wide in x: general tiles
+  // One depth for every block (one launch): enough planes for ~2048
+  // workgroups over all blocks, at most 8 -- the tiles are software-pipelined
+  // (the loads of plane k+1 are in flight while plane k is computed), so
+  // depth costs no occupancy and the plane window is re-read (kb+2)/kb
+  // times -- a power of two dividing every block's nk.
   h->tile_begin.assign(nblocks + 1, 0);
+  auto fine_xy = [&](const LevelDev &L, int &xs, int &ys) {
+    const int lpr = lanes_per_row(2 * L.H);
+    xs = lpr;
+    ys = TILE_Y * (64 / lpr);
+    return ((L.H + xs - 1) / xs) * ((L.nj + ys - 1) / ys);
+  };
+  {
+    long cols = 0;
+    for (int b = 0; b < nblocks; ++b) {
+      int xs, ys;
+      cols += (long)fine_xy(h->levs[b], xs, ys) * h->levs[b].nk;
+    }
+    const long want = std::min<long>(8, std::max<long>(1, cols / 2048));
+    int kb = want >= 8 ? 8 : want >= 4 ? 4 : want >= 2 ? 2 : 1;
+    bool general = false;
+    for (int b = 0; b < nblocks; ++b) {
+      while (h->levs[b].nk % kb) kb >>= 1;
+      general = general || h->levs[b].ni < 2;
+    }
+    h->fine_tiles.depth = kb;
+    fine_general = general;
+  }
   for (int b = 0; b < nblocks; ++b) {
     const LevelDev &L = h->levs[b];
     h->tile_begin[b] = h->fine_tiles.n();
-    const int lpr = lanes_per_row(2 * L.H);
-    const int xs = lpr, ys = TILE_Y * (64 / lpr);
-    const int txy = ((L.H + xs - 1) / xs) * ((L.nj + ys - 1) / ys);
-    // up to 8-plane tiles: interior tiles of 8 planes are software-pipelined
-    // (the loads of plane k+1 are in flight while plane k is computed), so
-    // depth costs no occupancy and the plane window is re-read 1.25x, not 2x
-    const int kb = std::min(8, march_depth(txy, L.nk, 2048));
+    int xs, ys;
+    fine_xy(L, xs, ys);
+    const int kb = h->fine_tiles.depth;
     for (int z = 0; z < L.nk; z += kb) {
-      const int z1 = std::min(z + kb, L.nk);
+      const int z1 = z + kb;
       for (int y = 0; y < L.nj; y += ys)
         for (int x = 0; x < L.H; x += xs) {
           const bool inner = y >= 1 && y + ys <= L.nj - 1 && z >= 1 && z1 <= L.nk - 1;
-          h->fine_tiles.h.push_back(Tile{b, x, y, z, z1, inner ? 0 : 1});
+          const int kind = L.ni < 2 ? 1 : inner ? 0 : 3;
+          h->fine_tiles.h.push_back(Tile{b, x, y, z, z1, kind});
         }
     }
     const long entries = (long)L.H * L.nj * L.nk;
@@ -638,6 +663,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       h->update_tiles.h.push_back(Tile{b, (int)e, 0, 0, 0, 0});
   }
   h->tile_begin[nblocks] = h->fine_tiles.n();
+  if (fine_general) h->fine_tiles.depth = 0;  // every launch on the general path
 
   std::vector<TileSet *> all;
   for (int g = 1; g <= Lm; ++g)
@@ -1072,10 +1098,10 @@ h3dmg_status h3dmg_set_rhs_device(h3dmg_handle *h, int b, const double *d_su) {
                          h->stream));
   const TileSet &ft = h->fine_tiles;
   if (is_mixed(h))
-    launch_residual<float>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+    launch_residual<float>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
                            h->d_partials, 0);
   else
-    launch_residual<double>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+    launch_residual<double>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
                             h->d_partials, 0);
   launch_finalize(h->stream, h->d_tile_begin, h->nb, h->d_partials, h->d_stats);
   HIP_TRY(hipGetLastError());
@@ -1164,10 +1190,10 @@ h3dmg_status h3dmg_vcycle(h3dmg_handle *h) {
   if (st) return st;
   const TileSet &ft = h->fine_tiles;
   if (is_mixed(h))
-    launch_residual<float>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+    launch_residual<float>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
                            h->d_partials, 0);
   else
-    launch_residual<double>(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+    launch_residual<double>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
                             h->d_partials, 0);
   enqueue_vcycle(h);
   HIP_TRY(hipGetLastError());

@@ -243,11 +243,11 @@ def test_gpu_march_depth_does_not_change_results(gpu_lib, prec):
             assert np.array_equal(ps[b], ref[1][b]), (depth, b)
 
 
-@pytest.mark.parametrize("n", [(64, 64, 64), (128, 128, 128), (128, 256, 256)],
-                         ids=["64", "128", "128x256x256"])
+@pytest.mark.parametrize("n", [(64, 64, 64), (128, 128, 128), (128, 128, 256), (128, 256, 256)],
+                         ids=["64", "128", "128x128x256", "128x256x256"])
 def test_gpu_residual_norm_matches_numpy(gpu_lib, n):
-    """The fine-level residual kernel at shapes whose tiles are 1, 2 and 8
-    planes deep (8 = the software-pipelined interior tiles): the relative
+    """The fine-level residual kernel at shapes whose tiles are 1, 2, 4 and 8
+    planes deep, boundary tiles included: the relative
     residual norm of a random pressure, hist[0] of a solve, equals the numpy
     norm of f - A p to 1e-12 (only the reduction order differs)."""
     rng = np.random.default_rng(3)
