@@ -80,8 +80,10 @@ struct RedPartial {
 template <class T>
 void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
                      const int *active, int colour, int src, int zero);
+// restriction into the coarse levels of a tile list whose tiles are all
+// `depth` (1, 2 or 4) coarse planes deep
 template <class Tf, class Tc>
-void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
                      const int *active, int src_f);
 template <class Tf, class Tc>
 void launch_prolong(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,

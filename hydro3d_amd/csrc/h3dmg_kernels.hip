@@ -323,8 +323,10 @@ __device__ __forceinline__ void march_tile_fast(const Tile &t, const LevelDev &L
   // the east ghost image for the lane of cell ni-1; one load serves both
   // unless a lane has both roles (one-lane rows, a last tile of one pair)
   const bool role_r = tile_right ? rl == lpr - 1 : m0 == H - 2;
-  const unsigned off_l = tile_left ? aO + cell - SZ : aGX + jrow + (unsigned)(px ? me : 0) * SZ;
-  const unsigned off_r = tile_right ? aO + cell + 2 * SZ : aGX + jrow + (unsigned)(px ? 0 : me) * SZ;
+  const unsigned off_l =
+      tile_left ? aO + cell - SZ : aGX + jrow + (unsigned)(px ? me : 0) * SZ;
+  const unsigned off_r =
+      tile_right ? aO + cell + 2 * SZ : aGX + jrow + (unsigned)(px ? 0 : me) * SZ;
   const bool dual = lpr == 1 || (tile_left && t.x0 == H - 2);
   const unsigned off_x = rl == 0 ? off_l : (role_r ? off_r : aO + cell);
   const unsigned off_x2 = role_r ? off_r : aO + cell;
@@ -524,142 +526,152 @@ __global__ void __launch_bounds__(NTHREADS)
 // The coarse phi is not touched: the next relax of the coarse level runs with
 // zero = 1, which is the zeroing of mgsolver.for:655-662.
 //
-// Work layout.  The x-children 2l, 2l+1 of coarse cell l are entry l of the
-// two fine colour arrays ("a" = cell 2l, "b" = cell 2l+1; in row j, plane k,
-// a has colour (j+k)&1).  A lane owns coarse cell (l, jc) of a coarse column:
-// entry l of fine rows 2jc, 2jc+1, and walks the fine planes 2*z0 .. 2*z1-1,
-// keeping planes k-1, k, k+1 of its two rows in a register window; rows 2jc-1
-// and 2jc+2 are loaded, x-neighbours come from the adjacent lanes (DPP
-// shift).  The eight residuals arrive in the reference's summation order
-// (a, b of row 2jc, a, b of row 2jc+1; plane 2kc, then 2kc+1).  C.lpr1 lanes
-// cover a coarse row; the tile's `lev` is the coarse descriptor.  Kind 0:
-// every fine row/plane the tile reads is inside the block (no ghost logic in
-// y and z).
-template <class Tf, class Tc, bool INNER>
-__device__ __forceinline__ void restrict_tile(const Tile &t, const LevelDev &C, const LevelDev &F,
-                                              int src_f) {
+// Restriction tile of KBC coarse planes (2*KBC fine planes), software-
+// pipelined over the fine planes: a rolling window of fine planes k-1, k, k+1
+// of rows 2jc and 2jc+1, with the loads of plane k+1 (rhs of both rows, rows
+// 2jc-1 and 2jc+2, x-edge values and window plane k+2) issued before plane k
+// is computed.  Neighbours are addressed by the coordinates of their image
+// as 32-bit byte offsets from the fine level's arena base, so tiles touching
+// a y/z face (BND) run the same code; their coefficients carry cell_coef's
+// zeros towards walls.  The eight residuals arrive in the reference's
+// summation order (a, b of row 2jc, a, b of row 2jc+1; plane 2kc, then
+// 2kc+1).  C.lpr1 lanes cover a coarse row; the tile's `lev` is the coarse
+// descriptor.
+template <class Tf, class Tc, int KBC, bool BND>
+__device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev &C,
+                                                   const LevelDev &F, int src_f) {
+  constexpr unsigned SZ = sizeof(Tf);
+  constexpr int NF = 2 * KBC;
   const int lpr = C.lpr1;
   const int lane = (int)threadIdx.x;
   const int rl = lane & (lpr - 1);
   const int l = t.x0 + rl;
   const int jc = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
-  const int ni = F.ni, nj = F.nj, nk = F.nk;
+  const int ni = F.ni, nj = F.nj, nk = F.nk, H = F.H;
   const bool ok = l < C.ni && jc < C.nj;
-  const unsigned row = (unsigned)F.H, plane = (unsigned)F.H * (unsigned)nj;
-  const int j0 = 2 * min(jc, C.nj - 1);  // even: colours of a row pair are uniform
-  const unsigned cbase = (unsigned)j0 * row + (unsigned)min(l, C.ni - 1);
-  const gmem<Tf> *P0 = (const gmem<Tf> *)F.phi[src_f][0];
-  const gmem<Tf> *P1 = (const gmem<Tf> *)F.phi[src_f][1];
-  const gmem<Tf> *R0 = (const gmem<Tf> *)F.rhs[0];
-  const gmem<Tf> *R1 = (const gmem<Tf> *)F.rhs[1];
-  const bool tile_left = l > 0 && rl == 0;
-  const bool tile_right = rl == lpr - 1 && l + 1 < C.ni;
-  const bool px = F.per & 1;
-  const Tf cx = (Tf)F.cx, cy = (Tf)F.cy, cz = (Tf)F.cz;
+  const bool px = F.per & 1, py = F.per & 2, pz = F.per & 4;
+  const unsigned row = (unsigned)H * SZ, plane = row * (unsigned)nj;  // bytes
+  const int j0 = 2 * min(jc, C.nj - 1);  // even: a has colour (r + k) & 1 in row j0 + r
+  const unsigned ent = (unsigned)min(l, C.ni - 1) * SZ;
+  const char *base = (const char *)F.phi[0][0];
+  auto aoff = [&](const void *a) -> unsigned { return (unsigned)((const char *)a - base); };
+  const unsigned aP0 = aoff(F.phi[src_f][0]), dP = aoff(F.phi[src_f][1]) - aP0;
+  const unsigned aR0 = aoff(F.rhs[0]), dR = aoff(F.rhs[1]) - aR0;
+  const gmem<char> *Bs = (const gmem<char> *)base;
+  auto ld = [&](unsigned off) -> Tf { return *(const gmem<Tf> *)(Bs + off); };
+  // colour array (parity & 1) of phi and of rhs, computed (not selected)
+  auto pcol = [=](int par) -> unsigned { return aP0 + (unsigned)(par & 1) * dP; };
+  auto rcol = [=](int par) -> unsigned { return aR0 + (unsigned)(par & 1) * dR; };
   const int ia = 2 * l, ib = ia + 1;
-  auto ldc = [&](int c, unsigned off) -> Tf { return c ? P1[off] : P0[off]; };
-  auto ghost = [&](int x, int y, int z) -> Tf { return level_value<Tf>(F, src_f, x, y, z); };
-  // a and b of row j0 + r at plane k (zeros outside the block: never used)
-  auto own = [&](int r, int k, Tf &a, Tf &b) {
-    if (!INNER && (unsigned)k >= (unsigned)nk) {
-      a = b = Tf(0);
-      return;
-    }
-    const unsigned q = (unsigned)k * plane + cbase + (unsigned)r * row;
-    const int ca = (r + k) & 1;
-    a = ldc(ca, q);
-    b = ldc(ca ^ 1, q);
-  };
-  // x-direction coefficients of a and b (only walls at the x faces)
+  // rows j0-1 and j0+2, ghosts resolved to their image row
+  const int ys = BND ? image(j0 - 1, nj, py) : j0 - 1;
+  const int yn = BND ? image(j0 + 2, nj, py) : j0 + 2;
+  const unsigned o_r0 = (unsigned)j0 * row + ent, o_r1 = o_r0 + row;
+  const unsigned o_s = (unsigned)ys * row + ent, o_n = (unsigned)yn * row + ent;
+  // x neighbours not held by the adjacent lane: x-ghost image or the next
+  // tile's entry
+  const bool w_edge = ia == 0 || (rl == 0 && l > 0);
+  const bool e_edge = ib + 1 >= ni || (rl == lpr - 1 && l + 1 < C.ni);
+  const int xw = image(ia - 1, ni, px), xe = image(ib + 1, ni, px);
+  const unsigned w_ent = w_edge ? (unsigned)(xw >> 1) * SZ : ent;
+  const unsigned e_ent = e_edge ? (unsigned)(xe >> 1) * SZ : ent;
+  const int w_par = w_edge ? (xw & 1) : 0, e_par = e_edge ? (xe & 1) : 0;
+  const Tf cx = (Tf)F.cx, cy = (Tf)F.cy, cz = (Tf)F.cz;
   const Tf aw_a = (ia == 0 && !px) ? Tf(0) : cx;
   const Tf ae_a = (ia == ni - 1 && !px) ? Tf(0) : cx;
   const Tf ae_b = (ib == ni - 1 && !px) ? Tf(0) : cx;
+  const Tf as0 = (BND && !py && j0 == 0) ? Tf(0) : cy;       // row j0 towards j0-1
+  const Tf an1 = (BND && !py && j0 + 1 == nj - 1) ? Tf(0) : cy;  // row j0+1 towards j0+2
 
-  const int k0 = 2 * t.z0, k1 = 2 * t.z1;
-  Tf am[2], bm[2], ac[2], bc[2], ap[2], bp[2];
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    own(r, k0 - 1, am[r], bm[r]);
-    own(r, k0, ac[r], bc[r]);
-  }
-  Tc acc = Tc(0);
-  for (int k = k0; k < k1; ++k) {
-#pragma unroll
-    for (int r = 0; r < 2; ++r) own(r, k + 1, ap[r], bp[r]);
-    const unsigned q0 = (unsigned)k * plane + cbase;
+  struct PlaneSet {
+    Tf fa0, fb0, fa1, fb1;  // rhs of a, b in rows j0, j0+1
+    Tf sa, sb, na, nb;      // rows j0-1 and j0+2
+    Tf wa0, wb0, wa1, wb1;  // window plane k+1 of rows j0, j0+1
+    Tf we0, we1, ee0, ee1;  // x-edge values of rows j0, j0+1
+  };
+  auto next_plane = [&](int k) -> int { return BND ? image(k + 1, nk, pz) : k + 1; };
+  auto load_set = [&](int k) -> PlaneSet {
+    PlaneSet q;
+    const unsigned o = (unsigned)k * plane;
     const int c0 = k & 1;  // colour of a in row j0 (j0 even)
-    // row j0-1 (colour of a there: c0^1) and row j0+2 (colour c0)
-    Tf sa, sb, na, nb;
-    if (INNER || j0 > 0) {
-      sa = ldc(c0 ^ 1, q0 - row);
-      sb = ldc(c0, q0 - row);
-    } else {
-      sa = ghost(ia, -1, k);
-      sb = ghost(ib, -1, k);
-    }
-    if (INNER || j0 + 2 < nj) {
-      na = ldc(c0, q0 + 2 * row);
-      nb = ldc(c0 ^ 1, q0 + 2 * row);
-    } else {
-      na = ghost(ia, nj, k);
-      nb = ghost(ib, nj, k);
-    }
-    Tf fa[2], fb[2];
+    q.fa0 = ld(rcol(c0) + o + o_r0);
+    q.fb0 = ld(rcol(c0 + 1) + o + o_r0);
+    q.fa1 = ld(rcol(c0 + 1) + o + o_r1);
+    q.fb1 = ld(rcol(c0) + o + o_r1);
+    q.sa = ld(pcol(ys + k) + o + o_s);
+    q.sb = ld(pcol(ys + k + 1) + o + o_s);
+    q.na = ld(pcol(yn + k) + o + o_n);
+    q.nb = ld(pcol(yn + k + 1) + o + o_n);
+    const int kp = next_plane(k);
+    const unsigned op = (unsigned)kp * plane;
+    q.wa0 = ld(pcol(kp) + op + o_r0);
+    q.wb0 = ld(pcol(kp + 1) + op + o_r0);
+    q.wa1 = ld(pcol(kp + 1) + op + o_r1);
+    q.wb1 = ld(pcol(kp) + op + o_r1);
+    const unsigned r0 = (unsigned)j0 * row;
+    q.we0 = ld(pcol(w_par + k) + o + r0 + w_ent);
+    q.we1 = ld(pcol(w_par + k + 1) + o + r0 + row + w_ent);
+    q.ee0 = ld(pcol(e_par + k) + o + r0 + e_ent);
+    q.ee1 = ld(pcol(e_par + k + 1) + o + r0 + row + e_ent);
+    return q;
+  };
+  const int k0 = 2 * t.z0;
+  Tf ma0, mb0, ma1, mb1, ca0, cb0, ca1, cb1;  // window planes k-1 and k
+  {
+    const int km = BND ? image(k0 - 1, nk, pz) : k0 - 1;
+    const unsigned om = (unsigned)km * plane, oc = (unsigned)k0 * plane;
+    ma0 = ld(pcol(km) + om + o_r0);
+    mb0 = ld(pcol(km + 1) + om + o_r0);
+    ma1 = ld(pcol(km + 1) + om + o_r1);
+    mb1 = ld(pcol(km) + om + o_r1);
+    ca0 = ld(pcol(k0) + oc + o_r0);
+    cb0 = ld(pcol(k0 + 1) + oc + o_r0);
+    ca1 = ld(pcol(k0 + 1) + oc + o_r1);
+    cb1 = ld(pcol(k0) + oc + o_r1);
+  }
+  PlaneSet cur = load_set(k0);
+  Tc acc = Tc(0);
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int ca = (r + k) & 1;
-      const unsigned q = q0 + (unsigned)r * row;
-      fa[r] = ca ? R1[q] : R0[q];
-      fb[r] = ca ? R0[q] : R1[q];
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int j = j0 + r;
-      const unsigned q = q0 + (unsigned)r * row;
-      const int ca = (r + k) & 1;
+  for (int p = 0; p < NF; ++p) {
+    const int k = k0 + p;
+    PlaneSet nxt;
+    if (p + 1 < NF) nxt = load_set(k + 1);
+    const Tf ab = (BND && !pz && k == 0) ? Tf(0) : cz;
+    const Tf at = (BND && !pz && k == nk - 1) ? Tf(0) : cz;
+    // row j0 (r = 0): S from row j0-1, N = row j0+1 of the window
+    {
+      const Tf Pa = ca0, Pb = cb0;
       // cell 2l-1 is lane l-1's b, cell 2l+2 is lane l+1's a
-      Tf lf = wave_shift(bc[r], 1), rt = wave_shift(ac[r], 0);
-      if (tile_left) lf = ldc(ca ^ 1, q - 1);
-      if (tile_right) rt = ldc(ca, q + 1);
-      const Tf Wa = ia == 0 ? ghost(-1, j, k) : lf;
-      const Tf Ea = bc[r];
-      const Tf Wb = ac[r];
-      const Tf Eb = ib + 1 < ni ? rt : ghost(ni, j, k);
-      const Tf Sa = r ? ac[0] : sa, Sb = r ? bc[0] : sb;
-      const Tf Na = r ? na : ac[1], Nb = r ? nb : bc[1];
-      Tf Ba = am[r], Bb = bm[r], Ta = ap[r], Tb = bp[r];
-      Coef<Tf> c_a, c_b;
-      if (INNER) {
-        c_a.as = c_a.an = c_b.as = c_b.an = cy;
-        c_a.ab = c_a.at = c_b.ab = c_b.at = cz;
-        c_a.aw = aw_a;
-        c_a.ae = ae_a;
-        c_b.aw = cx;
-        c_b.ae = ae_b;
-        c_a.ap = Tf(-1) * (c_a.aw + c_a.ae + c_a.as + c_a.an + c_a.ab + c_a.at);
-        c_b.ap = Tf(-1) * (c_b.aw + c_b.ae + c_b.as + c_b.an + c_b.ab + c_b.at);
-      } else {
-        c_a = cell_coef<Tf>(F, ia, j, k);
-        c_b = cell_coef<Tf>(F, ib, j, k);
-        if (k == 0) {
-          Ba = ghost(ia, j, -1);
-          Bb = ghost(ib, j, -1);
-        }
-        if (k + 1 == nk) {
-          Ta = ghost(ia, j, nk);
-          Tb = ghost(ib, j, nk);
-        }
-      }
-      const Tf Pa = ac[r], Pb = bc[r];
-      const Tf ra = fa[r] - (c_a.aw * Wa + c_a.ae * Ea + c_a.as * Sa + c_a.an * Na +
-                             c_a.ab * Ba + c_a.at * Ta + c_a.ap * Pa);
-      const Tf rb = fb[r] - (c_b.aw * Wb + c_b.ae * Eb + c_b.as * Sb + c_b.an * Nb +
-                             c_b.ab * Bb + c_b.at * Tb + c_b.ap * Pb);
-      acc = (r == 0 && (k & 1) == 0) ? (Tc)ra : acc + (Tc)ra;
+      const Tf lf = wave_shift(Pb, 1), rt = wave_shift(Pa, 0);
+      const Tf Wa = w_edge ? cur.we0 : lf;
+      const Tf Ea = ib < ni ? Pb : cur.ee0;
+      const Tf Eb = e_edge ? cur.ee0 : rt;
+      const Tf ap_a = Tf(-1) * (aw_a + ae_a + as0 + cy + ab + at);
+      const Tf ap_b = Tf(-1) * (cx + ae_b + as0 + cy + ab + at);
+      const Tf ra = cur.fa0 - (aw_a * Wa + ae_a * Ea + as0 * cur.sa + cy * ca1 + ab * ma0 +
+                               at * cur.wa0 + ap_a * Pa);
+      const Tf rb = cur.fb0 - (cx * Pa + ae_b * Eb + as0 * cur.sb + cy * cb1 + ab * mb0 +
+                               at * cur.wb0 + ap_b * Pb);
+      acc = (p & 1) ? acc + (Tc)ra : (Tc)ra;
       acc = acc + (Tc)rb;
-      (void)ca;
     }
-    if (k & 1) {
+    // row j0+1 (r = 1): S = row j0 of the window, N from row j0+2
+    {
+      const Tf Pa = ca1, Pb = cb1;
+      const Tf lf = wave_shift(Pb, 1), rt = wave_shift(Pa, 0);
+      const Tf Wa = w_edge ? cur.we1 : lf;
+      const Tf Ea = ib < ni ? Pb : cur.ee1;
+      const Tf Eb = e_edge ? cur.ee1 : rt;
+      const Tf ap_a = Tf(-1) * (aw_a + ae_a + cy + an1 + ab + at);
+      const Tf ap_b = Tf(-1) * (cx + ae_b + cy + an1 + ab + at);
+      const Tf ra = cur.fa1 - (aw_a * Wa + ae_a * Ea + cy * ca0 + an1 * cur.na + ab * ma1 +
+                               at * cur.wa1 + ap_a * Pa);
+      const Tf rb = cur.fb1 - (cx * Pa + ae_b * Eb + cy * cb0 + an1 * cur.nb + ab * mb1 +
+                               at * cur.wb1 + ap_b * Pb);
+      acc = acc + (Tc)ra;
+      acc = acc + (Tc)rb;
+    }
+    if (p & 1) {
       const int kc = k >> 1;
       if (ok) {
         gmem<Tc> *D = (gmem<Tc> *)C.rhs[(l + jc + kc) & 1];
@@ -667,17 +679,21 @@ __device__ __forceinline__ void restrict_tile(const Tile &t, const LevelDev &C, 
             Tc(0.125) * acc;
       }
     }
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      am[r] = ac[r];
-      bm[r] = bc[r];
-      ac[r] = ap[r];
-      bc[r] = bp[r];
-    }
+    ma0 = ca0;
+    mb0 = cb0;
+    ma1 = ca1;
+    mb1 = cb1;
+    ca0 = cur.wa0;
+    cb0 = cur.wb0;
+    ca1 = cur.wa1;
+    cb1 = cur.wb1;
+    if (p + 1 < NF) cur = nxt;
   }
 }
 
-template <class Tf, class Tc>
+// One launch per coarse level: tiles of KBC coarse planes, kind 0 when every
+// fine row and plane the tile reads is inside the block, kind 3 otherwise.
+template <class Tf, class Tc, int KBC>
 __global__ void __launch_bounds__(NTHREADS)
     k_restrict(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
                const int *__restrict__ active, int src_f) {
@@ -686,9 +702,9 @@ __global__ void __launch_bounds__(NTHREADS)
   if (!active[C.blk]) return;
   const LevelDev &F = levs[C.finer];
   if (t.kind == 0)
-    restrict_tile<Tf, Tc, true>(t, C, F, src_f);
+    restrict_tile_fast<Tf, Tc, KBC, false>(t, C, F, src_f);
   else
-    restrict_tile<Tf, Tc, false>(t, C, F, src_f);
+    restrict_tile_fast<Tf, Tc, KBC, true>(t, C, F, src_f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1793,11 +1809,13 @@ void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, int depth, co
 }
 
 template <class Tf, class Tc>
-void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
                      const int *active, int src_f) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((k_restrict<Tf, Tc>), dim3(ntiles), tile_block(), 0, s, tiles, levs, active,
-                     src_f);
+  auto k = depth == 4   ? k_restrict<Tf, Tc, 4>
+           : depth == 2 ? k_restrict<Tf, Tc, 2>
+                        : k_restrict<Tf, Tc, 1>;
+  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, src_f);
 }
 
 template <class Tf, class Tc>
@@ -1946,11 +1964,11 @@ template void launch_rb_sweep<double>(hipStream_t, const Tile *, int, int, const
                                       const int *, int, int, int);
 template void launch_rb_sweep<float>(hipStream_t, const Tile *, int, int, const LevelDev *,
                                      const int *, int, int, int);
-template void launch_restrict<double, double>(hipStream_t, const Tile *, int, const LevelDev *,
+template void launch_restrict<double, double>(hipStream_t, const Tile *, int, int, const LevelDev *,
                                               const int *, int);
-template void launch_restrict<float, float>(hipStream_t, const Tile *, int, const LevelDev *,
+template void launch_restrict<float, float>(hipStream_t, const Tile *, int, int, const LevelDev *,
                                             const int *, int);
-template void launch_restrict<float, double>(hipStream_t, const Tile *, int, const LevelDev *,
+template void launch_restrict<float, double>(hipStream_t, const Tile *, int, int, const LevelDev *,
                                              const int *, int);
 template void launch_prolong<double, double>(hipStream_t, const Tile *, int, const LevelDev *,
                                              const int *, int, int);

@@ -71,14 +71,19 @@ int lanes_per_row(int H) {
   return l;
 }
 
-// Plane-marching tiles: each workgroup walks kb planes.  kb is chosen so
-// that a level still spreads over >= ~4096 workgroups (16 per CU) when it
-// can; the autotuner may override it per level (h3dmg_set_march).
-int march_depth(int tiles_xy, int nz, long target = 4096) {
-  long kb = (long)tiles_xy * nz / target;
-  if (kb < 1) kb = 1;
-  if (kb > 8) kb = 8;
-  return (int)kb;
+// Planes per workgroup of one plane-marching launch (every tile of a launch
+// has the same depth, the kernels are instantiated per depth): `want` when
+// > 0 (the autotuner's choice), else enough for ~`target` workgroups over
+// `cols` = sum of tile columns x planes; capped at `cap`, then the largest
+// power of two not above it that divides every block's nk in the launch.
+int launch_depth(long cols, long target, int want, int cap, const std::vector<int> &nks) {
+  long kb = want > 0 ? want : std::max<long>(1, cols / target);
+  kb = std::min<long>(kb, cap);
+  int d = 1;
+  while (2 * d <= kb) d *= 2;
+  for (int nk : nks)
+    while (nk % d) d >>= 1;
+  return d;
 }
 
 // Smoother tiles of level descriptor `lev`, in plane-chunk order (the XCD
@@ -116,19 +121,36 @@ void add_tiles_march(TileSet &ts, int lev, const LevelDev &L, int kb) {
 // coarse cells x 4*(64/lpr1) coarse rows x kb coarse planes; kind 0 when every
 // fine row and plane the tile reads is inside the block.  Deeper columns than
 // the smoother's (~2048 workgroups): a column re-reads two fine planes.
-void add_tiles_restrict(TileSet &ts, int lev, const LevelDev &C) {
+int restrict_tiles_xy(const LevelDev &C) {
+  const int xs = C.lpr1, ys = TILE_Y * (64 / C.lpr1);
+  return ((C.ni + xs - 1) / xs) * ((C.nj + ys - 1) / ys);
+}
+
+// kb coarse planes per tile (kb divides C.nk); kind 3 for tiles touching a
+// y/z face
+void add_tiles_restrict(TileSet &ts, int lev, const LevelDev &C, int kb) {
   const int xs = C.lpr1, ys = TILE_Y * (64 / C.lpr1);
   const int nx = C.ni, ny = C.nj, nz = C.nk;
-  const int txy = ((nx + xs - 1) / xs) * ((ny + ys - 1) / ys);
-  const int kb = march_depth(txy, nz, 2048);
   for (int z = 0; z < nz; z += kb) {
-    const int z1 = std::min(z + kb, nz);
+    const int z1 = z + kb;
     for (int y = 0; y < ny; y += ys)
       for (int x = 0; x < nx; x += xs) {
         const bool inner = y >= 1 && y + ys <= ny - 1 && z >= 1 && z1 <= nz - 1;
-        ts.h.push_back(Tile{lev, x, y, z, z1, inner ? 0 : 1});
+        ts.h.push_back(Tile{lev, x, y, z, z1, inner ? 0 : 3});
       }
   }
+}
+
+// One depth for a restriction launch over the coarse descriptors `lev`:
+// ~2048 workgroups, at most 4 coarse planes.
+int restrict_depth(const std::vector<LevelDev> &levs, const std::vector<int> &lev) {
+  long cols = 0;
+  std::vector<int> nks;
+  for (int d : lev) {
+    cols += (long)restrict_tiles_xy(levs[d]) * levs[d].nk;
+    nks.push_back(levs[d].nk);
+  }
+  return launch_depth(cols, 2048, 0, 4, nks);
 }
 
 // Prolongation tiles of fine level descriptor `lev` (k_prolong): fine entries
@@ -281,18 +303,22 @@ void enqueue_vcycle(h3dmg_handle *h) {
       if (h->restr_nc[g].n()) {
         if (f64)
           launch_restrict<double, double>(h->stream, h->restr_nc[g].d, h->restr_nc[g].n(),
-                                          h->d_levs, h->d_active, buf_nc[g - 1]);
+                                          h->restr_nc[g].depth, h->d_levs, h->d_active,
+                                          buf_nc[g - 1]);
         else
           launch_restrict<float, float>(h->stream, h->restr_nc[g].d, h->restr_nc[g].n(),
-                                        h->d_levs, h->d_active, buf_nc[g - 1]);
+                                        h->restr_nc[g].depth, h->d_levs, h->d_active,
+                                        buf_nc[g - 1]);
       }
       if (h->restr_c[g].n()) {
         if (f64)
-          launch_restrict<double, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(), h->d_levs,
-                                          h->d_active, buf_nc[g - 1]);
+          launch_restrict<double, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(),
+                                          h->restr_c[g].depth, h->d_levs, h->d_active,
+                                          buf_nc[g - 1]);
         else
           launch_restrict<float, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(),
-                                         h->d_levs, h->d_active, buf_nc[g - 1]);
+                                         h->restr_c[g].depth, h->d_levs, h->d_active,
+                                         buf_nc[g - 1]);
       }
     }
     if (g == Lt) {
@@ -605,21 +631,31 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       if (g < B.L) {
         add_tiles(h->nat_nc[g], d, L.ni, L.nj, L.nk);
         add_tiles_prolong(g + 1 < B.L ? h->prol_nc[g] : h->prol_c[g], d, L);
-        if (g >= 2) add_tiles_restrict(h->restr_nc[g], d, L);
       } else {
         add_tiles(h->nat_c[g], d, L.ni, L.nj, L.nk);
-        if (g >= 2) add_tiles_restrict(h->restr_c[g], d, L);
       }
+    }
+  // restriction tiles into level g >= 2, one depth per launch group
+  for (int g = 2; g <= Lm; ++g)
+    for (int grp = 0; grp < 2; ++grp) {
+      std::vector<int> members;
+      for (int b = 0; b < nblocks; ++b) {
+        const BlockInfo &B = h->blk[b];
+        if (g <= B.L && (grp == 1) == (g == B.L)) members.push_back(B.desc[g]);
+      }
+      if (members.empty()) continue;
+      TileSet &ts = grp == 0 ? h->restr_nc[g] : h->restr_c[g];
+      ts.depth = restrict_depth(h->levs, members);
+      for (int d : members) add_tiles_restrict(ts, d, h->levs[d], ts.depth);
     }
   // finest level: residual tiles (one colour-array entry of both colours per
   // lane, lpr lanes per row, kb planes; kind 0 = y/z interior) and p += pp
   // tiles (1024 entries each)
   bool fine_general = false;  // a block one cell wide in x: general tiles
-  // One depth for every block (one launch): enough planes for ~2048
-  // workgroups over all blocks, at most 8 -- the tiles are software-pipelined
-  // (the loads of plane k+1 are in flight while plane k is computed), so
-  // depth costs no occupancy and the plane window is re-read (kb+2)/kb
-  // times -- a power of two dividing every block's nk.
+  // One depth for every block (one launch): ~2048 workgroups over all
+  // blocks, at most 8 planes -- the tiles are software-pipelined (the loads
+  // of plane k+1 are in flight while plane k is computed), so depth costs
+  // little occupancy and the plane window is re-read (kb+2)/kb times.
   h->tile_begin.assign(nblocks + 1, 0);
   auto fine_xy = [&](const LevelDev &L, int &xs, int &ys) {
     const int lpr = lanes_per_row(2 * L.H);
@@ -629,19 +665,14 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   };
   {
     long cols = 0;
+    std::vector<int> nks;
     for (int b = 0; b < nblocks; ++b) {
       int xs, ys;
       cols += (long)fine_xy(h->levs[b], xs, ys) * h->levs[b].nk;
+      nks.push_back(h->levs[b].nk);
+      fine_general = fine_general || h->levs[b].ni < 2;
     }
-    const long want = std::min<long>(8, std::max<long>(1, cols / 2048));
-    int kb = want >= 8 ? 8 : want >= 4 ? 4 : want >= 2 ? 2 : 1;
-    bool general = false;
-    for (int b = 0; b < nblocks; ++b) {
-      while (h->levs[b].nk % kb) kb >>= 1;
-      general = general || h->levs[b].ni < 2;
-    }
-    h->fine_tiles.depth = kb;
-    fine_general = general;
+    h->fine_tiles.depth = launch_depth(cols, 2048, 0, 8, nks);
   }
   for (int b = 0; b < nblocks; ++b) {
     const LevelDev &L = h->levs[b];
@@ -813,20 +844,16 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
     // dividing every block's nk; 1 where a level is one cell wide in x
     auto member = [&](const BlockInfo &B) { return B.L >= g && (grp == 1) == (B.L == g); };
     long cols = 0;
-    int want = h->march_kb[g];
     bool general = false;
+    std::vector<int> nks;
     for (const BlockInfo &B : h->blk)
       if (member(B)) {
         const LevelDev &L = h->levs[B.desc[g]];
         cols += (long)march_tiles_xy(L) * L.nk;
+        nks.push_back(L.nk);
         general = general || L.ni < 2;
       }
-    if (want <= 0) want = (int)std::min<long>(4, std::max<long>(1, cols / 4096));
-    int kb = want >= 4 ? 4 : want >= 2 ? 2 : 1;
-    for (const BlockInfo &B : h->blk)
-      if (member(B))
-        while (h->levs[B.desc[g]].nk % kb) kb >>= 1;
-    if (general) kb = 1;
+    const int kb = general ? 1 : launch_depth(cols, 4096, h->march_kb[g], 4, nks);
     ts.depth = kb;
     for (int b = 0; b < h->nb; ++b) {
       const BlockInfo &B = h->blk[b];
@@ -1438,18 +1465,18 @@ h3dmg_status h3dmg_dbg_restrict(h3dmg_handle *h, int level) {
   if (h->restr_nc[level].n()) {
     if (f64)
       launch_restrict<double, double>(h->stream, h->restr_nc[level].d, h->restr_nc[level].n(),
-                                      h->d_levs, h->d_active, 0);
+                                      h->restr_nc[level].depth, h->d_levs, h->d_active, 0);
     else
       launch_restrict<float, float>(h->stream, h->restr_nc[level].d, h->restr_nc[level].n(),
-                                    h->d_levs, h->d_active, 0);
+                                    h->restr_nc[level].depth, h->d_levs, h->d_active, 0);
   }
   if (h->restr_c[level].n()) {
     if (f64)
       launch_restrict<double, double>(h->stream, h->restr_c[level].d, h->restr_c[level].n(),
-                                      h->d_levs, h->d_active, 0);
+                                      h->restr_c[level].depth, h->d_levs, h->d_active, 0);
     else
       launch_restrict<float, double>(h->stream, h->restr_c[level].d, h->restr_c[level].n(),
-                                     h->d_levs, h->d_active, 0);
+                                     h->restr_c[level].depth, h->d_levs, h->d_active, 0);
   }
   // mgrestr zeroes the coarse phi (mgsolver.for:655-662)
   if (h->nat_nc[level].n()) fill_any(h, f64, h->nat_nc[level], 0);
