@@ -392,7 +392,7 @@ def main():
                        "fp64_p99": round(float(np.percentile(lat64, 99)) * 1e3, 3)},
         "pressure_rel_err_vs_converged_fp64": {"mixed": perr, "fp64_at_1e-8": perr64},
         "post_projection_divergence": dv,
-        "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<float> (256^3 colour half-sweep)",
+        "roofline": {"bound": "hbm", "kernel": "k_rb_sweep<float, 4> (256^3 colour half-sweep)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "avg_launch_us": round(ms_pass * 1e3, 2),
