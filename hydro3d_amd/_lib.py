@@ -68,6 +68,12 @@ SIGNATURES = {
     "h3dmg_set_block_enabled": (_i, [_vp, _i, _i]),
     "h3dmg_project_device": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _dp]),
     "h3dmg_get_tail": (_i, [_vp]),
+    "h3dmg_velocity_begin": (_i, [_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _d,
+                                  _dp]),
+    "h3dmg_velocity_step": (_i, [_vp, _dp]),
+    "h3dmg_solve_velocity": (_i, [_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _d,
+                                  _dp, _i, _ip, _dp]),
+    "h3dmg_get_su_device": (_i, [_vp, _i, _vp]),
     "h3dmg_dbg_put": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_get": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_relax": (_i, [_vp, _i, _i]),

@@ -127,8 +127,8 @@ constexpr int CALMAS_MAX_GROUPS = 1024;
 void launch_calvel(hipStream_t s, const FineDev *fd, int axis, const double *us, double *u,
                    double dt_alfa, double h, size_t faces);
 int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const double *v,
-                  const double *w, double *su, double dt, double dx, double dy, double dz,
-                  RedPartial *partials, size_t cells);
+                  const double *w, double *su, int to_fine, double dt, double dx, double dy,
+                  double dz, RedPartial *partials, size_t cells);
 
 // layout conversion: dense interior (i fastest) <-> colour split
 void launch_pack_fine(hipStream_t s, const FineDev *fd, int which, const double *src, int ni,

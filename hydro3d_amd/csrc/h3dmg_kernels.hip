@@ -1683,11 +1683,14 @@ __global__ void __launch_bounds__(256)
 }
 
 // calmas: su = ((du/dx + dv/dy) + dw/dz)/dt per cell, and per workgroup the
-// max of |fact*dx*dy*dz| (rmax before its MPI_ALLREDUCE, press.for:59-73)
+// max of |fact*dx*dy*dz| (rmax before its MPI_ALLREDUCE, press.for:59-73).
+// su goes to the dense array `su` (when not null) or, with to_fine, into the
+// block's colour-split f arrays (the next V-cycle's right-hand side).
 __global__ void __launch_bounds__(256)
     k_calmas(const FineDev *__restrict__ fd, const double *__restrict__ u,
              const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ su,
-             double dt, double dx, double dy, double dz, RedPartial *__restrict__ partials) {
+             int to_fine, double dt, double dx, double dy, double dz,
+             RedPartial *__restrict__ partials) {
   const FineDev &F = *fd;
   const int ni = F.ni, nj = F.nj, nk = F.nk;
   const size_t total = (size_t)ni * nj * nk;
@@ -1700,7 +1703,10 @@ __global__ void __launch_bounds__(256)
     const size_t qw = ((size_t)k * nj + j) * ni + i;
     const double fact = (u[qu + 1] - u[qu]) / dx + (v[qv + ni] - v[qv]) / dy +
                         (w[qw + (size_t)ni * nj] - w[qw]) / dz;
-    if (su) su[q] = fact / dt;
+    if (to_fine)
+      F.f[(i + j + k) & 1][cidx(F.H, F.nj, i, j, k)] = fact / dt;
+    else if (su)
+      su[q] = fact / dt;
     const double fact1 = fact * dx * dy * dz;
     am = fmax(am, fabs(fact1));
   }
@@ -1916,11 +1922,11 @@ void launch_calvel(hipStream_t s, const FineDev *fd, int axis, const double *us,
 }
 
 int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const double *v,
-                  const double *w, double *su, double dt, double dx, double dy, double dz,
-                  RedPartial *partials, size_t cells) {
+                  const double *w, double *su, int to_fine, double dt, double dx, double dy,
+                  double dz, RedPartial *partials, size_t cells) {
   const int grid = (int)std::min<size_t>((cells + 255) / 256, CALMAS_MAX_GROUPS);
-  hipLaunchKernelGGL(k_calmas, dim3(grid), dim3(256), 0, s, fd, u, v, w, su, dt, dx, dy, dz,
-                     partials);
+  hipLaunchKernelGGL(k_calmas, dim3(grid), dim3(256), 0, s, fd, u, v, w, su, to_fine, dt, dx, dy,
+                     dz, partials);
   return grid;
 }
 

@@ -211,7 +211,21 @@ struct h3dmg_handle {
   RedPartial *d_partials = nullptr;
   RedPartial *d_stats = nullptr;
   RedPartial *h_stats = nullptr;  // pinned
-  RedPartial *d_proj = nullptr;   // calmas partial maxima
+  RedPartial *d_proj = nullptr;   // calmas partial maxima, block b at proj_begin[b]
+  std::vector<int> proj_begin;    // per block range in d_proj (nb + 1 entries)
+  int *d_proj_begin = nullptr;
+
+  // velocity-form outer loop (h3dmg_velocity_begin / _step): per block the
+  // device arrays u*, v*, w* and u, v, w, and dt*alfapr
+  struct VelBlock {
+    const double *us = nullptr, *vs = nullptr, *ws = nullptr;
+    double *u = nullptr, *v = nullptr, *w = nullptr;
+  };
+  std::vector<VelBlock> vel;
+  double vel_dta = 0.0;
+  bool vel_ready = false;
+  hipGraph_t vgraph = nullptr;
+  hipGraphExec_t vgexec = nullptr;
 
   // exact coarse solve
   double **d_eig_q = nullptr, **d_eig_l = nullptr, **d_scratch = nullptr;
@@ -720,8 +734,18 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   off = align_up(off + std::max<size_t>(1, h->fine_tiles.h.size()) * sizeof(RedPartial), 256);
   const size_t stats_off = off;
   off = align_up(off + nblocks * sizeof(RedPartial), 256);
+  // calmas partials: one range per block (its workgroup count)
+  h->proj_begin.assign(nblocks + 1, 0);
+  for (int b = 0; b < nblocks; ++b) {
+    const BlockInfo &B = h->blk[b];
+    const size_t cells = (size_t)B.spec.ni * B.spec.nj * B.spec.nk;
+    h->proj_begin[b + 1] =
+        h->proj_begin[b] + (int)std::min<size_t>((cells + 255) / 256, CALMAS_MAX_GROUPS);
+  }
   const size_t proj_off = off;
-  off = align_up(off + CALMAS_MAX_GROUPS * sizeof(RedPartial), 256);
+  off = align_up(off + (size_t)h->proj_begin[nblocks] * sizeof(RedPartial), 256);
+  const size_t pb_off = off;
+  off = align_up(off + (nblocks + 1) * sizeof(int), 256);
   const size_t eptr_off = off;
   off = align_up(off + (size_t)nblocks * 8 * sizeof(double *), 256);
 
@@ -779,6 +803,10 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   h->d_partials = (RedPartial *)(A + part_off);
   h->d_stats = (RedPartial *)(A + stats_off);
   h->d_proj = (RedPartial *)(A + proj_off);
+  h->d_proj_begin = (int *)(A + pb_off);
+  HIP_TRY(hipMemcpy(h->d_proj_begin, h->proj_begin.data(), (nblocks + 1) * sizeof(int),
+                    hipMemcpyHostToDevice));
+  h->vel.assign(nblocks, {});
   HIP_TRY(hipHostMalloc((void **)&h->h_stats, nblocks * sizeof(RedPartial), hipHostMallocDefault));
   if (exact) {
     std::vector<double *> ptrs((size_t)nblocks * 8);
@@ -943,6 +971,77 @@ void drop_graph(h3dmg_handle *h) {
   if (h->graph) (void)hipGraphDestroy(h->graph);
   h->gexec = nullptr;
   h->graph = nullptr;
+  if (h->vgexec) (void)hipGraphExecDestroy(h->vgexec);
+  if (h->vgraph) (void)hipGraphDestroy(h->vgraph);
+  h->vgexec = nullptr;
+  h->vgraph = nullptr;
+}
+
+// calvel + calmas (press.for:77-144, 2-75) of every enabled block with the
+// current pressure: u, v, w from u*, v*, w*; su into the blocks' f arrays
+// (the next V-cycle's rhs); per-block max |div u| dx dy dz into d_stats.
+void enqueue_project(h3dmg_handle *h) {
+  const double dt = h->prm.dt;
+  for (int b = 0; b < h->nb; ++b) {
+    const BlockInfo &B = h->blk[b];
+    if (!B.enabled) continue;
+    const h3dmg_handle::VelBlock &V = h->vel[b];
+    const size_t ni = B.spec.ni, nj = B.spec.nj, nk = B.spec.nk;
+    const FineDev *fd = h->d_fine + b;
+    launch_calvel(h->stream, fd, 0, V.us, V.u, h->vel_dta, B.spec.dx, (ni + 1) * nj * nk);
+    launch_calvel(h->stream, fd, 1, V.vs, V.v, h->vel_dta, B.spec.dy, ni * (nj + 1) * nk);
+    launch_calvel(h->stream, fd, 2, V.ws, V.w, h->vel_dta, B.spec.dz, ni * nj * (nk + 1));
+    launch_calmas(h->stream, fd, V.u, V.v, V.w, nullptr, 1, dt, B.spec.dx, B.spec.dy, B.spec.dz,
+                  h->d_proj + h->proj_begin[b], ni * nj * nk);
+  }
+  launch_finalize(h->stream, h->d_proj_begin, h->nb, h->d_proj, h->d_stats);
+}
+
+// max over the enabled blocks of calmas' rmax (the reference's MPI MAX over
+// blocks and ranks, press.for:72-73, for the blocks of this handle)
+double projected_rmax(const h3dmg_handle *h) {
+  double m = 0.0;
+  for (int b = 0; b < h->nb; ++b)
+    if (h->blk[b].enabled) m = std::max(m, h->h_stats[b].amax);
+  return m;
+}
+
+// One iteration of newsolv_mg's loop (newsolv_mg.for:24-55): mgkcyc from su
+// (level-1 rhs <- f, pp = 0), p = p + (pp - 0), calvel + calmas.
+void enqueue_velocity_step(h3dmg_handle *h) {
+  const bool f64 = !is_mixed(h);
+  const TileSet &ft = h->fine_tiles, &ut = h->update_tiles;
+  if (f64)
+    launch_residual<double>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
+                            h->d_partials, 0);
+  else
+    launch_residual<float>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
+                           h->d_partials, 0);
+  enqueue_vcycle(h);
+  if (f64)
+    launch_update<double>(h->stream, ut.d, ut.n(), h->d_fine, h->d_levs, h->d_active, h->pp_buf);
+  else
+    launch_update<float>(h->stream, ut.d, ut.n(), h->d_fine, h->d_levs, h->d_active, h->pp_buf);
+  enqueue_project(h);
+}
+
+h3dmg_status run_velocity_step(h3dmg_handle *h) {
+  if (h->prm.use_graph) {
+    if (!h->vgexec) {
+      std::lock_guard<std::mutex> lock(g_device_mutex);
+      HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+      enqueue_velocity_step(h);
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamEndCapture(h->stream, &g));
+      h->vgraph = g;
+      HIP_TRY(hipGraphInstantiate(&h->vgexec, h->vgraph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(h->vgexec, h->stream));
+  } else {
+    enqueue_velocity_step(h);
+  }
+  HIP_TRY(hipGetLastError());
+  return H3DMG_OK;
 }
 
 // One outer iteration: V-cycle on the current level-1 rhs, p += pp, new
@@ -1208,6 +1307,17 @@ h3dmg_status h3dmg_get_p(h3dmg_handle *h, int b, double *p, int ld1, int ld2, in
   return H3DMG_OK;
 }
 
+h3dmg_status h3dmg_get_su_device(h3dmg_handle *h, int b, double *d_su) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!d_su) return fail(H3DMG_ERR_ARG, "null su");
+  const FineDev &F = h->fine[b];
+  launch_unpack_fine(h->stream, h->d_fine + b, 1, d_su, F.ni, F.nj, F.nk);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
 h3dmg_status h3dmg_vcycle(h3dmg_handle *h) {
   if (!h) return fail(H3DMG_ERR_ARG, "null handle");
   if (!h->rhs_ready) return fail(H3DMG_ERR_ARG, "rhs not set");
@@ -1355,12 +1465,13 @@ h3dmg_status h3dmg_project_device(h3dmg_handle *h, int b, const double *d_ustar,
   launch_calvel(h->stream, h->d_fine + b, 0, d_ustar, d_u, dta, B.spec.dx, (ni + 1) * nj * nk);
   launch_calvel(h->stream, h->d_fine + b, 1, d_vstar, d_v, dta, B.spec.dy, ni * (nj + 1) * nk);
   launch_calvel(h->stream, h->d_fine + b, 2, d_wstar, d_w, dta, B.spec.dz, ni * nj * (nk + 1));
-  const int groups = launch_calmas(h->stream, h->d_fine + b, d_u, d_v, d_w, d_su, dt, B.spec.dx,
-                                   B.spec.dy, B.spec.dz, h->d_proj, ni * nj * nk);
+  RedPartial *parts = h->d_proj + h->proj_begin[b];
+  const int groups = launch_calmas(h->stream, h->d_fine + b, d_u, d_v, d_w, d_su, 0, dt, B.spec.dx,
+                                   B.spec.dy, B.spec.dz, parts, ni * nj * nk);
   HIP_TRY(hipGetLastError());
   if (rmax) {
     std::vector<RedPartial> part(groups);
-    HIP_TRY(hipMemcpyAsync(part.data(), h->d_proj, groups * sizeof(RedPartial),
+    HIP_TRY(hipMemcpyAsync(part.data(), parts, groups * sizeof(RedPartial),
                            hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     double m = 0.0;
@@ -1369,6 +1480,93 @@ h3dmg_status h3dmg_project_device(h3dmg_handle *h, int b, const double *d_ustar,
   } else {
     HIP_TRY(hipStreamSynchronize(h->stream));
   }
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_velocity_begin(h3dmg_handle *h, const double *const *d_ustar,
+                                  const double *const *d_vstar, const double *const *d_wstar,
+                                  double *const *d_u, double *const *d_v, double *const *d_w,
+                                  double alfapr, double *rmax) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  if (!d_ustar || !d_vstar || !d_wstar || !d_u || !d_v || !d_w)
+    return fail(H3DMG_ERR_ARG, "null velocity array list");
+  if (!(h->prm.dt > 0)) return fail(H3DMG_ERR_ARG, "the velocity form needs params.dt > 0");
+  bool same = h->vel_ready && h->vel_dta == h->prm.dt * alfapr;
+  for (int b = 0; b < h->nb; ++b) {
+    if (!h->blk[b].enabled) continue;
+    if (!d_ustar[b] || !d_vstar[b] || !d_wstar[b] || !d_u[b] || !d_v[b] || !d_w[b])
+      return fail(H3DMG_ERR_ARG, "block %d: null velocity array", b);
+    const h3dmg_handle::VelBlock V{d_ustar[b], d_vstar[b], d_wstar[b], d_u[b], d_v[b], d_w[b]};
+    const h3dmg_handle::VelBlock &O = h->vel[b];
+    same = same && O.us == V.us && O.vs == V.vs && O.ws == V.ws && O.u == V.u && O.v == V.v &&
+           O.w == V.w;
+    h->vel[b] = V;
+  }
+  if (!same) {
+    std::lock_guard<std::mutex> lock(g_device_mutex);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->vgexec) (void)hipGraphExecDestroy(h->vgexec);
+    if (h->vgraph) (void)hipGraphDestroy(h->vgraph);
+    h->vgexec = nullptr;
+    h->vgraph = nullptr;
+  }
+  h->vel_dta = h->prm.dt * alfapr;
+  h->vel_ready = true;
+  reset_active(h);
+  h3dmg_status st = push_active(h);
+  if (st) return st;
+  enqueue_project(h);
+  HIP_TRY(hipGetLastError());
+  st = read_stats(h);
+  if (st) return st;
+  if (rmax) *rmax = projected_rmax(h);
+  h->rhs_ready = true;  // f holds su
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_velocity_step(h3dmg_handle *h, double *rmax) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  if (!h->vel_ready) return fail(H3DMG_ERR_ARG, "h3dmg_velocity_begin has not run");
+  reset_active(h);
+  h3dmg_status st = push_active(h);
+  if (st) return st;
+  st = run_velocity_step(h);
+  if (st) return st;
+  st = read_stats(h);
+  if (st) return st;
+  if (rmax) *rmax = projected_rmax(h);
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_solve_velocity(h3dmg_handle *h, const double *const *d_ustar,
+                                  const double *const *d_vstar, const double *const *d_wstar,
+                                  double *const *d_u, double *const *d_v, double *const *d_w,
+                                  double alfapr, double *hist, int hist_cap, int *ncycles,
+                                  double *rmax) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  double r = 0.0;
+  h3dmg_status st =
+      h3dmg_velocity_begin(h, d_ustar, d_vstar, d_wstar, d_u, d_v, d_w, alfapr, &r);
+  if (st) return st;
+  if (hist && hist_cap > 0) hist[0] = r;
+  int it = 0;
+  bool done = false;
+  while (it < h->prm.maxcy) {
+    st = h3dmg_velocity_step(h, &r);
+    if (st) return st;
+    ++it;
+    if (hist && it < hist_cap) hist[it] = r;
+    if (!std::isfinite(r)) break;
+    if (r < h->prm.tol && it > 1) {  // newsolv_mg.for:57
+      done = true;
+      break;
+    }
+  }
+  if (ncycles) *ncycles = it;
+  if (rmax) *rmax = r;
+  if (!std::isfinite(r)) return fail(H3DMG_ERR_DIVERGED, "non-finite rmax after %d cycles", it);
+  if (r > 10.0) return fail(H3DMG_ERR_DIVERGED, "rmax %g > 10", r);  // newsolv_mg.for:65-75
+  if (!done) return fail(H3DMG_ERR_NOT_CONVERGED, "not converged in %d cycles (rmax %g)", it, r);
   return H3DMG_OK;
 }
 

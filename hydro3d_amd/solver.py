@@ -177,6 +177,61 @@ class MultigridSolver:
                                               ctypes.byref(rmax)))
         return u, v, w, su, rmax.value
 
+    def _vel_lists(self, ustar, vstar, wstar):
+        """Per-block device pointer lists of the velocity-form loop; the
+        outputs u, v, w are allocated like the inputs."""
+        import torch
+        nb = len(self.blocks)
+        ins = [list(x) for x in (ustar, vstar, wstar)]
+        assert all(len(x) == nb for x in ins)
+        outs = [[None if t is None else torch.empty_like(t) for t in x] for x in ins]
+        for x in ins:
+            for t in x:
+                assert t is None or (_is_device_tensor(t) and t.dtype == torch.float64
+                                     and t.is_contiguous())
+        arr = lambda ts: (ctypes.c_void_p * nb)(*[t.data_ptr() if t is not None else 0  # noqa: E731
+                                                  for t in ts])
+        self._vel_keep = (ins, outs)  # the device arrays must outlive the loop
+        return [arr(x) for x in ins], [arr(x) for x in outs], outs
+
+    def solve_velocity(self, ustar, vstar, wstar, alfapr=1.0, raise_on_fail=False):
+        """newsolv_mg's velocity-form loop on the device for every block:
+        ustar/vstar/wstar are per-block lists of device float64 tensors
+        ((nk, nj, ni+1), (nk, nj+1, ni), (nk+1, nj, ni)); eps is `tol`, dt the
+        solver's dt.  Returns (status, ncycles, rmax history, (u, v, w) lists)."""
+        (us, vs, ws), (u, v, w), outs = self._vel_lists(ustar, vstar, wstar)
+        hist = np.zeros(self.maxcy + 1)
+        nc = ctypes.c_int(0)
+        rm = ctypes.c_double(0.0)
+        st = self.lib.h3dmg_solve_velocity(self.h, us, vs, ws, u, v, w, float(alfapr), _dptr(hist),
+                                           hist.size, ctypes.byref(nc), ctypes.byref(rm))
+        if raise_on_fail:
+            L.check(st)
+        else:
+            L.check(st, allow=(L.ERR_NOT_CONVERGED,))
+        return st, nc.value, hist[: nc.value + 1], outs
+
+    def velocity_begin(self, ustar, vstar, wstar, alfapr=1.0):
+        """First calvel + calmas of the velocity loop; returns (rmax, (u, v, w))."""
+        (us, vs, ws), (u, v, w), outs = self._vel_lists(ustar, vstar, wstar)
+        rm = ctypes.c_double(0.0)
+        L.check(self.lib.h3dmg_velocity_begin(self.h, us, vs, ws, u, v, w, float(alfapr),
+                                              ctypes.byref(rm)))
+        return rm.value, outs
+
+    def velocity_step(self):
+        """One iteration of the velocity loop; returns rmax."""
+        rm = ctypes.c_double(0.0)
+        L.check(self.lib.h3dmg_velocity_step(self.h, ctypes.byref(rm)))
+        return rm.value
+
+    def get_su(self, b):
+        """su as the block's rhs arrays hold it, a device tensor (nk, nj, ni)."""
+        import torch
+        out = torch.empty(self._shape(b), dtype=torch.float64, device="cuda")
+        L.check(self.lib.h3dmg_get_su_device(self.h, b, ctypes.c_void_p(out.data_ptr())))
+        return out
+
     def set_block_tol(self, b, tol):
         """REL_L2 tolerance of block b (0: the solver's)."""
         L.check(self.lib.h3dmg_set_block_tol(self.h, b, float(tol)))

@@ -159,6 +159,37 @@ h3dmg_status h3dmg_project_device(h3dmg_handle *h, int b, const double *d_ustar,
                                   double *d_v, double *d_w, double *d_su, double alfapr,
                                   double *rmax);
 
+/* The velocity form of the outer loop on the device, newsolv_mg
+ * (src_v70/newsolv_mg.for:17-59) for every enabled block of the handle:
+ *   calvel; calmas;
+ *   repeat: mgkcyc (pp from su, zero guess); p = p + (pp - 0);
+ *           calvel; calmas;            until rmax < eps and iter > 1
+ * The arrays are lists of nblocks device pointers (layouts as in
+ * h3dmg_project_device; entries of disabled blocks may be null), dt is
+ * params.dt and eps params.tol.  h3dmg_velocity_begin runs the first
+ * calvel + calmas with the current pressure (su goes to the blocks' rhs) and
+ * returns rmax, the max over the handle's blocks (the host takes the MPI MAX
+ * over ranks); each h3dmg_velocity_step runs one iteration and returns the
+ * new rmax.  h3dmg_solve_velocity runs the whole loop on one rank: hist[0]
+ * is the first rmax, hist[i] the rmax after iteration i; the status is
+ * H3DMG_ERR_NOT_CONVERGED after maxcy iterations (newsolv_mg.for:61-62) and
+ * H3DMG_ERR_DIVERGED for rmax > 10 (newsolv_mg.for:65-75).  In FP64 every
+ * step is the reference's arithmetic, so rmax and p are bitwise the
+ * reference loop's.  The final su is h3dmg_get_su_device's. */
+h3dmg_status h3dmg_velocity_begin(h3dmg_handle *h, const double *const *d_ustar,
+                                  const double *const *d_vstar, const double *const *d_wstar,
+                                  double *const *d_u, double *const *d_v, double *const *d_w,
+                                  double alfapr, double *rmax);
+h3dmg_status h3dmg_velocity_step(h3dmg_handle *h, double *rmax);
+h3dmg_status h3dmg_solve_velocity(h3dmg_handle *h, const double *const *d_ustar,
+                                  const double *const *d_vstar, const double *const *d_wstar,
+                                  double *const *d_u, double *const *d_v, double *const *d_w,
+                                  double alfapr, double *hist, int hist_cap, int *ncycles,
+                                  double *rmax);
+/* su of block b as the rhs arrays hold it (after h3dmg_set_rhs*, or the
+ * last calmas of the velocity loop), into a dense device array (nk, nj, ni). */
+h3dmg_status h3dmg_get_su_device(h3dmg_handle *h, int b, double *d_su);
+
 /* Per-block results of the last solve: cycles run and final relative
  * residual (arrays of nblocks). */
 h3dmg_status h3dmg_block_stats(const h3dmg_handle *h, int *cycles, double *rel_res);

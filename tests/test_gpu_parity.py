@@ -129,6 +129,69 @@ def test_gpu_reference_stopping_rule(gpu_lib, name):
     assert np.all(np.abs(hist - ref) <= 1e-10 * ref), (hist, ref)
 
 
+def _velocity_case(name):
+    from hydro3d_amd import configs as C
+    return C.cube(32, C.BC_PPN, 3, seed=0) if name == "newsolv_ppw32.npz" else C.single_64()
+
+
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+@pytest.mark.parametrize("name", ["newsolv_ppw32.npz", "newsolv_ppw64.npz"])
+def test_gpu_velocity_loop_bitwise_vs_reference(gpu_lib, name, graph):
+    """h3dmg_solve_velocity is newsolv_mg's loop on the device (calvel,
+    calmas, then per iteration mgkcyc from su, p = p + pp, calvel, calmas):
+    from the reference's velocity field and p = 0, the rmax of every iteration,
+    the final pressure (ghosts included) and the final su are bitwise the
+    reference build's; with eps = 1e-6 it stops where the reference stops."""
+    from hydro3d_amd import _lib as L
+    z = load_golden(name)
+    c = _velocity_case(name)
+    spec = _blockspec(z)
+    nc = int(z["ncycles"])
+    print(f"case {name} graph={graph}", flush=True)
+    vel = ([_dev(c["u"])], [_dev(c["v"])], [_dev(c["w"])])
+    with _solver(spec, maxcy=nc, tol=0.0, dt=float(z["dt"]), use_graph=graph) as S:
+        S.set_p(0, np.zeros(c["f"].shape))
+        st, ran, hist, _ = S.solve_velocity(*vel)
+        p = S.get_p(0)
+        su = _host(S.get_su(0))
+    assert st == L.ERR_NOT_CONVERGED and ran == nc
+    assert np.array_equal(hist, z["rmax"]), (hist, z["rmax"])
+    pin = p[1:-1, 1:-1, 1:-1]
+    if "p" in z:
+        assert np.array_equal(p, z["p"])
+        assert np.array_equal(su, z["final_su"])
+    # scalar summaries (the 64^3 fixture keeps no arrays): numpy's sum and
+    # norm depend on the array's alignment in their last bits
+    assert abs(pin.sum() - float(z["p_sum"])) <= 1e-14 * abs(float(z["p_sum"])) + 1e-300
+    p_l2 = np.linalg.norm(pin - pin.mean())
+    assert abs(p_l2 - float(z["p_l2_meanfree"])) <= 1e-13 * float(z["p_l2_meanfree"])
+    assert abs(np.linalg.norm(su) - float(z["final_su_l2"])) <= 1e-14 * float(z["final_su_l2"])
+    with _solver(spec, maxcy=nc, tol=1e-6, dt=float(z["dt"]), use_graph=graph) as S:
+        S.set_p(0, np.zeros(c["f"].shape))
+        st, ran, hist, _ = S.solve_velocity(*vel, raise_on_fail=True)
+    assert ran == int(z["stop_at"]) and np.array_equal(hist, z["rmax"][:ran + 1])
+
+
+def test_gpu_velocity_loop_mixed(gpu_lib):
+    """The velocity form with the FP32 V-cycle and exact coarsest solve:
+    each iteration's su is recomputed in FP64 from the corrected velocity, so
+    the loop converges past what the FP32 cycle alone resolves; the final
+    pressure is within 1e-7 of the FP64 loop's converged pressure."""
+    c = _velocity_case("newsolv_ppw64.npz")
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    vel = ([_dev(c["u"])], [_dev(c["v"])], [_dev(c["w"])])
+    out = {}
+    for prec, tol in (("fp64", 1e-13), ("mixed", 1e-11)):
+        kw = dict(precision="mixed", coarse="exact") if prec == "mixed" else {}
+        with _solver(spec, maxcy=60, tol=tol, dt=c["dt"], **kw) as S:
+            S.set_p(0, np.zeros(c["f"].shape))
+            st, ran, hist, _ = S.solve_velocity(*vel, raise_on_fail=True)
+            out[prec] = (ran, hist, mean_free(S.get_p(0)[1:-1, 1:-1, 1:-1]))
+        print(prec, out[prec][0], out[prec][1][-1], flush=True)
+    assert out["mixed"][1][-1] < 1e-11
+    assert rel_l2(out["mixed"][2], out["fp64"][2]) <= 1e-7
+
+
 def test_gpu_single_64_to_tolerance(gpu_lib):
     """64^3, periodic x/y, Neumann z, 4 levels, V(2,2), to 1e-8: same cycle
     count and bitwise the same pressure as the oracle."""
