@@ -74,6 +74,10 @@ SIGNATURES = {
     "h3dmg_solve_velocity": (_i, [_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _d,
                                   _dp, _i, _ip, _dp]),
     "h3dmg_get_su_device": (_i, [_vp, _i, _vp]),
+    "h3dmg_set_velocity": (_i, [_vp, _i, _dp, _dp, _dp, _dp, _dp, _dp, _i, _i, _i]),
+    "h3dmg_get_velocity": (_i, [_vp, _i, _dp, _dp, _dp, _i, _i, _i]),
+    "h3dmg_get_su": (_i, [_vp, _i, _dp, _i, _i, _i]),
+    "h3dmg_set_dt": (_i, [_vp, _d]),
     "h3dmg_dbg_put": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_get": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_relax": (_i, [_vp, _i, _i]),

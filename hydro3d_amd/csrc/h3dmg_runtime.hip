@@ -222,7 +222,12 @@ struct h3dmg_handle {
     double *u = nullptr, *v = nullptr, *w = nullptr;
   };
   std::vector<VelBlock> vel;
-  double vel_dta = 0.0;
+  // handle-owned velocity arrays (h3dmg_set_velocity from host arrays):
+  // per block u*, v*, w*, u, v, w, dense, in one allocation
+  double *vel_store = nullptr;
+  std::vector<size_t> vel_off;   // 6 offsets (doubles) per block
+  std::vector<char> vel_set;     // block's u*, v*, w* uploaded
+  double vel_dta = 0.0, vel_dt = 0.0;  // dt*alfapr and dt of the captured step
   bool vel_ready = false;
   hipGraph_t vgraph = nullptr;
   hipGraphExec_t vgexec = nullptr;
@@ -1098,6 +1103,7 @@ static void destroy_locked(h3dmg_handle *h) {
       if (ts.owned) (void)hipFree(ts.owned);
   if (h->tail_tiles.owned) (void)hipFree(h->tail_tiles.owned);
   if (h->h_stats) (void)hipHostFree(h->h_stats);
+  if (h->vel_store) (void)hipFree(h->vel_store);
   if (h->arena) (void)hipFree(h->arena);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1483,20 +1489,173 @@ h3dmg_status h3dmg_project_device(h3dmg_handle *h, int b, const double *d_ustar,
   return H3DMG_OK;
 }
 
+// Face arrays in the reference's layout (press.for:90-131): u(i,j,k) is the
+// face between cells i and i+1, so dense face f (between 0-based cells f-1
+// and f) is Fortran index isp-1+f, 0-based off-1+f.
+namespace {
+h3dmg_status alloc_velocity(h3dmg_handle *h) {
+  if (h->vel_store) return H3DMG_OK;
+  h->vel_off.assign(6 * (size_t)h->nb, 0);
+  size_t off = 0;
+  for (int b = 0; b < h->nb; ++b) {
+    const h3dmg_block &S = h->blk[b].spec;
+    const size_t n[3] = {(size_t)(S.ni + 1) * S.nj * S.nk, (size_t)S.ni * (S.nj + 1) * S.nk,
+                         (size_t)S.ni * S.nj * (S.nk + 1)};
+    for (int q = 0; q < 6; ++q) {
+      h->vel_off[6 * b + q] = off;
+      off += (n[q % 3] + 31) & ~(size_t)31;
+    }
+  }
+  hipError_t e = hipMalloc((void **)&h->vel_store, off * sizeof(double));
+  if (e != hipSuccess) {
+    h->vel_store = nullptr;
+    return fail(H3DMG_ERR_NOMEM, "velocity arrays (%zu bytes): %s", off * 8, hipGetErrorString(e));
+  }
+  h->bytes += off * sizeof(double);
+  h->vel_set.assign(h->nb, 0);
+  return H3DMG_OK;
+}
+
+// 0-based flat index of Fortran element (off+x, off+y, off+z) shifted by one
+// face on `axis` (x, y, z interior-relative, face f maps to f-1 on its axis)
+inline size_t face_index(int axis, int f, int a, int c, int ld1, int ld2, int off) {
+  int x, y, z;
+  if (axis == 0) {
+    x = off - 1 + f; y = off + a; z = off + c;
+  } else if (axis == 1) {
+    x = off + a; y = off - 1 + f; z = off + c;
+  } else {
+    x = off + a; y = off + c; z = off - 1 + f;
+  }
+  return (size_t)x + (size_t)ld1 * ((size_t)y + (size_t)ld2 * (size_t)z);
+}
+
+// dense index of face f of `axis` with the other two coordinates (a, c) in
+// (i, j, k) order minus the face axis
+inline size_t dense_face(int axis, int f, int a, int c, const h3dmg_block &S) {
+  if (axis == 0) return ((size_t)c * S.nj + a) * (S.ni + 1) + f;        // (k, j, f)
+  if (axis == 1) return ((size_t)c * (S.nj + 1) + f) * S.ni + a;        // (k, f, i)
+  return ((size_t)f * S.nj + c) * S.ni + a;                             // (f, j, i)
+}
+
+h3dmg_status check_face_layout(const h3dmg_block &S, int ld1, int ld2, int off) {
+  if (off < 1 || ld1 < S.ni + off + 1 || ld2 < S.nj + off + 1)
+    return fail(H3DMG_ERR_ARG, "bad host face array description (needs one ghost layer)");
+  return H3DMG_OK;
+}
+}  // namespace
+
+h3dmg_status h3dmg_set_velocity(h3dmg_handle *h, int b, const double *ustar, const double *vstar,
+                                const double *wstar, const double *u, const double *v,
+                                const double *w, int ld1, int ld2, int off) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!ustar || !vstar || !wstar) return fail(H3DMG_ERR_ARG, "null u*, v* or w*");
+  const h3dmg_block &S = h->blk[b].spec;
+  st = check_face_layout(S, ld1, ld2, off);
+  if (st) return st;
+  st = alloc_velocity(h);
+  if (st) return st;
+  const double *star[3] = {ustar, vstar, wstar}, *cur[3] = {u, v, w};
+  const int n[3] = {S.ni, S.nj, S.nk};
+  for (int axis = 0; axis < 3; ++axis) {
+    const bool per = (h->blk[b].per >> axis) & 1;
+    // the two axes that are not `axis`, in (i, j, k) order
+    const int na = axis == 0 ? S.nj : S.ni, nc = axis == 2 ? S.nj : S.nk;
+    std::vector<double> dense((size_t)(n[axis] + 1) * na * nc);
+    for (int c = 0; c < nc; ++c)
+      for (int a = 0; a < na; ++a)
+        for (int f = 0; f <= n[axis]; ++f) {
+          // faces calvel corrects take u*, the others keep the current u
+          const bool corrected = (f > 0 && f < n[axis]) || per;
+          const double *src = corrected || !cur[axis] ? star[axis] : cur[axis];
+          dense[dense_face(axis, f, a, c, S)] = src[face_index(axis, f, a, c, ld1, ld2, off)];
+        }
+    HIP_TRY(hipMemcpyAsync(h->vel_store + h->vel_off[6 * b + axis], dense.data(),
+                           dense.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  h->vel_set[b] = 1;
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_get_velocity(h3dmg_handle *h, int b, double *u, double *v, double *w, int ld1,
+                                int ld2, int off) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!u || !v || !w) return fail(H3DMG_ERR_ARG, "null u, v or w");
+  const h3dmg_block &S = h->blk[b].spec;
+  st = check_face_layout(S, ld1, ld2, off);
+  if (st) return st;
+  if (!h->vel_store || !h->vel_set[b])
+    return fail(H3DMG_ERR_ARG, "block %d: no velocity on the device", b);
+  double *dst[3] = {u, v, w};
+  const int n[3] = {S.ni, S.nj, S.nk};
+  for (int axis = 0; axis < 3; ++axis) {
+    const int na = axis == 0 ? S.nj : S.ni, nc = axis == 2 ? S.nj : S.nk;
+    std::vector<double> dense((size_t)(n[axis] + 1) * na * nc);
+    HIP_TRY(hipMemcpyAsync(dense.data(), h->vel_store + h->vel_off[6 * b + 3 + axis],
+                           dense.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < nc; ++c)
+      for (int a = 0; a < na; ++a)
+        for (int f = 0; f <= n[axis]; ++f)
+          dst[axis][face_index(axis, f, a, c, ld1, ld2, off)] = dense[dense_face(axis, f, a, c, S)];
+  }
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_get_su(h3dmg_handle *h, int b, double *su, int ld1, int ld2, int off) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  const h3dmg_block &S = h->blk[b].spec;
+  if (!su || off < 0 || ld1 < S.ni + off || ld2 < S.nj + off)
+    return fail(H3DMG_ERR_ARG, "bad host array description");
+  const FineDev &F = h->fine[b];
+  launch_unpack_fine(h->stream, h->d_fine + b, 1, h->d_stage, F.ni, F.nj, F.nk);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> dense((size_t)S.ni * S.nj * S.nk);
+  HIP_TRY(hipMemcpyAsync(dense.data(), h->d_stage, dense.size() * 8, hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int k = 0; k < S.nk; ++k)
+    for (int j = 0; j < S.nj; ++j)
+      std::memcpy(su + (size_t)off + (size_t)ld1 * ((size_t)(j + off) + (size_t)ld2 * (k + off)),
+                  &dense[((size_t)k * S.nj + j) * S.ni], sizeof(double) * S.ni);
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_set_dt(h3dmg_handle *h, double dt) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  if (!(dt > 0)) return fail(H3DMG_ERR_ARG, "dt must be > 0");
+  h->prm.dt = dt;  // the velocity step's graph is re-captured on the next begin
+  return H3DMG_OK;
+}
+
 h3dmg_status h3dmg_velocity_begin(h3dmg_handle *h, const double *const *d_ustar,
                                   const double *const *d_vstar, const double *const *d_wstar,
                                   double *const *d_u, double *const *d_v, double *const *d_w,
                                   double alfapr, double *rmax) {
   if (!h) return fail(H3DMG_ERR_ARG, "null handle");
-  if (!d_ustar || !d_vstar || !d_wstar || !d_u || !d_v || !d_w)
+  const bool own = !d_ustar && !d_vstar && !d_wstar && !d_u && !d_v && !d_w;
+  if (!own && (!d_ustar || !d_vstar || !d_wstar || !d_u || !d_v || !d_w))
     return fail(H3DMG_ERR_ARG, "null velocity array list");
   if (!(h->prm.dt > 0)) return fail(H3DMG_ERR_ARG, "the velocity form needs params.dt > 0");
-  bool same = h->vel_ready && h->vel_dta == h->prm.dt * alfapr;
+  bool same = h->vel_ready && h->vel_dta == h->prm.dt * alfapr && h->vel_dt == h->prm.dt;
   for (int b = 0; b < h->nb; ++b) {
     if (!h->blk[b].enabled) continue;
-    if (!d_ustar[b] || !d_vstar[b] || !d_wstar[b] || !d_u[b] || !d_v[b] || !d_w[b])
-      return fail(H3DMG_ERR_ARG, "block %d: null velocity array", b);
-    const h3dmg_handle::VelBlock V{d_ustar[b], d_vstar[b], d_wstar[b], d_u[b], d_v[b], d_w[b]};
+    h3dmg_handle::VelBlock V;
+    if (own) {
+      if (!h->vel_store || !h->vel_set[b])
+        return fail(H3DMG_ERR_ARG, "block %d: h3dmg_set_velocity has not run", b);
+      const size_t *o = &h->vel_off[6 * b];
+      double *S = h->vel_store;
+      V = h3dmg_handle::VelBlock{S + o[0], S + o[1], S + o[2], S + o[3], S + o[4], S + o[5]};
+    } else {
+      if (!d_ustar[b] || !d_vstar[b] || !d_wstar[b] || !d_u[b] || !d_v[b] || !d_w[b])
+        return fail(H3DMG_ERR_ARG, "block %d: null velocity array", b);
+      V = h3dmg_handle::VelBlock{d_ustar[b], d_vstar[b], d_wstar[b], d_u[b], d_v[b], d_w[b]};
+    }
     const h3dmg_handle::VelBlock &O = h->vel[b];
     same = same && O.us == V.us && O.vs == V.vs && O.ws == V.ws && O.u == V.u && O.v == V.v &&
            O.w == V.w;
@@ -1511,6 +1670,7 @@ h3dmg_status h3dmg_velocity_begin(h3dmg_handle *h, const double *const *d_ustar,
     h->vgraph = nullptr;
   }
   h->vel_dta = h->prm.dt * alfapr;
+  h->vel_dt = h->prm.dt;
   h->vel_ready = true;
   reset_active(h);
   h3dmg_status st = push_active(h);

@@ -9,8 +9,15 @@
 !     lambda = sum_a 2 (cos(theta_a) - 1) / h^2,
 !     theta = (2 pi/n, 2 pi/n, pi/n).
 ! The demo solves it in FP64 and in mixed precision and checks the pressure
-! against that closed form and the residual it computes itself.  Prints
-! "h3dmg fortran demo: OK" and exits 0 on success, stops with code 1 otherwise.
+! against that closed form and the residual it computes itself.
+!
+! It then runs the velocity-form loop of newsolv_mg on the device from face
+! arrays in Hydro3D's layout (u(i,j,k) on the face between cells i and i+1,
+! two ghost layers): a smooth u*, v*, w* (w* = 0 on the z walls), p = 0,
+! until rmax < 1e-10.  On the host it recomputes calvel from the returned p
+! and u* and calmas from the returned u, v, w, and checks them against the
+! returned u, v, w, su and rmax.  Prints "h3dmg fortran demo: OK" and exits 0
+! on success, stops with code 1 otherwise.
 program h3dmg_fortran_demo
   use, intrinsic :: iso_c_binding
   use h3dmg_mod
@@ -77,6 +84,7 @@ program h3dmg_fortran_demo
           ncyc, ' final ', fin, ' |p-pex|/|pex| ', err, ' |f-Ap|/|f| ', res / fn
      if (err > 1.0d-8 .or. res / fn > 1.0d-9) ok = .false.
   end do
+  call velocity_form(ok)
   if (.not. ok) then
      write (*, '(a)') 'h3dmg fortran demo: FAILED'
      stop 1
@@ -84,6 +92,110 @@ program h3dmg_fortran_demo
   write (*, '(a)') 'h3dmg fortran demo: OK'
 
 contains
+
+  subroutine velocity_form(ok)
+    logical, intent(inout) :: ok
+    real(c_double), allocatable :: us(:, :, :), vs(:, :, :), ws(:, :, :)
+    real(c_double), allocatable :: u(:, :, :), v(:, :, :), w(:, :, :), pv(:, :, :), su(:, :, :)
+    real(c_double), parameter :: dt = 1.0d-3, eps = 1.0d-10
+    real(c_double) :: r, x, y, z, eu, es, smax, rhost, fact, umax
+    integer(c_int) :: it
+    integer :: ii, jj, kk
+    allocate (us(ld, ld, ld), vs(ld, ld, ld), ws(ld, ld, ld), u(ld, ld, ld), v(ld, ld, ld), &
+              w(ld, ld, ld), pv(ld, ld, ld), su(ld, ld, ld))
+    us = 0.0d0; vs = 0.0d0; ws = 0.0d0
+    ! faces pl..pl+n on each axis: index pl+f is the face between cells f and
+    ! f+1 (cells pl+1..pl+n); periodic x, y; z walls at faces pl and pl+n
+    do k = 1, n
+       do j = 1, n
+          do i = 0, n
+             x = i * hh; y = (j - 0.5d0) * hh; z = (k - 0.5d0) * hh
+             us(pl + i, pl + j, pl + k) = sin(2.0d0 * pi * x) * cos(2.0d0 * pi * y) + z
+             vs(pl + j, pl + i, pl + k) = cos(2.0d0 * pi * x) * z * z
+          end do
+       end do
+    end do
+    do k = 1, n - 1
+       do j = 1, n
+          do i = 1, n
+             ws(pl + i, pl + j, pl + k) = sin(pi * k * hh) * cos(2.0d0 * pi * (i - 0.5d0) * hh)
+          end do
+       end do
+    end do
+    u = us; v = vs; w = ws
+    call h3dmg_default_params(prm)
+    prm%dt = dt
+    prm%tol = eps
+    prm%maxcy = 60
+    st = h3dmg_create(blk, 1, prm, h)
+    call check(st, 'create')
+    pv = 0.0d0
+    st = h3dmg_set_p(h, 0, pv, ld, ld, pl)
+    call check(st, 'set_p')
+    st = h3dmg_set_velocity(h, 0, us, vs, ws, u, v, w, ld, ld, pl)
+    call check(st, 'set_velocity')
+    st = h3dmg_velocity_begin(h, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                              c_null_ptr, 1.0d0, r)
+    call check(st, 'velocity_begin')
+    it = 0
+    do while (it < prm%maxcy)
+       st = h3dmg_velocity_step(h, r)
+       call check(st, 'velocity_step')
+       it = it + 1
+       if (r < eps .and. it > 1) exit
+    end do
+    st = h3dmg_get_p(h, 0, pv, ld, ld, pl)
+    call check(st, 'get_p')
+    st = h3dmg_get_velocity(h, 0, u, v, w, ld, ld, pl)
+    call check(st, 'get_velocity')
+    st = h3dmg_get_su(h, 0, su, ld, ld, pl)
+    call check(st, 'get_su')
+    call h3dmg_destroy(h)
+    ! calvel on the host: corrected faces of x and y (all, periodic) and the
+    ! interior z faces; the z-wall faces keep w*
+    eu = 0.0d0
+    umax = 0.0d0
+    do kk = pl + 1, pl + n
+       do jj = pl + 1, pl + n
+          do ii = pl, pl + n
+             if (ii > pl) eu = max(eu, abs(u(ii, jj, kk) - (us(ii, jj, kk) - dt * &
+                  (pv(ii + 1, jj, kk) - pv(ii, jj, kk)) / hh)))
+             umax = max(umax, abs(u(ii, jj, kk)))
+          end do
+       end do
+    end do
+    do kk = pl, pl + n
+       do jj = pl + 1, pl + n
+          do ii = pl + 1, pl + n
+             if (kk == pl .or. kk == pl + n) then
+                eu = max(eu, abs(w(ii, jj, kk) - ws(ii, jj, kk)))
+             else
+                eu = max(eu, abs(w(ii, jj, kk) - (ws(ii, jj, kk) - dt * &
+                     (pv(ii, jj, kk + 1) - pv(ii, jj, kk)) / hh)))
+             end if
+          end do
+       end do
+    end do
+    ! calmas on the host
+    es = 0.0d0
+    smax = 0.0d0
+    rhost = 0.0d0
+    do kk = pl + 1, pl + n
+       do jj = pl + 1, pl + n
+          do ii = pl + 1, pl + n
+             fact = (u(ii, jj, kk) - u(ii - 1, jj, kk)) / hh + (v(ii, jj, kk) - v(ii, jj - 1, kk)) / hh + &
+                    (w(ii, jj, kk) - w(ii, jj, kk - 1)) / hh
+             es = max(es, abs(su(ii, jj, kk) - fact / dt))
+             smax = max(smax, abs(fact / dt))
+             rhost = max(rhost, abs(fact * hh**3))
+          end do
+       end do
+    end do
+    write (*, '(a,i3,a,es10.3,a,es10.3,a,es10.3,a,es10.3)') ' velocity form: cycles', it, &
+         ' rmax ', r, ' host rmax ', rhost, ' |u-calvel| ', eu, ' |su-calmas| ', es
+    if (r >= eps .or. abs(rhost - r) > 1.0d-6 * r + 1.0d-20 .or. eu > 1.0d-12 * umax .or. &
+        es > 1.0d-9 * max(smax, 1.0d0)) ok = .false.
+  end subroutine velocity_form
 
   subroutine check(st, what)
     integer(c_int), intent(in) :: st

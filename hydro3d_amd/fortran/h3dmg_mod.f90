@@ -41,6 +41,9 @@ module h3dmg_mod
        h3dmg_solve, h3dmg_block_stats, h3dmg_remove_mean, h3dmg_set_stream, &
        h3dmg_set_march, h3dmg_get_march, h3dmg_set_tail, h3dmg_get_tail, &
        h3dmg_set_block_tol, h3dmg_set_block_enabled, h3dmg_project_device, &
+       h3dmg_velocity_begin, h3dmg_velocity_step, h3dmg_solve_velocity, &
+       h3dmg_get_su_device, h3dmg_set_velocity, h3dmg_get_velocity, h3dmg_get_su, &
+       h3dmg_set_dt, &
        h3dmg_error_message
 
   interface
@@ -152,6 +155,68 @@ module h3dmg_mod
        real(c_double), value :: alfapr
        real(c_double), intent(out) :: rmax
      end function h3dmg_project_device
+
+     ! velocity-form loop (newsolv_mg.for:17-59) on the device; the six list
+     ! arguments are c_null_ptr to use the arrays of h3dmg_set_velocity
+     integer(c_int) function h3dmg_velocity_begin(handle, ustar, vstar, wstar, u, v, w, alfapr, &
+          rmax) bind(C, name="h3dmg_velocity_begin")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: handle, ustar, vstar, wstar, u, v, w
+       real(c_double), value :: alfapr
+       real(c_double), intent(out) :: rmax
+     end function h3dmg_velocity_begin
+
+     integer(c_int) function h3dmg_velocity_step(handle, rmax) bind(C, name="h3dmg_velocity_step")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: handle
+       real(c_double), intent(out) :: rmax
+     end function h3dmg_velocity_step
+
+     integer(c_int) function h3dmg_solve_velocity(handle, ustar, vstar, wstar, u, v, w, alfapr, &
+          hist, hist_cap, ncycles, rmax) bind(C, name="h3dmg_solve_velocity")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: handle, ustar, vstar, wstar, u, v, w
+       real(c_double), value :: alfapr
+       real(c_double), intent(inout) :: hist(*)
+       integer(c_int), value :: hist_cap
+       integer(c_int), intent(out) :: ncycles
+       real(c_double), intent(out) :: rmax
+     end function h3dmg_solve_velocity
+
+     integer(c_int) function h3dmg_get_su_device(handle, b, su) bind(C, name="h3dmg_get_su_device")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: handle, su
+       integer(c_int), value :: b
+     end function h3dmg_get_su_device
+
+     integer(c_int) function h3dmg_set_velocity(handle, b, ustar, vstar, wstar, u, v, w, ld1, &
+          ld2, off) bind(C, name="h3dmg_set_velocity")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: handle
+       integer(c_int), value :: b, ld1, ld2, off
+       real(c_double), intent(in) :: ustar(*), vstar(*), wstar(*), u(*), v(*), w(*)
+     end function h3dmg_set_velocity
+
+     integer(c_int) function h3dmg_get_velocity(handle, b, u, v, w, ld1, ld2, off) &
+          bind(C, name="h3dmg_get_velocity")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: handle
+       integer(c_int), value :: b, ld1, ld2, off
+       real(c_double), intent(inout) :: u(*), v(*), w(*)
+     end function h3dmg_get_velocity
+
+     integer(c_int) function h3dmg_get_su(handle, b, su, ld1, ld2, off) bind(C, name="h3dmg_get_su")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: handle
+       integer(c_int), value :: b, ld1, ld2, off
+       real(c_double), intent(inout) :: su(*)
+     end function h3dmg_get_su
+
+     integer(c_int) function h3dmg_set_dt(handle, dt) bind(C, name="h3dmg_set_dt")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: handle
+       real(c_double), value :: dt
+     end function h3dmg_set_dt
 
      integer(c_int) function h3dmg_set_tail(handle, level) bind(C, name="h3dmg_set_tail")
        import :: c_ptr, c_int

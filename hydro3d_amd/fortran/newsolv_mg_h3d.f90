@@ -3,26 +3,24 @@
 !
 ! Use: compile this file and h3dmg_mod.f90 with the reference's modules, link
 ! libh3dmg.so, and call newsolv_mg_h3d where src_v70/flosol.for:215 calls
-! newsolv_mg.  Nothing else in the LES driver changes: the host steps of the
-! reference routine stay on the host -- calvel / calmas (press.for:2-144),
-! exchange(4), pbound, boundu/v/w and the log lines -- and only the cycle
-! loop (newsolv_mg.for:24-59) moves to the device.
+! newsolv_mg.  Nothing else in the LES driver changes.
 !
-! Mapping.  The reference iterates p <- p + mgkcyc(su) with su = div(u)/dt
-! recomputed from the corrected velocity after every cycle
-! (newsolv_mg.for:26-55); div(u* - dt grad p)/dt = f - A p, with A the
-! 7-point operator of mgsolver.for (calvel excludes the wall-normal faces,
-! localparameters.for:187-192, which makes div(grad) equal to A with Neumann
-! walls).  So one device solve of A pp = su0 from pp = 0, with
-! su0 = calmas(calvel(p_old)), under the reference criterion (rmax < eps and
-! iter > 1, rmax = max |r| dt dx dy dz) gives p = p_old + pp, after which
-! the host recomputes u and rmax exactly as the reference ends its loop.
+! The reference's loop itself runs on the device (h3dmg_velocity_begin /
+! h3dmg_velocity_step): calvel, calmas, then per iteration mgkcyc from su,
+! p = p + pp, calvel, calmas, in the reference's arithmetic.  u*, v*, w*,
+! u, v, w and p go to the device once per call; each iteration returns this
+! rank's rmax, which is reduced over the ranks here as calmas reduces it
+! (press.for:72-73); u, v, w, su and p come back once.  The host then fills
+! the remaining ghost layers as the reference does after its loop
+! (exchange(4), pbound, and calvel's exchange(1..3)), and keeps the
+! not-converged / BIG RMAX handling, boundu/v/w and the log lines.
 !
 ! Scope: the constant-coefficient RBGS multigrid on blocks without
 ! neighbours (every face a wall or periodic onto itself), one MPI rank per
 ! GPU.  Anything else stops with a message rather than running a different
-! algorithm.  The precision is FP64 unless the environment variable
-! H3DMG_PRECISION=mixed selects the FP32 V-cycle inside FP64 refinement.
+! algorithm.  The precision is FP64 (bitwise the reference loop) unless the
+! environment variable H3DMG_PRECISION=mixed selects the FP32 V-cycle with
+! the exact coarsest solve inside the FP64 velocity loop.
 subroutine newsolv_mg_h3d
   use vars
   use mpi
@@ -31,53 +29,63 @@ subroutine newsolv_mg_h3d
   use, intrinsic :: iso_c_binding
   implicit none
   type(c_ptr), save :: h = c_null_ptr
-  real(c_double), allocatable, save :: hist(:)
   integer :: ib
-  integer(c_int) :: st, ncyc
-  real(c_double) :: fin
+  integer(c_int) :: st
+  real(c_double) :: rloc
   logical :: converged
   double precision :: wtimedum
 
   if (L_LSM) call h3d_stop('the variable-density (LSM) operator is not on the device path')
   if (.not. c_associated(h)) call setup()
+  st = h3dmg_set_dt(h, dt)
+  call chk(st, 'set_dt')
 
-  ! su0 = f - A p_old from the current pressure (newsolv_mg.for:17-18)
-  call calvel
-  call calmas
   do ib = 1, nbp
-     dom(ib)%pp = 0.0d0
-     st = h3dmg_set_rhs(h, ib - 1, dom(ib)%su, dom(ib)%ttc_i, dom(ib)%ttc_j, dom(ib)%isp - 1)
-     call chk(st, 'set_rhs')
-     st = h3dmg_set_p(h, ib - 1, dom(ib)%pp, dom(ib)%ttc_i, dom(ib)%ttc_j, dom(ib)%isp - 1)
-     call chk(st, 'set_p')
+     associate (d => dom(ib))
+       st = h3dmg_set_velocity(h, ib - 1, d%ustar, d%vstar, d%wstar, d%u, d%v, d%w, &
+                               d%ttc_i, d%ttc_j, d%isp - 1)
+       call chk(st, 'set_velocity')
+       st = h3dmg_set_p(h, ib - 1, d%p, d%ttc_i, d%ttc_j, d%isp - 1)
+       call chk(st, 'set_p')
+       d%pp = 0.0d0
+     end associate
   end do
 
-  st = h3dmg_solve(h, hist, size(hist, kind=c_int), ncyc, fin)
-  converged = st == H3DMG_OK
-  if (st /= H3DMG_OK .and. st /= H3DMG_ERR_NOT_CONVERGED .and. st /= H3DMG_ERR_DIVERGED) &
-       call chk(st, 'solve')
+  ! calvel; calmas (newsolv_mg.for:17-18)
+  st = h3dmg_velocity_begin(h, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                            c_null_ptr, alfapr, rloc)
+  call chk(st, 'velocity_begin')
+  call MPI_ALLREDUCE(rloc, rmax, 1, MPI_DOUBLE_PRECISION, MPI_MAX, MPI_COMM_WORLD, ierr)
 
-  ! p = p + pp over the interior and one ghost layer (newsolv_mg.for:39-49,
-  ! ppref = 0)
+  ! newsolv_mg.for:24-59
+  converged = .false.
+  do iter = 1, maxcy
+     st = h3dmg_velocity_step(h, rloc)
+     call chk(st, 'velocity_step')
+     call MPI_ALLREDUCE(rloc, rmax, 1, MPI_DOUBLE_PRECISION, MPI_MAX, MPI_COMM_WORLD, ierr)
+     if (rmax < eps .and. iter > 1) then
+        converged = .true.
+        exit
+     end if
+  end do
+
   do ib = 1, nbp
-     st = h3dmg_get_p(h, ib - 1, dom(ib)%pp, dom(ib)%ttc_i, dom(ib)%ttc_j, dom(ib)%isp - 1)
-     call chk(st, 'get_p')
      associate (d => dom(ib))
-       d%p(d%isp - 1:d%iep + 1, d%jsp - 1:d%jep + 1, d%ksp - 1:d%kep + 1) = &
-            d%p(d%isp - 1:d%iep + 1, d%jsp - 1:d%jep + 1, d%ksp - 1:d%kep + 1) + &
-            d%pp(d%isp - 1:d%iep + 1, d%jsp - 1:d%jep + 1, d%ksp - 1:d%kep + 1)
-       d%pp = 0.0d0
+       st = h3dmg_get_p(h, ib - 1, d%p, d%ttc_i, d%ttc_j, d%isp - 1)
+       call chk(st, 'get_p')
+       st = h3dmg_get_velocity(h, ib - 1, d%u, d%v, d%w, d%ttc_i, d%ttc_j, d%isp - 1)
+       call chk(st, 'get_velocity')
+       st = h3dmg_get_su(h, ib - 1, d%su, d%ttc_i, d%ttc_j, d%isp - 1)
+       call chk(st, 'get_su')
      end associate
   end do
   call exchange(4)
   call pbound
-  call calvel
-  call calmas
+  call exchange(1)
+  call exchange(2)
+  call exchange(3)
 
-  if (converged) then
-     iter = ncyc
-  else
-     iter = maxcy + 1
+  if (.not. converged) then
      if (myrank == 0) print *, 'not converged!! ', maxcy, rmax
      if (myrank == 0) write (numfile, *) 'not converged!! ', maxcy, rmax
   end if
@@ -146,7 +154,6 @@ contains
     end if
     st = h3dmg_create(blk, int(nbp, c_int), prm, h)
     call chk(st, 'create')
-    allocate (hist(0:maxcy))
   end subroutine setup
 
   ! a block whose only neighbour on a periodic axis is itself

@@ -194,12 +194,18 @@ class MultigridSolver:
         self._vel_keep = (ins, outs)  # the device arrays must outlive the loop
         return [arr(x) for x in ins], [arr(x) for x in outs], outs
 
-    def solve_velocity(self, ustar, vstar, wstar, alfapr=1.0, raise_on_fail=False):
+    def solve_velocity(self, ustar=None, vstar=None, wstar=None, alfapr=1.0,
+                       raise_on_fail=False):
         """newsolv_mg's velocity-form loop on the device for every block:
         ustar/vstar/wstar are per-block lists of device float64 tensors
-        ((nk, nj, ni+1), (nk, nj+1, ni), (nk+1, nj, ni)); eps is `tol`, dt the
-        solver's dt.  Returns (status, ncycles, rmax history, (u, v, w) lists)."""
-        (us, vs, ws), (u, v, w), outs = self._vel_lists(ustar, vstar, wstar)
+        ((nk, nj, ni+1), (nk, nj+1, ni), (nk+1, nj, ni)), or None for the
+        arrays set_velocity uploaded; eps is `tol`, dt the solver's dt.
+        Returns (status, ncycles, rmax history, (u, v, w) lists or None)."""
+        if ustar is None:
+            us = vs = ws = u = v = w = None
+            outs = None
+        else:
+            (us, vs, ws), (u, v, w), outs = self._vel_lists(ustar, vstar, wstar)
         hist = np.zeros(self.maxcy + 1)
         nc = ctypes.c_int(0)
         rm = ctypes.c_double(0.0)
@@ -210,6 +216,37 @@ class MultigridSolver:
         else:
             L.check(st, allow=(L.ERR_NOT_CONVERGED,))
         return st, nc.value, hist[: nc.value + 1], outs
+
+    def set_velocity(self, b, ustar, vstar, wstar, u=None, v=None, w=None, pl=2):
+        """Host face arrays in the reference's layout, numpy (nk+2pl, nj+2pl,
+        ni+2pl) with u[.., i] the face between cells i and i+1 (0-based
+        padded index); u, v, w (optional) supply the faces calvel keeps."""
+        ni, nj, _ = self.blocks[b]["n"]
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+                for a in (ustar, vstar, wstar, u, v, w)]
+        star = arrs[:3]
+        cur = [a if a is not None else star[q] for q, a in enumerate(arrs[3:])]
+        L.check(self.lib.h3dmg_set_velocity(self.h, b, *[_dptr(a) for a in star + cur],
+                                            ni + 2 * pl, nj + 2 * pl, pl))
+
+    def get_velocity(self, b, out=None, pl=2):
+        """u, v, w of the last calvel into host arrays in the reference's
+        layout (zeros where nothing is written unless `out` is given)."""
+        ni, nj, nk = self.blocks[b]["n"]
+        shape = (nk + 2 * pl, nj + 2 * pl, ni + 2 * pl)
+        out = out if out is not None else [np.zeros(shape) for _ in range(3)]
+        L.check(self.lib.h3dmg_get_velocity(self.h, b, *[_dptr(a) for a in out],
+                                            ni + 2 * pl, nj + 2 * pl, pl))
+        return out
+
+    def get_su_host(self, b, pl=2):
+        ni, nj, nk = self.blocks[b]["n"]
+        out = np.zeros((nk + 2 * pl, nj + 2 * pl, ni + 2 * pl))
+        L.check(self.lib.h3dmg_get_su(self.h, b, _dptr(out), ni + 2 * pl, nj + 2 * pl, pl))
+        return out
+
+    def set_dt(self, dt):
+        L.check(self.lib.h3dmg_set_dt(self.h, float(dt)))
 
     def velocity_begin(self, ustar, vstar, wstar, alfapr=1.0):
         """First calvel + calmas of the velocity loop; returns (rmax, (u, v, w))."""

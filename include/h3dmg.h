@@ -186,9 +186,29 @@ h3dmg_status h3dmg_solve_velocity(h3dmg_handle *h, const double *const *d_ustar,
                                   double *const *d_u, double *const *d_v, double *const *d_w,
                                   double alfapr, double *hist, int hist_cap, int *ncycles,
                                   double *rmax);
+/* The time step of the next solve (params.dt): the reference's dt changes
+ * from step to step (checkdt.for:117-137). */
+h3dmg_status h3dmg_set_dt(h3dmg_handle *h, double dt);
+
 /* su of block b as the rhs arrays hold it (after h3dmg_set_rhs*, or the
  * last calmas of the velocity loop), into a dense device array (nk, nj, ni). */
 h3dmg_status h3dmg_get_su_device(h3dmg_handle *h, int b, double *d_su);
+
+/* Host Fortran face arrays for the velocity loop (dom(ib)%ustar, vstar,
+ * wstar and u, v, w; the layout of p: ld1, ld2 and `off` >= 1, u(i,j,k) the
+ * face between cells i and i+1, press.for:90-131).  set copies u*, v*, w*
+ * into arrays the handle owns (allocated on first use), taking the current
+ * u, v, w (may be null: then u*) on the faces calvel does not correct;
+ * h3dmg_velocity_begin/h3dmg_solve_velocity called with six null lists use
+ * them.  get writes u, v, w of every face of the interior rows back (the
+ * remaining ghost layers are the host's exchange(1..3), as after calvel).
+ * h3dmg_get_su writes su's interior into a host Fortran array. */
+h3dmg_status h3dmg_set_velocity(h3dmg_handle *h, int b, const double *ustar, const double *vstar,
+                                const double *wstar, const double *u, const double *v,
+                                const double *w, int ld1, int ld2, int off);
+h3dmg_status h3dmg_get_velocity(h3dmg_handle *h, int b, double *u, double *v, double *w, int ld1,
+                                int ld2, int off);
+h3dmg_status h3dmg_get_su(h3dmg_handle *h, int b, double *su, int ld1, int ld2, int off);
 
 /* Per-block results of the last solve: cycles run and final relative
  * residual (arrays of nblocks). */

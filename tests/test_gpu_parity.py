@@ -172,6 +172,47 @@ def test_gpu_velocity_loop_bitwise_vs_reference(gpu_lib, name, graph):
     assert ran == int(z["stop_at"]) and np.array_equal(hist, z["rmax"][:ran + 1])
 
 
+def test_gpu_velocity_loop_from_host_arrays(gpu_lib):
+    """The Fortran-layout entry points (h3dmg_set_velocity, h3dmg_get_velocity,
+    h3dmg_get_su: two ghost layers, u(i) the face between cells i and i+1)
+    feed the same loop: from the reference's velocity field the rmax history,
+    p and su are bitwise the reference's, and u, v, w land on the faces they
+    came from, bitwise the device-array path's; faces calvel does not
+    correct keep the host's u."""
+    z = load_golden("newsolv_ppw32.npz")
+    c = _velocity_case("newsolv_ppw32.npz")
+    spec = _blockspec(z)
+    ni, nj, nk = spec["n"]
+    nc, pl = int(z["ncycles"]), 2
+    shape = (nk + 2 * pl, nj + 2 * pl, ni + 2 * pl)
+    rng = np.random.default_rng(7)
+    US, VS, WS = (rng.standard_normal(shape) for _ in range(3))  # ghosts: noise
+    U, V, W = (rng.standard_normal(shape) for _ in range(3))
+    US[pl:pl + nk, pl:pl + nj, pl - 1:pl + ni] = c["u"]
+    VS[pl:pl + nk, pl - 1:pl + nj, pl:pl + ni] = c["v"]
+    WS[pl - 1:pl + nk, pl:pl + nj, pl:pl + ni] = c["w"]
+    # bc (P, P, P, P, W, W): only the two z-wall faces are not corrected;
+    # the host's u there must be what calvel leaves (the reference keeps u)
+    W[pl - 1, pl:pl + nj, pl:pl + ni] = c["w"][0]
+    W[pl + nk - 1, pl:pl + nj, pl:pl + ni] = c["w"][-1]
+    with _solver(spec, maxcy=nc, tol=0.0, dt=float(z["dt"])) as S:
+        S.set_p(0, np.zeros(c["f"].shape))
+        S.set_velocity(0, US, VS, WS, U, V, W, pl=pl)
+        st, ran, hist, _ = S.solve_velocity()
+        p = S.get_p(0)
+        su = S.get_su_host(0, pl=pl)
+        uo, vo, wo = S.get_velocity(0, pl=pl)
+    assert ran == nc and np.array_equal(hist, z["rmax"])
+    assert np.array_equal(p, z["p"])
+    assert np.array_equal(su[pl:pl + nk, pl:pl + nj, pl:pl + ni], z["final_su"])
+    with _solver(spec, maxcy=nc, tol=0.0, dt=float(z["dt"])) as S:
+        S.set_p(0, np.zeros(c["f"].shape))
+        _, _, _, (ud, vd, wd) = S.solve_velocity([_dev(c["u"])], [_dev(c["v"])], [_dev(c["w"])])
+    assert np.array_equal(uo[pl:pl + nk, pl:pl + nj, pl - 1:pl + ni], _host(ud[0]))
+    assert np.array_equal(vo[pl:pl + nk, pl - 1:pl + nj, pl:pl + ni], _host(vd[0]))
+    assert np.array_equal(wo[pl - 1:pl + nk, pl:pl + nj, pl:pl + ni], _host(wd[0]))
+
+
 def test_gpu_velocity_loop_mixed(gpu_lib):
     """The velocity form with the FP32 V-cycle and exact coarsest solve:
     each iteration's su is recomputed in FP64 from the corrected velocity, so
