@@ -63,6 +63,8 @@ SIGNATURES = {
     "h3dmg_set_stream": (_i, [_vp, _vp]),
     "h3dmg_set_march": (_i, [_vp, _i, _i]),
     "h3dmg_get_march": (_i, [_vp, _i]),
+    "h3dmg_set_fused": (_i, [_vp, ctypes.c_longlong]),
+    "h3dmg_get_fused": (ctypes.c_longlong, [_vp]),
     "h3dmg_set_tail": (_i, [_vp, _i]),
     "h3dmg_set_block_tol": (_i, [_vp, _i, _d]),
     "h3dmg_set_block_enabled": (_i, [_vp, _i, _i]),

@@ -10,7 +10,9 @@ cell is computed by the same arithmetic whatever the choice):
   workgroups;
 * tail level -- the level from which one workgroup per block runs the rest
   of the V-cycle in a single launch (h3dmg_set_tail): below some size a level
-  is launch-latency-bound and on-chip barriers beat full-chip launches.
+  is launch-latency-bound and on-chip barriers beat full-chip launches;
+* fused sweeps -- the largest level (cells per block) whose red-black sweeps
+  run as one launch each instead of two (h3dmg_set_fused, 0 = none).
 
 `autotune(solver)` measures the candidates on the solver's own blocks and
 right-hand side, applies the fastest and returns the choice; results are
@@ -74,8 +76,14 @@ def save_cache(cache: dict, path: str = DEFAULT_CACHE) -> None:
     os.replace(tmp, path)
 
 
+FUSED_CANDIDATES = (0, 32 ** 3, 64 ** 3)
+
+
 def apply(solver, choice: dict) -> None:
-    """Apply a tuning choice {"march": {level: planes}, "tail": level}."""
+    """Apply a tuning choice {"march": {level: planes}, "tail": level,
+    "fused": max_cells}."""
+    if "fused" in choice:
+        solver.set_fused(int(choice["fused"]))
     for g, kb in choice.get("march", {}).items():
         solver.set_march(int(g), int(kb))
     if "tail" in choice:
@@ -177,6 +185,16 @@ def autotune(solver, cache_path: str | None = DEFAULT_CACHE, reps: int = 5,
             best, tbest = lt, t
     solver.set_tail(best)
     choice["tail"] = best
+    # 3. fused sweeps on whole solves
+    fbest = 0
+    for fz in FUSED_CANDIDATES[1:]:
+        solver.set_fused(fz)
+        t = _time_solve(solver, reps)
+        log[f"fused{fz}"] = t
+        if t < tbest * 0.98:
+            fbest, tbest = fz, t
+    solver.set_fused(fbest)
+    choice["fused"] = fbest
     choice["solve_ms"] = tbest * 1e3
     for q, p in enumerate(saved):
         solver.set_p(q, p)

@@ -80,6 +80,13 @@ struct RedPartial {
 template <class T>
 void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
                      const int *active, int colour, int src, int zero);
+// one full red-black sweep (both colour halves) of small levels in one
+// launch: tiles of FUSED_X x FUSED_Y x FUSED_Z cells (Tile::x0, y0, z0 in
+// cells), kind 4
+constexpr int FUSED_X = 32, FUSED_Y = 8, FUSED_Z = 4;
+template <class T>
+void launch_rb_fused(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                     const int *active, int src, int zero);
 // restriction into the coarse levels of a tile list whose tiles are all
 // `depth` (1, 2 or 4) coarse planes deep
 template <class Tf, class Tc>

@@ -520,6 +520,169 @@ __global__ void __launch_bounds__(NTHREADS)
 }
 
 // ---------------------------------------------------------------------------
+// Fused red-black sweep for small levels: both colour halves of one sweep
+// in one launch (the split sweep costs two launches, and small levels are
+// bound by launch and memory latency, not by bandwidth).  A tile is FX x FY
+// x FZ cells at (x0, y0, z0) in cell coordinates.
+//   Phase 1: the new red value of every red cell of the block inside the
+//   tile grown by one cell on each side (the tile's own red cells and the
+//   halo its black cells need), into LDS; the tile's own red cells also go
+//   to buffer 1-src.  A red cell reads black neighbours from buffer src and
+//   ghosts as images in buffer src, as the split red half-sweep does.
+//   Phase 2: the tile's black cells from the red values in LDS (neighbours
+//   inside the block) and ghost images in buffer src (pre-sweep, as the
+//   split black half-sweep reads them), into buffer 1-src.
+// Halo red cells are also computed by the neighbouring tiles with the same
+// arithmetic, so every red value a black cell uses is bitwise the one the
+// split sweep stores.  zero != 0: the level holds zeros in buffer src.
+constexpr int FX = FUSED_X, FY = FUSED_Y, FZ = FUSED_Z;
+constexpr int FEX = FX + 2, FEY = FY + 2, FEZ = FZ + 2;
+constexpr int F_RED = FEX / 2 * FEY * FEZ;   // red positions of the grown tile
+constexpr int F_BLACK = FX / 2 * FY * FZ;    // black cells of the tile
+constexpr int F_RED_PER = (F_RED + NTHREADS - 1) / NTHREADS;
+constexpr int F_BLACK_PER = F_BLACK / NTHREADS;
+
+template <class T>
+__global__ void __launch_bounds__(NTHREADS)
+    k_rb_fused(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+               const int *__restrict__ active, int src, int zero) {
+  const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+  const LevelDev &L = levs[t.lev];
+  if (!active[L.blk]) return;
+  __shared__ T red[FEZ * FEY * FEX];
+  const int tid = (int)(threadIdx.x + blockDim.x * threadIdx.y);
+  const int ni = L.ni, nj = L.nj, nk = L.nk, H = L.H;
+  const bool px = L.per & 1, py = L.per & 2, pz = L.per & 4;
+  // Addresses are 32-bit byte offsets from the level's first array (all six
+  // arrays of a level lie in one arena allocation), through a global-
+  // address-space base: a generic pointer's loads would also count against
+  // the LDS wait counter.
+  constexpr unsigned SZ = sizeof(T);
+  const char *base = (const char *)L.phi[0][0];
+  auto aoff = [&](const void *a) -> unsigned { return (unsigned)((const char *)a - base); };
+  const unsigned aP0 = aoff(L.phi[src][0]), dP = aoff(L.phi[src][1]) - aP0;  // pre-sweep
+  const unsigned aR0 = aoff(L.rhs[0]), aR1 = aoff(L.rhs[1]);
+  const unsigned aD0 = aoff(L.phi[1 - src][0]), aD1 = aoff(L.phi[1 - src][1]);
+  const gmem<char> *Bs = (const gmem<char> *)base;
+  gmem<char> *Bd = (gmem<char> *)base;
+  const unsigned row = (unsigned)H * SZ, plane = row * (unsigned)nj;
+  auto inside = [&](int x, int y, int z) {
+    return (unsigned)x < (unsigned)ni && (unsigned)y < (unsigned)nj && (unsigned)z < (unsigned)nk;
+  };
+  // byte offset of cell (x, y, z) inside its colour array
+  auto at = [&](int x, int y, int z) -> unsigned {
+    return (unsigned)z * plane + (unsigned)y * row + (unsigned)(x >> 1) * SZ;
+  };
+  // pre-sweep value of the image of (x, y, z) (the cell itself inside the
+  // block; positions up to two cells outside map into the block)
+  auto pre = [&](int x, int y, int z) -> T {
+    return *(const gmem<T> *)(Bs + aP0 + (unsigned)((x + y + z) & 1) * dP + at(x, y, z));
+  };
+  auto prex = [&](int x, int y, int z) { return pre(image(x, ni, px), y, z); };
+  auto prey = [&](int x, int y, int z) { return pre(x, image(y, nj, py), z); };
+  auto prez = [&](int x, int y, int z) { return pre(x, y, image(z, nk, pz)); };
+  const bool zl = zero != 0;  // pre-sweep state all zero
+
+  // Every global load first.  Black cells of the tile: rhs and the images
+  // of their six neighbours (used where a neighbour is a ghost).
+  int bi[F_BLACK_PER], bj[F_BLACK_PER], bk[F_BLACK_PER];
+  T brhs[F_BLACK_PER], bg[F_BLACK_PER][6];
+#pragma unroll
+  for (int q = 0; q < F_BLACK_PER; ++q) {
+    const int b = tid + q * NTHREADS;
+    const int row = b / (FX / 2), m = b % (FX / 2);
+    const int j = t.y0 + row % FY, k = t.z0 + row / FY;
+    const int i = t.x0 + (((t.x0 + j + k) & 1) ^ 1) + 2 * m;
+    bi[q] = i;
+    bj[q] = j;
+    bk[q] = k;
+    // clamped into the block: cells of a clipped tile load a valid entry
+    const int ic = min(i, ni - 1), jc = min(j, nj - 1), kc = min(k, nk - 1);
+    brhs[q] = *(const gmem<T> *)(Bs + aR1 + at(ic, jc, kc));
+    bg[q][0] = prex(ic - 1, jc, kc);
+    bg[q][1] = prex(ic + 1, jc, kc);
+    bg[q][2] = prey(ic, jc - 1, kc);
+    bg[q][3] = prey(ic, jc + 1, kc);
+    bg[q][4] = prez(ic, jc, kc - 1);
+    bg[q][5] = prez(ic, jc, kc + 1);
+  }
+  // red cells of the grown tile: rhs and six (black) neighbours, all
+  // pre-sweep images
+  int ri[F_RED_PER], rj[F_RED_PER], rk[F_RED_PER], re[F_RED_PER];
+  T rrhs[F_RED_PER], rn[F_RED_PER][6];
+#pragma unroll
+  for (int q = 0; q < F_RED_PER; ++q) {
+    const int r = min(tid + q * NTHREADS, F_RED - 1);
+    const int row = r / (FEX / 2), m = r % (FEX / 2);
+    const int ey = row % FEY, ez = row / FEY;
+    const int j = t.y0 - 1 + ey, k = t.z0 - 1 + ez;
+    const int ex = ((t.x0 - 1 + j + k) & 1) + 2 * m;  // (i + j + k) even
+    const int i = t.x0 - 1 + ex;
+    ri[q] = i;
+    rj[q] = j;
+    rk[q] = k;
+    re[q] = (ez * FEY + ey) * FEX + ex;
+    // positions of the grown tile outside the block load their image (unused)
+    const int ic = image(i, ni, px), jc = image(j, nj, py), kc = image(k, nk, pz);
+    rrhs[q] = *(const gmem<T> *)(Bs + aR0 + at(ic, jc, kc));
+    rn[q][0] = prex(ic - 1, jc, kc);
+    rn[q][1] = prex(ic + 1, jc, kc);
+    rn[q][2] = prey(ic, jc - 1, kc);
+    rn[q][3] = prey(ic, jc + 1, kc);
+    rn[q][4] = prez(ic, jc, kc - 1);
+    rn[q][5] = prez(ic, jc, kc + 1);
+  }
+
+  // phase 1: red cells of the grown tile that lie in the block, into LDS
+  // (the tile's own red cells go to global memory after phase 2, so that
+  // the barrier waits for LDS only)
+  T rval[F_RED_PER];
+#pragma unroll
+  for (int q = 0; q < F_RED_PER; ++q) {
+    const int i = ri[q], j = rj[q], k = rk[q];
+    rval[q] = T(0);
+    if (tid + q * NTHREADS >= F_RED || !inside(i, j, k)) continue;
+    const Coef<T> c = cell_coef<T>(L, i, j, k);
+    T v[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) v[d] = zl ? T(0) : rn[q][d];
+    const T sum = c.aw * v[0] + c.ae * v[1] + c.as * v[2] + c.an * v[3] + c.ab * v[4] +
+                  c.at * v[5];
+    rval[q] = div_ap(rrhs[q] - sum, c.ap, T(1) / c.ap);
+    red[re[q]] = rval[q];
+  }
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's LDS writes done
+  __builtin_amdgcn_s_barrier();
+
+  // phase 2: the tile's black cells; neighbours in the block from LDS, ghost
+  // neighbours their pre-sweep images
+#pragma unroll
+  for (int q = 0; q < F_BLACK_PER; ++q) {
+    const int i = bi[q], j = bj[q], k = bk[q];
+    if (!inside(i, j, k)) continue;
+    const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
+    T v[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      const int x = i + di[d], y = j + dj[d], z = k + dk[d];
+      v[d] = inside(x, y, z) ? red[((z - t.z0 + 1) * FEY + (y - t.y0 + 1)) * FEX + (x - t.x0 + 1)]
+                             : (zl ? T(0) : bg[q][d]);
+    }
+    const Coef<T> c = cell_coef<T>(L, i, j, k);
+    const T sum = c.aw * v[0] + c.ae * v[1] + c.as * v[2] + c.an * v[3] + c.ab * v[4] +
+                  c.at * v[5];
+    *(gmem<T> *)(Bd + aD1 + at(i, j, k)) = div_ap(brhs[q] - sum, c.ap, T(1) / c.ap);
+  }
+#pragma unroll
+  for (int q = 0; q < F_RED_PER; ++q) {
+    const int i = ri[q], j = rj[q], k = rk[q];
+    if (tid + q * NTHREADS < F_RED && i >= t.x0 && i < t.x0 + FX && i < ni && j >= t.y0 &&
+        j < t.y0 + FY && j < nj && k >= t.z0 && k < t.z0 + FZ && k < nk)
+      *(gmem<T> *)(Bd + aD0 + at(i, j, k)) = rval[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // mgrestr (mgsolver.for:693-756): fine residual
 //   resf = rhs - (aw*W + ae*E + as*S + an*N + ab*B + at*T + ap*P)
 // and coarse rhs = 0.125 * (sum of the 8 children, i fastest, then j, k).
@@ -1814,6 +1977,14 @@ void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, int depth, co
   hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour, src, zero);
 }
 
+template <class T>
+void launch_rb_fused(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                     const int *active, int src, int zero) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_rb_fused<T>, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, src,
+                     zero);
+}
+
 template <class Tf, class Tc>
 void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
                      const int *active, int src_f) {
@@ -1966,6 +2137,10 @@ void launch_remove_mean(hipStream_t s, const FineDev *fd, int ni, int nj, int nk
 }
 
 // explicit instantiations
+template void launch_rb_fused<double>(hipStream_t, const Tile *, int, const LevelDev *,
+                                      const int *, int, int);
+template void launch_rb_fused<float>(hipStream_t, const Tile *, int, const LevelDev *, const int *,
+                                     int, int);
 template void launch_rb_sweep<double>(hipStream_t, const Tile *, int, int, const LevelDev *,
                                       const int *, int, int, int);
 template void launch_rb_sweep<float>(hipStream_t, const Tile *, int, int, const LevelDev *,

@@ -53,6 +53,7 @@ struct TileSet {
   Tile *d = nullptr;
   Tile *owned = nullptr;  // smoother sets: separately allocated (re-tunable)
   int depth = 1;          // smoother sets: planes of every tile (1, 2 or 4)
+  bool fused = false;     // smoother sets: fused red-black tiles (one launch per sweep)
   int n() const { return (int)h.size(); }
 };
 
@@ -239,6 +240,7 @@ struct h3dmg_handle {
   size_t stage_elems = 0;
 
   std::vector<int> march_kb;  // planes per smoother workgroup per level (0 = auto)
+  long fused_max = 0;  // levels whose blocks have at most this many cells: fused sweeps
   int tail_req = 0;           // requested tail level (0 = auto)
   int tail_lt = 1 << 30;      // levels >= tail_lt run in the single-workgroup tail kernel
   TileSet tail_tiles;         // one tile per block with L_b >= tail_lt
@@ -259,9 +261,12 @@ bool is_mixed(const h3dmg_handle *h) { return h->prm.precision == H3DMG_MIXED; }
 template <class T>
 void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool &zero) {
   for (int s = 0; s < nsweep; ++s) {
-    for (int colour = 0; colour < 2; ++colour)
-      launch_rb_sweep<T>(h->stream, ts.d, ts.n(), ts.depth, h->d_levs, h->d_active, colour, buf,
-                         zero ? 1 : 0);
+    if (ts.fused)
+      launch_rb_fused<T>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, buf, zero ? 1 : 0);
+    else
+      for (int colour = 0; colour < 2; ++colour)
+        launch_rb_sweep<T>(h->stream, ts.d, ts.n(), ts.depth, h->d_levs, h->d_active, colour,
+                           buf, zero ? 1 : 0);
     buf ^= 1;
     zero = false;
   }
@@ -886,15 +891,37 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
         nks.push_back(L.nk);
         general = general || L.ni < 2;
       }
-    const int kb = general ? 1 : launch_depth(cols, 4096, h->march_kb[g], 4, nks);
-    ts.depth = kb;
-    for (int b = 0; b < h->nb; ++b) {
-      const BlockInfo &B = h->blk[b];
-      if (!member(B)) continue;
-      add_tiles_march(ts, B.desc[g], h->levs[B.desc[g]], kb);
+    // small levels: one fused launch per sweep (latency-bound, the red halo
+    // recomputed per tile costs less than a second launch)
+    long most = 0;
+    for (const BlockInfo &B : h->blk)
+      if (member(B)) {
+        const LevelDev &L = h->levs[B.desc[g]];
+        most = std::max(most, (long)L.ni * L.nj * L.nk);
+      }
+    ts.fused = most > 0 && most <= h->fused_max;
+    if (ts.fused) {
+      ts.depth = FUSED_Z;
+      for (const BlockInfo &B : h->blk)
+        if (member(B)) {
+          const LevelDev &L = h->levs[B.desc[g]];
+          for (int z = 0; z < L.nk; z += FUSED_Z)
+            for (int y = 0; y < L.nj; y += FUSED_Y)
+              for (int x = 0; x < L.ni; x += FUSED_X)
+                ts.h.push_back(Tile{B.desc[g], x, y, z, std::min(z + FUSED_Z, L.nk), 4});
+        }
+    } else {
+      const int kb = general ? 1 : launch_depth(cols, 4096, h->march_kb[g], 4, nks);
+      ts.depth = kb;
+      for (int b = 0; b < h->nb; ++b) {
+        const BlockInfo &B = h->blk[b];
+        if (!member(B)) continue;
+        add_tiles_march(ts, B.desc[g], h->levs[B.desc[g]], kb);
+      }
     }
     for (const Tile &t : ts.h)
-      if (t.kind != 2 && t.z1 - t.z0 != kb) return fail(H3DMG_ERR_ARG, "internal: tile depth");
+      if (t.kind != 2 && t.kind != 4 && t.z1 - t.z0 != ts.depth)
+        return fail(H3DMG_ERR_ARG, "internal: tile depth");
     Tile *d = nullptr;
     if (!ts.h.empty()) {
       HIP_TRY(hipMalloc((void **)&d, ts.h.size() * sizeof(Tile)));
@@ -1898,8 +1925,23 @@ int h3dmg_get_march(const h3dmg_handle *h, int level) {
   if (!h || level < 1 || level > h->Lmax) return -1;
   for (const TileSet *ts : {&h->relax_nc[level], &h->relax_c[level]})
     for (const Tile &t : ts->h)
-      if (t.kind != 2) return t.z1 - t.z0;
+      if (t.kind != 2) return t.kind == 4 ? 0 : t.z1 - t.z0;
   return 0;
 }
+
+h3dmg_status h3dmg_set_fused(h3dmg_handle *h, long long max_cells) {
+  if (!h || max_cells < 0) return fail(H3DMG_ERR_ARG, "bad fused threshold");
+  std::lock_guard<std::mutex> lock(g_device_mutex);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  drop_graph(h);
+  h->fused_max = (long)max_cells;
+  for (int g = 1; g <= h->Lmax; ++g) {
+    h3dmg_status st = upload_relax(h, g);
+    if (st) return st;
+  }
+  return H3DMG_OK;
+}
+
+long long h3dmg_get_fused(const h3dmg_handle *h) { return h ? h->fused_max : -1; }
 
 }  // extern "C"

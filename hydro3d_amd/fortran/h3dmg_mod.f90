@@ -43,7 +43,7 @@ module h3dmg_mod
        h3dmg_set_block_tol, h3dmg_set_block_enabled, h3dmg_project_device, &
        h3dmg_velocity_begin, h3dmg_velocity_step, h3dmg_solve_velocity, &
        h3dmg_get_su_device, h3dmg_set_velocity, h3dmg_get_velocity, h3dmg_get_su, &
-       h3dmg_set_dt, &
+       h3dmg_set_dt, h3dmg_set_fused, h3dmg_get_fused, &
        h3dmg_error_message
 
   interface
@@ -211,6 +211,17 @@ module h3dmg_mod
        integer(c_int), value :: b, ld1, ld2, off
        real(c_double), intent(inout) :: su(*)
      end function h3dmg_get_su
+
+     integer(c_int) function h3dmg_set_fused(handle, max_cells) bind(C, name="h3dmg_set_fused")
+       import :: c_ptr, c_int, c_long_long
+       type(c_ptr), value :: handle
+       integer(c_long_long), value :: max_cells
+     end function h3dmg_set_fused
+
+     integer(c_long_long) function h3dmg_get_fused(handle) bind(C, name="h3dmg_get_fused")
+       import :: c_ptr, c_long_long
+       type(c_ptr), value :: handle
+     end function h3dmg_get_fused
 
      integer(c_int) function h3dmg_set_dt(handle, dt) bind(C, name="h3dmg_set_dt")
        import :: c_ptr, c_int, c_double

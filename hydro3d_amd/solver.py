@@ -290,6 +290,14 @@ class MultigridSolver:
     def get_march(self, level):
         return int(self.lib.h3dmg_get_march(self.h, level))
 
+    def set_fused(self, max_cells):
+        """Levels of at most max_cells cells per block: one fused launch per
+        red-black sweep (0 = never); results are identical either way."""
+        L.check(self.lib.h3dmg_set_fused(self.h, int(max_cells)))
+
+    def get_fused(self):
+        return int(self.lib.h3dmg_get_fused(self.h))
+
     def set_tail(self, level):
         """Levels >= level run in the single-workgroup V-cycle tail (0 = auto,
         above the coarsest = off); results are identical either way."""

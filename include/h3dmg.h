@@ -237,6 +237,13 @@ h3dmg_status h3dmg_set_stream(h3dmg_handle *h, void *stream);
  * h3dmg_get_march returns the depth in use (or -1 for a bad level). */
 h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes);
 int h3dmg_get_march(const h3dmg_handle *h, int level);
+/* Levels whose blocks have at most `max_cells` cells run each red-black
+ * sweep as one fused launch (both colour halves, the red halo recomputed
+ * per tile) instead of two; 0 (the default) turns fusion off.  Results are
+ * identical either way.  Drops a captured graph.  h3dmg_get_march returns 0
+ * for a fused level; h3dmg_get_fused returns the threshold in use. */
+h3dmg_status h3dmg_set_fused(h3dmg_handle *h, long long max_cells);
+long long h3dmg_get_fused(const h3dmg_handle *h);
 /* Levels >= `level` of every block run as one workgroup per block in a
  * single launch with every level in LDS (the V-cycle tail); 0 restores the
  * automatic choice (the first level from which each block's remaining levels

@@ -375,6 +375,40 @@ def test_gpu_residual_norm_matches_numpy(gpu_lib, n):
         assert abs(hist[0] - want) <= 1e-12 * want, (prec, hist[0], want)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_gpu_fused_sweeps_do_not_change_results(gpu_lib, prec):
+    """A fused red-black sweep (one launch per sweep, red halo recomputed per
+    tile) computes every cell as the two split half-sweeps do: with fusion
+    off, on the default levels and on every level, the pressure is bitwise
+    the same (odd, ragged and one-tile-wide shapes included)."""
+    rng = np.random.default_rng(13)
+    W, P = 0, 1
+    specs = [dict(n=(64, 64, 64), h=(1 / 64,) * 3, bc=(P, P, P, P, W, W), nlev=4),
+             dict(n=(48, 40, 24), h=(0.02, 0.03, 0.04), bc=(W,) * 6, nlev=3),
+             dict(n=(40, 8, 12), h=(0.025, 0.125, 0.1), bc=(P, P, W, W, P, P), nlev=2)]
+    fs = [mean_free(rng.standard_normal((s["n"][2], s["n"][1], s["n"][0]))) for s in specs]
+    kw = dict(tol=1e-9, maxcy=10)
+    if prec == "mixed":
+        kw.update(precision="mixed", coarse="exact")
+    results = {}
+    for fused in (0, 1 << 18, 1 << 40):
+        print(f"case: fused threshold {fused} ({prec})", flush=True)
+        with _solver(specs, **kw) as S:
+            S.set_tail(S.nlevels_max + 1)  # every level on the per-level kernels
+            S.set_fused(fused)
+            assert S.get_fused() == fused
+            for b, f in enumerate(fs):
+                S.set_rhs(b, f)
+            S.solve()
+            cyc, _ = S.block_stats()
+            results[fused] = (list(cyc), [S.get_p(b) for b in range(len(specs))])
+    ref = results[0]
+    for fused, (cyc, ps) in results.items():
+        assert cyc == ref[0], (fused, cyc, ref[0])
+        for b in range(len(specs)):
+            assert np.array_equal(ps[b], ref[1][b]), (fused, b)
+
+
 def test_gpu_projection_bitwise_vs_reference(gpu_lib):
     """calvel + calmas on the device (h3dmg_project_device) reproduce the
     reference's su bitwise: su0 from p = 0 and the final su from the
