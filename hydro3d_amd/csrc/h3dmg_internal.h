@@ -45,6 +45,11 @@ struct LevelDev {
   double cx, cy, cz;  // 1/h_g^2 (coeff, mgsolver.for:54-60,185-191)
   void *phi[2][2];    // [buffer][colour]
   void *rhs[2];       // [colour]
+  // variable-density operator (coeff under L_LSM, mgsolver.for:72-184), or
+  // null: per-cell FP64 coefficients aw, ae, as, an, ab, at, ap, each as two
+  // colour arrays of lsm_half doubles laid out like phi (array 2*q + colour)
+  double *lsm;
+  unsigned long long lsm_half;
 };
 
 // Finest-level FP64 state of one block (pressure, right-hand side).
@@ -54,6 +59,8 @@ struct FineDev {
   double cx, cy, cz;
   double *p[2];  // [colour]
   double *f[2];  // [colour]
+  double dx, dy, dz;     // spacing (the variable-density projection's volume factors)
+  const double *dens;    // variable-density operator: density, interior (nk, nj, ni), or null
 };
 
 // A workgroup's work item: descriptor index, the tile origin, for the
@@ -127,6 +134,24 @@ void launch_vtail(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *
 size_t vtail_lds_bytes(const LevelDev *levs_host, int top, bool mixed, bool exact);
 // permit `lds_bytes` of dynamic LDS for the tail kernel
 hipError_t vtail_allow_lds(size_t lds_bytes);
+
+// ---- variable-density operator (L_LSM) ----------------------------------
+// dens_mg of level `lev` (interior, natural layout) from the finer level's:
+// 0.125 * (sum of the 8 children, j fastest, then i, k: mgsolver.for:87-118)
+void launch_lsm_dens_restrict(hipStream_t s, const double *fine, int fni, int fnj, int fnk,
+                              double *coarse);
+// coefficients of one level of one block from its dens_mg into L.lsm
+// (mgsolver.for:121-184,194-210); dx, dy, dz the block's finest spacing
+void launch_lsm_coef(hipStream_t s, const LevelDev *lev_dev, const double *dens, int ni, int nj,
+                     int nk, double dx, double dy, double dz, int walls);
+// one colour half-sweep / the fine residual / the restriction with the
+// per-cell coefficients (every level of a variable-density handle is FP64)
+void launch_lsm_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                      const int *active, int colour, int src, int zero);
+void launch_lsm_residual(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,
+                         const LevelDev *levs, const int *active, RedPartial *partials);
+void launch_lsm_restrict(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                         const int *active, int src_f);
 
 // velocity projection (calvel per axis over `faces` faces; calmas writes
 // su (optional) and one partial max per workgroup, returns the group count)

@@ -1804,6 +1804,187 @@ __global__ void __launch_bounds__(TAIL_THREADS)
 }
 
 // ---------------------------------------------------------------------------
+// Variable-density operator (coeff under L_LSM, mgsolver.for:72-184).  The
+// reference builds, per level, dens_mg (the density of the level's cells:
+// the input at level 1, the 8-child average below) and from it per-cell
+// coefficients in finite-volume form with the FINEST spacing at every level:
+//   aw = dy*dz/dx/dens(i-1/2),  dens(i-1/2) = 0.5*(dens(i) + dens(i-1)), or
+//   dens(i) on the block's first cell; likewise ae, as, an, ab, at; zero
+//   towards a wall; ap = -(aw+ae+as+an+ab+at).
+// Kernels here are per cell (one thread per cell or coarse cell) with the
+// coefficients read from FP64 arrays laid out like phi.
+
+__device__ __forceinline__ const double *lsm_coef(const LevelDev &L, int q, int colour) {
+  return L.lsm + (size_t)(2 * q + colour) * L.lsm_half;
+}
+
+// dens_mg of a coarse level: 0.125 * (c(j0 i0 k0) + c(j1 i0 k0) + c(j0 i1 k0)
+// + c(j1 i1 k0) + c(j0 i0 k1) + ... ), the reference's j-fastest packing
+// order (mgsolver.for:99-118); arrays interior, i fastest
+__global__ void k_lsm_dens_restrict(const double *__restrict__ fine, int fni, int fnj, int fnk,
+                                    double *__restrict__ coarse) {
+  const int cni = fni / 2, cnj = fnj / 2, cnk = fnk / 2;
+  const size_t n = (size_t)cni * cnj * cnk;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int i = (int)(q % cni), j = (int)((q / cni) % cnj), k = (int)(q / ((size_t)cni * cnj));
+  auto f = [&](int di, int dj, int dk) {
+    return fine[((size_t)(2 * k + dk) * fnj + (2 * j + dj)) * fni + (2 * i + di)];
+  };
+  coarse[q] = 0.125 * (f(0, 0, 0) + f(0, 1, 0) + f(1, 0, 0) + f(1, 1, 0) + f(0, 0, 1) +
+                       f(0, 1, 1) + f(1, 0, 1) + f(1, 1, 1));
+}
+
+// coefficients of every cell of one level of one block; walls: bit a*2 (+1)
+// set = the low (high) face of axis a is a wall (coefficient 0)
+__global__ void k_lsm_coef(const LevelDev *__restrict__ lev_dev, const double *__restrict__ dens,
+                           int ni, int nj, int nk, double dx, double dy, double dz, int walls) {
+  const size_t n = (size_t)ni * nj * nk;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const LevelDev &L = *lev_dev;
+  const int i = (int)(q % ni), j = (int)((q / ni) % nj), k = (int)(q / ((size_t)ni * nj));
+  const size_t si = 1, sj = (size_t)ni, sk = (size_t)ni * nj;
+  const double d = dens[q];
+  // face densities (mgsolver.for:130-164): the cell's own on the block's
+  // first/last cell of an axis, else the mean with the neighbour (own first)
+  const double dim = i == 0 ? d : 0.5 * (d + dens[q - si]);
+  const double dip = i == ni - 1 ? d : 0.5 * (d + dens[q + si]);
+  const double djm = j == 0 ? d : 0.5 * (d + dens[q - sj]);
+  const double djp = j == nj - 1 ? d : 0.5 * (d + dens[q + sj]);
+  const double dkm = k == 0 ? d : 0.5 * (d + dens[q - sk]);
+  const double dkp = k == nk - 1 ? d : 0.5 * (d + dens[q + sk]);
+  double c[7];
+  c[0] = dy * dz / dx / dim;
+  c[1] = dy * dz / dx / dip;
+  c[2] = dx * dz / dy / djm;
+  c[3] = dx * dz / dy / djp;
+  c[4] = dx * dy / dz / dkm;
+  c[5] = dx * dy / dz / dkp;
+  // walls (mgsolver.for:194-205)
+  if ((walls & 1) && i == 0) c[0] = 0.0;
+  if ((walls & 2) && i == ni - 1) c[1] = 0.0;
+  if ((walls & 4) && j == 0) c[2] = 0.0;
+  if ((walls & 8) && j == nj - 1) c[3] = 0.0;
+  if ((walls & 16) && k == 0) c[4] = 0.0;
+  if ((walls & 32) && k == nk - 1) c[5] = 0.0;
+  c[6] = -1.0 * (c[0] + c[1] + c[2] + c[3] + c[4] + c[5]);
+  const int colour = (i + j + k) & 1;
+  const size_t e = cidx(L.H, L.nj, i, j, k);
+  for (int a = 0; a < 7; ++a) L.lsm[(size_t)(2 * a + colour) * L.lsm_half + e] = c[a];
+}
+
+// one colour half-sweep (mgsolver.for:553-605 with per-cell coefficients):
+// natural tiles (TILE_X x TILE_Y cells of one plane), threads of the other
+// colour idle; ghosts and the stale-ghost buffer rule as in march_tile
+__global__ void __launch_bounds__(NTHREADS)
+    k_lsm_sweep(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+                const int *__restrict__ active, int colour, int src, int zero) {
+  const Tile t = tiles[blockIdx.x];
+  const LevelDev &L = levs[t.lev];
+  if (!active[L.blk]) return;
+  const int i = t.x0 + (int)threadIdx.x, j = t.y0 + (int)threadIdx.y, k = t.z0;
+  if (i >= L.ni || j >= L.nj || k >= L.nk || ((i + j + k) & 1) != colour) return;
+  const double *O = (const double *)(colour == 0 ? L.phi[src][1] : L.phi[1 - src][0]);
+  const bool zero_int = zero && colour == 0, zero_ghost = zero != 0;
+  auto nb = [&](int x, int y, int z) -> double {
+    const bool ghost = (unsigned)x >= (unsigned)L.ni || (unsigned)y >= (unsigned)L.nj ||
+                       (unsigned)z >= (unsigned)L.nk;
+    if (!ghost) return zero_int ? 0.0 : O[cidx(L.H, L.nj, x, y, z)];
+    return zero_ghost ? 0.0 : level_value<double>(L, src, x, y, z);
+  };
+  const size_t e = cidx(L.H, L.nj, i, j, k);
+  double c[7];
+  for (int a = 0; a < 7; ++a) c[a] = lsm_coef(L, a, colour)[e];
+  const double W = nb(i - 1, j, k), E = nb(i + 1, j, k);
+  const double S = nb(i, j - 1, k), N = nb(i, j + 1, k);
+  const double B = nb(i, j, k - 1), T = nb(i, j, k + 1);
+  const double r = ((const double *)L.rhs[colour])[e];
+  const double sum = c[0] * W + c[1] * E + c[2] * S + c[3] * N + c[4] * B + c[5] * T;
+  ((double *)L.phi[1 - src][colour])[e] = (r - sum) / c[6];
+}
+
+// finest-level residual r = f - A p with per-cell coefficients into the
+// level-1 rhs, with per-tile partial sum of r^2 and max |r| (the residual
+// tiles' work layout: lane l holds cells 2l and 2l+1 of one row)
+__global__ void __launch_bounds__(NTHREADS)
+    k_lsm_residual(const Tile *__restrict__ tiles, const FineDev *__restrict__ fine,
+                   const LevelDev *__restrict__ levs, const int *__restrict__ active,
+                   RedPartial *__restrict__ partials) {
+  const int ti = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const Tile t = tiles[ti];
+  const FineDev &F = fine[t.lev];
+  const LevelDev &L = levs[t.lev];
+  double ss = 0.0, am = 0.0;
+  if (active[F.blk]) {
+    const int lpr = F.lpr;
+    const int lane = (int)threadIdx.x;
+    const int l = t.x0 + (lane & (lpr - 1));
+    const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
+    for (int k = t.z0; k < t.z1; ++k)
+      for (int s = 0; s < 2; ++s) {
+        const int i = 2 * l + s;
+        if (i >= F.ni || j >= F.nj) continue;
+        const int c = (i + j + k) & 1;
+        const size_t e = cidx(F.H, F.nj, i, j, k);
+        auto nb = [&](int x, int y, int z) -> double { return fine_value(F, x, y, z); };
+        double cf[7];
+        for (int a = 0; a < 7; ++a) cf[a] = lsm_coef(L, a, c)[e];
+        const double P = F.p[c][e];
+        const double r = F.f[c][e] - (cf[0] * nb(i - 1, j, k) + cf[1] * nb(i + 1, j, k) +
+                                      cf[2] * nb(i, j - 1, k) + cf[3] * nb(i, j + 1, k) +
+                                      cf[4] * nb(i, j, k - 1) + cf[5] * nb(i, j, k + 1) +
+                                      cf[6] * P);
+        ((double *)L.rhs[c])[e] = r;
+        ss += r * r;
+        am = fmax(am, fabs(r));
+      }
+  }
+  block_reduce(ss, am);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partials[ti] = RedPartial{ss, am};
+}
+
+// restriction (mgrestr, mgsolver.for:693-756) with per-cell coefficients:
+// a thread per coarse cell of the restriction tile's layout; the eight fine
+// residuals summed i fastest, then j, k, times 0.125
+__global__ void __launch_bounds__(NTHREADS)
+    k_lsm_restrict(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+                   const int *__restrict__ active, int src_f) {
+  const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
+  const LevelDev &C = levs[t.lev];
+  if (!active[C.blk]) return;
+  const LevelDev &F = levs[C.finer];
+  const int lpr = C.lpr1;
+  const int lane = (int)threadIdx.x;
+  const int l = t.x0 + (lane & (lpr - 1));
+  const int jc = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
+  if (l >= C.ni || jc >= C.nj) return;
+  auto res = [&](int i, int j, int k) -> double {
+    const int c = (i + j + k) & 1;
+    const size_t e = cidx(F.H, F.nj, i, j, k);
+    double cf[7];
+    for (int a = 0; a < 7; ++a) cf[a] = lsm_coef(F, a, c)[e];
+    auto nb = [&](int x, int y, int z) { return level_value<double>(F, src_f, x, y, z); };
+    return ((const double *)F.rhs[c])[e] -
+           (cf[0] * nb(i - 1, j, k) + cf[1] * nb(i + 1, j, k) + cf[2] * nb(i, j - 1, k) +
+            cf[3] * nb(i, j + 1, k) + cf[4] * nb(i, j, k - 1) + cf[5] * nb(i, j, k + 1) +
+            cf[6] * nb(i, j, k));
+  };
+  for (int kc = t.z0; kc < t.z1; ++kc) {
+    const int i0 = 2 * l, j0 = 2 * jc, k0 = 2 * kc;
+    double acc = res(i0, j0, k0);
+    acc = acc + res(i0 + 1, j0, k0);
+    acc = acc + res(i0, j0 + 1, k0);
+    acc = acc + res(i0 + 1, j0 + 1, k0);
+    acc = acc + res(i0, j0, k0 + 1);
+    acc = acc + res(i0 + 1, j0, k0 + 1);
+    acc = acc + res(i0, j0 + 1, k0 + 1);
+    acc = acc + res(i0 + 1, j0 + 1, k0 + 1);
+    ((double *)C.rhs[(l + jc + kc) & 1])[cidx(C.H, C.nj, l, jc, kc)] = 0.125 * acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Velocity projection of the outer loop (press.for:77-144 calvel and 2-75
 // calmas, 2nd-order branch) on the device.  Face arrays are dense, i
 // fastest: u (nk, nj, ni+1) with u[.., f] the face between cells f-1 and f,
@@ -1839,7 +2020,14 @@ __global__ void __launch_bounds__(256)
       lo[axis] = f == 0 ? n[axis] - 1 : f - 1;
       const double ph = F.p[(hi[0] + hi[1] + hi[2]) & 1][cidx(F.H, F.nj, hi[0], hi[1], hi[2])];
       const double pl = F.p[(lo[0] + lo[1] + lo[2]) & 1][cidx(F.H, F.nj, lo[0], lo[1], lo[2])];
-      v = v - dt_alfa * (ph - pl) / h;
+      if (F.dens) {
+        // L_LSM: divided by the face density (press.for:93-96)
+        const double dl = F.dens[((size_t)lo[2] * F.nj + lo[1]) * F.ni + lo[0]];
+        const double dh = F.dens[((size_t)hi[2] * F.nj + hi[1]) * F.ni + hi[0]];
+        v = v - dt_alfa * (ph - pl) / h / (0.5 * (dl + dh));
+      } else {
+        v = v - dt_alfa * (ph - pl) / h;
+      }
     }
     u[q] = v;
   }
@@ -1866,10 +2054,12 @@ __global__ void __launch_bounds__(256)
     const size_t qw = ((size_t)k * nj + j) * ni + i;
     const double fact = (u[qu + 1] - u[qu]) / dx + (v[qv + ni] - v[qv]) / dy +
                         (w[qw + (size_t)ni * nj] - w[qw]) / dz;
+    // L_LSM: sup = div(u) dx dy dz / dt (press.for:53-55)
+    const double rhs = F.dens ? fact * dx * dy * dz / dt : fact / dt;
     if (to_fine)
-      F.f[(i + j + k) & 1][cidx(F.H, F.nj, i, j, k)] = fact / dt;
+      F.f[(i + j + k) & 1][cidx(F.H, F.nj, i, j, k)] = rhs;
     else if (su)
-      su[q] = fact / dt;
+      su[q] = rhs;
     const double fact1 = fact * dx * dy * dz;
     am = fmax(am, fabs(fact1));
   }
@@ -1975,6 +2165,41 @@ void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, int depth, co
   if (ntiles <= 0) return;
   auto k = depth == 4 ? k_rb_sweep<T, 4> : depth == 2 ? k_rb_sweep<T, 2> : k_rb_sweep<T, 1>;
   hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour, src, zero);
+}
+
+void launch_lsm_dens_restrict(hipStream_t s, const double *fine, int fni, int fnj, int fnk,
+                              double *coarse) {
+  const size_t n = (size_t)(fni / 2) * (fnj / 2) * (fnk / 2);
+  if (!n) return;
+  hipLaunchKernelGGL(k_lsm_dens_restrict, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fine,
+                     fni, fnj, fnk, coarse);
+}
+
+void launch_lsm_coef(hipStream_t s, const LevelDev *lev_dev, const double *dens, int ni, int nj,
+                     int nk, double dx, double dy, double dz, int walls) {
+  const size_t n = (size_t)ni * nj * nk;
+  hipLaunchKernelGGL(k_lsm_coef, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lev_dev, dens,
+                     ni, nj, nk, dx, dy, dz, walls);
+}
+
+void launch_lsm_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                      const int *active, int colour, int src, int zero) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_lsm_sweep, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour,
+                     src, zero);
+}
+
+void launch_lsm_residual(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,
+                         const LevelDev *levs, const int *active, RedPartial *partials) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_lsm_residual, dim3(ntiles), tile_block(), 0, s, tiles, fine, levs, active,
+                     partials);
+}
+
+void launch_lsm_restrict(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                         const int *active, int src_f) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_lsm_restrict, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, src_f);
 }
 
 template <class T>

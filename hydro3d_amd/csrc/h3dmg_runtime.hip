@@ -54,6 +54,7 @@ struct TileSet {
   Tile *owned = nullptr;  // smoother sets: separately allocated (re-tunable)
   int depth = 1;          // smoother sets: planes of every tile (1, 2 or 4)
   bool fused = false;     // smoother sets: fused red-black tiles (one launch per sweep)
+  const TileSet *nat = nullptr;  // smoother sets: the group's natural tiles (variable density)
   int n() const { return (int)h.size(); }
 };
 
@@ -241,6 +242,13 @@ struct h3dmg_handle {
 
   std::vector<int> march_kb;  // planes per smoother workgroup per level (0 = auto)
   long fused_max = 0;  // levels whose blocks have at most this many cells: fused sweeps
+
+  // variable-density operator (h3dmg_set_density): per level descriptor 14
+  // coefficient arrays and dens_mg in one allocation
+  bool lsm = false;
+  double *lsm_store = nullptr;
+  std::vector<size_t> lsm_dens;  // per descriptor: offset (doubles) of its dens_mg
+  std::vector<char> lsm_set;     // block's density uploaded
   int tail_req = 0;           // requested tail level (0 = auto)
   int tail_lt = 1 << 30;      // levels >= tail_lt run in the single-workgroup tail kernel
   TileSet tail_tiles;         // one tile per block with L_b >= tail_lt
@@ -261,7 +269,11 @@ bool is_mixed(const h3dmg_handle *h) { return h->prm.precision == H3DMG_MIXED; }
 template <class T>
 void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool &zero) {
   for (int s = 0; s < nsweep; ++s) {
-    if (ts.fused)
+    if (h->lsm)  // per-cell coefficients (every level FP64)
+      for (int colour = 0; colour < 2; ++colour)
+        launch_lsm_sweep(h->stream, ts.nat->d, ts.nat->n(), h->d_levs, h->d_active, colour, buf,
+                         zero ? 1 : 0);
+    else if (ts.fused)
       launch_rb_fused<T>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, buf, zero ? 1 : 0);
     else
       for (int colour = 0; colour < 2; ++colour)
@@ -324,7 +336,10 @@ void enqueue_vcycle(h3dmg_handle *h) {
         fill_any(h, f64, h->nat_nc[g - 1], buf_nc[g - 1]);
         z_nc[g - 1] = 0;
       }
-      if (h->restr_nc[g].n()) {
+      if (h->lsm) {
+        for (const TileSet *ts : {&h->restr_nc[g], &h->restr_c[g]})
+          launch_lsm_restrict(h->stream, ts->d, ts->n(), h->d_levs, h->d_active, buf_nc[g - 1]);
+      } else if (h->restr_nc[g].n()) {
         if (f64)
           launch_restrict<double, double>(h->stream, h->restr_nc[g].d, h->restr_nc[g].n(),
                                           h->restr_nc[g].depth, h->d_levs, h->d_active,
@@ -334,7 +349,7 @@ void enqueue_vcycle(h3dmg_handle *h) {
                                         h->restr_nc[g].depth, h->d_levs, h->d_active,
                                         buf_nc[g - 1]);
       }
-      if (h->restr_c[g].n()) {
+      if (!h->lsm && h->restr_c[g].n()) {
         if (f64)
           launch_restrict<double, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(),
                                           h->restr_c[g].depth, h->d_levs, h->d_active,
@@ -428,7 +443,10 @@ void enqueue_update_residual(h3dmg_handle *h, bool update) {
     else
       launch_update<float>(h->stream, ut.d, ut.n(), h->d_fine, h->d_levs, h->d_active, h->pp_buf);
   }
-  if (f64)
+  if (h->lsm)
+    launch_lsm_residual(h->stream, ft.d, ft.n(), h->d_fine, h->d_levs, h->d_active,
+                        h->d_partials);
+  else if (f64)
     launch_residual<double>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
                             h->d_partials, 1);
   else
@@ -441,6 +459,15 @@ h3dmg_status read_stats(h3dmg_handle *h) {
   HIP_TRY(hipMemcpyAsync(h->h_stats, h->d_stats, sizeof(RedPartial) * h->nb,
                          hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  return H3DMG_OK;
+}
+
+// a variable-density handle needs the density of every enabled block
+h3dmg_status check_density(const h3dmg_handle *h) {
+  if (!h->lsm) return H3DMG_OK;
+  for (int b = 0; b < h->nb; ++b)
+    if (h->blk[b].enabled && !h->lsm_set[b])
+      return fail(H3DMG_ERR_ARG, "block %d: no density (h3dmg_set_density)", b);
   return H3DMG_OK;
 }
 
@@ -877,6 +904,7 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
   for (int grp = 0; grp < 2; ++grp) {
     TileSet &ts = grp == 0 ? h->relax_nc[g] : h->relax_c[g];
     ts.h.clear();
+    ts.nat = grp == 0 ? &h->nat_nc[g] : &h->nat_c[g];
     // one depth for the whole launch: the requested planes, or enough for
     // ~4096 workgroups over the group's blocks; at most 4, a power of two
     // dividing every block's nk; 1 where a level is one cell wide in x
@@ -1131,6 +1159,7 @@ static void destroy_locked(h3dmg_handle *h) {
   if (h->tail_tiles.owned) (void)hipFree(h->tail_tiles.owned);
   if (h->h_stats) (void)hipHostFree(h->h_stats);
   if (h->vel_store) (void)hipFree(h->vel_store);
+  if (h->lsm_store) (void)hipFree(h->lsm_store);
   if (h->arena) (void)hipFree(h->arena);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1354,6 +1383,7 @@ h3dmg_status h3dmg_get_su_device(h3dmg_handle *h, int b, double *d_su) {
 h3dmg_status h3dmg_vcycle(h3dmg_handle *h) {
   if (!h) return fail(H3DMG_ERR_ARG, "null handle");
   if (!h->rhs_ready) return fail(H3DMG_ERR_ARG, "rhs not set");
+  if (h3dmg_status st = check_density(h)) return st;
   // level-1 rhs <- f (su), pp = 0
   reset_active(h);
   h3dmg_status st = push_active(h);
@@ -1389,6 +1419,7 @@ h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycl
                          double *final_res) {
   if (!h) return fail(H3DMG_ERR_ARG, "null handle");
   if (!h->rhs_ready) return fail(H3DMG_ERR_ARG, "rhs not set");
+  if (h3dmg_status st = check_density(h)) return st;
   const h3dmg_params &P = h->prm;
   const bool ref = P.criterion == H3DMG_CRIT_REF_RMAX;
   const double rfac_base = P.dt;
@@ -1652,6 +1683,109 @@ h3dmg_status h3dmg_get_su(h3dmg_handle *h, int b, double *su, int ld1, int ld2, 
   return H3DMG_OK;
 }
 
+// Variable-density operator: coefficient arrays and dens_mg for every level
+// descriptor, allocated on the first h3dmg_set_density*.
+namespace {
+h3dmg_status enable_lsm(h3dmg_handle *h) {
+  if (h->lsm) return H3DMG_OK;
+  if (is_mixed(h) || h->prm.coarse != H3DMG_COARSE_SMOOTH)
+    return fail(H3DMG_ERR_UNSUPPORTED,
+                "the variable-density operator runs in FP64 with the reference coarsest level");
+  std::lock_guard<std::mutex> lock(g_device_mutex);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  drop_graph(h);
+  size_t off = 0;
+  h->lsm_dens.assign(h->levs.size(), 0);
+  std::vector<size_t> coef_off(h->levs.size());
+  for (size_t d = 0; d < h->levs.size(); ++d) {
+    const LevelDev &L = h->levs[d];
+    const size_t half = (size_t)L.H * L.nj * L.nk;
+    coef_off[d] = off;
+    off += (14 * half + 31) & ~(size_t)31;
+    h->lsm_dens[d] = off;
+    off += ((size_t)L.ni * L.nj * L.nk + 31) & ~(size_t)31;
+  }
+  hipError_t e = hipMalloc((void **)&h->lsm_store, off * sizeof(double));
+  if (e != hipSuccess) {
+    h->lsm_store = nullptr;
+    return fail(H3DMG_ERR_NOMEM, "density arrays (%zu bytes): %s", off * 8, hipGetErrorString(e));
+  }
+  h->bytes += off * sizeof(double);
+  for (size_t d = 0; d < h->levs.size(); ++d) {
+    LevelDev &L = h->levs[d];
+    L.lsm = h->lsm_store + coef_off[d];
+    L.lsm_half = (unsigned long long)L.H * L.nj * L.nk;
+  }
+  HIP_TRY(hipMemcpy(h->d_levs, h->levs.data(), h->levs.size() * sizeof(LevelDev),
+                    hipMemcpyHostToDevice));
+  for (int b = 0; b < h->nb; ++b) {
+    FineDev &F = h->fine[b];
+    F.dens = h->lsm_store + h->lsm_dens[h->blk[b].desc[1]];
+    F.dx = h->blk[b].spec.dx;
+    F.dy = h->blk[b].spec.dy;
+    F.dz = h->blk[b].spec.dz;
+  }
+  HIP_TRY(hipMemcpy(h->d_fine, h->fine.data(), h->nb * sizeof(FineDev), hipMemcpyHostToDevice));
+  h->lsm_set.assign(h->nb, 0);
+  h->lsm = true;
+  // the fused sweep and the LDS tail hold no coefficients: both off
+  h->fused_max = 0;
+  for (int g = 1; g <= h->Lmax; ++g) {
+    h3dmg_status st = upload_relax(h, g);
+    if (st) return st;
+  }
+  h->tail_req = h->Lmax + 1;
+  return upload_tail(h);
+}
+
+// dens_mg of every level of block b from its level-1 density (already in
+// place), then the coefficients of every level (coeff, mgsolver.for:72-210)
+h3dmg_status build_lsm_block(h3dmg_handle *h, int b) {
+  const BlockInfo &B = h->blk[b];
+  for (int g = 2; g <= B.L; ++g) {
+    const int df = B.desc[g - 1], dc = B.desc[g];
+    const LevelDev &Lf = h->levs[df];
+    launch_lsm_dens_restrict(h->stream, h->lsm_store + h->lsm_dens[df], Lf.ni, Lf.nj, Lf.nk,
+                             h->lsm_store + h->lsm_dens[dc]);
+  }
+  int walls = 0;
+  for (int f = 0; f < 6; ++f)
+    if (B.spec.bc[f] != H3DMG_BC_PERIODIC) walls |= 1 << f;
+  for (int g = 1; g <= B.L; ++g) {
+    const int d = B.desc[g];
+    const LevelDev &L = h->levs[d];
+    launch_lsm_coef(h->stream, h->d_levs + d, h->lsm_store + h->lsm_dens[d], L.ni, L.nj, L.nk,
+                    B.spec.dx, B.spec.dy, B.spec.dz, walls);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->lsm_set[b] = 1;
+  return H3DMG_OK;
+}
+}  // namespace
+
+h3dmg_status h3dmg_set_density_device(h3dmg_handle *h, int b, const double *d_dens) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!d_dens) return fail(H3DMG_ERR_ARG, "null density");
+  st = enable_lsm(h);
+  if (st) return st;
+  const h3dmg_block &S = h->blk[b].spec;
+  HIP_TRY(hipMemcpyAsync(h->lsm_store + h->lsm_dens[h->blk[b].desc[1]], d_dens,
+                         (size_t)S.ni * S.nj * S.nk * sizeof(double), hipMemcpyDeviceToDevice,
+                         h->stream));
+  return build_lsm_block(h, b);
+}
+
+h3dmg_status h3dmg_set_density(h3dmg_handle *h, int b, const double *dens, int ld1, int ld2,
+                               int off) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  st = host_to_stage(h, b, dens, ld1, ld2, off);
+  if (st) return st;
+  return h3dmg_set_density_device(h, b, h->d_stage);
+}
+
 h3dmg_status h3dmg_set_dt(h3dmg_handle *h, double dt) {
   if (!h) return fail(H3DMG_ERR_ARG, "null handle");
   if (!(dt > 0)) return fail(H3DMG_ERR_ARG, "dt must be > 0");
@@ -1668,6 +1802,7 @@ h3dmg_status h3dmg_velocity_begin(h3dmg_handle *h, const double *const *d_ustar,
   if (!own && (!d_ustar || !d_vstar || !d_wstar || !d_u || !d_v || !d_w))
     return fail(H3DMG_ERR_ARG, "null velocity array list");
   if (!(h->prm.dt > 0)) return fail(H3DMG_ERR_ARG, "the velocity form needs params.dt > 0");
+  if (h3dmg_status st = check_density(h)) return st;
   bool same = h->vel_ready && h->vel_dta == h->prm.dt * alfapr && h->vel_dt == h->prm.dt;
   for (int b = 0; b < h->nb; ++b) {
     if (!h->blk[b].enabled) continue;
@@ -1847,7 +1982,10 @@ h3dmg_status h3dmg_dbg_restrict(h3dmg_handle *h, int level) {
   h3dmg_status st = push_active(h);
   if (st) return st;
   const bool f64 = !is_mixed(h);
-  if (h->restr_nc[level].n()) {
+  if (h->lsm) {
+    for (const TileSet *ts : {&h->restr_nc[level], &h->restr_c[level]})
+      launch_lsm_restrict(h->stream, ts->d, ts->n(), h->d_levs, h->d_active, 0);
+  } else if (h->restr_nc[level].n()) {
     if (f64)
       launch_restrict<double, double>(h->stream, h->restr_nc[level].d, h->restr_nc[level].n(),
                                       h->restr_nc[level].depth, h->d_levs, h->d_active, 0);
@@ -1855,7 +1993,7 @@ h3dmg_status h3dmg_dbg_restrict(h3dmg_handle *h, int level) {
       launch_restrict<float, float>(h->stream, h->restr_nc[level].d, h->restr_nc[level].n(),
                                     h->restr_nc[level].depth, h->d_levs, h->d_active, 0);
   }
-  if (h->restr_c[level].n()) {
+  if (!h->lsm && h->restr_c[level].n()) {
     if (f64)
       launch_restrict<double, double>(h->stream, h->restr_c[level].d, h->restr_c[level].n(),
                                       h->restr_c[level].depth, h->d_levs, h->d_active, 0);
@@ -1909,6 +2047,8 @@ h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes) {
 
 h3dmg_status h3dmg_set_tail(h3dmg_handle *h, int level) {
   if (!h || level < 0) return fail(H3DMG_ERR_ARG, "bad tail level");
+  if (h->lsm && level <= h->Lmax)
+    return fail(H3DMG_ERR_UNSUPPORTED, "the tail kernel holds no variable-density coefficients");
   std::lock_guard<std::mutex> lock(g_device_mutex);
   HIP_TRY(hipStreamSynchronize(h->stream));
   drop_graph(h);
@@ -1931,6 +2071,8 @@ int h3dmg_get_march(const h3dmg_handle *h, int level) {
 
 h3dmg_status h3dmg_set_fused(h3dmg_handle *h, long long max_cells) {
   if (!h || max_cells < 0) return fail(H3DMG_ERR_ARG, "bad fused threshold");
+  if (h->lsm && max_cells > 0)
+    return fail(H3DMG_ERR_UNSUPPORTED, "fused sweeps hold no variable-density coefficients");
   std::lock_guard<std::mutex> lock(g_device_mutex);
   HIP_TRY(hipStreamSynchronize(h->stream));
   drop_graph(h);

@@ -43,7 +43,7 @@ module h3dmg_mod
        h3dmg_set_block_tol, h3dmg_set_block_enabled, h3dmg_project_device, &
        h3dmg_velocity_begin, h3dmg_velocity_step, h3dmg_solve_velocity, &
        h3dmg_get_su_device, h3dmg_set_velocity, h3dmg_get_velocity, h3dmg_get_su, &
-       h3dmg_set_dt, h3dmg_set_fused, h3dmg_get_fused, &
+       h3dmg_set_dt, h3dmg_set_fused, h3dmg_get_fused, h3dmg_set_density, &
        h3dmg_error_message
 
   interface
@@ -222,6 +222,14 @@ module h3dmg_mod
        import :: c_ptr, c_long_long
        type(c_ptr), value :: handle
      end function h3dmg_get_fused
+
+     integer(c_int) function h3dmg_set_density(handle, b, dens, ld1, ld2, off) &
+          bind(C, name="h3dmg_set_density")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: handle
+       integer(c_int), value :: b, ld1, ld2, off
+       real(c_double), intent(in) :: dens(*)
+     end function h3dmg_set_density
 
      integer(c_int) function h3dmg_set_dt(handle, dt) bind(C, name="h3dmg_set_dt")
        import :: c_ptr, c_int, c_double

@@ -245,6 +245,18 @@ class MultigridSolver:
         L.check(self.lib.h3dmg_get_su(self.h, b, _dptr(out), ni + 2 * pl, nj + 2 * pl, pl))
         return out
 
+    def set_density(self, b, dens):
+        """Density of block b (interior (nk, nj, ni), numpy or a device
+        tensor): the reference's variable-density (L_LSM) operator."""
+        if _is_device_tensor(dens):
+            assert tuple(dens.shape) == self._shape(b) and dens.is_contiguous()
+            L.check(self.lib.h3dmg_set_density_device(self.h, b, ctypes.c_void_p(dens.data_ptr())))
+            return
+        d = np.ascontiguousarray(dens, dtype=np.float64)
+        assert d.shape == self._shape(b)
+        ni, nj, _ = self.blocks[b]["n"]
+        L.check(self.lib.h3dmg_set_density(self.h, b, _dptr(d), ni, nj, 0))
+
     def set_dt(self, dt):
         L.check(self.lib.h3dmg_set_dt(self.h, float(dt)))
 

@@ -186,6 +186,20 @@ h3dmg_status h3dmg_solve_velocity(h3dmg_handle *h, const double *const *d_ustar,
                                   double *const *d_u, double *const *d_v, double *const *d_w,
                                   double alfapr, double *hist, int hist_cap, int *ncycles,
                                   double *rmax);
+/* Density of block b (dom(ib)%dens, interior cells) switches the handle to
+ * the reference's variable-density operator (L_LSM): coeff's LSM branch
+ * (src_v70/mgsolver.for:72-184) builds dens_mg and density-weighted
+ * coefficients for every level, calvel divides by the face density
+ * (press.for:93-131) and calmas writes sup = div(u) dx dy dz / dt
+ * (press.for:53-55).  Call it for every block, and again whenever the density
+ * changes (the reference rebuilds the coefficients every solve,
+ * newsolv_mg.for:13-15).  Needs H3DMG_FP64 and H3DMG_COARSE_SMOOTH; the LDS
+ * tail and fused sweeps are turned off.  Host Fortran array as h3dmg_set_rhs
+ * takes it, or a dense device array (nk, nj, ni). */
+h3dmg_status h3dmg_set_density(h3dmg_handle *h, int b, const double *dens, int ld1, int ld2,
+                               int off);
+h3dmg_status h3dmg_set_density_device(h3dmg_handle *h, int b, const double *d_dens);
+
 /* The time step of the next solve (params.dt): the reference's dt changes
  * from step to step (checkdt.for:117-137). */
 h3dmg_status h3dmg_set_dt(h3dmg_handle *h, double dt);

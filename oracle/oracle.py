@@ -186,9 +186,11 @@ def ref_available() -> bool:
 
 
 def run_ref(block, mode, inputs, nout, nu1=2, nu2=2, level=1, nsweep=1, ncycles=1,
-            scheme=SCHEME_RB, dt=1e-3, eps=1e-6, timeout=600):
-    """Run the reference driver; `inputs` is a list of arrays written in order;
-    returns the flat float64 output (length nout, or all if nout is None)."""
+            scheme=SCHEME_RB, dt=1e-3, eps=1e-6, timeout=600, lsm=False):
+    """Run the reference driver; `inputs` is a list of arrays written in order
+    (with lsm, the interior density comes first: the reference's L_LSM
+    variable-density operator); returns the flat float64 output (length nout,
+    or all if nout is None)."""
     if not ref_available():
         raise RuntimeError(f"reference driver not built: {REF_DRIVER}")
     bcs = ",".join(str(_REF_BC[x]) for x in block.bc)
@@ -202,7 +204,7 @@ def run_ref(block, mode, inputs, nout, nu1=2, nu2=2, level=1, nsweep=1, ncycles=
                 f" mode='{mode}', ni={block.n[0]}, nj={block.n[1]}, nk={block.n[2]},\n"
                 f" nlev={block.nlev}, nu1={nu1}, nu2={nu2}, level={level}, nsw={nsweep},\n"
                 f" ncycles={ncycles}, bcs={bcs}, h={block.h[0]!r},{block.h[1]!r},{block.h[2]!r},\n"
-                f" dtin={dt!r}, epsin={eps!r}, scheme={scheme}\n/\n")
+                f" dtin={dt!r}, epsin={eps!r}, scheme={scheme}, lsm={1 if lsm else 0}\n/\n")
         with open(fin, "wb") as fh:
             for a in inputs:
                 fh.write(np.ascontiguousarray(a, dtype="<f8").tobytes())

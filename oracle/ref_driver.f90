@@ -28,6 +28,11 @@
 !             rejects) -> stop cycle, history(rmax, ||su||_2), final p
 !             (with ghosts), final su, su after the first calmas (the f of
 !             the pressure form).
+! lsm = 1 switches the reference's variable-density operator on (L_LSM):
+! the density (interior n_i*n_j*n_k, ghosts set to images) is read first,
+! coeff builds the density-weighted coefficients of every level
+! (src_v70/mgsolver.for:72-184), calvel divides by the face density and
+! calmas writes sup (press.for:53-55) into the V-cycle's rhs.
 program ref_driver
   use vars
   use multidata
@@ -37,17 +42,17 @@ program ref_driver
   character(len=512) :: fjob, fin, fout
   character(len=16)  :: mode
   integer :: ni, nj, nk, nlev, nu1, nu2, level, nsw, ncycles, bcs(6)
-  integer :: scheme, pl_extra
+  integer :: scheme, pl_extra, lsm
   double precision :: h(3), dtin, epsin
   namelist /job/ mode, ni, nj, nk, nlev, nu1, nu2, level, nsw, ncycles, &
-       bcs, h, dtin, epsin, scheme, pl_extra
+       bcs, h, dtin, epsin, scheme, pl_extra, lsm
 
   integer :: uin, uout, ib, g
   double precision, allocatable :: a3(:,:,:), b3(:,:,:)
 
   mode = 'vcycle'; ni = 8; nj = 8; nk = 8; nlev = 1; nu1 = 2; nu2 = 2
   level = 1; nsw = 1; ncycles = 1; bcs = 1; h = 1.0d0; dtin = 1.0d-3
-  epsin = 1.0d-6; scheme = 3; pl_extra = 1
+  epsin = 1.0d-6; scheme = 3; pl_extra = 1; lsm = 0
 
   call get_command_argument(1, fjob)
   call get_command_argument(2, fin)
@@ -62,10 +67,11 @@ program ref_driver
   MPI_FLT = MPI_DOUBLE_PRECISION
 
   call setup_block()
-  call coeff
 
   open(newunit=uin, file=trim(fin), access='stream', form='unformatted', &
        status='old', action='read')
+  if (lsm /= 0) call read_density()
+  call coeff
   open(newunit=uout, file=trim(fout), access='stream', form='unformatted', &
        status='replace', action='write')
 
@@ -107,7 +113,7 @@ contains
     ngrid_input = nlev; ngrd_gl = nlev
     allocate(rdv(0:0, ngrd_gl)); rdv = 1
     maxcy = ncycles; irestr = nu1; iproln = nu2; mg_itrsch = scheme
-    solver = 2; L_LSM = .false.; L_LSMbase = .false.
+    solver = 2; L_LSM = lsm /= 0; L_LSMbase = .false.
     differencing = 1; dt = dtin; alfapr = 1.0d0; eps = epsin
     bc_w = bcs(1); bc_e = bcs(2); bc_s = bcs(3); bc_n = bcs(4)
     bc_b = bcs(5); bc_t = bcs(6)
@@ -174,6 +180,19 @@ contains
     dom(b)%p = 0; dom(b)%pp = 0; dom(b)%su = 0
     dom(b)%u = 0; dom(b)%v = 0; dom(b)%w = 0
     dom(b)%ustar = 0; dom(b)%vstar = 0; dom(b)%wstar = 0
+    allocate(dom(b)%sup(tt(1),tt(2),tt(3))); dom(b)%sup = 0
+    if (L_LSM) then
+       ! as initial.for:196-214 allocates them
+       allocate(dom(b)%dens(tt(1),tt(2),tt(3)), dom(b)%ijkp_lsm(0:dom(b)%ngrid))
+       dom(b)%ijkp_lsm = 0
+       dom(b)%ijkp_lsm(1) = ni*nj*nk
+       do g = 2, dom(b)%ngrid
+          dom(b)%ijkp_lsm(g) = dom(b)%ijkp_lsm(g-1) + &
+               (ni/2**(g-1))*(nj/2**(g-1))*(nk/2**(g-1))
+       end do
+       dom(b)%tot = dom(b)%ijkp_lsm(dom(b)%ngrid)
+       allocate(dom(b)%dens_mg(dom(b)%tot))
+    end if
 
     allocate(dom(b)%sendb_m1(ngg), dom(b)%sendb_p1(ngg))
     allocate(dom(b)%recvb_m1(ngg), dom(b)%recvb_p1(ngg))
@@ -192,6 +211,47 @@ contains
     allocate(dom(b)%se5m(nge), dom(b)%se5p(nge), dom(b)%re5m(nge), dom(b)%re5p(nge))
     allocate(dom(b)%se6m(nge), dom(b)%se6p(nge), dom(b)%re6m(nge), dom(b)%re6p(nge))
   end subroutine setup_block
+
+  ! density of the interior cells; the ghost layers hold images (mirror at
+  ! walls, wrap at periodic faces), as the LES's exchange leaves them
+  subroutine read_density()
+    double precision, allocatable :: d(:,:,:)
+    integer :: i, j, k, ii, jj, kk, isp, jsp, ksp
+    allocate(d(ni, nj, nk))
+    read(uin) d
+    isp = dom(1)%isp; jsp = dom(1)%jsp; ksp = dom(1)%ksp
+    do k = ksp - pl, ksp + nk - 1 + pl
+       do j = jsp - pl, jsp + nj - 1 + pl
+          do i = isp - pl, isp + ni - 1 + pl
+             ii = img(i - isp + 1, ni, bcs(1) == 5)
+             jj = img(j - jsp + 1, nj, bcs(3) == 5)
+             kk = img(k - ksp + 1, nk, bcs(5) == 5)
+             dom(1)%dens(i, j, k) = d(ii, jj, kk)
+          end do
+       end do
+    end do
+  end subroutine read_density
+
+  ! 1-based image of x in [1-pl, n+pl]
+  integer function img(x, n, periodic)
+    integer, intent(in) :: x, n
+    logical, intent(in) :: periodic
+    if (x < 1) then
+       if (periodic) then
+          img = x + n
+       else
+          img = 1 - x
+       end if
+    else if (x > n) then
+       if (periodic) then
+          img = x - n
+       else
+          img = 2*n + 1 - x
+       end if
+    else
+       img = x
+    end if
+  end function img
 
   ! ------------------------------------------------------------------
   ! Level geometry and packing between the reference's j-fastest packed
@@ -276,6 +336,7 @@ contains
     dom(1)%pp = 0.0d0
     do k = 1, nk; do j = 1, nj; do i = 1, ni
        dom(1)%su(i+pl, j+pl, k+pl) = a3(i, j, k)
+       dom(1)%sup(i+pl, j+pl, k+pl) = a3(i, j, k)
     end do; end do; end do
     call mgkcyc
     allocate(pad(ni+2, nj+2, nk+2))
@@ -367,7 +428,7 @@ contains
     hist(1, 0) = rmax; hist(2, 0) = su_norm()
     allocate(su0(ni, nj, nk))
     do k = 1, nk; do j = 1, nj; do i = 1, ni
-       su0(i, j, k) = dom(1)%su(isp-1+i, jsp-1+j, ksp-1+k)
+       su0(i, j, k) = rhs_field(isp-1+i, jsp-1+j, ksp-1+k)
     end do; end do; end do
     dom(1)%pp = 0.0d0
     do it = 1, ncycles
@@ -395,7 +456,7 @@ contains
        pout(i, j, k) = dom(1)%p(isp-2+i, jsp-2+j, ksp-2+k)
     end do; end do; end do
     do k = 1, nk; do j = 1, nj; do i = 1, ni
-       sout(i, j, k) = dom(1)%su(isp-1+i, jsp-1+j, ksp-1+k)
+       sout(i, j, k) = rhs_field(isp-1+i, jsp-1+j, ksp-1+k)
     end do; end do; end do
     write(uout) dble(stop_at)
     write(uout) hist
@@ -411,11 +472,21 @@ contains
     do k = dom(1)%ksp, dom(1)%kep
        do j = dom(1)%jsp, dom(1)%jep
           do i = dom(1)%isp, dom(1)%iep
-             s = s + dom(1)%su(i,j,k)**2
+             s = s + rhs_field(i,j,k)**2
           end do
        end do
     end do
     su_norm = sqrt(s)
   end function su_norm
+
+  ! the V-cycle's right-hand side as calmas leaves it: su, or sup under L_LSM
+  double precision function rhs_field(i, j, k)
+    integer, intent(in) :: i, j, k
+    if (L_LSM) then
+       rhs_field = dom(1)%sup(i, j, k)
+    else
+       rhs_field = dom(1)%su(i, j, k)
+    end if
+  end function rhs_field
 
 end program ref_driver

@@ -114,6 +114,72 @@ def make_newsolv(name, cfg, ncycles, keep_p=True):
     np.savez_compressed(os.path.join(HERE, f"newsolv_{name}.npz"), **d)
 
 
+def lsm_density(n, seed):
+    """A positive density with a sharp two-phase interface (1 : 1000 across
+    z = 0.5, as water under air in the LSM benchmarks) and log-normal
+    noise, interior (nk, nj, ni)."""
+    rng = np.random.default_rng(seed)
+    ni, nj, nk = n
+    z = (np.arange(nk) + 0.5) / nk
+    phase = 1.0 + 999.0 * (z < 0.5)
+    return phase[:, None, None] * np.exp(0.3 * rng.standard_normal((nk, nj, ni)))
+
+
+def make_lsm_ops(name, n, bc, L, seed):
+    """Operators of the variable-density (L_LSM) multigrid: coeff's density-
+    weighted coefficients (mgsolver.for:72-184) under relax, restriction and
+    one V-cycle (prolongation has no coefficients)."""
+    rng = np.random.default_rng(seed)
+    B = O.Block(n, tuple(1.0 / x for x in n), bc, L)
+    dens = lsm_density(n, seed + 1)
+    out = dict(n=np.array(n), bc=np.array(bc), nlev=L, h=np.array(B.h), dens=dens)
+    su = rng.standard_normal(B.interior_shape(1))
+    su -= su.mean()
+    pp = O.run_ref(B, "vcycle", [dens, su], int(np.prod(B.padded_shape(1))), lsm=True)
+    out["vcycle_su"] = su
+    out["vcycle_pp"] = pp.reshape(B.padded_shape(1))
+    for g in range(1, L + 1):
+        phi = image_ghosts(rng.standard_normal(B.padded_shape(g)), bc)
+        rhs = rng.standard_normal(B.interior_shape(g))
+        res = O.run_ref(B, "relax", [dens, phi, rhs], phi.size, level=g, nsweep=2, lsm=True)
+        out[f"relax{g}_in_phi"] = phi
+        out[f"relax{g}_in_rhs"] = rhs
+        out[f"relax{g}_out"] = res.reshape(phi.shape)
+        if g > 1:
+            phif = image_ghosts(rng.standard_normal(B.padded_shape(g - 1)), bc)
+            rhsf = rng.standard_normal(B.interior_shape(g - 1))
+            npc = int(np.prod(B.padded_shape(g)))
+            res = O.run_ref(B, "restr", [dens, phif, rhsf], None, level=g, lsm=True)
+            out[f"restr{g}_in_phi"] = phif
+            out[f"restr{g}_in_rhs"] = rhsf
+            out[f"restr{g}_out_rhs"] = res[npc:].reshape(B.interior_shape(g))
+    np.savez_compressed(os.path.join(HERE, f"lsm_ops_{name}.npz"), **out)
+
+
+def make_lsm_newsolv(name, cfg, ncycles):
+    """The velocity-form loop under L_LSM: calvel divides by the face
+    density, calmas writes sup = div(u) dx dy dz / dt (press.for:53-55)."""
+    n, bc, L = cfg["n"], cfg["bc"], cfg["nlev"]
+    B = O.Block(n, cfg["h"], bc, L)
+    ni, nj, nk = n
+    npad = (ni + 2) * (nj + 2) * (nk + 2)
+    nint = ni * nj * nk
+    dens = lsm_density(n, 77)
+    res = O.run_ref(B, "newsolv", [dens, cfg["u"], cfg["v"], cfg["w"], np.zeros((nk, nj, ni))],
+                    1 + 2 * (ncycles + 1) + npad + 2 * nint, ncycles=ncycles, dt=cfg["dt"],
+                    eps=1e-6, lsm=True)
+    q = 0
+    stop_at = int(res[q]); q += 1
+    hist = res[q:q + 2 * (ncycles + 1)].reshape(ncycles + 1, 2); q += 2 * (ncycles + 1)
+    p = res[q:q + npad].reshape(nk + 2, nj + 2, ni + 2); q += npad
+    sup = res[q:q + nint].reshape(nk, nj, ni); q += nint
+    sup0 = res[q:q + nint].reshape(nk, nj, ni)
+    np.savez_compressed(os.path.join(HERE, f"lsm_newsolv_{name}.npz"),
+                        n=np.array(n), bc=np.array(bc), nlev=L, h=np.array(cfg["h"]),
+                        dt=cfg["dt"], ncycles=ncycles, stop_at=stop_at, rmax=hist[:, 0],
+                        dens=dens, p=p, sup0=sup0, final_sup=sup)
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("reference driver not built: make -C oracle ref")
@@ -126,6 +192,12 @@ def main():
     # the 64^3 workload itself (4 levels): history and checksums
     make_newsolv("ppw64", C.single_64(), ncycles=30, keep_p=False)
     print("wrote newsolv ppw64")
+    # the variable-density (L_LSM) operator
+    make_lsm_ops("ppw16", (16, 16, 16), (P, P, P, P, W, W), 3, seed=300)
+    make_lsm_ops("wall8x16x8", (8, 16, 8), (W, W, W, W, W, W), 2, seed=301)
+    print("wrote lsm ops")
+    make_lsm_newsolv("ppw16", C.cube(16, C.BC_PPN, 3, seed=5), ncycles=10)
+    print("wrote lsm newsolv")
 
 
 if __name__ == "__main__":

@@ -82,6 +82,84 @@ def test_gpu_operators_bitwise_vs_reference(gpu_lib, path):
         assert np.array_equal(pp, z["vcycle_s3_pp"]), "vcycle"
 
 
+LSM_OPS = sorted(glob.glob(os.path.join(GOLDEN, "lsm_ops_*.npz")))
+
+
+@pytest.mark.parametrize("path", LSM_OPS, ids=[os.path.basename(p) for p in LSM_OPS])
+def test_gpu_lsm_operators_bitwise_vs_reference(gpu_lib, path):
+    """Variable-density (L_LSM) operator: from the reference build's density,
+    the per-level coefficients (coeff's LSM branch, dens_mg restricted level
+    by level) under relax, restriction and a whole V-cycle are bitwise the
+    reference's."""
+    import torch
+    from hydro3d_amd import _lib as L
+    z = load_golden(os.path.basename(path))
+    spec = _blockspec(z)
+    Lv = spec["nlev"]
+    print(f"case {os.path.basename(path)} {spec}", flush=True)
+    with _solver(spec, use_graph=False) as S:
+        S.set_density(0, z["dens"])
+        for g in range(1, Lv + 1):
+            phi, rhs = z[f"relax{g}_in_phi"], z[f"relax{g}_in_rhs"]
+            S.dbg_put(0, g, L.ARR_PHI, _dev(phi))
+            S.dbg_put(0, g, L.ARR_RHS, _dev(rhs))
+            S.dbg_relax(g, 2)
+            out = _host(S.dbg_get(0, g, L.ARR_PHI, torch.empty(phi.shape, dtype=torch.float64,
+                                                                 device="cuda")))
+            assert np.array_equal(out, z[f"relax{g}_out"]), f"relax level {g}"
+            if g > 1:
+                S.dbg_put(0, g - 1, L.ARR_PHI, _dev(z[f"restr{g}_in_phi"]))
+                S.dbg_put(0, g - 1, L.ARR_RHS, _dev(z[f"restr{g}_in_rhs"]))
+                S.dbg_restrict(g)
+                rc = _host(S.dbg_get(0, g, L.ARR_RHS, torch.empty(z[f"restr{g}_out_rhs"].shape,
+                                                                    dtype=torch.float64,
+                                                                    device="cuda")))
+                assert np.array_equal(rc, z[f"restr{g}_out_rhs"]), f"restr rhs level {g}"
+        S.set_rhs(0, z["vcycle_su"])
+        S.vcycle()
+        assert np.array_equal(S.get_pp(0), z["vcycle_pp"]), "vcycle"
+
+
+def test_gpu_lsm_velocity_loop_bitwise_vs_reference(gpu_lib):
+    """The velocity-form loop under L_LSM (calvel over the face density,
+    calmas writing sup, the density-weighted V-cycle): rmax of every
+    iteration, the final pressure and the final sup are bitwise the reference
+    build's."""
+    from hydro3d_amd import configs as C
+    from hydro3d_amd import _lib as L
+    z = load_golden("lsm_newsolv_ppw16.npz")
+    c = C.cube(16, C.BC_PPN, 3, seed=5)
+    spec = _blockspec(z)
+    nc = int(z["ncycles"])
+    with _solver(spec, maxcy=nc, tol=0.0, dt=float(z["dt"])) as S:
+        S.set_density(0, z["dens"])
+        S.set_p(0, np.zeros(c["f"].shape))
+        st, ran, hist, _ = S.solve_velocity([_dev(c["u"])], [_dev(c["v"])], [_dev(c["w"])])
+        p = S.get_p(0)
+        sup = _host(S.get_su(0))
+    assert st == L.ERR_NOT_CONVERGED and ran == nc
+    assert np.array_equal(hist, z["rmax"]), (hist, z["rmax"])
+    assert np.array_equal(p, z["p"])
+    assert np.array_equal(sup, z["final_sup"])
+
+
+def test_gpu_lsm_refuses_mixed_and_tail(gpu_lib):
+    from hydro3d_amd import _lib as L
+    spec = dict(n=(16, 16, 16), h=(1 / 16,) * 3, bc=(1, 1, 1, 1, 0, 0), nlev=3)
+    dens = np.ones((16, 16, 16))
+    with _solver(spec, precision="mixed", coarse="exact") as S:
+        with pytest.raises(L.H3dError) as e:
+            S.set_density(0, dens)
+        assert e.value.status == L.ERR_UNSUPPORTED
+    with _solver(spec) as S:
+        S.set_density(0, dens)
+        assert S.get_tail() == S.nlevels_max + 1
+        with pytest.raises(L.H3dError):
+            S.set_tail(1)
+        with pytest.raises(L.H3dError):
+            S.set_fused(1 << 20)
+
+
 def test_gpu_solve_tracks_reference_velocity_form(gpu_lib):
     from oracle import oracle as O
     z = load_golden("newsolv_ppw32.npz")
