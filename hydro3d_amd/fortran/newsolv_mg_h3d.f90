@@ -15,12 +15,15 @@
 ! (exchange(4), pbound, and calvel's exchange(1..3)), and keeps the
 ! not-converged / BIG RMAX handling, boundu/v/w and the log lines.
 !
-! Scope: the constant-coefficient RBGS multigrid on blocks without
-! neighbours (every face a wall or periodic onto itself), one MPI rank per
-! GPU.  Anything else stops with a message rather than running a different
-! algorithm.  The precision is FP64 (bitwise the reference loop) unless the
-! environment variable H3DMG_PRECISION=mixed selects the FP32 V-cycle with
-! the exact coarsest solve inside the FP64 velocity loop.
+! Scope: the RBGS multigrid, constant-coefficient or variable-density
+! (L_LSM: the density goes to the device every call, as the reference
+! rebuilds its coefficients every solve, newsolv_mg.for:13-15), on blocks
+! without neighbours (every face a wall or periodic onto itself), one MPI
+! rank per GPU.  Anything else stops with a message rather than running a
+! different algorithm.  The precision is FP64 (bitwise the reference loop)
+! unless the environment variable H3DMG_PRECISION=mixed selects the FP32
+! V-cycle with the exact coarsest solve inside the FP64 velocity loop
+! (constant coefficients only).
 subroutine newsolv_mg_h3d
   use vars
   use mpi
@@ -35,7 +38,6 @@ subroutine newsolv_mg_h3d
   logical :: converged
   double precision :: wtimedum
 
-  if (L_LSM) call h3d_stop('the variable-density (LSM) operator is not on the device path')
   if (.not. c_associated(h)) call setup()
   st = h3dmg_set_dt(h, dt)
   call chk(st, 'set_dt')
@@ -47,6 +49,10 @@ subroutine newsolv_mg_h3d
        call chk(st, 'set_velocity')
        st = h3dmg_set_p(h, ib - 1, d%p, d%ttc_i, d%ttc_j, d%isp - 1)
        call chk(st, 'set_p')
+       if (L_LSM) then
+          st = h3dmg_set_density(h, ib - 1, d%dens, d%ttc_i, d%ttc_j, d%isp - 1)
+          call chk(st, 'set_density')
+       end if
        d%pp = 0.0d0
      end associate
   end do
@@ -75,7 +81,11 @@ subroutine newsolv_mg_h3d
        call chk(st, 'get_p')
        st = h3dmg_get_velocity(h, ib - 1, d%u, d%v, d%w, d%ttc_i, d%ttc_j, d%isp - 1)
        call chk(st, 'get_velocity')
-       st = h3dmg_get_su(h, ib - 1, d%su, d%ttc_i, d%ttc_j, d%isp - 1)
+       if (L_LSM) then
+          st = h3dmg_get_su(h, ib - 1, d%sup, d%ttc_i, d%ttc_j, d%isp - 1)
+       else
+          st = h3dmg_get_su(h, ib - 1, d%su, d%ttc_i, d%ttc_j, d%isp - 1)
+       end if
        call chk(st, 'get_su')
      end associate
   end do
@@ -148,7 +158,7 @@ contains
     prm%dt = dt
     prm%min_cycles = 2
     call get_environment_variable('H3DMG_PRECISION', pv, plen, pst)
-    if (pst == 0 .and. pv(1:5) == 'mixed') then
+    if (pst == 0 .and. pv(1:5) == 'mixed' .and. .not. L_LSM) then
        prm%precision = H3DMG_MIXED
        prm%coarse = H3DMG_COARSE_EXACT
     end if
