@@ -50,6 +50,9 @@ struct LevelDev {
   // colour arrays of lsm_half doubles laid out like phi (array 2*q + colour)
   double *lsm;
   unsigned long long lsm_half;
+  // line-TDMA smoother scratch of the block (2 * level-1 cells), or null:
+  // the Thomas coefficients a and c of the line being solved
+  double *line;
 };
 
 // Finest-level FP64 state of one block (pressure, right-hand side).
@@ -152,6 +155,17 @@ void launch_lsm_residual(hipStream_t s, const Tile *tiles, int ntiles, const Fin
                          const LevelDev *levs, const int *active, RedPartial *partials);
 void launch_lsm_restrict(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                          const int *active, int src_f);
+
+// ---- lexicographic GS and line-TDMA smoothers (mg_itrsch = 1, 2) ---------
+// Tiles: GS 64 x 4 (i, j) tiles of every descriptor (Tile::x0, y0); TDMA
+// tiles of NTHREADS lines (Tile::x0 = first line).  One launch per
+// hyperplane d = i+j+k (GS) or line diagonal d (TDMA pass 0: j-lines,
+// i+k = d; 1: i-lines, j+k = d; 2: k-lines, i+j = d).  A sweep reads
+// buffer src and writes 1-src; every level is FP64.
+void launch_gs_plane(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                     const int *active, int d, int src, int zero);
+void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                      const int *active, int pass, int d, int src, int zero);
 
 // velocity projection (calvel per axis over `faces` faces; calmas writes
 // su (optional) and one partial max per workgroup, returns the group count)

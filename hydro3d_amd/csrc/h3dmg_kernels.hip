@@ -2156,6 +2156,198 @@ __global__ void __launch_bounds__(256)
 
 inline dim3 tile_block() { return dim3(TILE_X, TILE_Y, 1); }
 
+
+// ---------------------------------------------------------------------------
+// Lexicographic Gauss-Seidel (mgsolver.for:407-432) and line TDMA
+// (mgsolver.for:434-549), mg_itrsch = 1 and 2.
+//
+// The reference updates in place, in loop order.  A GS cell (loop k, j, i)
+// reads its W, S and B neighbours after their update and E, N, T before it,
+// so the cells of the hyperplane i+j+k = d depend on plane d-1 (updated) and
+// plane d+1 (not yet) only: one launch per hyperplane performs exactly the
+// reference's arithmetic.  The TDMA passes are line-wise GS over the other
+// two axes: j-lines (loop k, i) on diagonals i+k, i-lines (loop k, j) on
+// j+k, k-lines (loop i, j) on i+j.  A sweep reads buffer src and writes
+// 1-src:
+//   GS and the j-line pass: values not yet updated from src, updated ones
+//     from 1-src;
+//   i-line and k-line passes: in place in 1-src;
+//   ghosts, every pass: their images in src -- the pre-sweep state the
+//     reference's ghosts keep until mgbound + exchangepp at the end of the
+//     sweep (mgsolver.for:614-615).
+// zero != 0: buffer src holds zeros.
+
+// coefficients of one cell: the level's constant operator, or the
+// variable-density arrays
+__device__ __forceinline__ Coef<double> smoother_coef(const LevelDev &L, int i, int j, int k) {
+  if (!L.lsm) return cell_coef<double>(L, i, j, k);
+  const int colour = (i + j + k) & 1;
+  const size_t e = cidx(L.H, L.nj, i, j, k);
+  Coef<double> c;
+  c.aw = lsm_coef(L, 0, colour)[e];
+  c.ae = lsm_coef(L, 1, colour)[e];
+  c.as = lsm_coef(L, 2, colour)[e];
+  c.an = lsm_coef(L, 3, colour)[e];
+  c.ab = lsm_coef(L, 4, colour)[e];
+  c.at = lsm_coef(L, 5, colour)[e];
+  c.ap = lsm_coef(L, 6, colour)[e];
+  return c;
+}
+
+__device__ __forceinline__ double cell_at(const LevelDev &L, int buf, int i, int j, int k) {
+  return ((const double *)L.phi[buf][(i + j + k) & 1])[cidx(L.H, L.nj, i, j, k)];
+}
+
+// ghost or not-yet-updated value in buffer src
+__device__ __forceinline__ double before_sweep(const LevelDev &L, int src, int zero, int x, int y,
+                                               int z) {
+  return zero ? 0.0 : level_value<double>(L, src, x, y, z);
+}
+
+// phi = (rhs - (aw*W + ae*E + as*S + an*N + ab*B + at*T)) / ap at the cells
+// of hyperplane d
+__global__ void __launch_bounds__(NTHREADS)
+    k_gs_plane(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+               const int *__restrict__ active, int d, int src, int zero) {
+  const Tile t = tiles[blockIdx.x];
+  const LevelDev &L = levs[t.lev];
+  if (!active[L.blk]) return;
+  const int i = t.x0 + (int)threadIdx.x, j = t.y0 + (int)threadIdx.y, k = d - i - j;
+  if (i >= L.ni || j >= L.nj || k < 0 || k >= L.nk) return;
+  const int dst = 1 - src;
+  const Coef<double> c = smoother_coef(L, i, j, k);
+  const double W = i > 0 ? cell_at(L, dst, i - 1, j, k) : before_sweep(L, src, zero, i - 1, j, k);
+  const double S = j > 0 ? cell_at(L, dst, i, j - 1, k) : before_sweep(L, src, zero, i, j - 1, k);
+  const double B = k > 0 ? cell_at(L, dst, i, j, k - 1) : before_sweep(L, src, zero, i, j, k - 1);
+  const double E = before_sweep(L, src, zero, i + 1, j, k);
+  const double N = before_sweep(L, src, zero, i, j + 1, k);
+  const double T = before_sweep(L, src, zero, i, j, k + 1);
+  const int colour = (i + j + k) & 1;
+  const size_t e = cidx(L.H, L.nj, i, j, k);
+  const double r = ((const double *)L.rhs[colour])[e];
+  const double sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * T;
+  ((double *)L.phi[dst][colour])[e] = (r - sum) / c.ap;
+}
+
+// positions of one line whose neighbour loads are issued together
+constexpr int TDMA_CHUNK = 8;
+
+// cell (i, j, k) of position p on the line (u, v) of pass PASS
+template <int PASS>
+__device__ __forceinline__ void tdma_cell(int u, int v, int p, int &i, int &j, int &k) {
+  if (PASS == 0) {
+    i = u, j = p, k = v;
+  } else if (PASS == 1) {
+    i = p, j = u, k = v;
+  } else {
+    i = u, j = v, k = p;
+  }
+}
+
+// One line of a TDMA pass, the Thomas recurrence of the reference
+// (a(0) = 0, c(0) = the ghost):
+//   apr = 1/(ap - alo*a(p-1)),  a(p) = ahi*apr,
+//   c(p) = (rhs - x1*X1 - x2*X2 - x3*X3 - x4*X4 - alo*c(p-1))*apr,
+//   phi(p) = c(p) - a(p)*phi(p+1),  phi(n) = the ghost,
+// with (alo, ahi) = (as, an), (aw, ae), (ab, at) and the cross terms in the
+// reference's order: j-lines ae*E, aw*W, at*T, ab*B (mgsolver.for:459-466);
+// i-lines an*N, as*S, at*T, ab*B (:496-503); k-lines an*N, as*S, ae*E, aw*W
+// (:533-540).  The cross neighbours on the previous diagonal (W, B; S, B;
+// W, S) are updated ones.  a and c go to the line scratch at p*nu + u and
+// cells + p*nu + u (consecutive lines, consecutive addresses).
+template <int PASS>
+__device__ void tdma_line(const LevelDev &L, int u, int v, int nu, int len, int src, int zero) {
+  const int dst = 1 - src;
+  const size_t cells = (size_t)L.ni * L.nj * L.nk;
+  double *sa = L.line, *sc = L.line + cells;
+  // interior value of a cross neighbour: updated ones and (after the first
+  // pass) not-yet-updated ones in dst, the first pass's others in src
+  auto cross = [&](int x, int y, int z, bool updated) -> double {
+    if ((unsigned)x >= (unsigned)L.ni || (unsigned)y >= (unsigned)L.nj ||
+        (unsigned)z >= (unsigned)L.nk)
+      return before_sweep(L, src, zero, x, y, z);
+    if (updated || PASS > 0) return cell_at(L, dst, x, y, z);
+    return zero ? 0.0 : cell_at(L, src, x, y, z);
+  };
+  int i, j, k;
+  tdma_cell<PASS>(u, v, -1, i, j, k);
+  double aprev = 0.0, cprev = before_sweep(L, src, zero, i, j, k);
+  for (int p0 = 0; p0 < len; p0 += TDMA_CHUNK) {
+    double rs[TDMA_CHUNK], ap[TDMA_CHUNK], alo[TDMA_CHUNK], ahi[TDMA_CHUNK];
+#pragma unroll
+    for (int q = 0; q < TDMA_CHUNK; ++q) {
+      const int p = p0 + q;
+      if (p >= len) break;
+      tdma_cell<PASS>(u, v, p, i, j, k);
+      const Coef<double> c = smoother_coef(L, i, j, k);
+      const double r = ((const double *)L.rhs[(i + j + k) & 1])[cidx(L.H, L.nj, i, j, k)];
+      if (PASS == 0) {
+        rs[q] = r - c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true) -
+                c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
+        alo[q] = c.as;
+        ahi[q] = c.an;
+      } else if (PASS == 1) {
+        rs[q] = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
+                c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
+        alo[q] = c.aw;
+        ahi[q] = c.ae;
+      } else {
+        rs[q] = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
+                c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true);
+        alo[q] = c.ab;
+        ahi[q] = c.at;
+      }
+      ap[q] = c.ap;
+    }
+#pragma unroll
+    for (int q = 0; q < TDMA_CHUNK; ++q) {
+      const int p = p0 + q;
+      if (p >= len) break;
+      const double apr = 1.0 / (ap[q] - alo[q] * aprev);
+      aprev = ahi[q] * apr;
+      cprev = (rs[q] - alo[q] * cprev) * apr;
+      sa[(size_t)p * nu + u] = aprev;
+      sc[(size_t)p * nu + u] = cprev;
+    }
+  }
+  tdma_cell<PASS>(u, v, len, i, j, k);
+  double next = before_sweep(L, src, zero, i, j, k);
+  for (int p1 = len - 1; p1 >= 0; p1 -= TDMA_CHUNK) {
+    double av[TDMA_CHUNK], cv[TDMA_CHUNK];
+#pragma unroll
+    for (int q = 0; q < TDMA_CHUNK; ++q) {
+      const int p = p1 - q;
+      if (p < 0) break;
+      av[q] = sa[(size_t)p * nu + u];
+      cv[q] = sc[(size_t)p * nu + u];
+    }
+#pragma unroll
+    for (int q = 0; q < TDMA_CHUNK; ++q) {
+      const int p = p1 - q;
+      if (p < 0) break;
+      next = cv[q] - av[q] * next;
+      tdma_cell<PASS>(u, v, p, i, j, k);
+      ((double *)L.phi[dst][(i + j + k) & 1])[cidx(L.H, L.nj, i, j, k)] = next;
+    }
+  }
+}
+
+template <int PASS>
+__global__ void __launch_bounds__(NTHREADS)
+    k_tdma_diag(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+                const int *__restrict__ active, int d, int src, int zero) {
+  const Tile t = tiles[blockIdx.x];
+  const LevelDev &L = levs[t.lev];
+  if (!active[L.blk]) return;
+  const int u = t.x0 + (int)threadIdx.x + TILE_X * (int)threadIdx.y;
+  const int v = d - u;
+  // lines: pass 0 (i, k) along j, 1 (j, k) along i, 2 (i, j) along k
+  const int nu = PASS == 1 ? L.nj : L.ni;
+  const int nv = PASS == 2 ? L.nj : L.nk;
+  const int len = PASS == 0 ? L.nj : PASS == 1 ? L.ni : L.nk;
+  if (u < nu && v >= 0 && v < nv) tdma_line<PASS>(L, u, v, nu, len, src, zero);
+}
+
 }  // namespace
 
 // ---- launchers -----------------------------------------------------------
@@ -2187,6 +2379,20 @@ void launch_lsm_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_lsm_sweep, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour,
                      src, zero);
+}
+
+void launch_gs_plane(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                     const int *active, int d, int src, int zero) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_gs_plane, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, d, src,
+                     zero);
+}
+
+void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                      const int *active, int pass, int d, int src, int zero) {
+  if (ntiles <= 0) return;
+  auto k = pass == 0 ? k_tdma_diag<0> : pass == 1 ? k_tdma_diag<1> : k_tdma_diag<2>;
+  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, d, src, zero);
 }
 
 void launch_lsm_residual(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,

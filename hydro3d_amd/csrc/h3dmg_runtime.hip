@@ -55,6 +55,7 @@ struct TileSet {
   int depth = 1;          // smoother sets: planes of every tile (1, 2 or 4)
   bool fused = false;     // smoother sets: fused red-black tiles (one launch per sweep)
   const TileSet *nat = nullptr;  // smoother sets: the group's natural tiles (variable density)
+  int ndiag[3] = {0, 0, 0};      // GS / TDMA sets: launches per sweep (pass)
   int n() const { return (int)h.size(); }
 };
 
@@ -269,7 +270,15 @@ bool is_mixed(const h3dmg_handle *h) { return h->prm.precision == H3DMG_MIXED; }
 template <class T>
 void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool &zero) {
   for (int s = 0; s < nsweep; ++s) {
-    if (h->lsm)  // per-cell coefficients (every level FP64)
+    if (h->prm.smoother == H3DMG_SMOOTH_GS)  // hyperplane wavefront (every level FP64)
+      for (int d = 0; d < ts.ndiag[0]; ++d)
+        launch_gs_plane(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, d, buf, zero ? 1 : 0);
+    else if (h->prm.smoother == H3DMG_SMOOTH_TDMA)  // j-, i-, k-line diagonals
+      for (int pass = 0; pass < 3; ++pass)
+        for (int d = 0; d < ts.ndiag[pass]; ++d)
+          launch_tdma_diag(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, pass, d, buf,
+                           zero ? 1 : 0);
+    else if (h->lsm)  // per-cell coefficients (every level FP64)
       for (int colour = 0; colour < 2; ++colour)
         launch_lsm_sweep(h->stream, ts.nat->d, ts.nat->n(), h->d_levs, h->d_active, colour, buf,
                          zero ? 1 : 0);
@@ -651,6 +660,15 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       off = align_up(off + n3 * 8, 256);
     }
   }
+  // line-TDMA scratch: per block 2 x its level-1 cells (a and c of the
+  // Thomas recurrence; every level of the block reuses it)
+  std::vector<size_t> line_off(nblocks, 0);
+  if (h->prm.smoother == H3DMG_SMOOTH_TDMA)
+    for (int b = 0; b < nblocks; ++b) {
+      const BlockInfo &B = h->blk[b];
+      line_off[b] = off;
+      off = align_up(off + 2 * (size_t)B.spec.ni * B.spec.nj * B.spec.nk * 8, 256);
+    }
   // staging: largest padded level-1 array
   size_t stage = 0;
   for (int b = 0; b < nblocks; ++b)
@@ -798,6 +816,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     for (int bb = 0; bb < 2; ++bb)
       for (int c = 0; c < 2; ++c) L.phi[bb][c] = A + P.phi[bb][c];
     for (int c = 0; c < 2; ++c) L.rhs[c] = A + P.rhs[c];
+    if (h->prm.smoother == H3DMG_SMOOTH_TDMA) L.line = (double *)(A + line_off[L.blk]);
   }
   h->fine.resize(nblocks);
   for (int b = 0; b < nblocks; ++b) {
@@ -893,6 +912,8 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     if (st != H3DMG_OK) return st;
   }
   {
+    // the tail kernel runs the red-black smoother only
+    if (h->prm.smoother != H3DMG_SMOOTH_RBGS) h->tail_req = Lm + 1;
     h3dmg_status st = upload_tail(h);
     if (st != H3DMG_OK) return st;
   }
@@ -927,8 +948,30 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
         const LevelDev &L = h->levs[B.desc[g]];
         most = std::max(most, (long)L.ni * L.nj * L.nk);
       }
-    ts.fused = most > 0 && most <= h->fused_max;
-    if (ts.fused) {
+    ts.fused = most > 0 && most <= h->fused_max && h->prm.smoother == H3DMG_SMOOTH_RBGS;
+    ts.ndiag[0] = ts.ndiag[1] = ts.ndiag[2] = 0;
+    if (h->prm.smoother != H3DMG_SMOOTH_RBGS) {
+      // wavefront smoothers: GS (i, j) tiles of 64 x 4 (kind 5), TDMA tiles
+      // of NTHREADS lines (kind 6); one plane deep
+      const bool gs = h->prm.smoother == H3DMG_SMOOTH_GS;
+      ts.depth = 1;
+      for (const BlockInfo &B : h->blk)
+        if (member(B)) {
+          const int d = B.desc[g];
+          const LevelDev &L = h->levs[d];
+          if (gs) {
+            for (int y = 0; y < L.nj; y += TILE_Y)
+              for (int x = 0; x < L.ni; x += TILE_X) ts.h.push_back(Tile{d, x, y, 0, 1, 5});
+            ts.ndiag[0] = std::max(ts.ndiag[0], L.ni + L.nj + L.nk - 2);
+          } else {
+            for (int x = 0; x < std::max(L.ni, L.nj); x += NTHREADS)
+              ts.h.push_back(Tile{d, x, 0, 0, 1, 6});
+            ts.ndiag[0] = std::max(ts.ndiag[0], L.ni + L.nk - 1);
+            ts.ndiag[1] = std::max(ts.ndiag[1], L.nj + L.nk - 1);
+            ts.ndiag[2] = std::max(ts.ndiag[2], L.ni + L.nj - 1);
+          }
+        }
+    } else if (ts.fused) {
       ts.depth = FUSED_Z;
       for (const BlockInfo &B : h->blk)
         if (member(B)) {
@@ -1177,10 +1220,14 @@ h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_pa
   else
     h3dmg_default_params(&h->prm);
   const h3dmg_params &P = h->prm;
-  if (P.smoother != H3DMG_SMOOTH_RBGS) {
+  if (P.smoother != H3DMG_SMOOTH_GS && P.smoother != H3DMG_SMOOTH_TDMA &&
+      P.smoother != H3DMG_SMOOTH_RBGS) {
     delete h;
-    return fail(H3DMG_ERR_UNSUPPORTED, "device smoother %d not available (RBGS = 3 only)",
-                P.smoother);
+    return fail(H3DMG_ERR_ARG, "unknown smoother %d", P.smoother);
+  }
+  if (P.smoother != H3DMG_SMOOTH_RBGS && P.precision != H3DMG_FP64) {
+    delete h;
+    return fail(H3DMG_ERR_UNSUPPORTED, "the GS and line-TDMA smoothers run in FP64");
   }
   if (P.nu1 < 0 || P.nu2 < 0 || P.nu1 + P.nu2 < 1 || P.maxcy < 0 ||
       (P.precision != H3DMG_FP64 && P.precision != H3DMG_MIXED) ||
@@ -2049,6 +2096,8 @@ h3dmg_status h3dmg_set_tail(h3dmg_handle *h, int level) {
   if (!h || level < 0) return fail(H3DMG_ERR_ARG, "bad tail level");
   if (h->lsm && level <= h->Lmax)
     return fail(H3DMG_ERR_UNSUPPORTED, "the tail kernel holds no variable-density coefficients");
+  if (h->prm.smoother != H3DMG_SMOOTH_RBGS && level <= h->Lmax)
+    return fail(H3DMG_ERR_UNSUPPORTED, "the tail kernel runs the red-black smoother only");
   std::lock_guard<std::mutex> lock(g_device_mutex);
   HIP_TRY(hipStreamSynchronize(h->stream));
   drop_graph(h);
@@ -2065,7 +2114,7 @@ int h3dmg_get_march(const h3dmg_handle *h, int level) {
   if (!h || level < 1 || level > h->Lmax) return -1;
   for (const TileSet *ts : {&h->relax_nc[level], &h->relax_c[level]})
     for (const Tile &t : ts->h)
-      if (t.kind != 2) return t.kind == 4 ? 0 : t.z1 - t.z0;
+      if (t.kind != 2) return t.kind >= 4 ? 0 : t.z1 - t.z0;
   return 0;
 }
 
@@ -2073,6 +2122,8 @@ h3dmg_status h3dmg_set_fused(h3dmg_handle *h, long long max_cells) {
   if (!h || max_cells < 0) return fail(H3DMG_ERR_ARG, "bad fused threshold");
   if (h->lsm && max_cells > 0)
     return fail(H3DMG_ERR_UNSUPPORTED, "fused sweeps hold no variable-density coefficients");
+  if (h->prm.smoother != H3DMG_SMOOTH_RBGS && max_cells > 0)
+    return fail(H3DMG_ERR_UNSUPPORTED, "fused sweeps run the red-black smoother only");
   std::lock_guard<std::mutex> lock(g_device_mutex);
   HIP_TRY(hipStreamSynchronize(h->stream));
   drop_graph(h);

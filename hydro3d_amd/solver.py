@@ -20,6 +20,8 @@ from . import _lib as L
 _PREC = {"fp64": L.FP64, "mixed": L.MIXED}
 _COARSE = {"smooth": L.COARSE_SMOOTH, "exact": L.COARSE_EXACT}
 _CRIT = {"rel_l2": L.CRIT_REL_L2, "ref_rmax": L.CRIT_REF_RMAX}
+# mg_itrsch (src_v70/control.cin): lexicographic GS, line TDMA, red-black GS
+_SMOOTH = {"gs": L.SMOOTH_GS, "tdma": L.SMOOTH_TDMA, "rbgs": L.SMOOTH_RBGS}
 
 
 def _dptr(a: np.ndarray):
@@ -36,7 +38,8 @@ class MultigridSolver:
     bc=(w,e,s,n,b,t) with 0 wall / 1 periodic, nlev=levels)."""
 
     def __init__(self, blocks, nu1=2, nu2=2, maxcy=60, tol=1e-8, criterion="rel_l2",
-                 precision="fp64", coarse="smooth", min_cycles=1, use_graph=True, dt=1e-3):
+                 precision="fp64", coarse="smooth", min_cycles=1, use_graph=True, dt=1e-3,
+                 smoother="rbgs"):
         self.lib = L.load()
         nb = len(blocks)
         arr = (L.H3dBlock * nb)()
@@ -55,6 +58,7 @@ class MultigridSolver:
         prm.criterion = _CRIT[criterion]
         prm.precision = _PREC[precision]
         prm.coarse = _COARSE[coarse]
+        prm.smoother = _SMOOTH[smoother]
         prm.min_cycles = min_cycles
         prm.use_graph = 1 if use_graph else 0
         self.params = prm

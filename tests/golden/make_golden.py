@@ -156,6 +156,32 @@ def make_lsm_ops(name, n, bc, L, seed):
     np.savez_compressed(os.path.join(HERE, f"lsm_ops_{name}.npz"), **out)
 
 
+def make_lsm_line_ops(name, n, bc, L, seed):
+    """The lexicographic GS and line-TDMA smoothers (mg_itrsch = 1, 2,
+    mgsolver.for:407-549) over the variable-density coefficients: one sweep
+    per level and one V-cycle of each."""
+    rng = np.random.default_rng(seed)
+    B = O.Block(n, tuple(1.0 / x for x in n), bc, L)
+    dens = lsm_density(n, seed + 1)
+    out = dict(n=np.array(n), bc=np.array(bc), nlev=L, h=np.array(B.h), dens=dens)
+    for scheme in (O.SCHEME_GS, O.SCHEME_TDMA):
+        su = rng.standard_normal(B.interior_shape(1))
+        su -= su.mean()
+        pp = O.run_ref(B, "vcycle", [dens, su], int(np.prod(B.padded_shape(1))), lsm=True,
+                       scheme=scheme)
+        out[f"vcycle_s{scheme}_su"] = su
+        out[f"vcycle_s{scheme}_pp"] = pp.reshape(B.padded_shape(1))
+        for g in range(1, L + 1):
+            phi = image_ghosts(rng.standard_normal(B.padded_shape(g)), bc)
+            rhs = rng.standard_normal(B.interior_shape(g))
+            res = O.run_ref(B, "relax", [dens, phi, rhs], phi.size, level=g, nsweep=1, lsm=True,
+                            scheme=scheme)
+            out[f"relax{g}_s{scheme}_in_phi"] = phi
+            out[f"relax{g}_s{scheme}_in_rhs"] = rhs
+            out[f"relax{g}_s{scheme}_out"] = res.reshape(phi.shape)
+    np.savez_compressed(os.path.join(HERE, f"lsm_line_{name}.npz"), **out)
+
+
 def make_lsm_newsolv(name, cfg, ncycles):
     """The velocity-form loop under L_LSM: calvel divides by the face
     density, calmas writes sup = div(u) dx dy dz / dt (press.for:53-55)."""
@@ -195,6 +221,8 @@ def main():
     # the variable-density (L_LSM) operator
     make_lsm_ops("ppw16", (16, 16, 16), (P, P, P, P, W, W), 3, seed=300)
     make_lsm_ops("wall8x16x8", (8, 16, 8), (W, W, W, W, W, W), 2, seed=301)
+    make_lsm_line_ops("ppw16", (16, 16, 16), (P, P, P, P, W, W), 3, seed=310)
+    make_lsm_line_ops("wall8x16x12", (8, 16, 12), (W, W, W, W, W, W), 2, seed=311)
     print("wrote lsm ops")
     make_lsm_newsolv("ppw16", C.cube(16, C.BC_PPN, 3, seed=5), ncycles=10)
     print("wrote lsm newsolv")

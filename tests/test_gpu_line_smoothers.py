@@ -1,0 +1,123 @@
+"""The lexicographic Gauss-Seidel and line-TDMA smoothers (mg_itrsch = 1 and
+2, src_v70/mgsolver.for:407-549) on the device, through the C-ABI.  Needs an
+MI355X.
+
+Bar: bitwise.  One sweep per level and one V-cycle against the reference
+build's fixtures (constant and variable-density coefficients); whole solves
+of two unequal blocks in one handle against the oracle, in a captured graph
+and eagerly.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from h3d_util import GOLDEN, load_golden
+
+pytestmark = pytest.mark.gpu
+
+OPS = sorted(glob.glob(os.path.join(GOLDEN, "ops_*.npz")))
+LSM_LINE = sorted(glob.glob(os.path.join(GOLDEN, "lsm_line_*.npz")))
+SMOOTHERS = [("gs", 1), ("tdma", 2)]
+
+
+def _blockspec(z):
+    return dict(n=tuple(int(x) for x in z["n"]), h=tuple(float(x) for x in z["h"]),
+                bc=tuple(int(x) for x in z["bc"]), nlev=int(z["nlev"]))
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def _solver(specs, **kw):
+    from hydro3d_amd import MultigridSolver
+    return MultigridSolver(specs if isinstance(specs, list) else [specs], **kw)
+
+
+def _relax_and_vcycle(S, z, scheme, Lv):
+    import torch
+    from hydro3d_amd import _lib as L
+    for g in range(1, Lv + 1):
+        phi, rhs = z[f"relax{g}_s{scheme}_in_phi"], z[f"relax{g}_s{scheme}_in_rhs"]
+        S.dbg_put(0, g, L.ARR_PHI, _dev(phi))
+        S.dbg_put(0, g, L.ARR_RHS, _dev(rhs))
+        S.dbg_relax(g, 1)
+        out = S.dbg_get(0, g, L.ARR_PHI, torch.empty(phi.shape, dtype=torch.float64,
+                                                     device="cuda")).cpu().numpy()
+        assert np.array_equal(out, z[f"relax{g}_s{scheme}_out"]), f"relax level {g}"
+    S.set_rhs(0, z[f"vcycle_s{scheme}_su"])
+    S.vcycle()
+    assert np.array_equal(S.get_pp(0), z[f"vcycle_s{scheme}_pp"]), "vcycle"
+
+
+@pytest.mark.parametrize("smoother,scheme", SMOOTHERS, ids=[s for s, _ in SMOOTHERS])
+@pytest.mark.parametrize("path", OPS, ids=[os.path.basename(p) for p in OPS])
+def test_gpu_line_smoothers_bitwise_vs_reference(gpu_lib, path, smoother, scheme):
+    """One GS / TDMA sweep of every level from the reference's inputs, and a
+    V-cycle (nu1 = nu2 = 2) from its su, are bitwise the reference build's
+    (ghosts included: walls, periodic faces, anisotropic spacing)."""
+    z = load_golden(os.path.basename(path))
+    spec = _blockspec(z)
+    print(f"case {os.path.basename(path)} {smoother} {spec}", flush=True)
+    with _solver(spec, use_graph=False, smoother=smoother) as S:
+        _relax_and_vcycle(S, z, scheme, spec["nlev"])
+
+
+@pytest.mark.parametrize("smoother,scheme", SMOOTHERS, ids=[s for s, _ in SMOOTHERS])
+@pytest.mark.parametrize("path", LSM_LINE, ids=[os.path.basename(p) for p in LSM_LINE])
+def test_gpu_line_smoothers_variable_density_bitwise(gpu_lib, path, smoother, scheme):
+    """The same under L_LSM: the per-cell density-weighted coefficients feed
+    the GS update and the Thomas recurrences; bitwise the reference build."""
+    z = load_golden(os.path.basename(path))
+    spec = _blockspec(z)
+    print(f"case {os.path.basename(path)} {smoother} {spec}", flush=True)
+    with _solver(spec, use_graph=False, smoother=smoother) as S:
+        S.set_density(0, z["dens"])
+        _relax_and_vcycle(S, z, scheme, spec["nlev"])
+
+
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+@pytest.mark.parametrize("smoother,scheme", SMOOTHERS, ids=[s for s, _ in SMOOTHERS])
+def test_gpu_line_smoothers_solve_bitwise_vs_oracle(gpu_lib, smoother, scheme, graph):
+    """Two unequal blocks in one handle (their hyperplanes and line diagonals
+    share launches): six cycles of each are bitwise the oracle's solve of that
+    block alone, residual history to 1e-12."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(17)
+    specs = [dict(n=(32, 32, 32), h=(1 / 32,) * 3, bc=(1, 1, 1, 1, 0, 0), nlev=3),
+             dict(n=(24, 16, 40), h=(1 / 24, 1 / 16, 1 / 40), bc=(0, 0, 1, 1, 0, 0), nlev=3)]
+    fs = []
+    for s in specs:
+        f = rng.standard_normal(s["n"][::-1])
+        fs.append(f - f.mean())
+    nc = 6
+    print(f"case {smoother} graph={graph}", flush=True)
+    with _solver(specs, maxcy=nc, tol=0.0, use_graph=graph, smoother=smoother) as S:
+        for b, f in enumerate(fs):
+            S.set_rhs(b, f)
+        S.solve()
+        got = [S.get_p(b) for b in range(len(specs))]
+    for b, (s, f) in enumerate(zip(specs, fs)):
+        B = O.Block(s["n"], s["h"], s["bc"], s["nlev"])
+        po, _ = O.solve(B, f, nc, 0.0, scheme=scheme)
+        assert np.array_equal(got[b][1:-1, 1:-1, 1:-1], po[1:-1, 1:-1, 1:-1]), f"block {b}"
+
+
+def test_gpu_line_smoothers_refuse_mixed_tail_fused(gpu_lib):
+    """GS and TDMA run in FP64 on the level-by-level path: mixed precision,
+    the LDS tail and fused red-black sweeps are refused."""
+    from hydro3d_amd import _lib as L
+    spec = dict(n=(16, 16, 16), h=(1 / 16,) * 3, bc=(1, 1, 1, 1, 0, 0), nlev=3)
+    for sm in ("gs", "tdma"):
+        with pytest.raises(L.H3dError) as e:
+            _solver(spec, smoother=sm, precision="mixed")
+        assert e.value.status == L.ERR_UNSUPPORTED
+        with _solver(spec, smoother=sm) as S:
+            assert S.get_tail() == 4  # no tail level
+            with pytest.raises(L.H3dError):
+                S.set_tail(2)
+            with pytest.raises(L.H3dError):
+                S.set_fused(1 << 20)
