@@ -197,6 +197,34 @@ def single_64(steps, warmup):
     return out
 
 
+def line_smoothers(steps, warmup):
+    """The reference's other smoothers (mg_itrsch 1 lexicographic GS, 2 line
+    TDMA; bitwise the reference, FP64) on the 64^3 workload to 1e-8: solve
+    latency and one V-cycle.  Both are wavefronts over the reference's loop
+    order (a launch per hyperplane / line diagonal), so they are bound by
+    launch latency and the Thomas recurrence's dependent chain, not HBM."""
+    import torch
+    from hydro3d_amd import MultigridSolver, configs
+    c = configs.single_64()
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    f = torch.from_numpy(c["f"]).cuda()
+    z = torch.zeros_like(f)
+    out = {"workload": "single block 64^3 (periodic x/y, Neumann z), 4 levels, seed 0, "
+                       "FP64 V(2,2) to 1e-8"}
+    for sm in ("gs", "tdma"):
+        with MultigridSolver([spec], tol=1e-8, smoother=sm) as S:
+            lat, cyc, fin = timed_solves(S, f, z, steps, warmup)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                S.vcycle()
+            torch.cuda.synchronize()
+            tv = (time.perf_counter() - t0) / 3
+        out[sm] = {"ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3), "cycles": cyc,
+                   "final_rel_residual": fin, "vcycle_ms": round(tv * 1e3, 3)}
+    return out
+
+
 def serving(nreq):
     """The serving stream (seeded Poisson arrivals at 200/s, sizes 32..256,
     tolerances 1e-6/1e-8, 20% warm reuse): continuous-batching mixed server
@@ -215,6 +243,20 @@ def serving(nreq):
         res, wall = SV.run_stream(srv, reqs)
         out["sequential_fp64"] = SV.summarize(res, wall)
     out["workload"] = f"{nreq} requests of the seeded stream (seed 4, 200/s)"
+    return out
+
+
+def cpu_line_smoothers():
+    """One V-cycle of the 64^3 workload per smoother, C oracle, one core."""
+    from oracle import oracle as O
+    from hydro3d_amd import configs
+    c = configs.single_64()
+    B = O.Block(c["n"], c["h"], c["bc"], c["nlev"])
+    out = {}
+    for name, scheme in (("rbgs", O.SCHEME_RB), ("gs", O.SCHEME_GS), ("tdma", O.SCHEME_TDMA)):
+        t0 = time.perf_counter()
+        O.vcycle(B, c["f"], scheme=scheme)
+        out[name] = round((time.perf_counter() - t0) * 1e3, 2)
     return out
 
 
@@ -342,6 +384,9 @@ def main():
     log("single 64^3")
     small = single_64(args.steps, args.warmup)
     log(small)
+    log("line smoothers 64^3")
+    lines = line_smoothers(max(2, args.steps // 4), 1)
+    log(lines)
     batched = None
     if not args.no_batched:
         log("batched 48 blocks")
@@ -357,7 +402,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(c, cyc64)
-        log(cpu)
+        lines["cpu_oracle_vcycle_ms_1core"] = cpu_line_smoothers()
+        log(cpu, lines["cpu_oracle_vcycle_ms_1core"])
 
     S64.close()
     Smx.close()
@@ -401,6 +447,7 @@ def main():
                                    "frac": round(ach64 / HBM_PEAK_GBS, 4)}},
         "cpu_baseline": cpu,
         "single_64": small,
+        "line_smoothers_64": lines,
         "batched_uneven_48": batched,
         "serving": serve,
     }

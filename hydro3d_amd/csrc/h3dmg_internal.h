@@ -50,9 +50,10 @@ struct LevelDev {
   // colour arrays of lsm_half doubles laid out like phi (array 2*q + colour)
   double *lsm;
   unsigned long long lsm_half;
-  // line-TDMA smoother scratch of the block (2 * level-1 cells), or null:
-  // the Thomas coefficients a and c of the line being solved
-  double *line;
+  // line-TDMA tables of a constant-coefficient level, or null: per pass
+  // (j-, i-, k-lines) and wall-state pair of the line (16), a(p) and
+  // 1/(ap - alo*a(p-1)) at every position p of the line
+  double *tdma;
 };
 
 // Finest-level FP64 state of one block (pressure, right-hand side).
@@ -158,14 +159,33 @@ void launch_lsm_restrict(hipStream_t s, const Tile *tiles, int ntiles, const Lev
 
 // ---- lexicographic GS and line-TDMA smoothers (mg_itrsch = 1, 2) ---------
 // Tiles: GS 64 x 4 (i, j) tiles of every descriptor (Tile::x0, y0); TDMA
-// tiles of NTHREADS lines (Tile::x0 = first line).  One launch per
-// hyperplane d = i+j+k (GS) or line diagonal d (TDMA pass 0: j-lines,
-// i+k = d; 1: i-lines, j+k = d; 2: k-lines, i+j = d).  A sweep reads
-// buffer src and writes 1-src; every level is FP64.
+// tiles of tdma_group_lines(len) lines of one pass (Tile::x0 = first line),
+// a line of len cells.  One launch per hyperplane d = i+j+k (GS) or line
+// diagonal d (TDMA pass 0: j-lines, i+k = d; 1: i-lines, j+k = d; 2:
+// k-lines, i+j = d).  A sweep reads buffer src and writes 1-src; every
+// level is FP64.
+constexpr int TDMA_CELLS = 1024;    // line cells one TDMA workgroup stages in LDS
+constexpr int TDMA_MAX_LEN = 1024;  // longest line (cells along one axis)
+__host__ __device__ inline int tdma_group_lines(int len) {
+  int lw = 1;
+  while (lw < NTHREADS && 2 * lw * len <= TDMA_CELLS) lw *= 2;
+  return lw;
+}
 void launch_gs_plane(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                      const int *active, int d, int src, int zero);
 void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                       const int *active, int pass, int d, int src, int zero);
+// a whole TDMA sweep of small levels, one workgroup per block (Tile::lev),
+// every line of a diagonal in LDS: tdma_sweep_lds_bytes(level) of dynamic
+// LDS (the largest over the launch, under 64 KiB), NTHREADS >= the lines
+// of a diagonal; used where every pass has at most TDMA_SWEEP_CELLS cells
+// per diagonal
+constexpr int TDMA_SWEEP_CELLS = 2048;
+void launch_tdma_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                       const int *active, int src, int zero, size_t lds_bytes);
+size_t tdma_sweep_lds_bytes(const LevelDev &L);
+// fill LevelDev::tdma of every descriptor that has one (32*(ni+nj+nk) doubles)
+void launch_tdma_tables(hipStream_t s, const LevelDev *levs, int ndesc);
 
 // velocity projection (calvel per axis over `faces` faces; calmas writes
 // su (optional) and one partial max per workgroup, returns the group count)

@@ -2177,6 +2177,11 @@ inline dim3 tile_block() { return dim3(TILE_X, TILE_Y, 1); }
 //     sweep (mgsolver.for:614-615).
 // zero != 0: buffer src holds zeros.
 
+// element e of a level array, read as global memory (no flat access)
+__device__ __forceinline__ double gload(const void *base, size_t e) {
+  return ((const gmem<double> *)base)[e];
+}
+
 // coefficients of one cell: the level's constant operator, or the
 // variable-density arrays
 __device__ __forceinline__ Coef<double> smoother_coef(const LevelDev &L, int i, int j, int k) {
@@ -2184,24 +2189,29 @@ __device__ __forceinline__ Coef<double> smoother_coef(const LevelDev &L, int i, 
   const int colour = (i + j + k) & 1;
   const size_t e = cidx(L.H, L.nj, i, j, k);
   Coef<double> c;
-  c.aw = lsm_coef(L, 0, colour)[e];
-  c.ae = lsm_coef(L, 1, colour)[e];
-  c.as = lsm_coef(L, 2, colour)[e];
-  c.an = lsm_coef(L, 3, colour)[e];
-  c.ab = lsm_coef(L, 4, colour)[e];
-  c.at = lsm_coef(L, 5, colour)[e];
-  c.ap = lsm_coef(L, 6, colour)[e];
+  c.aw = gload(lsm_coef(L, 0, colour), e);
+  c.ae = gload(lsm_coef(L, 1, colour), e);
+  c.as = gload(lsm_coef(L, 2, colour), e);
+  c.an = gload(lsm_coef(L, 3, colour), e);
+  c.ab = gload(lsm_coef(L, 4, colour), e);
+  c.at = gload(lsm_coef(L, 5, colour), e);
+  c.ap = gload(lsm_coef(L, 6, colour), e);
   return c;
 }
 
 __device__ __forceinline__ double cell_at(const LevelDev &L, int buf, int i, int j, int k) {
-  return ((const double *)L.phi[buf][(i + j + k) & 1])[cidx(L.H, L.nj, i, j, k)];
+  return gload(L.phi[buf][(i + j + k) & 1], cidx(L.H, L.nj, i, j, k));
 }
 
-// ghost or not-yet-updated value in buffer src
+// ghost or not-yet-updated value in buffer src (ghosts resolved to their
+// image)
 __device__ __forceinline__ double before_sweep(const LevelDev &L, int src, int zero, int x, int y,
                                                int z) {
-  return zero ? 0.0 : level_value<double>(L, src, x, y, z);
+  if (zero) return 0.0;
+  x = image(x, L.ni, L.per & 1);
+  y = image(y, L.nj, L.per & 2);
+  z = image(z, L.nk, L.per & 4);
+  return cell_at(L, src, x, y, z);
 }
 
 // phi = (rhs - (aw*W + ae*E + as*S + an*N + ab*B + at*T)) / ap at the cells
@@ -2224,12 +2234,12 @@ __global__ void __launch_bounds__(NTHREADS)
   const double T = before_sweep(L, src, zero, i, j, k + 1);
   const int colour = (i + j + k) & 1;
   const size_t e = cidx(L.H, L.nj, i, j, k);
-  const double r = ((const double *)L.rhs[colour])[e];
+  const double r = gload(L.rhs[colour], e);
   const double sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * T;
-  ((double *)L.phi[dst][colour])[e] = (r - sum) / c.ap;
+  ((gmem<double> *)L.phi[dst][colour])[e] = (r - sum) / c.ap;
 }
 
-// positions of one line whose neighbour loads are issued together
+// positions of one line whose coefficients are loaded together
 constexpr int TDMA_CHUNK = 8;
 
 // cell (i, j, k) of position p on the line (u, v) of pass PASS
@@ -2244,22 +2254,109 @@ __device__ __forceinline__ void tdma_cell(int u, int v, int p, int &i, int &j, i
   }
 }
 
-// One line of a TDMA pass, the Thomas recurrence of the reference
-// (a(0) = 0, c(0) = the ghost):
-//   apr = 1/(ap - alo*a(p-1)),  a(p) = ahi*apr,
-//   c(p) = (rhs - x1*X1 - x2*X2 - x3*X3 - x4*X4 - alo*c(p-1))*apr,
-//   phi(p) = c(p) - a(p)*phi(p+1),  phi(n) = the ghost,
-// with (alo, ahi) = (as, an), (aw, ae), (ab, at) and the cross terms in the
-// reference's order: j-lines ae*E, aw*W, at*T, ab*B (mgsolver.for:459-466);
-// i-lines an*N, as*S, at*T, ab*B (:496-503); k-lines an*N, as*S, ae*E, aw*W
-// (:533-540).  The cross neighbours on the previous diagonal (W, B; S, B;
-// W, S) are updated ones.  a and c go to the line scratch at p*nu + u and
-// cells + p*nu + u (consecutive lines, consecutive addresses).
+// Wall state of a cross coordinate c of a line (n cells on its axis): bit 0
+// at the low wall, bit 1 at the high wall, 0 on a periodic axis.  Lines of
+// one pass and state pair have the same coefficients along their length
+// (cell_coef), so the same Thomas a(p) and 1/(ap - alo*a(p-1)).
+__device__ __forceinline__ int wall_state(int c, int n, bool per) {
+  return per ? 0 : (c == 0 ? 1 : 0) | (c == n - 1 ? 2 : 0);
+}
+// a coordinate of state s, or -1 when none has it
+__device__ __forceinline__ int wall_rep(int s, int n, bool per) {
+  if (per) return s == 0 ? 0 : -1;
+  if (s == 0) return n >= 3 ? 1 : -1;
+  if (s == 1 || s == 2) return n >= 2 ? (s == 1 ? 0 : n - 1) : -1;
+  return n == 1 ? 0 : -1;
+}
+
+// geometry of the lines of pass PASS: lines (u, v), len cells each
 template <int PASS>
-__device__ void tdma_line(const LevelDev &L, int u, int v, int nu, int len, int src, int zero) {
+struct TdmaGeom {
+  int nu, nv, len;
+  bool pu, pv;  // u / v axis periodic
+  __device__ explicit TdmaGeom(const LevelDev &L) {
+    nu = PASS == 1 ? L.nj : L.ni;
+    nv = PASS == 2 ? L.nj : L.nk;
+    len = PASS == 0 ? L.nj : PASS == 1 ? L.ni : L.nk;
+    pu = PASS == 1 ? (L.per & 2) : (L.per & 1);
+    pv = PASS == 2 ? (L.per & 2) : (L.per & 4);
+  }
+};
+
+// (a(p), 1/(ap - alo*a(p-1))) of pass PASS, state pair `var`, position p
+// of a constant-coefficient level: L.tdma + tdma_table_offset + 2*(var*len+p)
+template <int PASS>
+__device__ __forceinline__ size_t tdma_table_offset(const LevelDev &L) {
+  return PASS == 0 ? 0 : PASS == 1 ? (size_t)32 * L.nj : (size_t)32 * (L.nj + L.ni);
+}
+
+template <int PASS>
+__device__ __forceinline__ void tdma_table_fill(const LevelDev &L, int var) {
+  const TdmaGeom<PASS> G(L);
+  const int ru = wall_rep(var & 3, G.nu, G.pu), rv = wall_rep(var >> 2, G.nv, G.pv);
+  if (ru < 0 || rv < 0) return;
+  gmem<double> *tab = (gmem<double> *)(L.tdma + tdma_table_offset<PASS>(L) + (size_t)2 * var * G.len);
+  double aprev = 0.0;
+  for (int p = 0; p < G.len; ++p) {
+    int i, j, k;
+    tdma_cell<PASS>(ru, rv, p, i, j, k);
+    const Coef<double> c = cell_coef<double>(L, i, j, k);
+    const double alo = PASS == 0 ? c.as : PASS == 1 ? c.aw : c.ab;
+    const double ahi = PASS == 0 ? c.an : PASS == 1 ? c.ae : c.at;
+    const double apr = 1.0 / (c.ap - alo * aprev);
+    aprev = ahi * apr;
+    tab[2 * p] = aprev;
+    tab[2 * p + 1] = apr;
+  }
+}
+
+// a thread per (descriptor, pass, state pair)
+__global__ void __launch_bounds__(256)
+    k_tdma_tables(const LevelDev *__restrict__ levs, int ndesc) {
+  const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (q >= ndesc * 48) return;
+  const LevelDev &L = levs[q / 48];
+  if (!L.tdma) return;
+  const int pass = (q / 16) % 3, var = q % 16;
+  if (pass == 0)
+    tdma_table_fill<0>(L, var);
+  else if (pass == 1)
+    tdma_table_fill<1>(L, var);
+  else
+    tdma_table_fill<2>(L, var);
+}
+
+// One diagonal d of one TDMA pass for the lines u0 .. u0+lw-1 (all threads
+// of the workgroup call it; it holds one barrier), in two phases:
+//  A (every thread, a cell at a time): the right-hand side of the Thomas
+//    recurrence without its last term,
+//      rs = rhs - x1*X1 - x2*X2 - x3*X3 - x4*X4,
+//    with the cross terms in the reference's order -- j-lines ae*E, aw*W,
+//    at*T, ab*B (mgsolver.for:459-466); i-lines an*N, as*S, at*T, ab*B
+//    (:496-503); k-lines an*N, as*S, ae*E, aw*W (:533-540) -- into LDS at
+//    xs[p*lwp + line].  The neighbours on the previous diagonal (W, B; S, B;
+//    W, S) are updated ones.  On a constant-coefficient level the cell's
+//    a(p) and 1/(ap - alo*a(p-1)) also go to LDS, from its line's table
+//    (the same values the recurrence gives: a does not depend on the data).
+//  B (a thread per line): the reference's recurrence (a(0) = 0, c(0) = the
+//    ghost),
+//      apr = 1/(ap - alo*a(p-1)),  a(p) = ahi*apr,
+//      c(p) = (rs - alo*c(p-1))*apr,
+//      phi(p) = c(p) - a(p)*phi(p+1),  phi(n) = the ghost,
+//    (alo, ahi) = (as, an), (aw, ae), (ab, at); c overwrites rs in LDS.
+//    Operands are read a chunk of positions ahead of the dependent chain.
+// The subtraction of alo*c(p-1) is the last one of the reference's
+// left-to-right sum, so splitting it off changes no rounding.  Lines of one
+// diagonal never neighbour each other: phase A reads only lines of the
+// diagonals before and after, which nothing writes meanwhile.
+template <int PASS>
+__device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw, int lwp, int d, int src,
+                              int zero, double *xs, double *av, double *rv, int tid) {
+  const TdmaGeom<PASS> G(L);
+  const int nu = G.nu, nv = G.nv, len = G.len;
+  const int ulo = max(u0, d - nv + 1), uhi = min(min(u0 + lw, nu), d + 1);
   const int dst = 1 - src;
-  const size_t cells = (size_t)L.ni * L.nj * L.nk;
-  double *sa = L.line, *sc = L.line + cells;
+  const bool tab = L.lsm == nullptr;
   // interior value of a cross neighbour: updated ones and (after the first
   // pass) not-yet-updated ones in dst, the first pass's others in src
   auto cross = [&](int x, int y, int z, bool updated) -> double {
@@ -2269,83 +2366,167 @@ __device__ void tdma_line(const LevelDev &L, int u, int v, int nu, int len, int 
     if (updated || PASS > 0) return cell_at(L, dst, x, y, z);
     return zero ? 0.0 : cell_at(L, src, x, y, z);
   };
+  const int ncell = (uhi - ulo) * len;
+  // q / len through a float reciprocal: exact while q < 2^12 * len
+  const float inv_len = 1.0f / (float)len;
+#pragma unroll 4
+  for (int q = tid; q < ncell; q += NTHREADS) {
+    const int lq = (int)(((float)q + 0.5f) * inv_len);
+    const int l = ulo - u0 + lq, p = q - lq * len;
+    const int u = u0 + l, v = d - u;
+    int i, j, k;
+    tdma_cell<PASS>(u, v, p, i, j, k);
+    const Coef<double> c = smoother_coef(L, i, j, k);
+    const double r = gload(L.rhs[(i + j + k) & 1], cidx(L.H, L.nj, i, j, k));
+    double rs;
+    if (PASS == 0)
+      rs = r - c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true) -
+           c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
+    else if (PASS == 1)
+      rs = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
+           c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
+    else
+      rs = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
+           c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true);
+    xs[p * lwp + l] = rs;
+    if (tab) {
+      const int var = wall_state(u, nu, G.pu) | (wall_state(v, nv, G.pv) << 2);
+      const size_t t = tdma_table_offset<PASS>(L) + (size_t)2 * (var * len + p);
+      av[p * lwp + l] = gload(L.tdma, t);
+      rv[p * lwp + l] = gload(L.tdma, t + 1);
+    }
+  }
+  __syncthreads();
+  const int l = tid, u = u0 + l, v = d - u;
+  if (l >= lw || u < ulo || u >= uhi) return;
   int i, j, k;
   tdma_cell<PASS>(u, v, -1, i, j, k);
-  double aprev = 0.0, cprev = before_sweep(L, src, zero, i, j, k);
-  for (int p0 = 0; p0 < len; p0 += TDMA_CHUNK) {
-    double rs[TDMA_CHUNK], ap[TDMA_CHUNK], alo[TDMA_CHUNK], ahi[TDMA_CHUNK];
+  double cprev = before_sweep(L, src, zero, i, j, k);
+  if (tab) {
+    // constant coefficients: alo is 1/h^2 of the line's axis, 0 at its
+    // first cell next to a wall (cell_coef)
+    const double cax = PASS == 0 ? L.cy : PASS == 1 ? L.cx : L.cz;
+    const bool pax = PASS == 0 ? (L.per & 2) : PASS == 1 ? (L.per & 1) : (L.per & 4);
+    for (int p0 = 0; p0 < len; p0 += TDMA_CHUNK) {
+      double rs[TDMA_CHUNK], apr[TDMA_CHUNK];
 #pragma unroll
-    for (int q = 0; q < TDMA_CHUNK; ++q) {
-      const int p = p0 + q;
-      if (p >= len) break;
-      tdma_cell<PASS>(u, v, p, i, j, k);
-      const Coef<double> c = smoother_coef(L, i, j, k);
-      const double r = ((const double *)L.rhs[(i + j + k) & 1])[cidx(L.H, L.nj, i, j, k)];
-      if (PASS == 0) {
-        rs[q] = r - c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true) -
-                c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
-        alo[q] = c.as;
-        ahi[q] = c.an;
-      } else if (PASS == 1) {
-        rs[q] = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
-                c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
-        alo[q] = c.aw;
-        ahi[q] = c.ae;
-      } else {
-        rs[q] = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
-                c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true);
-        alo[q] = c.ab;
-        ahi[q] = c.at;
+      for (int q = 0; q < TDMA_CHUNK; ++q) {
+        const int p = p0 + q;
+        if (p >= len) break;
+        rs[q] = xs[p * lwp + l];
+        apr[q] = rv[p * lwp + l];
       }
-      ap[q] = c.ap;
-    }
 #pragma unroll
-    for (int q = 0; q < TDMA_CHUNK; ++q) {
-      const int p = p0 + q;
-      if (p >= len) break;
-      const double apr = 1.0 / (ap[q] - alo[q] * aprev);
-      aprev = ahi[q] * apr;
-      cprev = (rs[q] - alo[q] * cprev) * apr;
-      sa[(size_t)p * nu + u] = aprev;
-      sc[(size_t)p * nu + u] = cprev;
+      for (int q = 0; q < TDMA_CHUNK; ++q) {
+        const int p = p0 + q;
+        if (p >= len) break;
+        const double alo = (p == 0 && !pax) ? 0.0 : cax;
+        cprev = (rs[q] - alo * cprev) * apr[q];
+        xs[p * lwp + l] = cprev;
+      }
+    }
+  } else {
+    double aprev = 0.0;
+    for (int p0 = 0; p0 < len; p0 += TDMA_CHUNK) {
+      double rs[TDMA_CHUNK], ap[TDMA_CHUNK], alo[TDMA_CHUNK], ahi[TDMA_CHUNK];
+#pragma unroll
+      for (int q = 0; q < TDMA_CHUNK; ++q) {
+        const int p = p0 + q;
+        if (p >= len) break;
+        tdma_cell<PASS>(u, v, p, i, j, k);
+        const Coef<double> c = smoother_coef(L, i, j, k);
+        rs[q] = xs[p * lwp + l];
+        ap[q] = c.ap;
+        alo[q] = PASS == 0 ? c.as : PASS == 1 ? c.aw : c.ab;
+        ahi[q] = PASS == 0 ? c.an : PASS == 1 ? c.ae : c.at;
+      }
+#pragma unroll
+      for (int q = 0; q < TDMA_CHUNK; ++q) {
+        const int p = p0 + q;
+        if (p >= len) break;
+        const double a_r = 1.0 / (ap[q] - alo[q] * aprev);
+        aprev = ahi[q] * a_r;
+        cprev = (rs[q] - alo[q] * cprev) * a_r;
+        xs[p * lwp + l] = cprev;
+        av[p * lwp + l] = aprev;
+      }
     }
   }
   tdma_cell<PASS>(u, v, len, i, j, k);
   double next = before_sweep(L, src, zero, i, j, k);
+  // colour-split index of position p: base + p*step (j- and k-lines) or
+  // base + p/2 (i-lines); colour (u+v+p) & 1.  The level's fields stay in
+  // registers across the stores.
+  gmem<double> *const out0 = (gmem<double> *)L.phi[dst][0];
+  gmem<double> *const out1 = (gmem<double> *)L.phi[dst][1];
+  const int H = L.H, nj = L.nj;
+  const int base = PASS == 0 ? v * nj * H + (u >> 1) : PASS == 1 ? (v * nj + u) * H : v * H + (u >> 1);
+  const int step = PASS == 0 ? H : PASS == 1 ? 0 : nj * H;
   for (int p1 = len - 1; p1 >= 0; p1 -= TDMA_CHUNK) {
-    double av[TDMA_CHUNK], cv[TDMA_CHUNK];
+    double cv[TDMA_CHUNK], aq[TDMA_CHUNK];
 #pragma unroll
     for (int q = 0; q < TDMA_CHUNK; ++q) {
       const int p = p1 - q;
       if (p < 0) break;
-      av[q] = sa[(size_t)p * nu + u];
-      cv[q] = sc[(size_t)p * nu + u];
+      cv[q] = xs[p * lwp + l];
+      aq[q] = av[p * lwp + l];
     }
 #pragma unroll
     for (int q = 0; q < TDMA_CHUNK; ++q) {
       const int p = p1 - q;
       if (p < 0) break;
-      next = cv[q] - av[q] * next;
-      tdma_cell<PASS>(u, v, p, i, j, k);
-      ((double *)L.phi[dst][(i + j + k) & 1])[cidx(L.H, L.nj, i, j, k)] = next;
+      next = cv[q] - aq[q] * next;
+      const int e = PASS == 1 ? base + (p >> 1) : base + p * step;
+      (((u + v + p) & 1) ? out1 : out0)[e] = next;
     }
   }
 }
 
+// one diagonal of one pass over groups of tdma_group_lines(len) lines
+// (Tile::x0 = the group's first line)
 template <int PASS>
 __global__ void __launch_bounds__(NTHREADS)
     k_tdma_diag(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
                 const int *__restrict__ active, int d, int src, int zero) {
+  __shared__ double xs[2 * TDMA_CELLS];  // rs, then c
+  __shared__ double av[2 * TDMA_CELLS];  // a
+  __shared__ double rv[2 * TDMA_CELLS];  // 1/(ap - alo*a(p-1)) (constant coefficients)
   const Tile t = tiles[blockIdx.x];
   const LevelDev &L = levs[t.lev];
   if (!active[L.blk]) return;
-  const int u = t.x0 + (int)threadIdx.x + TILE_X * (int)threadIdx.y;
-  const int v = d - u;
-  // lines: pass 0 (i, k) along j, 1 (j, k) along i, 2 (i, j) along k
-  const int nu = PASS == 1 ? L.nj : L.ni;
-  const int nv = PASS == 2 ? L.nj : L.nk;
-  const int len = PASS == 0 ? L.nj : PASS == 1 ? L.ni : L.nk;
-  if (u < nu && v >= 0 && v < nv) tdma_line<PASS>(L, u, v, nu, len, src, zero);
+  const TdmaGeom<PASS> G(L);
+  const int lw = tdma_group_lines(G.len), lwp = lw > 1 ? lw + 1 : 1;
+  const int u0 = t.x0;
+  if (max(u0, d - G.nv + 1) >= min(min(u0 + lw, G.nu), d + 1)) return;  // no line of d
+  tdma_diagonal<PASS>(L, u0, lw, lwp, d, src, zero, xs, av, rv,
+                      (int)threadIdx.x + TILE_X * (int)threadIdx.y);
+}
+
+// a whole TDMA sweep (three passes, every diagonal) of one small level in
+// one workgroup per block: every line of a diagonal in dynamic LDS,
+// 3 * (nu+1) * len doubles (tdma_sweep_lds_bytes)
+template <int PASS>
+__device__ __forceinline__ void tdma_pass_in_group(const LevelDev &L, int src, int zero, double *buf, int tid) {
+  const TdmaGeom<PASS> G(L);
+  const int lwp = G.nu + 1;
+  const size_t n = (size_t)lwp * G.len;
+  for (int d = 0; d < G.nu + G.nv - 1; ++d) {
+    tdma_diagonal<PASS>(L, 0, G.nu, lwp, d, src, zero, buf, buf + n, buf + 2 * n, tid);
+    __syncthreads();  // the diagonal's phi before the next one reads it
+  }
+}
+
+__global__ void __launch_bounds__(NTHREADS)
+    k_tdma_sweep(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+                 const int *__restrict__ active, int src, int zero) {
+  extern __shared__ double line_lds[];
+  const Tile t = tiles[blockIdx.x];
+  const LevelDev &L = levs[t.lev];
+  if (!active[L.blk]) return;
+  const int tid = (int)threadIdx.x + TILE_X * (int)threadIdx.y;
+  tdma_pass_in_group<0>(L, src, zero, line_lds, tid);
+  tdma_pass_in_group<1>(L, src, zero, line_lds, tid);
+  tdma_pass_in_group<2>(L, src, zero, line_lds, tid);
 }
 
 }  // namespace
@@ -2393,6 +2574,25 @@ void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
   if (ntiles <= 0) return;
   auto k = pass == 0 ? k_tdma_diag<0> : pass == 1 ? k_tdma_diag<1> : k_tdma_diag<2>;
   hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, d, src, zero);
+}
+
+void launch_tdma_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                       const int *active, int src, int zero, size_t lds_bytes) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_tdma_sweep, dim3(ntiles), tile_block(), lds_bytes, s, tiles, levs, active,
+                     src, zero);
+}
+
+void launch_tdma_tables(hipStream_t s, const LevelDev *levs, int ndesc) {
+  if (ndesc <= 0) return;
+  hipLaunchKernelGGL(k_tdma_tables, dim3((unsigned)((ndesc * 48 + 255) / 256)), dim3(256), 0, s,
+                     levs, ndesc);
+}
+
+size_t tdma_sweep_lds_bytes(const LevelDev &L) {
+  const size_t p0 = (size_t)(L.ni + 1) * L.nj, p1 = (size_t)(L.nj + 1) * L.ni,
+               p2 = (size_t)(L.ni + 1) * L.nk;
+  return 3 * std::max(p0, std::max(p1, p2)) * sizeof(double);
 }
 
 void launch_lsm_residual(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,

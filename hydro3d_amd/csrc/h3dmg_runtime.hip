@@ -56,6 +56,8 @@ struct TileSet {
   bool fused = false;     // smoother sets: fused red-black tiles (one launch per sweep)
   const TileSet *nat = nullptr;  // smoother sets: the group's natural tiles (variable density)
   int ndiag[3] = {0, 0, 0};      // GS / TDMA sets: launches per sweep (pass)
+  int pass_begin[4] = {0, 0, 0, 0};  // TDMA sets: tiles of pass q at [begin[q], begin[q+1])
+  size_t tdma_lds = 0;  // TDMA sets: > 0 = whole sweeps in one workgroup per block (LDS bytes)
   int n() const { return (int)h.size(); }
 };
 
@@ -273,11 +275,15 @@ void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool 
     if (h->prm.smoother == H3DMG_SMOOTH_GS)  // hyperplane wavefront (every level FP64)
       for (int d = 0; d < ts.ndiag[0]; ++d)
         launch_gs_plane(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, d, buf, zero ? 1 : 0);
+    else if (h->prm.smoother == H3DMG_SMOOTH_TDMA && ts.tdma_lds)  // small levels
+      launch_tdma_sweep(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, buf, zero ? 1 : 0,
+                        ts.tdma_lds);
     else if (h->prm.smoother == H3DMG_SMOOTH_TDMA)  // j-, i-, k-line diagonals
       for (int pass = 0; pass < 3; ++pass)
         for (int d = 0; d < ts.ndiag[pass]; ++d)
-          launch_tdma_diag(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, pass, d, buf,
-                           zero ? 1 : 0);
+          launch_tdma_diag(h->stream, ts.d + ts.pass_begin[pass],
+                           ts.pass_begin[pass + 1] - ts.pass_begin[pass], h->d_levs, h->d_active,
+                           pass, d, buf, zero ? 1 : 0);
     else if (h->lsm)  // per-cell coefficients (every level FP64)
       for (int colour = 0; colour < 2; ++colour)
         launch_lsm_sweep(h->stream, ts.nat->d, ts.nat->n(), h->d_levs, h->d_active, colour, buf,
@@ -660,14 +666,13 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       off = align_up(off + n3 * 8, 256);
     }
   }
-  // line-TDMA scratch: per block 2 x its level-1 cells (a and c of the
-  // Thomas recurrence; every level of the block reuses it)
-  std::vector<size_t> line_off(nblocks, 0);
+  // line-TDMA tables: per descriptor 32*(ni+nj+nk) doubles
+  std::vector<size_t> tdma_off(h->levs.size(), 0);
   if (h->prm.smoother == H3DMG_SMOOTH_TDMA)
-    for (int b = 0; b < nblocks; ++b) {
-      const BlockInfo &B = h->blk[b];
-      line_off[b] = off;
-      off = align_up(off + 2 * (size_t)B.spec.ni * B.spec.nj * B.spec.nk * 8, 256);
+    for (size_t d = 0; d < h->levs.size(); ++d) {
+      const LevelDev &L = h->levs[d];
+      tdma_off[d] = off;
+      off = align_up(off + (size_t)32 * (L.ni + L.nj + L.nk) * 8, 256);
     }
   // staging: largest padded level-1 array
   size_t stage = 0;
@@ -816,7 +821,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     for (int bb = 0; bb < 2; ++bb)
       for (int c = 0; c < 2; ++c) L.phi[bb][c] = A + P.phi[bb][c];
     for (int c = 0; c < 2; ++c) L.rhs[c] = A + P.rhs[c];
-    if (h->prm.smoother == H3DMG_SMOOTH_TDMA) L.line = (double *)(A + line_off[L.blk]);
+    if (h->prm.smoother == H3DMG_SMOOTH_TDMA) L.tdma = (double *)(A + tdma_off[P.d]);
   }
   h->fine.resize(nblocks);
   for (int b = 0; b < nblocks; ++b) {
@@ -848,6 +853,10 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   h->d_levs = (LevelDev *)(A + levs_off);
   HIP_TRY(hipMemcpy(h->d_levs, h->levs.data(), h->levs.size() * sizeof(LevelDev),
                     hipMemcpyHostToDevice));
+  if (h->prm.smoother == H3DMG_SMOOTH_TDMA) {
+    launch_tdma_tables(h->stream, h->d_levs, (int)h->levs.size());
+    HIP_TRY(hipGetLastError());
+  }
   h->d_fine = (FineDev *)(A + fine_off);
   HIP_TRY(hipMemcpy(h->d_fine, h->fine.data(), nblocks * sizeof(FineDev), hipMemcpyHostToDevice));
   h->active.assign(nblocks, 1);
@@ -951,26 +960,49 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
     ts.fused = most > 0 && most <= h->fused_max && h->prm.smoother == H3DMG_SMOOTH_RBGS;
     ts.ndiag[0] = ts.ndiag[1] = ts.ndiag[2] = 0;
     if (h->prm.smoother != H3DMG_SMOOTH_RBGS) {
-      // wavefront smoothers: GS (i, j) tiles of 64 x 4 (kind 5), TDMA tiles
-      // of NTHREADS lines (kind 6); one plane deep
-      const bool gs = h->prm.smoother == H3DMG_SMOOTH_GS;
+      // wavefront smoothers: GS (i, j) tiles of 64 x 4 (kind 5); TDMA tiles
+      // of tdma_group_lines(len) lines, pass by pass (kind 6); one plane deep
       ts.depth = 1;
-      for (const BlockInfo &B : h->blk)
-        if (member(B)) {
-          const int d = B.desc[g];
-          const LevelDev &L = h->levs[d];
-          if (gs) {
+      if (h->prm.smoother == H3DMG_SMOOTH_GS) {
+        for (const BlockInfo &B : h->blk)
+          if (member(B)) {
+            const int d = B.desc[g];
+            const LevelDev &L = h->levs[d];
             for (int y = 0; y < L.nj; y += TILE_Y)
               for (int x = 0; x < L.ni; x += TILE_X) ts.h.push_back(Tile{d, x, y, 0, 1, 5});
             ts.ndiag[0] = std::max(ts.ndiag[0], L.ni + L.nj + L.nk - 2);
-          } else {
-            for (int x = 0; x < std::max(L.ni, L.nj); x += NTHREADS)
-              ts.h.push_back(Tile{d, x, 0, 0, 1, 6});
-            ts.ndiag[0] = std::max(ts.ndiag[0], L.ni + L.nk - 1);
-            ts.ndiag[1] = std::max(ts.ndiag[1], L.nj + L.nk - 1);
-            ts.ndiag[2] = std::max(ts.ndiag[2], L.ni + L.nj - 1);
           }
+      } else {
+        // small levels: whole sweeps in one workgroup per block
+        bool small = true;
+        size_t lds = 0;
+        for (const BlockInfo &B : h->blk)
+          if (member(B)) {
+            const LevelDev &L = h->levs[B.desc[g]];
+            small = small && std::max(L.ni, L.nj) <= NTHREADS &&
+                    (long)L.ni * L.nj <= TDMA_SWEEP_CELLS && (long)L.nj * L.nk <= TDMA_SWEEP_CELLS &&
+                    (long)L.ni * L.nk <= TDMA_SWEEP_CELLS;
+            lds = std::max(lds, tdma_sweep_lds_bytes(L));
+          }
+        ts.tdma_lds = small ? lds : 0;
+        if (small)
+          for (const BlockInfo &B : h->blk)
+            if (member(B)) ts.h.push_back(Tile{B.desc[g], 0, 0, 0, 1, 7});
+        for (int pass = 0; pass < 3 && !small; ++pass) {
+          ts.pass_begin[pass] = ts.n();
+          for (const BlockInfo &B : h->blk)
+            if (member(B)) {
+              const int d = B.desc[g];
+              const LevelDev &L = h->levs[d];
+              const int nu = pass == 1 ? L.nj : L.ni, nv = pass == 2 ? L.nj : L.nk;
+              const int len = pass == 0 ? L.nj : pass == 1 ? L.ni : L.nk;
+              const int lw = tdma_group_lines(len);
+              for (int x = 0; x < nu; x += lw) ts.h.push_back(Tile{d, x, 0, 0, 1, 6});
+              ts.ndiag[pass] = std::max(ts.ndiag[pass], nu + nv - 1);
+            }
         }
+        ts.pass_begin[3] = ts.n();
+      }
     } else if (ts.fused) {
       ts.depth = FUSED_Z;
       for (const BlockInfo &B : h->blk)
@@ -1229,6 +1261,14 @@ h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_pa
     delete h;
     return fail(H3DMG_ERR_UNSUPPORTED, "the GS and line-TDMA smoothers run in FP64");
   }
+  if (P.smoother == H3DMG_SMOOTH_TDMA)
+    for (int b = 0; b < nblocks; ++b)
+      if (blocks[b].ni > TDMA_MAX_LEN || blocks[b].nj > TDMA_MAX_LEN ||
+          blocks[b].nk > TDMA_MAX_LEN) {
+        delete h;
+        return fail(H3DMG_ERR_UNSUPPORTED, "line TDMA: block %d has lines over %d cells", b,
+                    TDMA_MAX_LEN);
+      }
   if (P.nu1 < 0 || P.nu2 < 0 || P.nu1 + P.nu2 < 1 || P.maxcy < 0 ||
       (P.precision != H3DMG_FP64 && P.precision != H3DMG_MIXED) ||
       (P.coarse != H3DMG_COARSE_SMOOTH && P.coarse != H3DMG_COARSE_EXACT) ||

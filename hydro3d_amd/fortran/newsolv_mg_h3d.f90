@@ -15,15 +15,16 @@
 ! (exchange(4), pbound, and calvel's exchange(1..3)), and keeps the
 ! not-converged / BIG RMAX handling, boundu/v/w and the log lines.
 !
-! Scope: the RBGS multigrid, constant-coefficient or variable-density
-! (L_LSM: the density goes to the device every call, as the reference
-! rebuilds its coefficients every solve, newsolv_mg.for:13-15), on blocks
-! without neighbours (every face a wall or periodic onto itself), one MPI
-! rank per GPU.  Anything else stops with a message rather than running a
-! different algorithm.  The precision is FP64 (bitwise the reference loop)
+! Scope: the multigrid with any of the reference's smoothers (mg_itrsch 1
+! lexicographic GS, 2 line TDMA, 3 red-black GS), constant-coefficient or
+! variable-density (L_LSM: the density goes to the device every call, as the
+! reference rebuilds its coefficients every solve, newsolv_mg.for:13-15), on
+! blocks without neighbours (every face a wall or periodic onto itself), one
+! MPI rank per GPU.  Anything else stops with a message rather than running
+! a different algorithm.  The precision is FP64 (bitwise the reference loop)
 ! unless the environment variable H3DMG_PRECISION=mixed selects the FP32
 ! V-cycle with the exact coarsest solve inside the FP64 velocity loop
-! (constant coefficients only).
+! (constant coefficients and red-black GS only).
 subroutine newsolv_mg_h3d
   use vars
   use mpi
@@ -158,7 +159,7 @@ contains
     prm%dt = dt
     prm%min_cycles = 2
     call get_environment_variable('H3DMG_PRECISION', pv, plen, pst)
-    if (pst == 0 .and. pv(1:5) == 'mixed' .and. .not. L_LSM) then
+    if (pst == 0 .and. pv(1:5) == 'mixed' .and. .not. L_LSM .and. mg_itrsch == 3) then
        prm%precision = H3DMG_MIXED
        prm%coarse = H3DMG_COARSE_EXACT
     end if

@@ -52,7 +52,11 @@ typedef enum {
  * wall for the pressure (src_v70/mgsolver.for:194-205). */
 enum { H3DMG_BC_WALL = 0, H3DMG_BC_PERIODIC = 1 };
 
-/* smoother (mg_itrsch, src_v70/control.cin line 10) */
+/* smoother (mg_itrsch, src_v70/control.cin line 10; mgrelax,
+ * src_v70/mgsolver.for:407-605).  GS and TDMA run in FP64 on the
+ * level-by-level path (no mixed precision, LDS tail or fused sweeps): one
+ * launch per hyperplane i+j+k (GS) or per line diagonal of each TDMA pass,
+ * bitwise the reference's in-place loop order. */
 enum { H3DMG_SMOOTH_GS = 1, H3DMG_SMOOTH_TDMA = 2, H3DMG_SMOOTH_RBGS = 3 };
 
 /* stopping criterion */
@@ -87,7 +91,7 @@ typedef struct {
 } h3dmg_block;
 
 typedef struct {
-  int smoother;      /* H3DMG_SMOOTH_* (device: RBGS)              */
+  int smoother;      /* H3DMG_SMOOTH_*                              */
   int nu1, nu2;      /* irestr, iproln (control.cin line 11)        */
   int maxcy;         /* cycle cap (control.cin line 11)             */
   int criterion;     /* H3DMG_CRIT_*                                */
@@ -255,7 +259,8 @@ int h3dmg_get_march(const h3dmg_handle *h, int level);
  * sweep as one fused launch (both colour halves, the red halo recomputed
  * per tile) instead of two; 0 (the default) turns fusion off.  Results are
  * identical either way.  Drops a captured graph.  h3dmg_get_march returns 0
- * for a fused level; h3dmg_get_fused returns the threshold in use. */
+ * for a fused level; h3dmg_get_fused returns the threshold in use.  Red-black
+ * smoother only (H3DMG_ERR_UNSUPPORTED otherwise). */
 h3dmg_status h3dmg_set_fused(h3dmg_handle *h, long long max_cells);
 long long h3dmg_get_fused(const h3dmg_handle *h);
 /* Levels >= `level` of every block run as one workgroup per block in a
@@ -264,7 +269,9 @@ long long h3dmg_get_fused(const h3dmg_handle *h);
  * fit in one workgroup's LDS), a level above every block's coarsest turns the
  * tail off; a level that does not fit returns H3DMG_ERR_ARG.  Results are
  * identical either way.  Drops a captured graph.  h3dmg_get_tail returns the
- * level in use (nlevels_max + 1 when off). */
+ * level in use (nlevels_max + 1 when off).  Red-black smoother only: with GS
+ * or TDMA the tail is off and a level that would turn it on returns
+ * H3DMG_ERR_UNSUPPORTED. */
 h3dmg_status h3dmg_set_tail(h3dmg_handle *h, int level);
 int h3dmg_get_tail(const h3dmg_handle *h);
 

@@ -82,13 +82,16 @@ def test_gpu_line_smoothers_variable_density_bitwise(gpu_lib, path, smoother, sc
 @pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
 @pytest.mark.parametrize("smoother,scheme", SMOOTHERS, ids=[s for s, _ in SMOOTHERS])
 def test_gpu_line_smoothers_solve_bitwise_vs_oracle(gpu_lib, smoother, scheme, graph):
-    """Two unequal blocks in one handle (their hyperplanes and line diagonals
-    share launches): six cycles of each are bitwise the oracle's solve of that
-    block alone, residual history to 1e-12."""
+    """Three unequal blocks in one handle (their hyperplanes and line
+    diagonals share launches; the finest TDMA level runs diagonal by
+    diagonal over workgroups of lines, the coarser ones as whole sweeps in
+    one workgroup per block): six cycles of each are bitwise the oracle's
+    solve of that block alone."""
     from oracle import oracle as O
     rng = np.random.default_rng(17)
     specs = [dict(n=(32, 32, 32), h=(1 / 32,) * 3, bc=(1, 1, 1, 1, 0, 0), nlev=3),
-             dict(n=(24, 16, 40), h=(1 / 24, 1 / 16, 1 / 40), bc=(0, 0, 1, 1, 0, 0), nlev=3)]
+             dict(n=(24, 16, 40), h=(1 / 24, 1 / 16, 1 / 40), bc=(0, 0, 1, 1, 0, 0), nlev=3),
+             dict(n=(64, 48, 32), h=(1 / 64, 1 / 48, 1 / 32), bc=(1, 1, 0, 0, 1, 1), nlev=3)]
     fs = []
     for s in specs:
         f = rng.standard_normal(s["n"][::-1])
