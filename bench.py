@@ -309,6 +309,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=400,
                     help="requests of the serving stream to replay (0: skip)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 48-block batch")
+    ap.add_argument("--no-lines", action="store_true", help="skip the GS / line-TDMA solves")
+    ap.add_argument("--only-lines", action="store_true",
+                    help="only time the GS / line-TDMA solves (profiling) and print their JSON")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "hydro3d_amd", "tuned.json"))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -324,6 +327,10 @@ def main():
     torch.cuda.set_device(0)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.only_lines:
+        print(json.dumps({"line_smoothers_64": line_smoothers(args.steps, args.warmup)}),
+              flush=True)
         return
     c = configs.single_256()
     spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
@@ -384,9 +391,11 @@ def main():
     log("single 64^3")
     small = single_64(args.steps, args.warmup)
     log(small)
-    log("line smoothers 64^3")
-    lines = line_smoothers(max(2, args.steps // 4), 1)
-    log(lines)
+    lines = None
+    if not args.no_lines:
+        log("line smoothers 64^3")
+        lines = line_smoothers(max(2, args.steps // 4), 1)
+        log(lines)
     batched = None
     if not args.no_batched:
         log("batched 48 blocks")
@@ -402,8 +411,10 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(c, cyc64)
-        lines["cpu_oracle_vcycle_ms_1core"] = cpu_line_smoothers()
-        log(cpu, lines["cpu_oracle_vcycle_ms_1core"])
+        log(cpu)
+        if lines is not None:
+            lines["cpu_oracle_vcycle_ms_1core"] = cpu_line_smoothers()
+            log(lines["cpu_oracle_vcycle_ms_1core"])
 
     S64.close()
     Smx.close()

@@ -179,7 +179,8 @@ void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
 // every line of a diagonal in LDS: tdma_sweep_lds_bytes(level) of dynamic
 // LDS (the largest over the launch, under 64 KiB), NTHREADS >= the lines
 // of a diagonal; used where every pass has at most TDMA_SWEEP_CELLS cells
-// per diagonal
+// per diagonal (above that, one launch per diagonal over several
+// workgroups is faster: measured at 64^3)
 constexpr int TDMA_SWEEP_CELLS = 2048;
 void launch_tdma_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                        const int *active, int src, int zero, size_t lds_bytes);

@@ -979,9 +979,9 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
         for (const BlockInfo &B : h->blk)
           if (member(B)) {
             const LevelDev &L = h->levs[B.desc[g]];
+            // cells per diagonal: ni*nj (j-lines), nj*ni (i-lines), ni*nk (k-lines)
             small = small && std::max(L.ni, L.nj) <= NTHREADS &&
-                    (long)L.ni * L.nj <= TDMA_SWEEP_CELLS && (long)L.nj * L.nk <= TDMA_SWEEP_CELLS &&
-                    (long)L.ni * L.nk <= TDMA_SWEEP_CELLS;
+                    (long)L.ni * L.nj <= TDMA_SWEEP_CELLS && (long)L.ni * L.nk <= TDMA_SWEEP_CELLS;
             lds = std::max(lds, tdma_sweep_lds_bytes(L));
           }
         ts.tdma_lds = small ? lds : 0;

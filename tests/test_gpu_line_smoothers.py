@@ -4,8 +4,8 @@ MI355X.
 
 Bar: bitwise.  One sweep per level and one V-cycle against the reference
 build's fixtures (constant and variable-density coefficients); whole solves
-of two unequal blocks in one handle against the oracle, in a captured graph
-and eagerly.
+of unequal blocks in one handle against the oracle, in a captured graph and
+eagerly.
 """
 import glob
 import os
@@ -81,23 +81,30 @@ def test_gpu_line_smoothers_variable_density_bitwise(gpu_lib, path, smoother, sc
 
 @pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
 @pytest.mark.parametrize("smoother,scheme", SMOOTHERS, ids=[s for s, _ in SMOOTHERS])
-def test_gpu_line_smoothers_solve_bitwise_vs_oracle(gpu_lib, smoother, scheme, graph):
-    """Three unequal blocks in one handle (their hyperplanes and line
-    diagonals share launches; the finest TDMA level runs diagonal by
-    diagonal over workgroups of lines, the coarser ones as whole sweeps in
-    one workgroup per block): six cycles of each are bitwise the oracle's
-    solve of that block alone."""
+@pytest.mark.parametrize("group", ["whole_sweeps", "diagonals"])
+def test_gpu_line_smoothers_solve_bitwise_vs_oracle(gpu_lib, smoother, scheme, graph, group):
+    """Unequal blocks in one handle (their hyperplanes and line diagonals
+    share launches): six cycles of each are bitwise the oracle's solve of
+    that block alone.  TDMA levels of up to 2048 cells per diagonal run as
+    whole sweeps in one workgroup per block ("whole_sweeps"); a 128 x 40
+    finest level makes its group run diagonal by diagonal over workgroups of
+    lines ("diagonals")."""
     from oracle import oracle as O
     rng = np.random.default_rng(17)
-    specs = [dict(n=(32, 32, 32), h=(1 / 32,) * 3, bc=(1, 1, 1, 1, 0, 0), nlev=3),
-             dict(n=(24, 16, 40), h=(1 / 24, 1 / 16, 1 / 40), bc=(0, 0, 1, 1, 0, 0), nlev=3),
-             dict(n=(64, 48, 32), h=(1 / 64, 1 / 48, 1 / 32), bc=(1, 1, 0, 0, 1, 1), nlev=3)]
+    if group == "whole_sweeps":
+        specs = [dict(n=(32, 32, 32), h=(1 / 32,) * 3, bc=(1, 1, 1, 1, 0, 0), nlev=3),
+                 dict(n=(24, 16, 40), h=(1 / 24, 1 / 16, 1 / 40), bc=(0, 0, 1, 1, 0, 0), nlev=3),
+                 dict(n=(48, 32, 40), h=(1 / 48, 1 / 32, 1 / 40), bc=(1, 1, 0, 0, 1, 1), nlev=3)]
+    else:
+        specs = [dict(n=(128, 40, 16), h=(1 / 128, 1 / 40, 1 / 16), bc=(1, 1, 0, 0, 1, 1),
+                      nlev=3),
+                 dict(n=(32, 32, 32), h=(1 / 32,) * 3, bc=(0, 0, 1, 1, 0, 0), nlev=3)]
     fs = []
     for s in specs:
         f = rng.standard_normal(s["n"][::-1])
         fs.append(f - f.mean())
     nc = 6
-    print(f"case {smoother} graph={graph}", flush=True)
+    print(f"case {smoother} graph={graph} {group}", flush=True)
     with _solver(specs, maxcy=nc, tol=0.0, use_graph=graph, smoother=smoother) as S:
         for b, f in enumerate(fs):
             S.set_rhs(b, f)
