@@ -197,21 +197,23 @@ def single_64(steps, warmup):
     return out
 
 
-def line_smoothers(steps, warmup):
+def line_smoothers(steps, warmup, n=64):
     """The reference's other smoothers (mg_itrsch 1 lexicographic GS, 2 line
-    TDMA; bitwise the reference, FP64) on the 64^3 workload to 1e-8: solve
-    latency and one V-cycle.  Both are wavefronts over the reference's loop
-    order (a launch per hyperplane / line diagonal), so they are bound by
-    launch latency and the Thomas recurrence's dependent chain, not HBM."""
+    TDMA; bitwise the reference, FP64) beside red-black on the n^3 workload
+    (64^3: the single_64 block) to 1e-8: solve latency and one V-cycle.  GS
+    and TDMA are wavefronts over the reference's loop order (a launch per
+    hyperplane / line diagonal), so they are bound by launch latency and the
+    Thomas recurrence's dependent chain, not HBM."""
     import torch
     from hydro3d_amd import MultigridSolver, configs
-    c = configs.single_64()
+    nlev = int(round(np.log2(n))) - 2
+    c = configs.cube(n, configs.BC_PPN, nlev, seed=0)
     spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
     f = torch.from_numpy(c["f"]).cuda()
     z = torch.zeros_like(f)
-    out = {"workload": "single block 64^3 (periodic x/y, Neumann z), 4 levels, seed 0, "
+    out = {"workload": f"single block {n}^3 (periodic x/y, Neumann z), {nlev} levels, seed 0, "
                        "FP64 V(2,2) to 1e-8"}
-    for sm in ("gs", "tdma"):
+    for sm in ("rbgs", "gs", "tdma"):
         with MultigridSolver([spec], tol=1e-8, smoother=sm) as S:
             lat, cyc, fin = timed_solves(S, f, z, steps, warmup)
             torch.cuda.synchronize()
@@ -312,6 +314,8 @@ def main():
     ap.add_argument("--no-lines", action="store_true", help="skip the GS / line-TDMA solves")
     ap.add_argument("--only-lines", action="store_true",
                     help="only time the GS / line-TDMA solves (profiling) and print their JSON")
+    ap.add_argument("--lines-size", type=int, default=64, choices=(64, 128, 256),
+                    help="cube size of the --only-lines workload")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "hydro3d_amd", "tuned.json"))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -329,8 +333,8 @@ def main():
         copy_probe()
         return
     if args.only_lines:
-        print(json.dumps({"line_smoothers_64": line_smoothers(args.steps, args.warmup)}),
-              flush=True)
+        print(json.dumps({f"line_smoothers_{args.lines_size}":
+                          line_smoothers(args.steps, args.warmup, args.lines_size)}), flush=True)
         return
     c = configs.single_256()
     spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
