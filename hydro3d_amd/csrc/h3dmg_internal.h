@@ -158,12 +158,13 @@ void launch_lsm_restrict(hipStream_t s, const Tile *tiles, int ntiles, const Lev
                          const int *active, int src_f);
 
 // ---- lexicographic GS and line-TDMA smoothers (mg_itrsch = 1, 2) ---------
-// Tiles: GS 64 x 4 (i, j) tiles of every descriptor (Tile::x0, y0); TDMA
+// Tiles: GS tiles of GS_TILE^3 cells (Tile::x0, y0, z0), one launch per
+// tile-diagonal (the tiles whose tile indices sum to D, every block); TDMA
 // tiles of tdma_group_lines(len) lines of one pass (Tile::x0 = first line),
-// a line of len cells.  One launch per hyperplane d = i+j+k (GS) or line
-// diagonal d (TDMA pass 0: j-lines, i+k = d; 1: i-lines, j+k = d; 2:
-// k-lines, i+j = d).  A sweep reads buffer src and writes 1-src; every
-// level is FP64.
+// a line of len cells, one launch per line diagonal d (pass 0: j-lines,
+// i+k = d; 1: i-lines, j+k = d; 2: k-lines, i+j = d).  A sweep reads buffer
+// src and writes 1-src; every level is FP64.
+constexpr int GS_TILE = 8;
 constexpr int TDMA_CELLS = 1024;    // line cells one TDMA workgroup stages in LDS
 constexpr int TDMA_MAX_LEN = 1024;  // longest line (cells along one axis)
 __host__ __device__ inline int tdma_group_lines(int len) {
@@ -171,8 +172,8 @@ __host__ __device__ inline int tdma_group_lines(int len) {
   while (lw < NTHREADS && 2 * lw * len <= TDMA_CELLS) lw *= 2;
   return lw;
 }
-void launch_gs_plane(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                     const int *active, int d, int src, int zero);
+void launch_gs_tiles(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                     const int *active, int src, int zero);
 void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                       const int *active, int pass, int d, int src, int zero);
 // a whole TDMA sweep of small levels, one workgroup per block (Tile::lev),

@@ -2214,29 +2214,106 @@ __device__ __forceinline__ double before_sweep(const LevelDev &L, int src, int z
   return cell_at(L, src, x, y, z);
 }
 
-// phi = (rhs - (aw*W + ae*E + as*S + an*N + ab*B + at*T)) / ap at the cells
-// of hyperplane d
-__global__ void __launch_bounds__(NTHREADS)
-    k_gs_plane(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
-               const int *__restrict__ active, int d, int src, int zero) {
+// One GS tile of GS_T^3 cells (Tile::x0, y0, z0): the tiles of one
+// tile-diagonal I+J+K = D depend only on tiles of D-1 (updated) and D+1
+// (not yet), as cells of a hyperplane do, so one launch per tile-diagonal
+// repeats the reference's loop order.  The tile and its six face halos go
+// to LDS -- the tile's cells and its east/north/top halo as before the
+// sweep (src), its west/south/bottom halo as updated (dst, finished by
+// earlier tile-diagonals), ghost positions as images of src -- and the
+// wavefront walks the tile's own hyperplanes in place:
+//   phi = (rhs - (aw*W + ae*E + as*S + an*N + ab*B + at*T)) / ap
+// One wavefront per tile: thread (lx, ly) owns lz = h - lx - ly of plane h.
+constexpr int GS_T = GS_TILE;
+constexpr int GS_P = GS_T + 2;  // with the halo
+
+__global__ void __launch_bounds__(GS_T * GS_T)
+    k_gs_tile(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+              const int *__restrict__ active, int src, int zero) {
+  __shared__ double v[GS_P * GS_P * GS_P];      // phi, tile + face halos
+  __shared__ double rl[GS_T * GS_T * GS_T];     // rhs of the tile
+  __shared__ double cl[7][GS_T * GS_T * GS_T];  // variable-density coefficients
   const Tile t = tiles[blockIdx.x];
   const LevelDev &L = levs[t.lev];
   if (!active[L.blk]) return;
-  const int i = t.x0 + (int)threadIdx.x, j = t.y0 + (int)threadIdx.y, k = d - i - j;
-  if (i >= L.ni || j >= L.nj || k < 0 || k >= L.nk) return;
   const int dst = 1 - src;
-  const Coef<double> c = smoother_coef(L, i, j, k);
-  const double W = i > 0 ? cell_at(L, dst, i - 1, j, k) : before_sweep(L, src, zero, i - 1, j, k);
-  const double S = j > 0 ? cell_at(L, dst, i, j - 1, k) : before_sweep(L, src, zero, i, j - 1, k);
-  const double B = k > 0 ? cell_at(L, dst, i, j, k - 1) : before_sweep(L, src, zero, i, j, k - 1);
-  const double E = before_sweep(L, src, zero, i + 1, j, k);
-  const double N = before_sweep(L, src, zero, i, j + 1, k);
-  const double T = before_sweep(L, src, zero, i, j, k + 1);
-  const int colour = (i + j + k) & 1;
-  const size_t e = cidx(L.H, L.nj, i, j, k);
-  const double r = gload(L.rhs[colour], e);
-  const double sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * T;
-  ((gmem<double> *)L.phi[dst][colour])[e] = (r - sum) / c.ap;
+  const int tid = (int)threadIdx.x;
+  const bool var = L.lsm != nullptr;
+  // every load of the tile is issued before the first LDS store
+  constexpr int NT = GS_T * GS_T;
+  constexpr int NR = GS_T * GS_T * GS_T / NT, NV = (GS_P * GS_P * GS_P + NT - 1) / NT;
+  double rr[NR], vv[NV];
+#pragma unroll
+  for (int it = 0; it < NR; ++it) {
+    const int q = tid + it * NT;
+    const int x = t.x0 + q % GS_T, y = t.y0 + (q / GS_T) % GS_T, z = t.z0 + q / (GS_T * GS_T);
+    rr[it] = 0.0;
+    if (x < L.ni && y < L.nj && z < L.nk)
+      rr[it] = gload(L.rhs[(x + y + z) & 1], cidx(L.H, L.nj, x, y, z));
+  }
+#pragma unroll
+  for (int it = 0; it < NV; ++it) {
+    const int q = tid + it * NT;
+    const int lz = q / (GS_P * GS_P) - 1, ly = (q / GS_P) % GS_P - 1, lx = q % GS_P - 1;
+    const int outside = (lx < 0 || lx >= GS_T) + (ly < 0 || ly >= GS_T) + (lz < 0 || lz >= GS_T);
+    const int x = t.x0 + lx, y = t.y0 + ly, z = t.z0 + lz;
+    vv[it] = 0.0;
+    // edges and corners of the halo: no 7-point stencil reads them
+    if (q >= GS_P * GS_P * GS_P || outside > 1) continue;
+    if ((unsigned)x >= (unsigned)L.ni || (unsigned)y >= (unsigned)L.nj ||
+        (unsigned)z >= (unsigned)L.nk)
+      vv[it] = before_sweep(L, src, zero, x, y, z);
+    else if (lx < 0 || ly < 0 || lz < 0)
+      vv[it] = cell_at(L, dst, x, y, z);
+    else
+      vv[it] = zero ? 0.0 : cell_at(L, src, x, y, z);
+  }
+#pragma unroll
+  for (int it = 0; it < NR; ++it) rl[tid + it * NT] = rr[it];
+#pragma unroll
+  for (int it = 0; it < NV; ++it)
+    if (tid + it * NT < GS_P * GS_P * GS_P) v[tid + it * NT] = vv[it];
+  if (var)
+    for (int q = tid; q < GS_T * GS_T * GS_T; q += NT) {
+      const int x = t.x0 + q % GS_T, y = t.y0 + (q / GS_T) % GS_T, z = t.z0 + q / (GS_T * GS_T);
+      if (x >= L.ni || y >= L.nj || z >= L.nk) continue;
+      const size_t e = cidx(L.H, L.nj, x, y, z);
+      for (int a = 0; a < 7; ++a) cl[a][q] = gload(lsm_coef(L, a, (x + y + z) & 1), e);
+    }
+  __syncthreads();
+  const int lx = tid % GS_T, ly = tid / GS_T;
+  const int i = t.x0 + lx, j = t.y0 + ly;
+  // the planes touch LDS only: a barrier then waits for no global store
+  for (int h = 0; h < 3 * GS_T - 2; ++h) {
+    const int lz = h - lx - ly, k = t.z0 + lz;
+    if (lz >= 0 && lz < GS_T && i < L.ni && j < L.nj && k < L.nk) {
+      const int c0 = ((lz + 1) * GS_P + ly + 1) * GS_P + lx + 1;
+      const int q = (lz * GS_T + ly) * GS_T + lx;
+      Coef<double> c;
+      if (var) {
+        c.aw = cl[0][q], c.ae = cl[1][q], c.as = cl[2][q], c.an = cl[3][q];
+        c.ab = cl[4][q], c.at = cl[5][q], c.ap = cl[6][q];
+      } else {
+        c = cell_coef<double>(L, i, j, k);
+      }
+      const double W = v[c0 - 1], E = v[c0 + 1];
+      const double S = v[c0 - GS_P], N = v[c0 + GS_P];
+      const double B = v[c0 - GS_P * GS_P], T = v[c0 + GS_P * GS_P];
+      const double r = rl[q];
+      const double sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * T;
+      v[c0] = (r - sum) / c.ap;
+    }
+    __syncthreads();
+  }
+  // the tile's column (lx, ly) to dst
+  gmem<double> *const out0 = (gmem<double> *)L.phi[dst][0];
+  gmem<double> *const out1 = (gmem<double> *)L.phi[dst][1];
+  if (i < L.ni && j < L.nj)
+    for (int lz = 0; lz < GS_T && t.z0 + lz < L.nk; ++lz) {
+      const int k = t.z0 + lz;
+      (((i + j + k) & 1) ? out1 : out0)[cidx(L.H, L.nj, i, j, k)] =
+          v[((lz + 1) * GS_P + ly + 1) * GS_P + lx + 1];
+    }
 }
 
 // positions of one line whose coefficients are loaded together
@@ -2562,10 +2639,10 @@ void launch_lsm_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
                      src, zero);
 }
 
-void launch_gs_plane(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                     const int *active, int d, int src, int zero) {
+void launch_gs_tiles(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
+                     const int *active, int src, int zero) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_gs_plane, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, d, src,
+  hipLaunchKernelGGL(k_gs_tile, dim3(ntiles), dim3(GS_T * GS_T), 0, s, tiles, levs, active, src,
                      zero);
 }
 

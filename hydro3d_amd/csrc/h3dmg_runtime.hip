@@ -57,6 +57,7 @@ struct TileSet {
   const TileSet *nat = nullptr;  // smoother sets: the group's natural tiles (variable density)
   int ndiag[3] = {0, 0, 0};      // GS / TDMA sets: launches per sweep (pass)
   int pass_begin[4] = {0, 0, 0, 0};  // TDMA sets: tiles of pass q at [begin[q], begin[q+1])
+  std::vector<int> diag_begin;       // GS sets: tiles of tile-diagonal D at [begin[D], begin[D+1])
   size_t tdma_lds = 0;  // TDMA sets: > 0 = whole sweeps in one workgroup per block (LDS bytes)
   int n() const { return (int)h.size(); }
 };
@@ -272,9 +273,10 @@ bool is_mixed(const h3dmg_handle *h) { return h->prm.precision == H3DMG_MIXED; }
 template <class T>
 void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool &zero) {
   for (int s = 0; s < nsweep; ++s) {
-    if (h->prm.smoother == H3DMG_SMOOTH_GS)  // hyperplane wavefront (every level FP64)
-      for (int d = 0; d < ts.ndiag[0]; ++d)
-        launch_gs_plane(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, d, buf, zero ? 1 : 0);
+    if (h->prm.smoother == H3DMG_SMOOTH_GS)  // tile wavefront (every level FP64)
+      for (int d = 0; d + 1 < (int)ts.diag_begin.size(); ++d)
+        launch_gs_tiles(h->stream, ts.d + ts.diag_begin[d], ts.diag_begin[d + 1] - ts.diag_begin[d],
+                        h->d_levs, h->d_active, buf, zero ? 1 : 0);
     else if (h->prm.smoother == H3DMG_SMOOTH_TDMA && ts.tdma_lds)  // small levels
       launch_tdma_sweep(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, buf, zero ? 1 : 0,
                         ts.tdma_lds);
@@ -960,18 +962,35 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
     ts.fused = most > 0 && most <= h->fused_max && h->prm.smoother == H3DMG_SMOOTH_RBGS;
     ts.ndiag[0] = ts.ndiag[1] = ts.ndiag[2] = 0;
     if (h->prm.smoother != H3DMG_SMOOTH_RBGS) {
-      // wavefront smoothers: GS (i, j) tiles of 64 x 4 (kind 5); TDMA tiles
-      // of tdma_group_lines(len) lines, pass by pass (kind 6); one plane deep
+      // wavefront smoothers: GS tiles of GS_TILE^3 cells sorted by
+      // tile-diagonal (kind 5); TDMA tiles of tdma_group_lines(len) lines,
+      // pass by pass (kind 6)
       ts.depth = 1;
+      ts.diag_begin.clear();
       if (h->prm.smoother == H3DMG_SMOOTH_GS) {
+        auto tiles_of = [](int n) { return (n + GS_TILE - 1) / GS_TILE; };
+        int nd = 0;
         for (const BlockInfo &B : h->blk)
           if (member(B)) {
-            const int d = B.desc[g];
-            const LevelDev &L = h->levs[d];
-            for (int y = 0; y < L.nj; y += TILE_Y)
-              for (int x = 0; x < L.ni; x += TILE_X) ts.h.push_back(Tile{d, x, y, 0, 1, 5});
-            ts.ndiag[0] = std::max(ts.ndiag[0], L.ni + L.nj + L.nk - 2);
+            const LevelDev &L = h->levs[B.desc[g]];
+            nd = std::max(nd, tiles_of(L.ni) + tiles_of(L.nj) + tiles_of(L.nk) - 2);
           }
+        ts.ndiag[0] = nd;
+        for (int D = 0; D < nd; ++D) {
+          ts.diag_begin.push_back(ts.n());
+          for (const BlockInfo &B : h->blk)
+            if (member(B)) {
+              const int d = B.desc[g];
+              const LevelDev &L = h->levs[d];
+              for (int K = 0; K < tiles_of(L.nk); ++K)
+                for (int J = 0; J < tiles_of(L.nj); ++J) {
+                  const int I = D - K - J;
+                  if (I >= 0 && I < tiles_of(L.ni))
+                    ts.h.push_back(Tile{d, I * GS_TILE, J * GS_TILE, K * GS_TILE, K * GS_TILE + 1, 5});
+                }
+            }
+        }
+        ts.diag_begin.push_back(ts.n());
       } else {
         // small levels: whole sweeps in one workgroup per block
         bool small = true;

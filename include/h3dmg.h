@@ -55,8 +55,8 @@ enum { H3DMG_BC_WALL = 0, H3DMG_BC_PERIODIC = 1 };
 /* smoother (mg_itrsch, src_v70/control.cin line 10; mgrelax,
  * src_v70/mgsolver.for:407-605).  GS and TDMA run in FP64 on the
  * level-by-level path (no mixed precision, LDS tail or fused sweeps): one
- * launch per hyperplane i+j+k (GS) or per line diagonal of each TDMA pass,
- * bitwise the reference's in-place loop order. */
+ * launch per diagonal of 8^3 tiles (GS) or per line diagonal of each TDMA
+ * pass, bitwise the reference's in-place loop order. */
 enum { H3DMG_SMOOTH_GS = 1, H3DMG_SMOOTH_TDMA = 2, H3DMG_SMOOTH_RBGS = 3 };
 
 /* stopping criterion */
