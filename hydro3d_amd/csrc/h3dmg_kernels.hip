@@ -2283,18 +2283,29 @@ __global__ void __launch_bounds__(GS_T * GS_T)
   __syncthreads();
   const int lx = tid % GS_T, ly = tid / GS_T;
   const int i = t.x0 + lx, j = t.y0 + ly;
+  // the level's fields in registers: a barrier would make the compiler
+  // reload them from memory on every plane
+  const int ni = L.ni, nj = L.nj, nk = L.nk;
+  const double cx = L.cx, cy = L.cy, cz = L.cz;
+  const bool px = L.per & 1, py = L.per & 2, pz = L.per & 4;
   // the planes touch LDS only: a barrier then waits for no global store
   for (int h = 0; h < 3 * GS_T - 2; ++h) {
     const int lz = h - lx - ly, k = t.z0 + lz;
-    if (lz >= 0 && lz < GS_T && i < L.ni && j < L.nj && k < L.nk) {
+    if (lz >= 0 && lz < GS_T && i < ni && j < nj && k < nk) {
       const int c0 = ((lz + 1) * GS_P + ly + 1) * GS_P + lx + 1;
       const int q = (lz * GS_T + ly) * GS_T + lx;
       Coef<double> c;
       if (var) {
         c.aw = cl[0][q], c.ae = cl[1][q], c.as = cl[2][q], c.an = cl[3][q];
         c.ab = cl[4][q], c.at = cl[5][q], c.ap = cl[6][q];
-      } else {
-        c = cell_coef<double>(L, i, j, k);
+      } else {  // cell_coef
+        c.aw = (!px && i == 0) ? 0.0 : cx;
+        c.ae = (!px && i == ni - 1) ? 0.0 : cx;
+        c.as = (!py && j == 0) ? 0.0 : cy;
+        c.an = (!py && j == nj - 1) ? 0.0 : cy;
+        c.ab = (!pz && k == 0) ? 0.0 : cz;
+        c.at = (!pz && k == nk - 1) ? 0.0 : cz;
+        c.ap = -1.0 * (c.aw + c.ae + c.as + c.an + c.ab + c.at);
       }
       const double W = v[c0 - 1], E = v[c0 + 1];
       const double S = v[c0 - GS_P], N = v[c0 + GS_P];
@@ -2308,8 +2319,8 @@ __global__ void __launch_bounds__(GS_T * GS_T)
   // the tile's column (lx, ly) to dst
   gmem<double> *const out0 = (gmem<double> *)L.phi[dst][0];
   gmem<double> *const out1 = (gmem<double> *)L.phi[dst][1];
-  if (i < L.ni && j < L.nj)
-    for (int lz = 0; lz < GS_T && t.z0 + lz < L.nk; ++lz) {
+  if (i < ni && j < nj)
+    for (int lz = 0; lz < GS_T && t.z0 + lz < nk; ++lz) {
       const int k = t.z0 + lz;
       (((i + j + k) & 1) ? out1 : out0)[cidx(L.H, L.nj, i, j, k)] =
           v[((lz + 1) * GS_P + ly + 1) * GS_P + lx + 1];
@@ -2426,35 +2437,56 @@ __global__ void __launch_bounds__(256)
 // left-to-right sum, so splitting it off changes no rounding.  Lines of one
 // diagonal never neighbour each other: phase A reads only lines of the
 // diagonals before and after, which nothing writes meanwhile.
-template <int PASS>
-__device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw, int lwp, int d, int src,
-                              int zero, double *xs, double *av, double *rv, int tid) {
+template <int PASS, int CPT>
+__device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw, int lwp, int d,
+                                              int src, int zero, double *xs, double *av,
+                                              double *rv, int tid) {
   const TdmaGeom<PASS> G(L);
   const int nu = G.nu, nv = G.nv, len = G.len;
   const int ulo = max(u0, d - nv + 1), uhi = min(min(u0 + lw, nu), d + 1);
   const int dst = 1 - src;
   const bool tab = L.lsm == nullptr;
-  // interior value of a cross neighbour: updated ones and (after the first
-  // pass) not-yet-updated ones in dst, the first pass's others in src
-  auto cross = [&](int x, int y, int z, bool updated) -> double {
-    if ((unsigned)x >= (unsigned)L.ni || (unsigned)y >= (unsigned)L.nj ||
-        (unsigned)z >= (unsigned)L.nk)
-      return before_sweep(L, src, zero, x, y, z);
-    if (updated || PASS > 0) return cell_at(L, dst, x, y, z);
-    return zero ? 0.0 : cell_at(L, src, x, y, z);
+  const int ni = L.ni, nj = L.nj, nk = L.nk, H = L.H;
+  const bool px = L.per & 1, py = L.per & 2, pz = L.per & 4;
+  // the four phi arrays of the level as byte offsets from src's colour 0
+  const gmem<char> *const s0 = (const gmem<char> *)L.phi[src][0];
+  const long long o_s1 = (const char *)L.phi[src][1] - (const char *)L.phi[src][0];
+  const long long o_d0 = (const char *)L.phi[dst][0] - (const char *)L.phi[src][0];
+  const long long o_d1 = (const char *)L.phi[dst][1] - (const char *)L.phi[src][0];
+  // value of a cross neighbour, with no branch: ghosts are images in src;
+  // interior neighbours updated ones and (after the first pass) not-yet-
+  // updated ones in dst, the first pass's others in src; src reads are 0
+  // when src holds zeros
+  auto cross = [=](int x, int y, int z, bool updated) -> double {
+    const bool ghost = (unsigned)x >= (unsigned)ni || (unsigned)y >= (unsigned)nj ||
+                       (unsigned)z >= (unsigned)nk;
+    x = image(x, ni, px);
+    y = image(y, nj, py);
+    z = image(z, nk, pz);
+    const bool from_dst = !ghost && (updated || PASS > 0);
+    const bool odd = (x + y + z) & 1;
+    const long long o = from_dst ? (odd ? o_d1 : o_d0) : (odd ? o_s1 : 0);
+    const double val = *(const gmem<double> *)(s0 + o + 8 * (((long long)z * nj + y) * H + (x >> 1)));
+    return (zero && !from_dst) ? 0.0 : val;
   };
+  // phase A: a fixed number of cells per thread, every load of them issued
+  // before the first LDS store (positions past the end repeat the last cell
+  // and are not stored)
   const int ncell = (uhi - ulo) * len;
   // q / len through a float reciprocal: exact while q < 2^12 * len
   const float inv_len = 1.0f / (float)len;
-#pragma unroll 4
-  for (int q = tid; q < ncell; q += NTHREADS) {
+  double rsv[CPT], atv[CPT], rtv[CPT];
+  int slot[CPT];
+#pragma unroll
+  for (int it = 0; it < CPT; ++it) {
+    const int q = min(tid + it * NTHREADS, ncell - 1);
     const int lq = (int)(((float)q + 0.5f) * inv_len);
     const int l = ulo - u0 + lq, p = q - lq * len;
     const int u = u0 + l, v = d - u;
     int i, j, k;
     tdma_cell<PASS>(u, v, p, i, j, k);
     const Coef<double> c = smoother_coef(L, i, j, k);
-    const double r = gload(L.rhs[(i + j + k) & 1], cidx(L.H, L.nj, i, j, k));
+    const double r = gload(L.rhs[(i + j + k) & 1], cidx(H, nj, i, j, k));
     double rs;
     if (PASS == 0)
       rs = r - c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true) -
@@ -2465,14 +2497,25 @@ __device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw,
     else
       rs = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
            c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true);
-    xs[p * lwp + l] = rs;
+    rsv[it] = rs;
+    slot[it] = p * lwp + l;
+    atv[it] = rtv[it] = 0.0;
     if (tab) {
       const int var = wall_state(u, nu, G.pu) | (wall_state(v, nv, G.pv) << 2);
       const size_t t = tdma_table_offset<PASS>(L) + (size_t)2 * (var * len + p);
-      av[p * lwp + l] = gload(L.tdma, t);
-      rv[p * lwp + l] = gload(L.tdma, t + 1);
+      atv[it] = gload(L.tdma, t);
+      rtv[it] = gload(L.tdma, t + 1);
     }
   }
+#pragma unroll
+  for (int it = 0; it < CPT; ++it)
+    if (tid + it * NTHREADS < ncell) {
+      xs[slot[it]] = rsv[it];
+      if (tab) {
+        av[slot[it]] = atv[it];
+        rv[slot[it]] = rtv[it];
+      }
+    }
   __syncthreads();
   const int l = tid, u = u0 + l, v = d - u;
   if (l >= lw || u < ulo || u >= uhi) return;
@@ -2485,11 +2528,12 @@ __device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw,
     const double cax = PASS == 0 ? L.cy : PASS == 1 ? L.cx : L.cz;
     const bool pax = PASS == 0 ? (L.per & 2) : PASS == 1 ? (L.per & 1) : (L.per & 4);
     for (int p0 = 0; p0 < len; p0 += TDMA_CHUNK) {
+      // a chunk's operands, read with no branch in between (positions past
+      // the line's end read its last cell and are not used)
       double rs[TDMA_CHUNK], apr[TDMA_CHUNK];
 #pragma unroll
       for (int q = 0; q < TDMA_CHUNK; ++q) {
-        const int p = p0 + q;
-        if (p >= len) break;
+        const int p = min(p0 + q, len - 1);
         rs[q] = xs[p * lwp + l];
         apr[q] = rv[p * lwp + l];
       }
@@ -2508,8 +2552,7 @@ __device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw,
       double rs[TDMA_CHUNK], ap[TDMA_CHUNK], alo[TDMA_CHUNK], ahi[TDMA_CHUNK];
 #pragma unroll
       for (int q = 0; q < TDMA_CHUNK; ++q) {
-        const int p = p0 + q;
-        if (p >= len) break;
+        const int p = min(p0 + q, len - 1);
         tdma_cell<PASS>(u, v, p, i, j, k);
         const Coef<double> c = smoother_coef(L, i, j, k);
         rs[q] = xs[p * lwp + l];
@@ -2536,15 +2579,13 @@ __device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw,
   // registers across the stores.
   gmem<double> *const out0 = (gmem<double> *)L.phi[dst][0];
   gmem<double> *const out1 = (gmem<double> *)L.phi[dst][1];
-  const int H = L.H, nj = L.nj;
   const int base = PASS == 0 ? v * nj * H + (u >> 1) : PASS == 1 ? (v * nj + u) * H : v * H + (u >> 1);
   const int step = PASS == 0 ? H : PASS == 1 ? 0 : nj * H;
   for (int p1 = len - 1; p1 >= 0; p1 -= TDMA_CHUNK) {
     double cv[TDMA_CHUNK], aq[TDMA_CHUNK];
 #pragma unroll
     for (int q = 0; q < TDMA_CHUNK; ++q) {
-      const int p = p1 - q;
-      if (p < 0) break;
+      const int p = max(p1 - q, 0);
       cv[q] = xs[p * lwp + l];
       aq[q] = av[p * lwp + l];
     }
@@ -2575,8 +2616,8 @@ __global__ void __launch_bounds__(NTHREADS)
   const int lw = tdma_group_lines(G.len), lwp = lw > 1 ? lw + 1 : 1;
   const int u0 = t.x0;
   if (max(u0, d - G.nv + 1) >= min(min(u0 + lw, G.nu), d + 1)) return;  // no line of d
-  tdma_diagonal<PASS>(L, u0, lw, lwp, d, src, zero, xs, av, rv,
-                      (int)threadIdx.x + TILE_X * (int)threadIdx.y);
+  tdma_diagonal<PASS, TDMA_CELLS / NTHREADS>(L, u0, lw, lwp, d, src, zero, xs, av, rv,
+                                            (int)threadIdx.x + TILE_X * (int)threadIdx.y);
 }
 
 // a whole TDMA sweep (three passes, every diagonal) of one small level in
@@ -2588,7 +2629,8 @@ __device__ __forceinline__ void tdma_pass_in_group(const LevelDev &L, int src, i
   const int lwp = G.nu + 1;
   const size_t n = (size_t)lwp * G.len;
   for (int d = 0; d < G.nu + G.nv - 1; ++d) {
-    tdma_diagonal<PASS>(L, 0, G.nu, lwp, d, src, zero, buf, buf + n, buf + 2 * n, tid);
+    tdma_diagonal<PASS, TDMA_SWEEP_CELLS / NTHREADS>(L, 0, G.nu, lwp, d, src, zero, buf, buf + n,
+                                                     buf + 2 * n, tid);
     __syncthreads();  // the diagonal's phi before the next one reads it
   }
 }
