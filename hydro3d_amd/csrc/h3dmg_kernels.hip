@@ -2469,53 +2469,55 @@ __device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw,
     const double val = *(const gmem<double> *)(s0 + o + 8 * (((long long)z * nj + y) * H + (x >> 1)));
     return (zero && !from_dst) ? 0.0 : val;
   };
-  // phase A: a fixed number of cells per thread, every load of them issued
-  // before the first LDS store (positions past the end repeat the last cell
+  // phase A: batches of CPT cells per thread, every load of a batch issued
+  // before its first LDS store (positions past the end repeat the last cell
   // and are not stored)
   const int ncell = (uhi - ulo) * len;
   // q / len through a float reciprocal: exact while q < 2^12 * len
   const float inv_len = 1.0f / (float)len;
-  double rsv[CPT], atv[CPT], rtv[CPT];
-  int slot[CPT];
+  for (int q0 = 0; q0 < ncell; q0 += CPT * NTHREADS) {
+    double rsv[CPT], atv[CPT], rtv[CPT];
+    int slot[CPT];
 #pragma unroll
-  for (int it = 0; it < CPT; ++it) {
-    const int q = min(tid + it * NTHREADS, ncell - 1);
-    const int lq = (int)(((float)q + 0.5f) * inv_len);
-    const int l = ulo - u0 + lq, p = q - lq * len;
-    const int u = u0 + l, v = d - u;
-    int i, j, k;
-    tdma_cell<PASS>(u, v, p, i, j, k);
-    const Coef<double> c = smoother_coef(L, i, j, k);
-    const double r = gload(L.rhs[(i + j + k) & 1], cidx(H, nj, i, j, k));
-    double rs;
-    if (PASS == 0)
-      rs = r - c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true) -
-           c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
-    else if (PASS == 1)
-      rs = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
-           c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
-    else
-      rs = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
-           c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true);
-    rsv[it] = rs;
-    slot[it] = p * lwp + l;
-    atv[it] = rtv[it] = 0.0;
-    if (tab) {
-      const int var = wall_state(u, nu, G.pu) | (wall_state(v, nv, G.pv) << 2);
-      const size_t t = tdma_table_offset<PASS>(L) + (size_t)2 * (var * len + p);
-      atv[it] = gload(L.tdma, t);
-      rtv[it] = gload(L.tdma, t + 1);
-    }
-  }
-#pragma unroll
-  for (int it = 0; it < CPT; ++it)
-    if (tid + it * NTHREADS < ncell) {
-      xs[slot[it]] = rsv[it];
+    for (int it = 0; it < CPT; ++it) {
+      const int q = min(q0 + tid + it * NTHREADS, ncell - 1);
+      const int lq = (int)(((float)q + 0.5f) * inv_len);
+      const int l = ulo - u0 + lq, p = q - lq * len;
+      const int u = u0 + l, v = d - u;
+      int i, j, k;
+      tdma_cell<PASS>(u, v, p, i, j, k);
+      const Coef<double> c = smoother_coef(L, i, j, k);
+      const double r = gload(L.rhs[(i + j + k) & 1], cidx(H, nj, i, j, k));
+      double rs;
+      if (PASS == 0)
+        rs = r - c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true) -
+             c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
+      else if (PASS == 1)
+        rs = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
+             c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
+      else
+        rs = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
+             c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true);
+      rsv[it] = rs;
+      slot[it] = p * lwp + l;
+      atv[it] = rtv[it] = 0.0;
       if (tab) {
-        av[slot[it]] = atv[it];
-        rv[slot[it]] = rtv[it];
+        const int var = wall_state(u, nu, G.pu) | (wall_state(v, nv, G.pv) << 2);
+        const size_t t = tdma_table_offset<PASS>(L) + (size_t)2 * (var * len + p);
+        atv[it] = gload(L.tdma, t);
+        rtv[it] = gload(L.tdma, t + 1);
       }
     }
+#pragma unroll
+    for (int it = 0; it < CPT; ++it)
+      if (q0 + tid + it * NTHREADS < ncell) {
+        xs[slot[it]] = rsv[it];
+        if (tab) {
+          av[slot[it]] = atv[it];
+          rv[slot[it]] = rtv[it];
+        }
+      }
+  }
   __syncthreads();
   const int l = tid, u = u0 + l, v = d - u;
   if (l >= lw || u < ulo || u >= uhi) return;
@@ -2629,8 +2631,7 @@ __device__ __forceinline__ void tdma_pass_in_group(const LevelDev &L, int src, i
   const int lwp = G.nu + 1;
   const size_t n = (size_t)lwp * G.len;
   for (int d = 0; d < G.nu + G.nv - 1; ++d) {
-    tdma_diagonal<PASS, TDMA_SWEEP_CELLS / NTHREADS>(L, 0, G.nu, lwp, d, src, zero, buf, buf + n,
-                                                     buf + 2 * n, tid);
+    tdma_diagonal<PASS, 2>(L, 0, G.nu, lwp, d, src, zero, buf, buf + n, buf + 2 * n, tid);
     __syncthreads();  // the diagonal's phi before the next one reads it
   }
 }

@@ -29,7 +29,7 @@ program h3dmg_fortran_demo
   type(c_ptr) :: h
   real(c_double) :: hh, lam, pi, hist(0:60), fin, err, res, fn
   integer(c_int) :: st, ncyc, prec
-  integer :: i, j, k
+  integer :: i, j, k, run
   logical :: ok
 
   pi = acos(-1.0d0)
@@ -55,11 +55,16 @@ program h3dmg_fortran_demo
                H3DMG_BC_PERIODIC, H3DMG_BC_WALL, H3DMG_BC_WALL]
 
   ok = .true.
-  do prec = H3DMG_FP64, H3DMG_MIXED
+  ! red-black in FP64 and mixed, then the reference's other two smoothers
+  ! (mg_itrsch 1 lexicographic GS, 2 line TDMA) in FP64
+  do run = 1, 4
      call h3dmg_default_params(prm)
      prm%tol = 1.0d-10
+     prec = merge(H3DMG_MIXED, H3DMG_FP64, run == 2)
      prm%precision = prec
      if (prec == H3DMG_MIXED) prm%coarse = H3DMG_COARSE_EXACT
+     if (run == 3) prm%smoother = H3DMG_SMOOTH_GS
+     if (run == 4) prm%smoother = H3DMG_SMOOTH_TDMA
      st = h3dmg_create(blk, 1, prm, h)
      call check(st, 'create')
      st = h3dmg_set_rhs(h, 0, f, ld, ld, pl)
@@ -80,8 +85,9 @@ program h3dmg_fortran_demo
      err = sqrt(sum((p(pl + 1:pl + n, pl + 1:pl + n, pl + 1:pl + n) - &
                      pex(pl + 1:pl + n, pl + 1:pl + n, pl + 1:pl + n))**2)) / &
            sqrt(sum(pex(pl + 1:pl + n, pl + 1:pl + n, pl + 1:pl + n)**2))
-     write (*, '(a,i1,a,i3,a,es10.3,a,es10.3,a,es10.3)') ' precision ', prec, ': cycles', &
-          ncyc, ' final ', fin, ' |p-pex|/|pex| ', err, ' |f-Ap|/|f| ', res / fn
+     write (*, '(a,i1,a,i1,a,i3,a,es10.3,a,es10.3,a,es10.3)') ' precision ', prec, &
+          ' smoother ', prm%smoother, ': cycles', ncyc, ' final ', fin, ' |p-pex|/|pex| ', err, &
+          ' |f-Ap|/|f| ', res / fn
      if (err > 1.0d-8 .or. res / fn > 1.0d-9) ok = .false.
   end do
   call velocity_form(ok)

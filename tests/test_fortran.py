@@ -3,7 +3,8 @@
 CPU: the binding and the drop-in newsolv_mg_h3d compile, and the demo
 program runs through the C-ABI far enough to get a clean error status when
 no GPU is present.  GPU: the demo solves a discrete eigenmode (closed-form
-solution) in FP64 and mixed precision through the Fortran interface.
+solution) through the Fortran interface: red-black in FP64 and mixed
+precision, lexicographic GS and line TDMA in FP64.
 """
 import os
 import subprocess
@@ -39,7 +40,7 @@ def test_dropin_shim_compiled_when_reference_modules_exist():
 @pytest.mark.gpu
 @needs_demo
 def test_fortran_demo_solves_eigenmode_on_gpu():
-    print("case: Fortran demo 32^3 PPN eigenmode, fp64 + mixed", flush=True)
+    print("case: Fortran demo 32^3 PPN eigenmode, rbgs fp64 + mixed, gs, tdma", flush=True)
     r = subprocess.run([DEMO], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
