@@ -43,9 +43,11 @@ def config_key(solver, device: str | None = None) -> str:
     p = solver.params
     blocks = sorted((tuple(b["n"]), tuple(int(x) for x in b["bc"]), b["nlev"])
                     for b in solver.blocks)
-    return json.dumps({"device": device or _device_name(), "precision": int(p.precision),
-                       "coarse": int(p.coarse), "nu": [int(p.nu1), int(p.nu2)],
-                       "blocks": blocks}, sort_keys=True)
+    key = {"device": device or _device_name(), "precision": int(p.precision),
+           "coarse": int(p.coarse), "nu": [int(p.nu1), int(p.nu2)], "blocks": blocks}
+    if int(p.smoother) != L.SMOOTH_RBGS:
+        key["smoother"] = int(p.smoother)
+    return json.dumps(key, sort_keys=True)
 
 
 def _device_name() -> str:
@@ -142,8 +144,12 @@ def autotune(solver, cache_path: str | None = DEFAULT_CACHE, reps: int = 5,
     """Measure, apply and cache the fastest march depths and tail level.
 
     The solver's right-hand sides must be set.  Its pressures are saved and
-    restored around the measurement solves."""
+    restored around the measurement solves.  The knobs belong to the
+    red-black smoother's launches: a lexicographic GS or line-TDMA handle
+    has none, and gets an empty choice."""
     import torch
+    if int(solver.params.smoother) != L.SMOOTH_RBGS:
+        return {}
     key = config_key(solver)
     cache = load_cache(cache_path) if cache_path else {}
     if not force and key in cache:

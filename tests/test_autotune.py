@@ -10,7 +10,7 @@ from hydro3d_amd import autotune as AT
 
 
 class _FakeParams:
-    precision, coarse, nu1, nu2 = 1, 1, 2, 2
+    precision, coarse, nu1, nu2, smoother = 1, 1, 2, 2, 3
 
 
 class _FakeSolver:
@@ -28,6 +28,19 @@ def test_config_key_ignores_block_order_and_spacing():
     assert k1 == k2
     assert k1 != AT.config_key(_FakeSolver([a]), device="gfx950")
     assert json.loads(k1)["blocks"][0][0] == [16, 16, 16]
+
+
+def test_line_smoother_handles_have_their_own_key_and_no_knobs():
+    """GS / TDMA handles key separately (red-black keys keep their form) and
+    the autotuner leaves them alone: their launches have no tunable shape."""
+    a = dict(n=(32, 32, 32), h=(0.1,) * 3, bc=(0,) * 6, nlev=3)
+    rb = _FakeSolver([a])
+    tdma = _FakeSolver([a])
+    tdma.params = type("P", (_FakeParams,), {"smoother": 2})()
+    k_rb, k_tdma = AT.config_key(rb, device="gfx950"), AT.config_key(tdma, device="gfx950")
+    assert "smoother" not in json.loads(k_rb)
+    assert json.loads(k_tdma)["smoother"] == 2 and k_rb != k_tdma
+    assert AT.autotune(tdma, cache_path=None) == {}
 
 
 def test_cache_round_trip(tmp_path):
