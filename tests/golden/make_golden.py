@@ -88,8 +88,9 @@ def make_ops(name, n, bc, L, seed):
     np.savez_compressed(os.path.join(HERE, f"ops_{name}.npz"), **out)
 
 
-def make_newsolv(name, cfg, ncycles, keep_p=True):
-    """Velocity-form outer loop of the reference (newsolv_mg.for:17-59)."""
+def make_newsolv(name, cfg, ncycles, keep_p=True, scheme=O.SCHEME_RB):
+    """Velocity-form outer loop of the reference (newsolv_mg.for:17-59),
+    with the smoother mg_itrsch = scheme."""
     n, bc, L = cfg["n"], cfg["bc"], cfg["nlev"]
     B = O.Block(n, cfg["h"], bc, L)
     ni, nj, nk = n
@@ -97,7 +98,7 @@ def make_newsolv(name, cfg, ncycles, keep_p=True):
     nint = ni * nj * nk
     res = O.run_ref(B, "newsolv", [cfg["u"], cfg["v"], cfg["w"], np.zeros((nk, nj, ni))],
                     1 + 2 * (ncycles + 1) + npad + 2 * nint, ncycles=ncycles, dt=cfg["dt"],
-                    eps=1e-6)
+                    eps=1e-6, scheme=scheme)
     q = 0
     stop_at = int(res[q]); q += 1
     hist = res[q:q + 2 * (ncycles + 1)].reshape(ncycles + 1, 2); q += 2 * (ncycles + 1)
@@ -218,6 +219,10 @@ def main():
     # the 64^3 workload itself (4 levels): history and checksums
     make_newsolv("ppw64", C.single_64(), ncycles=30, keep_p=False)
     print("wrote newsolv ppw64")
+    # the loop with the other two smoothers (lexicographic GS, line TDMA)
+    make_newsolv("gs_ppw16", C.cube(16, C.BC_PPN, 3, seed=6), ncycles=10, scheme=O.SCHEME_GS)
+    make_newsolv("tdma_ppw16", C.cube(16, C.BC_PPN, 3, seed=6), ncycles=8, scheme=O.SCHEME_TDMA)
+    print("wrote newsolv gs/tdma")
     # the variable-density (L_LSM) operator
     make_lsm_ops("ppw16", (16, 16, 16), (P, P, P, P, W, W), 3, seed=300)
     make_lsm_ops("wall8x16x8", (8, 16, 8), (W, W, W, W, W, W), 2, seed=301)

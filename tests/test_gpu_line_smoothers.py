@@ -131,3 +131,33 @@ def test_gpu_line_smoothers_refuse_mixed_tail_fused(gpu_lib):
                 S.set_tail(2)
             with pytest.raises(L.H3dError):
                 S.set_fused(1 << 20)
+
+
+@pytest.mark.parametrize("smoother,name", [("gs", "newsolv_gs_ppw16.npz"),
+                                           ("tdma", "newsolv_tdma_ppw16.npz")])
+def test_gpu_line_smoothers_velocity_loop_bitwise_vs_reference(gpu_lib, smoother, name):
+    """newsolv_mg's velocity-form loop on the device (the drop-in's path)
+    with mg_itrsch 1 and 2: from the reference's velocity field and p = 0,
+    the rmax of every iteration, p (ghosts included) and su are bitwise the
+    reference build's; with eps = 1e-6 it stops where the reference stops."""
+    from hydro3d_amd import _lib as L
+    from hydro3d_amd import configs as C
+    z = load_golden(name)
+    c = C.cube(16, C.BC_PPN, 3, seed=6)
+    spec = _blockspec(z)
+    nc = int(z["ncycles"])
+    print(f"case {name}", flush=True)
+    vel = ([_dev(c["u"])], [_dev(c["v"])], [_dev(c["w"])])
+    with _solver(spec, maxcy=nc, tol=0.0, dt=float(z["dt"]), smoother=smoother) as S:
+        S.set_p(0, np.zeros(c["f"].shape))
+        st, ran, hist, _ = S.solve_velocity(*vel)
+        p = S.get_p(0)
+        su = S.get_su(0).cpu().numpy()
+    assert st == L.ERR_NOT_CONVERGED and ran == nc
+    assert np.array_equal(hist, z["rmax"]), (hist, z["rmax"])
+    assert np.array_equal(p, z["p"])
+    assert np.array_equal(su, z["final_su"])
+    with _solver(spec, maxcy=nc, tol=1e-6, dt=float(z["dt"]), smoother=smoother) as S:
+        S.set_p(0, np.zeros(c["f"].shape))
+        st, ran, hist, _ = S.solve_velocity(*vel, raise_on_fail=True)
+    assert ran == int(z["stop_at"]) and np.array_equal(hist, z["rmax"][:ran + 1])

@@ -49,10 +49,14 @@ def test_oracle_matches_reference_ops_bitwise(path):
             assert np.array_equal(out, z[f"corr{g}_out"]), f"corr g{g}"
 
 
-@pytest.mark.parametrize("name", ["newsolv_ppw32.npz", "newsolv_ppw64.npz"])
-def test_pressure_form_tracks_reference_velocity_form(name):
+@pytest.mark.parametrize("name,scheme", [("newsolv_ppw32.npz", O.SCHEME_RB),
+                                         ("newsolv_ppw64.npz", O.SCHEME_RB),
+                                         ("newsolv_gs_ppw16.npz", O.SCHEME_GS),
+                                         ("newsolv_tdma_ppw16.npz", O.SCHEME_TDMA)])
+def test_pressure_form_tracks_reference_velocity_form(name, scheme):
     """The reference's velocity-form loop and the oracle's pressure-form loop
-    agree per cycle to 1e-10 ||f|| and in p to 1e-10 (mean removed)."""
+    agree per cycle to 1e-10 ||f|| and in p to 1e-10 (mean removed), with
+    each of the reference's smoothers."""
     z = load_golden(name)
     B = _block(z)
     nc = int(z["ncycles"])
@@ -63,7 +67,7 @@ def test_pressure_form_tracks_reference_velocity_form(name):
         f = C.divergence_rhs(*C.face_velocity(B.n, B.bc, 0, 0.0, B.h), B.h, float(z["dt"]),
                              remove_mean=False)
         assert abs(np.linalg.norm(f) - float(z["su0_l2"])) <= 1e-12 * float(z["su0_l2"])
-    p, hist = O.solve(B, f, nc, 0.0)
+    p, hist = O.solve(B, f, nc, 0.0, scheme=scheme)
     fn = float(z["su_norm"][0])
     assert abs(hist[0] - fn) <= 1e-12 * fn
     assert np.all(np.abs(hist - z["su_norm"]) <= 1e-10 * fn), np.abs(hist - z["su_norm"]).max() / fn
