@@ -183,9 +183,10 @@ def make_lsm_line_ops(name, n, bc, L, seed):
     np.savez_compressed(os.path.join(HERE, f"lsm_line_{name}.npz"), **out)
 
 
-def make_lsm_newsolv(name, cfg, ncycles):
+def make_lsm_newsolv(name, cfg, ncycles, scheme=O.SCHEME_RB):
     """The velocity-form loop under L_LSM: calvel divides by the face
-    density, calmas writes sup = div(u) dx dy dz / dt (press.for:53-55)."""
+    density, calmas writes sup = div(u) dx dy dz / dt (press.for:53-55);
+    smoother mg_itrsch = scheme."""
     n, bc, L = cfg["n"], cfg["bc"], cfg["nlev"]
     B = O.Block(n, cfg["h"], bc, L)
     ni, nj, nk = n
@@ -194,7 +195,7 @@ def make_lsm_newsolv(name, cfg, ncycles):
     dens = lsm_density(n, 77)
     res = O.run_ref(B, "newsolv", [dens, cfg["u"], cfg["v"], cfg["w"], np.zeros((nk, nj, ni))],
                     1 + 2 * (ncycles + 1) + npad + 2 * nint, ncycles=ncycles, dt=cfg["dt"],
-                    eps=1e-6, lsm=True)
+                    eps=1e-6, lsm=True, scheme=scheme)
     q = 0
     stop_at = int(res[q]); q += 1
     hist = res[q:q + 2 * (ncycles + 1)].reshape(ncycles + 1, 2); q += 2 * (ncycles + 1)
@@ -230,6 +231,9 @@ def main():
     make_lsm_line_ops("wall8x16x12", (8, 16, 12), (W, W, W, W, W, W), 2, seed=311)
     print("wrote lsm ops")
     make_lsm_newsolv("ppw16", C.cube(16, C.BC_PPN, 3, seed=5), ncycles=10)
+    # L_LSM with line TDMA, as the SolitaryWave benchmark runs it
+    make_lsm_newsolv("tdma_ppw16", C.cube(16, C.BC_PPN, 3, seed=5), ncycles=8,
+                     scheme=O.SCHEME_TDMA)
     print("wrote lsm newsolv")
 
 

@@ -120,18 +120,22 @@ def test_gpu_lsm_operators_bitwise_vs_reference(gpu_lib, path):
         assert np.array_equal(S.get_pp(0), z["vcycle_pp"]), "vcycle"
 
 
-def test_gpu_lsm_velocity_loop_bitwise_vs_reference(gpu_lib):
+@pytest.mark.parametrize("name,smoother", [("lsm_newsolv_ppw16.npz", "rbgs"),
+                                           ("lsm_newsolv_tdma_ppw16.npz", "tdma")])
+def test_gpu_lsm_velocity_loop_bitwise_vs_reference(gpu_lib, name, smoother):
     """The velocity-form loop under L_LSM (calvel over the face density,
-    calmas writing sup, the density-weighted V-cycle): rmax of every
+    calmas writing sup, the density-weighted V-cycle), with red-black and
+    with line TDMA (the SolitaryWave benchmark's combination): rmax of every
     iteration, the final pressure and the final sup are bitwise the reference
     build's."""
     from hydro3d_amd import configs as C
     from hydro3d_amd import _lib as L
-    z = load_golden("lsm_newsolv_ppw16.npz")
+    z = load_golden(name)
     c = C.cube(16, C.BC_PPN, 3, seed=5)
     spec = _blockspec(z)
     nc = int(z["ncycles"])
-    with _solver(spec, maxcy=nc, tol=0.0, dt=float(z["dt"])) as S:
+    print(f"case {name}", flush=True)
+    with _solver(spec, maxcy=nc, tol=0.0, dt=float(z["dt"]), smoother=smoother) as S:
         S.set_density(0, z["dens"])
         S.set_p(0, np.zeros(c["f"].shape))
         st, ran, hist, _ = S.solve_velocity([_dev(c["u"])], [_dev(c["v"])], [_dev(c["w"])])
