@@ -98,7 +98,9 @@ def test_gpu_line_smoothers_solve_bitwise_vs_oracle(gpu_lib, smoother, scheme, g
     else:
         specs = [dict(n=(128, 40, 16), h=(1 / 128, 1 / 40, 1 / 16), bc=(1, 1, 0, 0, 1, 1),
                       nlev=3),
-                 dict(n=(32, 32, 32), h=(1 / 32,) * 3, bc=(0, 0, 1, 1, 0, 0), nlev=3)]
+                 dict(n=(32, 32, 32), h=(1 / 32,) * 3, bc=(0, 0, 1, 1, 0, 0), nlev=3),
+                 # a thin block: its coarsest level is one cell deep (k-lines of one cell)
+                 dict(n=(32, 16, 4), h=(1 / 32, 1 / 16, 1 / 4), bc=(0,) * 6, nlev=3)]
     fs = []
     for s in specs:
         f = rng.standard_normal(s["n"][::-1])
