@@ -2164,11 +2164,11 @@ inline dim3 tile_block() { return dim3(TILE_X, TILE_Y, 1); }
 // The reference updates in place, in loop order.  A GS cell (loop k, j, i)
 // reads its W, S and B neighbours after their update and E, N, T before it,
 // so the cells of the hyperplane i+j+k = d depend on plane d-1 (updated) and
-// plane d+1 (not yet) only: one launch per hyperplane performs exactly the
-// reference's arithmetic.  The TDMA passes are line-wise GS over the other
-// two axes: j-lines (loop k, i) on diagonals i+k, i-lines (loop k, j) on
-// j+k, k-lines (loop i, j) on i+j.  A sweep reads buffer src and writes
-// 1-src:
+// plane d+1 (not yet) only: a hyperplane wavefront performs exactly the
+// reference's arithmetic (k_gs_tile runs it over 8^3 tiles).  The TDMA
+// passes are line-wise GS over the other two axes: j-lines (loop k, i) on
+// diagonals i+k, i-lines (loop k, j) on j+k, k-lines (loop i, j) on i+j.  A
+// sweep reads buffer src and writes 1-src:
 //   GS and the j-line pass: values not yet updated from src, updated ones
 //     from 1-src;
 //   i-line and k-line passes: in place in 1-src;
