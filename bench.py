@@ -167,6 +167,20 @@ def batched_blocks(steps, warmup, tune_cache):
             if it >= warmup:
                 lat.append(time.perf_counter() - t0)
         cyc, rel = S.block_stats()
+        pm = [S.get_p(b, out=torch.empty_like(f)) for b, f in enumerate(fs)]
+    # pressure error of every block against its converged FP64 solution
+    # (FP64 with the exact coarsest solve, to 1e-12), mean removed; the worst
+    # block
+    perr = 0.0
+    with MultigridSolver(specs, tol=1e-12, maxcy=200, precision="fp64", coarse="exact") as St:
+        for b, f in enumerate(fs):
+            St.set_rhs(b, f)
+        St.solve(raise_on_fail=True)
+        for b, f in enumerate(fs):
+            pc = St.get_p(b, out=torch.empty_like(f))
+            pc -= pc.mean()
+            d = pm[b] - pm[b].mean() - pc
+            perr = max(perr, float(torch.linalg.norm(d) / torch.linalg.norm(pc)))
     lat = np.array(lat)
     return {"workload": "48 uneven blocks (seed 2), sizes {32,48,64,96,128}^3 per axis, "
                         "Neumann walls, 4 levels, mixed to 1e-8 per block",
@@ -174,7 +188,8 @@ def batched_blocks(steps, warmup, tune_cache):
             "ms_per_batch": round(lat.mean() * 1e3, 3),
             "mcell_solves_per_s": round(cells / lat.mean() / 1e6, 1),
             "cycles_max": int(max(cyc)), "cycles_min": int(min(cyc)),
-            "worst_rel_residual": float(max(rel)), "tuning": choice}
+            "worst_rel_residual": float(max(rel)),
+            "worst_pressure_rel_err_vs_converged_fp64": perr, "tuning": choice}
 
 
 def single_64(steps, warmup):
@@ -188,12 +203,28 @@ def single_64(steps, warmup):
     f = torch.from_numpy(c["f"]).cuda()
     z = torch.zeros_like(f)
     out = {"workload": "single block 64^3 (periodic x/y, Neumann z), 4 levels, seed 0, to 1e-8"}
+    ps = {}
     for name, prec, coarse in (("fp64", "fp64", "smooth"), ("mixed", "mixed", "exact")):
         with MultigridSolver([spec], tol=1e-8, precision=prec, coarse=coarse) as S:
             lat, cyc, fin = timed_solves(S, f, z, steps, warmup)
+            ps[name] = S.get_p(0, out=torch.empty_like(f))
         out[name] = {"ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                      "ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
                      "cycles": cyc, "final_rel_residual": fin}
+    # pressure error against the converged FP64 solution (FP64 with the
+    # exact coarsest solve, to 1e-12), mean removed, and the post-projection
+    # divergence of the mixed pressure
+    with MultigridSolver([spec], tol=1e-12, maxcy=200, precision="fp64", coarse="exact") as St:
+        St.set_rhs(0, f)
+        St.solve(raise_on_fail=True)
+        pc = St.get_p(0, out=torch.empty_like(f))
+    pc -= pc.mean()
+    for name, pt in ps.items():
+        out[name]["pressure_rel_err_vs_converged_fp64"] = float(
+            torch.linalg.norm(pt - pt.mean() - pc) / torch.linalg.norm(pc))
+    rmax, l2 = projected_divergence(c, ps["mixed"].cpu().numpy())
+    out["mixed"]["post_projection_divergence"] = {
+        "rmax": rmax, "rel_l2": l2 / (float(np.linalg.norm(c["f"])) * c["dt"])}
     return out
 
 
