@@ -258,6 +258,33 @@ def line_smoothers(steps, warmup, n=64):
     return out
 
 
+def served_pressure_error(reqs, results, sample=16, seed=5):
+    """Pressure error of a seeded sample of served results against the
+    converged FP64 solution of the same request (FP64 with the exact
+    coarsest solve, to 1e-12), mean removed: the largest per tolerance."""
+    import torch
+    from hydro3d_amd import MultigridSolver
+    from hydro3d_amd import serving as SV
+    rng = np.random.default_rng(seed)
+    by_rid = {r.rid: r for r in results}
+    worst = {}
+    for q in rng.choice(len(reqs), size=min(sample, len(reqs)), replace=False):
+        d = reqs[int(q)]
+        r = by_rid[d["rid"]]
+        req = SV.Request(n=d["n"], h=d["h"], bc=d["bc"], f=None)
+        with MultigridSolver([req.spec()], tol=1e-12, maxcy=200, precision="fp64",
+                             coarse="exact") as St:
+            f = SV.device_rhs(d)
+            St.set_rhs(0, f)
+            St.solve(raise_on_fail=True)
+            pc = St.get_p(0, out=torch.empty_like(f))
+        pc -= pc.mean()
+        e = float(torch.linalg.norm(r.p - r.p.mean() - pc) / torch.linalg.norm(pc))
+        key = f"tol_{d['tol']:.0e}"
+        worst[key] = max(worst.get(key, 0.0), e)
+    return {"sample": int(min(sample, len(reqs))), "max_by_tolerance": worst}
+
+
 def serving(nreq):
     """The serving stream (seeded Poisson arrivals at 200/s, sizes 32..256,
     tolerances 1e-6/1e-8, 20% warm reuse): continuous-batching mixed server
@@ -268,6 +295,8 @@ def serving(nreq):
     with SV.PoissonServer(hbm_budget=64 << 30, max_batches=4) as srv:
         res, wall = SV.run_stream(srv, reqs)
         out["served_mixed"] = SV.summarize(res, wall)
+        out["served_mixed"]["pressure_rel_err_vs_converged_fp64"] = served_pressure_error(
+            reqs, res)
     # the reference schedule (coarsest level smoothed) needs many cycles when
     # the level count is capped by the power of two dividing the size
     with SV.PoissonServer(hbm_budget=64 << 30, max_batches=1, max_batch_blocks=1,
