@@ -2239,34 +2239,47 @@ __global__ void __launch_bounds__(GS_T * GS_T)
   const int dst = 1 - src;
   const int tid = (int)threadIdx.x;
   const bool var = L.lsm != nullptr;
-  // every load of the tile is issued before the first LDS store
+  // every load of the tile is issued before the first LDS store, with no
+  // branch between them (clamped positions load an in-range cell whose
+  // value is not stored or never read)
   constexpr int NT = GS_T * GS_T;
   constexpr int NR = GS_T * GS_T * GS_T / NT, NV = (GS_P * GS_P * GS_P + NT - 1) / NT;
+  const int ni0 = L.ni, nj0 = L.nj, nk0 = L.nk, H0 = L.H;
+  const bool px0 = L.per & 1, py0 = L.per & 2, pz0 = L.per & 4;
+  // the level's phi arrays as byte offsets from src's colour 0
+  const gmem<char> *const s0 = (const gmem<char> *)L.phi[src][0];
+  const long long o_s1 = (const char *)L.phi[src][1] - (const char *)L.phi[src][0];
+  const long long o_d0 = (const char *)L.phi[dst][0] - (const char *)L.phi[src][0];
+  const long long o_d1 = (const char *)L.phi[dst][1] - (const char *)L.phi[src][0];
   double rr[NR], vv[NV];
 #pragma unroll
   for (int it = 0; it < NR; ++it) {
     const int q = tid + it * NT;
-    const int x = t.x0 + q % GS_T, y = t.y0 + (q / GS_T) % GS_T, z = t.z0 + q / (GS_T * GS_T);
-    rr[it] = 0.0;
-    if (x < L.ni && y < L.nj && z < L.nk)
-      rr[it] = gload(L.rhs[(x + y + z) & 1], cidx(L.H, L.nj, x, y, z));
+    const int x = min(t.x0 + q % GS_T, ni0 - 1), y = min(t.y0 + (q / GS_T) % GS_T, nj0 - 1);
+    const int z = min(t.z0 + q / (GS_T * GS_T), nk0 - 1);
+    rr[it] = gload(L.rhs[(x + y + z) & 1], cidx(H0, nj0, x, y, z));
   }
 #pragma unroll
   for (int it = 0; it < NV; ++it) {
-    const int q = tid + it * NT;
+    const int q = min(tid + it * NT, GS_P * GS_P * GS_P - 1);
     const int lz = q / (GS_P * GS_P) - 1, ly = (q / GS_P) % GS_P - 1, lx = q % GS_P - 1;
-    const int outside = (lx < 0 || lx >= GS_T) + (ly < 0 || ly >= GS_T) + (lz < 0 || lz >= GS_T);
-    const int x = t.x0 + lx, y = t.y0 + ly, z = t.z0 + lz;
-    vv[it] = 0.0;
-    // edges and corners of the halo: no 7-point stencil reads them
-    if (q >= GS_P * GS_P * GS_P || outside > 1) continue;
-    if ((unsigned)x >= (unsigned)L.ni || (unsigned)y >= (unsigned)L.nj ||
-        (unsigned)z >= (unsigned)L.nk)
-      vv[it] = before_sweep(L, src, zero, x, y, z);
-    else if (lx < 0 || ly < 0 || lz < 0)
-      vv[it] = cell_at(L, dst, x, y, z);
-    else
-      vv[it] = zero ? 0.0 : cell_at(L, src, x, y, z);
+    int x = t.x0 + lx, y = t.y0 + ly, z = t.z0 + lz;
+    const bool ghost = (unsigned)x >= (unsigned)ni0 || (unsigned)y >= (unsigned)nj0 ||
+                       (unsigned)z >= (unsigned)nk0;
+    // positions more than one cell outside the level (halo edges, corners
+    // and the far side of a partial tile) are clamped; their values are
+    // never read
+    x = image(max(min(x, ni0), -1), ni0, px0);
+    y = image(max(min(y, nj0), -1), nj0, py0);
+    z = image(max(min(z, nk0), -1), nk0, pz0);
+    // interior cells of the low-side halo: updated (dst); the tile, its
+    // high-side halo and ghost images: before the sweep (src)
+    const bool from_dst = !ghost && (lx < 0 || ly < 0 || lz < 0);
+    const bool odd = (x + y + z) & 1;
+    const long long o = from_dst ? (odd ? o_d1 : o_d0) : (odd ? o_s1 : 0);
+    const double val =
+        *(const gmem<double> *)(s0 + o + 8 * (((long long)z * nj0 + y) * H0 + (x >> 1)));
+    vv[it] = (zero && !from_dst) ? 0.0 : val;
   }
 #pragma unroll
   for (int it = 0; it < NR; ++it) rl[tid + it * NT] = rr[it];
