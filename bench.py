@@ -379,8 +379,8 @@ def main():
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "hydro3d_amd", "tuned.json"))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the solver is single-GPU (independent blocks shard "
-              "across GPUs without communication, see DESIGN.md); run with --gpus 1",
+        print("bench.py measures one GPU: the solver is single-GPU (independent blocks would "
+              "shard across GPUs without communication); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
