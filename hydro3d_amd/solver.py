@@ -33,6 +33,17 @@ def _is_device_tensor(x) -> bool:
     return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
 
 
+def _vp(t):
+    """Device pointer of tensor t for the library, after torch's current
+    stream on t's device has drained.  The library runs on its own
+    non-blocking stream, which does not wait for torch's: without this, a
+    tensor torch has just computed (or memory torch is still reading) could
+    be touched before torch's kernels on it have finished."""
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
+    return ctypes.c_void_p(t.data_ptr())
+
+
 class MultigridSolver:
     """Independent blocks, each given as dict(n=(ni,nj,nk), h=(dx,dy,dz),
     bc=(w,e,s,n,b,t) with 0 wall / 1 periodic, nlev=levels)."""
@@ -97,7 +108,7 @@ class MultigridSolver:
     def set_rhs(self, b, f):
         if _is_device_tensor(f):
             assert tuple(f.shape) == self._shape(b) and f.is_contiguous()
-            L.check(self.lib.h3dmg_set_rhs_device(self.h, b, ctypes.c_void_p(f.data_ptr())))
+            L.check(self.lib.h3dmg_set_rhs_device(self.h, b, _vp(f)))
             return
         f = np.ascontiguousarray(f, dtype=np.float64)
         assert f.shape == self._shape(b), (f.shape, self._shape(b))
@@ -107,7 +118,7 @@ class MultigridSolver:
     def set_p(self, b, p):
         if _is_device_tensor(p):
             assert tuple(p.shape) == self._shape(b) and p.is_contiguous()
-            L.check(self.lib.h3dmg_set_p_device(self.h, b, ctypes.c_void_p(p.data_ptr())))
+            L.check(self.lib.h3dmg_set_p_device(self.h, b, _vp(p)))
             return
         p = np.ascontiguousarray(p, dtype=np.float64)
         ni, nj, _ = self.blocks[b]["n"]
@@ -121,7 +132,7 @@ class MultigridSolver:
         """Pressure with its one-cell ghost layer, shape (nk+2, nj+2, ni+2);
         with a device tensor `out` (nk, nj, ni): the interior, on the GPU."""
         if out is not None and _is_device_tensor(out):
-            L.check(self.lib.h3dmg_get_p_device(self.h, b, ctypes.c_void_p(out.data_ptr())))
+            L.check(self.lib.h3dmg_get_p_device(self.h, b, _vp(out)))
             return out
         ni, nj, _ = self.blocks[b]["n"]
         p = np.zeros(self._shape(b, 1))
@@ -175,7 +186,7 @@ class MultigridSolver:
         u, v, w = torch.empty_like(ustar), torch.empty_like(vstar), torch.empty_like(wstar)
         su = torch.empty(self._shape(b), dtype=torch.float64, device=ustar.device) if want_su else None
         rmax = ctypes.c_double(0.0)
-        ptr = lambda t: ctypes.c_void_p(t.data_ptr() if t is not None else 0)  # noqa: E731
+        ptr = lambda t: _vp(t) if t is not None else ctypes.c_void_p(0)  # noqa: E731
         L.check(self.lib.h3dmg_project_device(self.h, b, ptr(ustar), ptr(vstar), ptr(wstar),
                                               ptr(u), ptr(v), ptr(w), ptr(su), float(alfapr),
                                               ctypes.byref(rmax)))
@@ -193,6 +204,8 @@ class MultigridSolver:
             for t in x:
                 assert t is None or (_is_device_tensor(t) and t.dtype == torch.float64
                                      and t.is_contiguous())
+        if any(t is not None for x in ins for t in x):
+            torch.cuda.current_stream().synchronize()  # see _vp
         arr = lambda ts: (ctypes.c_void_p * nb)(*[t.data_ptr() if t is not None else 0  # noqa: E731
                                                   for t in ts])
         self._vel_keep = (ins, outs)  # the device arrays must outlive the loop
@@ -254,7 +267,7 @@ class MultigridSolver:
         tensor): the reference's variable-density (L_LSM) operator."""
         if _is_device_tensor(dens):
             assert tuple(dens.shape) == self._shape(b) and dens.is_contiguous()
-            L.check(self.lib.h3dmg_set_density_device(self.h, b, ctypes.c_void_p(dens.data_ptr())))
+            L.check(self.lib.h3dmg_set_density_device(self.h, b, _vp(dens)))
             return
         d = np.ascontiguousarray(dens, dtype=np.float64)
         assert d.shape == self._shape(b)
@@ -282,7 +295,7 @@ class MultigridSolver:
         """su as the block's rhs arrays hold it, a device tensor (nk, nj, ni)."""
         import torch
         out = torch.empty(self._shape(b), dtype=torch.float64, device="cuda")
-        L.check(self.lib.h3dmg_get_su_device(self.h, b, ctypes.c_void_p(out.data_ptr())))
+        L.check(self.lib.h3dmg_get_su_device(self.h, b, _vp(out)))
         return out
 
     def set_block_tol(self, b, tol):
@@ -328,10 +341,10 @@ class MultigridSolver:
 
     # ---- operator seams (device tensors in the oracle's level layout)
     def dbg_put(self, b, level, which, t):
-        L.check(self.lib.h3dmg_dbg_put(self.h, b, level, which, ctypes.c_void_p(t.data_ptr())))
+        L.check(self.lib.h3dmg_dbg_put(self.h, b, level, which, _vp(t)))
 
     def dbg_get(self, b, level, which, t):
-        L.check(self.lib.h3dmg_dbg_get(self.h, b, level, which, ctypes.c_void_p(t.data_ptr())))
+        L.check(self.lib.h3dmg_dbg_get(self.h, b, level, which, _vp(t)))
         return t
 
     def dbg_relax(self, level, nsweep):
