@@ -376,6 +376,8 @@ def main():
                     help="only time the GS / line-TDMA solves (profiling) and print their JSON")
     ap.add_argument("--lines-size", type=int, default=64, choices=(64, 128, 256),
                     help="cube size of the --only-lines workload")
+    ap.add_argument("--les-steps", type=int, default=50,
+                    help="time steps of the 192^3 LES projection workload (0: skip)")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "hydro3d_amd", "tuned.json"))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -455,6 +457,12 @@ def main():
     log("single 64^3")
     small = single_64(args.steps, args.warmup)
     log(small)
+    les = None
+    if args.les_steps > 0:
+        log("192^3 LES projection")
+        from hydro3d_amd import les as LES
+        les = LES.run(steps=args.les_steps)
+        log(les)
     lines = None
     if not args.no_lines:
         log("line smoothers 64^3")
@@ -523,6 +531,7 @@ def main():
         "cpu_baseline": cpu,
         "single_64": small,
         "line_smoothers_64": lines,
+        "les_192_projection": les,
         "batched_uneven_48": batched,
         "serving": serve,
     }
