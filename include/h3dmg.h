@@ -244,7 +244,11 @@ h3dmg_status h3dmg_set_block_enabled(h3dmg_handle *h, int b, int enabled);
 /* Remove the mean of p over the interior of every block (FP64). */
 h3dmg_status h3dmg_remove_mean(h3dmg_handle *h);
 
-/* Stream the handle launches on (a hipStream_t); null = the handle's own. */
+/* Stream the handle launches on (a hipStream_t); null = the handle's own, a
+ * non-blocking stream that does not wait for work on other streams.  Device
+ * pointers passed to the *_device entry points must be ready on the handle's
+ * stream when the call is made: synchronize the producer's stream first, or
+ * hand the producer's stream to h3dmg_set_stream. */
 h3dmg_status h3dmg_set_stream(h3dmg_handle *h, void *stream);
 
 /* ---- tuning (used by the autotuner, hydro3d_amd/autotune.py) -------------
