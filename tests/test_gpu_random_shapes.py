@@ -50,3 +50,47 @@ def test_gpu_vcycle_random_shapes_bitwise_vs_oracle(gpu_lib, smoother, scheme, s
         B = O.Block(s["n"], s["h"], s["bc"], s["nlev"])
         ref = O.vcycle(B, su, scheme=scheme)
         assert np.array_equal(got[b], ref), f"block {b} {s}"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_wide_rows_solve_vs_oracle(gpu_lib, precision):
+    """Rows of 128 colour entries or more (ni >= 256): several residual and
+    smoother tiles per row.  Blocks 256 wide, 272 wide (a ragged last tile of
+    8 entries) and 512 wide, periodic and wall x faces, in one handle: FP64 is bitwise
+    the oracle's solve of each block (p, every cycle's residual to 1e-12 --
+    the norm's partial sums follow the tiles); the mixed solution's residual
+    under the oracle's operator is below 1e-9 of ||f||."""
+    from hydro3d_amd import MultigridSolver
+    from oracle import oracle as O
+    h = (1 / 64,) * 3
+    specs = [dict(n=(256, 24, 16), h=h, bc=(1, 1, 0, 0, 0, 0), nlev=3),
+             dict(n=(272, 16, 8), h=h, bc=(0, 0, 1, 1, 0, 0), nlev=2),
+             dict(n=(512, 8, 8), h=h, bc=(0, 0, 1, 1, 1, 1), nlev=2)]
+    rng = np.random.default_rng(21)
+    fs = []
+    for s in specs:
+        f = rng.standard_normal(s["n"][::-1])
+        fs.append(f - f.mean())
+    nc = 5 if precision == "fp64" else 40
+    kw = dict(maxcy=nc, tol=0.0) if precision == "fp64" else dict(
+        maxcy=nc, tol=1e-10, precision="mixed", coarse="exact")
+    print(f"case wide rows {precision}", flush=True)
+    with MultigridSolver(specs, **kw) as S:
+        for b, f in enumerate(fs):
+            S.set_rhs(b, f)
+        _, _, hist = S.solve()
+        got = [S.get_p(b) for b in range(len(specs))]
+    for b, (s, f) in enumerate(zip(specs, fs)):
+        B = O.Block(s["n"], s["h"], s["bc"], s["nlev"])
+        if precision == "fp64":
+            po, ho = O.solve(B, f, nc, 0.0)
+            assert np.array_equal(got[b][1:-1, 1:-1, 1:-1], po[1:-1, 1:-1, 1:-1]), f"block {b}"
+        else:
+            # the oracle's operator on the device's pressure (ghosts included)
+            rel = np.linalg.norm(f - O.apply(B, got[b])) / np.linalg.norm(f)
+            assert rel < 1e-9, (b, rel)
+    if precision == "fp64":
+        # the handle's history is the largest relative residual over blocks
+        ref = np.max([O.solve(O.Block(s["n"], s["h"], s["bc"], s["nlev"]), f, nc, 0.0)[1] /
+                      np.linalg.norm(f) for s, f in zip(specs, fs)], axis=0)  # relative
+        np.testing.assert_allclose(hist, ref, rtol=1e-12)
