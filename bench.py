@@ -37,6 +37,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def aux(name, fn):
+    """Run one of the secondary workloads.  A failure (an exception such as a
+    solve that did not converge) is printed on stderr and reported as that
+    workload's entry, {"error": ...}, so the headline line is still printed."""
+    log(name)
+    try:
+        r = fn()
+    except Exception as e:  # noqa: BLE001 -- reported, not swallowed
+        import traceback
+        traceback.print_exc(file=sys.stderr)
+        r = {"error": f"{type(e).__name__}: {e}"}
+    log(r)
+    return r
+
+
 def timed_solves(S, f_dev, zero_dev, steps, warmup):
     import torch
     S.set_rhs(0, f_dev)
@@ -454,37 +469,28 @@ def main():
     ms64, bytes64, ach64 = smoother_roofline(S64, N, 8)
     log(f"smoother pass (fp64, 256^3): {ms64*1e3:.1f} us, {ach64:.0f} GB/s")
 
-    log("single 64^3")
-    small = single_64(args.steps, args.warmup)
-    log(small)
+    small = aux("single 64^3", lambda: single_64(args.steps, args.warmup))
     les = None
     if args.les_steps > 0:
-        log("192^3 LES projection")
         from hydro3d_amd import les as LES
-        les = LES.run(steps=args.les_steps)
-        log(les)
+        les = aux("192^3 LES projection", lambda: LES.run(steps=args.les_steps))
     lines = None
     if not args.no_lines:
-        log("line smoothers 64^3")
-        lines = line_smoothers(max(2, args.steps // 4), 1)
-        log(lines)
+        lines = aux("line smoothers 64^3", lambda: line_smoothers(max(2, args.steps // 4), 1))
     batched = None
     if not args.no_batched:
-        log("batched 48 blocks")
-        batched = batched_blocks(max(3, args.steps // 2), 1, args.tune_cache)
-        log(batched)
+        batched = aux("batched 48 blocks",
+                      lambda: batched_blocks(max(3, args.steps // 2), 1, args.tune_cache))
     serve = None
     if args.serve_requests > 0:
-        log("serving stream")
-        serve = serving(args.serve_requests)
-        log(serve)
+        serve = aux("serving stream", lambda: serving(args.serve_requests))
 
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(c, cyc64)
         log(cpu)
-        if lines is not None:
+        if lines is not None and "error" not in lines:
             lines["cpu_oracle_vcycle_ms_1core"] = cpu_line_smoothers()
             log(lines["cpu_oracle_vcycle_ms_1core"])
 
