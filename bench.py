@@ -470,10 +470,12 @@ def main():
     log(f"smoother pass (fp64, 256^3): {ms64*1e3:.1f} us, {ach64:.0f} GB/s")
 
     small = aux("single 64^3", lambda: single_64(args.steps, args.warmup))
-    les = None
+    les = les_ref = None
     if args.les_steps > 0:
         from hydro3d_amd import les as LES
-        les = aux("192^3 LES projection", lambda: LES.run(steps=args.les_steps))
+        les = aux("192^3 LES projection, per stage", lambda: LES.run(steps=args.les_steps))
+        les_ref = aux("192^3 LES projection, per step",
+                      lambda: LES.run(steps=args.les_steps, mode="per_step"))
     lines = None
     if not args.no_lines:
         lines = aux("line smoothers 64^3", lambda: line_smoothers(max(2, args.steps // 4), 1))
@@ -538,6 +540,7 @@ def main():
         "single_64": small,
         "line_smoothers_64": lines,
         "les_192_projection": les,
+        "les_192_projection_per_step": les_ref,
         "batched_uneven_48": batched,
         "serving": serve,
     }

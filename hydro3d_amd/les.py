@@ -10,8 +10,13 @@ on the device:
   * 2nd-order central convection in advective form (differencing = 1);
   * diffusion with nu + nu_t, nu_t = (Cs*delta)^2 |S| (Smagorinsky,
     Cs = 0.1 as src_v70/eddyvis_smag.for:23 hard-codes it);
-  * 3-stage SSP Runge-Kutta, each stage followed by a projection
-    (u = u* - dt grad p with A p = div(u*)/dt).
+  * 3-stage SSP Runge-Kutta, in one of two modes:
+    - "per_stage": every stage is followed by a projection (u = u* - dt
+      grad p with A p = div(u*)/dt);
+    - "per_step", the reference's: the intermediate stages only subtract
+      the old pressure's gradient (calvel, src_v70/flosol.for:155-182), and
+      the pressure is solved once, after the last stage (newsolv_mg,
+      flosol.for:211-217).
 
 The reference's own LES (flosol.for and its callees) is not built here, so
 the velocity fields -- and therefore the right-hand sides -- are not the
@@ -55,10 +60,12 @@ class PeriodicLES:
     """State (u, v, w, p) on the device and the stage operators."""
 
     def __init__(self, n=192, nu=1e-4, cs=0.1, dt=1.4e-2, seed=3, nlev=6, tol=1e-8,
-                 precision="mixed"):
+                 precision="mixed", mode="per_stage"):
         import torch
         from .solver import MultigridSolver
-        self.n, self.nu, self.cs, self.dt = n, nu, cs, dt
+        if mode not in ("per_stage", "per_step"):
+            raise ValueError(f"mode {mode!r}: 'per_stage' or 'per_step'")
+        self.n, self.nu, self.cs, self.dt, self.mode = n, nu, cs, dt, mode
         self.h = 2.0 * math.pi / n
         u, v, w = taylor_green(n, seed)
         self.u, self.v, self.w = (torch.from_numpy(a).cuda() for a in (u, v, w))
@@ -141,8 +148,16 @@ class PeriodicLES:
         self.max_rel_div = max(self.max_rel_div, amax / max(float(div.abs().max()), 1e-300))
         return u, v, w
 
+    def old_pressure(self, us, vs, ws, dt):
+        """An intermediate stage of the reference's mode: u = u* - dt grad p
+        with the pressure of the last solve, no solve (calvel)."""
+        h, p = self.h, self.p
+        return us - dt * _d(p, 2, h, 1), vs - dt * _d(p, 1, h, 1), ws - dt * _d(p, 0, h, 1)
+
     def step(self):
-        """One SSP-RK3 step, a projection after every stage.  Stage s builds
+        """One SSP-RK3 step: a projection after every stage ("per_stage"), or
+        the old pressure's gradient after the first two and one projection
+        after the last ("per_step").  Stage s builds
         u* = a_s u0 + b_s (u_prev + dt r) with div u0 = div u_prev = 0, so
         div u* = b_s dt div r: projecting with the stage's effective step
         b_s dt (1, 1/4, 2/3) gives every stage the right-hand side div r and
@@ -151,12 +166,13 @@ class PeriodicLES:
         scale (u = u* - b_s dt grad p with A p = div u* / (b_s dt))."""
         dt = self.dt
         u0, v0, w0 = self.u, self.v, self.w
+        stage = self.project if self.mode == "per_stage" else self.old_pressure
         r = self._rhs(u0, v0, w0)
-        u1, v1, w1 = self.project(u0 + dt * r[0], v0 + dt * r[1], w0 + dt * r[2], dt)
+        u1, v1, w1 = stage(u0 + dt * r[0], v0 + dt * r[1], w0 + dt * r[2], dt)
         r = self._rhs(u1, v1, w1)
-        u2, v2, w2 = self.project(0.75 * u0 + 0.25 * (u1 + dt * r[0]),
-                                  0.75 * v0 + 0.25 * (v1 + dt * r[1]),
-                                  0.75 * w0 + 0.25 * (w1 + dt * r[2]), 0.25 * dt)
+        u2, v2, w2 = stage(0.75 * u0 + 0.25 * (u1 + dt * r[0]),
+                           0.75 * v0 + 0.25 * (v1 + dt * r[1]),
+                           0.75 * w0 + 0.25 * (w1 + dt * r[2]), 0.25 * dt)
         r = self._rhs(u2, v2, w2)
         self.u, self.v, self.w = self.project(
             u0 / 3.0 + 2.0 / 3.0 * (u2 + dt * r[0]), v0 / 3.0 + 2.0 / 3.0 * (v2 + dt * r[1]),
@@ -185,11 +201,13 @@ def run(steps=50, n=192, **kw):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ke = 0.5 * float((les.u ** 2 + les.v ** 2 + les.w ** 2).mean())
+        per = "per stage" if les.mode == "per_stage" else \
+            "per step (after the last stage; the earlier stages use the old pressure)"
         return {"workload": f"{n}^3 periodic Taylor-Green + 5% N(0,1) noise (seed 3), nu=1e-4, "
                             f"Smagorinsky Cs=0.1, SSP-RK3, dt={les.dt}, {steps} steps, "
-                            "a warm-started mixed-precision projection (to 1e-8) per stage; "
+                            f"a warm-started mixed-precision projection (to 1e-8) {per}; "
                             "momentum terms: torch stand-in of the host path",
-                "steps": steps, "solves": les.solves,
+                "mode": les.mode, "steps": steps, "solves": les.solves,
                 "solves_per_s": round(les.solves / les.solve_s, 2),
                 "ms_per_solve": round(les.solve_s / les.solves * 1e3, 3),
                 "cycles_per_solve": round(les.cycles / les.solves, 2),

@@ -11,13 +11,16 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("mode,solves", [("per_stage", 9), ("per_step", 3)])
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_gpu_les_projection_converges_and_is_divergence_free(gpu_lib, precision):
+def test_gpu_les_projection_converges_and_is_divergence_free(gpu_lib, precision, mode, solves):
+    """Three steps: a projection per stage, or one per step after the last
+    stage (the reference's mode)."""
     from hydro3d_amd import les as LES
-    print(f"case les 32^3 {precision}", flush=True)
-    r1 = LES.run(steps=3, n=32, nlev=4, precision=precision)
-    r2 = LES.run(steps=3, n=32, nlev=4, precision=precision)
-    assert r1["solves"] == 9
+    print(f"case les 32^3 {precision} {mode}", flush=True)
+    r1 = LES.run(steps=3, n=32, nlev=4, precision=precision, mode=mode)
+    r2 = LES.run(steps=3, n=32, nlev=4, precision=precision, mode=mode)
+    assert r1["solves"] == solves
     assert r1["max_post_projection_div_rel"] < 1e-6, r1
     assert 0.05 < r1["kinetic_energy"] < 0.2, r1
     for k in ("kinetic_energy", "max_post_projection_div_rel", "cycles_per_solve"):

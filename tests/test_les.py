@@ -39,3 +39,10 @@ def test_taylor_green_noise_is_seeded():
     assert not np.array_equal(a[0], c[0])
     clean = taylor_green(16, seed=3, noise=0.0)
     assert 0.04 < float(np.std(a[2] - clean[2])) < 0.06
+
+
+def test_les_mode_is_checked():
+    import pytest
+    from hydro3d_amd.les import PeriodicLES
+    with pytest.raises(ValueError):
+        PeriodicLES(n=8, mode="per_cycle")
