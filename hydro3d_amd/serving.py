@@ -107,6 +107,7 @@ class _Resident:
     nbytes: int
     busy: bool = False
     last_used: float = field(default_factory=time.perf_counter)
+    specs: list = None        # block specs (known before the handle is built)
 
 
 class PoissonServer:
@@ -152,7 +153,7 @@ class PoissonServer:
             hit = self._by_key.get(req.key) if req.key is not None else None
             if hit is not None:
                 res, b = hit
-                sp = res.solver.blocks[b]
+                sp = res.specs[b]
                 if tuple(sp["n"]) == tuple(req.n) and tuple(sp["bc"]) == tuple(req.bc):
                     self._reuse.append((req, fut, res, b))
                     self._lock.notify_all()
@@ -167,7 +168,8 @@ class PoissonServer:
             self._lock.notify_all()
         self._thread.join()
         for r in self._resident:
-            r.solver.close()
+            if r.solver is not None:
+                r.solver.close()
         self._resident.clear()
         self._by_key.clear()
 
@@ -266,24 +268,46 @@ class PoissonServer:
             specs = trial
         if not batch:
             return None
-        try:
-            solver = self._factory(specs, tol=min(r.tol for r, _ in batch), **self._kw)
-        except Exception as e:  # noqa: BLE001 - device refused: fail this batch
-            for _, fut in batch:
-                fut.set_exception(e)
-            return None
-        res = _Resident(solver=solver, keys=[r.key for r, _ in batch],
-                        nbytes=footprint_bytes(specs, self.precision, self.coarse), busy=True)
+        # The footprint is reserved here, under the lock; the handle itself is
+        # built by the worker (device allocations, tables and uploads take
+        # milliseconds, and building here would hold every other dispatch).
+        res = _Resident(solver=None, keys=[r.key for r, _ in batch],
+                        nbytes=footprint_bytes(specs, self.precision, self.coarse), busy=True,
+                        specs=specs)
         self._resident.append(res)
         for b, (r, _) in enumerate(batch):
             if r.key is not None:
                 self._by_key[r.key] = (res, b)
         self.stats["batches"] += 1
         self.stats["batched_blocks"] += len(batch)
-        return (res, [(r, f, b) for b, (r, f) in enumerate(batch)], False)
+        return (res, [(r, f, b) for b, (r, f) in enumerate(batch)], False,
+                min(r.tol for r, _ in batch))
 
     # -- worker
-    def _run_job(self, res, batch, warm):
+    def _drop(self, res):
+        """Forget resident `res` (lock held)."""
+        if res in self._resident:
+            self._resident.remove(res)
+        for k in res.keys:
+            if k is not None and self._by_key.get(k, (None,))[0] is res:
+                del self._by_key[k]
+
+    def _run_job(self, res, batch, warm, tol=None):
+        if res.solver is None:
+            try:
+                res.solver = self._factory(res.specs, tol=tol, **self._kw)
+            except Exception as e:  # noqa: BLE001 - device refused: fail this batch
+                with self._lock:
+                    self._drop(res)
+                    # requests queued meanwhile for this handle's keys fail too
+                    for x in [x for x in self._reuse if x[2] is res]:
+                        self._reuse.remove(x)
+                        x[1].set_exception(e)
+                    for _, fut, _ in batch:
+                        fut.set_exception(e)
+                    self._running -= 1
+                    self._lock.notify_all()
+                return
         S = res.solver
         try:
             blocks = {b for (_, _, b) in batch}
@@ -311,10 +335,7 @@ class PoissonServer:
                 res.busy = False
                 res.last_used = time.perf_counter()
                 if not self.keep_resident and not any(x[2] is res for x in self._reuse):
-                    self._resident.remove(res)
-                    for k in res.keys:
-                        if k is not None and self._by_key.get(k, (None,))[0] is res:
-                            del self._by_key[k]
+                    self._drop(res)
                     S.close()
                 self._running -= 1
                 self._lock.notify_all()

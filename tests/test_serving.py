@@ -125,6 +125,61 @@ def test_request_larger_than_budget_fails_cleanly():
         srv.close()
 
 
+def test_handle_build_failure_fails_its_batch_and_queued_reuse():
+    """Handles are built by the worker thread (the footprint is reserved
+    first).  A build that raises fails its batch and any request queued for
+    its key meanwhile; the server keeps serving."""
+    gate = threading.Event()
+
+    class SlowFailing(FakeSolver):
+        def __init__(self, specs, **kw):
+            if specs[0]["n"][0] == 48:
+                gate.wait(5)  # let the same key's second request queue up
+                raise RuntimeError("device refused")
+            super().__init__(specs, **kw)
+
+    srv = SV.PoissonServer(solver_factory=SlowFailing, max_batch_blocks=1)
+    try:
+        f1 = srv.submit(_req((48, 16, 16), key="bad", rid=1))
+        time.sleep(0.05)
+        f2 = srv.submit(_req((48, 16, 16), key="bad", rid=2))
+        gate.set()
+        for f in (f1, f2):
+            with pytest.raises(RuntimeError, match="device refused"):
+                f.result(timeout=10)
+        ok = srv.submit(_req((32, 16, 16), key="good", rid=3)).result(timeout=10)
+        assert ok.status == 0 and srv.resident_bytes > 0
+    finally:
+        srv.close()
+    assert "bad" not in srv._by_key
+
+
+def test_handles_are_built_concurrently():
+    """Two new hierarchies whose builds overlap in time: the dispatcher does
+    not hold the lock across a build."""
+    inside, peak = [0], [0]
+    lk = threading.Lock()
+
+    class Slow(FakeSolver):
+        def __init__(self, specs, **kw):
+            with lk:
+                inside[0] += 1
+                peak[0] = max(peak[0], inside[0])
+            time.sleep(0.2)
+            with lk:
+                inside[0] -= 1
+            super().__init__(specs, **kw)
+
+    srv = SV.PoissonServer(solver_factory=Slow, max_batch_blocks=1, max_batches=4)
+    try:
+        futs = [srv.submit(_req((32 + 16 * i, 16, 16), rid=i)) for i in range(3)]
+        for f in futs:
+            f.result(timeout=10)
+    finally:
+        srv.close()
+    assert peak[0] >= 2
+
+
 def test_request_stream_follows_the_recipe():
     reqs = SV.request_stream(400, rate=200.0, seed=4)
     t = np.array([r["t"] for r in reqs])
