@@ -273,17 +273,23 @@ def line_smoothers(steps, warmup, n=64):
     return out
 
 
-def served_pressure_error(reqs, results, sample=16, seed=5):
+def served_sample(nreq, sample=16, seed=5):
+    """Indices of the seeded sample of served results that are checked."""
+    rng = np.random.default_rng(seed)
+    return [int(q) for q in rng.choice(nreq, size=min(sample, nreq), replace=False)]
+
+
+def served_pressure_error(reqs, results):
     """Pressure error of a seeded sample of served results against the
     converged FP64 solution of the same request (FP64 with the exact
     coarsest solve, to 1e-12), mean removed: the largest per tolerance."""
     import torch
     from hydro3d_amd import MultigridSolver
     from hydro3d_amd import serving as SV
-    rng = np.random.default_rng(seed)
     by_rid = {r.rid: r for r in results}
     worst = {}
-    for q in rng.choice(len(reqs), size=min(sample, len(reqs)), replace=False):
+    idx = served_sample(len(reqs))
+    for q in idx:
         d = reqs[int(q)]
         r = by_rid[d["rid"]]
         req = SV.Request(n=d["n"], h=d["h"], bc=d["bc"], f=None)
@@ -297,7 +303,7 @@ def served_pressure_error(reqs, results, sample=16, seed=5):
         e = float(torch.linalg.norm(r.p - r.p.mean() - pc) / torch.linalg.norm(pc))
         key = f"tol_{d['tol']:.0e}"
         worst[key] = max(worst.get(key, 0.0), e)
-    return {"sample": int(min(sample, len(reqs))), "max_by_tolerance": worst}
+    return {"sample": len(idx), "max_by_tolerance": worst}
 
 
 def serving(nreq):
@@ -307,8 +313,11 @@ def serving(nreq):
     from hydro3d_amd import serving as SV
     reqs = SV.request_stream(nreq, 200.0, 4)
     out = {}
+    keep = {reqs[q]["rid"] for q in served_sample(len(reqs))}
+    progress = 1000 if nreq >= 2000 else None
     with SV.PoissonServer(hbm_budget=64 << 30, max_batches=4) as srv:
-        res, wall = SV.run_stream(srv, reqs)
+        srv.warmup()
+        res, wall = SV.run_stream(srv, reqs, keep=keep, progress=progress)
         out["served_mixed"] = SV.summarize(res, wall)
         out["served_mixed"]["pressure_rel_err_vs_converged_fp64"] = served_pressure_error(
             reqs, res)
@@ -317,7 +326,8 @@ def serving(nreq):
     with SV.PoissonServer(hbm_budget=64 << 30, max_batches=1, max_batch_blocks=1,
                           precision="fp64", coarse="smooth", keep_resident=False,
                           maxcy=400) as srv:
-        res, wall = SV.run_stream(srv, reqs)
+        srv.warmup()
+        res, wall = SV.run_stream(srv, reqs, keep=set(), progress=progress)
         out["sequential_fp64"] = SV.summarize(res, wall)
     out["workload"] = f"{nreq} requests of the seeded stream (seed 4, 200/s)"
     return out
