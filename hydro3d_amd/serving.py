@@ -334,7 +334,8 @@ class PoissonServer:
             done = time.perf_counter()
             for req, fut, b in batch:
                 p = S.result_p(b)
-                fut.set_result(Result(rid=req.rid, p=p, status=st, cycles=cyc[b],
+                fut.set_result(Result(rid=req.rid, p=p, status=block_status(st, rel[b], req.tol),
+                                      cycles=cyc[b],
                                       rel_residual=rel[b], latency_s=done - req.t_submit,
                                       batch_blocks=len(batch), warm=warm))
             self.stats["solves"] += len(batch)
@@ -352,6 +353,16 @@ class PoissonServer:
                     S.close()
                 self._running -= 1
                 self._lock.notify_all()
+
+
+def block_status(batch_status, rel, tol):
+    """Status of one block of a multi-block solve: the handle's status
+    speaks for the whole batch (NOT_CONVERGED when any block reached maxcy),
+    so each block is judged by its own final relative residual; a diverged
+    or failed solve keeps the batch's status."""
+    if batch_status not in (0, 5):
+        return int(batch_status)
+    return 0 if math.isfinite(rel) and rel <= tol else 5
 
 
 # ---- the serving workload: seeded request stream --------------------------

@@ -34,8 +34,8 @@ def test_python_binding_covers_every_symbol():
 
 
 def test_struct_layouts_match_header():
-    # h3dmg_block: 3 ints, 3 doubles, int, 6 ints
-    assert ctypes.sizeof(_lib.H3dBlock) == 4 * 3 + 4 + 8 * 3 + 4 + 4 * 6 + 4
+    # h3dmg_block: 3 ints (+pad), 3 doubles, int, 6 ints, 6 ints (+pad)
+    assert ctypes.sizeof(_lib.H3dBlock) == 4 * 3 + 4 + 8 * 3 + 4 + 4 * 6 + 4 * 6 + 4
     # 5 ints + pad, 2 doubles, 4 ints
     assert ctypes.sizeof(_lib.H3dParams) == 24 + 16 + 16
 
@@ -48,14 +48,44 @@ def test_default_params_without_device():
     assert p.tol == 1e-8 and p.precision == _lib.FP64
 
 
-def test_create_rejects_bad_levels_before_touching_the_device():
-    lib = _lib.load()
+def _one_block(ni, nj, nk, nlev, bc=(0, 0, 0, 0, 0, 0)):
     blk = (_lib.H3dBlock * 1)()
-    blk[0].ni, blk[0].nj, blk[0].nk = 12, 8, 8
+    blk[0].ni, blk[0].nj, blk[0].nk = ni, nj, nk
     blk[0].dx = blk[0].dy = blk[0].dz = 1.0
-    blk[0].nlevels = 4
+    blk[0].nlevels = nlev
+    for f in range(6):
+        blk[0].bc[f] = bc[f]
+    return blk
+
+
+def test_create_rejects_bad_levels_before_touching_the_device():
+    """localparameters.for:202-208: every axis divisible by 2^(ngrid-1), or
+    the run stops.  The rule is checked on the host before any device call,
+    so the refusal is ERR_LEVELS here, without a GPU."""
+    lib = _lib.load()
     h = ctypes.c_void_p()
-    st = lib.h3dmg_create(blk, 1, None, ctypes.byref(h))
-    # no device here: either the device probe or the level rule refuses
-    assert st in (_lib.ERR_LEVELS, _lib.ERR_DEVICE)
+    st = lib.h3dmg_create(_one_block(12, 8, 8, 4), 1, None, ctypes.byref(h))
+    assert st == _lib.ERR_LEVELS, lib.h3dmg_last_error()
+    assert b"not divisible by 2^(nlevels-1) = 8" in lib.h3dmg_last_error()
+    assert not h.value
+    # the last axis alone breaks the rule
+    st = lib.h3dmg_create(_one_block(16, 16, 20, 4), 1, None, ctypes.byref(h))
+    assert st == _lib.ERR_LEVELS and not h.value
+
+
+def test_create_rejects_bad_blocks_before_touching_the_device():
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    # a periodic axis needs both faces periodic
+    st = lib.h3dmg_create(_one_block(8, 8, 8, 2, bc=(1, 0, 0, 0, 0, 0)), 1, None,
+                          ctypes.byref(h))
+    assert st == _lib.ERR_ARG and b"periodic on one face" in lib.h3dmg_last_error()
+    st = lib.h3dmg_create(_one_block(8, 8, 8, 0), 1, None, ctypes.byref(h))
+    assert st == _lib.ERR_ARG and b"nlevels" in lib.h3dmg_last_error()
+    # mixed precision needs a coarser level to run in FP32
+    p = _lib.H3dParams()
+    lib.h3dmg_default_params(ctypes.byref(p))
+    p.precision = _lib.MIXED
+    st = lib.h3dmg_create(_one_block(8, 8, 8, 1), 1, ctypes.byref(p), ctypes.byref(h))
+    assert st == _lib.ERR_ARG and b"mixed precision" in lib.h3dmg_last_error()
     assert not h.value

@@ -180,6 +180,51 @@ def test_handles_are_built_concurrently():
     assert peak[0] >= 2
 
 
+def test_block_status_judges_each_block_by_its_own_residual():
+    """A multi-block solve returns NOT_CONVERGED (5) when any block reaches
+    maxcy; blocks that met their own tolerance are still converged."""
+    assert SV.block_status(5, 3e-9, 1e-8) == 0
+    assert SV.block_status(5, 2e-8, 1e-8) == 5
+    assert SV.block_status(0, 1e-9, 1e-8) == 0
+    assert SV.block_status(5, float("nan"), 1e-8) == 5
+    # a diverged or failed solve speaks for every block
+    assert SV.block_status(6, 1e-12, 1e-8) == 6
+    assert SV.block_status(3, 1e-12, 1e-8) == 3
+
+
+def test_served_status_is_per_block():
+    class OneSlow(FakeSolver):
+        def solve(self):
+            super().solve()
+            return 5, 60, None  # the batch as a whole hit maxcy
+
+        def block_stats(self):
+            # block 0 met its tolerance, every other block did not
+            n = len(self.blocks)
+            return [7] + [60] * (n - 1), [self.tol[0] / 2] + [t * 3 for t in self.tol[1:]]
+
+    gate = threading.Event()
+
+    class Gated(OneSlow):
+        def __init__(self, specs, **kw):
+            gate.wait(5)
+            super().__init__(specs, **kw)
+
+    srv = SV.PoissonServer(max_batches=1, solver_factory=Gated)
+    try:
+        f0 = srv.submit(_req((32, 32, 32), rid=0))
+        f1 = srv.submit(_req((32, 32, 32), rid=1))
+        gate.set()
+        r0, r1 = f0.result(timeout=10), f1.result(timeout=10)
+    finally:
+        srv.close()
+    # whichever batch shape the dispatcher chose, each result is judged alone
+    for r in (r0, r1):
+        assert r.status in (0, 5)
+    if r0.batch_blocks == 2:
+        assert (r0.status, r1.status) == (0, 5)
+
+
 def test_request_stream_follows_the_recipe():
     reqs = SV.request_stream(400, rate=200.0, seed=4)
     t = np.array([r["t"] for r in reqs])
@@ -199,8 +244,8 @@ def test_served_results_equal_solving_alone():
     import torch
     from hydro3d_amd import MultigridSolver
     torch.cuda.set_device(0)
-    reqs = SV.request_stream(24, rate=2000.0, seed=11, hi=96)
-    print("case: 24 served requests vs alone", flush=True)
+    reqs = SV.request_stream(120, rate=200.0, seed=11, hi=256)
+    print("case: 120 served requests (sizes 32..256) vs alone", flush=True)
     with SV.PoissonServer(max_batches=3) as srv:
         results, _ = SV.run_stream(srv, reqs)
     by_key = {}
