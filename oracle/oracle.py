@@ -214,3 +214,105 @@ def run_ref(block, mode, inputs, nout, nu1=2, nu2=2, level=1, nsweep=1, ncycles=
     if nout is not None and out.size != nout:
         raise RuntimeError(f"reference driver wrote {out.size} doubles, expected {nout}")
     return out
+
+
+# ---------------------------------------------------------------------------
+# Several blocks on one rank: the reference's own layout files and neighbour
+# search (alloc_dom.for) drive the reference multigrid (_ref/ref_multiblock).
+
+REF_MULTIBLOCK = os.path.join(HERE, "_ref", "ref_multiblock")
+
+
+def multiblock_available() -> bool:
+    return os.path.exists(REF_MULTIBLOCK)
+
+
+def box_layout(xs, ys, zs, gdx):
+    """A box of blocks: per-axis lists of block cell counts (a Cartesian
+    product, x fastest in the domain numbering, as the reference's type-2
+    infodom numbers them, alloc_dom.for:470-481).  Returns a list of domain
+    dicts: id, n (cells), lo/hi coordinates (multiples of gdx)."""
+    doms = []
+    z0 = 0
+    for nz in zs:
+        y0 = 0
+        for ny in ys:
+            x0 = 0
+            for nx in xs:
+                lo = (x0 * gdx[0], y0 * gdx[1], z0 * gdx[2])
+                hi = ((x0 + nx) * gdx[0], (y0 + ny) * gdx[1], (z0 + nz) * gdx[2])
+                doms.append(dict(id=len(doms), n=(nx, ny, nz), lo=lo, hi=hi))
+                x0 += nx
+            y0 += ny
+        z0 += nz
+    return doms
+
+
+def infodom_text(doms):
+    """infodom.cin, type 1 (per-domain coordinates, alloc_dom.for:442-461)."""
+    lines = [" 1  (1)sub-domains description; (2)entire domain description (Lx,Ly,Lz)",
+             f"  {len(doms)} number of domains",
+             "========== Domain description ==============",
+             "0.0  Lx", "0.0  Ly", "0.0  Lz", "  1  nx", "  1  ny", "  1  nz",
+             "== CPU number, Local mesh refinement level,xini,xend,yini,yend,zini,zend=="]
+    for d in doms:
+        c = " ".join(repr(float(v)) for a in range(3) for v in (d["lo"][a], d["hi"][a]))
+        lines.append(f"  {d['id']}  1  {c}")
+    return "\n".join(lines) + "\n"
+
+
+def mdmap_text(ndoms, order=None):
+    """mdmap.cin, type 1, every domain on rank 0 (alloc_dom.for:301-345)."""
+    order = list(range(ndoms)) if order is None else list(order)
+    return (" 1  (1)sub-domains description\n"
+            f"  {ndoms} number of domain\n"
+            "  1 number of processor\n"
+            " ===============================================\n"
+            f"  0  {len(order)}  " + " ".join(str(x) for x in order) + "\n")
+
+
+def run_multiblock(doms, mode, inputs, nout, gdx, bcs, **kw):
+    """Run the multi-block reference driver on the layout `doms` (one rank;
+    blocks in domain order); `bcs` are the six global boundary kinds
+    (BC_WALL / BC_PERIODIC) as control.cin sets them for every block."""
+    return run_multiblock_files(infodom_text(doms), mdmap_text(len(doms)), mode, inputs, nout,
+                                gdx, bcs, **kw)
+
+
+DROPIN_DRIVER = os.path.join(HERE, "_ref", "dropin_driver")
+
+
+def dropin_available() -> bool:
+    return os.path.exists(DROPIN_DRIVER)
+
+
+def run_multiblock_files(infodom, mdmap, mode, inputs, nout, gdx, bcs, nlev=1, nu1=2, nu2=2,
+                         level=1, nsweep=1, ncycles=1, scheme=SCHEME_RB, dt=1e-3, eps=1e-6,
+                         timeout=600, early=0, driver=None):
+    """The same from the texts of infodom.cin and mdmap.cin (one rank).
+    `driver` = DROPIN_DRIVER runs the product's drop-in newsolv_mg on the
+    same module state instead of the reference's loop (mode 'newsolv')."""
+    driver = driver or REF_MULTIBLOCK
+    if not os.path.exists(driver):
+        raise RuntimeError(f"driver not built: {driver}")
+    codes = ",".join(str(_REF_BC[x]) for x in bcs)
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "infodom.cin"), "w") as fh:
+            fh.write(infodom)
+        with open(os.path.join(td, "mdmap.cin"), "w") as fh:
+            fh.write(mdmap)
+        with open(os.path.join(td, "job.nml"), "w") as fh:
+            fh.write("&job\n"
+                     f" mode='{mode}', nlev={nlev}, nu1={nu1}, nu2={nu2}, level={level},\n"
+                     f" nsw={nsweep}, ncycles={ncycles}, bcs={codes},\n"
+                     f" gdx={gdx[0]!r},{gdx[1]!r},{gdx[2]!r}, dtin={dt!r}, epsin={eps!r},\n"
+                     f" scheme={scheme}, early={early}\n/\n")
+        with open(os.path.join(td, "in.bin"), "wb") as fh:
+            for a in inputs:
+                fh.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        subprocess.run([driver, "job.nml", "in.bin", "out.bin"], check=True,
+                       timeout=timeout, cwd=td, stdout=subprocess.DEVNULL)
+        out = np.fromfile(os.path.join(td, "out.bin"), dtype="<f8")
+    if nout is not None and out.size != nout:
+        raise RuntimeError(f"multi-block driver wrote {out.size} doubles, expected {nout}")
+    return out
