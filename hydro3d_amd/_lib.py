@@ -15,7 +15,7 @@ OK = 0
 ERR_ARG, ERR_LEVELS, ERR_DEVICE, ERR_NOMEM = 1, 2, 3, 4
 ERR_NOT_CONVERGED, ERR_DIVERGED, ERR_UNSUPPORTED, ERR_BUSY = 5, 6, 7, 8
 
-BC_WALL, BC_PERIODIC = 0, 1
+BC_WALL, BC_PERIODIC, BC_NEIGHBOUR = 0, 1, 2
 SMOOTH_GS, SMOOTH_TDMA, SMOOTH_RBGS = 1, 2, 3
 CRIT_REL_L2, CRIT_REF_RMAX = 0, 1
 FP64, MIXED = 0, 1
@@ -26,7 +26,7 @@ ARR_PHI, ARR_RHS = 0, 1
 class H3dBlock(ctypes.Structure):
     _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("nk", ctypes.c_int),
                 ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
-                ("nlevels", ctypes.c_int), ("bc", ctypes.c_int * 6)]
+                ("nlevels", ctypes.c_int), ("bc", ctypes.c_int * 6), ("nbr", ctypes.c_int * 6)]
 
 
 class H3dParams(ctypes.Structure):
@@ -69,6 +69,14 @@ SIGNATURES = {
     "h3dmg_set_block_tol": (_i, [_vp, _i, _d]),
     "h3dmg_set_block_enabled": (_i, [_vp, _i, _i]),
     "h3dmg_project_device": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _dp]),
+    "h3dmg_layout_read": (_i, [ctypes.c_char_p, ctypes.c_char_p, _P(_vp)]),
+    "h3dmg_layout_destroy": (None, [_vp]),
+    "h3dmg_layout_error": (ctypes.c_char_p, []),
+    "h3dmg_layout_ndomains": (_i, [_vp]),
+    "h3dmg_layout_nranks": (_i, [_vp]),
+    "h3dmg_layout_domain": (_i, [_vp, _i, _ip, _ip, _dp, _dp]),
+    "h3dmg_layout_neighbours": (_i, [_vp, _i, _ip, _ip, _ip]),
+    "h3dmg_layout_blocks": (_i, [_vp, _i, _dp, _ip, _i, _P(H3dBlock), _i, _ip, _ip]),
     "h3dmg_get_tail": (_i, [_vp]),
     "h3dmg_velocity_begin": (_i, [_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _d,
                                   _dp]),

@@ -20,7 +20,7 @@ LIB = os.path.join(PKG, "libh3dmg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
-HIP_SOURCES = ["h3dmg_kernels.hip", "h3dmg_runtime.hip"]
+HIP_SOURCES = ["h3dmg_kernels.hip", "h3dmg_runtime.hip", "h3dmg_layout.cpp"]
 
 # -ffp-contract=off: the FP64 path must execute the reference's products and
 # sums as written (no FMA contraction) to stay bitwise equal to it.
@@ -43,7 +43,7 @@ def build_lib(verbose=True) -> str:
     os.makedirs(objdir, exist_ok=True)
     objs = []
     for src in HIP_SOURCES:
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c",
               os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
@@ -68,10 +68,10 @@ def build_fortran() -> str | None:
     # multidata): compile-checked where the reference objects were built
     ref_mods = os.path.join(os.path.dirname(PKG), "oracle", "_ref", "obj")
     if os.path.exists(os.path.join(ref_mods, "multidata.mod")):
-        _run([FLANG, "-O2", "-ffp-contract=off", "-fdefault-real-8", "-fdefault-double-8",
-              "-I", ref_mods, "-I", out, "-I/opt/conda/include", "-c",
-              os.path.join(fdir, "newsolv_mg_h3d.f90"), "-o",
-              os.path.join(out, "newsolv_mg_h3d.o")])
+        for name in ("newsolv_mg_h3d", "newsolv_mg"):
+            _run([FLANG, "-O2", "-ffp-contract=off", "-fdefault-real-8", "-fdefault-double-8",
+                  "-I", ref_mods, "-I", out, "-I/opt/conda/include", "-c",
+                  os.path.join(fdir, name + ".f90"), "-o", os.path.join(out, name + ".o")])
     prog = os.path.join(fdir, "h3dmg_fortran_demo.f90")
     if os.path.exists(prog):
         exe = os.path.join(fdir, "h3dmg_fortran_demo")

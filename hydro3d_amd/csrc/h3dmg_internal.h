@@ -42,6 +42,11 @@ struct LevelDev {
   int blk;         // block index
   int finer;       // descriptor index of level g-1 (or -1)
   int coarser;     // descriptor index of level g+1 (or -1)
+  // same-level neighbour across face f (west, east, south, north, bottom,
+  // top): its descriptor index, or -1 (a wall, or a periodic axis that wraps
+  // onto this block: `per`).  A face is a wall when it has no neighbour and
+  // its axis is not periodic.
+  int nbr[6];
   double cx, cy, cz;  // 1/h_g^2 (coeff, mgsolver.for:54-60,185-191)
   void *phi[2][2];    // [buffer][colour]
   void *rhs[2];       // [colour]
@@ -59,6 +64,7 @@ struct LevelDev {
 // Finest-level FP64 state of one block (pressure, right-hand side).
 struct FineDev {
   int ni, nj, nk, H, per, blk;
+  int nbr[6];  // neighbouring BLOCK across face f (the level-1 descriptors), or -1
   int lpr;  // residual lanes per row (one colour-array entry per lane)
   double cx, cy, cz;
   double *p[2];  // [colour]
@@ -100,12 +106,16 @@ void launch_rb_fused(hipStream_t s, const Tile *tiles, int ntiles, const LevelDe
                      const int *active, int src, int zero);
 // restriction into the coarse levels of a tile list whose tiles are all
 // `depth` (1, 2 or 4) coarse planes deep
+// (depth 0: natural tiles of the coarse level and the general kernel, for
+// blocks with neighbours)
 template <class Tf, class Tc>
 void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
                      const int *active, int src_f);
+// gen != 0: coarse ghosts read through the neighbour table (blocks with
+// neighbours)
 template <class Tf, class Tc>
 void launch_prolong(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                    const int *active, int src_f, int src_c);
+                    const int *active, int src_f, int src_c, int gen);
 template <class T>
 void launch_fill_zero(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                       const int *active, int buf);
@@ -194,6 +204,16 @@ void launch_tdma_tables(hipStream_t s, const LevelDev *levs, int ndesc);
 constexpr int CALMAS_MAX_GROUPS = 1024;
 void launch_calvel(hipStream_t s, const FineDev *fd, int axis, const double *us, double *u,
                    double dt_alfa, double h, size_t faces);
+// blocks with neighbours: calvel of block b (fine = every block's
+// descriptor) on the faces the block computes, and the copy of a near face
+// from the far face of the block across it (dims of the face: nu x nv)
+void launch_calvel_nb(hipStream_t s, const FineDev *fine, int b, int axis, const double *us,
+                      double *u, double dt_alfa, double h, size_t faces);
+void launch_face_copy(hipStream_t s, int axis, const double *src, int n_src, double *dst,
+                      int n_dst, int nu, int nv);
+// p of block b with its ghost layer (dense padded, ghosts resolved)
+void launch_unpack_fine_padded(hipStream_t s, const FineDev *fine, int b, double *dst, int ni,
+                               int nj, int nk);
 int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const double *v,
                   const double *w, double *su, int to_fine, double dt, double dx, double dy,
                   double dz, RedPartial *partials, size_t cells);
@@ -207,7 +227,7 @@ template <class T>
 void launch_pack_level(hipStream_t s, const LevelDev *ld, int which, int buf, const double *src,
                        int ni, int nj, int nk);
 template <class T>
-void launch_unpack_level(hipStream_t s, const LevelDev *ld, int which, int buf, double *dst,
+void launch_unpack_level(hipStream_t s, const LevelDev *levs, int d, int which, int buf, double *dst,
                          int ni, int nj, int nk);
 void launch_remove_mean(hipStream_t s, const FineDev *fd, int ni, int nj, int nk, double *scratch);
 

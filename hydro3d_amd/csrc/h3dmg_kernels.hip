@@ -34,51 +34,106 @@ struct Coef {
   T aw, ae, as, an, ab, at, ap;
 };
 
+// Face f (0 west, 1 east, 2 south, 3 north, 4 bottom, 5 top) of a level or
+// finest-level descriptor is a wall when its axis is not periodic and no
+// neighbour block lies across it.
+template <class LD>
+__device__ __forceinline__ bool wall_face(const LD &L, int f) {
+  return !((L.per >> (f >> 1)) & 1) && L.nbr[f] < 0;
+}
+
 // coeff, mgsolver.for:185-210: 1/h^2 towards every face, 0 towards a
-// non-periodic physical face, ap = -1*(sum) summed aw..at left to right.
+// physical (non-periodic, neighbourless) face, ap = -1*(sum) summed aw..at
+// left to right.
 template <class T>
 __device__ __forceinline__ Coef<T> cell_coef(const LevelDev &L, int i, int j, int k) {
   const T cx = (T)L.cx, cy = (T)L.cy, cz = (T)L.cz;
-  const bool px = L.per & 1, py = L.per & 2, pz = L.per & 4;
   Coef<T> c;
-  c.aw = (!px && i == 0) ? T(0) : cx;
-  c.ae = (!px && i == L.ni - 1) ? T(0) : cx;
-  c.as = (!py && j == 0) ? T(0) : cy;
-  c.an = (!py && j == L.nj - 1) ? T(0) : cy;
-  c.ab = (!pz && k == 0) ? T(0) : cz;
-  c.at = (!pz && k == L.nk - 1) ? T(0) : cz;
+  c.aw = (i == 0 && wall_face(L, 0)) ? T(0) : cx;
+  c.ae = (i == L.ni - 1 && wall_face(L, 1)) ? T(0) : cx;
+  c.as = (j == 0 && wall_face(L, 2)) ? T(0) : cy;
+  c.an = (j == L.nj - 1 && wall_face(L, 3)) ? T(0) : cy;
+  c.ab = (k == 0 && wall_face(L, 4)) ? T(0) : cz;
+  c.at = (k == L.nk - 1 && wall_face(L, 5)) ? T(0) : cz;
   c.ap = T(-1) * (c.aw + c.ae + c.as + c.an + c.ab + c.at);
   return c;
 }
 
 __device__ __forceinline__ Coef<double> fine_coef(const FineDev &F, int i, int j, int k) {
-  const bool px = F.per & 1, py = F.per & 2, pz = F.per & 4;
   Coef<double> c;
-  c.aw = (!px && i == 0) ? 0.0 : F.cx;
-  c.ae = (!px && i == F.ni - 1) ? 0.0 : F.cx;
-  c.as = (!py && j == 0) ? 0.0 : F.cy;
-  c.an = (!py && j == F.nj - 1) ? 0.0 : F.cy;
-  c.ab = (!pz && k == 0) ? 0.0 : F.cz;
-  c.at = (!pz && k == F.nk - 1) ? 0.0 : F.cz;
+  c.aw = (i == 0 && wall_face(F, 0)) ? 0.0 : F.cx;
+  c.ae = (i == F.ni - 1 && wall_face(F, 1)) ? 0.0 : F.cx;
+  c.as = (j == 0 && wall_face(F, 2)) ? 0.0 : F.cy;
+  c.an = (j == F.nj - 1 && wall_face(F, 3)) ? 0.0 : F.cy;
+  c.ab = (k == 0 && wall_face(F, 4)) ? 0.0 : F.cz;
+  c.at = (k == F.nk - 1 && wall_face(F, 5)) ? 0.0 : F.cz;
   c.ap = -1.0 * (c.aw + c.ae + c.as + c.an + c.ab + c.at);
   return c;
 }
 
-// value of any cell at padded position (x,y,z) in [-1, n] of a level,
-// buffer `buf`, ghosts resolved to their image
+// The cell whose value a ghost holds after exchangepp (exchangepp.for:44-
+// 1105, then mgbound and exchbc_mgpp, :2830-2832): (x, y, z) is a padded
+// position in [-1, n] of descriptor d of `tab`; on return it is an interior
+// position of the returned descriptor.  Walls mirror first, in the block
+// itself (mgbound runs over whole faces after the face syncs); then along x,
+// y and z in turn a coordinate outside the block moves into the neighbour
+// across that face, or wraps around a periodic axis.  In a box of blocks
+// this is the value the reference's sequence of whole-face copies, corner
+// and edge messages and mirrors leaves in every face, edge and corner ghost.
+template <class LD>
+__device__ __forceinline__ int resolve_cell(const LD *__restrict__ tab, int d, int &x, int &y,
+                                            int &z) {
+  const LD *P = &tab[d];
+  if (x < 0 && wall_face(*P, 0)) x = 0;
+  if (x >= P->ni && wall_face(*P, 1)) x = P->ni - 1;
+  if (y < 0 && wall_face(*P, 2)) y = 0;
+  if (y >= P->nj && wall_face(*P, 3)) y = P->nj - 1;
+  if (z < 0 && wall_face(*P, 4)) z = 0;
+  if (z >= P->nk && wall_face(*P, 5)) z = P->nk - 1;
+  int c[3] = {x, y, z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int n = a == 0 ? P->ni : a == 1 ? P->nj : P->nk;
+    if (c[a] < 0) {
+      if (P->nbr[2 * a] >= 0) {
+        d = P->nbr[2 * a];
+        P = &tab[d];
+        c[a] += a == 0 ? P->ni : a == 1 ? P->nj : P->nk;
+      } else {
+        c[a] = ((P->per >> a) & 1) ? c[a] + n : 0;
+      }
+    } else if (c[a] >= n) {
+      if (P->nbr[2 * a + 1] >= 0) {
+        d = P->nbr[2 * a + 1];
+        P = &tab[d];
+        c[a] -= n;
+      } else {
+        c[a] = ((P->per >> a) & 1) ? c[a] - n : n - 1;
+      }
+    }
+  }
+  x = c[0];
+  y = c[1];
+  z = c[2];
+  return d;
+}
+
+// value of any cell at padded position (x,y,z) in [-1, n] of level
+// descriptor d, buffer `buf`, ghosts resolved to the cell they hold
 template <class T>
-__device__ __forceinline__ T level_value(const LevelDev &L, int buf, int x, int y, int z) {
-  x = image(x, L.ni, L.per & 1);
-  y = image(y, L.nj, L.per & 2);
-  z = image(z, L.nk, L.per & 4);
+__device__ __forceinline__ T level_value(const LevelDev *__restrict__ levs, int d, int buf, int x,
+                                         int y, int z) {
+  d = resolve_cell(levs, d, x, y, z);
+  const LevelDev &L = levs[d];
   const T *a = (const T *)L.phi[buf][(x + y + z) & 1];
   return a[cidx(L.H, L.nj, x, y, z)];
 }
 
-__device__ __forceinline__ double fine_value(const FineDev &F, int x, int y, int z) {
-  x = image(x, F.ni, F.per & 1);
-  y = image(y, F.nj, F.per & 2);
-  z = image(z, F.nk, F.per & 4);
+// the same for the finest-level FP64 pressure of block b
+__device__ __forceinline__ double fine_value(const FineDev *__restrict__ fine, int b, int x, int y,
+                                             int z) {
+  b = resolve_cell(fine, b, x, y, z);
+  const FineDev &F = fine[b];
   return F.p[(x + y + z) & 1][cidx(F.H, F.nj, x, y, z)];
 }
 
@@ -162,8 +217,9 @@ template <class T>
 using gmem = __attribute__((address_space(1))) T;
 
 template <class T, bool INNER>
-__device__ __forceinline__ void march_tile(const Tile &t, const LevelDev &L, int colour, int src,
-                                           int zero) {
+__device__ __forceinline__ void march_tile(const Tile &t, const LevelDev *__restrict__ levs,
+                                           int colour, int src, int zero) {
+  const LevelDev &L = levs[t.lev];
   typedef typename Vec2<T>::type v2;
   const int lpr = L.lpr;
   const int lane = (int)threadIdx.x;
@@ -200,7 +256,7 @@ __device__ __forceinline__ void march_tile(const Tile &t, const LevelDev &L, int
     return ld2(O, (unsigned)k * plane + cbase);
   };
   auto ghost = [&](int x, int y, int z) -> T {
-    return zero_ghost ? T(0) : level_value<T>(L, src, x, y, z);
+    return zero_ghost ? T(0) : level_value<T>(levs, t.lev, src, x, y, z);
   };
 
   int k = t.z0;
@@ -461,8 +517,9 @@ __device__ __forceinline__ void march_tile_fast(const Tile &t, const LevelDev &L
 // the full ghost treatment.  The tile covers 256 rows j = t.x0 + ... of plane
 // k = t.y0; one thread per row does both faces.
 template <class T>
-__device__ __forceinline__ void xface_tile(const Tile &t, const LevelDev &L, int colour, int src,
-                                           int zero) {
+__device__ __forceinline__ void xface_tile(const Tile &t, const LevelDev *__restrict__ levs,
+                                           int colour, int src, int zero) {
+  const LevelDev &L = levs[t.lev];
   const int j = t.x0 + (int)threadIdx.x + TILE_X * (int)threadIdx.y;
   const int k = t.y0;
   if (j >= L.nj || k >= L.nk) return;
@@ -473,7 +530,7 @@ __device__ __forceinline__ void xface_tile(const Tile &t, const LevelDev &L, int
     const bool ghost = (unsigned)x >= (unsigned)L.ni || (unsigned)y >= (unsigned)L.nj ||
                        (unsigned)z >= (unsigned)L.nk;
     if (!ghost) return zero_int ? T(0) : nb_int[cidx(L.H, L.nj, x, y, z)];
-    return zero_ghost ? T(0) : level_value<T>(L, src, x, y, z);
+    return zero_ghost ? T(0) : level_value<T>(levs, t.lev, src, x, y, z);
   };
   for (int side = 0; side < 2; ++side) {
     const int i = side ? L.ni - 1 : 0;
@@ -513,9 +570,9 @@ __global__ void __launch_bounds__(NTHREADS)
          : march_tile_fast<T, KB, false, true>(t, L, colour, src, zero);
   } else if (KB == 1) {
     if (t.kind == 1)
-      march_tile<T, false>(t, L, colour, src, zero);
+      march_tile<T, false>(t, levs, colour, src, zero);
     else
-      xface_tile<T>(t, L, colour, src, zero);
+      xface_tile<T>(t, levs, colour, src, zero);
   }
 }
 
@@ -870,6 +927,43 @@ __global__ void __launch_bounds__(NTHREADS)
     restrict_tile_fast<Tf, Tc, KBC, true>(t, C, F, src_f);
 }
 
+// Restriction for blocks with neighbours: one thread per coarse cell of
+// natural tiles (TILE_X x TILE_Y cells of one coarse plane); the fine
+// residual of each of the eight children, its neighbours read through
+// resolve_cell where they are ghosts, in restrict_tile_fast's arithmetic:
+// r = f - (aw*W + ae*E + as*S + an*N + ab*B + at*T + ap*P), the children
+// summed i fastest, then j, then k, times 0.125 (mgsolver.for:693-756).
+template <class Tf, class Tc>
+__global__ void __launch_bounds__(NTHREADS)
+    k_restrict_gen(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+                   const int *__restrict__ active, int src_f) {
+  const Tile t = tiles[blockIdx.x];
+  const LevelDev &C = levs[t.lev];
+  if (!active[C.blk]) return;
+  const int l = t.x0 + (int)threadIdx.x, jc = t.y0 + (int)threadIdx.y, kc = t.z0;
+  if (l >= C.ni || jc >= C.nj || kc >= C.nk) return;
+  const int fd = C.finer;
+  const LevelDev &F = levs[fd];
+  auto val = [&](int x, int y, int z) -> Tf {
+    if ((unsigned)x < (unsigned)F.ni && (unsigned)y < (unsigned)F.nj &&
+        (unsigned)z < (unsigned)F.nk)
+      return ((const Tf *)F.phi[src_f][(x + y + z) & 1])[cidx(F.H, F.nj, x, y, z)];
+    return level_value<Tf>(levs, fd, src_f, x, y, z);
+  };
+  Tc acc = Tc(0);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int i = 2 * l + (q & 1), j = 2 * jc + ((q >> 1) & 1), k = 2 * kc + (q >> 2);
+    const Coef<Tf> c = cell_coef<Tf>(F, i, j, k);
+    const Tf f = ((const Tf *)F.rhs[(i + j + k) & 1])[cidx(F.H, F.nj, i, j, k)];
+    const Tf r = f - (c.aw * val(i - 1, j, k) + c.ae * val(i + 1, j, k) + c.as * val(i, j - 1, k) +
+                      c.an * val(i, j + 1, k) + c.ab * val(i, j, k - 1) + c.at * val(i, j, k + 1) +
+                      c.ap * val(i, j, k));
+    acc = q == 0 ? (Tc)r : acc + (Tc)r;
+  }
+  ((Tc *)C.rhs[(l + jc + kc) & 1])[cidx(C.H, C.nj, l, jc, kc)] = Tc(0.125) * acc;
+}
+
 // ---------------------------------------------------------------------------
 // mgcorr (mgsolver.for:828-896): fine += trilinear interpolation of coarse.
 // Fine padded index ip = 2*icp + d with icp the coarse padded index; the
@@ -883,7 +977,9 @@ __global__ void __launch_bounds__(NTHREADS)
 // y in {m-1, m}, z in {n-1, n} (interior numbering), which are loaded once.
 // Tile::x0 = first l, y0 = first m (TILE_Y per tile), z0 = n.  The fine
 // level is updated in place (each fine cell by exactly one thread).
-template <class Tf, class Tc>
+// GEN: blocks with neighbours -- the coarse ghost corners (edges and corners
+// included) are read through resolve_cell, possibly in another block.
+template <class Tf, class Tc, bool GEN>
 __global__ void __launch_bounds__(NTHREADS)
     k_prolong(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
               const int *__restrict__ active, int src_f, int src_c) {
@@ -912,9 +1008,13 @@ __global__ void __launch_bounds__(NTHREADS)
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        const unsigned off =
-            ((unsigned)zc[c] * (unsigned)C.nj + (unsigned)yc[b]) * (unsigned)C.H + (unsigned)(xc[a] >> 1);
-        cv[a][b][c] = ((xc[a] + yc[b] + zc[c]) & 1) ? C1[off] : C0[off];
+        if (GEN) {
+          cv[a][b][c] = level_value<Tc>(levs, F.coarser, src_c, l - 1 + a, m - 1 + b, n - 1 + c);
+        } else {
+          const unsigned off = ((unsigned)zc[c] * (unsigned)C.nj + (unsigned)yc[b]) *
+                                   (unsigned)C.H + (unsigned)(xc[a] >> 1);
+          cv[a][b][c] = ((xc[a] + yc[b] + zc[c]) & 1) ? C1[off] : C0[off];
+        }
       }
   gmem<Tf> *F0 = (gmem<Tf> *)F.phi[src_f][0];
   gmem<Tf> *F1 = (gmem<Tf> *)F.phi[src_f][1];
@@ -1001,8 +1101,10 @@ __device__ __forceinline__ void block_reduce(double &ss, double &am) {
 // neighbours inside the block; kind 1: the general path with ghost images.
 // use_p = 0 gives r = f (the norm of the right-hand side).
 template <class T, bool INNER>
-__device__ __forceinline__ void residual_tile(const Tile &t, const FineDev &F, const LevelDev &L,
-                                              int use_p, double &ss, double &am) {
+__device__ __forceinline__ void residual_tile(const Tile &t, const FineDev *__restrict__ fine,
+                                              const LevelDev &L, int use_p, double &ss,
+                                              double &am) {
+  const FineDev &F = fine[t.lev];
   const int lpr = F.lpr;
   const int lane = (int)threadIdx.x;
   const int rl = lane & (lpr - 1);
@@ -1050,30 +1152,30 @@ __device__ __forceinline__ void residual_tile(const Tile &t, const FineDev &F, c
       if (tile_left) lf = ldp(cb, q - 1);
       if (tile_right) rt = ldp(ca, q + 1);
       const int ia = 2 * l, ib = ia + 1;
-      const double Wa = ia == 0 ? fine_value(F, -1, j, k) : lf;
-      const double Ea = ib < ni ? Pb : fine_value(F, ni, j, k);
+      const double Wa = ia == 0 ? fine_value(fine, t.lev, -1, j, k) : lf;
+      const double Ea = ib < ni ? Pb : fine_value(fine, t.lev, ni, j, k);
       const double Wb = Pa;
-      const double Eb = ib + 1 < ni ? rt : fine_value(F, ni, j, k);
+      const double Eb = ib + 1 < ni ? rt : fine_value(fine, t.lev, ni, j, k);
       // y- and z-neighbours of a (colour cb there) and b (colour ca)
       double Sa, Na, Sb, Nb;
       if (INNER || j > 0) {
         Sa = ldp(cb, q - row);
         Sb = ldp(ca, q - row);
       } else {
-        Sa = fine_value(F, ia, -1, k);
-        Sb = fine_value(F, ib, -1, k);
+        Sa = fine_value(fine, t.lev, ia, -1, k);
+        Sb = fine_value(fine, t.lev, ib, -1, k);
       }
       if (INNER || j + 1 < nj) {
         Na = ldp(cb, q + row);
         Nb = ldp(ca, q + row);
       } else {
-        Na = fine_value(F, ia, nj, k);
-        Nb = fine_value(F, ib, nj, k);
+        Na = fine_value(fine, t.lev, ia, nj, k);
+        Nb = fine_value(fine, t.lev, ib, nj, k);
       }
-      const double Ba = (INNER || k > 0) ? (cb ? w1m : w0m) : fine_value(F, ia, j, -1);
-      const double Bb = (INNER || k > 0) ? (ca ? w1m : w0m) : fine_value(F, ib, j, -1);
-      const double Ta = (INNER || k + 1 < nk) ? (cb ? w1p : w0p) : fine_value(F, ia, j, nk);
-      const double Tb = (INNER || k + 1 < nk) ? (ca ? w1p : w0p) : fine_value(F, ib, j, nk);
+      const double Ba = (INNER || k > 0) ? (cb ? w1m : w0m) : fine_value(fine, t.lev, ia, j, -1);
+      const double Bb = (INNER || k > 0) ? (ca ? w1m : w0m) : fine_value(fine, t.lev, ib, j, -1);
+      const double Ta = (INNER || k + 1 < nk) ? (cb ? w1p : w0p) : fine_value(fine, t.lev, ia, j, nk);
+      const double Tb = (INNER || k + 1 < nk) ? (ca ? w1p : w0p) : fine_value(fine, t.lev, ib, j, nk);
       Coef<double> c_a, c_b;
       if (INNER) {
         c_a.as = c_a.an = c_b.as = c_b.an = cy;
@@ -1276,9 +1378,9 @@ __global__ void __launch_bounds__(NTHREADS)
   double ss = 0.0, am = 0.0;
   if (active[F.blk]) {
     if (t.kind == 0)
-      residual_tile<T, true>(t, F, L, use_p, ss, am);
+      residual_tile<T, true>(t, fine, L, use_p, ss, am);
     else
-      residual_tile<T, false>(t, F, L, use_p, ss, am);
+      residual_tile<T, false>(t, fine, L, use_p, ss, am);
   }
   block_reduce(ss, am);
   if (threadIdx.x == 0 && threadIdx.y == 0) partials[ti] = RedPartial{ss, am};
@@ -1891,7 +1993,7 @@ __global__ void __launch_bounds__(NTHREADS)
     const bool ghost = (unsigned)x >= (unsigned)L.ni || (unsigned)y >= (unsigned)L.nj ||
                        (unsigned)z >= (unsigned)L.nk;
     if (!ghost) return zero_int ? 0.0 : O[cidx(L.H, L.nj, x, y, z)];
-    return zero_ghost ? 0.0 : level_value<double>(L, src, x, y, z);
+    return zero_ghost ? 0.0 : level_value<double>(levs, t.lev, src, x, y, z);
   };
   const size_t e = cidx(L.H, L.nj, i, j, k);
   double c[7];
@@ -1927,7 +2029,7 @@ __global__ void __launch_bounds__(NTHREADS)
         if (i >= F.ni || j >= F.nj) continue;
         const int c = (i + j + k) & 1;
         const size_t e = cidx(F.H, F.nj, i, j, k);
-        auto nb = [&](int x, int y, int z) -> double { return fine_value(F, x, y, z); };
+        auto nb = [&](int x, int y, int z) -> double { return fine_value(fine, t.lev, x, y, z); };
         double cf[7];
         for (int a = 0; a < 7; ++a) cf[a] = lsm_coef(L, a, c)[e];
         const double P = F.p[c][e];
@@ -1964,7 +2066,7 @@ __global__ void __launch_bounds__(NTHREADS)
     const size_t e = cidx(F.H, F.nj, i, j, k);
     double cf[7];
     for (int a = 0; a < 7; ++a) cf[a] = lsm_coef(F, a, c)[e];
-    auto nb = [&](int x, int y, int z) { return level_value<double>(F, src_f, x, y, z); };
+    auto nb = [&](int x, int y, int z) { return level_value<double>(levs, C.finer, src_f, x, y, z); };
     return ((const double *)F.rhs[c])[e] -
            (cf[0] * nb(i - 1, j, k) + cf[1] * nb(i + 1, j, k) + cf[2] * nb(i, j - 1, k) +
             cf[3] * nb(i, j + 1, k) + cf[4] * nb(i, j, k - 1) + cf[5] * nb(i, j, k + 1) +
@@ -2033,6 +2135,68 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// calvel for blocks with neighbours (the same update): face f of `axis` is
+// computed for 0 < f < n and, when a neighbour or a periodic wrap lies
+// across the block's far face, for f = n (the reference's isu..ieu,
+// localparameters.for:106-192); the near face f = 0 is not written here --
+// it is the neighbour's far face, which exchange(1..3) copies
+// (k_face_copy); wall-normal faces keep u*.  Pressure ghosts are read
+// through resolve_cell (pbound/exchange(4)).
+__global__ void __launch_bounds__(256)
+    k_calvel_nb(const FineDev *__restrict__ fine, int b, int axis, const double *__restrict__ us,
+                double *__restrict__ u, double dt_alfa, double h) {
+  const FineDev &F = fine[b];
+  const int n[3] = {F.ni, F.nj, F.nk};
+  int dim[3] = {F.ni, F.nj, F.nk};
+  dim[axis] += 1;
+  const size_t total = (size_t)dim[0] * dim[1] * dim[2];
+  const bool far_open = !wall_face(F, 2 * axis + 1), near_open = !wall_face(F, 2 * axis);
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    int c[3];
+    c[0] = (int)(q % dim[0]);
+    c[1] = (int)((q / dim[0]) % dim[1]);
+    c[2] = (int)(q / ((size_t)dim[0] * dim[1]));
+    const int f = c[axis];
+    if (f == 0 && near_open) continue;
+    const bool upd = (f > 0 && f < n[axis]) || (f == n[axis] && far_open);
+    double v = us[q];
+    if (upd) {
+      int lo[3] = {c[0], c[1], c[2]};
+      lo[axis] = f - 1;
+      const double ph = fine_value(fine, b, c[0], c[1], c[2]);
+      const double pl = fine_value(fine, b, lo[0], lo[1], lo[2]);
+      v = v - dt_alfa * (ph - pl) / h;
+    }
+    u[q] = v;
+  }
+}
+
+// exchange(1..3) of one face: the near face (f = 0) of `axis` in dst takes
+// the far face (f = n_src) of the block across it (a neighbour, or the block
+// itself on a periodic axis); both blocks share the face's extent (nu, nv =
+// the two other axes' cell counts, in dst's order).
+__global__ void __launch_bounds__(256)
+    k_face_copy(int axis, const double *__restrict__ src, int n_src, double *__restrict__ dst,
+                int n_dst, int nu, int nv) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)nu * nv) return;
+  const size_t a = q % nu, c = q / nu;  // u faster than v
+  // face arrays are (nk, nj, ni) with the face axis extended by one
+  size_t qs, qd;
+  if (axis == 0) {  // (c = k, a = j): u[(k*nj + j)*(ni+1) + f]
+    qs = (c * nu + a) * (size_t)(n_src + 1) + n_src;
+    qd = (c * nu + a) * (size_t)(n_dst + 1);
+  } else if (axis == 1) {  // (c = k, a = i): v[(k*(nj+1) + f)*ni + i]
+    qs = (c * (size_t)(n_src + 1) + n_src) * nu + a;
+    qd = (c * (size_t)(n_dst + 1)) * nu + a;
+  } else {  // (c = j, a = i): w[(f*nj + j)*ni + i]
+    qs = ((size_t)n_src * nv + c) * nu + a;
+    qd = c * nu + a;
+  }
+  dst[qd] = src[qs];
+}
+
 // calmas: su = ((du/dx + dv/dy) + dw/dz)/dt per cell, and per workgroup the
 // max of |fact*dx*dy*dz| (rmax before its MPI_ALLREDUCE, press.for:59-73).
 // su goes to the dense array `su` (when not null) or, with to_fine, into the
@@ -2091,6 +2255,18 @@ __global__ void k_unpack_fine(const FineDev *__restrict__ fd, int which, double 
   dst[q] = src[cidx(F.H, F.nj, i, j, k)];
 }
 
+// p of block b with its one-cell ghost layer, dense (nk+2, nj+2, ni+2), i
+// fastest; ghosts are the cells resolve_cell names (pbound / exchange(4))
+__global__ void k_unpack_fine_padded(const FineDev *__restrict__ fine, int b,
+                                     double *__restrict__ dst) {
+  const FineDev &F = fine[b];
+  const int pi = F.ni + 2, pj = F.nj + 2, pk = F.nk + 2;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)pi * pj * pk) return;
+  const int x = (int)(q % pi) - 1, y = (int)((q / pi) % pj) - 1, z = (int)(q / ((size_t)pi * pj)) - 1;
+  dst[q] = fine_value(fine, b, x, y, z);
+}
+
 // which 0: phi (dense padded (n+2)^3 <-> buffer buf), 1: rhs (dense interior)
 template <class T>
 __global__ void k_pack_level(const LevelDev *__restrict__ ld, int which, int buf,
@@ -2111,16 +2287,16 @@ __global__ void k_pack_level(const LevelDev *__restrict__ ld, int which, int buf
 }
 
 template <class T>
-__global__ void k_unpack_level(const LevelDev *__restrict__ ld, int which, int buf,
+__global__ void k_unpack_level(const LevelDev *__restrict__ levs, int d, int which, int buf,
                                double *__restrict__ dst, int ni, int nj, int nk) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const LevelDev &L = *ld;
+  const LevelDev &L = levs[d];
   if (which == 0) {
     const int pi = ni + 2, pj = nj + 2, pk = nk + 2;
     const size_t n = (size_t)pi * pj * pk;
     if (q >= n) return;
     const int x = (int)(q % pi) - 1, y = (int)((q / pi) % pj) - 1, z = (int)(q / ((size_t)pi * pj)) - 1;
-    dst[q] = (double)level_value<T>(L, buf, x, y, z);
+    dst[q] = (double)level_value<T>(levs, d, buf, x, y, z);
   } else {
     const size_t n = (size_t)ni * nj * nk;
     if (q >= n) return;
@@ -2753,6 +2929,11 @@ template <class Tf, class Tc>
 void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
                      const int *active, int src_f) {
   if (ntiles <= 0) return;
+  if (depth == 0) {  // natural tiles of the coarse level, blocks with neighbours
+    hipLaunchKernelGGL((k_restrict_gen<Tf, Tc>), dim3(ntiles), tile_block(), 0, s, tiles, levs,
+                       active, src_f);
+    return;
+  }
   auto k = depth == 4   ? k_restrict<Tf, Tc, 4>
            : depth == 2 ? k_restrict<Tf, Tc, 2>
                         : k_restrict<Tf, Tc, 1>;
@@ -2761,10 +2942,14 @@ void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, co
 
 template <class Tf, class Tc>
 void launch_prolong(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                    const int *active, int src_f, int src_c) {
+                    const int *active, int src_f, int src_c, int gen) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((k_prolong<Tf, Tc>), dim3(ntiles), tile_block(), 0, s, tiles, levs, active,
-                     src_f, src_c);
+  if (gen)
+    hipLaunchKernelGGL((k_prolong<Tf, Tc, true>), dim3(ntiles), tile_block(), 0, s, tiles, levs,
+                       active, src_f, src_c);
+  else
+    hipLaunchKernelGGL((k_prolong<Tf, Tc, false>), dim3(ntiles), tile_block(), 0, s, tiles, levs,
+                       active, src_f, src_c);
 }
 
 template <class T>
@@ -2856,6 +3041,26 @@ void launch_calvel(hipStream_t s, const FineDev *fd, int axis, const double *us,
   hipLaunchKernelGGL(k_calvel, dim3(grid), dim3(256), 0, s, fd, axis, us, u, dt_alfa, h);
 }
 
+void launch_calvel_nb(hipStream_t s, const FineDev *fine, int b, int axis, const double *us,
+                      double *u, double dt_alfa, double h, size_t faces) {
+  const unsigned grid = (unsigned)std::min<size_t>((faces + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_calvel_nb, dim3(grid), dim3(256), 0, s, fine, b, axis, us, u, dt_alfa, h);
+}
+
+void launch_face_copy(hipStream_t s, int axis, const double *src, int n_src, double *dst,
+                      int n_dst, int nu, int nv) {
+  const size_t n = (size_t)nu * nv;
+  hipLaunchKernelGGL(k_face_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, axis, src,
+                     n_src, dst, n_dst, nu, nv);
+}
+
+void launch_unpack_fine_padded(hipStream_t s, const FineDev *fine, int b, double *dst, int ni,
+                               int nj, int nk) {
+  const size_t n = (size_t)(ni + 2) * (nj + 2) * (nk + 2);
+  hipLaunchKernelGGL(k_unpack_fine_padded, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     fine, b, dst);
+}
+
 int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const double *v,
                   const double *w, double *su, int to_fine, double dt, double dx, double dy,
                   double dz, RedPartial *partials, size_t cells) {
@@ -2888,10 +3093,10 @@ void launch_pack_level(hipStream_t s, const LevelDev *ld, int which, int buf, co
 }
 
 template <class T>
-void launch_unpack_level(hipStream_t s, const LevelDev *ld, int which, int buf, double *dst,
+void launch_unpack_level(hipStream_t s, const LevelDev *levs, int d, int which, int buf, double *dst,
                          int ni, int nj, int nk) {
   const size_t n = (size_t)(ni + 2) * (nj + 2) * (nk + 2);
-  hipLaunchKernelGGL(k_unpack_level<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ld,
+  hipLaunchKernelGGL(k_unpack_level<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, levs, d,
                      which, buf, dst, ni, nj, nk);
 }
 
@@ -2916,11 +3121,11 @@ template void launch_restrict<float, float>(hipStream_t, const Tile *, int, int,
 template void launch_restrict<float, double>(hipStream_t, const Tile *, int, int, const LevelDev *,
                                              const int *, int);
 template void launch_prolong<double, double>(hipStream_t, const Tile *, int, const LevelDev *,
-                                             const int *, int, int);
+                                             const int *, int, int, int);
 template void launch_prolong<float, float>(hipStream_t, const Tile *, int, const LevelDev *,
-                                           const int *, int, int);
+                                           const int *, int, int, int);
 template void launch_prolong<float, double>(hipStream_t, const Tile *, int, const LevelDev *,
-                                            const int *, int, int);
+                                            const int *, int, int, int);
 template void launch_fill_zero<double>(hipStream_t, const Tile *, int, const LevelDev *,
                                        const int *, int);
 template void launch_fill_zero<float>(hipStream_t, const Tile *, int, const LevelDev *,
@@ -2937,9 +3142,9 @@ template void launch_pack_level<double>(hipStream_t, const LevelDev *, int, int,
                                         int, int, int);
 template void launch_pack_level<float>(hipStream_t, const LevelDev *, int, int, const double *,
                                        int, int, int);
-template void launch_unpack_level<double>(hipStream_t, const LevelDev *, int, int, double *, int,
+template void launch_unpack_level<double>(hipStream_t, const LevelDev *, int, int, int, double *, int,
                                           int, int);
-template void launch_unpack_level<float>(hipStream_t, const LevelDev *, int, int, double *, int,
+template void launch_unpack_level<float>(hipStream_t, const LevelDev *, int, int, int, double *, int,
                                          int, int);
 
 }  // namespace h3dmg

@@ -104,11 +104,12 @@ int march_tiles_xy(const LevelDev &L) {
   return ((L.H + xs - 1) / xs) * ((L.nj + ys - 1) / ys);
 }
 
-// kb planes per tile; kb divides L.nk
-void add_tiles_march(TileSet &ts, int lev, const LevelDev &L, int kb) {
+// kb planes per tile; kb divides L.nk; `general`: kinds 1 and 2 (every
+// ghost through the neighbour table: blocks with neighbours)
+void add_tiles_march(TileSet &ts, int lev, const LevelDev &L, int kb, bool general) {
   const int xs = 2 * L.lpr, ys = TILE_Y * (64 / L.lpr);
   const int nx = L.H, ny = L.nj, nz = L.nk;
-  const bool faces_in_tiles = L.ni >= 2;
+  const bool faces_in_tiles = L.ni >= 2 && !general;
   for (int z = 0; z < nz; z += kb) {
     const int z1 = std::min(z + kb, nz);
     if (!faces_in_tiles)
@@ -186,6 +187,10 @@ struct BlockInfo {
 
 struct h3dmg_handle {
   h3dmg_params prm{};
+  int device = 0;  // the HIP device the handle's memory lives on
+  // blocks joined by shared faces (H3DMG_BC_NEIGHBOUR): one domain, every
+  // kernel on its general path (ghosts through the neighbour table)
+  bool connected = false;
   int nb = 0;
   int Lmax = 0;
   std::vector<BlockInfo> blk;
@@ -429,18 +434,18 @@ void enqueue_vcycle(h3dmg_handle *h) {
         }
         if (f64)
           launch_prolong<double, double>(h->stream, h->prol_nc[g].d, h->prol_nc[g].n(), h->d_levs,
-                                         h->d_active, buf_nc[g], buf_nc[g + 1]);
+                                         h->d_active, buf_nc[g], buf_nc[g + 1], h->connected);
         else
           launch_prolong<float, float>(h->stream, h->prol_nc[g].d, h->prol_nc[g].n(), h->d_levs,
-                                       h->d_active, buf_nc[g], buf_nc[g + 1]);
+                                       h->d_active, buf_nc[g], buf_nc[g + 1], h->connected);
       }
       if (h->prol_c[g].n()) {
         if (f64)
           launch_prolong<double, double>(h->stream, h->prol_c[g].d, h->prol_c[g].n(), h->d_levs,
-                                         h->d_active, buf_nc[g], buf_c[g + 1]);
+                                         h->d_active, buf_nc[g], buf_c[g + 1], h->connected);
         else
           launch_prolong<float, double>(h->stream, h->prol_c[g].d, h->prol_c[g].n(), h->d_levs,
-                                        h->d_active, buf_nc[g], buf_c[g + 1]);
+                                        h->d_active, buf_nc[g], buf_c[g + 1], h->connected);
       }
       bool z = false;
       relax_any(h, f64, h->relax_nc[g], nu2, buf_nc[g], z);
@@ -537,6 +542,147 @@ void eig1d(int n, bool periodic, double c, std::vector<double> &Q, std::vector<d
   }
 }
 
+// Host mirror of the device's ghost rule (resolve_cell): the block and
+// interior cell whose value the ghost at padded position c of block b holds
+// after exchangepp, with the axes resolved in the order `ord` (the device
+// and the reference use x, y, z).  Walls mirror first; then each axis moves
+// into the block across the face (H3DMG_BC_NEIGHBOUR) or wraps (periodic).
+int host_resolve(const h3dmg_block *blocks, int b, int c[3], const int ord[3]) {
+  auto n_of = [&](int q, int a) { return a == 0 ? blocks[q].ni : a == 1 ? blocks[q].nj : blocks[q].nk; };
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] < 0 && blocks[b].bc[2 * a] == H3DMG_BC_WALL) c[a] = 0;
+    if (c[a] >= n_of(b, a) && blocks[b].bc[2 * a + 1] == H3DMG_BC_WALL) c[a] = n_of(b, a) - 1;
+  }
+  for (int q = 0; q < 3; ++q) {
+    const int a = ord[q];
+    const int n = n_of(b, a);
+    if (c[a] < 0) {
+      const int k = blocks[b].bc[2 * a];
+      if (k == H3DMG_BC_NEIGHBOUR) {
+        b = blocks[b].nbr[2 * a];
+        c[a] += n_of(b, a);
+      } else {
+        c[a] = k == H3DMG_BC_PERIODIC ? c[a] + n : 0;
+      }
+    } else if (c[a] >= n) {
+      const int k = blocks[b].bc[2 * a + 1];
+      if (k == H3DMG_BC_NEIGHBOUR) {
+        b = blocks[b].nbr[2 * a + 1];
+        c[a] -= n;
+      } else {
+        c[a] = k == H3DMG_BC_PERIODIC ? c[a] - n : n - 1;
+      }
+    }
+  }
+  return b;
+}
+
+// Blocks joined by shared faces (H3DMG_BC_NEIGHBOUR): each shared face names
+// its partner on the opposite face of the partner, with the same extent,
+// spacing and level count; and the blocks form a box -- for every block and
+// every edge or corner ghost, resolving the axes in any order reaches the
+// same cell, which holds exactly where each edge or corner between shared
+// faces has its diagonal block (the reference's ghosts there are stale
+// otherwise: exchangepp.for:44-1105 copies whole faces in the order x, y, z).
+h3dmg_status validate_neighbours(const h3dmg_block *blocks, int nblocks, const h3dmg_params &P) {
+  bool any = false;
+  for (int b = 0; b < nblocks; ++b)
+    for (int f = 0; f < 6; ++f) {
+      if (blocks[b].bc[f] != H3DMG_BC_NEIGHBOUR) continue;
+      any = true;
+      const int a = f >> 1, o = f ^ 1;
+      const int q = blocks[b].nbr[f];
+      if (q < 0 || q >= nblocks)
+        return fail(H3DMG_ERR_ARG, "block %d face %d: neighbour %d out of range", b, f, q);
+      if (blocks[q].bc[o] != H3DMG_BC_NEIGHBOUR || blocks[q].nbr[o] != b)
+        return fail(H3DMG_ERR_ARG, "block %d face %d: block %d does not name it back on face %d",
+                    b, f, q, o);
+      if (blocks[b].bc[f ^ 1] == H3DMG_BC_PERIODIC)
+        return fail(H3DMG_ERR_ARG, "block %d: axis %d both periodic onto itself and shared", b, a);
+      const int nb[3] = {blocks[b].ni, blocks[b].nj, blocks[b].nk};
+      const int nq[3] = {blocks[q].ni, blocks[q].nj, blocks[q].nk};
+      for (int e = 0; e < 3; ++e)
+        if (e != a && nb[e] != nq[e])
+          return fail(H3DMG_ERR_UNSUPPORTED,
+                      "blocks %d and %d share face %d with different extents along axis %d", b, q,
+                      f, e);
+      if (blocks[q].dx != blocks[b].dx || blocks[q].dy != blocks[b].dy ||
+          blocks[q].dz != blocks[b].dz || blocks[q].nlevels != blocks[b].nlevels)
+        return fail(H3DMG_ERR_UNSUPPORTED,
+                    "blocks %d and %d: neighbours with different spacing or level count (LMR "
+                    "refinement)", b, q);
+    }
+  if (!any) return H3DMG_OK;
+  if (P.smoother != H3DMG_SMOOTH_RBGS)
+    return fail(H3DMG_ERR_UNSUPPORTED, "blocks with neighbours: red-black smoother only");
+  if (P.coarse != H3DMG_COARSE_SMOOTH)
+    return fail(H3DMG_ERR_UNSUPPORTED,
+                "blocks with neighbours: the coarsest level is smoothed (H3DMG_COARSE_SMOOTH)");
+  static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  for (int b = 0; b < nblocks; ++b) {
+    const int n[3] = {blocks[b].ni, blocks[b].nj, blocks[b].nk};
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int d[3] = {dx, dy, dz};
+          if ((dx != 0) + (dy != 0) + (dz != 0) < 2) continue;
+          int first_b = -1, first[3] = {0, 0, 0};
+          for (const auto &ord : perms) {
+            int c[3];
+            for (int a = 0; a < 3; ++a) c[a] = d[a] < 0 ? -1 : d[a] > 0 ? n[a] : 0;
+            const int r = host_resolve(blocks, b, c, ord);
+            if (first_b < 0) {
+              first_b = r;
+              for (int a = 0; a < 3; ++a) first[a] = c[a];
+            } else if (r != first_b || c[0] != first[0] || c[1] != first[1] || c[2] != first[2]) {
+              return fail(H3DMG_ERR_UNSUPPORTED,
+                          "block %d: the edge/corner ghost towards (%d,%d,%d) has no diagonal "
+                          "block (the blocks do not form a box)",
+                          b, dx, dy, dz);
+            }
+          }
+        }
+  }
+  return H3DMG_OK;
+}
+
+// Host-side checks of the block specifications, run before any device call:
+// the level rule of localparameters.for:202-208 (every axis divisible by
+// 2^(ngrid-1)), sizes, spacings and periodic pairs.
+h3dmg_status validate_blocks(const h3dmg_block *blocks, int nblocks, const h3dmg_params &P) {
+  for (int b = 0; b < nblocks; ++b) {
+    const int n[3] = {blocks[b].ni, blocks[b].nj, blocks[b].nk};
+    const int L = blocks[b].nlevels;
+    if (L < 1 || L > 16) return fail(H3DMG_ERR_ARG, "block %d: nlevels %d out of range", b, L);
+    for (int a = 0; a < 3; ++a) {
+      if (n[a] < 1) return fail(H3DMG_ERR_ARG, "block %d: n[%d] = %d", b, a, n[a]);
+      if (n[a] % (1 << (L - 1)) != 0)
+        return fail(H3DMG_ERR_LEVELS,
+                    "block %d: %d cells along axis %d not divisible by 2^(nlevels-1) = %d", b,
+                    n[a], a, 1 << (L - 1));
+    }
+    for (int f = 0; f < 6; ++f) {
+      const int k = blocks[b].bc[f];
+      if (k != H3DMG_BC_WALL && k != H3DMG_BC_PERIODIC && k != H3DMG_BC_NEIGHBOUR)
+        return fail(H3DMG_ERR_ARG, "block %d: face %d has boundary kind %d", b, f, k);
+    }
+    for (int a = 0; a < 3; ++a) {
+      const int lo = blocks[b].bc[2 * a], hi = blocks[b].bc[2 * a + 1];
+      if ((lo == H3DMG_BC_PERIODIC) != (hi == H3DMG_BC_PERIODIC))
+        return fail(H3DMG_ERR_ARG, "block %d: axis %d periodic on one face only", b, a);
+    }
+    if (!(blocks[b].dx > 0 && blocks[b].dy > 0 && blocks[b].dz > 0))
+      return fail(H3DMG_ERR_ARG, "block %d: non-positive spacing", b);
+    if (P.precision == H3DMG_MIXED && L < 2)
+      return fail(H3DMG_ERR_ARG, "block %d: mixed precision needs nlevels >= 2", b);
+    if (P.smoother == H3DMG_SMOOTH_TDMA &&
+        (n[0] > TDMA_MAX_LEN || n[1] > TDMA_MAX_LEN || n[2] > TDMA_MAX_LEN))
+      return fail(H3DMG_ERR_UNSUPPORTED, "line TDMA: block %d has lines over %d cells", b,
+                  TDMA_MAX_LEN);
+  }
+  return validate_neighbours(blocks, nblocks, P);
+}
+
 h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   h->nb = nblocks;
   h->blk.resize(nblocks);
@@ -544,27 +690,12 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   for (int b = 0; b < nblocks; ++b) {
     BlockInfo &B = h->blk[b];
     B.spec = blocks[b];
-    const int n[3] = {blocks[b].ni, blocks[b].nj, blocks[b].nk};
     B.L = blocks[b].nlevels;
-    if (B.L < 1 || B.L > 16) return fail(H3DMG_ERR_ARG, "block %d: nlevels %d out of range", b, B.L);
-    for (int a = 0; a < 3; ++a) {
-      if (n[a] < 1) return fail(H3DMG_ERR_ARG, "block %d: n[%d] = %d", b, a, n[a]);
-      if (n[a] % (1 << (B.L - 1)) != 0)
-        return fail(H3DMG_ERR_LEVELS,
-                    "block %d: %d cells along axis %d not divisible by 2^(nlevels-1) = %d", b,
-                    n[a], a, 1 << (B.L - 1));
-    }
     B.per = 0;
-    for (int a = 0; a < 3; ++a) {
-      const int lo = blocks[b].bc[2 * a], hi = blocks[b].bc[2 * a + 1];
-      if ((lo == H3DMG_BC_PERIODIC) != (hi == H3DMG_BC_PERIODIC))
-        return fail(H3DMG_ERR_ARG, "block %d: axis %d periodic on one face only", b, a);
-      if (lo == H3DMG_BC_PERIODIC) B.per |= 1 << a;
-    }
-    if (!(blocks[b].dx > 0 && blocks[b].dy > 0 && blocks[b].dz > 0))
-      return fail(H3DMG_ERR_ARG, "block %d: non-positive spacing", b);
-    if (is_mixed(h) && B.L < 2)
-      return fail(H3DMG_ERR_ARG, "block %d: mixed precision needs nlevels >= 2", b);
+    for (int a = 0; a < 3; ++a)
+      if (blocks[b].bc[2 * a] == H3DMG_BC_PERIODIC) B.per |= 1 << a;
+    for (int f = 0; f < 6; ++f)
+      if (blocks[b].bc[f] == H3DMG_BC_NEIGHBOUR) h->connected = true;
     h->Lmax = std::max(h->Lmax, B.L);
   }
   const bool exact = h->prm.coarse == H3DMG_COARSE_EXACT;
@@ -630,6 +761,9 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       LevelDev &L = h->levs[B.desc[g]];
       L.finer = g > 1 ? B.desc[g - 1] : -1;
       L.coarser = g < B.L ? B.desc[g + 1] : -1;
+      // neighbours share the level count, so level g exists on both sides
+      for (int f = 0; f < 6; ++f)
+        L.nbr[f] = B.spec.bc[f] == H3DMG_BC_NEIGHBOUR ? h->blk[B.spec.nbr[f]].desc[g] : -1;
     }
   }
   // finest-level FP64 p and f
@@ -721,6 +855,11 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       }
       if (members.empty()) continue;
       TileSet &ts = grp == 0 ? h->restr_nc[g] : h->restr_c[g];
+      if (h->connected) {  // natural coarse tiles, the general kernel (depth 0)
+        ts.depth = 0;
+        for (int d : members) add_tiles(ts, d, h->levs[d].ni, h->levs[d].nj, h->levs[d].nk);
+        continue;
+      }
       ts.depth = restrict_depth(h->levs, members);
       for (int d : members) add_tiles_restrict(ts, d, h->levs[d], ts.depth);
     }
@@ -746,7 +885,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       int xs, ys;
       cols += (long)fine_xy(h->levs[b], xs, ys) * h->levs[b].nk;
       nks.push_back(h->levs[b].nk);
-      fine_general = fine_general || h->levs[b].ni < 2;
+      fine_general = fine_general || h->levs[b].ni < 2 || h->connected;
     }
     h->fine_tiles.depth = launch_depth(cols, 2048, 0, 8, nks);
   }
@@ -761,7 +900,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       for (int y = 0; y < L.nj; y += ys)
         for (int x = 0; x < L.H; x += xs) {
           const bool inner = y >= 1 && y + ys <= L.nj - 1 && z >= 1 && z1 <= L.nk - 1;
-          const int kind = L.ni < 2 ? 1 : inner ? 0 : 3;
+          const int kind = (L.ni < 2 || h->connected) ? 1 : inner ? 0 : 3;
           h->fine_tiles.h.push_back(Tile{b, x, y, z, z1, kind});
         }
     }
@@ -834,6 +973,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     F.nk = L.nk;
     F.H = L.H;
     F.per = L.per;
+    for (int f = 0; f < 6; ++f) F.nbr[f] = L.nbr[f];  // level-1 descriptor d == block d
     F.lpr = lanes_per_row(2 * L.H);
     F.blk = b;
     F.cx = L.cx;
@@ -923,8 +1063,8 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     if (st != H3DMG_OK) return st;
   }
   {
-    // the tail kernel runs the red-black smoother only
-    if (h->prm.smoother != H3DMG_SMOOTH_RBGS) h->tail_req = Lm + 1;
+    // the tail kernel runs the red-black smoother on independent blocks only
+    if (h->prm.smoother != H3DMG_SMOOTH_RBGS || h->connected) h->tail_req = Lm + 1;
     h3dmg_status st = upload_tail(h);
     if (st != H3DMG_OK) return st;
   }
@@ -949,7 +1089,7 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
         const LevelDev &L = h->levs[B.desc[g]];
         cols += (long)march_tiles_xy(L) * L.nk;
         nks.push_back(L.nk);
-        general = general || L.ni < 2;
+        general = general || L.ni < 2 || h->connected;
       }
     // small levels: one fused launch per sweep (latency-bound, the red halo
     // recomputed per tile costs less than a second launch)
@@ -959,7 +1099,8 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
         const LevelDev &L = h->levs[B.desc[g]];
         most = std::max(most, (long)L.ni * L.nj * L.nk);
       }
-    ts.fused = most > 0 && most <= h->fused_max && h->prm.smoother == H3DMG_SMOOTH_RBGS;
+    ts.fused = most > 0 && most <= h->fused_max && h->prm.smoother == H3DMG_SMOOTH_RBGS &&
+               !h->connected;
     ts.ndiag[0] = ts.ndiag[1] = ts.ndiag[2] = 0;
     if (h->prm.smoother != H3DMG_SMOOTH_RBGS) {
       // wavefront smoothers: GS tiles of GS_TILE^3 cells sorted by
@@ -1038,7 +1179,7 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
       for (int b = 0; b < h->nb; ++b) {
         const BlockInfo &B = h->blk[b];
         if (!member(B)) continue;
-        add_tiles_march(ts, B.desc[g], h->levs[B.desc[g]], kb);
+        add_tiles_march(ts, B.desc[g], h->levs[B.desc[g]], kb, h->connected);
       }
     }
     for (const Tile &t : ts.h)
@@ -1136,6 +1277,45 @@ void drop_graph(h3dmg_handle *h) {
 // (the next V-cycle's rhs); per-block max |div u| dx dy dz into d_stats.
 void enqueue_project(h3dmg_handle *h) {
   const double dt = h->prm.dt;
+  if (h->connected) {
+    // calvel on the faces each block computes, then exchange(1..3) of the
+    // near faces from the blocks across them (press.for:77-144), then calmas
+    for (int b = 0; b < h->nb; ++b) {
+      const BlockInfo &B = h->blk[b];
+      const h3dmg_handle::VelBlock &V = h->vel[b];
+      const size_t ni = B.spec.ni, nj = B.spec.nj, nk = B.spec.nk;
+      launch_calvel_nb(h->stream, h->d_fine, b, 0, V.us, V.u, h->vel_dta, B.spec.dx,
+                       (ni + 1) * nj * nk);
+      launch_calvel_nb(h->stream, h->d_fine, b, 1, V.vs, V.v, h->vel_dta, B.spec.dy,
+                       ni * (nj + 1) * nk);
+      launch_calvel_nb(h->stream, h->d_fine, b, 2, V.ws, V.w, h->vel_dta, B.spec.dz,
+                       ni * nj * (nk + 1));
+    }
+    for (int b = 0; b < h->nb; ++b) {
+      const BlockInfo &B = h->blk[b];
+      const int n[3] = {B.spec.ni, B.spec.nj, B.spec.nk};
+      for (int a = 0; a < 3; ++a) {
+        const int k = B.spec.bc[2 * a];
+        if (k == H3DMG_BC_WALL) continue;
+        const int q = k == H3DMG_BC_NEIGHBOUR ? B.spec.nbr[2 * a] : b;
+        const h3dmg_block &Q = h->blk[q].spec;
+        const int nq[3] = {Q.ni, Q.nj, Q.nk};
+        double *dst = a == 0 ? h->vel[b].u : a == 1 ? h->vel[b].v : h->vel[b].w;
+        const double *src = a == 0 ? h->vel[q].u : a == 1 ? h->vel[q].v : h->vel[q].w;
+        const int nu = a == 0 ? n[1] : n[0], nv = a == 2 ? n[1] : n[2];
+        launch_face_copy(h->stream, a, src, nq[a], dst, n[a], nu, nv);
+      }
+    }
+    for (int b = 0; b < h->nb; ++b) {
+      const BlockInfo &B = h->blk[b];
+      const h3dmg_handle::VelBlock &V = h->vel[b];
+      launch_calmas(h->stream, h->d_fine + b, V.u, V.v, V.w, nullptr, 1, dt, B.spec.dx, B.spec.dy,
+                    B.spec.dz, h->d_proj + h->proj_begin[b],
+                    (size_t)B.spec.ni * B.spec.nj * B.spec.nk);
+    }
+    launch_finalize(h->stream, h->d_proj_begin, h->nb, h->d_proj, h->d_stats);
+    return;
+  }
   for (int b = 0; b < h->nb; ++b) {
     const BlockInfo &B = h->blk[b];
     if (!B.enabled) continue;
@@ -1280,14 +1460,6 @@ h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_pa
     delete h;
     return fail(H3DMG_ERR_UNSUPPORTED, "the GS and line-TDMA smoothers run in FP64");
   }
-  if (P.smoother == H3DMG_SMOOTH_TDMA)
-    for (int b = 0; b < nblocks; ++b)
-      if (blocks[b].ni > TDMA_MAX_LEN || blocks[b].nj > TDMA_MAX_LEN ||
-          blocks[b].nk > TDMA_MAX_LEN) {
-        delete h;
-        return fail(H3DMG_ERR_UNSUPPORTED, "line TDMA: block %d has lines over %d cells", b,
-                    TDMA_MAX_LEN);
-      }
   if (P.nu1 < 0 || P.nu2 < 0 || P.nu1 + P.nu2 < 1 || P.maxcy < 0 ||
       (P.precision != H3DMG_FP64 && P.precision != H3DMG_MIXED) ||
       (P.coarse != H3DMG_COARSE_SMOOTH && P.coarse != H3DMG_COARSE_EXACT) ||
@@ -1295,11 +1467,19 @@ h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_pa
     delete h;
     return fail(H3DMG_ERR_ARG, "invalid parameters");
   }
+  {
+    const h3dmg_status vs = validate_blocks(blocks, nblocks, P);
+    if (vs != H3DMG_OK) {
+      delete h;
+      return vs;
+    }
+  }
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) {
     delete h;
     return fail(H3DMG_ERR_DEVICE, "no HIP device");
   }
+  h->device = dev;
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
     lds = 65536;
@@ -1449,30 +1629,12 @@ h3dmg_status h3dmg_set_p(h3dmg_handle *h, int b, const double *p, int ld1, int l
 h3dmg_status h3dmg_get_p(h3dmg_handle *h, int b, double *p, int ld1, int ld2, int off) {
   h3dmg_status st = check_block(h, b);
   if (st) return st;
-  // unpack interior, then build the ghost layer (images) on the host
-  const h3dmg_block &S = h->blk[b].spec;
+  // the interior and the ghost layer (the cells pbound and exchange(4)
+  // copy into it: mirror, wrap or the neighbour block's cell)
   const FineDev &F = h->fine[b];
-  launch_unpack_fine(h->stream, h->d_fine + b, 0, h->d_stage, F.ni, F.nj, F.nk);
+  launch_unpack_fine_padded(h->stream, h->d_fine, b, h->d_stage, F.ni, F.nj, F.nk);
   HIP_TRY(hipGetLastError());
-  std::vector<double> in((size_t)S.ni * S.nj * S.nk);
-  HIP_TRY(hipMemcpyAsync(in.data(), h->d_stage, in.size() * 8, hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  if (!p || off < 1 || ld1 < S.ni + off + 1 || ld2 < S.nj + off + 1)
-    return fail(H3DMG_ERR_ARG, "bad host array description (needs one ghost layer)");
-  const int per = h->blk[b].per;
-  auto img = [](int x, int n, bool periodic) {
-    if (x < 0) return periodic ? n - 1 : 0;
-    if (x >= n) return periodic ? 0 : n - 1;
-    return x;
-  };
-  for (int k = -1; k <= S.nk; ++k)
-    for (int j = -1; j <= S.nj; ++j)
-      for (int i = -1; i <= S.ni; ++i) {
-        const int a = img(i, S.ni, per & 1), c = img(j, S.nj, per & 2), d = img(k, S.nk, per & 4);
-        p[(size_t)(i + off) + (size_t)ld1 * ((size_t)(j + off) + (size_t)ld2 * (size_t)(k + off))] =
-            in[(size_t)a + (size_t)S.ni * ((size_t)c + (size_t)S.nj * d)];
-      }
-  return H3DMG_OK;
+  return stage_padded_to_host(h, b, p, ld1, ld2, off);
 }
 
 h3dmg_status h3dmg_get_su_device(h3dmg_handle *h, int b, double *d_su) {
@@ -1512,10 +1674,10 @@ h3dmg_status h3dmg_get_pp(h3dmg_handle *h, int b, double *pp, int ld1, int ld2, 
   if (st) return st;
   const LevelDev &L = h->levs[b];
   if (h->lev_is64[b])
-    launch_unpack_level<double>(h->stream, h->d_levs + b, 0, h->pp_buf, h->d_stage, L.ni, L.nj,
+    launch_unpack_level<double>(h->stream, h->d_levs, b, 0, h->pp_buf, h->d_stage, L.ni, L.nj,
                                 L.nk);
   else
-    launch_unpack_level<float>(h->stream, h->d_levs + b, 0, h->pp_buf, h->d_stage, L.ni, L.nj,
+    launch_unpack_level<float>(h->stream, h->d_levs, b, 0, h->pp_buf, h->d_stage, L.ni, L.nj,
                                L.nk);
   HIP_TRY(hipGetLastError());
   return stage_padded_to_host(h, b, pp, ld1, ld2, off);
@@ -1546,13 +1708,24 @@ h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycl
     all_done = true;
     bad = false;
     double gmax = 0.0;
+    // blocks joined by shared faces are one domain: one relative residual
+    double grel = 0.0;
+    if (h->connected) {
+      double ss = 0.0, ff = 0.0;
+      for (int b = 0; b < h->nb; ++b)
+        if (h->blk[b].enabled) {
+          ss += h->h_stats[b].ss;
+          ff += h->blk[b].fnorm * h->blk[b].fnorm;
+        }
+      grel = ff > 0 ? std::sqrt(ss) / std::sqrt(ff) : std::sqrt(ss);
+    }
     for (int b = 0; b < h->nb; ++b) {
       BlockInfo &B = h->blk[b];
       if (!B.enabled) continue;
       const double nrm = std::sqrt(h->h_stats[b].ss);
       const double vol = B.spec.dx * B.spec.dy * B.spec.dz;
       if (h->active[b]) {
-        B.rel = B.fnorm > 0 ? nrm / B.fnorm : nrm;
+        B.rel = h->connected ? grel : B.fnorm > 0 ? nrm / B.fnorm : nrm;
         B.rmax = h->h_stats[b].amax * rfac_base * vol;
         B.cycles = it;
       }
@@ -1607,6 +1780,8 @@ h3dmg_status h3dmg_set_block_tol(h3dmg_handle *h, int b, double tol) {
   h3dmg_status st = check_block(h, b);
   if (st) return st;
   if (!(tol >= 0.0)) return fail(H3DMG_ERR_ARG, "negative tolerance");
+  if (h->connected)
+    return fail(H3DMG_ERR_UNSUPPORTED, "blocks with neighbours stop together (one tolerance)");
   h->blk[b].tol = tol;
   return H3DMG_OK;
 }
@@ -1614,7 +1789,21 @@ h3dmg_status h3dmg_set_block_tol(h3dmg_handle *h, int b, double tol) {
 h3dmg_status h3dmg_set_block_enabled(h3dmg_handle *h, int b, int enabled) {
   h3dmg_status st = check_block(h, b);
   if (st) return st;
+  if (h->blk[b].enabled == (enabled != 0)) return H3DMG_OK;
+  if (h->connected)
+    return fail(H3DMG_ERR_UNSUPPORTED, "blocks with neighbours are solved together");
   h->blk[b].enabled = enabled != 0;
+  // the velocity loop's block set (and its captured step, which projects
+  // the blocks chosen at h3dmg_velocity_begin) is no longer this one
+  if (h->vel_ready || h->vgexec) {
+    std::lock_guard<std::mutex> lock(g_device_mutex);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->vgexec) (void)hipGraphExecDestroy(h->vgexec);
+    if (h->vgraph) (void)hipGraphDestroy(h->vgraph);
+    h->vgexec = nullptr;
+    h->vgraph = nullptr;
+    h->vel_ready = false;
+  }
   return H3DMG_OK;
 }
 
@@ -1624,6 +1813,10 @@ h3dmg_status h3dmg_project_device(h3dmg_handle *h, int b, const double *d_ustar,
                                   double *rmax) {
   h3dmg_status st = check_block(h, b);
   if (st) return st;
+  if (h->connected)
+    return fail(H3DMG_ERR_UNSUPPORTED,
+                "one block of blocks with neighbours: use the velocity loop (its faces are "
+                "shared)");
   if (!d_ustar || !d_vstar || !d_wstar || !d_u || !d_v || !d_w)
     return fail(H3DMG_ERR_ARG, "null velocity array");
   const FineDev &F = h->fine[b];
@@ -1794,6 +1987,9 @@ h3dmg_status h3dmg_get_su(h3dmg_handle *h, int b, double *su, int ld1, int ld2, 
 namespace {
 h3dmg_status enable_lsm(h3dmg_handle *h) {
   if (h->lsm) return H3DMG_OK;
+  if (h->connected)
+    return fail(H3DMG_ERR_UNSUPPORTED,
+                "the variable-density operator on blocks with neighbours is not built");
   if (is_mixed(h) || h->prm.coarse != H3DMG_COARSE_SMOOTH)
     return fail(H3DMG_ERR_UNSUPPORTED,
                 "the variable-density operator runs in FP64 with the reference coarsest level");
@@ -1954,7 +2150,13 @@ h3dmg_status h3dmg_velocity_begin(h3dmg_handle *h, const double *const *d_ustar,
 
 h3dmg_status h3dmg_velocity_step(h3dmg_handle *h, double *rmax) {
   if (!h) return fail(H3DMG_ERR_ARG, "null handle");
-  if (!h->vel_ready) return fail(H3DMG_ERR_ARG, "h3dmg_velocity_begin has not run");
+  if (!h->vel_ready)
+    return fail(H3DMG_ERR_ARG,
+                "h3dmg_velocity_begin has not run since the handle's enabled blocks last "
+                "changed");
+  for (int b = 0; b < h->nb; ++b)  // every projected block has its arrays
+    if (h->blk[b].enabled && (!h->vel[b].us || !h->vel[b].u))
+      return fail(H3DMG_ERR_ARG, "block %d: no velocity arrays", b);
   reset_active(h);
   h3dmg_status st = push_active(h);
   if (st) return st;
@@ -2009,6 +2211,8 @@ h3dmg_status h3dmg_block_stats(const h3dmg_handle *h, int *cycles, double *rel_r
 
 h3dmg_status h3dmg_remove_mean(h3dmg_handle *h) {
   if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  if (h->connected)
+    return fail(H3DMG_ERR_UNSUPPORTED, "the mean of blocks with neighbours is not per block");
   for (int b = 0; b < h->nb; ++b) {
     const FineDev &F = h->fine[b];
     launch_remove_mean(h->stream, h->d_fine + b, F.ni, F.nj, F.nk, nullptr);
@@ -2041,9 +2245,9 @@ h3dmg_status h3dmg_dbg_get(h3dmg_handle *h, int b, int level, int which, double 
   const int d = h->blk[b].desc[level];
   const LevelDev &L = h->levs[d];
   if (h->lev_is64[d])
-    launch_unpack_level<double>(h->stream, h->d_levs + d, which, 0, d_dst, L.ni, L.nj, L.nk);
+    launch_unpack_level<double>(h->stream, h->d_levs, d, which, 0, d_dst, L.ni, L.nj, L.nk);
   else
-    launch_unpack_level<float>(h->stream, h->d_levs + d, which, 0, d_dst, L.ni, L.nj, L.nk);
+    launch_unpack_level<float>(h->stream, h->d_levs, d, which, 0, d_dst, L.ni, L.nj, L.nk);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
   return H3DMG_OK;
@@ -2124,18 +2328,18 @@ h3dmg_status h3dmg_dbg_prolong(h3dmg_handle *h, int level) {
   if (h->prol_nc[level].n()) {
     if (f64)
       launch_prolong<double, double>(h->stream, h->prol_nc[level].d, h->prol_nc[level].n(),
-                                     h->d_levs, h->d_active, 0, 0);
+                                     h->d_levs, h->d_active, 0, 0, h->connected);
     else
       launch_prolong<float, float>(h->stream, h->prol_nc[level].d, h->prol_nc[level].n(),
-                                   h->d_levs, h->d_active, 0, 0);
+                                   h->d_levs, h->d_active, 0, 0, h->connected);
   }
   if (h->prol_c[level].n()) {
     if (f64)
       launch_prolong<double, double>(h->stream, h->prol_c[level].d, h->prol_c[level].n(),
-                                     h->d_levs, h->d_active, 0, 0);
+                                     h->d_levs, h->d_active, 0, 0, h->connected);
     else
       launch_prolong<float, double>(h->stream, h->prol_c[level].d, h->prol_c[level].n(),
-                                    h->d_levs, h->d_active, 0, 0);
+                                    h->d_levs, h->d_active, 0, 0, h->connected);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -2157,6 +2361,8 @@ h3dmg_status h3dmg_set_tail(h3dmg_handle *h, int level) {
     return fail(H3DMG_ERR_UNSUPPORTED, "the tail kernel holds no variable-density coefficients");
   if (h->prm.smoother != H3DMG_SMOOTH_RBGS && level <= h->Lmax)
     return fail(H3DMG_ERR_UNSUPPORTED, "the tail kernel runs the red-black smoother only");
+  if (h->connected && level <= h->Lmax)
+    return fail(H3DMG_ERR_UNSUPPORTED, "the tail kernel runs independent blocks only");
   std::lock_guard<std::mutex> lock(g_device_mutex);
   HIP_TRY(hipStreamSynchronize(h->stream));
   drop_graph(h);
@@ -2183,6 +2389,8 @@ h3dmg_status h3dmg_set_fused(h3dmg_handle *h, long long max_cells) {
     return fail(H3DMG_ERR_UNSUPPORTED, "fused sweeps hold no variable-density coefficients");
   if (h->prm.smoother != H3DMG_SMOOTH_RBGS && max_cells > 0)
     return fail(H3DMG_ERR_UNSUPPORTED, "fused sweeps run the red-black smoother only");
+  if (h->connected && max_cells > 0)
+    return fail(H3DMG_ERR_UNSUPPORTED, "fused sweeps run independent blocks only");
   std::lock_guard<std::mutex> lock(g_device_mutex);
   HIP_TRY(hipStreamSynchronize(h->stream));
   drop_graph(h);

@@ -45,8 +45,9 @@ def _vp(t):
 
 
 class MultigridSolver:
-    """Independent blocks, each given as dict(n=(ni,nj,nk), h=(dx,dy,dz),
-    bc=(w,e,s,n,b,t) with 0 wall / 1 periodic, nlev=levels)."""
+    """Blocks, each given as dict(n=(ni,nj,nk), h=(dx,dy,dz),
+    bc=(w,e,s,n,b,t) with 0 wall / 1 periodic / 2 shared with block nbr[f],
+    nlev=levels, and for shared faces nbr=(six block indices))."""
 
     def __init__(self, blocks, nu1=2, nu2=2, maxcy=60, tol=1e-8, criterion="rel_l2",
                  precision="fp64", coarse="smooth", min_cycles=1, use_graph=True, dt=1e-3,
@@ -61,7 +62,10 @@ class MultigridSolver:
             arr[q].dx, arr[q].dy, arr[q].dz = (float(x) for x in b["h"])
             arr[q].nlevels = int(b["nlev"])
             arr[q].bc[:] = [int(x) for x in b["bc"]]
-            self.blocks.append(dict(n=n, h=tuple(b["h"]), bc=tuple(b["bc"]), nlev=int(b["nlev"])))
+            nbr = [int(x) for x in b.get("nbr", (-1,) * 6)]
+            arr[q].nbr[:] = nbr
+            self.blocks.append(dict(n=n, h=tuple(b["h"]), bc=tuple(b["bc"]), nlev=int(b["nlev"]),
+                                    nbr=tuple(nbr)))
         prm = L.H3dParams()
         self.lib.h3dmg_default_params(ctypes.byref(prm))
         prm.nu1, prm.nu2, prm.maxcy = nu1, nu2, maxcy

@@ -21,9 +21,10 @@
  *   Device-pointer entry points take dense interior arrays n_i*n_j*n_k
  *   (i fastest) that already live in HBM.
  *
- * Threading: one handle per process and GPU; calls on one handle must not
- * overlap.  All calls are synchronous with respect to the host unless named
- * *_async.
+ * Threading: a process may hold any number of handles and drive them from
+ * different host threads (each handle launches on its own stream); calls on
+ * one handle must not overlap.  All calls are synchronous with respect to
+ * the host.
  */
 #ifndef H3DMG_H
 #define H3DMG_H
@@ -49,8 +50,12 @@ typedef enum {
 
 /* boundary kind per face (west, east, south, north, bottom, top).  The
  * reference treats bc code 5 as periodic and every other code as a Neumann
- * wall for the pressure (src_v70/mgsolver.for:194-205). */
-enum { H3DMG_BC_WALL = 0, H3DMG_BC_PERIODIC = 1 };
+ * wall for the pressure (src_v70/mgsolver.for:194-205).  H3DMG_BC_NEIGHBOUR:
+ * the face is shared with another block of the same handle (h3dmg_block.nbr),
+ * the reference's same-level neighbour (iprev..knext, alloc_dom.for:524-690)
+ * or, on a periodic axis spanning several blocks, its periodic partner
+ * (per_ip..per_kn, alloc_dom.for:1040-1215). */
+enum { H3DMG_BC_WALL = 0, H3DMG_BC_PERIODIC = 1, H3DMG_BC_NEIGHBOUR = 2 };
 
 /* smoother (mg_itrsch, src_v70/control.cin line 10; mgrelax,
  * src_v70/mgsolver.for:407-605).  GS and TDMA run in FP64 on the
@@ -88,6 +93,8 @@ typedef struct {
   int nlevels;        /* multigrid levels (ngrid) */
   int bc[6];          /* H3DMG_BC_* per face; a periodic axis must have both
                          faces periodic */
+  int nbr[6];         /* for bc[f] == H3DMG_BC_NEIGHBOUR: index of the block
+                         across face f (read only for such faces) */
 } h3dmg_block;
 
 typedef struct {
@@ -111,8 +118,22 @@ void h3dmg_default_params(h3dmg_params *p);
 
 /* Build the hierarchy of every block in device memory (the role of coeff,
  * mgsolver.for:2-224, and of the cof allocation, localparameters.for:225-239).
- * Blocks are independent: no two blocks share a face (the reference's
- * neighbour search, alloc_dom.for:524-690, finds none). */
+ *
+ * Blocks without H3DMG_BC_NEIGHBOUR faces are independent solves.  Blocks
+ * joined by H3DMG_BC_NEIGHBOUR faces form one domain, as the reference's
+ * multi-block layout (one rank's blocks): every ghost a smoother, residual,
+ * restriction or prolongation reads is the value exchangepp leaves in it
+ * (exchangepp.for:2-2835, exchbc_mgpp :2837-3453) -- the neighbour's cell
+ * from before the sweep, edge and corner ghosts included -- with no copies:
+ * kernels read the neighbour's arrays.  Requirements (H3DMG_ERR_ARG or
+ * H3DMG_ERR_UNSUPPORTED otherwise): the two blocks of a shared face name
+ * each other on opposite faces, have the same spacing and level count and
+ * the same extent along the face; the blocks form a box, every edge or
+ * corner where two or three shared faces meet having its diagonal block
+ * (the reference leaves such ghosts stale otherwise); red-black smoother,
+ * H3DMG_COARSE_SMOOTH, constant coefficients.  Such a handle stops all its
+ * blocks together: REL_L2 on the norms summed over the blocks, REF_RMAX on
+ * the maximum. */
 h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_params *prm,
                           h3dmg_handle **out);
 void h3dmg_destroy(h3dmg_handle *h);
@@ -278,6 +299,39 @@ long long h3dmg_get_fused(const h3dmg_handle *h);
  * H3DMG_ERR_UNSUPPORTED. */
 h3dmg_status h3dmg_set_tail(h3dmg_handle *h, int level);
 int h3dmg_get_tail(const h3dmg_handle *h);
+
+/* ---- multi-block layout files (host only) -------------------------------
+ * infodom.cin and mdmap.cin as read_infodom / read_mdmap read them
+ * (src_v70/alloc_dom.for:284-482; infodom types 1 and 2, mdmap types 1 and
+ * 2), with the reference's face-neighbour search (alloc_dom.for:524-690)
+ * and periodic-partner search (alloc_dom.for:1040-1215).  Errors carry
+ * their text in h3dmg_layout_error(). */
+typedef struct h3dmg_layout h3dmg_layout;
+h3dmg_status h3dmg_layout_read(const char *infodom_path, const char *mdmap_path,
+                               h3dmg_layout **out);
+void h3dmg_layout_destroy(h3dmg_layout *L);
+const char *h3dmg_layout_error(void);
+int h3dmg_layout_ndomains(const h3dmg_layout *L);
+int h3dmg_layout_nranks(const h3dmg_layout *L);
+/* domain d: its rank (dom_ad), refinement (rdiv) and coordinates */
+h3dmg_status h3dmg_layout_domain(const h3dmg_layout *L, int d, int *rank, int *rdiv, double *lo,
+                                 double *hi);
+/* domain d's neighbours (iprev, inext, jprev, jnext, kprev, knext; -1 none)
+ * and periodic partners (per_ip, per_in, per_jp, per_jn, per_kp, per_kn),
+ * given the six global boundary codes of control.cin (5 = periodic); either
+ * output may be null */
+h3dmg_status h3dmg_layout_neighbours(const h3dmg_layout *L, int d, const int *bc_codes,
+                                     int *prev_next, int *partner);
+/* The blocks of one rank as an h3dmg_create table (localparameters.for:
+ * 80-104: cells = extent / the global spacing gdx[3]): shared faces become
+ * H3DMG_BC_NEIGHBOUR with local block indices, a periodic axis spanned by
+ * one block H3DMG_BC_PERIODIC, the rest walls.  dom_ids (may be null)
+ * receives each block's domain number.  With blocks == null only *nblocks
+ * is set.  A neighbour on another rank or rdiv > 1 is refused
+ * (H3DMG_ERR_UNSUPPORTED). */
+h3dmg_status h3dmg_layout_blocks(const h3dmg_layout *L, int rank, const double *gdx,
+                                 const int *bc_codes, int nlevels, h3dmg_block *blocks, int cap,
+                                 int *nblocks, int *dom_ids);
 
 /* ---- test seams: one operator at a time on level arrays ------------------
  * Level arrays cross this seam as dense device arrays in the layout of the
