@@ -51,6 +51,7 @@ program h3dmg_fortran_demo
   blk(1)%ni = n; blk(1)%nj = n; blk(1)%nk = n
   blk(1)%dx = hh; blk(1)%dy = hh; blk(1)%dz = hh
   blk(1)%nlevels = 4
+  blk(1)%nbr = -1
   blk(1)%bc = [H3DMG_BC_PERIODIC, H3DMG_BC_PERIODIC, H3DMG_BC_PERIODIC, &
                H3DMG_BC_PERIODIC, H3DMG_BC_WALL, H3DMG_BC_WALL]
 

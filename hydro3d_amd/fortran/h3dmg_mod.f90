@@ -16,7 +16,8 @@ module h3dmg_mod
        H3DMG_ERR_LEVELS = 2, H3DMG_ERR_DEVICE = 3, H3DMG_ERR_NOMEM = 4, &
        H3DMG_ERR_NOT_CONVERGED = 5, H3DMG_ERR_DIVERGED = 6, &
        H3DMG_ERR_UNSUPPORTED = 7, H3DMG_ERR_BUSY = 8
-  integer(c_int), parameter, public :: H3DMG_BC_WALL = 0, H3DMG_BC_PERIODIC = 1
+  integer(c_int), parameter, public :: H3DMG_BC_WALL = 0, H3DMG_BC_PERIODIC = 1, &
+       H3DMG_BC_NEIGHBOUR = 2
   integer(c_int), parameter, public :: H3DMG_SMOOTH_GS = 1, H3DMG_SMOOTH_TDMA = 2, &
        H3DMG_SMOOTH_RBGS = 3
   integer(c_int), parameter, public :: H3DMG_CRIT_REL_L2 = 0, H3DMG_CRIT_REF_RMAX = 1
@@ -28,6 +29,7 @@ module h3dmg_mod
      real(c_double) :: dx, dy, dz
      integer(c_int) :: nlevels
      integer(c_int) :: bc(6)
+     integer(c_int) :: nbr(6)   ! 0-based block index across a NEIGHBOUR face
   end type h3dmg_block
 
   type, bind(C), public :: h3dmg_params

@@ -19,9 +19,12 @@
 ! lexicographic GS, 2 line TDMA, 3 red-black GS), constant-coefficient or
 ! variable-density (L_LSM: the density goes to the device every call, as the
 ! reference rebuilds its coefficients every solve, newsolv_mg.for:13-15), on
-! blocks without neighbours (every face a wall or periodic onto itself), one
-! MPI rank per GPU.  Anything else stops with a message rather than running
-! a different algorithm.  The precision is FP64 (bitwise the reference loop)
+! independent blocks; and the red-black multigrid with constant coefficients
+! on the blocks of a rank that share faces (same-level neighbours and
+! periodic partners; the halo exchange is the device reading the
+! neighbour's cells), one MPI rank per GPU.  Anything else -- a neighbour on
+! another rank, LMR refinement, calmas' 4th-order branch -- stops with a
+! message rather than running a different algorithm.  The precision is FP64 (bitwise the reference loop)
 ! unless the environment variable H3DMG_PRECISION=mixed selects the FP32
 ! V-cycle with the exact coarsest solve inside the FP64 velocity loop
 ! (constant coefficients and red-black GS only).
@@ -33,6 +36,7 @@ subroutine newsolv_mg_h3d
   use, intrinsic :: iso_c_binding
   implicit none
   type(c_ptr), save :: h = c_null_ptr
+  type(h3dmg_block), allocatable :: blk(:)   ! the block table of setup()
   integer :: ib
   integer(c_int) :: st
   real(c_double) :: rloc
@@ -126,27 +130,35 @@ subroutine newsolv_mg_h3d
 contains
 
   subroutine setup()
-    type(h3dmg_block), allocatable :: blk(:)
     type(h3dmg_params) :: prm
     character(len=16) :: pv
     integer :: b, plen, pst
     allocate (blk(nbp))
+    ! calmas' 4th-order divergence (press.for:17-37) is not on the device path
+    if (differencing == 2) &
+         call h3d_stop('differencing = 2 (4th-order calmas) is not on the device path')
     do b = 1, nbp
        associate (d => dom(b))
-         if (d%iprev >= 0 .or. d%inext >= 0 .or. d%jprev >= 0 .or. d%jnext >= 0 .or. &
-             d%kprev >= 0 .or. d%knext >= 0) then
-            if (.not. self_periodic(d)) &
-                 call h3d_stop('blocks with neighbours need the halo exchange, not on the device path')
-         end if
+         ! every block coarsens to the global level count (no LMR levels)
+         if (d%ngrid /= ngrd_gl) &
+              call h3d_stop('blocks with their own level count (LMR, rdiv > 1) are not on the device path')
          blk(b)%ni = d%iep - d%isp + 1
          blk(b)%nj = d%jep - d%jsp + 1
          blk(b)%nk = d%kep - d%ksp + 1
          blk(b)%dx = d%dx
          blk(b)%dy = d%dy
          blk(b)%dz = d%dz
-         blk(b)%nlevels = ngrd_gl
-         blk(b)%bc = [bcmap(d%bc_west), bcmap(d%bc_east), bcmap(d%bc_south), &
-                      bcmap(d%bc_north), bcmap(d%bc_bottom), bcmap(d%bc_top)]
+         blk(b)%nlevels = d%ngrid
+         ! faces shared with another block of this rank (alloc_dom.for:524-690)
+         ! or, on a periodic axis spanning several blocks, with the periodic
+         ! partner (alloc_dom.for:1040-1215); the device reads the neighbour's
+         ! cells where exchangepp would copy them
+         call face(b, 1, d%iprev, d%inext, d%per_ip, d%bc_west)
+         call face(b, 2, d%inext, d%iprev, d%per_in, d%bc_east)
+         call face(b, 3, d%jprev, d%jnext, d%per_jp, d%bc_south)
+         call face(b, 4, d%jnext, d%jprev, d%per_jn, d%bc_north)
+         call face(b, 5, d%kprev, d%knext, d%per_kp, d%bc_bottom)
+         call face(b, 6, d%knext, d%kprev, d%per_kn, d%bc_top)
        end associate
     end do
     call h3dmg_default_params(prm)
@@ -167,14 +179,30 @@ contains
     call chk(st, 'create')
   end subroutine setup
 
-  ! a block whose only neighbour on a periodic axis is itself
-  logical function self_periodic(d)
-    type(multidom), intent(in) :: d
-    self_periodic = (d%iprev < 0 .or. d%bc_west == 5) .and. (d%inext < 0 .or. d%bc_east == 5) .and. &
-         (d%jprev < 0 .or. d%bc_south == 5) .and. (d%jnext < 0 .or. d%bc_north == 5) .and. &
-         (d%kprev < 0 .or. d%bc_bottom == 5) .and. (d%knext < 0 .or. d%bc_top == 5) .and. &
-         nprocs == 1 .and. nbp == 1
-  end function self_periodic
+  ! face f (1..6) of block b: `across` is the neighbour's domain number
+  ! (-1: none), `opposite` the neighbour on the other face of the axis,
+  ! `partner` the periodic partner, `code` the boundary code (5 periodic)
+  subroutine face(b, f, across, opposite, partner, code)
+    integer, intent(in) :: b, f, across, opposite, partner, code
+    integer :: other
+    blk(b)%nbr(f) = -1
+    other = across
+    if (other < 0 .and. code == 5) then
+       if (opposite < 0) then
+          blk(b)%bc(f) = H3DMG_BC_PERIODIC   ! the block spans the axis
+          return
+       end if
+       other = partner
+    end if
+    if (other < 0) then
+       blk(b)%bc(f) = H3DMG_BC_WALL
+       return
+    end if
+    if (dom_ad(other) /= myrank) &
+         call h3d_stop('a block''s neighbour on another rank: the exchange between ranks is not on the device path')
+    blk(b)%bc(f) = H3DMG_BC_NEIGHBOUR
+    blk(b)%nbr(f) = dom_indid(other) - 1
+  end subroutine face
 
   ! bc code 5 is periodic, every other code a Neumann wall for the pressure
   ! (mgsolver.for:194-205)
