@@ -218,6 +218,12 @@ int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const doubl
                   const double *w, double *su, int to_fine, double dt, double dx, double dy,
                   double dz, RedPartial *partials, size_t cells);
 
+// workgroups launch_calmas uses for `cells` cells
+inline int calmas_groups(size_t cells) {
+  return (int)((cells + 255) / 256 < (size_t)CALMAS_MAX_GROUPS ? (cells + 255) / 256
+                                                                : (size_t)CALMAS_MAX_GROUPS);
+}
+
 // layout conversion: dense interior (i fastest) <-> colour split
 void launch_pack_fine(hipStream_t s, const FineDev *fd, int which, const double *src, int ni,
                       int nj, int nk);

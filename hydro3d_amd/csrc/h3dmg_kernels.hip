@@ -112,9 +112,12 @@ __device__ __forceinline__ int resolve_cell(const LD *__restrict__ tab, int d, i
       }
     }
   }
-  x = c[0];
-  y = c[1];
-  z = c[2];
+  // A position at most one cell outside the block is now inside block d.
+  // Lanes that lie beyond their level (their results are discarded) may
+  // pass positions further out; clamping keeps their reads inside block d.
+  x = min(max(c[0], 0), P->ni - 1);
+  y = min(max(c[1], 0), P->nj - 1);
+  z = min(max(c[2], 0), P->nk - 1);
   return d;
 }
 
@@ -235,7 +238,8 @@ __device__ __forceinline__ void march_tile(const Tile &t, const LevelDev *__rest
   const bool zero_int = zero && colour == 0;  // uniform
   const bool zero_ghost = zero != 0;
   // lanes outside the level read a valid entry (clamped) and store nothing
-  const unsigned cbase = (unsigned)min(j, nj - 1) * row + (unsigned)min(m0, H - 2);
+  const int jc = min(j, nj - 1);
+  const unsigned cbase = (unsigned)jc * row + (unsigned)min(m0, H - 2);
   const v2 vz = {T(0), T(0)};
   // tiles that do not start/end a row: the end lanes fetch entries m0-1 and
   // m0+2 from the neighbouring tile's range (uniform test, then one lane)
@@ -267,8 +271,10 @@ __device__ __forceinline__ void march_tile(const Tile &t, const LevelDev *__rest
     const unsigned q = (unsigned)k * plane + cbase;
     v2 S2 = vz, N2 = vz;
     if (!zero_int) {
-      if (INNER || j > 0) S2 = ld2(O, q - row);
-      if (INNER || j + 1 < nj) N2 = ld2(O, q + row);
+      // jc: the lane's row, clamped like cbase (lanes beyond the level load
+      // in-bounds rows; their results are not stored)
+      if (INNER || jc > 0) S2 = ld2(O, q - row);
+      if (INNER || jc + 1 < nj) N2 = ld2(O, q + row);
     }
     const v2 R2 = ld2(R, q);
     T lf = wave_shift(o_c.y, 1);  // entry m0-1: lane-1's second entry
@@ -294,11 +300,11 @@ __device__ __forceinline__ void march_tile(const Tile &t, const LevelDev *__rest
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
         const int i = e ? i1 : i0;
-        const T S = (j > 0) ? (e ? S2.y : S2.x) : ghost(i, -1, k);
-        const T N = (j + 1 < nj) ? (e ? N2.y : N2.x) : ghost(i, nj, k);
-        const T B = (k > 0) ? (e ? o_m.y : o_m.x) : ghost(i, j, -1);
-        const T Tt = (k + 1 < nk) ? (e ? o_p.y : o_p.x) : ghost(i, j, nk);
-        const Coef<T> c = cell_coef<T>(L, i, j, k);
+        const T S = (jc > 0) ? (e ? S2.y : S2.x) : ghost(i, -1, k);
+        const T N = (jc + 1 < nj) ? (e ? N2.y : N2.x) : ghost(i, nj, k);
+        const T B = (k > 0) ? (e ? o_m.y : o_m.x) : ghost(i, jc, -1);
+        const T Tt = (k + 1 < nk) ? (e ? o_p.y : o_p.x) : ghost(i, jc, nk);
+        const Coef<T> c = cell_coef<T>(L, i, jc, k);
         const T sum = c.aw * Wv[e] + c.ae * Ev[e] + c.as * S + c.an * N + c.ab * B + c.at * Tt;
         o[e] = div_ap((e ? R2.y : R2.x) - sum, c.ap, T(1) / c.ap);
       }
@@ -1114,7 +1120,11 @@ __device__ __forceinline__ void residual_tile(const Tile &t, const FineDev *__re
   const bool ok = 2 * l < ni && j < nj;
   const bool b_ok = 2 * l + 1 < ni && j < nj;
   const unsigned row = (unsigned)H, plane = (unsigned)H * (unsigned)nj;
-  const unsigned cbase = (unsigned)min(j, nj - 1) * row + (unsigned)min(l, H - 1);
+  // jc: the lane's row clamped like cbase -- lanes beyond the last row load
+  // in-bounds rows and resolve in-bounds ghosts (their results are not
+  // stored); for every stored cell jc == j
+  const int jc = min(j, nj - 1);
+  const unsigned cbase = (unsigned)jc * row + (unsigned)min(l, H - 1);
   const gmem<double> *P0 = (const gmem<double> *)F.p[0], *P1 = (const gmem<double> *)F.p[1];
   const gmem<double> *F0 = (const gmem<double> *)F.f[0], *F1 = (const gmem<double> *)F.f[1];
   gmem<T> *R0 = (gmem<T> *)L.rhs[0], *R1 = (gmem<T> *)L.rhs[1];
@@ -1152,30 +1162,30 @@ __device__ __forceinline__ void residual_tile(const Tile &t, const FineDev *__re
       if (tile_left) lf = ldp(cb, q - 1);
       if (tile_right) rt = ldp(ca, q + 1);
       const int ia = 2 * l, ib = ia + 1;
-      const double Wa = ia == 0 ? fine_value(fine, t.lev, -1, j, k) : lf;
-      const double Ea = ib < ni ? Pb : fine_value(fine, t.lev, ni, j, k);
+      const double Wa = ia == 0 ? fine_value(fine, t.lev, -1, jc, k) : lf;
+      const double Ea = ib < ni ? Pb : fine_value(fine, t.lev, ni, jc, k);
       const double Wb = Pa;
-      const double Eb = ib + 1 < ni ? rt : fine_value(fine, t.lev, ni, j, k);
+      const double Eb = ib + 1 < ni ? rt : fine_value(fine, t.lev, ni, jc, k);
       // y- and z-neighbours of a (colour cb there) and b (colour ca)
       double Sa, Na, Sb, Nb;
-      if (INNER || j > 0) {
+      if (INNER || jc > 0) {
         Sa = ldp(cb, q - row);
         Sb = ldp(ca, q - row);
       } else {
         Sa = fine_value(fine, t.lev, ia, -1, k);
         Sb = fine_value(fine, t.lev, ib, -1, k);
       }
-      if (INNER || j + 1 < nj) {
+      if (INNER || jc + 1 < nj) {
         Na = ldp(cb, q + row);
         Nb = ldp(ca, q + row);
       } else {
         Na = fine_value(fine, t.lev, ia, nj, k);
         Nb = fine_value(fine, t.lev, ib, nj, k);
       }
-      const double Ba = (INNER || k > 0) ? (cb ? w1m : w0m) : fine_value(fine, t.lev, ia, j, -1);
-      const double Bb = (INNER || k > 0) ? (ca ? w1m : w0m) : fine_value(fine, t.lev, ib, j, -1);
-      const double Ta = (INNER || k + 1 < nk) ? (cb ? w1p : w0p) : fine_value(fine, t.lev, ia, j, nk);
-      const double Tb = (INNER || k + 1 < nk) ? (ca ? w1p : w0p) : fine_value(fine, t.lev, ib, j, nk);
+      const double Ba = (INNER || k > 0) ? (cb ? w1m : w0m) : fine_value(fine, t.lev, ia, jc, -1);
+      const double Bb = (INNER || k > 0) ? (ca ? w1m : w0m) : fine_value(fine, t.lev, ib, jc, -1);
+      const double Ta = (INNER || k + 1 < nk) ? (cb ? w1p : w0p) : fine_value(fine, t.lev, ia, jc, nk);
+      const double Tb = (INNER || k + 1 < nk) ? (ca ? w1p : w0p) : fine_value(fine, t.lev, ib, jc, nk);
       Coef<double> c_a, c_b;
       if (INNER) {
         c_a.as = c_a.an = c_b.as = c_b.an = cy;
@@ -3064,7 +3074,7 @@ void launch_unpack_fine_padded(hipStream_t s, const FineDev *fine, int b, double
 int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const double *v,
                   const double *w, double *su, int to_fine, double dt, double dx, double dy,
                   double dz, RedPartial *partials, size_t cells) {
-  const int grid = (int)std::min<size_t>((cells + 255) / 256, CALMAS_MAX_GROUPS);
+  const int grid = calmas_groups(cells);
   hipLaunchKernelGGL(k_calmas, dim3(grid), dim3(256), 0, s, fd, u, v, w, su, to_fine, dt, dx, dy,
                      dz, partials);
   return grid;
