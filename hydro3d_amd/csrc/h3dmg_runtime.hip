@@ -1936,8 +1936,14 @@ h3dmg_status h3dmg_set_velocity(h3dmg_handle *h, int b, const double *ustar, con
   return H3DMG_OK;
 }
 
-h3dmg_status h3dmg_get_velocity(h3dmg_handle *h, int b, double *u, double *v, double *w, int ld1,
-                                int ld2, int off) {
+namespace {
+// u, v, w of block b into host Fortran arrays: every face of the interior
+// rows, or (calvel_only) just the faces calvel computes -- isu..ieu and the
+// like (press.for:90-131 with localparameters.for:106-192): faces 1..n of an
+// axis whose far face is open (a neighbour or a periodic wrap), 1..n-1
+// against a far wall; face 0 is never calvel's (exchange(1..3) fills it).
+h3dmg_status velocity_to_host(h3dmg_handle *h, int b, double *u, double *v, double *w, int ld1,
+                              int ld2, int off, bool calvel_only) {
   h3dmg_status st = check_block(h, b);
   if (st) return st;
   if (!u || !v || !w) return fail(H3DMG_ERR_ARG, "null u, v or w");
@@ -1950,16 +1956,30 @@ h3dmg_status h3dmg_get_velocity(h3dmg_handle *h, int b, double *u, double *v, do
   const int n[3] = {S.ni, S.nj, S.nk};
   for (int axis = 0; axis < 3; ++axis) {
     const int na = axis == 0 ? S.nj : S.ni, nc = axis == 2 ? S.nj : S.nk;
+    const bool far_open = S.bc[2 * axis + 1] != H3DMG_BC_WALL;
+    const int f0 = calvel_only ? 1 : 0;
+    const int f1 = calvel_only && !far_open ? n[axis] - 1 : n[axis];
     std::vector<double> dense((size_t)(n[axis] + 1) * na * nc);
     HIP_TRY(hipMemcpyAsync(dense.data(), h->vel_store + h->vel_off[6 * b + 3 + axis],
                            dense.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     for (int c = 0; c < nc; ++c)
       for (int a = 0; a < na; ++a)
-        for (int f = 0; f <= n[axis]; ++f)
+        for (int f = f0; f <= f1; ++f)
           dst[axis][face_index(axis, f, a, c, ld1, ld2, off)] = dense[dense_face(axis, f, a, c, S)];
   }
   return H3DMG_OK;
+}
+}  // namespace
+
+h3dmg_status h3dmg_get_velocity(h3dmg_handle *h, int b, double *u, double *v, double *w, int ld1,
+                                int ld2, int off) {
+  return velocity_to_host(h, b, u, v, w, ld1, ld2, off, false);
+}
+
+h3dmg_status h3dmg_get_velocity_calvel(h3dmg_handle *h, int b, double *u, double *v, double *w,
+                                       int ld1, int ld2, int off) {
+  return velocity_to_host(h, b, u, v, w, ld1, ld2, off, true);
 }
 
 h3dmg_status h3dmg_get_su(h3dmg_handle *h, int b, double *su, int ld1, int ld2, int off) {

@@ -10,10 +10,13 @@
 ! p = p + pp, calvel, calmas, in the reference's arithmetic.  u*, v*, w*,
 ! u, v, w and p go to the device once per call; each iteration returns this
 ! rank's rmax, which is reduced over the ranks here as calmas reduces it
-! (press.for:72-73); u, v, w, su and p come back once.  The host then fills
-! the remaining ghost layers as the reference does after its loop
-! (exchange(4), pbound, and calvel's exchange(1..3)), and keeps the
-! not-converged / BIG RMAX handling, boundu/v/w and the log lines.
+! (press.for:72-73).  After every device calvel the faces it computed come
+! back and the host runs calvel's own exchange(1..3) (press.for:139-141), so
+! the velocity ghost layers evolve exactly as in the reference's loop (its
+! exchanges carry data from call to call in their send/receive buffers);
+! su and p come back once, followed by exchange(4) and pbound.  The host
+! keeps the not-converged / BIG RMAX handling, boundu/v/w and the log
+! lines.
 !
 ! Scope: the multigrid with any of the reference's smoothers (mg_itrsch 1
 ! lexicographic GS, 2 line TDMA, 3 red-black GS), constant-coefficient or
@@ -42,6 +45,11 @@ subroutine newsolv_mg_h3d
   real(c_double) :: rloc
   logical :: converged
   double precision :: wtimedum
+  ! the host's velocity halo exchange after every device iteration (the
+  ! default: ghost layers bitwise the reference loop's), or once after the
+  ! loop (H3DMG_VELOCITY_HALO=final: one download instead of one per
+  ! iteration; every face of the interior rows is still the reference's)
+  logical, save :: every_halo = .true.
 
   if (.not. c_associated(h)) call setup()
   st = h3dmg_set_dt(h, dt)
@@ -66,6 +74,7 @@ subroutine newsolv_mg_h3d
   st = h3dmg_velocity_begin(h, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
                             c_null_ptr, alfapr, rloc)
   call chk(st, 'velocity_begin')
+  if (every_halo) call velocity_halo()
   call MPI_ALLREDUCE(rloc, rmax, 1, MPI_DOUBLE_PRECISION, MPI_MAX, MPI_COMM_WORLD, ierr)
 
   ! newsolv_mg.for:24-59
@@ -73,6 +82,7 @@ subroutine newsolv_mg_h3d
   do iter = 1, maxcy
      st = h3dmg_velocity_step(h, rloc)
      call chk(st, 'velocity_step')
+     if (every_halo) call velocity_halo()
      call MPI_ALLREDUCE(rloc, rmax, 1, MPI_DOUBLE_PRECISION, MPI_MAX, MPI_COMM_WORLD, ierr)
      if (rmax < eps .and. iter > 1) then
         converged = .true.
@@ -84,8 +94,6 @@ subroutine newsolv_mg_h3d
      associate (d => dom(ib))
        st = h3dmg_get_p(h, ib - 1, d%p, d%ttc_i, d%ttc_j, d%isp - 1)
        call chk(st, 'get_p')
-       st = h3dmg_get_velocity(h, ib - 1, d%u, d%v, d%w, d%ttc_i, d%ttc_j, d%isp - 1)
-       call chk(st, 'get_velocity')
        if (L_LSM) then
           st = h3dmg_get_su(h, ib - 1, d%sup, d%ttc_i, d%ttc_j, d%isp - 1)
        else
@@ -96,9 +104,7 @@ subroutine newsolv_mg_h3d
   end do
   call exchange(4)
   call pbound
-  call exchange(1)
-  call exchange(2)
-  call exchange(3)
+  if (.not. every_halo) call velocity_halo()
 
   if (.not. converged) then
      if (myrank == 0) print *, 'not converged!! ', maxcy, rmax
@@ -170,6 +176,8 @@ contains
     prm%tol = eps
     prm%dt = dt
     prm%min_cycles = 2
+    call get_environment_variable('H3DMG_VELOCITY_HALO', pv, plen, pst)
+    every_halo = .not. (pst == 0 .and. pv(1:5) == 'final')
     call get_environment_variable('H3DMG_PRECISION', pv, plen, pst)
     if (pst == 0 .and. pv(1:5) == 'mixed' .and. .not. L_LSM .and. mg_itrsch == 3) then
        prm%precision = H3DMG_MIXED
@@ -178,6 +186,23 @@ contains
     st = h3dmg_create(blk, int(nbp, c_int), prm, h)
     call chk(st, 'create')
   end subroutine setup
+
+  ! u, v, w of the last device calvel on the faces calvel computes, then the
+  ! exchange(1..3) that ends the reference's calvel (press.for:139-141): the
+  ! ghost layers evolve as in the reference's loop, whose exchanges also
+  ! carry data between calls in their send/receive buffers
+  subroutine velocity_halo()
+    integer :: b
+    do b = 1, nbp
+       associate (d => dom(b))
+         st = h3dmg_get_velocity_calvel(h, b - 1, d%u, d%v, d%w, d%ttc_i, d%ttc_j, d%isp - 1)
+         call chk(st, 'get_velocity_calvel')
+       end associate
+    end do
+    call exchange(1)
+    call exchange(2)
+    call exchange(3)
+  end subroutine velocity_halo
 
   ! face f (1..6) of block b: `across` is the neighbour's domain number
   ! (-1: none), `opposite` the neighbour on the other face of the axis,

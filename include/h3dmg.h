@@ -248,6 +248,14 @@ h3dmg_status h3dmg_set_velocity(h3dmg_handle *h, int b, const double *ustar, con
 h3dmg_status h3dmg_get_velocity(h3dmg_handle *h, int b, double *u, double *v, double *w, int ld1,
                                 int ld2, int off);
 h3dmg_status h3dmg_get_su(h3dmg_handle *h, int b, double *su, int ld1, int ld2, int off);
+/* The faces calvel itself computes (press.for:90-131 over isu..ieu, jsv..jev,
+ * ksw..kew, localparameters.for:106-192: faces 1..n of an axis, 1..n-1
+ * against a wall at the far end), the rest of the host arrays untouched --
+ * the host state after the reference's calvel loops and before its
+ * exchange(1..3).  A host that runs exchange(1..3) after each such copy
+ * reproduces the ghost layers the reference's loop leaves. */
+h3dmg_status h3dmg_get_velocity_calvel(h3dmg_handle *h, int b, double *u, double *v, double *w,
+                                       int ld1, int ld2, int off);
 
 /* Per-block results of the last solve: cycles run and final relative
  * residual (arrays of nblocks). */
