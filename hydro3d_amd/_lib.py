@@ -29,6 +29,14 @@ class H3dBlock(ctypes.Structure):
                 ("nlevels", ctypes.c_int), ("bc", ctypes.c_int * 6), ("nbr", ctypes.c_int * 6)]
 
 
+TECBIN_FIELDS = ("p", "pm", "ppm", "vis", "vism", "u", "um", "uum", "v", "vm", "vvm", "w", "wm",
+                 "wwm", "uvm", "uwm", "vwm", "ksgs", "eps", "epsm")
+
+
+class H3dTecbinHeader(ctypes.Structure):
+    _fields_ = [("tt", ctypes.c_int * 3), ("pl", ctypes.c_int), ("ind", ctypes.c_int * 3)]
+
+
 class H3dParams(ctypes.Structure):
     _fields_ = [("smoother", ctypes.c_int), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int),
                 ("maxcy", ctypes.c_int), ("criterion", ctypes.c_int), ("tol", ctypes.c_double),
@@ -91,6 +99,10 @@ SIGNATURES = {
     "h3dmg_set_dt": (_i, [_vp, _d]),
     "h3dmg_set_density": (_i, [_vp, _i, _dp, _i, _i, _i]),
     "h3dmg_set_density_device": (_i, [_vp, _i, _vp]),
+    "h3dmg_tecbin_write": (_i, [ctypes.c_char_p, _P(H3dTecbinHeader), _dp, _dp, _dp, _P(_dp)]),
+    "h3dmg_tecbin_read_header": (_i, [ctypes.c_char_p, _P(H3dTecbinHeader)]),
+    "h3dmg_tecbin_read": (_i, [ctypes.c_char_p, _i, _dp, _dp, _dp, _P(_dp)]),
+    "h3dmg_tecbin_error": (ctypes.c_char_p, []),
     "h3dmg_dbg_put": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_get": (_i, [_vp, _i, _i, _i, _vp]),
     "h3dmg_dbg_relax": (_i, [_vp, _i, _i]),

@@ -20,7 +20,7 @@ LIB = os.path.join(PKG, "libh3dmg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
-HIP_SOURCES = ["h3dmg_kernels.hip", "h3dmg_runtime.hip", "h3dmg_layout.cpp"]
+HIP_SOURCES = ["h3dmg_kernels.hip", "h3dmg_runtime.hip", "h3dmg_layout.cpp", "h3dmg_tecbin.cpp"]
 
 # -ffp-contract=off: the FP64 path must execute the reference's products and
 # sums as written (no FMA contraction) to stay bitwise equal to it.

@@ -341,6 +341,37 @@ h3dmg_status h3dmg_layout_blocks(const h3dmg_layout *L, int rank, const double *
                                  const int *bc_codes, int nlevels, h3dmg_block *blocks, int cap,
                                  int *nblocks, int *dom_ids);
 
+/* ---- tecbin restart files (host only) ------------------------------------
+ * The binary restart file of one block: written by tecbin
+ * (src_v70/post.for:324-375), read back for a warm start by the restart
+ * branch of src_v70/initial.for:715-747.  Fortran unformatted sequential
+ * records (4-byte length, payload, 4-byte length): the dimensions
+ * ttc_i, ttc_j, ttc_k; pl; inind, jnind, knind (-1 where the block's east,
+ * north or top face is a non-periodic wall); then one record of 23 doubles
+ * per cell (k slowest, i fastest): x(i), y(j), z(k) and the 20 fields below
+ * at (i, j, k).  Field arrays are Fortran (ttc_i, ttc_j, ttc_k), ghost
+ * layers included; x, y, z are the cell coordinates along each axis. */
+enum {
+  H3DMG_TECBIN_P = 0, H3DMG_TECBIN_PM, H3DMG_TECBIN_PPM, H3DMG_TECBIN_VIS, H3DMG_TECBIN_VISM,
+  H3DMG_TECBIN_U, H3DMG_TECBIN_UM, H3DMG_TECBIN_UUM, H3DMG_TECBIN_V, H3DMG_TECBIN_VM,
+  H3DMG_TECBIN_VVM, H3DMG_TECBIN_W, H3DMG_TECBIN_WM, H3DMG_TECBIN_WWM, H3DMG_TECBIN_UVM,
+  H3DMG_TECBIN_UWM, H3DMG_TECBIN_VWM, H3DMG_TECBIN_KSGS, H3DMG_TECBIN_EPS, H3DMG_TECBIN_EPSM,
+  H3DMG_TECBIN_FIELDS
+};
+typedef struct {
+  int tt[3];  /* ttc_i, ttc_j, ttc_k */
+  int pl;     /* ghost layers */
+  int ind[3]; /* inind, jnind, knind */
+} h3dmg_tecbin_header;
+h3dmg_status h3dmg_tecbin_write(const char *path, const h3dmg_tecbin_header *hd, const double *x,
+                                const double *y, const double *z, const double *const *fields);
+h3dmg_status h3dmg_tecbin_read_header(const char *path, h3dmg_tecbin_header *hd);
+/* pl >= 0: refuse a file with another ghost-layer count (initial.for:723-728
+ * stops); x, y, z, fields and any fields[q] may be null (not read out) */
+h3dmg_status h3dmg_tecbin_read(const char *path, int pl, double *x, double *y, double *z,
+                               double *const *fields);
+const char *h3dmg_tecbin_error(void);
+
 /* ---- test seams: one operator at a time on level arrays ------------------
  * Level arrays cross this seam as dense device arrays in the layout of the
  * oracle: phi padded (n+2)^3 i-fastest, rhs interior.  Ghost cells of an
