@@ -77,6 +77,7 @@ SIGNATURES = {
     "h3dmg_set_block_tol": (_i, [_vp, _i, _d]),
     "h3dmg_set_block_enabled": (_i, [_vp, _i, _i]),
     "h3dmg_project_device": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _dp]),
+    "h3dmg_project_device_noalloc": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp]),
     "h3dmg_layout_read": (_i, [ctypes.c_char_p, ctypes.c_char_p, _P(_vp)]),
     "h3dmg_layout_destroy": (None, [_vp]),
     "h3dmg_layout_error": (ctypes.c_char_p, []),

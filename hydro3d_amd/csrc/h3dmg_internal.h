@@ -218,6 +218,8 @@ int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const doubl
                   const double *w, double *su, int to_fine, double dt, double dx, double dy,
                   double dz, RedPartial *partials, size_t cells);
 
+// *out = max over n partials of RedPartial::amax (one workgroup)
+void launch_partials_max(hipStream_t s, const RedPartial *parts, int n, double *out);
 // workgroups launch_calmas uses for `cells` cells
 inline int calmas_groups(size_t cells) {
   return (int)((cells + 255) / 256 < (size_t)CALMAS_MAX_GROUPS ? (cells + 255) / 256

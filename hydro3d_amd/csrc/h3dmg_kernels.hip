@@ -2242,6 +2242,16 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) partials[blockIdx.x] = RedPartial{0.0, am};
 }
 
+// max of the `amax` fields of n partials into *out (one workgroup; a max is
+// exact in any order)
+__global__ void __launch_bounds__(256)
+    k_partials_max(const RedPartial *__restrict__ parts, int n, double *__restrict__ out) {
+  double ss = 0.0, am = 0.0;
+  for (int q = (int)threadIdx.x; q < n; q += 256) am = fmax(am, parts[q].amax);
+  block_reduce(ss, am);
+  if (threadIdx.x == 0) *out = am;
+}
+
 // ---- layout conversion ---------------------------------------------------
 __global__ void k_pack_fine(const FineDev *__restrict__ fd, int which,
                             const double *__restrict__ src, int ni, int nj, int nk) {
@@ -3078,6 +3088,10 @@ int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const doubl
   hipLaunchKernelGGL(k_calmas, dim3(grid), dim3(256), 0, s, fd, u, v, w, su, to_fine, dt, dx, dy,
                      dz, partials);
   return grid;
+}
+
+void launch_partials_max(hipStream_t s, const RedPartial *parts, int n, double *out) {
+  hipLaunchKernelGGL(k_partials_max, dim3(1), dim3(256), 0, s, parts, n, out);
 }
 
 void launch_pack_fine(hipStream_t s, const FineDev *fd, int which, const double *src, int ni,

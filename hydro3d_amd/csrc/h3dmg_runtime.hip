@@ -48,6 +48,46 @@ h3dmg_status fail(h3dmg_status st, const char *fmt, ...) {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+}  // namespace
+
+// ---- scratch of the allocation-free projection ----------------------------
+// calmas' per-workgroup partial maxima of h3dmg_project_device_noalloc live
+// in this file-scope device array (one copy per GPU, held until the process
+// ends) instead of a handle's arena.  Its name is a host-side stand-in, not a
+// device address: host code never passes `scratch` to a kernel or a copy.
+// Each device's address is looked up once, with that device current
+// (hipGetSymbolAddress), and kept in g_scratch_ptr.
+constexpr int NOALLOC_PARTIALS = CALMAS_MAX_GROUPS;  // 16 KiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) RedPartial scratch[NOALLOC_PARTIALS];
+
+namespace {
+std::mutex g_scratch_mutex;
+RedPartial *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// the device address of `scratch` on device `dev`
+h3dmg_status scratch_on(int dev, RedPartial **out) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return fail(H3DMG_ERR_UNSUPPORTED, "device %d: beyond the %d devices the scratch table holds",
+                dev, NOALLOC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != dev) HIP_TRY(hipSetDevice(dev));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) HIP_TRY(hipSetDevice(cur));
+    if (e != hipSuccess)
+      return fail(H3DMG_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    if (!ptr) return fail(H3DMG_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = (RedPartial *)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return H3DMG_OK;
+}
+
 struct TileSet {
   std::vector<Tile> h;
   Tile *d = nullptr;
@@ -1843,6 +1883,43 @@ h3dmg_status h3dmg_project_device(h3dmg_handle *h, int b, const double *d_ustar,
   } else {
     HIP_TRY(hipStreamSynchronize(h->stream));
   }
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_project_device_noalloc(h3dmg_handle *h, int b, const double *d_ustar,
+                                          const double *d_vstar, const double *d_wstar,
+                                          double *d_u, double *d_v, double *d_w, double *d_su,
+                                          double alfapr, double *d_rmax) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (h->connected)
+    return fail(H3DMG_ERR_UNSUPPORTED,
+                "one block of blocks with neighbours: use the velocity loop (its faces are "
+                "shared)");
+  if (!d_ustar || !d_vstar || !d_wstar || !d_u || !d_v || !d_w)
+    return fail(H3DMG_ERR_ARG, "null velocity array");
+  const FineDev &F = h->fine[b];
+  const BlockInfo &B = h->blk[b];
+  const double dt = h->prm.dt;
+  if (!(dt > 0)) return fail(H3DMG_ERR_ARG, "projection needs params.dt > 0");
+  const size_t ni = F.ni, nj = F.nj, nk = F.nk;
+  const int groups = calmas_groups(ni * nj * nk);
+  if (groups > NOALLOC_PARTIALS)
+    return fail(H3DMG_ERR_ARG, "block %d: %d calmas partials exceed the %d the scratch holds", b,
+                groups, NOALLOC_PARTIALS);
+  RedPartial *parts = nullptr;
+  st = scratch_on(h->device, &parts);
+  if (st) return st;
+  const double dta = dt * alfapr;
+  launch_calvel(h->stream, h->d_fine + b, 0, d_ustar, d_u, dta, B.spec.dx, (ni + 1) * nj * nk);
+  launch_calvel(h->stream, h->d_fine + b, 1, d_vstar, d_v, dta, B.spec.dy, ni * (nj + 1) * nk);
+  launch_calvel(h->stream, h->d_fine + b, 2, d_wstar, d_w, dta, B.spec.dz, ni * nj * (nk + 1));
+  // every one of the `groups` partials is written before it is read: nothing
+  // in the scratch needs clearing
+  launch_calmas(h->stream, h->d_fine + b, d_u, d_v, d_w, d_su, 0, dt, B.spec.dx, B.spec.dy,
+                B.spec.dz, parts, ni * nj * nk);
+  if (d_rmax) launch_partials_max(h->stream, parts, groups, d_rmax);
+  HIP_TRY(hipGetLastError());
   return H3DMG_OK;
 }
 

@@ -24,7 +24,8 @@
  * Threading: a process may hold any number of handles and drive them from
  * different host threads (each handle launches on its own stream); calls on
  * one handle must not overlap.  All calls are synchronous with respect to
- * the host.
+ * the host, except h3dmg_project_device_noalloc, which only enqueues work on
+ * the handle's stream.
  */
 #ifndef H3DMG_H
 #define H3DMG_H
@@ -183,6 +184,22 @@ h3dmg_status h3dmg_project_device(h3dmg_handle *h, int b, const double *d_ustar,
                                   const double *d_vstar, const double *d_wstar, double *d_u,
                                   double *d_v, double *d_w, double *d_su, double alfapr,
                                   double *rmax);
+
+/* h3dmg_project_device without any allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  Same arrays and the same arithmetic as
+ * h3dmg_project_device, but it only enqueues work on the handle's stream
+ * (no host synchronisation, no hipMalloc, no host buffers) and writes rmax
+ * into the device scalar d_rmax (may be null).  calmas' partial maxima go to
+ * a 16 KiB static device array per GPU instead of the handle: at most 1024
+ * partials, one per calmas workgroup (calmas caps its grid there; a block
+ * that would need more returns H3DMG_ERR_ARG).  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (two streams, two threads) race
+ * and must be serialised by the caller. */
+h3dmg_status h3dmg_project_device_noalloc(h3dmg_handle *h, int b, const double *d_ustar,
+                                          const double *d_vstar, const double *d_wstar,
+                                          double *d_u, double *d_v, double *d_w, double *d_su,
+                                          double alfapr, double *d_rmax);
 
 /* The velocity form of the outer loop on the device, newsolv_mg
  * (src_v70/newsolv_mg.for:17-59) for every enabled block of the handle:
