@@ -404,6 +404,9 @@ def main():
     ap.add_argument("--les-steps", type=int, default=50,
                     help="time steps of the 192^3 LES projection workload (0: skip)")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "hydro3d_amd", "tuned.json"))
+    ap.add_argument("--solves-only", choices=("mixed", "fp64"),
+                    help="only time the 256^3 solves of this precision (profiling) and print "
+                         "their JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver is single-GPU (independent blocks would "
@@ -428,6 +431,16 @@ def main():
     N = int(np.prod(c["n"]))
     f_dev = torch.from_numpy(c["f"]).cuda()
     zero_dev = torch.zeros_like(f_dev)
+    if args.solves_only:
+        from hydro3d_amd import autotune as AT
+        mixed = args.solves_only == "mixed"
+        with MultigridSolver([spec], tol=1e-8, precision=args.solves_only,
+                             coarse="exact" if mixed else "smooth") as S:
+            AT.apply_cached(S, args.tune_cache)
+            lat, cyc, fin = timed_solves(S, f_dev, zero_dev, args.steps, args.warmup)
+        print(json.dumps({"solves_only": args.solves_only, "cycles": cyc, "final_rel_res": fin,
+                          "ms_per_solve": [round(x * 1e3, 3) for x in lat]}), flush=True)
+        return
 
     log("building solvers")
     from hydro3d_amd import autotune as AT
