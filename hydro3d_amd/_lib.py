@@ -98,6 +98,7 @@ SIGNATURES = {
     "h3dmg_get_su": (_i, [_vp, _i, _dp, _i, _i, _i]),
     "h3dmg_get_velocity_calvel": (_i, [_vp, _i, _dp, _dp, _dp, _i, _i, _i]),
     "h3dmg_set_dt": (_i, [_vp, _d]),
+    "h3dmg_set_differencing": (_i, [_vp, _i]),
     "h3dmg_set_density": (_i, [_vp, _i, _dp, _i, _i, _i]),
     "h3dmg_set_density_device": (_i, [_vp, _i, _vp]),
     "h3dmg_tecbin_write": (_i, [ctypes.c_char_p, _P(H3dTecbinHeader), _dp, _dp, _dp, _P(_dp)]),

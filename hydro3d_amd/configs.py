@@ -52,6 +52,27 @@ def face_velocity(n, bc, seed, smooth_amp=0.0, h=None):
     return u, v, w
 
 
+def extend_faces(u, v, w, bc, seed):
+    """The face arrays with one more face at each end of their own axis
+    (faces -1..n+1: the second ghost faces calmas' 4th-order branch reads,
+    src_v70/press.for:17-37): a periodic axis wraps (face -1 = face n-1,
+    face n+1 = face 1); across a wall the outer faces are seeded N(0,1)
+    values (the host velocity there is whatever the LES left in it)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for arr, ax, lo in ((u, 2, bc[0]), (v, 1, bc[2]), (w, 0, bc[4])):
+        n = arr.shape[ax] - 1
+        if lo == BC_PERIODIC:
+            first = np.take(arr, [n - 1], axis=ax)
+            last = np.take(arr, [1], axis=ax)
+        else:
+            shp = list(arr.shape)
+            shp[ax] = 1
+            first, last = rng.standard_normal(shp), rng.standard_normal(shp)
+        out.append(np.ascontiguousarray(np.concatenate([first, arr, last], axis=ax)))
+    return tuple(out)
+
+
 def divergence_rhs(u, v, w, h, dt, remove_mean=True):
     """f = div(u)/dt in the evaluation order of calmas (press.for:50-58)."""
     dx, dy, dz = h

@@ -226,6 +226,23 @@ inline int calmas_groups(size_t cells) {
                                                                 : (size_t)CALMAS_MAX_GROUPS);
 }
 
+// calmas' 4th-order branch (press.for:17-37) reads faces -1 and n+1 of
+// every axis.  For axis a, side s (0: face -1, 1: face n+1), entry 2a+s:
+// n > 0: src is the dense face array of the block across that face (itself
+// on an axis it spans periodically) with n cells along a, whose face n-1
+// (s = 0) or face 1 (s = 1) is the value -- what exchange(1..3) puts in the
+// second ghost layer; n == 0: src is this block's plane of host values at
+// that face (a wall), indexed like a face of the dense arrays with the
+// face coordinate dropped.
+struct Outer4 {
+  const double *src[6];
+  int n[6];
+};
+// calmas with the 4th-order divergence: same outputs as launch_calmas
+int launch_calmas4(hipStream_t s, const FineDev *fd, const double *u, const double *v,
+                   const double *w, double *su, int to_fine, double dt, double dx, double dy,
+                   double dz, RedPartial *partials, size_t cells, const Outer4 &outer);
+
 // layout conversion: dense interior (i fastest) <-> colour split
 void launch_pack_fine(hipStream_t s, const FineDev *fd, int which, const double *src, int ni,
                       int nj, int nk);

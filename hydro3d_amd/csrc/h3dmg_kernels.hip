@@ -2252,6 +2252,62 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) *out = am;
 }
 
+// calmas, differencing = 2 (press.for:17-37): per axis
+//   (((-u(i+1) + 27 u(i)) - 27 u(i-1)) + u(i-2)) / (24 dx)
+// in the reference's order (u(i) the east face of cell i: dense faces
+// c+2, c+1, c, c-1 of cell c), the three axes summed x + y, then + z;
+// su = fact/dt, rmax from |fact*dx*dy*dz| as in k_calmas.  Faces -1 and
+// n+1 come from Outer4.
+__global__ void __launch_bounds__(256)
+    k_calmas4(const FineDev *__restrict__ fd, const double *__restrict__ u,
+              const double *__restrict__ v, const double *__restrict__ w, double *__restrict__ su,
+              int to_fine, double dt, double dx, double dy, double dz,
+              RedPartial *__restrict__ partials, Outer4 o) {
+  const FineDev &F = *fd;
+  const int ni = F.ni, nj = F.nj, nk = F.nk;
+  const size_t total = (size_t)ni * nj * nk;
+  // face f of axis 0 at (j, k): own array for 0..ni, else the outer source
+  auto uface = [&](int f, int j, int k) -> double {
+    if (f >= 0 && f <= ni) return u[((size_t)k * nj + j) * (ni + 1) + f];
+    const int s = f < 0 ? 0 : 1, m = o.n[s];
+    if (m == 0) return o.src[s][(size_t)k * nj + j];
+    return o.src[s][((size_t)k * nj + j) * (m + 1) + (s ? 1 : m - 1)];
+  };
+  auto vface = [&](int f, int i, int k) -> double {
+    if (f >= 0 && f <= nj) return v[((size_t)k * (nj + 1) + f) * ni + i];
+    const int s = f < 0 ? 0 : 1, m = o.n[2 + s];
+    if (m == 0) return o.src[2 + s][(size_t)k * ni + i];
+    return o.src[2 + s][((size_t)k * (m + 1) + (s ? 1 : m - 1)) * ni + i];
+  };
+  auto wface = [&](int f, int i, int j) -> double {
+    if (f >= 0 && f <= nk) return w[((size_t)f * nj + j) * ni + i];
+    const int s = f < 0 ? 0 : 1, m = o.n[4 + s];
+    if (m == 0) return o.src[4 + s][(size_t)j * ni + i];
+    return o.src[4 + s][((size_t)(s ? 1 : m - 1) * nj + j) * ni + i];
+  };
+  double am = 0.0, ss = 0.0;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % ni), j = (int)((q / ni) % nj), k = (int)(q / ((size_t)ni * nj));
+    const double dux = ((-uface(i + 2, j, k) + 27.0 * uface(i + 1, j, k)) - 27.0 * uface(i, j, k)) +
+                       uface(i - 1, j, k);
+    const double dvy = ((-vface(j + 2, i, k) + 27.0 * vface(j + 1, i, k)) - 27.0 * vface(j, i, k)) +
+                       vface(j - 1, i, k);
+    const double dwz = ((-wface(k + 2, i, j) + 27.0 * wface(k + 1, i, j)) - 27.0 * wface(k, i, j)) +
+                       wface(k - 1, i, j);
+    const double fact = (dux / (24.0 * dx) + dvy / (24.0 * dy)) + dwz / (24.0 * dz);
+    const double rhs = fact / dt;
+    if (to_fine)
+      F.f[(i + j + k) & 1][cidx(F.H, F.nj, i, j, k)] = rhs;
+    else if (su)
+      su[q] = rhs;
+    const double fact1 = fact * dx * dy * dz;
+    am = fmax(am, fabs(fact1));
+  }
+  block_reduce(ss, am);
+  if (threadIdx.x == 0) partials[blockIdx.x] = RedPartial{0.0, am};
+}
+
 // ---- layout conversion ---------------------------------------------------
 __global__ void k_pack_fine(const FineDev *__restrict__ fd, int which,
                             const double *__restrict__ src, int ni, int nj, int nk) {
@@ -3092,6 +3148,15 @@ int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const doubl
 
 void launch_partials_max(hipStream_t s, const RedPartial *parts, int n, double *out) {
   hipLaunchKernelGGL(k_partials_max, dim3(1), dim3(256), 0, s, parts, n, out);
+}
+
+int launch_calmas4(hipStream_t s, const FineDev *fd, const double *u, const double *v,
+                   const double *w, double *su, int to_fine, double dt, double dx, double dy,
+                   double dz, RedPartial *partials, size_t cells, const Outer4 &outer) {
+  const int grid = calmas_groups(cells);
+  hipLaunchKernelGGL(k_calmas4, dim3(grid), dim3(256), 0, s, fd, u, v, w, su, to_fine, dt, dx, dy,
+                     dz, partials, outer);
+  return grid;
 }
 
 void launch_pack_fine(hipStream_t s, const FineDev *fd, int which, const double *src, int ni,

@@ -276,8 +276,15 @@ struct h3dmg_handle {
   // handle-owned velocity arrays (h3dmg_set_velocity from host arrays):
   // per block u*, v*, w*, u, v, w, dense, in one allocation
   double *vel_store = nullptr;
-  std::vector<size_t> vel_off;   // 6 offsets (doubles) per block
+  // 9 offsets (doubles) per block: u*, v*, w*, u, v, w, and per axis the two
+  // outer face planes (faces -1 and n+1: the second ghost faces of calmas'
+  // 4th-order branch where the axis ends in a wall), west plane first
+  std::vector<size_t> vel_off;
   std::vector<char> vel_set;     // block's u*, v*, w* uploaded
+  std::vector<char> vel_outer;   // block's outer face planes uploaded
+  // calmas' branch (the reference's `differencing`): 1 second order
+  // (press.for:48-63), 2 fourth order (press.for:17-37)
+  int differencing = 1;
   double vel_dta = 0.0, vel_dt = 0.0;  // dt*alfapr and dt of the captured step
   bool vel_ready = false;
   hipGraph_t vgraph = nullptr;
@@ -1315,8 +1322,50 @@ void drop_graph(h3dmg_handle *h) {
 // calvel + calmas (press.for:77-144, 2-75) of every enabled block with the
 // current pressure: u, v, w from u*, v*, w*; su into the blocks' f arrays
 // (the next V-cycle's rhs); per-block max |div u| dx dy dz into d_stats.
+// Where calmas' 4th-order stencil finds faces -1 and n+1 of block b (the
+// second ghost faces exchange(1..3) fills, press.for:17-37 and
+// exchange_bc/exchange_smlvl): the block across the face -- the neighbour,
+// or the block itself on an axis it spans periodically -- or, at a wall,
+// the host values uploaded by h3dmg_set_velocity.
+Outer4 outer_faces(const h3dmg_handle *h, int b) {
+  const BlockInfo &B = h->blk[b];
+  const size_t *o = &h->vel_off[9 * (size_t)b];
+  Outer4 r{};
+  for (int a = 0; a < 3; ++a) {
+    const size_t plane = a == 0 ? (size_t)B.spec.nj * B.spec.nk
+                         : a == 1 ? (size_t)B.spec.ni * B.spec.nk
+                                  : (size_t)B.spec.ni * B.spec.nj;
+    for (int side = 0; side < 2; ++side) {
+      const int f = 2 * a + side, k = B.spec.bc[f];
+      if (k == H3DMG_BC_WALL) {
+        r.src[f] = h->vel_store + o[6 + a] + side * plane;
+        r.n[f] = 0;
+        continue;
+      }
+      const int q = k == H3DMG_BC_NEIGHBOUR ? B.spec.nbr[f] : b;
+      const h3dmg_block &Q = h->blk[q].spec;
+      const h3dmg_handle::VelBlock &V = h->vel[q];
+      r.src[f] = a == 0 ? V.u : a == 1 ? V.v : V.w;
+      r.n[f] = a == 0 ? Q.ni : a == 1 ? Q.nj : Q.nk;
+    }
+  }
+  return r;
+}
+
+// calmas of block b into its rhs arrays, in the handle's branch
+void enqueue_calmas(h3dmg_handle *h, int b) {
+  const BlockInfo &B = h->blk[b];
+  const h3dmg_handle::VelBlock &V = h->vel[b];
+  const size_t cells = (size_t)B.spec.ni * B.spec.nj * B.spec.nk;
+  if (h->differencing == 2)
+    launch_calmas4(h->stream, h->d_fine + b, V.u, V.v, V.w, nullptr, 1, h->prm.dt, B.spec.dx,
+                   B.spec.dy, B.spec.dz, h->d_proj + h->proj_begin[b], cells, outer_faces(h, b));
+  else
+    launch_calmas(h->stream, h->d_fine + b, V.u, V.v, V.w, nullptr, 1, h->prm.dt, B.spec.dx,
+                  B.spec.dy, B.spec.dz, h->d_proj + h->proj_begin[b], cells);
+}
+
 void enqueue_project(h3dmg_handle *h) {
-  const double dt = h->prm.dt;
   if (h->connected) {
     // calvel on the faces each block computes, then exchange(1..3) of the
     // near faces from the blocks across them (press.for:77-144), then calmas
@@ -1346,13 +1395,7 @@ void enqueue_project(h3dmg_handle *h) {
         launch_face_copy(h->stream, a, src, nq[a], dst, n[a], nu, nv);
       }
     }
-    for (int b = 0; b < h->nb; ++b) {
-      const BlockInfo &B = h->blk[b];
-      const h3dmg_handle::VelBlock &V = h->vel[b];
-      launch_calmas(h->stream, h->d_fine + b, V.u, V.v, V.w, nullptr, 1, dt, B.spec.dx, B.spec.dy,
-                    B.spec.dz, h->d_proj + h->proj_begin[b],
-                    (size_t)B.spec.ni * B.spec.nj * B.spec.nk);
-    }
+    for (int b = 0; b < h->nb; ++b) enqueue_calmas(h, b);
     launch_finalize(h->stream, h->d_proj_begin, h->nb, h->d_proj, h->d_stats);
     return;
   }
@@ -1365,8 +1408,7 @@ void enqueue_project(h3dmg_handle *h) {
     launch_calvel(h->stream, fd, 0, V.us, V.u, h->vel_dta, B.spec.dx, (ni + 1) * nj * nk);
     launch_calvel(h->stream, fd, 1, V.vs, V.v, h->vel_dta, B.spec.dy, ni * (nj + 1) * nk);
     launch_calvel(h->stream, fd, 2, V.ws, V.w, h->vel_dta, B.spec.dz, ni * nj * (nk + 1));
-    launch_calmas(h->stream, fd, V.u, V.v, V.w, nullptr, 1, dt, B.spec.dx, B.spec.dy, B.spec.dz,
-                  h->d_proj + h->proj_begin[b], ni * nj * nk);
+    enqueue_calmas(h, b);
   }
   launch_finalize(h->stream, h->d_proj_begin, h->nb, h->d_proj, h->d_stats);
 }
@@ -1859,6 +1901,10 @@ h3dmg_status h3dmg_project_device(h3dmg_handle *h, int b, const double *d_ustar,
                 "shared)");
   if (!d_ustar || !d_vstar || !d_wstar || !d_u || !d_v || !d_w)
     return fail(H3DMG_ERR_ARG, "null velocity array");
+  if (h->differencing == 2)
+    return fail(H3DMG_ERR_UNSUPPORTED,
+                "differencing = 2: the projection of device arrays has no outer faces (use the "
+                "velocity loop with h3dmg_set_velocity)");
   const FineDev &F = h->fine[b];
   const BlockInfo &B = h->blk[b];
   const double dt = h->prm.dt;
@@ -1898,6 +1944,10 @@ h3dmg_status h3dmg_project_device_noalloc(h3dmg_handle *h, int b, const double *
                 "shared)");
   if (!d_ustar || !d_vstar || !d_wstar || !d_u || !d_v || !d_w)
     return fail(H3DMG_ERR_ARG, "null velocity array");
+  if (h->differencing == 2)
+    return fail(H3DMG_ERR_UNSUPPORTED,
+                "differencing = 2: the projection of device arrays has no outer faces (use the "
+                "velocity loop with h3dmg_set_velocity)");
   const FineDev &F = h->fine[b];
   const BlockInfo &B = h->blk[b];
   const double dt = h->prm.dt;
@@ -1929,15 +1979,17 @@ h3dmg_status h3dmg_project_device_noalloc(h3dmg_handle *h, int b, const double *
 namespace {
 h3dmg_status alloc_velocity(h3dmg_handle *h) {
   if (h->vel_store) return H3DMG_OK;
-  h->vel_off.assign(6 * (size_t)h->nb, 0);
+  h->vel_off.assign(9 * (size_t)h->nb, 0);
   size_t off = 0;
   for (int b = 0; b < h->nb; ++b) {
     const h3dmg_block &S = h->blk[b].spec;
     const size_t n[3] = {(size_t)(S.ni + 1) * S.nj * S.nk, (size_t)S.ni * (S.nj + 1) * S.nk,
                          (size_t)S.ni * S.nj * (S.nk + 1)};
-    for (int q = 0; q < 6; ++q) {
-      h->vel_off[6 * b + q] = off;
-      off += (n[q % 3] + 31) & ~(size_t)31;
+    const size_t planes[3] = {2 * (size_t)S.nj * S.nk, 2 * (size_t)S.ni * S.nk,
+                              2 * (size_t)S.ni * S.nj};
+    for (int q = 0; q < 9; ++q) {
+      h->vel_off[9 * b + q] = off;
+      off += ((q < 6 ? n[q % 3] : planes[q - 6]) + 31) & ~(size_t)31;
     }
   }
   hipError_t e = hipMalloc((void **)&h->vel_store, off * sizeof(double));
@@ -1947,6 +1999,7 @@ h3dmg_status alloc_velocity(h3dmg_handle *h) {
   }
   h->bytes += off * sizeof(double);
   h->vel_set.assign(h->nb, 0);
+  h->vel_outer.assign(h->nb, 0);
   return H3DMG_OK;
 }
 
@@ -2005,11 +2058,25 @@ h3dmg_status h3dmg_set_velocity(h3dmg_handle *h, int b, const double *ustar, con
           const double *src = corrected || !cur[axis] ? star[axis] : cur[axis];
           dense[dense_face(axis, f, a, c, S)] = src[face_index(axis, f, a, c, ld1, ld2, off)];
         }
-    HIP_TRY(hipMemcpyAsync(h->vel_store + h->vel_off[6 * b + axis], dense.data(),
+    HIP_TRY(hipMemcpyAsync(h->vel_store + h->vel_off[9 * b + axis], dense.data(),
                            dense.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    if (off >= 2) {
+      // faces -1 and n+1 (the host's current u there, as calmas reads it)
+      std::vector<double> planes(2 * (size_t)na * nc);
+      const double *src = cur[axis] ? cur[axis] : star[axis];
+      for (int side = 0; side < 2; ++side)
+        for (int c = 0; c < nc; ++c)
+          for (int a = 0; a < na; ++a)
+            planes[((size_t)side * nc + c) * na + a] =
+                src[face_index(axis, side ? n[axis] + 1 : -1, a, c, ld1, ld2, off)];
+      HIP_TRY(hipMemcpyAsync(h->vel_store + h->vel_off[9 * b + 6 + axis], planes.data(),
+                             planes.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
+    }
   }
   h->vel_set[b] = 1;
+  h->vel_outer[b] = off >= 2 ? 1 : 0;
   return H3DMG_OK;
 }
 
@@ -2037,7 +2104,7 @@ h3dmg_status velocity_to_host(h3dmg_handle *h, int b, double *u, double *v, doub
     const int f0 = calvel_only ? 1 : 0;
     const int f1 = calvel_only && !far_open ? n[axis] - 1 : n[axis];
     std::vector<double> dense((size_t)(n[axis] + 1) * na * nc);
-    HIP_TRY(hipMemcpyAsync(dense.data(), h->vel_store + h->vel_off[6 * b + 3 + axis],
+    HIP_TRY(hipMemcpyAsync(dense.data(), h->vel_store + h->vel_off[9 * b + 3 + axis],
                            dense.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     for (int c = 0; c < nc; ++c)
@@ -2090,6 +2157,10 @@ h3dmg_status enable_lsm(h3dmg_handle *h) {
   if (is_mixed(h) || h->prm.coarse != H3DMG_COARSE_SMOOTH)
     return fail(H3DMG_ERR_UNSUPPORTED,
                 "the variable-density operator runs in FP64 with the reference coarsest level");
+  if (h->differencing == 2)
+    return fail(H3DMG_ERR_UNSUPPORTED,
+                "the variable-density operator with differencing = 2 (the reference's 4th-order "
+                "calmas writes su, not sup)");
   std::lock_guard<std::mutex> lock(g_device_mutex);
   HIP_TRY(hipStreamSynchronize(h->stream));
   drop_graph(h);
@@ -2185,6 +2256,27 @@ h3dmg_status h3dmg_set_density(h3dmg_handle *h, int b, const double *dens, int l
   return h3dmg_set_density_device(h, b, h->d_stage);
 }
 
+h3dmg_status h3dmg_set_differencing(h3dmg_handle *h, int differencing) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  if (differencing != 1 && differencing != 2)
+    return fail(H3DMG_ERR_ARG, "differencing %d (1: 2nd order, 2: 4th order)", differencing);
+  if (differencing == 2 && h->lsm)
+    return fail(H3DMG_ERR_UNSUPPORTED,
+                "differencing = 2 with the variable-density operator (the reference's 4th-order "
+                "calmas writes su, not sup)");
+  if (h->differencing == differencing) return H3DMG_OK;
+  h->differencing = differencing;
+  // the captured velocity step launches the other calmas
+  std::lock_guard<std::mutex> lock(g_device_mutex);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->vgexec) (void)hipGraphExecDestroy(h->vgexec);
+  if (h->vgraph) (void)hipGraphDestroy(h->vgraph);
+  h->vgexec = nullptr;
+  h->vgraph = nullptr;
+  h->vel_ready = false;
+  return H3DMG_OK;
+}
+
 h3dmg_status h3dmg_set_dt(h3dmg_handle *h, double dt) {
   if (!h) return fail(H3DMG_ERR_ARG, "null handle");
   if (!(dt > 0)) return fail(H3DMG_ERR_ARG, "dt must be > 0");
@@ -2202,6 +2294,19 @@ h3dmg_status h3dmg_velocity_begin(h3dmg_handle *h, const double *const *d_ustar,
     return fail(H3DMG_ERR_ARG, "null velocity array list");
   if (!(h->prm.dt > 0)) return fail(H3DMG_ERR_ARG, "the velocity form needs params.dt > 0");
   if (h3dmg_status st = check_density(h)) return st;
+  if (h->differencing == 2) {
+    // the 4th-order stencil reads faces -1 and n+1: at walls they are host
+    // values that only h3dmg_set_velocity brings in
+    if (!own)
+      return fail(H3DMG_ERR_UNSUPPORTED,
+                  "differencing = 2 takes the velocity from h3dmg_set_velocity (host arrays with "
+                  "two ghost layers)");
+    for (int b = 0; b < h->nb; ++b)
+      if (h->blk[b].enabled && (!h->vel_store || !h->vel_outer[b]))
+        return fail(H3DMG_ERR_ARG,
+                    "block %d: differencing = 2 needs faces -1 and n+1 (host arrays with off >= 2)",
+                    b);
+  }
   bool same = h->vel_ready && h->vel_dta == h->prm.dt * alfapr && h->vel_dt == h->prm.dt;
   for (int b = 0; b < h->nb; ++b) {
     if (!h->blk[b].enabled) continue;
@@ -2209,7 +2314,7 @@ h3dmg_status h3dmg_velocity_begin(h3dmg_handle *h, const double *const *d_ustar,
     if (own) {
       if (!h->vel_store || !h->vel_set[b])
         return fail(H3DMG_ERR_ARG, "block %d: h3dmg_set_velocity has not run", b);
-      const size_t *o = &h->vel_off[6 * b];
+      const size_t *o = &h->vel_off[9 * b];
       double *S = h->vel_store;
       V = h3dmg_handle::VelBlock{S + o[0], S + o[1], S + o[2], S + o[3], S + o[4], S + o[5]};
     } else {

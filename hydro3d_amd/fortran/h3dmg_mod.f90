@@ -45,7 +45,7 @@ module h3dmg_mod
        h3dmg_set_block_tol, h3dmg_set_block_enabled, h3dmg_project_device, &
        h3dmg_velocity_begin, h3dmg_velocity_step, h3dmg_solve_velocity, &
        h3dmg_get_su_device, h3dmg_set_velocity, h3dmg_get_velocity, h3dmg_get_su, &
-       h3dmg_get_velocity_calvel, &
+       h3dmg_get_velocity_calvel, h3dmg_set_differencing, &
        h3dmg_set_dt, h3dmg_set_fused, h3dmg_get_fused, h3dmg_set_density, &
        h3dmg_error_message
 
@@ -207,6 +207,14 @@ module h3dmg_mod
        integer(c_int), value :: b, ld1, ld2, off
        real(c_double), intent(inout) :: u(*), v(*), w(*)
      end function h3dmg_get_velocity
+
+     ! calmas' branch: the reference's differencing (1: 2nd order, 2: 4th)
+     integer(c_int) function h3dmg_set_differencing(handle, differencing) &
+          bind(C, name="h3dmg_set_differencing")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: handle
+       integer(c_int), value :: differencing
+     end function h3dmg_set_differencing
 
      ! the faces calvel computes (isu..ieu, jsv..jev, ksw..kew); the host's
      ! exchange(1..3) fills the others, as after the reference's calvel

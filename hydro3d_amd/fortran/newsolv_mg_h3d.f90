@@ -25,9 +25,10 @@
 ! independent blocks; and the red-black multigrid with constant coefficients
 ! on the blocks of a rank that share faces (same-level neighbours and
 ! periodic partners; the halo exchange is the device reading the
-! neighbour's cells), one MPI rank per GPU.  Anything else -- a neighbour on
-! another rank, LMR refinement, calmas' 4th-order branch -- stops with a
-! message rather than running a different algorithm.  The precision is FP64 (bitwise the reference loop)
+! neighbour's cells), one MPI rank per GPU; calmas in either branch
+! (differencing 1 or 2, press.for:17-63).  Anything else -- a neighbour on
+! another rank, LMR refinement, the 4th-order branch under L_LSM -- stops
+! with a message rather than running a different algorithm.  The precision is FP64 (bitwise the reference loop)
 ! unless the environment variable H3DMG_PRECISION=mixed selects the FP32
 ! V-cycle with the exact coarsest solve inside the FP64 velocity loop
 ! (constant coefficients and red-black GS only).
@@ -140,9 +141,6 @@ contains
     character(len=16) :: pv
     integer :: b, plen, pst
     allocate (blk(nbp))
-    ! calmas' 4th-order divergence (press.for:17-37) is not on the device path
-    if (differencing == 2) &
-         call h3d_stop('differencing = 2 (4th-order calmas) is not on the device path')
     do b = 1, nbp
        associate (d => dom(b))
          ! every block coarsens to the global level count (no LMR levels)
@@ -185,6 +183,10 @@ contains
     end if
     st = h3dmg_create(blk, int(nbp, c_int), prm, h)
     call chk(st, 'create')
+    ! calmas' branch (press.for:17-37 or :48-63); the 4th-order stencil's
+    ! second ghost faces at walls come with h3dmg_set_velocity (pl >= 2)
+    st = h3dmg_set_differencing(h, int(differencing, c_int))
+    call chk(st, 'set_differencing')
   end subroutine setup
 
   ! u, v, w of the last device calvel on the faces calvel computes, then the

@@ -260,6 +260,11 @@ class MultigridSolver:
                                             ni + 2 * pl, nj + 2 * pl, pl))
         return out
 
+    def set_differencing(self, differencing):
+        """calmas' branch in the velocity loop: the reference's differencing,
+        1 (2nd order) or 2 (4th order, press.for:17-37)."""
+        L.check(self.lib.h3dmg_set_differencing(self.h, int(differencing)))
+
     def get_su_host(self, b, pl=2):
         ni, nj, nk = self.blocks[b]["n"]
         out = np.zeros((nk + 2 * pl, nj + 2 * pl, ni + 2 * pl))

@@ -242,6 +242,20 @@ h3dmg_status h3dmg_set_density(h3dmg_handle *h, int b, const double *dens, int l
                                int off);
 h3dmg_status h3dmg_set_density_device(h3dmg_handle *h, int b, const double *d_dens);
 
+/* calmas' branch in the velocity loop, the reference's `differencing`
+ * (control.cin): 1 the 2nd-order divergence (press.for:48-63, the default),
+ * 2 the 4th-order one (press.for:17-37),
+ *   ((-u(i+1) + 27 u(i) - 27 u(i-1) + u(i-2)) / (24 dx) + ... for v, w),
+ * which reads the second ghost faces u(isp-2), u(iep+1) (and v, w alike):
+ * across a shared face or a periodic wrap the current face of the block
+ * there (what exchange(1..3) copies), at a wall the host's value, which
+ * h3dmg_set_velocity takes from its arrays (they then need off >= 2).
+ * Differencing 2 needs the velocity of h3dmg_set_velocity, refuses the
+ * variable-density operator (whose sup the reference's 4th-order branch
+ * does not write), and makes h3dmg_project_device* return
+ * H3DMG_ERR_UNSUPPORTED. */
+h3dmg_status h3dmg_set_differencing(h3dmg_handle *h, int differencing);
+
 /* The time step of the next solve (params.dt): the reference's dt changes
  * from step to step (checkdt.for:117-137). */
 h3dmg_status h3dmg_set_dt(h3dmg_handle *h, double dt);

@@ -16,10 +16,12 @@ module mb_setup
   implicit none
 
   character(len=16)  :: mode
-  integer :: nlev, nu1, nu2, level, nsw, ncycles, bcs(6), scheme, pl_extra, early
+  integer :: nlev, nu1, nu2, level, nsw, ncycles, bcs(6), scheme, pl_extra, early, cds4
   double precision :: gdx(3), dtin, epsin
+  ! cds4 = 1: differencing = 2 (calmas' 4th-order divergence, press.for:17-37);
+  ! the velocity input then holds faces -1..n+1 of every axis per block
   namelist /job/ mode, nlev, nu1, nu2, level, nsw, ncycles, bcs, gdx, dtin, epsin, scheme, &
-       pl_extra, early
+       pl_extra, early, cds4
 
   integer :: uin, uout
   integer, allocatable :: nc(:,:)   ! cells per axis of every block
@@ -32,6 +34,7 @@ contains
     character(len=512) :: fjob, fin, fout
     mode = 'nbr'; nlev = 1; nu1 = 2; nu2 = 2; level = 1; nsw = 1; ncycles = 1
     bcs = 1; gdx = 1.0d0; dtin = 1.0d-3; epsin = 1.0d-6; scheme = 3; pl_extra = 1; early = 0
+    cds4 = 0
     call get_command_argument(1, fjob)
     call get_command_argument(2, fin)
     call get_command_argument(3, fout)
@@ -65,6 +68,7 @@ contains
     maxcy = ncycles; irestr = nu1; iproln = nu2; mg_itrsch = scheme
     solver = 2; L_LSM = .false.; L_LSMbase = .false.
     differencing = 1; dt = dtin; alfapr = 1.0d0; eps = epsin
+    if (cds4 /= 0) differencing = 2
     g_dx = gdx(1); g_dy = gdx(2); g_dz = gdx(3)
     bc_w = bcs(1); bc_e = bcs(2); bc_s = bcs(3); bc_n = bcs(4)
     bc_b = bcs(5); bc_t = bcs(6)
@@ -225,22 +229,25 @@ contains
   ! then exchange(4) and pbound, the state newsolv_mg starts from
   subroutine read_velocity_state()
     double precision, allocatable :: uf(:,:,:), vf(:,:,:), wf(:,:,:), p0(:,:,:)
-    integer :: b, i, j, k, isp, jsp, ksp, ni, nj, nk
+    integer :: b, i, j, k, isp, jsp, ksp, ni, nj, nk, fo
+    fo = 0
+    if (cds4 /= 0) fo = 1
     do b = 1, nbp
        ni = nc(1, b); nj = nc(2, b); nk = nc(3, b)
        isp = dom(b)%isp; jsp = dom(b)%jsp; ksp = dom(b)%ksp
-       allocate(uf(0:ni, nj, nk), vf(ni, 0:nj, nk), wf(ni, nj, 0:nk), p0(ni, nj, nk))
+       allocate(uf(-fo:ni+fo, nj, nk), vf(ni, -fo:nj+fo, nk), wf(ni, nj, -fo:nk+fo))
+       allocate(p0(ni, nj, nk))
        read(uin) uf
        read(uin) vf
        read(uin) wf
        read(uin) p0
-       do k = 1, nk; do j = 1, nj; do i = 0, ni
+       do k = 1, nk; do j = 1, nj; do i = -fo, ni + fo
           dom(b)%ustar(isp-1+i, jsp-1+j, ksp-1+k) = uf(i, j, k)
        end do; end do; end do
-       do k = 1, nk; do j = 0, nj; do i = 1, ni
+       do k = 1, nk; do j = -fo, nj + fo; do i = 1, ni
           dom(b)%vstar(isp-1+i, jsp-1+j, ksp-1+k) = vf(i, j, k)
        end do; end do; end do
-       do k = 0, nk; do j = 1, nj; do i = 1, ni
+       do k = -fo, nk + fo; do j = 1, nj; do i = 1, ni
           dom(b)%wstar(isp-1+i, jsp-1+j, ksp-1+k) = wf(i, j, k)
        end do; end do; end do
        dom(b)%u = dom(b)%ustar; dom(b)%v = dom(b)%vstar; dom(b)%w = dom(b)%wstar

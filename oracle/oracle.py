@@ -186,11 +186,12 @@ def ref_available() -> bool:
 
 
 def run_ref(block, mode, inputs, nout, nu1=2, nu2=2, level=1, nsweep=1, ncycles=1,
-            scheme=SCHEME_RB, dt=1e-3, eps=1e-6, timeout=600, lsm=False):
+            scheme=SCHEME_RB, dt=1e-3, eps=1e-6, timeout=600, lsm=False, cds4=False):
     """Run the reference driver; `inputs` is a list of arrays written in order
     (with lsm, the interior density comes first: the reference's L_LSM
-    variable-density operator); returns the flat float64 output (length nout,
-    or all if nout is None)."""
+    variable-density operator; with cds4, differencing = 2 and the velocity
+    faces of every axis run -1..n+1); returns the flat float64 output
+    (length nout, or all if nout is None)."""
     if not ref_available():
         raise RuntimeError(f"reference driver not built: {REF_DRIVER}")
     bcs = ",".join(str(_REF_BC[x]) for x in block.bc)
@@ -204,7 +205,8 @@ def run_ref(block, mode, inputs, nout, nu1=2, nu2=2, level=1, nsweep=1, ncycles=
                 f" mode='{mode}', ni={block.n[0]}, nj={block.n[1]}, nk={block.n[2]},\n"
                 f" nlev={block.nlev}, nu1={nu1}, nu2={nu2}, level={level}, nsw={nsweep},\n"
                 f" ncycles={ncycles}, bcs={bcs}, h={block.h[0]!r},{block.h[1]!r},{block.h[2]!r},\n"
-                f" dtin={dt!r}, epsin={eps!r}, scheme={scheme}, lsm={1 if lsm else 0}\n/\n")
+                f" dtin={dt!r}, epsin={eps!r}, scheme={scheme}, lsm={1 if lsm else 0},\n"
+                f" cds4={1 if cds4 else 0}\n/\n")
         with open(fin, "wb") as fh:
             for a in inputs:
                 fh.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
@@ -288,7 +290,7 @@ def dropin_available() -> bool:
 
 def run_multiblock_files(infodom, mdmap, mode, inputs, nout, gdx, bcs, nlev=1, nu1=2, nu2=2,
                          level=1, nsweep=1, ncycles=1, scheme=SCHEME_RB, dt=1e-3, eps=1e-6,
-                         timeout=600, early=0, driver=None):
+                         timeout=600, early=0, driver=None, cds4=False):
     """The same from the texts of infodom.cin and mdmap.cin (one rank).
     `driver` = DROPIN_DRIVER runs the product's drop-in newsolv_mg on the
     same module state instead of the reference's loop (mode 'newsolv')."""
@@ -306,7 +308,7 @@ def run_multiblock_files(infodom, mdmap, mode, inputs, nout, gdx, bcs, nlev=1, n
                      f" mode='{mode}', nlev={nlev}, nu1={nu1}, nu2={nu2}, level={level},\n"
                      f" nsw={nsweep}, ncycles={ncycles}, bcs={codes},\n"
                      f" gdx={gdx[0]!r},{gdx[1]!r},{gdx[2]!r}, dtin={dt!r}, epsin={eps!r},\n"
-                     f" scheme={scheme}, early={early}\n/\n")
+                     f" scheme={scheme}, early={early}, cds4={1 if cds4 else 0}\n/\n")
         with open(os.path.join(td, "in.bin"), "wb") as fh:
             for a in inputs:
                 fh.write(np.ascontiguousarray(a, dtype="<f8").tobytes())

@@ -28,6 +28,9 @@
 !             rejects) -> stop cycle, history(rmax, ||su||_2), final p
 !             (with ghosts), final su, su after the first calmas (the f of
 !             the pressure form).
+! cds4 = 1 sets differencing = 2 (calmas' 4th-order divergence,
+! press.for:17-37); newsolv then reads faces -1..n+1 of every axis (the
+! second ghost faces the 4th-order stencil reaches).
 ! lsm = 1 switches the reference's variable-density operator on (L_LSM):
 ! the density (interior n_i*n_j*n_k, ghosts set to images) is read first,
 ! coeff builds the density-weighted coefficients of every level
@@ -42,17 +45,17 @@ program ref_driver
   character(len=512) :: fjob, fin, fout
   character(len=16)  :: mode
   integer :: ni, nj, nk, nlev, nu1, nu2, level, nsw, ncycles, bcs(6)
-  integer :: scheme, pl_extra, lsm
+  integer :: scheme, pl_extra, lsm, cds4
   double precision :: h(3), dtin, epsin
   namelist /job/ mode, ni, nj, nk, nlev, nu1, nu2, level, nsw, ncycles, &
-       bcs, h, dtin, epsin, scheme, pl_extra, lsm
+       bcs, h, dtin, epsin, scheme, pl_extra, lsm, cds4
 
   integer :: uin, uout, ib, g
   double precision, allocatable :: a3(:,:,:), b3(:,:,:)
 
   mode = 'vcycle'; ni = 8; nj = 8; nk = 8; nlev = 1; nu1 = 2; nu2 = 2
   level = 1; nsw = 1; ncycles = 1; bcs = 1; h = 1.0d0; dtin = 1.0d-3
-  epsin = 1.0d-6; scheme = 3; pl_extra = 1; lsm = 0
+  epsin = 1.0d-6; scheme = 3; pl_extra = 1; lsm = 0; cds4 = 0
 
   call get_command_argument(1, fjob)
   call get_command_argument(2, fin)
@@ -115,6 +118,7 @@ contains
     maxcy = ncycles; irestr = nu1; iproln = nu2; mg_itrsch = scheme
     solver = 2; L_LSM = lsm /= 0; L_LSMbase = .false.
     differencing = 1; dt = dtin; alfapr = 1.0d0; eps = epsin
+    if (cds4 /= 0) differencing = 2
     bc_w = bcs(1); bc_e = bcs(2); bc_s = bcs(3); bc_n = bcs(4)
     bc_b = bcs(5); bc_t = bcs(6)
     PERIODIC = any(bcs == 5)
@@ -393,23 +397,28 @@ contains
   subroutine run_newsolv()
     double precision, allocatable :: uf(:,:,:), vf(:,:,:), wf(:,:,:), p0(:,:,:)
     double precision, allocatable :: hist(:,:), pout(:,:,:), sout(:,:,:), su0(:,:,:)
-    integer :: i, j, k, it, stop_at, isp, jsp, ksp
+    integer :: i, j, k, it, stop_at, isp, jsp, ksp, fo
     double precision :: ppref
 
     isp = dom(1)%isp; jsp = dom(1)%jsp; ksp = dom(1)%ksp
-    allocate(uf(0:ni, nj, nk), vf(ni, 0:nj, nk), wf(ni, nj, 0:nk), p0(ni, nj, nk))
+    ! cds4: faces -1..n+1 along the axis (the second ghost faces calmas'
+    ! 4th-order branch reads, press.for:17-37); else faces 0..n
+    fo = 0
+    if (cds4 /= 0) fo = 1
+    allocate(uf(-fo:ni+fo, nj, nk), vf(ni, -fo:nj+fo, nk), wf(ni, nj, -fo:nk+fo))
+    allocate(p0(ni, nj, nk))
     read(uin) uf
     read(uin) vf
     read(uin) wf
     read(uin) p0
     ! face f of cell i (east face) lives at index i+pl; face 0 at isp-1
-    do k = 1, nk; do j = 1, nj; do i = 0, ni
+    do k = 1, nk; do j = 1, nj; do i = -fo, ni + fo
        dom(1)%ustar(isp-1+i, jsp-1+j, ksp-1+k) = uf(i, j, k)
     end do; end do; end do
-    do k = 1, nk; do j = 0, nj; do i = 1, ni
+    do k = 1, nk; do j = -fo, nj + fo; do i = 1, ni
        dom(1)%vstar(isp-1+i, jsp-1+j, ksp-1+k) = vf(i, j, k)
     end do; end do; end do
-    do k = 0, nk; do j = 1, nj; do i = 1, ni
+    do k = -fo, nk + fo; do j = 1, nj; do i = 1, ni
        dom(1)%wstar(isp-1+i, jsp-1+j, ksp-1+k) = wf(i, j, k)
     end do; end do; end do
     dom(1)%u = dom(1)%ustar; dom(1)%v = dom(1)%vstar; dom(1)%w = dom(1)%wstar

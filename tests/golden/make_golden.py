@@ -115,6 +115,39 @@ def make_newsolv(name, cfg, ncycles, keep_p=True, scheme=O.SCHEME_RB):
     np.savez_compressed(os.path.join(HERE, f"newsolv_{name}.npz"), **d)
 
 
+def make_newsolv_cds4(name, cfg, ncycles, face_seed):
+    """The reference loop with differencing = 2 (calmas' 4th-order
+    divergence, press.for:17-37): velocity faces -1..n+1 per axis from
+    configs.extend_faces(u, v, w, bc, face_seed)."""
+    n, bc, L = cfg["n"], cfg["bc"], cfg["nlev"]
+    B = O.Block(n, cfg["h"], bc, L)
+    ni, nj, nk = n
+    npad = (ni + 2) * (nj + 2) * (nk + 2)
+    nint = ni * nj * nk
+    ue, ve, we = C.extend_faces(cfg["u"], cfg["v"], cfg["w"], bc, face_seed)
+    res = O.run_ref(B, "newsolv", [ue, ve, we, np.zeros((nk, nj, ni))],
+                    1 + 2 * (ncycles + 1) + npad + 2 * nint, ncycles=ncycles, dt=cfg["dt"],
+                    eps=1e-6, cds4=True)
+    q = 0
+    stop_at = int(res[q]); q += 1
+    hist = res[q:q + 2 * (ncycles + 1)].reshape(ncycles + 1, 2); q += 2 * (ncycles + 1)
+    p = res[q:q + npad].reshape(nk + 2, nj + 2, ni + 2); q += npad
+    su = res[q:q + nint].reshape(nk, nj, ni); q += nint
+    su0 = res[q:q + nint].reshape(nk, nj, ni)
+    np.savez_compressed(os.path.join(HERE, f"newsolv_{name}.npz"),
+                        n=np.array(n), bc=np.array(bc), nlev=L, h=np.array(cfg["h"]),
+                        dt=cfg["dt"], ncycles=ncycles, stop_at=stop_at, rmax=hist[:, 0],
+                        su_norm=hist[:, 1], p=p, su0=su0, final_su=su, face_seed=face_seed,
+                        vel_seed=cfg["vel_seed"])
+
+
+def cds4_case(n, bc, nlev, seed, dt=1e-3):
+    """Face velocities of an n = (ni, nj, nk) block on the unit cube."""
+    h = tuple(1.0 / x for x in n)
+    u, v, w = C.face_velocity(n, bc, seed, 0.0, h)
+    return dict(n=tuple(n), h=h, bc=tuple(bc), nlev=nlev, u=u, v=v, w=w, dt=dt, vel_seed=seed)
+
+
 def lsm_density(n, seed):
     """A positive density with a sharp two-phase interface (1 : 1000 across
     z = 0.5, as water under air in the LSM benchmarks) and log-normal
@@ -208,9 +241,22 @@ def make_lsm_newsolv(name, cfg, ncycles, scheme=O.SCHEME_RB):
                         dens=dens, p=p, sup0=sup0, final_sup=sup)
 
 
+def make_cds4():
+    # the loop with calmas' 4th-order divergence (differencing = 2): periodic
+    # x/y with walls in z, and walls on every face of a non-cubic block
+    make_newsolv_cds4("cds4_ppw32", cds4_case((32, 32, 32), C.BC_PPN, 3, seed=0), ncycles=12,
+                      face_seed=21)
+    make_newsolv_cds4("cds4_wall16", cds4_case((16, 24, 16), C.BC_WALLS, 3, seed=8), ncycles=10,
+                      face_seed=22)
+    print("wrote newsolv cds4")
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("reference driver not built: make -C oracle ref")
+    if sys.argv[1:] == ["cds4"]:  # only the 4th-order fixtures
+        make_cds4()
+        return
     for q, (name, n, bc, L) in enumerate(CASES):
         make_ops(name, n, bc, L, seed=100 + q)
         print("wrote ops", name)
@@ -220,6 +266,7 @@ def main():
     # the 64^3 workload itself (4 levels): history and checksums
     make_newsolv("ppw64", C.single_64(), ncycles=30, keep_p=False)
     print("wrote newsolv ppw64")
+    make_cds4()
     # the loop with the other two smoothers (lexicographic GS, line TDMA)
     make_newsolv("gs_ppw16", C.cube(16, C.BC_PPN, 3, seed=6), ncycles=10, scheme=O.SCHEME_GS)
     make_newsolv("tdma_ppw16", C.cube(16, C.BC_PPN, 3, seed=6), ncycles=8, scheme=O.SCHEME_TDMA)

@@ -27,19 +27,26 @@ def _need_drivers():
     return O
 
 
-def _inputs(doms, bcs, seed, gdx):
+def _inputs(doms, bcs, seed, gdx, cds4=False):
+    """Per block u*, v*, w* (faces 0..n of their axis, or -1..n+1 with cds4:
+    the domain's face arrays extended by configs.extend_faces) and p = 0."""
     from hydro3d_amd import configs as C
     N = [0, 0, 0]
     for d in doms:
         for a in range(3):
             N[a] = max(N[a], int(round(d["hi"][a] / gdx[a])))
     u, v, w = C.face_velocity(tuple(N), bcs, seed, 0.0, gdx)
+    e = 0
+    if cds4:
+        u, v, w = C.extend_faces(u, v, w, bcs, seed + 1)
+        e = 1
     ins = []
     for d in doms:
         x0, y0, z0 = (int(round(d["lo"][a] / gdx[a])) for a in range(3))
         ni, nj, nk = d["n"]
-        ins += [u[z0:z0 + nk, y0:y0 + nj, x0:x0 + ni + 1], v[z0:z0 + nk, y0:y0 + nj + 1, x0:x0 + ni],
-                w[z0:z0 + nk + 1, y0:y0 + nj, x0:x0 + ni], np.zeros((nk, nj, ni))]
+        ins += [u[z0:z0 + nk, y0:y0 + nj, x0:x0 + ni + 1 + 2 * e],
+                v[z0:z0 + nk, y0:y0 + nj + 1 + 2 * e, x0:x0 + ni],
+                w[z0:z0 + nk + 1 + 2 * e, y0:y0 + nj, x0:x0 + ni], np.zeros((nk, nj, ni))]
     return ins
 
 
@@ -66,11 +73,11 @@ def _split(out, doms, pl):
     return it, rmax, blocks
 
 
-def _compare(O, doms, gdx, bcs, nlev, maxcy, eps, seed, nu2=2):
+def _compare(O, doms, gdx, bcs, nlev, maxcy, eps, seed, nu2=2, cds4=False):
     from oracle.oracle import infodom_text, mdmap_text
     info, mdmap = infodom_text(doms), mdmap_text(len(doms))
-    ins = _inputs(doms, bcs, seed, gdx)
-    kw = dict(nlev=nlev, ncycles=maxcy, eps=eps, dt=1e-3, nu2=nu2, early=1, timeout=900)
+    ins = _inputs(doms, bcs, seed, gdx, cds4)
+    kw = dict(nlev=nlev, ncycles=maxcy, eps=eps, dt=1e-3, nu2=nu2, early=1, timeout=900, cds4=cds4)
     ref = O.run_multiblock_files(info, mdmap, "newsolv", ins, None, gdx, bcs, **kw)
     print("reference loop done", flush=True)
     dev = O.run_multiblock_files(info, mdmap, "newsolv", ins, None, gdx, bcs,
@@ -96,6 +103,23 @@ def test_gpu_dropin_newsolv_mg_bitwise_vs_reference_loop(gpu_lib, name):
     bcs = tuple(int(x) for x in z["bc"])
     print(f"case drop-in {name}: {len(doms)} blocks", flush=True)
     _compare(O, doms, gdx, bcs, int(z["nlev"]), 30, 1e-6, seed=901)
+
+
+@pytest.mark.parametrize("name", ["mb_kh.npz", "mb_box.npz", "mb_perx.npz"])
+def test_gpu_dropin_cds4_bitwise_vs_reference_loop(gpu_lib, name):
+    """differencing = 2 (calmas' 4th-order divergence, press.for:17-37, the
+    setting of four shipped benchmarks): the drop-in against the reference's
+    loop on the same layouts, bitwise as above; the stencil's second ghost
+    faces come from the neighbour or partner across a shared face and from
+    the host arrays at walls."""
+    O = _need_drivers()
+    z = load_golden(name)
+    doms = [dict(id=d, n=tuple(int(x) for x in z["n"][d]), lo=tuple(float(x) for x in z["lo"][d]),
+                 hi=tuple(float(x) for x in z["hi"][d])) for d in range(len(z["n"]))]
+    gdx = tuple(float(x) for x in z["h"])
+    bcs = tuple(int(x) for x in z["bc"])
+    print(f"case drop-in cds4 {name}: {len(doms)} blocks", flush=True)
+    _compare(O, doms, gdx, bcs, int(z["nlev"]), 30, 1e-6, seed=911, cds4=True)
 
 
 def test_gpu_dropin_single_block(gpu_lib):

@@ -598,3 +598,61 @@ def test_gpu_256_fp64_and_mixed(gpu_lib):
     e_64 = rel_l2(res[("fp64", 1e-8)], pstar)
     print(f"pressure error vs converged FP64: mixed {e_m:.3e}, fp64@1e-8 {e_64:.3e}")
     assert e_m <= 1e-7
+
+
+@pytest.mark.parametrize("name", ["newsolv_cds4_ppw32.npz", "newsolv_cds4_wall16.npz"])
+def test_gpu_velocity_loop_cds4_bitwise_vs_reference(gpu_lib, name):
+    """differencing = 2 (calmas' 4th-order divergence, press.for:17-37): from
+    host arrays with two ghost layers -- faces -1 and n+1 included, a wrap
+    on periodic axes and seeded host values beyond walls -- the loop's rmax
+    at every iteration, the final p (one ghost layer) and su are bitwise the
+    reference build's (newsolv_cds4_*.npz)."""
+    from hydro3d_amd import _lib as L
+    from hydro3d_amd import configs as C
+    z = load_golden(name)
+    spec = _blockspec(z)
+    ni, nj, nk = spec["n"]
+    nc, pl = int(z["ncycles"]), 2
+    u, v, w = C.face_velocity(spec["n"], spec["bc"], int(z["vel_seed"]), 0.0, spec["h"])
+    ue, ve, we = C.extend_faces(u, v, w, spec["bc"], int(z["face_seed"]))
+    shape = (nk + 2 * pl, nj + 2 * pl, ni + 2 * pl)
+    US, VS, WS = (np.zeros(shape) for _ in range(3))
+    US[pl:pl + nk, pl:pl + nj, pl - 2:pl + ni + 1] = ue
+    VS[pl:pl + nk, pl - 2:pl + nj + 1, pl:pl + ni] = ve
+    WS[pl - 2:pl + nk + 1, pl:pl + nj, pl:pl + ni] = we
+    print(f"case {name}", flush=True)
+    with _solver(spec, maxcy=nc, tol=0.0, dt=float(z["dt"])) as S:
+        S.set_differencing(2)
+        S.set_p(0, np.zeros((nk, nj, ni)))
+        S.set_velocity(0, US, VS, WS, pl=pl)
+        st, ran, hist, _ = S.solve_velocity()
+        p = S.get_p(0)
+        su = S.get_su_host(0, pl=pl)
+    assert st == L.ERR_NOT_CONVERGED and ran == nc
+    assert np.array_equal(hist, z["rmax"]), (hist, z["rmax"])
+    assert np.array_equal(p, z["p"])
+    assert np.array_equal(su[pl:pl + nk, pl:pl + nj, pl:pl + ni], z["final_su"])
+
+
+def test_gpu_cds4_refusals(gpu_lib):
+    """differencing = 2 needs the host arrays' outer faces: device-array
+    projections and a loop over device arrays are refused, as is the
+    variable-density operator."""
+    from hydro3d_amd import _lib as L
+    from hydro3d_amd import configs as C
+    c = C.cube(16, C.BC_PPN, 3, seed=0)
+    spec = dict(n=c["n"], h=c["h"], bc=c["bc"], nlev=c["nlev"])
+    with _solver(spec, maxcy=2, tol=0.0, dt=1e-3) as S:
+        S.set_differencing(2)
+        with pytest.raises(L.H3dError) as e:
+            S.project(0, _dev(c["u"]), _dev(c["v"]), _dev(c["w"]))
+        assert e.value.status == L.ERR_UNSUPPORTED
+        with pytest.raises(L.H3dError) as e:
+            S.solve_velocity([_dev(c["u"])], [_dev(c["v"])], [_dev(c["w"])])
+        assert e.value.status == L.ERR_UNSUPPORTED
+        with pytest.raises(L.H3dError) as e:
+            S.set_density(0, np.ones(c["f"].shape))
+        assert e.value.status == L.ERR_UNSUPPORTED
+        S.set_differencing(1)  # back to the 2nd-order branch: accepted
+        st, ran, _, _ = S.solve_velocity([_dev(c["u"])], [_dev(c["v"])], [_dev(c["w"])])
+        assert ran == 2

@@ -188,3 +188,47 @@ def test_oracle_matches_live_reference_random():
     pp = O.vcycle(B, su)
     ref = O.run_ref(B, "vcycle", [su], pp.size).reshape(pp.shape)
     assert np.array_equal(pp, ref)
+
+
+def _cds4_divergence(ue, ve, we, h):
+    """calmas' 4th-order divergence (press.for:17-37) in its own evaluation
+    order, on face arrays extended to faces -1..n+1 of their axis."""
+    def d4(a, ax, hh):
+        m = a.shape[ax] - 3
+        t = lambda k: np.take(a, range(k, k + m), axis=ax)  # noqa: E731
+        return (((-t(3) + 27.0 * t(2)) - 27.0 * t(1)) + t(0)) / (24.0 * hh)
+    dx, dy, dz = h
+    return (d4(ue, 2, dx) + d4(ve, 1, dy)) + d4(we, 0, dz)
+
+
+@pytest.mark.parametrize("name", ["newsolv_cds4_ppw32.npz", "newsolv_cds4_wall16.npz"])
+def test_cds4_fixture_first_divergence_is_the_stencil(name):
+    """The reference build's first calmas under differencing = 2 (p = 0, so
+    u = u*) is the 4th-order stencil over faces -1..n+1 restated in numpy:
+    su and rmax bitwise.  This pins the fixture's face layout (periodic
+    wraps, host values beyond walls) and the stencil's order of terms."""
+    from hydro3d_amd import configs as C
+    z = np.load(os.path.join(GOLDEN, name))
+    n, h, bc = tuple(int(x) for x in z["n"]), tuple(z["h"]), tuple(int(x) for x in z["bc"])
+    u, v, w = C.face_velocity(n, bc, int(z["vel_seed"]), 0.0, h)
+    ue, ve, we = C.extend_faces(u, v, w, bc, int(z["face_seed"]))
+    fact = _cds4_divergence(ue, ve, we, h)
+    assert np.array_equal(fact / float(z["dt"]), z["su0"])
+    assert np.abs(fact * h[0] * h[1] * h[2]).max() == z["rmax"][0]
+
+
+def test_cds4_stencil_is_exact_for_cubics():
+    """The 4th-order staggered derivative is exact for cubic polynomials:
+    faces of u = x^3 give 3 x^2 at the cell centres (to rounding), while the
+    2nd-order difference is off by h^2/4."""
+    n, hh = 16, 1.0 / 16
+    xf = (np.arange(-1, n + 2)) * hh  # faces -1..n+1
+    # only the x term is non-zero
+    ue = np.broadcast_to(xf ** 3, (4, 4, n + 3)).copy()
+    ve = np.zeros((4, 4 + 3, n))
+    we = np.zeros((4 + 3, 4, n))
+    fact = _cds4_divergence(ue, ve, we, (hh, 1.0, 1.0))
+    xc = (np.arange(n) + 0.5) * hh
+    assert np.allclose(fact[0, 0], 3 * xc ** 2, rtol=0, atol=1e-12)
+    second = (ue[..., 2:-1] - ue[..., 1:-2]) / hh
+    assert np.allclose(second[0, 0] - 3 * xc ** 2, hh ** 2 / 4, atol=1e-12)
