@@ -133,6 +133,10 @@ contains
     nge = nemax*(pl+1)**2
     ngc = (pl+1)**3
     do b = 1, nbp
+       ! MPI tags of the block's receives (initial.for:139-141)
+       do g = 1, 26
+          dom(b)%tg(g) = g*tag_offset1 + dom_id(b)
+       end do
        allocate(dom(b)%stfcinf(6, pl, ngg))
        allocate(dom(b)%sendb_m1(ngg), dom(b)%sendb_p1(ngg))
        allocate(dom(b)%recvb_m1(ngg), dom(b)%recvb_p1(ngg))

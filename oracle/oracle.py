@@ -288,6 +288,58 @@ def dropin_available() -> bool:
     return os.path.exists(DROPIN_DRIVER)
 
 
+def mdmap_ranks_text(ndoms, nranks):
+    """mdmap.cin, type 1, the domains dealt out in contiguous runs over
+    `nranks` ranks (alloc_dom.for:301-345)."""
+    per = [ndoms // nranks + (1 if r < ndoms % nranks else 0) for r in range(nranks)]
+    lines = [" 1  (1)sub-domains description", f"  {ndoms} number of domain",
+             f"  {nranks} number of processor", " ==============================================="]
+    d = 0
+    for r, m in enumerate(per):
+        lines.append(f"  {r}  {m}  " + " ".join(str(x) for x in range(d, d + m)))
+        d += m
+    return "\n".join(lines) + "\n"
+
+
+def block_inputs(doms, u, v, w, gdx):
+    """Per domain u*, v*, w* (faces 0..n of each axis) cut from the face
+    arrays of the whole domain, and p = 0, in domain order."""
+    ins = []
+    for d in doms:
+        x0, y0, z0 = (int(round(d["lo"][a] / gdx[a])) for a in range(3))
+        ni, nj, nk = d["n"]
+        ins += [u[z0:z0 + nk, y0:y0 + nj, x0:x0 + ni + 1], v[z0:z0 + nk, y0:y0 + nj + 1, x0:x0 + ni],
+                w[z0:z0 + nk + 1, y0:y0 + nj, x0:x0 + ni], np.zeros((nk, nj, ni))]
+    return ins
+
+
+def time_multiblock(doms, inputs, gdx, bcs, nlev, ncycles, nranks, nu1=2, nu2=2, dt=1e-3,
+                    timeout=900, mpiexec="/opt/conda/bin/mpiexec"):
+    """The reference's newsolv_mg loop on `nranks` MPI ranks (the blocks of
+    `doms`, all of one size, dealt out over the ranks), `ncycles`
+    iterations: returns (seconds of the loop on rank 0 between barriers,
+    last rmax).  `inputs`: per domain u*, v*, w*, p0 in domain order."""
+    if not os.path.exists(REF_MULTIBLOCK):
+        raise RuntimeError(f"driver not built: {REF_MULTIBLOCK}")
+    codes = ",".join(str(_REF_BC[x]) for x in bcs)
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "infodom.cin"), "w") as fh:
+            fh.write(infodom_text(doms))
+        with open(os.path.join(td, "mdmap.cin"), "w") as fh:
+            fh.write(mdmap_ranks_text(len(doms), nranks))
+        with open(os.path.join(td, "job.nml"), "w") as fh:
+            fh.write("&job\n"
+                     f" mode='timeloop', nlev={nlev}, nu1={nu1}, nu2={nu2}, ncycles={ncycles},\n"
+                     f" bcs={codes}, gdx={gdx[0]!r},{gdx[1]!r},{gdx[2]!r}, dtin={dt!r}\n/\n")
+        with open(os.path.join(td, "in.bin"), "wb") as fh:
+            for a in inputs:
+                fh.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        subprocess.run([mpiexec, "-n", str(nranks), REF_MULTIBLOCK, "job.nml", "in.bin", "out.bin"],
+                       check=True, timeout=timeout, cwd=td, stdout=subprocess.DEVNULL)
+        out = np.fromfile(os.path.join(td, "out.bin"), dtype="<f8")
+    return float(out[0]), float(out[2])
+
+
 def run_multiblock_files(infodom, mdmap, mode, inputs, nout, gdx, bcs, nlev=1, nu1=2, nu2=2,
                          level=1, nsweep=1, ncycles=1, scheme=SCHEME_RB, dt=1e-3, eps=1e-6,
                          timeout=600, early=0, driver=None, cds4=False):

@@ -27,6 +27,12 @@
 !   newsolv : per block u*, v*, w* face arrays and p0 -> the newsolv_mg loop
 !             (newsolv_mg.for:17-59, statement by statement) -> rmax per
 !             iteration, then per block p (padded) and su
+!   timeloop: the same loop for exactly ncycles iterations on every rank of
+!             an MPI run (mpiexec -n <ranks of mdmap.cin>), each rank reading
+!             its own blocks' inputs at their place in the input (blocks of
+!             one size, in domain order) -> rank 0 writes the seconds between
+!             barriers around the loop (first calvel + calmas included),
+!             ncycles and the last rmax
 program ref_multiblock
   use mb_setup
   implicit none
@@ -45,6 +51,8 @@ program ref_multiblock
      call run_vcycle()
   case ('newsolv')
      call run_newsolv()
+  case ('timeloop')
+     call run_timeloop()
   case default
      print *, 'ref_multiblock: unknown mode ', trim(mode)
      call MPI_ABORT(MPI_COMM_WORLD, 2, ierr)
@@ -204,5 +212,69 @@ contains
     write(uout) hist
     call write_pressure_state()
   end subroutine run_newsolv
+
+  subroutine run_timeloop()
+    double precision, allocatable :: uf(:,:,:), vf(:,:,:), wf(:,:,:), p0(:,:,:)
+    integer :: b, i, j, k, it, isp, jsp, ksp, ni, nj, nk
+    integer(kind=8) :: per_block, pos
+    double precision :: ppref, t0, t1
+    do b = 1, nbp
+       ni = nc(1, b); nj = nc(2, b); nk = nc(3, b)
+       per_block = 8_8 * (int(ni + 1, 8) * nj * nk + int(ni, 8) * (nj + 1) * nk + &
+                          int(ni, 8) * nj * (nk + 1) + int(ni, 8) * nj * nk)
+       pos = 1_8 + int(dom_id(b), 8) * per_block
+       isp = dom(b)%isp; jsp = dom(b)%jsp; ksp = dom(b)%ksp
+       allocate(uf(0:ni, nj, nk), vf(ni, 0:nj, nk), wf(ni, nj, 0:nk), p0(ni, nj, nk))
+       read(uin, pos=pos) uf
+       read(uin) vf
+       read(uin) wf
+       read(uin) p0
+       do k = 1, nk; do j = 1, nj; do i = 0, ni
+          dom(b)%ustar(isp-1+i, jsp-1+j, ksp-1+k) = uf(i, j, k)
+       end do; end do; end do
+       do k = 1, nk; do j = 0, nj; do i = 1, ni
+          dom(b)%vstar(isp-1+i, jsp-1+j, ksp-1+k) = vf(i, j, k)
+       end do; end do; end do
+       do k = 0, nk; do j = 1, nj; do i = 1, ni
+          dom(b)%wstar(isp-1+i, jsp-1+j, ksp-1+k) = wf(i, j, k)
+       end do; end do; end do
+       dom(b)%u = dom(b)%ustar; dom(b)%v = dom(b)%vstar; dom(b)%w = dom(b)%wstar
+       do k = 1, nk; do j = 1, nj; do i = 1, ni
+          dom(b)%p(isp-1+i, jsp-1+j, ksp-1+k) = p0(i, j, k)
+       end do; end do; end do
+       deallocate(uf, vf, wf, p0)
+    end do
+    call exchange(4)
+    call pbound
+    call MPI_BARRIER(MPI_COMM_WORLD, ierr)
+    t0 = MPI_WTIME()
+    call calvel
+    call calmas
+    do b = 1, nbp
+       dom(b)%pp = 0.0d0
+    end do
+    do it = 1, ncycles
+       iter = it
+       call mgkcyc
+       ppref = 0.0d0
+       do b = 1, nbp
+          do k = dom(b)%ksp-1, dom(b)%kep+1
+             do i = dom(b)%isp-1, dom(b)%iep+1
+                do j = dom(b)%jsp-1, dom(b)%jep+1
+                   dom(b)%p(i,j,k) = (dom(b)%p(i,j,k) + (dom(b)%pp(i,j,k) - ppref))
+                   dom(b)%pp(i,j,k) = 0.0
+                end do
+             end do
+          end do
+       end do
+       call exchange(4)
+       call pbound
+       call calvel
+       call calmas
+    end do
+    call MPI_BARRIER(MPI_COMM_WORLD, ierr)
+    t1 = MPI_WTIME()
+    if (myrank == 0) write(uout) t1 - t0, dble(ncycles), rmax
+  end subroutine run_timeloop
 
 end program ref_multiblock
