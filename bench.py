@@ -306,30 +306,37 @@ def served_pressure_error(reqs, results):
     return {"sample": len(idx), "max_by_tolerance": worst}
 
 
-def serving(nreq):
+def serving(nreq, fp64_prefix=1000):
     """The serving stream (seeded Poisson arrivals at 200/s, sizes 32..256,
-    tolerances 1e-6/1e-8, 20% warm reuse): continuous-batching mixed server
-    against sequential FP64 solves of the same stream."""
+    tolerances 1e-6/1e-8, 20% warm reuse): the continuous-batching mixed
+    server over all `nreq` requests, against sequential FP64 solves (one at
+    a time, no residency, maxcy 400) of the stream's first `fp64_prefix`
+    requests at the same arrival times."""
     from hydro3d_amd import serving as SV
     reqs = SV.request_stream(nreq, 200.0, 4)
     out = {}
     keep = {reqs[q]["rid"] for q in served_sample(len(reqs))}
     progress = 1000 if nreq >= 2000 else None
-    with SV.PoissonServer(hbm_budget=64 << 30, max_batches=4) as srv:
+    with SV.PoissonServer(hbm_budget=32 << 30, lanes=3) as srv:
         srv.warmup()
         res, wall = SV.run_stream(srv, reqs, keep=keep, progress=progress)
         out["served_mixed"] = SV.summarize(res, wall)
+        out["served_mixed"]["server"] = {k: int(v) for k, v in srv.stats.items()}
         out["served_mixed"]["pressure_rel_err_vs_converged_fp64"] = served_pressure_error(
             reqs, res)
+    del res
     # the reference schedule (coarsest level smoothed) needs many cycles when
     # the level count is capped by the power of two dividing the size
-    with SV.PoissonServer(hbm_budget=64 << 30, max_batches=1, max_batch_blocks=1,
+    pre = reqs[:min(fp64_prefix, nreq)]
+    with SV.PoissonServer(hbm_budget=32 << 30, lanes=1, max_batch_blocks=1,
                           precision="fp64", coarse="smooth", keep_resident=False,
                           maxcy=400) as srv:
         srv.warmup()
-        res, wall = SV.run_stream(srv, reqs, keep=set(), progress=progress)
+        res, wall = SV.run_stream(srv, pre, keep=set(), progress=progress)
         out["sequential_fp64"] = SV.summarize(res, wall)
-    out["workload"] = f"{nreq} requests of the seeded stream (seed 4, 200/s)"
+        out["sequential_fp64"]["prefix_requests"] = len(pre)
+    out["workload"] = (f"{nreq} requests of the seeded stream (seed 4, 200/s); FP64 baseline on "
+                       f"the first {len(pre)}")
     return out
 
 
@@ -415,7 +422,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and exit")
-    ap.add_argument("--serve-requests", type=int, default=400,
+    ap.add_argument("--serve-requests", type=int, default=10000,
                     help="requests of the serving stream to replay (0: skip)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 48-block batch")
     ap.add_argument("--no-lines", action="store_true", help="skip the GS / line-TDMA solves")

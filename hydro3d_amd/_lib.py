@@ -44,6 +44,12 @@ class H3dParams(ctypes.Structure):
                 ("min_cycles", ctypes.c_int), ("use_graph", ctypes.c_int)]
 
 
+class H3dBlockState(ctypes.Structure):
+    _fields_ = [("running", ctypes.c_int), ("cycles", ctypes.c_int), ("status", ctypes.c_int),
+                ("enabled", ctypes.c_int), ("fnorm", ctypes.c_double), ("rel", ctypes.c_double),
+                ("tol", ctypes.c_double)]
+
+
 # every symbol the header declares: name -> (restype, argtypes)
 _P = ctypes.POINTER
 _vp = ctypes.c_void_p
@@ -71,11 +77,26 @@ SIGNATURES = {
     "h3dmg_set_stream": (_i, [_vp, _vp]),
     "h3dmg_set_march": (_i, [_vp, _i, _i]),
     "h3dmg_get_march": (_i, [_vp, _i]),
+    "h3dmg_set_lanes": (_i, [_vp, _i, _i]),
+    "h3dmg_get_lanes": (_i, [_vp, _i]),
     "h3dmg_set_fused": (_i, [_vp, ctypes.c_longlong]),
     "h3dmg_get_fused": (ctypes.c_longlong, [_vp]),
     "h3dmg_set_tail": (_i, [_vp, _i]),
     "h3dmg_set_block_tol": (_i, [_vp, _i, _d]),
     "h3dmg_set_block_enabled": (_i, [_vp, _i, _i]),
+    "h3dmg_pool_create": (_i, [ctypes.c_ulonglong, _i, _P(_vp)]),
+    "h3dmg_pool_destroy": (None, [_vp]),
+    "h3dmg_pool_alloc": (_i, [_vp, ctypes.c_ulonglong, _P(ctypes.c_ulonglong)]),
+    "h3dmg_pool_free": (_i, [_vp, ctypes.c_ulonglong]),
+    "h3dmg_pool_free_bytes": (ctypes.c_ulonglong, [_vp]),
+    "h3dmg_pool_largest_free": (ctypes.c_ulonglong, [_vp]),
+    "h3dmg_block_bytes": (ctypes.c_ulonglong, [_P(H3dBlock), _P(H3dParams)]),
+    "h3dmg_create_pooled": (_i, [_vp, _P(H3dBlock), _i, _P(H3dParams), _P(ctypes.c_ulonglong),
+                                 _ip, _vp, _P(_vp)]),
+    "h3dmg_get_block_state": (_i, [_vp, _i, _P(H3dBlockState)]),
+    "h3dmg_set_block_state": (_i, [_vp, _i, _P(H3dBlockState)]),
+    "h3dmg_solve_begin": (_i, [_vp, _ip, _i]),
+    "h3dmg_solve_step": (_i, [_vp, _ip]),
     "h3dmg_project_device": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _dp]),
     "h3dmg_project_device_noalloc": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp]),
     "h3dmg_layout_read": (_i, [ctypes.c_char_p, ctypes.c_char_p, _P(_vp)]),

@@ -1,9 +1,11 @@
 """Autotuner of the device multigrid's launch shapes and schedule.
 
-Two knobs change only how the work is launched, never the results (every
+These knobs change only how the work is launched, never the results (every
 cell is computed by the same arithmetic whatever the choice):
 
-* march depth per level -- planes each smoother workgroup walks
+* smoother tile shape per level -- lanes per row segment
+  (h3dmg_set_lanes: the row's count, a half or a quarter; the tile's x/y
+  aspect) crossed with the march depth, the planes each smoother workgroup walks
   (h3dmg_set_march: 1, 2 or 4): deeper tiles reuse more of the register
   window and keep more loads in flight per wave, shallower ones need fewer
   registers (more waves per SIMD) and spread a small level over more
@@ -35,6 +37,8 @@ DEFAULT_CACHE = os.environ.get("H3DMG_TUNE_CACHE",
                                os.path.join(os.path.expanduser("~"), ".cache", "h3dmg",
                                             "autotune.json"))
 MARCH_CANDIDATES = (1, 2, 4)
+# smoother lanes per row segment: the whole row's count divided by these
+LANES_DIVISORS = (1, 2, 4)
 TAIL_MAX_CELLS = 64 ** 3  # levels larger than this never go to the one-workgroup tail
 
 
@@ -51,10 +55,15 @@ def config_key(solver, device: str | None = None) -> str:
 
 
 def _device_name() -> str:
+    """The device's ISA and CU count (e.g. "gfx950 256CU"): the marketing
+    name is not reliable (boxes without the device-name table report a
+    generic one)."""
     try:
         import torch
         if torch.cuda.is_available():
-            return torch.cuda.get_device_name(0)
+            pr = torch.cuda.get_device_properties(0)
+            arch = getattr(pr, "gcnArchName", "") or torch.cuda.get_device_name(0)
+            return f"{arch.split(':')[0]} {pr.multi_processor_count}CU"
     except Exception:  # pragma: no cover - torch absent
         pass
     return "unknown"
@@ -86,6 +95,8 @@ def apply(solver, choice: dict) -> None:
     "fused": max_cells}."""
     if "fused" in choice:
         solver.set_fused(int(choice["fused"]))
+    for g, lanes in choice.get("lanes", {}).items():
+        solver.set_lanes(int(g), int(lanes))
     for g, kb in choice.get("march", {}).items():
         solver.set_march(int(g), int(kb))
     if "tail" in choice:
@@ -159,23 +170,34 @@ def autotune(solver, cache_path: str | None = DEFAULT_CACHE, reps: int = 5,
                                               device="cuda"))
              for q, b in enumerate(solver.blocks)]
     Lmax = solver.nlevels_max
-    choice = {"march": {}, "tail": 0}
+    choice = {"march": {}, "lanes": {}, "tail": 0}
     log = {}
-    # 1. march depth per level (smoother launches only)
+    # 1. smoother tile shape per level: lanes per row segment (the tile's
+    #    x/y aspect) x march depth, timed on the level's sweeps
     for g in range(1, Lmax + 1):
         nk = max((b["n"][2] >> (g - 1)) for b in solver.blocks if b["nlev"] >= g)
-        best, tbest = 0, _time_relax(solver, g, reps)
-        log[f"march{g}_auto"] = tbest
-        for kb in MARCH_CANDIDATES:
-            if kb > nk:
+        solver.set_lanes(g, 0)
+        solver.set_march(g, 0)
+        natural = solver.get_lanes(g)
+        best, tbest = (0, 0), _time_relax(solver, g, reps)
+        log[f"level{g}_auto"] = tbest
+        for lanes in LANES_DIVISORS:
+            lw = natural // lanes
+            if lw < 1 or (lanes > 1 and lw == natural):
                 continue
-            solver.set_march(g, kb)
-            t = _time_relax(solver, g, reps)
-            log[f"march{g}_{kb}"] = t
-            if t < tbest * 0.98:
-                best, tbest = kb, t
-        solver.set_march(g, best)
-        choice["march"][str(g)] = best
+            solver.set_lanes(g, 0 if lanes == 1 else lw)
+            for kb in (0,) + MARCH_CANDIDATES:
+                if kb > nk or (lanes == 1 and kb == 0):
+                    continue
+                solver.set_march(g, kb)
+                t = _time_relax(solver, g, reps)
+                log[f"level{g}_lanes{lw}_march{kb}"] = t
+                if t < tbest * 0.98:
+                    best, tbest = (0 if lanes == 1 else lw, kb), t
+        solver.set_lanes(g, best[0])
+        solver.set_march(g, best[1])
+        choice["lanes"][str(g)] = best[0]
+        choice["march"][str(g)] = best[1]
     # 2. tail level on whole solves
     cands = [0] + [g for g in range(1, Lmax + 1) if _level_cells(solver, g) <= TAIL_MAX_CELLS]
     cands.append(Lmax + 1)  # off
@@ -202,6 +224,9 @@ def autotune(solver, cache_path: str | None = DEFAULT_CACHE, reps: int = 5,
     solver.set_fused(fbest)
     choice["fused"] = fbest
     choice["solve_ms"] = tbest * 1e3
+    # the space searched: every candidate's median time (ms; level* =
+    # four sweeps of that level, tail* / fused* = whole solves)
+    choice["searched"] = {k: round(v * 1e3, 4) for k, v in log.items()}
     for q, p in enumerate(saved):
         solver.set_p(q, p)
     torch.cuda.synchronize()

@@ -175,6 +175,15 @@ void launch_lsm_restrict(hipStream_t s, const Tile *tiles, int ntiles, const Lev
 // i+k = d; 1: i-lines, j+k = d; 2: k-lines, i+j = d).  A sweep reads buffer
 // src and writes 1-src; every level is FP64.
 constexpr int GS_TILE = 8;
+// 16^3 GS tiles on constant-coefficient levels: a tile-diagonal of a 64^3
+// sweep holds 10 launches instead of 22; the tile, its halo and its rhs in
+// dynamic LDS
+constexpr int GS_BIG_TILE = 16;
+constexpr size_t GS_BIG_LDS =
+    sizeof(double) * ((GS_BIG_TILE + 2) * (GS_BIG_TILE + 2) * (GS_BIG_TILE + 2) +
+                      GS_BIG_TILE * GS_BIG_TILE * GS_BIG_TILE);
+// permits GS_BIG_LDS of dynamic LDS for the 16^3 GS kernel (once per process)
+hipError_t gs_allow_lds();
 constexpr int TDMA_CELLS = 1024;    // line cells one TDMA workgroup stages in LDS
 constexpr int TDMA_MAX_LEN = 1024;  // longest line (cells along one axis)
 __host__ __device__ inline int tdma_group_lines(int len) {
@@ -183,7 +192,7 @@ __host__ __device__ inline int tdma_group_lines(int len) {
   return lw;
 }
 void launch_gs_tiles(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                     const int *active, int src, int zero);
+                     const int *active, int src, int zero, int tile);
 void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                       const int *active, int pass, int d, int src, int zero);
 // a whole TDMA sweep of small levels, one workgroup per block (Tile::lev),

@@ -444,7 +444,8 @@ __device__ __forceinline__ void march_tile_fast(const Tile &t, const LevelDev &L
         const unsigned pp = p0 + (unsigned)p * plane;
         S2[p] = ld2(off_s + pp);
         N2[p] = ld2(off_n + pp);
-        x1[p] = ld1(off_x + pp);
+        // x slots: read by the lanes at a row segment's ends only
+        x1[p] = (rl == 0 || role_r) ? ld1(off_x + pp) : T(0);
         x2[p] = dual ? ld1(off_x2 + pp) : T(0);
       }
 #pragma unroll
@@ -802,6 +803,8 @@ __device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev
   const unsigned w_ent = w_edge ? (unsigned)(xw >> 1) * SZ : ent;
   const unsigned e_ent = e_edge ? (unsigned)(xe >> 1) * SZ : ent;
   const int w_par = w_edge ? (xw & 1) : 0, e_par = e_edge ? (xe & 1) : 0;
+  const int x1_par = w_edge ? w_par : e_par;
+  const unsigned x1_ent = w_edge ? w_ent : e_ent;
   const Tf cx = (Tf)F.cx, cy = (Tf)F.cy, cz = (Tf)F.cz;
   const Tf aw_a = (ia == 0 && !px) ? Tf(0) : cx;
   const Tf ae_a = (ia == ni - 1 && !px) ? Tf(0) : cx;
@@ -834,11 +837,22 @@ __device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev
     q.wb0 = ld(pcol(kp + 1) + op + o_r0);
     q.wa1 = ld(pcol(kp + 1) + op + o_r1);
     q.wb1 = ld(pcol(kp) + op + o_r1);
+    // x-edge values: only the lanes at a row segment's ends use them (the
+    // others read no x-edge value at all)
     const unsigned r0 = (unsigned)j0 * row;
-    q.we0 = ld(pcol(w_par + k) + o + r0 + w_ent);
-    q.we1 = ld(pcol(w_par + k + 1) + o + r0 + row + w_ent);
-    q.ee0 = ld(pcol(e_par + k) + o + r0 + e_ent);
-    q.ee1 = ld(pcol(e_par + k + 1) + o + r0 + row + e_ent);
+    Tf x10 = Tf(0), x11 = Tf(0), x20 = Tf(0), x21 = Tf(0);
+    if (w_edge || e_edge) {
+      x10 = ld(pcol(x1_par + k) + o + r0 + x1_ent);
+      x11 = ld(pcol(x1_par + k + 1) + o + r0 + row + x1_ent);
+    }
+    if (w_edge && e_edge) {
+      x20 = ld(pcol(e_par + k) + o + r0 + e_ent);
+      x21 = ld(pcol(e_par + k + 1) + o + r0 + row + e_ent);
+    }
+    q.we0 = x10;
+    q.we1 = x11;
+    q.ee0 = w_edge ? x20 : x10;
+    q.ee1 = w_edge ? x21 : x11;
     return q;
   };
   const int k0 = 2 * t.z0;
@@ -1271,6 +1285,8 @@ __device__ __forceinline__ void residual_tile_fast(const Tile &t, const FineDev 
   const unsigned w_ent = (unsigned)jc * row + (w_edge ? (unsigned)(xw >> 1) * SZ : ent);
   const unsigned e_ent = (unsigned)jc * row + (e_edge ? (unsigned)(xe >> 1) * SZ : ent);
   const int w_par = w_edge ? (xw & 1) : 0, e_par = e_edge ? (xe & 1) : 0;
+  const int x1_par = w_edge ? w_par : e_par;
+  const unsigned x1_ent = w_edge ? w_ent : e_ent;
   const double cx = F.cx, cy = F.cy, cz = F.cz;
   const double aw_a = (ia == 0 && !px) ? 0.0 : cx, ae_a = (ia == ni - 1 && !px) ? 0.0 : cx;
   const double aw_b = cx, ae_b = (ib == ni - 1 && !px) ? 0.0 : cx;
@@ -1295,8 +1311,12 @@ __device__ __forceinline__ void residual_tile_fast(const Tile &t, const FineDev 
     const int kp = next_plane(k);
     q.ap1 = ld(pcol(jc + kp) + (unsigned)kp * plane + o_c);
     q.bp1 = ld(pcol(jc + kp + 1) + (unsigned)kp * plane + o_c);
-    q.we = ld(pcol(w_par + jc + k) + o + w_ent);
-    q.ee = ld(pcol(e_par + jc + k) + o + e_ent);
+    // x-edge values: only the lanes at a row segment's ends use them
+    double x1 = 0.0, x2 = 0.0;
+    if (w_edge || e_edge) x1 = ld(pcol(x1_par + jc + k) + o + x1_ent);
+    if (w_edge && e_edge) x2 = ld(pcol(e_par + jc + k) + o + e_ent);
+    q.we = x1;
+    q.ee = w_edge ? x2 : x1;
     return q;
   };
   const int k0 = t.z0;
@@ -2466,36 +2486,31 @@ __device__ __forceinline__ double before_sweep(const LevelDev &L, int src, int z
   return cell_at(L, src, x, y, z);
 }
 
-// One GS tile of GS_T^3 cells (Tile::x0, y0, z0): the tiles of one
+// One GS tile of GT^3 cells (Tile::x0, y0, z0): the tiles of one
 // tile-diagonal I+J+K = D depend only on tiles of D-1 (updated) and D+1
 // (not yet), as cells of a hyperplane do, so one launch per tile-diagonal
-// repeats the reference's loop order.  The tile and its six face halos go
-// to LDS -- the tile's cells and its east/north/top halo as before the
-// sweep (src), its west/south/bottom halo as updated (dst, finished by
-// earlier tile-diagonals), ghost positions as images of src -- and the
-// wavefront walks the tile's own hyperplanes in place:
+// repeats the reference's loop order, whatever the tile size.  The tile and
+// its six face halos go to LDS -- the tile's cells and its east/north/top
+// halo as before the sweep (src), its west/south/bottom halo as updated
+// (dst, finished by earlier tile-diagonals), ghost positions as images of
+// src -- and the wavefront walks the tile's own hyperplanes in place:
 //   phi = (rhs - (aw*W + ae*E + as*S + an*N + ab*B + at*T)) / ap
-// One wavefront per tile: thread (lx, ly) owns lz = h - lx - ly of plane h.
-constexpr int GS_T = GS_TILE;
-constexpr int GS_P = GS_T + 2;  // with the halo
-
-__global__ void __launch_bounds__(GS_T * GS_T)
-    k_gs_tile(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
-              const int *__restrict__ active, int src, int zero) {
-  __shared__ double v[GS_P * GS_P * GS_P];      // phi, tile + face halos
-  __shared__ double rl[GS_T * GS_T * GS_T];     // rhs of the tile
-  __shared__ double cl[7][GS_T * GS_T * GS_T];  // variable-density coefficients
-  const Tile t = tiles[blockIdx.x];
-  const LevelDev &L = levs[t.lev];
-  if (!active[L.blk]) return;
+// Thread (lx, ly) owns lz = h - lx - ly of plane h.  8^3 tiles (one
+// wavefront, static LDS, variable-density coefficients staged too) and 16^3
+// tiles (four wavefronts, constant coefficients, 79 KB of dynamic LDS:
+// fewer tile-diagonals, so fewer launches per sweep).
+template <int GT>
+__device__ __forceinline__ void gs_tile_run(const Tile &t, const LevelDev &L, int src, int zero,
+                                            double *v, double *rl, double *cl) {
+  constexpr int GP = GT + 2;  // with the halo
   const int dst = 1 - src;
   const int tid = (int)threadIdx.x;
-  const bool var = L.lsm != nullptr;
+  const bool var = cl != nullptr && L.lsm != nullptr;
   // every load of the tile is issued before the first LDS store, with no
   // branch between them (clamped positions load an in-range cell whose
   // value is not stored or never read)
-  constexpr int NT = GS_T * GS_T;
-  constexpr int NR = GS_T * GS_T * GS_T / NT, NV = (GS_P * GS_P * GS_P + NT - 1) / NT;
+  constexpr int NT = GT * GT;
+  constexpr int NR = GT * GT * GT / NT, NV = (GP * GP * GP + NT - 1) / NT;
   const int ni0 = L.ni, nj0 = L.nj, nk0 = L.nk, H0 = L.H;
   const bool px0 = L.per & 1, py0 = L.per & 2, pz0 = L.per & 4;
   // the level's phi arrays as byte offsets from src's colour 0
@@ -2507,14 +2522,14 @@ __global__ void __launch_bounds__(GS_T * GS_T)
 #pragma unroll
   for (int it = 0; it < NR; ++it) {
     const int q = tid + it * NT;
-    const int x = min(t.x0 + q % GS_T, ni0 - 1), y = min(t.y0 + (q / GS_T) % GS_T, nj0 - 1);
-    const int z = min(t.z0 + q / (GS_T * GS_T), nk0 - 1);
+    const int x = min(t.x0 + q % GT, ni0 - 1), y = min(t.y0 + (q / GT) % GT, nj0 - 1);
+    const int z = min(t.z0 + q / (GT * GT), nk0 - 1);
     rr[it] = gload(L.rhs[(x + y + z) & 1], cidx(H0, nj0, x, y, z));
   }
 #pragma unroll
   for (int it = 0; it < NV; ++it) {
-    const int q = min(tid + it * NT, GS_P * GS_P * GS_P - 1);
-    const int lz = q / (GS_P * GS_P) - 1, ly = (q / GS_P) % GS_P - 1, lx = q % GS_P - 1;
+    const int q = min(tid + it * NT, GP * GP * GP - 1);
+    const int lz = q / (GP * GP) - 1, ly = (q / GP) % GP - 1, lx = q % GP - 1;
     int x = t.x0 + lx, y = t.y0 + ly, z = t.z0 + lz;
     const bool ghost = (unsigned)x >= (unsigned)ni0 || (unsigned)y >= (unsigned)nj0 ||
                        (unsigned)z >= (unsigned)nk0;
@@ -2537,16 +2552,16 @@ __global__ void __launch_bounds__(GS_T * GS_T)
   for (int it = 0; it < NR; ++it) rl[tid + it * NT] = rr[it];
 #pragma unroll
   for (int it = 0; it < NV; ++it)
-    if (tid + it * NT < GS_P * GS_P * GS_P) v[tid + it * NT] = vv[it];
+    if (tid + it * NT < GP * GP * GP) v[tid + it * NT] = vv[it];
   if (var)
-    for (int q = tid; q < GS_T * GS_T * GS_T; q += NT) {
-      const int x = t.x0 + q % GS_T, y = t.y0 + (q / GS_T) % GS_T, z = t.z0 + q / (GS_T * GS_T);
+    for (int q = tid; q < GT * GT * GT; q += NT) {
+      const int x = t.x0 + q % GT, y = t.y0 + (q / GT) % GT, z = t.z0 + q / (GT * GT);
       if (x >= L.ni || y >= L.nj || z >= L.nk) continue;
       const size_t e = cidx(L.H, L.nj, x, y, z);
-      for (int a = 0; a < 7; ++a) cl[a][q] = gload(lsm_coef(L, a, (x + y + z) & 1), e);
+      for (int a = 0; a < 7; ++a) cl[a * GT * GT * GT + q] = gload(lsm_coef(L, a, (x + y + z) & 1), e);
     }
   __syncthreads();
-  const int lx = tid % GS_T, ly = tid / GS_T;
+  const int lx = tid % GT, ly = tid / GT;
   const int i = t.x0 + lx, j = t.y0 + ly;
   // the level's fields in registers: a barrier would make the compiler
   // reload them from memory on every plane
@@ -2554,15 +2569,16 @@ __global__ void __launch_bounds__(GS_T * GS_T)
   const double cx = L.cx, cy = L.cy, cz = L.cz;
   const bool px = L.per & 1, py = L.per & 2, pz = L.per & 4;
   // the planes touch LDS only: a barrier then waits for no global store
-  for (int h = 0; h < 3 * GS_T - 2; ++h) {
+  for (int h = 0; h < 3 * GT - 2; ++h) {
     const int lz = h - lx - ly, k = t.z0 + lz;
-    if (lz >= 0 && lz < GS_T && i < ni && j < nj && k < nk) {
-      const int c0 = ((lz + 1) * GS_P + ly + 1) * GS_P + lx + 1;
-      const int q = (lz * GS_T + ly) * GS_T + lx;
+    if (lz >= 0 && lz < GT && i < ni && j < nj && k < nk) {
+      const int c0 = ((lz + 1) * GP + ly + 1) * GP + lx + 1;
+      const int q = (lz * GT + ly) * GT + lx;
       Coef<double> c;
       if (var) {
-        c.aw = cl[0][q], c.ae = cl[1][q], c.as = cl[2][q], c.an = cl[3][q];
-        c.ab = cl[4][q], c.at = cl[5][q], c.ap = cl[6][q];
+        constexpr int S = GT * GT * GT;
+        c.aw = cl[q], c.ae = cl[S + q], c.as = cl[2 * S + q], c.an = cl[3 * S + q];
+        c.ab = cl[4 * S + q], c.at = cl[5 * S + q], c.ap = cl[6 * S + q];
       } else {  // cell_coef
         c.aw = (!px && i == 0) ? 0.0 : cx;
         c.ae = (!px && i == ni - 1) ? 0.0 : cx;
@@ -2573,8 +2589,8 @@ __global__ void __launch_bounds__(GS_T * GS_T)
         c.ap = -1.0 * (c.aw + c.ae + c.as + c.an + c.ab + c.at);
       }
       const double W = v[c0 - 1], E = v[c0 + 1];
-      const double S = v[c0 - GS_P], N = v[c0 + GS_P];
-      const double B = v[c0 - GS_P * GS_P], T = v[c0 + GS_P * GS_P];
+      const double S = v[c0 - GP], N = v[c0 + GP];
+      const double B = v[c0 - GP * GP], T = v[c0 + GP * GP];
       const double r = rl[q];
       const double sum = c.aw * W + c.ae * E + c.as * S + c.an * N + c.ab * B + c.at * T;
       v[c0] = (r - sum) / c.ap;
@@ -2585,11 +2601,37 @@ __global__ void __launch_bounds__(GS_T * GS_T)
   gmem<double> *const out0 = (gmem<double> *)L.phi[dst][0];
   gmem<double> *const out1 = (gmem<double> *)L.phi[dst][1];
   if (i < ni && j < nj)
-    for (int lz = 0; lz < GS_T && t.z0 + lz < nk; ++lz) {
+    for (int lz = 0; lz < GT && t.z0 + lz < nk; ++lz) {
       const int k = t.z0 + lz;
       (((i + j + k) & 1) ? out1 : out0)[cidx(L.H, L.nj, i, j, k)] =
-          v[((lz + 1) * GS_P + ly + 1) * GS_P + lx + 1];
+          v[((lz + 1) * GP + ly + 1) * GP + lx + 1];
     }
+}
+
+__global__ void __launch_bounds__(GS_TILE * GS_TILE)
+    k_gs_tile(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+              const int *__restrict__ active, int src, int zero) {
+  constexpr int GP = GS_TILE + 2;
+  __shared__ double v[GP * GP * GP];                    // phi, tile + face halos
+  __shared__ double rl[GS_TILE * GS_TILE * GS_TILE];      // rhs of the tile
+  __shared__ double cl[7 * GS_TILE * GS_TILE * GS_TILE];  // variable-density coefficients
+  const Tile t = tiles[blockIdx.x];
+  const LevelDev &L = levs[t.lev];
+  if (!active[L.blk]) return;
+  gs_tile_run<GS_TILE>(t, L, src, zero, v, rl, cl);
+}
+
+// 16^3 tiles, constant coefficients only (the host never launches it on a
+// variable-density level): GS_BIG_LDS bytes of dynamic LDS
+__global__ void __launch_bounds__(GS_BIG_TILE * GS_BIG_TILE)
+    k_gs_tile_big(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+                  const int *__restrict__ active, int src, int zero) {
+  extern __shared__ double gs_lds[];
+  constexpr int GP = GS_BIG_TILE + 2;
+  const Tile t = tiles[blockIdx.x];
+  const LevelDev &L = levs[t.lev];
+  if (!active[L.blk] || L.lsm) return;
+  gs_tile_run<GS_BIG_TILE>(t, L, src, zero, gs_lds, gs_lds + GP * GP * GP, nullptr);
 }
 
 // positions of one line whose coefficients are loaded together
@@ -2948,10 +2990,19 @@ void launch_lsm_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
 }
 
 void launch_gs_tiles(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                     const int *active, int src, int zero) {
+                     const int *active, int src, int zero, int tile) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_gs_tile, dim3(ntiles), dim3(GS_T * GS_T), 0, s, tiles, levs, active, src,
-                     zero);
+  if (tile == GS_BIG_TILE)
+    hipLaunchKernelGGL(k_gs_tile_big, dim3(ntiles), dim3(GS_BIG_TILE * GS_BIG_TILE), GS_BIG_LDS,
+                       s, tiles, levs, active, src, zero);
+  else
+    hipLaunchKernelGGL(k_gs_tile, dim3(ntiles), dim3(GS_TILE * GS_TILE), 0, s, tiles, levs,
+                       active, src, zero);
+}
+
+hipError_t gs_allow_lds() {
+  return hipFuncSetAttribute((const void *)k_gs_tile_big,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)GS_BIG_LDS);
 }
 
 void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -98,6 +99,7 @@ struct TileSet {
   int ndiag[3] = {0, 0, 0};      // GS / TDMA sets: launches per sweep (pass)
   int pass_begin[4] = {0, 0, 0, 0};  // TDMA sets: tiles of pass q at [begin[q], begin[q+1])
   std::vector<int> diag_begin;       // GS sets: tiles of tile-diagonal D at [begin[D], begin[D+1])
+  int gs_tile = GS_TILE;             // GS sets: tile edge (GS_TILE or GS_BIG_TILE)
   size_t tdma_lds = 0;  // TDMA sets: > 0 = whole sweeps in one workgroup per block (LDS bytes)
   int n() const { return (int)h.size(); }
 };
@@ -316,11 +318,105 @@ struct h3dmg_handle {
   bool rhs_ready = false;
   int pp_buf = 0;  // level-1 buffer that holds pp after the last V-cycle
   unsigned long long bytes = 0;
+
+  // a handle built in a pool (h3dmg_create_pooled): the block regions are
+  // the caller's pieces of the pool, the global region and the smoother
+  // tables pieces the handle takes and gives back; nothing is allocated
+  h3dmg_pool *pool = nullptr;
+  std::vector<size_t> blk_off;  // pool offset of each block's region
+  size_t glob_off = 0;
+  bool glob_taken = false;
+  int hs_slot = -1;             // first of the handle's entries in the pool's pinned stats
+  // stepped solves (h3dmg_solve_begin / _step): per block running flag and
+  // outcome
+  std::vector<char> running;
+  std::vector<int> outcome;
+};
+
+// One device allocation carved into pieces (first fit, 256-byte aligned),
+// and a pinned array of per-block statistics handed out in runs: the memory
+// of handles whose blocks join and leave between V-cycles.
+struct h3dmg_pool {
+  int device = 0;
+  char *base = nullptr;
+  size_t bytes = 0;
+  std::map<size_t, size_t> free_;  // offset -> size
+  std::map<size_t, size_t> used_;  // offset -> size
+  RedPartial *h_stats = nullptr;   // pinned, max_blocks entries
+  std::vector<char> hs_used;
+  std::mutex mu;
 };
 
 namespace {
 
 bool is_mixed(const h3dmg_handle *h) { return h->prm.precision == H3DMG_MIXED; }
+
+// ---- pool pieces ----
+h3dmg_status pool_take(h3dmg_pool *pool, size_t bytes, size_t *off) {
+  std::lock_guard<std::mutex> lock(pool->mu);
+  const size_t need = align_up(std::max<size_t>(bytes, 1), 256);
+  for (auto it = pool->free_.begin(); it != pool->free_.end(); ++it) {
+    if (it->second < need) continue;
+    const size_t o = it->first, rest = it->second - need;
+    pool->free_.erase(it);
+    if (rest) pool->free_[o + need] = rest;
+    pool->used_[o] = need;
+    *off = o;
+    return H3DMG_OK;
+  }
+  return fail(H3DMG_ERR_NOMEM, "pool: no free piece of %zu bytes", need);
+}
+
+h3dmg_status pool_give(h3dmg_pool *pool, size_t off) {
+  std::lock_guard<std::mutex> lock(pool->mu);
+  auto u = pool->used_.find(off);
+  if (u == pool->used_.end()) return fail(H3DMG_ERR_ARG, "pool: no piece at offset %zu", off);
+  size_t o = off, n = u->second;
+  pool->used_.erase(u);
+  auto nx = pool->free_.lower_bound(o);
+  if (nx != pool->free_.end() && nx->first == o + n) {
+    n += nx->second;
+    nx = pool->free_.erase(nx);
+  }
+  if (nx != pool->free_.begin()) {
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second == o) {
+      o = pv->first;
+      n += pv->second;
+      pool->free_.erase(pv);
+    }
+  }
+  pool->free_[o] = n;
+  return H3DMG_OK;
+}
+
+// bytes of the piece that starts at `off` (0: none does)
+size_t pool_size_at(h3dmg_pool *pool, size_t off) {
+  std::lock_guard<std::mutex> lock(pool->mu);
+  auto u = pool->used_.find(off);
+  return u == pool->used_.end() ? 0 : u->second;
+}
+
+// device memory of a handle's separately held tables: the pool's or its own
+h3dmg_status dev_alloc(h3dmg_handle *h, size_t bytes, void **p) {
+  if (h->pool) {
+    size_t off = 0;
+    h3dmg_status st = pool_take(h->pool, bytes, &off);
+    if (st) return st;
+    *p = h->pool->base + off;
+    return H3DMG_OK;
+  }
+  HIP_TRY(hipMalloc(p, bytes));
+  return H3DMG_OK;
+}
+
+void dev_free(h3dmg_handle *h, void *p) {
+  if (!p) return;
+  if (h->pool)
+    (void)pool_give(h->pool, (size_t)((char *)p - h->pool->base));
+  else
+    (void)hipFree(p);
+}
 
 template <class T>
 void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool &zero) {
@@ -328,7 +424,7 @@ void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool 
     if (h->prm.smoother == H3DMG_SMOOTH_GS)  // tile wavefront (every level FP64)
       for (int d = 0; d + 1 < (int)ts.diag_begin.size(); ++d)
         launch_gs_tiles(h->stream, ts.d + ts.diag_begin[d], ts.diag_begin[d + 1] - ts.diag_begin[d],
-                        h->d_levs, h->d_active, buf, zero ? 1 : 0);
+                        h->d_levs, h->d_active, buf, zero ? 1 : 0, ts.gs_tile);
     else if (h->prm.smoother == H3DMG_SMOOTH_TDMA && ts.tdma_lds)  // small levels
       launch_tdma_sweep(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, buf, zero ? 1 : 0,
                         ts.tdma_lds);
@@ -730,7 +826,12 @@ h3dmg_status validate_blocks(const h3dmg_block *blocks, int nblocks, const h3dmg
   return validate_neighbours(blocks, nblocks, P);
 }
 
-h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
+// Lays out and uploads a handle over `blocks`.  `fresh` (pooled handles):
+// blocks whose region starts new (zero p and f, the exact solve's tables
+// uploaded); the others carry theirs over.  With `region_bytes` set, only
+// the per-block region sizes are computed, and nothing is allocated.
+h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
+                   const int *fresh = nullptr, size_t *region_bytes = nullptr) {
   h->nb = nblocks;
   h->blk.resize(nblocks);
   h->Lmax = 0;
@@ -747,10 +848,15 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   }
   const bool exact = h->prm.coarse == H3DMG_COARSE_EXACT;
 
-  // ---- descriptors and arena layout (level-major) ----
-  size_t off = 0;
+  // ---- descriptors (level-major) and per-block regions ----
+  // Everything one block owns (its levels, finest p and f, the exact coarse
+  // solve's tables, the TDMA tables) lies in one contiguous region of the
+  // block, at offsets boff[b] relative to the region; the region sizes are
+  // h3dmg_block_bytes'.  The tables shared by the handle's blocks follow in
+  // a global region (offsets `off`).
+  std::vector<size_t> boff(nblocks, 0);
   struct Pending {
-    int d;
+    int d, b;
     size_t phi[2][2], rhs[2];
   };
   std::vector<Pending> pend;
@@ -784,14 +890,15 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
                     L.ni, L.nj, L.nk);
       Pending P{};
       P.d = (int)h->levs.size();
+      P.b = b;
       for (int bb = 0; bb < 2; ++bb)
         for (int c = 0; c < 2; ++c) {
-          P.phi[bb][c] = off;
-          off = align_up(off + half, 256);
+          P.phi[bb][c] = boff[b];
+          boff[b] = align_up(boff[b] + half, 256);
         }
       for (int c = 0; c < 2; ++c) {
-        P.rhs[c] = off;
-        off = align_up(off + half, 256);
+        P.rhs[c] = boff[b];
+        boff[b] = align_up(boff[b] + half, 256);
       }
       B.desc[g] = P.d;
       h->levs.push_back(L);
@@ -819,8 +926,8 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     const LevelDev &L = h->levs[b];
     const size_t half = (size_t)L.H * L.nj * L.nk * 8;
     for (int q = 0; q < 4; ++q) {
-      foff[4 * b + q] = off;
-      off = align_up(off + half, 256);
+      foff[4 * b + q] = boff[b];
+      boff[b] = align_up(boff[b] + half, 256);
     }
   }
   // exact coarse solve tables
@@ -835,18 +942,18 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
       for (int a = 0; a < 3; ++a) {
         std::vector<double> Q, lam;
         eig1d(nn[a], (L.per >> a) & 1, cc[a], Q, lam);
-        eoff[8 * b + a] = off;
-        off = align_up(off + Q.size() * 8, 256);
+        eoff[8 * b + a] = boff[b];
+        boff[b] = align_up(boff[b] + Q.size() * 8, 256);
         edata.push_back(Q);
-        eoff[8 * b + 3 + a] = off;
-        off = align_up(off + lam.size() * 8, 256);
+        eoff[8 * b + 3 + a] = boff[b];
+        boff[b] = align_up(boff[b] + lam.size() * 8, 256);
         edata.push_back(lam);
       }
       const size_t n3 = (size_t)nn[0] * nn[1] * nn[2];
-      eoff[8 * b + 6] = off;
-      off = align_up(off + n3 * 8, 256);
-      eoff[8 * b + 7] = off;
-      off = align_up(off + n3 * 8, 256);
+      eoff[8 * b + 6] = boff[b];
+      boff[b] = align_up(boff[b] + n3 * 8, 256);
+      eoff[8 * b + 7] = boff[b];
+      boff[b] = align_up(boff[b] + n3 * 8, 256);
     }
   }
   // line-TDMA tables: per descriptor 32*(ni+nj+nk) doubles
@@ -854,15 +961,21 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   if (h->prm.smoother == H3DMG_SMOOTH_TDMA)
     for (size_t d = 0; d < h->levs.size(); ++d) {
       const LevelDev &L = h->levs[d];
-      tdma_off[d] = off;
-      off = align_up(off + (size_t)32 * (L.ni + L.nj + L.nk) * 8, 256);
+      tdma_off[d] = boff[L.blk];
+      boff[L.blk] = align_up(boff[L.blk] + (size_t)32 * (L.ni + L.nj + L.nk) * 8, 256);
     }
-  // staging: largest padded level-1 array
+  if (region_bytes) {
+    for (int b = 0; b < nblocks; ++b) region_bytes[b] = boff[b];
+    return H3DMG_OK;
+  }
+  // staging: largest padded level-1 array (a pooled handle takes device
+  // arrays only, and has none)
   size_t stage = 0;
-  for (int b = 0; b < nblocks; ++b)
+  for (int b = 0; b < nblocks && !h->pool; ++b)
     stage = std::max(stage, (size_t)(h->blk[b].spec.ni + 2) * (h->blk[b].spec.nj + 2) *
                                 (h->blk[b].spec.nk + 2));
   h->stage_elems = stage;
+  size_t off = 0;
   const size_t stage_off = off;
   off = align_up(off + stage * 8, 256);
 
@@ -997,19 +1110,48 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
   const size_t eptr_off = off;
   off = align_up(off + (size_t)nblocks * 8 * sizeof(double *), 256);
 
-  h->arena_bytes = off;
-  hipError_t e = hipMalloc((void **)&h->arena, off);
-  if (e != hipSuccess)
-    return fail(H3DMG_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", off, hipGetErrorString(e));
-  h->bytes = off;
-  char *A = h->arena;
+  // block regions and the global region: in one allocation, or (a handle
+  // in a pool) at the caller's pool offsets plus a piece of the pool
+  std::vector<char *> R(nblocks);
+  char *A = nullptr;  // the global region
+  if (h->pool) {
+    for (int b = 0; b < nblocks; ++b) {
+      const size_t have = pool_size_at(h->pool, h->blk_off[b]);
+      if (have < boff[b])
+        return fail(H3DMG_ERR_ARG, "block %d: pool piece at %zu holds %zu bytes, needs %zu", b,
+                    h->blk_off[b], have, boff[b]);
+      R[b] = h->pool->base + h->blk_off[b];
+    }
+    h3dmg_status st = pool_take(h->pool, off, &h->glob_off);
+    if (st) return st;
+    h->glob_taken = true;
+    A = h->pool->base + h->glob_off;
+    h->bytes = off;
+  } else {
+    size_t total = 0;
+    std::vector<size_t> rbase(nblocks);
+    for (int b = 0; b < nblocks; ++b) {
+      rbase[b] = total;
+      total = align_up(total + boff[b], 256);
+    }
+    const size_t gbase = total;
+    total += off;
+    h->arena_bytes = total;
+    hipError_t e = hipMalloc((void **)&h->arena, total);
+    if (e != hipSuccess)
+      return fail(H3DMG_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", total,
+                  hipGetErrorString(e));
+    h->bytes = total;
+    for (int b = 0; b < nblocks; ++b) R[b] = h->arena + rbase[b];
+    A = h->arena + gbase;
+  }
 
   for (const Pending &P : pend) {
     LevelDev &L = h->levs[P.d];
     for (int bb = 0; bb < 2; ++bb)
-      for (int c = 0; c < 2; ++c) L.phi[bb][c] = A + P.phi[bb][c];
-    for (int c = 0; c < 2; ++c) L.rhs[c] = A + P.rhs[c];
-    if (h->prm.smoother == H3DMG_SMOOTH_TDMA) L.tdma = (double *)(A + tdma_off[P.d]);
+      for (int c = 0; c < 2; ++c) L.phi[bb][c] = R[P.b] + P.phi[bb][c];
+    for (int c = 0; c < 2; ++c) L.rhs[c] = R[P.b] + P.rhs[c];
+    if (h->prm.smoother == H3DMG_SMOOTH_TDMA) L.tdma = (double *)(R[P.b] + tdma_off[P.d]);
   }
   h->fine.resize(nblocks);
   for (int b = 0; b < nblocks; ++b) {
@@ -1026,58 +1168,73 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     F.cx = L.cx;
     F.cy = L.cy;
     F.cz = L.cz;
-    F.p[0] = (double *)(A + foff[4 * b + 0]);
-    F.p[1] = (double *)(A + foff[4 * b + 1]);
-    F.f[0] = (double *)(A + foff[4 * b + 2]);
-    F.f[1] = (double *)(A + foff[4 * b + 3]);
+    F.p[0] = (double *)(R[b] + foff[4 * b + 0]);
+    F.p[1] = (double *)(R[b] + foff[4 * b + 1]);
+    F.f[0] = (double *)(R[b] + foff[4 * b + 2]);
+    F.f[1] = (double *)(R[b] + foff[4 * b + 3]);
     h->fine[b] = F;
   }
-  h->d_stage = (double *)(A + stage_off);
+  h->d_stage = stage ? (double *)(A + stage_off) : nullptr;
   for (size_t q = 0; q < all.size(); ++q) {
     all[q]->d = (Tile *)(A + toff[q]);
     if (!all[q]->h.empty())
-      HIP_TRY(hipMemcpy(all[q]->d, all[q]->h.data(), all[q]->h.size() * sizeof(Tile),
-                        hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpyAsync(all[q]->d, all[q]->h.data(), all[q]->h.size() * sizeof(Tile), hipMemcpyHostToDevice, h->stream));
   }
   h->d_levs = (LevelDev *)(A + levs_off);
-  HIP_TRY(hipMemcpy(h->d_levs, h->levs.data(), h->levs.size() * sizeof(LevelDev),
-                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(h->d_levs, h->levs.data(), h->levs.size() * sizeof(LevelDev), hipMemcpyHostToDevice, h->stream));
   if (h->prm.smoother == H3DMG_SMOOTH_TDMA) {
     launch_tdma_tables(h->stream, h->d_levs, (int)h->levs.size());
     HIP_TRY(hipGetLastError());
   }
   h->d_fine = (FineDev *)(A + fine_off);
-  HIP_TRY(hipMemcpy(h->d_fine, h->fine.data(), nblocks * sizeof(FineDev), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(h->d_fine, h->fine.data(), nblocks * sizeof(FineDev), hipMemcpyHostToDevice, h->stream));
   h->active.assign(nblocks, 1);
   h->d_active = (int *)(A + act_off);
-  HIP_TRY(hipMemcpy(h->d_active, h->active.data(), nblocks * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(h->d_active, h->active.data(), nblocks * sizeof(int), hipMemcpyHostToDevice, h->stream));
   h->d_tile_begin = (int *)(A + tb_off);
-  HIP_TRY(hipMemcpy(h->d_tile_begin, h->tile_begin.data(), (nblocks + 1) * sizeof(int),
-                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(h->d_tile_begin, h->tile_begin.data(), (nblocks + 1) * sizeof(int), hipMemcpyHostToDevice, h->stream));
   h->d_partials = (RedPartial *)(A + part_off);
   h->d_stats = (RedPartial *)(A + stats_off);
   h->d_proj = (RedPartial *)(A + proj_off);
   h->d_proj_begin = (int *)(A + pb_off);
-  HIP_TRY(hipMemcpy(h->d_proj_begin, h->proj_begin.data(), (nblocks + 1) * sizeof(int),
-                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(h->d_proj_begin, h->proj_begin.data(), (nblocks + 1) * sizeof(int), hipMemcpyHostToDevice, h->stream));
   h->vel.assign(nblocks, {});
-  HIP_TRY(hipHostMalloc((void **)&h->h_stats, nblocks * sizeof(RedPartial), hipHostMallocDefault));
+  if (h->pool) {
+    // a run of the pool's pinned statistics entries
+    std::lock_guard<std::mutex> lock(h->pool->mu);
+    const int cap = (int)h->pool->hs_used.size();
+    int first = -1;
+    for (int i = 0, run = 0; i < cap && first < 0; ++i) {
+      run = h->pool->hs_used[i] ? 0 : run + 1;
+      if (run == nblocks) first = i - nblocks + 1;
+    }
+    if (first < 0)
+      return fail(H3DMG_ERR_NOMEM, "pool: no run of %d free block entries (of %d)", nblocks, cap);
+    for (int b = 0; b < nblocks; ++b) h->pool->hs_used[first + b] = 1;
+    h->hs_slot = first;
+    h->h_stats = h->pool->h_stats + first;
+  } else {
+    HIP_TRY(hipHostMalloc((void **)&h->h_stats, nblocks * sizeof(RedPartial),
+                          hipHostMallocDefault));
+  }
   if (exact) {
     std::vector<double *> ptrs((size_t)nblocks * 8);
     size_t ed = 0;
     for (int b = 0; b < nblocks; ++b) {
+      // a block carried over into a pooled handle keeps its tables
+      const bool upload = fresh == nullptr || fresh[b];
       for (int a = 0; a < 3; ++a) {
-        ptrs[8 * b + a] = (double *)(A + eoff[8 * b + a]);
-        HIP_TRY(hipMemcpy(ptrs[8 * b + a], edata[ed].data(), edata[ed].size() * 8,
-                          hipMemcpyHostToDevice));
+        ptrs[8 * b + a] = (double *)(R[b] + eoff[8 * b + a]);
+        if (upload)
+          HIP_TRY(hipMemcpyAsync(ptrs[8 * b + a], edata[ed].data(), edata[ed].size() * 8, hipMemcpyHostToDevice, h->stream));
         ++ed;
-        ptrs[8 * b + 3 + a] = (double *)(A + eoff[8 * b + 3 + a]);
-        HIP_TRY(hipMemcpy(ptrs[8 * b + 3 + a], edata[ed].data(), edata[ed].size() * 8,
-                          hipMemcpyHostToDevice));
+        ptrs[8 * b + 3 + a] = (double *)(R[b] + eoff[8 * b + 3 + a]);
+        if (upload)
+          HIP_TRY(hipMemcpyAsync(ptrs[8 * b + 3 + a], edata[ed].data(), edata[ed].size() * 8, hipMemcpyHostToDevice, h->stream));
         ++ed;
       }
-      ptrs[8 * b + 6] = (double *)(A + eoff[8 * b + 6]);
-      ptrs[8 * b + 7] = (double *)(A + eoff[8 * b + 7]);
+      ptrs[8 * b + 6] = (double *)(R[b] + eoff[8 * b + 6]);
+      ptrs[8 * b + 7] = (double *)(R[b] + eoff[8 * b + 7]);
     }
     // the kernel indexes q/l tables with stride 3 and scratch with stride 2
     std::vector<double *> q3(3 * nblocks), l3(3 * nblocks), s2(2 * nblocks);
@@ -1093,16 +1250,17 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     h->d_eig_q = base;
     h->d_eig_l = base + 3 * nblocks;
     h->d_scratch = base + 6 * nblocks;
-    HIP_TRY(hipMemcpy(h->d_eig_q, q3.data(), q3.size() * sizeof(double *), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->d_eig_l, l3.data(), l3.size() * sizeof(double *), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->d_scratch, s2.data(), s2.size() * sizeof(double *),
-                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(h->d_eig_q, q3.data(), q3.size() * sizeof(double *), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->d_eig_l, l3.data(), l3.size() * sizeof(double *), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->d_scratch, s2.data(), s2.size() * sizeof(double *), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));  // the host tables above go out of scope
   }
-  // p = 0, f = 0
+  // p = 0, f = 0 (a block carried over into a pooled handle keeps its own)
   for (int b = 0; b < nblocks; ++b) {
+    if (fresh && !fresh[b]) continue;
     const LevelDev &L = h->levs[b];
     const size_t half = (size_t)L.H * L.nj * L.nk * 8;
-    for (int q = 0; q < 4; ++q) HIP_TRY(hipMemset(A + foff[4 * b + q], 0, half));
+    for (int q = 0; q < 4; ++q) HIP_TRY(hipMemsetAsync(R[b] + foff[4 * b + q], 0, half, h->stream));
   }
   h->march_kb.assign(Lm + 1, 0);
   for (int g = 1; g <= Lm; ++g) {
@@ -1115,8 +1273,17 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks) {
     h3dmg_status st = upload_tail(h);
     if (st != H3DMG_OK) return st;
   }
-  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return H3DMG_OK;
+}
+
+// whether the 16^3 GS kernel may use its dynamic LDS (the attribute is
+// process-wide: set once, to what the kernel needs)
+bool gs_big_ok() {
+  static std::once_flag once;
+  static bool ok = false;
+  std::call_once(once, []() { ok = gs_allow_lds() == hipSuccess; });
+  return ok;
 }
 
 h3dmg_status upload_relax(h3dmg_handle *h, int g) {
@@ -1156,7 +1323,18 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
       ts.depth = 1;
       ts.diag_begin.clear();
       if (h->prm.smoother == H3DMG_SMOOTH_GS) {
-        auto tiles_of = [](int n) { return (n + GS_TILE - 1) / GS_TILE; };
+        // 16^3 tiles where every block of the group is at least 16 cells
+        // on each axis and the operator has constant coefficients (8^3
+        // tiles stage the variable-density coefficients as well)
+        bool big = !h->lsm && gs_big_ok();
+        for (const BlockInfo &B : h->blk)
+          if (member(B)) {
+            const LevelDev &L = h->levs[B.desc[g]];
+            big = big && std::min(L.ni, std::min(L.nj, L.nk)) >= GS_BIG_TILE;
+          }
+        const int gt = big ? GS_BIG_TILE : GS_TILE;
+        ts.gs_tile = gt;
+        auto tiles_of = [gt](int n) { return (n + gt - 1) / gt; };
         int nd = 0;
         for (const BlockInfo &B : h->blk)
           if (member(B)) {
@@ -1174,7 +1352,7 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
                 for (int J = 0; J < tiles_of(L.nj); ++J) {
                   const int I = D - K - J;
                   if (I >= 0 && I < tiles_of(L.ni))
-                    ts.h.push_back(Tile{d, I * GS_TILE, J * GS_TILE, K * GS_TILE, K * GS_TILE + 1, 5});
+                    ts.h.push_back(Tile{d, I * gt, J * gt, K * gt, K * gt + 1, 5});
                 }
             }
         }
@@ -1234,10 +1412,13 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
         return fail(H3DMG_ERR_ARG, "internal: tile depth");
     Tile *d = nullptr;
     if (!ts.h.empty()) {
-      HIP_TRY(hipMalloc((void **)&d, ts.h.size() * sizeof(Tile)));
-      HIP_TRY(hipMemcpy(d, ts.h.data(), ts.h.size() * sizeof(Tile), hipMemcpyHostToDevice));
+      h3dmg_status st = dev_alloc(h, ts.h.size() * sizeof(Tile), (void **)&d);
+      if (st) return st;
+      HIP_TRY(hipMemcpyAsync(d, ts.h.data(), ts.h.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                             h->stream));
     }
-    if (ts.owned) HIP_TRY(hipFree(ts.owned));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    dev_free(h, ts.owned);
     ts.owned = d;
     ts.d = d;
   }
@@ -1293,10 +1474,13 @@ h3dmg_status upload_tail(h3dmg_handle *h) {
       if (h->blk[b].L >= h->tail_lt) ts.h.push_back(Tile{h->blk[b].desc[h->tail_lt], b, 0, 0, 0, 0});
   Tile *d = nullptr;
   if (!ts.h.empty()) {
-    HIP_TRY(hipMalloc((void **)&d, ts.h.size() * sizeof(Tile)));
-    HIP_TRY(hipMemcpy(d, ts.h.data(), ts.h.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    h3dmg_status st = dev_alloc(h, ts.h.size() * sizeof(Tile), (void **)&d);
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(d, ts.h.data(), ts.h.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                           h->stream));
   }
-  if (ts.owned) HIP_TRY(hipFree(ts.owned));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  dev_free(h, ts.owned);
   ts.owned = d;
   ts.d = d;
   return H3DMG_OK;
@@ -1510,9 +1694,16 @@ static void destroy_locked(h3dmg_handle *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graph(h);
   for (auto *v : {&h->relax_nc, &h->relax_c})
-    for (TileSet &ts : *v)
-      if (ts.owned) (void)hipFree(ts.owned);
-  if (h->tail_tiles.owned) (void)hipFree(h->tail_tiles.owned);
+    for (TileSet &ts : *v) dev_free(h, ts.owned);
+  dev_free(h, h->tail_tiles.owned);
+  if (h->pool) {
+    if (h->glob_taken) (void)pool_give(h->pool, h->glob_off);
+    if (h->hs_slot >= 0) {
+      std::lock_guard<std::mutex> lock(h->pool->mu);
+      for (int b = 0; b < h->nb; ++b) h->pool->hs_used[h->hs_slot + b] = 0;
+    }
+    h->h_stats = nullptr;  // the pool's
+  }
   if (h->h_stats) (void)hipHostFree(h->h_stats);
   if (h->vel_store) (void)hipFree(h->vel_store);
   if (h->lsm_store) (void)hipFree(h->lsm_store);
@@ -1521,64 +1712,185 @@ static void destroy_locked(h3dmg_handle *h) {
   delete h;
 }
 
-h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_params *prm,
-                          h3dmg_handle **out) {
-  std::lock_guard<std::mutex> lock(g_device_mutex);
-  if (!out) return fail(H3DMG_ERR_ARG, "null output handle pointer");
-  *out = nullptr;
-  if (!blocks || nblocks < 1) return fail(H3DMG_ERR_ARG, "no blocks");
-  h3dmg_handle *h = new h3dmg_handle();
-  if (prm)
-    h->prm = *prm;
-  else
-    h3dmg_default_params(&h->prm);
-  const h3dmg_params &P = h->prm;
+}  // extern "C"
+
+namespace {
+
+h3dmg_status check_params(const h3dmg_params &P) {
   if (P.smoother != H3DMG_SMOOTH_GS && P.smoother != H3DMG_SMOOTH_TDMA &&
-      P.smoother != H3DMG_SMOOTH_RBGS) {
-    delete h;
+      P.smoother != H3DMG_SMOOTH_RBGS)
     return fail(H3DMG_ERR_ARG, "unknown smoother %d", P.smoother);
-  }
-  if (P.smoother != H3DMG_SMOOTH_RBGS && P.precision != H3DMG_FP64) {
-    delete h;
+  if (P.smoother != H3DMG_SMOOTH_RBGS && P.precision != H3DMG_FP64)
     return fail(H3DMG_ERR_UNSUPPORTED, "the GS and line-TDMA smoothers run in FP64");
-  }
   if (P.nu1 < 0 || P.nu2 < 0 || P.nu1 + P.nu2 < 1 || P.maxcy < 0 ||
       (P.precision != H3DMG_FP64 && P.precision != H3DMG_MIXED) ||
       (P.coarse != H3DMG_COARSE_SMOOTH && P.coarse != H3DMG_COARSE_EXACT) ||
-      (P.criterion != H3DMG_CRIT_REL_L2 && P.criterion != H3DMG_CRIT_REF_RMAX)) {
-    delete h;
+      (P.criterion != H3DMG_CRIT_REL_L2 && P.criterion != H3DMG_CRIT_REF_RMAX))
     return fail(H3DMG_ERR_ARG, "invalid parameters");
-  }
-  {
-    const h3dmg_status vs = validate_blocks(blocks, nblocks, P);
-    if (vs != H3DMG_OK) {
-      delete h;
-      return vs;
-    }
+  return H3DMG_OK;
+}
+
+// h3dmg_create and h3dmg_create_pooled; the caller holds g_device_mutex
+h3dmg_status create_locked(const h3dmg_block *blocks, int nblocks, const h3dmg_params *prm,
+                           h3dmg_pool *pool, const unsigned long long *offsets,
+                           const int *fresh, void *stream, h3dmg_handle **out) {
+  if (!out) return fail(H3DMG_ERR_ARG, "null output handle pointer");
+  *out = nullptr;
+  if (!blocks || nblocks < 1) return fail(H3DMG_ERR_ARG, "no blocks");
+  h3dmg_params P;
+  if (prm)
+    P = *prm;
+  else
+    h3dmg_default_params(&P);
+  if (h3dmg_status st = check_params(P)) return st;
+  if (h3dmg_status st = validate_blocks(blocks, nblocks, P)) return st;
+  if (pool) {
+    if (!offsets) return fail(H3DMG_ERR_ARG, "null block offsets");
+    for (int b = 0; b < nblocks; ++b)
+      for (int f = 0; f < 6; ++f)
+        if (blocks[b].bc[f] == H3DMG_BC_NEIGHBOUR)
+          return fail(H3DMG_ERR_UNSUPPORTED,
+                      "a pooled handle holds independent blocks (block %d has a neighbour)", b);
   }
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) {
-    delete h;
-    return fail(H3DMG_ERR_DEVICE, "no HIP device");
-  }
+  if (hipGetDevice(&dev) != hipSuccess) return fail(H3DMG_ERR_DEVICE, "no HIP device");
+  if (pool && dev != pool->device)
+    return fail(H3DMG_ERR_ARG, "the pool lives on device %d, the current device is %d",
+                pool->device, dev);
+  h3dmg_handle *h = new h3dmg_handle();
+  h->prm = P;
   h->device = dev;
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
     lds = 65536;
   // the tail kernel's static LDS (its level descriptors) comes on top
   h->lds_budget = lds > 8192 ? (size_t)lds - 8192 : 0;
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete h;
-    return fail(H3DMG_ERR_DEVICE, "hipStreamCreate failed");
+  if (stream) {
+    h->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete h;
+      return fail(H3DMG_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    h->own_stream = true;
   }
-  h->own_stream = true;
-  h3dmg_status st = build(h, blocks, nblocks);
+  if (pool) {
+    h->pool = pool;
+    h->blk_off.assign(offsets, offsets + nblocks);
+  }
+  h3dmg_status st = build(h, blocks, nblocks, fresh);
   if (st != H3DMG_OK) {
-    destroy_locked(h);  // the device mutex is held here
+    destroy_locked(h);
     return st;
   }
+  h->running.assign(nblocks, 0);
+  h->outcome.assign(nblocks, -1);
   *out = h;
   return H3DMG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+h3dmg_status h3dmg_create(const h3dmg_block *blocks, int nblocks, const h3dmg_params *prm,
+                          h3dmg_handle **out) {
+  std::lock_guard<std::mutex> lock(g_device_mutex);
+  return create_locked(blocks, nblocks, prm, nullptr, nullptr, nullptr, nullptr, out);
+}
+
+h3dmg_status h3dmg_pool_create(unsigned long long bytes, int max_blocks, h3dmg_pool **out) {
+  std::lock_guard<std::mutex> lock(g_device_mutex);
+  if (!out) return fail(H3DMG_ERR_ARG, "null output pool pointer");
+  *out = nullptr;
+  if (bytes < 256 || max_blocks < 1) return fail(H3DMG_ERR_ARG, "empty pool");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(H3DMG_ERR_DEVICE, "no HIP device");
+  h3dmg_pool *p = new h3dmg_pool();
+  p->device = dev;
+  p->bytes = align_up(bytes, 256);
+  hipError_t e = hipMalloc((void **)&p->base, p->bytes);
+  if (e != hipSuccess) {
+    delete p;
+    return fail(H3DMG_ERR_NOMEM, "hipMalloc(%llu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  e = hipHostMalloc((void **)&p->h_stats, (size_t)max_blocks * sizeof(RedPartial),
+                    hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipFree(p->base);
+    delete p;
+    return fail(H3DMG_ERR_NOMEM, "hipHostMalloc failed: %s", hipGetErrorString(e));
+  }
+  p->hs_used.assign(max_blocks, 0);
+  p->free_[0] = p->bytes;
+  *out = p;
+  return H3DMG_OK;
+}
+
+void h3dmg_pool_destroy(h3dmg_pool *pool) {
+  if (!pool) return;
+  std::lock_guard<std::mutex> lock(g_device_mutex);
+  (void)hipDeviceSynchronize();
+  (void)hipFree(pool->base);
+  (void)hipHostFree(pool->h_stats);
+  delete pool;
+}
+
+h3dmg_status h3dmg_pool_alloc(h3dmg_pool *pool, unsigned long long bytes,
+                              unsigned long long *offset) {
+  if (!pool || !offset) return fail(H3DMG_ERR_ARG, "null pool or offset");
+  size_t off = 0;
+  h3dmg_status st = pool_take(pool, bytes, &off);
+  if (st) return st;
+  *offset = off;
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_pool_free(h3dmg_pool *pool, unsigned long long offset) {
+  if (!pool) return fail(H3DMG_ERR_ARG, "null pool");
+  return pool_give(pool, offset);
+}
+
+unsigned long long h3dmg_pool_free_bytes(h3dmg_pool *pool) {
+  if (!pool) return 0;
+  std::lock_guard<std::mutex> lock(pool->mu);
+  size_t n = 0;
+  for (const auto &f : pool->free_) n += f.second;
+  return n;
+}
+
+unsigned long long h3dmg_pool_largest_free(h3dmg_pool *pool) {
+  if (!pool) return 0;
+  std::lock_guard<std::mutex> lock(pool->mu);
+  size_t n = 0;
+  for (const auto &f : pool->free_) n = std::max(n, f.second);
+  return n;
+}
+
+unsigned long long h3dmg_block_bytes(const h3dmg_block *blk, const h3dmg_params *prm) {
+  if (!blk) {
+    fail(H3DMG_ERR_ARG, "null block");
+    return 0;
+  }
+  h3dmg_params P;
+  if (prm)
+    P = *prm;
+  else
+    h3dmg_default_params(&P);
+  if (check_params(P) || validate_blocks(blk, 1, P)) return 0;
+  h3dmg_handle tmp;
+  tmp.prm = P;
+  size_t bytes = 0;
+  if (build(&tmp, blk, 1, nullptr, &bytes)) return 0;
+  return bytes;
+}
+
+h3dmg_status h3dmg_create_pooled(h3dmg_pool *pool, const h3dmg_block *blocks, int nblocks,
+                                 const h3dmg_params *prm, const unsigned long long *offsets,
+                                 const int *fresh, void *stream, h3dmg_handle **out) {
+  if (!pool) return fail(H3DMG_ERR_ARG, "null pool");
+  std::lock_guard<std::mutex> lock(g_device_mutex);
+  return create_locked(blocks, nblocks, prm, pool, offsets, fresh, stream, out);
 }
 
 void h3dmg_destroy(h3dmg_handle *h) {
@@ -1603,10 +1915,15 @@ h3dmg_status h3dmg_set_stream(h3dmg_handle *h, void *stream) {
   return H3DMG_OK;
 }
 
+static h3dmg_status no_stage() {
+  return fail(H3DMG_ERR_UNSUPPORTED, "a pooled handle takes device arrays only");
+}
+
 // host Fortran array (i fastest, leading dims ld1, ld2, interior at off) ->
 // dense interior staging on the device
 static h3dmg_status host_to_stage(h3dmg_handle *h, int b, const double *src, int ld1, int ld2,
                                   int off) {
+  if (!h->d_stage) return no_stage();
   const h3dmg_block &S = h->blk[b].spec;
   if (!src || ld1 < S.ni + off || ld2 < S.nj + off || off < 0)
     return fail(H3DMG_ERR_ARG, "bad host array description");
@@ -1711,6 +2028,7 @@ h3dmg_status h3dmg_set_p(h3dmg_handle *h, int b, const double *p, int ld1, int l
 h3dmg_status h3dmg_get_p(h3dmg_handle *h, int b, double *p, int ld1, int ld2, int off) {
   h3dmg_status st = check_block(h, b);
   if (st) return st;
+  if (!h->d_stage) return no_stage();
   // the interior and the ghost layer (the cells pbound and exchange(4)
   // copy into it: mirror, wrap or the neighbour block's cell)
   const FineDev &F = h->fine[b];
@@ -1754,6 +2072,7 @@ h3dmg_status h3dmg_vcycle(h3dmg_handle *h) {
 h3dmg_status h3dmg_get_pp(h3dmg_handle *h, int b, double *pp, int ld1, int ld2, int off) {
   h3dmg_status st = check_block(h, b);
   if (st) return st;
+  if (!h->d_stage) return no_stage();
   const LevelDev &L = h->levs[b];
   if (h->lev_is64[b])
     launch_unpack_level<double>(h->stream, h->d_levs, b, 0, h->pp_buf, h->d_stage, L.ni, L.nj,
@@ -1875,6 +2194,7 @@ h3dmg_status h3dmg_set_block_enabled(h3dmg_handle *h, int b, int enabled) {
   if (h->connected)
     return fail(H3DMG_ERR_UNSUPPORTED, "blocks with neighbours are solved together");
   h->blk[b].enabled = enabled != 0;
+  if (!enabled) h->running[b] = 0;
   // the velocity loop's block set (and its captured step, which projects
   // the blocks chosen at h3dmg_velocity_begin) is no longer this one
   if (h->vel_ready || h->vgexec) {
@@ -1886,6 +2206,122 @@ h3dmg_status h3dmg_set_block_enabled(h3dmg_handle *h, int b, int enabled) {
     h->vgraph = nullptr;
     h->vel_ready = false;
   }
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_get_block_state(const h3dmg_handle *h, int b, h3dmg_block_state *out) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!out) return fail(H3DMG_ERR_ARG, "null state");
+  const BlockInfo &B = h->blk[b];
+  out->running = h->running[b];
+  out->cycles = B.cycles;
+  out->status = h->outcome[b];
+  out->enabled = B.enabled ? 1 : 0;
+  out->fnorm = B.fnorm;
+  out->rel = B.rel;
+  out->tol = B.tol;
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_set_block_state(h3dmg_handle *h, int b, const h3dmg_block_state *in) {
+  h3dmg_status st = check_block(h, b);
+  if (st) return st;
+  if (!in) return fail(H3DMG_ERR_ARG, "null state");
+  if (!(in->fnorm >= 0.0) || !(in->tol >= 0.0) || in->cycles < 0)
+    return fail(H3DMG_ERR_ARG, "invalid block state");
+  if (h->connected) return fail(H3DMG_ERR_UNSUPPORTED, "blocks with neighbours are solved together");
+  BlockInfo &B = h->blk[b];
+  B.cycles = in->cycles;
+  B.fnorm = in->fnorm;
+  B.rel = in->rel;
+  B.tol = in->tol;
+  B.enabled = in->enabled != 0;
+  h->running[b] = (in->running && B.enabled) ? 1 : 0;
+  h->outcome[b] = in->status;
+  h->rhs_ready = true;
+  return H3DMG_OK;
+}
+
+namespace {
+// tolerance and stopping rule of one block's stepped solve (the REL_L2 rule
+// of h3dmg_solve): returns the outcome once it stops, -1 while it runs
+int stepped_outcome(const h3dmg_handle *h, int b, double nrm) {
+  const h3dmg_params &P = h->prm;
+  const BlockInfo &B = h->blk[b];
+  if (!std::isfinite(nrm)) return H3DMG_ERR_DIVERGED;
+  if (B.cycles >= P.min_cycles && B.rel <= (B.tol > 0 ? B.tol : P.tol)) return H3DMG_OK;
+  if (B.cycles >= P.maxcy) return H3DMG_ERR_NOT_CONVERGED;
+  return -1;
+}
+
+h3dmg_status check_stepped(const h3dmg_handle *h) {
+  if (!h) return fail(H3DMG_ERR_ARG, "null handle");
+  if (h->prm.criterion != H3DMG_CRIT_REL_L2)
+    return fail(H3DMG_ERR_UNSUPPORTED, "stepped solves stop each block on its own (REL_L2 only)");
+  if (h->connected)
+    return fail(H3DMG_ERR_UNSUPPORTED, "blocks with neighbours are solved together (h3dmg_solve)");
+  return check_density(h);
+}
+}  // namespace
+
+h3dmg_status h3dmg_solve_begin(h3dmg_handle *h, const int *blocks, int n) {
+  h3dmg_status st = check_stepped(h);
+  if (st) return st;
+  if (!blocks || n < 1) return fail(H3DMG_ERR_ARG, "no blocks to begin");
+  std::vector<char> pick(h->nb, 0);
+  for (int i = 0; i < n; ++i) {
+    const int b = blocks[i];
+    if ((st = check_block(h, b))) return st;
+    if (!h->blk[b].enabled) return fail(H3DMG_ERR_ARG, "block %d is disabled", b);
+    if (pick[b]) return fail(H3DMG_ERR_ARG, "block %d listed twice", b);
+    pick[b] = 1;
+  }
+  // r = f - A p of the listed blocks only: the running blocks' residuals
+  // (the next V-cycle's right-hand sides) stay as they are
+  for (int b = 0; b < h->nb; ++b) h->active[b] = pick[b];
+  if ((st = push_active(h))) return st;
+  enqueue_update_residual(h, false);
+  HIP_TRY(hipGetLastError());
+  if ((st = read_stats(h))) return st;
+  for (int b = 0; b < h->nb; ++b) {
+    if (!pick[b]) continue;
+    BlockInfo &B = h->blk[b];
+    const double nrm = std::sqrt(h->h_stats[b].ss);
+    B.cycles = 0;
+    B.rel = B.fnorm > 0 ? nrm / B.fnorm : nrm;
+    B.rmax = 0.0;
+    h->outcome[b] = stepped_outcome(h, b, nrm);
+    h->running[b] = h->outcome[b] < 0;
+  }
+  return H3DMG_OK;
+}
+
+h3dmg_status h3dmg_solve_step(h3dmg_handle *h, int *nrunning) {
+  h3dmg_status st = check_stepped(h);
+  if (st) return st;
+  int count = 0;
+  for (int b = 0; b < h->nb; ++b) {
+    h->active[b] = h->running[b] && h->blk[b].enabled;
+    count += h->active[b];
+  }
+  if (count) {
+    if ((st = push_active(h))) return st;
+    if ((st = run_cycle(h))) return st;
+    if ((st = read_stats(h))) return st;
+    count = 0;
+    for (int b = 0; b < h->nb; ++b) {
+      if (!h->active[b]) continue;
+      BlockInfo &B = h->blk[b];
+      const double nrm = std::sqrt(h->h_stats[b].ss);
+      B.cycles += 1;
+      B.rel = B.fnorm > 0 ? nrm / B.fnorm : nrm;
+      h->outcome[b] = stepped_outcome(h, b, nrm);
+      h->running[b] = h->outcome[b] < 0;
+      count += h->running[b];
+    }
+  }
+  if (nrunning) *nrunning = count;
   return H3DMG_OK;
 }
 
@@ -1979,6 +2415,7 @@ h3dmg_status h3dmg_project_device_noalloc(h3dmg_handle *h, int b, const double *
 namespace {
 h3dmg_status alloc_velocity(h3dmg_handle *h) {
   if (h->vel_store) return H3DMG_OK;
+  if (h->pool) return fail(H3DMG_ERR_UNSUPPORTED, "a pooled handle has no velocity loop");
   h->vel_off.assign(9 * (size_t)h->nb, 0);
   size_t off = 0;
   for (int b = 0; b < h->nb; ++b) {
@@ -2129,6 +2566,7 @@ h3dmg_status h3dmg_get_velocity_calvel(h3dmg_handle *h, int b, double *u, double
 h3dmg_status h3dmg_get_su(h3dmg_handle *h, int b, double *su, int ld1, int ld2, int off) {
   h3dmg_status st = check_block(h, b);
   if (st) return st;
+  if (!h->d_stage) return no_stage();
   const h3dmg_block &S = h->blk[b].spec;
   if (!su || off < 0 || ld1 < S.ni + off || ld2 < S.nj + off)
     return fail(H3DMG_ERR_ARG, "bad host array description");
@@ -2151,6 +2589,8 @@ h3dmg_status h3dmg_get_su(h3dmg_handle *h, int b, double *su, int ld1, int ld2, 
 namespace {
 h3dmg_status enable_lsm(h3dmg_handle *h) {
   if (h->lsm) return H3DMG_OK;
+  if (h->pool)
+    return fail(H3DMG_ERR_UNSUPPORTED, "a pooled handle has no variable-density operator");
   if (h->connected)
     return fail(H3DMG_ERR_UNSUPPORTED,
                 "the variable-density operator on blocks with neighbours is not built");
@@ -2292,6 +2732,7 @@ h3dmg_status h3dmg_velocity_begin(h3dmg_handle *h, const double *const *d_ustar,
   const bool own = !d_ustar && !d_vstar && !d_wstar && !d_u && !d_v && !d_w;
   if (!own && (!d_ustar || !d_vstar || !d_wstar || !d_u || !d_v || !d_w))
     return fail(H3DMG_ERR_ARG, "null velocity array list");
+  if (h->pool) return fail(H3DMG_ERR_UNSUPPORTED, "a pooled handle has no velocity loop");
   if (!(h->prm.dt > 0)) return fail(H3DMG_ERR_ARG, "the velocity form needs params.dt > 0");
   if (h3dmg_status st = check_density(h)) return st;
   if (h->differencing == 2) {
@@ -2555,6 +2996,31 @@ h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes) {
   drop_graph(h);
   h->march_kb[level] = planes > 0 ? planes : 0;
   return upload_relax(h, level);
+}
+
+h3dmg_status h3dmg_set_lanes(h3dmg_handle *h, int level, int lanes) {
+  if (!h || level < 1 || level > h->Lmax) return fail(H3DMG_ERR_ARG, "bad level");
+  if (lanes < 0 || lanes > 64 || (lanes & (lanes - 1)))
+    return fail(H3DMG_ERR_ARG, "lanes per row: 0 or a power of two up to 64");
+  std::lock_guard<std::mutex> lock(g_device_mutex);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  drop_graph(h);
+  for (const BlockInfo &B : h->blk) {
+    if (B.L < level) continue;
+    LevelDev &L = h->levs[B.desc[level]];
+    const int natural = lanes_per_row(L.H);
+    L.lpr = lanes ? std::min(lanes, natural) : natural;
+  }
+  HIP_TRY(hipMemcpyAsync(h->d_levs, h->levs.data(), h->levs.size() * sizeof(LevelDev),
+                         hipMemcpyHostToDevice, h->stream));
+  return upload_relax(h, level);  // synchronises the stream
+}
+
+int h3dmg_get_lanes(const h3dmg_handle *h, int level) {
+  if (!h || level < 1 || level > h->Lmax) return -1;
+  for (const BlockInfo &B : h->blk)
+    if (B.L >= level) return h->levs[B.desc[level]].lpr;
+  return -1;
 }
 
 h3dmg_status h3dmg_set_tail(h3dmg_handle *h, int level) {

@@ -41,7 +41,8 @@ module h3dmg_mod
   public :: h3dmg_default_params, h3dmg_create, h3dmg_destroy, h3dmg_device_bytes, &
        h3dmg_set_rhs, h3dmg_set_p, h3dmg_get_p, h3dmg_vcycle, h3dmg_get_pp, &
        h3dmg_solve, h3dmg_block_stats, h3dmg_remove_mean, h3dmg_set_stream, &
-       h3dmg_set_march, h3dmg_get_march, h3dmg_set_tail, h3dmg_get_tail, &
+       h3dmg_set_march, h3dmg_get_march, h3dmg_set_lanes, h3dmg_get_lanes, &
+       h3dmg_set_tail, h3dmg_get_tail, &
        h3dmg_set_block_tol, h3dmg_set_block_enabled, h3dmg_project_device, &
        h3dmg_velocity_begin, h3dmg_velocity_step, h3dmg_solve_velocity, &
        h3dmg_get_su_device, h3dmg_set_velocity, h3dmg_get_velocity, h3dmg_get_su, &
@@ -291,6 +292,19 @@ module h3dmg_mod
        type(c_ptr), value :: handle
        integer(c_int), value :: level
      end function h3dmg_get_march
+
+     integer(c_int) function h3dmg_set_lanes(handle, level, lanes) &
+          bind(C, name="h3dmg_set_lanes")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: handle
+       integer(c_int), value :: level, lanes
+     end function h3dmg_set_lanes
+
+     integer(c_int) function h3dmg_get_lanes(handle, level) bind(C, name="h3dmg_get_lanes")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: handle
+       integer(c_int), value :: level
+     end function h3dmg_get_lanes
   end interface
 
 contains

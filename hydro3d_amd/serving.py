@@ -3,23 +3,32 @@
 Requests (one block each: size, spacing, boundaries, right-hand side,
 tolerance, optional residency key) arrive at any time.  The server
 
-* admits queued requests by HBM footprint (hierarchies of every admitted or
-  resident request must fit the budget; idle resident hierarchies are evicted
-  least-recently-used first),
-* batches the requests that are waiting when a worker slot frees up into ONE
-  multi-block handle -- one launch per level and operator covers all of them,
-  each block with its own tolerance (h3dmg_set_block_tol) and its own
-  retirement,
-* runs up to `max_batches` handles at once, each on its own thread and HIP
-  stream, so that work admitted while others are on the GPU overlaps with
-  them (continuous batching, not a queue),
-* keeps every finished hierarchy and its pressure resident under the
-  request's key: a later request with the same key reuses it, warm-started
-  from the previous pressure (h3dmg_set_block_enabled restricts the solve to
-  the requested blocks of a resident handle).
+* admits a request by HBM footprint: the block's region (its hierarchy,
+  pressure and coarse-solve tables, h3dmg_block_bytes) is cut from one
+  device pool made at start-up.  When the pool is full, idle resident
+  regions are evicted least-recently-used first; a request that still does
+  not fit waits until running solves leave, and one larger than the whole
+  pool fails at once;
+* runs `lanes` batches at once, each a pooled handle on its own host thread
+  and HIP stream; a request goes to the lane of its size class (cells), so
+  small solves do not step in lock-step with the largest ones;
+* admits waiting requests INTO a running batch at the next V-cycle
+  boundary: the lane builds a new handle over the running blocks plus the
+  newcomers (the running blocks keep their state in their regions,
+  h3dmg_create_pooled with fresh = 0, their counters carried), starts the
+  newcomers (h3dmg_solve_begin) and steps every running block one V-cycle
+  per launch sequence (h3dmg_solve_step); each block leaves when it meets
+  its own tolerance;
+* keeps a finished block's region -- hierarchy and pressure -- resident
+  under the request's key: a later request with the same key joins with
+  that region, warm-started from the previous pressure.
 
 Blocks of one handle never interact (per-block norms, tolerances and
-retirement), so each result is the one the request gets alone.
+stopping), so each result is bitwise the one the request gets alone.
+Nothing is allocated on the request path: device memory comes from the pool
+(block regions and every handle's tables), the per-block statistics from the
+pool's pinned array; only the result tensor comes from torch's caching
+allocator.
 
     python -m hydro3d_amd.serving --requests 2000 --rate 200 --seed 4
 """
@@ -38,37 +47,6 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .configs import levels_for
-
-
-# ---- footprint ------------------------------------------------------------
-def _align(x, a=256):
-    return (x + a - 1) // a * a
-
-
-def footprint_bytes(blocks, precision="mixed", coarse="exact") -> int:
-    """Device bytes a handle over `blocks` allocates (the arena layout of
-    h3dmg_create plus its separately allocated tile tables, rounded up)."""
-    mixed = precision == "mixed"
-    exact = coarse == "exact"
-    total = 0
-    stage = 0
-    for b in blocks:
-        ni, nj, nk = b["n"]
-        L = b["nlev"]
-        for g in range(1, L + 1):
-            n = (ni >> (g - 1), nj >> (g - 1), nk >> (g - 1))
-            H = (((n[0] + 1) // 2) + 1) & ~1
-            w = 4 if (mixed and g < L) else 8
-            total += 6 * _align(H * n[1] * n[2] * w)
-            # tile tables (smoother, restriction, prolongation, zero fill)
-            total += 24 * max(1, (H * n[1] * n[2]) // 512) * 4
-        H = (((ni + 1) // 2) + 1) & ~1
-        total += 4 * _align(H * nj * nk * 8)
-        if exact:
-            c = (ni >> (L - 1), nj >> (L - 1), nk >> (L - 1))
-            total += sum(_align(x * x * 8) + _align(x * 8) for x in c) + 2 * _align(c[0] * c[1] * c[2] * 8)
-        stage = max(stage, (ni + 2) * (nj + 2) * (nk + 2) * 8)
-    return int((total + stage) * 1.05) + (1 << 20)
 
 
 # ---- requests and results -------------------------------------------------
@@ -93,56 +71,127 @@ class Request:
 class Result:
     rid: int
     p: object                 # device tensor (nk, nj, ni): the pressure
-    status: int
+    status: int               # 0 converged, 5 maxcy reached, 6 diverged
     cycles: int
     rel_residual: float
     latency_s: float
-    batch_blocks: int
+    batch_blocks: float       # blocks stepping beside it, mean over its cycles (itself included)
     warm: bool
 
 
 @dataclass
-class _Resident:
-    solver: object
-    keys: list                # key of each block (None: not reusable)
+class _Region:
+    """A block's piece of the pool: its hierarchy and pressure."""
+    off: int
     nbytes: int
-    busy: bool = False
+    spec: dict
+    key: object
+    busy: bool = True         # queued for or held by a lane
+    fresh: bool = True        # no handle has used it yet: starts from zero
     last_used: float = field(default_factory=time.perf_counter)
-    specs: list = None        # block specs (known before the handle is built)
+
+
+@dataclass
+class _Job:
+    req: Request
+    fut: Future
+    region: _Region
+    warm: bool
+    b: int = -1               # index in the lane's current handle
+    running: bool = True
+    batch_sum: int = 0
+    steps: int = 0
+
+
+def _same_block(a, b):
+    return (tuple(a["n"]) == tuple(b["n"]) and tuple(a["bc"]) == tuple(b["bc"])
+            and tuple(a["h"]) == tuple(b["h"]) and a["nlev"] == b["nlev"])
+
+
+# ---- device backend -------------------------------------------------------
+class DeviceBackend:
+    """The pool and pooled handles of the device library.  The server only
+    needs this surface; tests substitute a host model of it."""
+
+    # device memory held back for the handles' own tables (per lane)
+    RESERVE_PER_LANE = 256 << 20
+
+    def __init__(self, nbytes, lanes, max_blocks, params):
+        from .solver import DevicePool
+        self.params = params
+        self.pool = DevicePool(nbytes, max_blocks)
+        self.capacity = nbytes - lanes * self.RESERVE_PER_LANE
+        self._bytes = {}
+
+    def block_bytes(self, spec):
+        from .solver import block_bytes
+        k = (tuple(spec["n"]), tuple(spec["bc"]), tuple(spec["h"]), spec["nlev"])
+        if k not in self._bytes:
+            self._bytes[k] = block_bytes(spec, precision=self.params["precision"],
+                                         coarse=self.params["coarse"])
+        return self._bytes[k]
+
+    def alloc(self, nbytes):
+        return self.pool.alloc(nbytes)  # MemoryError when no piece is free
+
+    def free(self, off):
+        self.pool.free(off)
+
+    def new_stream(self):
+        import torch
+        s = torch.cuda.Stream()
+        return s, s.cuda_stream
+
+    def handle(self, specs, offsets, fresh, stream):
+        from .solver import MultigridSolver
+        return MultigridSolver(specs, pool=self.pool, offsets=offsets, fresh=fresh,
+                               stream=stream, **self.params)
+
+    def close(self):
+        self.pool.close()
+
+
+# ---- the server -----------------------------------------------------------
+def default_lane_cells(lanes):
+    """Upper cell counts of the lanes' size classes: 64^3 and 128^3 for
+    three lanes (a quarter, half and a quarter of the stream's requests)."""
+    if lanes <= 1:
+        return []
+    if lanes == 2:
+        return [96 ** 3]
+    edges = np.geomspace(64 ** 3, 128 ** 3, lanes - 1)
+    return [int(x) for x in edges]
 
 
 class PoissonServer:
-    """Continuous-batching server over the device multigrid.
+    """Continuous-batching server over pooled device handles."""
 
-    solver_factory(specs, **kw) builds a handle (default: MultigridSolver);
-    tests substitute a host model of it."""
-
-    def __init__(self, hbm_budget=64 << 30, max_batches=4, max_batch_blocks=16,
-                 precision="mixed", coarse="exact", nu1=2, nu2=2, maxcy=60,
-                 solver_factory=None, keep_resident=True, use_graph=False):
-        if solver_factory is None:
-            from .solver import MultigridSolver as solver_factory  # noqa: N813
-        self._factory = solver_factory
-        self.precision, self.coarse = precision, coarse
-        # Handles run on several host threads; by default they launch eagerly
-        # (each on its own stream), since a handle's first solve would
-        # otherwise capture its graph while other threads create handles.
-        self._kw = dict(nu1=nu1, nu2=nu2, maxcy=maxcy, precision=precision, coarse=coarse,
-                        use_graph=use_graph)
-        self.budget = int(hbm_budget)
-        self.max_batches = max_batches
-        self.max_batch_blocks = max_batch_blocks
+    def __init__(self, hbm_budget=32 << 30, lanes=3, max_batch_blocks=32, precision="mixed",
+                 coarse="exact", nu1=2, nu2=2, maxcy=60, keep_resident=True, use_graph=False,
+                 lane_cells=None, backend=None):
+        self.params = dict(nu1=nu1, nu2=nu2, maxcy=maxcy, precision=precision, coarse=coarse,
+                           use_graph=use_graph)
+        self.lanes = int(lanes)
+        self.max_batch_blocks = int(max_batch_blocks)
         self.keep_resident = keep_resident
+        self.lane_cells = list(default_lane_cells(self.lanes) if lane_cells is None else lane_cells)
+        assert len(self.lane_cells) == self.lanes - 1
+        self.backend = backend or DeviceBackend(int(hbm_budget), self.lanes,
+                                                4 * self.lanes * self.max_batch_blocks + 64,
+                                                self.params)
+        self.capacity = self.backend.capacity
         self._lock = threading.Condition()
-        self._pending = collections.deque()          # (Request, Future) for new hierarchies
-        self._reuse = collections.deque()            # (Request, Future, _Resident, block)
-        self._resident = []                          # list of _Resident
-        self._by_key = {}                            # key -> (_Resident, block)
-        self._running = 0
+        self._admit = collections.deque()                    # (Request, Future) waiting
+        self._queues = [collections.deque() for _ in range(self.lanes)]  # admitted _Jobs
+        self._by_key = {}                                    # key -> _Region
+        self._idle = collections.OrderedDict()               # id(_Region) -> _Region, LRU first
+        self._used = 0                                       # bytes of live regions
         self._closed = False
         self.stats = collections.Counter()
-        self._thread = threading.Thread(target=self._dispatch, name="h3dmg-dispatch", daemon=True)
-        self._thread.start()
+        self._threads = [threading.Thread(target=self._lane, args=(q,), daemon=True,
+                                          name=f"h3dmg-lane{q}") for q in range(self.lanes)]
+        for t in self._threads:
+            t.start()
 
     # -- public
     def submit(self, req: Request) -> Future:
@@ -151,22 +200,14 @@ class PoissonServer:
         with self._lock:
             if self._closed:
                 raise RuntimeError("server closed")
-            hit = self._by_key.get(req.key) if req.key is not None else None
-            if hit is not None:
-                res, b = hit
-                sp = res.specs[b]
-                if tuple(sp["n"]) == tuple(req.n) and tuple(sp["bc"]) == tuple(req.bc):
-                    self._reuse.append((req, fut, res, b))
-                    self._lock.notify_all()
-                    return fut
-            self._pending.append((req, fut))
+            self._admit.append((req, fut))
+            self._try_admit()
             self._lock.notify_all()
         return fut
 
     def warmup(self, n=(32, 32, 32)):
         """Serve one small request and forget it: the kernels' code objects
-        and the device context are loaded before traffic arrives, instead of
-        by the first requests (whose handles are built concurrently).  The
+        and the device context are loaded before traffic arrives.  The
         counters start from zero afterwards."""
         d = dict(n=tuple(n), h=(1.0 / 64,) * 3, bc=(0,) * 6, tol=1e-6, rid=-1, t=0.0,
                  seed=0)
@@ -179,12 +220,21 @@ class PoissonServer:
         with self._lock:
             self._closed = True
             self._lock.notify_all()
-        self._thread.join()
-        for r in self._resident:
-            if r.solver is not None:
-                r.solver.close()
-        self._resident.clear()
-        self._by_key.clear()
+        for t in self._threads:
+            t.join()
+        with self._lock:
+            while self._admit:  # could never be admitted
+                _, fut = self._admit.popleft()
+                fut.set_exception(MemoryError("server closed before the request fit"))
+            for qu in self._queues:
+                while qu:
+                    job = qu.popleft()
+                    job.fut.set_exception(RuntimeError("server closed"))
+                    self._release(job)
+            for r in list(self._idle.values()):
+                self._free_region(r)
+            self._by_key.clear()
+        self.backend.close()
 
     def __enter__(self):
         return self
@@ -195,174 +245,227 @@ class PoissonServer:
     @property
     def resident_bytes(self):
         with self._lock:
-            return sum(r.nbytes for r in self._resident)
+            return self._used
 
-    # -- dispatcher
-    def _used(self):
-        return sum(r.nbytes for r in self._resident)
+    # -- admission (lock held)
+    def _lane_of(self, spec):
+        cells = spec["n"][0] * spec["n"][1] * spec["n"][2]
+        for q, top in enumerate(self.lane_cells):
+            if cells <= top:
+                return q
+        return self.lanes - 1
 
-    def _evict_for(self, need):
-        """Evict idle resident handles (LRU) until `need` more bytes fit."""
-        idle = sorted((r for r in self._resident if not r.busy), key=lambda r: r.last_used)
-        while self._used() + need > self.budget and idle:
-            r = idle.pop(0)
-            if any(x[2] is r for x in self._reuse):
-                continue
-            self._resident.remove(r)
-            for k in r.keys:
-                if k is not None and self._by_key.get(k, (None,))[0] is r:
-                    del self._by_key[k]
-            r.solver.close()
-            self.stats["evicted"] += 1
-        return self._used() + need <= self.budget
+    def _free_region(self, r):
+        self._idle.pop(id(r), None)
+        if r.key is not None and self._by_key.get(r.key) is r:
+            del self._by_key[r.key]
+        self.backend.free(r.off)
+        self._used -= r.nbytes
 
-    def _dispatch(self):
+    def _take_memory(self, nbytes):
+        """Pool offset of a new region, evicting idle resident regions (LRU)
+        as needed; None when it does not fit now."""
         while True:
-            with self._lock:
-                while not self._closed and (self._running >= self.max_batches or
-                                            not (self._pending or self._ready_reuse())):
-                    self._lock.wait()
-                if self._closed and not self._pending and not self._reuse and self._running == 0:
-                    return
-                if self._closed and self._running >= self.max_batches:
-                    self._lock.wait(0.01)
-                    continue
-                job = self._next_job()
-                if job is None:
-                    if self._closed and not self._running:
-                        # nothing admissible will ever fit: fail what is left
-                        while self._pending:
-                            req, fut = self._pending.popleft()
-                            fut.set_exception(MemoryError("request does not fit the HBM budget"))
-                        return
-                    self._lock.wait(0.05)
-                    continue
-                self._running += 1
-            threading.Thread(target=self._run_job, args=job, daemon=True).start()
+            if self._used + nbytes <= self.capacity:
+                try:
+                    off = self.backend.alloc(nbytes)
+                    self._used += nbytes
+                    return off
+                except MemoryError:
+                    pass  # fragmented: evict and retry
+            if not self._idle:
+                return None
+            _, victim = self._idle.popitem(last=False)
+            self._free_region(victim)
+            self.stats["evicted"] += 1
 
-    def _ready_reuse(self):
-        return any(not r.busy for (_, _, r, _) in self._reuse)
-
-    def _next_job(self):
-        # 1. warm-start requests on idle resident handles (all requests for
-        #    the same handle go together)
-        for (req, fut, res, b) in list(self._reuse):
-            if res.busy:
+    def _try_admit(self):
+        blocked = False
+        keep = collections.deque()
+        while self._admit:
+            req, fut = self._admit.popleft()
+            if blocked:
+                keep.append((req, fut))
                 continue
-            group = [(q, f, bb) for (q, f, r, bb) in self._reuse if r is res]
-            seen, batch = set(), []
-            for q, f, bb in group:
-                if bb in seen:
-                    continue  # a second request for the same block waits its turn
-                seen.add(bb)
-                batch.append((q, f, bb))
-            for q, f, bb in batch:
-                self._reuse.remove(next(x for x in self._reuse if x[0] is q))
-            res.busy = True
-            return (res, batch, True)
-        # 2. a new batch from the queue, admitted by footprint
-        if not self._pending:
-            return None
-        batch, specs = [], []
-        while self._pending and len(batch) < self.max_batch_blocks:
-            req, fut = self._pending[0]
-            trial = specs + [req.spec()]
-            need = footprint_bytes(trial, self.precision, self.coarse)
-            if need > self.budget:
-                if not batch:  # alone it can never fit
-                    self._pending.popleft()
-                    fut.set_exception(MemoryError(f"request needs {need} bytes, budget {self.budget}"))
+            spec = req.spec()
+            hit = self._by_key.get(req.key) if req.key is not None else None
+            if hit is not None and _same_block(hit.spec, spec):
+                if hit.busy:  # the key's previous request still holds it
+                    keep.append((req, fut))
                     continue
-                break
-            if not self._evict_for(need):
-                break
-            self._pending.popleft()
-            batch.append((req, fut))
-            specs = trial
-        if not batch:
-            return None
-        # The footprint is reserved here, under the lock; the handle itself is
-        # built by the worker (device allocations, tables and uploads take
-        # milliseconds, and building here would hold every other dispatch).
-        res = _Resident(solver=None, keys=[r.key for r, _ in batch],
-                        nbytes=footprint_bytes(specs, self.precision, self.coarse), busy=True,
-                        specs=specs)
-        self._resident.append(res)
-        for b, (r, _) in enumerate(batch):
-            if r.key is not None:
-                self._by_key[r.key] = (res, b)
-        self.stats["batches"] += 1
-        self.stats["batched_blocks"] += len(batch)
-        return (res, [(r, f, b) for b, (r, f) in enumerate(batch)], False,
-                min(r.tol for r, _ in batch))
-
-    # -- worker
-    def _drop(self, res):
-        """Forget resident `res` (lock held)."""
-        if res in self._resident:
-            self._resident.remove(res)
-        for k in res.keys:
-            if k is not None and self._by_key.get(k, (None,))[0] is res:
-                del self._by_key[k]
-
-    def _run_job(self, res, batch, warm, tol=None):
-        if res.solver is None:
+                hit.busy = True
+                self._idle.pop(id(hit), None)
+                self._queues[self._lane_of(spec)].append(_Job(req, fut, hit, warm=True))
+                continue
             try:
-                res.solver = self._factory(res.specs, tol=tol, **self._kw)
-            except Exception as e:  # noqa: BLE001 - device refused: fail this batch
+                nbytes = self.backend.block_bytes(spec)
+            except Exception as e:  # noqa: BLE001 - invalid block: fail the request
+                fut.set_exception(e)
+                continue
+            if nbytes > self.capacity:
+                fut.set_exception(MemoryError(f"request needs {nbytes} bytes, "
+                                              f"the pool holds {self.capacity}"))
+                continue
+            off = self._take_memory(nbytes)
+            if off is None:
+                blocked = True  # first come, first served: later requests wait too
+                keep.append((req, fut))
+                continue
+            region = _Region(off, nbytes, spec, req.key)
+            if req.key is not None:
+                old = self._by_key.get(req.key)
+                if old is not None and not old.busy:
+                    self._free_region(old)
+                self._by_key[req.key] = region
+            self._queues[self._lane_of(spec)].append(_Job(req, fut, region, warm=False))
+        self._admit = keep
+
+    def _release(self, job):
+        """A finished job's region leaves its handle (lock held)."""
+        r = job.region
+        r.busy = False
+        r.last_used = time.perf_counter()
+        if self.keep_resident and r.key is not None and self._by_key.get(r.key) is r:
+            self._idle[id(r)] = r
+        else:
+            self._free_region(r)
+
+    # -- lanes
+    def _lane(self, q):
+        """One lane: a pooled handle stepped one V-cycle at a time; waiting
+        requests join at the cycle boundary, finished blocks leave at once
+        (their regions go back: every stepping kernel skips a block that is
+        not running, so the handle never touches a region it no longer
+        owns).  A device error fails the lane's requests in flight; the lane
+        goes on with the next ones."""
+        lane = dict(S=None, running=[], joiners=[], stream=None)
+        queue = self._queues[q]
+        while True:
+            try:
+                if not self._lane_iteration(lane, queue):
+                    break
+            except Exception as e:  # noqa: BLE001
                 with self._lock:
-                    self._drop(res)
-                    # requests queued meanwhile for this handle's keys fail too
-                    for x in [x for x in self._reuse if x[2] is res]:
-                        self._reuse.remove(x)
-                        x[1].set_exception(e)
-                    for _, fut, _ in batch:
-                        fut.set_exception(e)
-                    self._running -= 1
+                    for m in lane["running"] + lane["joiners"]:
+                        if not m.fut.done():
+                            m.fut.set_exception(e)
+                        if m.region.busy:
+                            m.region.busy = False
+                            self._free_region(m.region)  # state unknown: not resident
+                    self._try_admit()
                     self._lock.notify_all()
-                return
-        S = res.solver
+                lane["running"], lane["joiners"] = [], []
+                if lane["S"] is not None:
+                    try:
+                        lane["S"].close()
+                    except Exception:  # noqa: BLE001
+                        pass
+                    lane["S"] = None
+        if lane["S"] is not None:
+            lane["S"].close()
+
+    def _lane_iteration(self, lane, queue):
+        """One V-cycle boundary of a lane; False when the lane is done."""
+        S, running = lane["S"], lane["running"]
+        with self._lock:
+            while not self._closed and not queue and not running and S is None:
+                self._lock.wait()
+            if self._closed and not queue and not running and S is None and not self._admit:
+                return False
+            if not queue and not running and S is None:
+                # closing, but requests still wait for memory or for their
+                # key: another lane's finish admits them
+                self._lock.wait(0.01)
+                return True
+            room = self.max_batch_blocks - len(running)
+            lane["joiners"] = joiners = [queue.popleft() for _ in range(min(room, len(queue)))]
+        if not joiners and not running:
+            S.close()  # idle
+            lane["S"] = None
+            return True
+        if lane["stream"] is None:
+            lane["stream"] = self.backend.new_stream()
+        if joiners:
+            S, running = self._rebuild(lane, joiners)
+            lane["S"], lane["running"], lane["joiners"] = S, running, []
+            if not running:
+                return True
+        S.solve_step()
+        self.stats["steps"] += 1
+        self.stats["block_steps"] += len(running)
+        now, left = None, []
+        for m in running:
+            m.batch_sum += len(running)
+            m.steps += 1
+        for m in running:
+            st = S.block_state(m.b)
+            if st.running:
+                left.append(m)
+                continue
+            now = now or time.perf_counter()
+            self._finish(S, m, st, now)
+        lane["running"] = left
+        return True
+
+    def _finish(self, S, job, st, now=None):
+        p = S.result_p(job.b)
+        done = now or time.perf_counter()
+        job.running = False
+        job.fut.set_result(Result(
+            rid=job.req.rid, p=p, status=int(st.status), cycles=int(st.cycles),
+            rel_residual=float(st.rel), latency_s=done - job.req.t_submit,
+            batch_blocks=(job.batch_sum / job.steps) if job.steps else 1.0, warm=job.warm))
+        with self._lock:
+            self.stats["solves"] += 1
+            self.stats["warm"] += 1 if job.warm else 0
+            self._release(job)
+            self._try_admit()
+            self._lock.notify_all()
+
+    def _rebuild(self, lane, joiners):
+        """At the V-cycle boundary: a new handle over the running blocks
+        (state carried in their regions and counters) and the joiners."""
+        S, running = lane["S"], lane["running"]
+        states = [S.block_state(m.b) for m in running]
+        new = running + joiners
         try:
-            blocks = {b for (_, _, b) in batch}
-            for b in range(len(S.blocks)):
-                S.set_block_enabled(b, b in blocks)
-            for req, _, b in batch:
-                S.set_rhs(b, req.f)
-                S.set_block_tol(b, req.tol)
-            st, _, _ = S.solve()
-            cyc, rel = S.block_stats()
-            done = time.perf_counter()
-            for req, fut, b in batch:
-                p = S.result_p(b)
-                fut.set_result(Result(rid=req.rid, p=p, status=block_status(st, rel[b], req.tol),
-                                      cycles=cyc[b],
-                                      rel_residual=rel[b], latency_s=done - req.t_submit,
-                                      batch_blocks=len(batch), warm=warm))
-            self.stats["solves"] += len(batch)
-            self.stats["warm"] += len(batch) if warm else 0
-        except Exception as e:  # noqa: BLE001
-            for _, fut, _ in batch:
-                if not fut.done():
-                    fut.set_exception(e)
-        finally:
+            S2 = self.backend.handle([m.region.spec for m in new], [m.region.off for m in new],
+                                     [False] * len(running) + [j.region.fresh for j in joiners],
+                                     lane["stream"][1])
+        except Exception as e:  # noqa: BLE001 - no handle: the joiners fail, the batch runs on
             with self._lock:
-                res.busy = False
-                res.last_used = time.perf_counter()
-                if not self.keep_resident and not any(x[2] is res for x in self._reuse):
-                    self._drop(res)
-                    S.close()
-                self._running -= 1
+                for j in joiners:
+                    j.running = False
+                    j.fut.set_exception(e)
+                    j.region.busy = False
+                    self._free_region(j.region)  # its state is unknown: not resident
+                self._try_admit()
                 self._lock.notify_all()
-
-
-def block_status(batch_status, rel, tol):
-    """Status of one block of a multi-block solve: the handle's status
-    speaks for the whole batch (NOT_CONVERGED when any block reached maxcy),
-    so each block is judged by its own final relative residual; a diverged
-    or failed solve keeps the batch's status."""
-    if batch_status not in (0, 5):
-        return int(batch_status)
-    return 0 if math.isfinite(rel) and rel <= tol else 5
+            return S, running
+        if S is not None:
+            S.close()
+        lane["S"] = S2  # from here on, an error closes S2
+        for i, m in enumerate(new):
+            m.b = i
+        for i, m in enumerate(running):
+            S2.set_block_state(i, states[i])
+        for j in joiners:
+            S2.set_rhs(j.b, j.req.f)
+            S2.set_block_tol(j.b, j.req.tol)
+            j.region.fresh = False
+        S2.solve_begin([j.b for j in joiners])
+        with self._lock:
+            self.stats["rebuilds"] += 1
+            self.stats["joined"] += len(joiners)
+            self.stats["joined_running"] += len(joiners) if running else 0
+        left = list(running)
+        for j in joiners:
+            st = S2.block_state(j.b)
+            if st.running:
+                left.append(j)
+            else:  # finished at the start (a non-finite rhs)
+                self._finish(S2, j, st)
+        return S2, left
 
 
 # ---- the serving workload: seeded request stream --------------------------
@@ -473,8 +576,8 @@ def main(argv=None) -> int:
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--budget-gb", type=float, default=64.0)
-    ap.add_argument("--max-batches", type=int, default=4)
+    ap.add_argument("--budget-gb", type=float, default=32.0)
+    ap.add_argument("--lanes", type=int, default=3)
     ap.add_argument("--no-baseline", action="store_true")
     a = ap.parse_args(argv)
     import torch
@@ -482,13 +585,13 @@ def main(argv=None) -> int:
     reqs = request_stream(a.requests, a.rate, a.seed)
     out = {"workload": f"{a.requests} requests, Poisson arrivals {a.rate}/s, sizes 32..256 "
                        f"(x16), Neumann/periodic, tol 1e-6/1e-8, 20% warm reuse, seed {a.seed}"}
-    with PoissonServer(hbm_budget=int(a.budget_gb * 2**30), max_batches=a.max_batches) as srv:
+    with PoissonServer(hbm_budget=int(a.budget_gb * 2**30), lanes=a.lanes) as srv:
         srv.warmup()
         res, wall = run_stream(srv, reqs, progress=500)
         out["served_mixed"] = summarize(res, wall)
         out["served_mixed"]["server"] = dict(srv.stats)
     if not a.no_baseline:
-        with PoissonServer(hbm_budget=int(a.budget_gb * 2**30), max_batches=1, max_batch_blocks=1,
+        with PoissonServer(hbm_budget=int(a.budget_gb * 2**30), lanes=1, max_batch_blocks=1,
                            precision="fp64", coarse="smooth", keep_resident=False,
                            maxcy=400) as srv:
             srv.warmup()

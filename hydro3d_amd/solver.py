@@ -44,41 +44,125 @@ def _vp(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def block_array(blocks):
+    """h3dmg_block array of block dicts; returns (array, normalised dicts)."""
+    nb = len(blocks)
+    arr = (L.H3dBlock * nb)()
+    norm = []
+    for q, b in enumerate(blocks):
+        n = tuple(int(x) for x in b["n"])
+        arr[q].ni, arr[q].nj, arr[q].nk = n
+        arr[q].dx, arr[q].dy, arr[q].dz = (float(x) for x in b["h"])
+        arr[q].nlevels = int(b["nlev"])
+        arr[q].bc[:] = [int(x) for x in b["bc"]]
+        nbr = [int(x) for x in b.get("nbr", (-1,) * 6)]
+        arr[q].nbr[:] = nbr
+        norm.append(dict(n=n, h=tuple(b["h"]), bc=tuple(b["bc"]), nlev=int(b["nlev"]),
+                         nbr=tuple(nbr)))
+    return arr, norm
+
+
+def make_params(nu1=2, nu2=2, maxcy=60, tol=1e-8, criterion="rel_l2", precision="fp64",
+                coarse="smooth", min_cycles=1, use_graph=True, dt=1e-3, smoother="rbgs"):
+    lib = L.load()
+    prm = L.H3dParams()
+    lib.h3dmg_default_params(ctypes.byref(prm))
+    prm.nu1, prm.nu2, prm.maxcy = nu1, nu2, maxcy
+    prm.tol, prm.dt = tol, dt
+    prm.criterion = _CRIT[criterion]
+    prm.precision = _PREC[precision]
+    prm.coarse = _COARSE[coarse]
+    prm.smoother = _SMOOTH[smoother]
+    prm.min_cycles = min_cycles
+    prm.use_graph = 1 if use_graph else 0
+    return prm
+
+
+class DevicePool:
+    """h3dmg_pool: one device allocation, cut into the regions of blocks that
+    outlive the handles built over them (continuous batching)."""
+
+    def __init__(self, nbytes, max_blocks=4096):
+        self.lib = L.load()
+        p = ctypes.c_void_p()
+        L.check(self.lib.h3dmg_pool_create(int(nbytes), int(max_blocks), ctypes.byref(p)))
+        self.p = p
+        self.nbytes = int(nbytes)
+
+    def alloc(self, nbytes):
+        """Offset of a new piece of `nbytes`; MemoryError when none is free."""
+        off = ctypes.c_ulonglong(0)
+        st = self.lib.h3dmg_pool_alloc(self.p, int(nbytes), ctypes.byref(off))
+        if st == L.ERR_NOMEM:
+            raise MemoryError(self.lib.h3dmg_last_error().decode())
+        L.check(st)
+        return int(off.value)
+
+    def free(self, off):
+        L.check(self.lib.h3dmg_pool_free(self.p, int(off)))
+
+    @property
+    def free_bytes(self):
+        return int(self.lib.h3dmg_pool_free_bytes(self.p))
+
+    @property
+    def largest_free(self):
+        return int(self.lib.h3dmg_pool_largest_free(self.p))
+
+    def close(self):
+        if getattr(self, "p", None):
+            self.lib.h3dmg_pool_destroy(self.p)
+            self.p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def block_bytes(spec, **params):
+    """Bytes of one block's region in a pool (h3dmg_block_bytes)."""
+    lib = L.load()
+    arr, _ = block_array([spec])
+    prm = make_params(**params)
+    n = int(lib.h3dmg_block_bytes(arr, ctypes.byref(prm)))
+    if n == 0:
+        raise L.H3dError(L.ERR_ARG, lib.h3dmg_last_error().decode())
+    return n
+
+
 class MultigridSolver:
     """Blocks, each given as dict(n=(ni,nj,nk), h=(dx,dy,dz),
     bc=(w,e,s,n,b,t) with 0 wall / 1 periodic / 2 shared with block nbr[f],
-    nlev=levels, and for shared faces nbr=(six block indices))."""
+    nlev=levels, and for shared faces nbr=(six block indices)).
+
+    With `pool` (a DevicePool) the handle is built in the pool over block
+    regions at `offsets` (h3dmg_create_pooled): `fresh[b]` false keeps block
+    b's state from an earlier handle over the same region; `stream` is the
+    HIP stream pointer to launch on (None: a stream of the handle's own)."""
 
     def __init__(self, blocks, nu1=2, nu2=2, maxcy=60, tol=1e-8, criterion="rel_l2",
                  precision="fp64", coarse="smooth", min_cycles=1, use_graph=True, dt=1e-3,
-                 smoother="rbgs"):
+                 smoother="rbgs", pool=None, offsets=None, fresh=None, stream=None):
         self.lib = L.load()
         nb = len(blocks)
-        arr = (L.H3dBlock * nb)()
-        self.blocks = []
-        for q, b in enumerate(blocks):
-            n = tuple(int(x) for x in b["n"])
-            arr[q].ni, arr[q].nj, arr[q].nk = n
-            arr[q].dx, arr[q].dy, arr[q].dz = (float(x) for x in b["h"])
-            arr[q].nlevels = int(b["nlev"])
-            arr[q].bc[:] = [int(x) for x in b["bc"]]
-            nbr = [int(x) for x in b.get("nbr", (-1,) * 6)]
-            arr[q].nbr[:] = nbr
-            self.blocks.append(dict(n=n, h=tuple(b["h"]), bc=tuple(b["bc"]), nlev=int(b["nlev"]),
-                                    nbr=tuple(nbr)))
-        prm = L.H3dParams()
-        self.lib.h3dmg_default_params(ctypes.byref(prm))
-        prm.nu1, prm.nu2, prm.maxcy = nu1, nu2, maxcy
-        prm.tol, prm.dt = tol, dt
-        prm.criterion = _CRIT[criterion]
-        prm.precision = _PREC[precision]
-        prm.coarse = _COARSE[coarse]
-        prm.smoother = _SMOOTH[smoother]
-        prm.min_cycles = min_cycles
-        prm.use_graph = 1 if use_graph else 0
+        arr, self.blocks = block_array(blocks)
+        prm = make_params(nu1=nu1, nu2=nu2, maxcy=maxcy, tol=tol, criterion=criterion,
+                          precision=precision, coarse=coarse, min_cycles=min_cycles,
+                          use_graph=use_graph, dt=dt, smoother=smoother)
         self.params = prm
         h = ctypes.c_void_p()
-        L.check(self.lib.h3dmg_create(arr, nb, ctypes.byref(prm), ctypes.byref(h)))
+        if pool is None:
+            L.check(self.lib.h3dmg_create(arr, nb, ctypes.byref(prm), ctypes.byref(h)))
+        else:
+            offs = (ctypes.c_ulonglong * nb)(*[int(o) for o in offsets])
+            fr = None if fresh is None else (ctypes.c_int * nb)(*[1 if x else 0 for x in fresh])
+            st = self.lib.h3dmg_create_pooled(pool.p, arr, nb, ctypes.byref(prm), offs, fr,
+                                              ctypes.c_void_p(stream or 0), ctypes.byref(h))
+            if st == L.ERR_NOMEM:
+                raise MemoryError(self.lib.h3dmg_last_error().decode())
+            L.check(st)
         self.h = h
         self.maxcy = maxcy
 
@@ -315,6 +399,27 @@ class MultigridSolver:
         """A disabled block is skipped by solve/vcycle; its state stays."""
         L.check(self.lib.h3dmg_set_block_enabled(self.h, b, 1 if enabled else 0))
 
+    # ---- stepped solves (continuous batching)
+    def block_state(self, b):
+        st = L.H3dBlockState()
+        L.check(self.lib.h3dmg_get_block_state(self.h, b, ctypes.byref(st)))
+        return st
+
+    def set_block_state(self, b, st):
+        L.check(self.lib.h3dmg_set_block_state(self.h, b, ctypes.byref(st)))
+
+    def solve_begin(self, blocks):
+        """Start the stepped solve of the listed blocks (r = f - A p)."""
+        idx = (ctypes.c_int * len(blocks))(*[int(b) for b in blocks])
+        L.check(self.lib.h3dmg_solve_begin(self.h, idx, len(blocks)))
+
+    def solve_step(self):
+        """One outer iteration of every running block; returns how many
+        still run.  Each block's outcome is in block_state(b).status."""
+        n = ctypes.c_int(0)
+        L.check(self.lib.h3dmg_solve_step(self.h, ctypes.byref(n)))
+        return n.value
+
     def remove_mean(self):
         L.check(self.lib.h3dmg_remove_mean(self.h))
 
@@ -327,6 +432,12 @@ class MultigridSolver:
 
     def get_march(self, level):
         return int(self.lib.h3dmg_get_march(self.h, level))
+
+    def set_lanes(self, level, lanes):
+        L.check(self.lib.h3dmg_set_lanes(self.h, level, lanes))
+
+    def get_lanes(self, level):
+        return int(self.lib.h3dmg_get_lanes(self.h, level))
 
     def set_fused(self, max_cells):
         """Levels of at most max_cells cells per block: one fused launch per

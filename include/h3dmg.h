@@ -301,6 +301,73 @@ h3dmg_status h3dmg_block_stats(const h3dmg_handle *h, int *cycles, double *rel_r
 h3dmg_status h3dmg_set_block_tol(h3dmg_handle *h, int b, double tol);
 h3dmg_status h3dmg_set_block_enabled(h3dmg_handle *h, int b, int enabled);
 
+/* ---- continuous batching (no reference counterpart: the serving layer) ---
+ * Solves that join and leave a running batch between V-cycles.  The memory
+ * of such batches comes from a pool: one device allocation made up front,
+ * cut into pieces.  Each block owns a region of the pool (its levels, p, f
+ * and coarse-solve tables; h3dmg_block_bytes long, taken with
+ * h3dmg_pool_alloc) that outlives the handles built over it: when the set
+ * of blocks changes, the caller builds a new handle over the new set with
+ * h3dmg_create_pooled, naming each block's region, and destroys the old one;
+ * the blocks carried over keep their state in their regions, and their
+ * per-block counters move with h3dmg_get/set_block_state.  A pooled handle
+ * allocates no device or pinned memory: its tables take pieces of the pool
+ * and return them on h3dmg_destroy (H3DMG_ERR_NOMEM when the pool is full).
+ * A pooled handle holds independent blocks only (no H3DMG_BC_NEIGHBOUR
+ * faces), and refuses the velocity loop and the variable-density operator
+ * (H3DMG_ERR_UNSUPPORTED).  One pool serves several handles at once, on
+ * several streams and host threads. */
+typedef struct h3dmg_pool h3dmg_pool;
+/* `bytes` of device memory and `max_blocks` pinned per-block statistics
+ * entries (the blocks of all the pool's live handles together). */
+h3dmg_status h3dmg_pool_create(unsigned long long bytes, int max_blocks, h3dmg_pool **out);
+/* Waits for the device; no handle of the pool may be alive. */
+void h3dmg_pool_destroy(h3dmg_pool *pool);
+/* A piece of at least `bytes` (256-byte aligned offset from the pool's start). */
+h3dmg_status h3dmg_pool_alloc(h3dmg_pool *pool, unsigned long long bytes,
+                              unsigned long long *offset);
+h3dmg_status h3dmg_pool_free(h3dmg_pool *pool, unsigned long long offset);
+unsigned long long h3dmg_pool_free_bytes(h3dmg_pool *pool);
+unsigned long long h3dmg_pool_largest_free(h3dmg_pool *pool);
+/* Bytes of one block's region under `prm` (0 for an invalid block or
+ * parameters; h3dmg_last_error says why).  Host-only. */
+unsigned long long h3dmg_block_bytes(const h3dmg_block *blk, const h3dmg_params *prm);
+/* A handle over `blocks` whose regions start at `offsets` (pieces from
+ * h3dmg_pool_alloc of at least h3dmg_block_bytes).  fresh[b] != 0 (or fresh
+ * null): block b's region starts new (p = 0, f = 0, tables uploaded);
+ * fresh[b] == 0: the region holds block b from an earlier handle with the
+ * same block and parameters, and keeps its p, f and residual.  `stream`: the
+ * hipStream_t to launch on (null: a new stream of the handle's own). */
+h3dmg_status h3dmg_create_pooled(h3dmg_pool *pool, const h3dmg_block *blocks, int nblocks,
+                                 const h3dmg_params *prm, const unsigned long long *offsets,
+                                 const int *fresh, void *stream, h3dmg_handle **out);
+
+/* Per-block counters of the stepped solve, carried from one handle to the
+ * next: running (1 while its solve goes on), cycles, status (the outcome of
+ * its last stepped solve: H3DMG_OK, H3DMG_ERR_NOT_CONVERGED at maxcy,
+ * H3DMG_ERR_DIVERGED for a non-finite residual; -1 while running or before
+ * the first), enabled, fnorm (||f||_2 of its rhs), rel (last relative
+ * residual) and tol (0: the handle's). */
+typedef struct {
+  int running, cycles, status, enabled;
+  double fnorm, rel, tol;
+} h3dmg_block_state;
+h3dmg_status h3dmg_get_block_state(const h3dmg_handle *h, int b, h3dmg_block_state *out);
+h3dmg_status h3dmg_set_block_state(h3dmg_handle *h, int b, const h3dmg_block_state *in);
+
+/* h3dmg_solve one V-cycle at a time, each block on its own clock (REL_L2
+ * only; independent blocks).  h3dmg_solve_begin starts the listed blocks
+ * (r = f - A p with their current p, as h3dmg_solve starts) while others
+ * may be running; h3dmg_solve_step runs one outer iteration (V-cycle,
+ * p += pp, new residual) of every running block and stops each block by
+ * h3dmg_solve's rule (cycles >= min_cycles and rel <= tol: H3DMG_OK; maxcy
+ * cycles: H3DMG_ERR_NOT_CONVERGED; non-finite: H3DMG_ERR_DIVERGED, in the
+ * block's state).  *nrunning (may be null) gets the blocks still running.
+ * A block's p after its stepped solve is bitwise the one h3dmg_solve gives
+ * it alone, whichever blocks ran beside it. */
+h3dmg_status h3dmg_solve_begin(h3dmg_handle *h, const int *blocks, int n);
+h3dmg_status h3dmg_solve_step(h3dmg_handle *h, int *nrunning);
+
 /* Remove the mean of p over the interior of every block (FP64). */
 h3dmg_status h3dmg_remove_mean(h3dmg_handle *h);
 
@@ -319,6 +386,14 @@ h3dmg_status h3dmg_set_stream(h3dmg_handle *h, void *stream);
  * h3dmg_get_march returns the depth in use (or -1 for a bad level). */
 h3dmg_status h3dmg_set_march(h3dmg_handle *h, int level, int planes);
 int h3dmg_get_march(const h3dmg_handle *h, int level);
+/* Lanes of a wavefront that cover one row segment of the red-black
+ * smoother's tiles at `level` (a power of two up to 64; the tile is 2*lanes
+ * colour entries by 4*64/lanes rows): a lower count cuts rows into several
+ * tiles and stacks more rows per tile.  Capped at the count that covers a
+ * whole row; 0 restores it.  Results do not depend on it.  Drops a captured
+ * graph.  h3dmg_get_lanes returns the count in use (-1 for a bad level). */
+h3dmg_status h3dmg_set_lanes(h3dmg_handle *h, int level, int lanes);
+int h3dmg_get_lanes(const h3dmg_handle *h, int level);
 /* Levels whose blocks have at most `max_cells` cells run each red-black
  * sweep as one fused launch (both colour halves, the red halo recomputed
  * per tile) instead of two; 0 (the default) turns fusion off.  Results are

@@ -72,3 +72,36 @@ def test_autotune_keeps_results_and_caches(tmp_path):
         assert AT.apply_cached(S, path) == choice
     assert np.array_equal(p_ref, p_tuned)
     assert os.path.exists(path) and len(AT.load_cache(path)) == 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_smoother_lanes_per_row_leave_results_bitwise(prec):
+    """h3dmg_set_lanes reshapes the smoother's tiles (row segments of fewer
+    lanes, more rows per tile): every choice gives the same pressure."""
+    import torch
+    from hydro3d_amd import MultigridSolver
+    spec = dict(n=(128, 96, 64), h=(1 / 64,) * 3, bc=(1, 1, 0, 0, 0, 0), nlev=4)
+    rng = np.random.default_rng(21)
+    f = rng.standard_normal((64, 96, 128))
+    f -= f.mean()
+    f = torch.from_numpy(f).cuda()
+    kw = dict(precision=prec, coarse="exact" if prec == "mixed" else "smooth", tol=0.0, maxcy=4)
+    print(f"case: lanes per row, {prec}", flush=True)
+    with MultigridSolver([spec], **kw) as S:
+        S.set_rhs(0, f)
+        S.solve()
+        ref = S.get_p(0, out=torch.empty_like(f))
+        natural = [S.get_lanes(g) for g in range(1, 5)]
+        assert natural[0] == 32 and S.get_lanes(5) == -1
+        for g, lanes in ((1, 16), (1, 4), (2, 8), (2, 1), (3, 2)):
+            S.set_lanes(g, lanes)
+            assert S.get_lanes(g) == min(lanes, natural[g - 1])
+            for kb in (1, 2):
+                S.set_march(g, kb)
+                S.set_p(0, torch.zeros_like(f))
+                S.solve()
+                got = S.get_p(0, out=torch.empty_like(f))
+                assert torch.equal(got, ref), (g, lanes, kb)
+        S.set_lanes(1, 0)
+        assert S.get_lanes(1) == natural[0]

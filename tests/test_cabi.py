@@ -89,3 +89,38 @@ def test_create_rejects_bad_blocks_before_touching_the_device():
     st = lib.h3dmg_create(_one_block(8, 8, 8, 1), 1, ctypes.byref(p), ctypes.byref(h))
     assert st == _lib.ERR_ARG and b"mixed precision" in lib.h3dmg_last_error()
     assert not h.value
+
+
+def test_block_bytes_is_host_only_and_counts_the_region():
+    """h3dmg_block_bytes (a pooled block's region) needs no device: per
+    level six colour-split arrays (phi twice, rhs; red and black halves of
+    H = (ni+1)//2 + 1 rounded up to even entries per row), FP32 below the
+    coarsest level in mixed precision; four FP64 finest arrays (p, f); the
+    exact coarse solve's eigen tables; each array 256-byte aligned."""
+    from hydro3d_amd.solver import block_bytes
+
+    def al(x):
+        return (x + 255) // 256 * 256
+
+    def expect(n, nlev, mixed, exact):
+        tot = 0
+        for g in range(1, nlev + 1):
+            ni, nj, nk = (x >> (g - 1) for x in n)
+            half = (((ni + 1) // 2 + 1) & ~1) * nj * nk * (4 if mixed and g < nlev else 8)
+            tot += 6 * al(half)
+        ni, nj, nk = n
+        tot += 4 * al((((ni + 1) // 2 + 1) & ~1) * nj * nk * 8)
+        if exact:
+            c = [x >> (nlev - 1) for x in n]
+            tot += sum(al(x * x * 8) + al(x * 8) for x in c) + 2 * al(c[0] * c[1] * c[2] * 8)
+        return tot
+
+    for n, nlev in (((64, 48, 32), 4), ((96, 64, 64), 5), ((256, 256, 256), 6), ((48, 80, 16), 3)):
+        spec = dict(n=n, h=(0.01,) * 3, bc=(0,) * 6, nlev=nlev)
+        assert block_bytes(spec, precision="mixed", coarse="exact") == expect(n, nlev, True, True)
+        assert block_bytes(spec, precision="fp64", coarse="smooth") == expect(n, nlev, False, False)
+    lib = _lib.load()
+    p = _lib.H3dParams()
+    lib.h3dmg_default_params(ctypes.byref(p))
+    assert lib.h3dmg_block_bytes(_one_block(12, 8, 8, 4), ctypes.byref(p)) == 0
+    assert b"not divisible" in lib.h3dmg_last_error()
