@@ -195,16 +195,20 @@ void launch_gs_tiles(hipStream_t s, const Tile *tiles, int ntiles, const LevelDe
                      const int *active, int src, int zero, int tile);
 void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                       const int *active, int pass, int d, int src, int zero);
-// a whole TDMA sweep of small levels, one workgroup per block (Tile::lev),
-// every line of a diagonal in LDS: tdma_sweep_lds_bytes(level) of dynamic
-// LDS (the largest over the launch, under 64 KiB), NTHREADS >= the lines
-// of a diagonal; used where every pass has at most TDMA_SWEEP_CELLS cells
-// per diagonal (above that, one launch per diagonal over several
-// workgroups is faster: measured at 64^3)
-constexpr int TDMA_SWEEP_CELLS = 2048;
+// a whole TDMA sweep of a level, one workgroup of TDMA_SWEEP_THREADS per
+// block (Tile::lev), every line of a diagonal and the previous diagonal's
+// solution in LDS: tdma_sweep_lds_bytes(level) of dynamic LDS (the largest
+// over the launch), TDMA_SWEEP_THREADS >= the lines of a diagonal; used
+// where every pass has at most TDMA_SWEEP_CELLS cells per diagonal and the
+// LDS fits the device (64^3 levels: 130 KB)
+constexpr int TDMA_SWEEP_THREADS = 1024;
+constexpr int TDMA_SWEEP_CPT = 2;  // cells per thread in each load batch
+constexpr int TDMA_SWEEP_CELLS = 4096;
 void launch_tdma_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                        const int *active, int src, int zero, size_t lds_bytes);
 size_t tdma_sweep_lds_bytes(const LevelDev &L);
+// permits `lds_bytes` of dynamic LDS for the whole-sweep TDMA kernel
+hipError_t tdma_sweep_allow_lds(size_t lds_bytes);
 // fill LevelDev::tdma of every descriptor that has one (32*(ni+nj+nk) doubles)
 void launch_tdma_tables(hipStream_t s, const LevelDev *levs, int ndesc);
 

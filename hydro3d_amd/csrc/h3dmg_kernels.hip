@@ -2744,10 +2744,10 @@ __global__ void __launch_bounds__(256)
 // left-to-right sum, so splitting it off changes no rounding.  Lines of one
 // diagonal never neighbour each other: phase A reads only lines of the
 // diagonals before and after, which nothing writes meanwhile.
-template <int PASS, int CPT>
+template <int PASS, int CPT, int NT = NTHREADS, bool PREV = false>
 __device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw, int lwp, int d,
                                               int src, int zero, double *xs, double *av,
-                                              double *rv, int tid) {
+                                              double *rv, int tid, double *prev = nullptr) {
   const TdmaGeom<PASS> G(L);
   const int nu = G.nu, nv = G.nv, len = G.len;
   const int ulo = max(u0, d - nv + 1), uhi = min(min(u0 + lw, nu), d + 1);
@@ -2776,18 +2776,29 @@ __device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw,
     const double val = *(const gmem<double> *)(s0 + o + 8 * (((long long)z * nj + y) * H + (x >> 1)));
     return (zero && !from_dst) ? 0.0 : val;
   };
+  // the updated neighbours of line (u, v) lie on lines (u-1, v) and (u, v-1)
+  // of diagonal d-1, at the same position: with `prev` (the previous
+  // diagonal's solution in LDS, line u at prev[u*len + p]) they are read
+  // from there -- the values the previous diagonal stored in dst -- instead
+  // of from global memory
+  auto upd = [=](int u, int v, int p, bool along_u, int x, int y, int z) -> double {
+    if constexpr (PREV) {
+      if (along_u ? u > 0 : v > 0) return prev[(along_u ? u - 1 : u) * len + p];
+    }
+    return cross(x, y, z, true);
+  };
   // phase A: batches of CPT cells per thread, every load of a batch issued
   // before its first LDS store (positions past the end repeat the last cell
   // and are not stored)
   const int ncell = (uhi - ulo) * len;
   // q / len through a float reciprocal: exact while q < 2^12 * len
   const float inv_len = 1.0f / (float)len;
-  for (int q0 = 0; q0 < ncell; q0 += CPT * NTHREADS) {
+  for (int q0 = 0; q0 < ncell; q0 += CPT * NT) {
     double rsv[CPT], atv[CPT], rtv[CPT];
     int slot[CPT];
 #pragma unroll
     for (int it = 0; it < CPT; ++it) {
-      const int q = min(q0 + tid + it * NTHREADS, ncell - 1);
+      const int q = min(q0 + tid + it * NT, ncell - 1);
       const int lq = (int)(((float)q + 0.5f) * inv_len);
       const int l = ulo - u0 + lq, p = q - lq * len;
       const int u = u0 + l, v = d - u;
@@ -2796,15 +2807,15 @@ __device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw,
       const Coef<double> c = smoother_coef(L, i, j, k);
       const double r = gload(L.rhs[(i + j + k) & 1], cidx(H, nj, i, j, k));
       double rs;
-      if (PASS == 0)
-        rs = r - c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true) -
-             c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
-      else if (PASS == 1)
-        rs = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
-             c.at * cross(i, j, k + 1, false) - c.ab * cross(i, j, k - 1, true);
-      else
-        rs = r - c.an * cross(i, j + 1, k, false) - c.as * cross(i, j - 1, k, true) -
-             c.ae * cross(i + 1, j, k, false) - c.aw * cross(i - 1, j, k, true);
+      if (PASS == 0)  // W on line (u-1, v), B on (u, v-1)
+        rs = r - c.ae * cross(i + 1, j, k, false) - c.aw * upd(u, v, p, true, i - 1, j, k) -
+             c.at * cross(i, j, k + 1, false) - c.ab * upd(u, v, p, false, i, j, k - 1);
+      else if (PASS == 1)  // S on (u-1, v), B on (u, v-1)
+        rs = r - c.an * cross(i, j + 1, k, false) - c.as * upd(u, v, p, true, i, j - 1, k) -
+             c.at * cross(i, j, k + 1, false) - c.ab * upd(u, v, p, false, i, j, k - 1);
+      else  // S on (u, v-1), W on (u-1, v)
+        rs = r - c.an * cross(i, j + 1, k, false) - c.as * upd(u, v, p, false, i, j - 1, k) -
+             c.ae * cross(i + 1, j, k, false) - c.aw * upd(u, v, p, true, i - 1, j, k);
       rsv[it] = rs;
       slot[it] = p * lwp + l;
       atv[it] = rtv[it] = 0.0;
@@ -2817,7 +2828,7 @@ __device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw,
     }
 #pragma unroll
     for (int it = 0; it < CPT; ++it)
-      if (q0 + tid + it * NTHREADS < ncell) {
+      if (q0 + tid + it * NT < ncell) {
         xs[slot[it]] = rsv[it];
         if (tab) {
           av[slot[it]] = atv[it];
@@ -2905,6 +2916,7 @@ __device__ __forceinline__ void tdma_diagonal(const LevelDev &L, int u0, int lw,
       next = cv[q] - aq[q] * next;
       const int e = PASS == 1 ? base + (p >> 1) : base + p * step;
       (((u + v + p) & 1) ? out1 : out0)[e] = next;
+      if (PREV) prev[u * len + p] = next;
     }
   }
 }
@@ -2938,12 +2950,13 @@ __device__ __forceinline__ void tdma_pass_in_group(const LevelDev &L, int src, i
   const int lwp = G.nu + 1;
   const size_t n = (size_t)lwp * G.len;
   for (int d = 0; d < G.nu + G.nv - 1; ++d) {
-    tdma_diagonal<PASS, 2>(L, 0, G.nu, lwp, d, src, zero, buf, buf + n, buf + 2 * n, tid);
+    tdma_diagonal<PASS, TDMA_SWEEP_CPT, TDMA_SWEEP_THREADS, true>(
+        L, 0, G.nu, lwp, d, src, zero, buf, buf + n, buf + 2 * n, tid, buf + 3 * n);
     __syncthreads();  // the diagonal's phi before the next one reads it
   }
 }
 
-__global__ void __launch_bounds__(NTHREADS)
+__global__ void __launch_bounds__(TDMA_SWEEP_THREADS)
     k_tdma_sweep(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
                  const int *__restrict__ active, int src, int zero) {
   extern __shared__ double line_lds[];
@@ -3015,8 +3028,13 @@ void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
 void launch_tdma_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                        const int *active, int src, int zero, size_t lds_bytes) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_tdma_sweep, dim3(ntiles), tile_block(), lds_bytes, s, tiles, levs, active,
-                     src, zero);
+  hipLaunchKernelGGL(k_tdma_sweep, dim3(ntiles), dim3(TILE_X, TDMA_SWEEP_THREADS / TILE_X),
+                     lds_bytes, s, tiles, levs, active, src, zero);
+}
+
+hipError_t tdma_sweep_allow_lds(size_t lds_bytes) {
+  return hipFuncSetAttribute((const void *)k_tdma_sweep,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
 void launch_tdma_tables(hipStream_t s, const LevelDev *levs, int ndesc) {
@@ -3026,9 +3044,12 @@ void launch_tdma_tables(hipStream_t s, const LevelDev *levs, int ndesc) {
 }
 
 size_t tdma_sweep_lds_bytes(const LevelDev &L) {
+  // per pass (nu lines of len cells): rs/c, a and 1/(ap - alo*a) for
+  // (nu+1)*len positions, and the previous diagonal's solution, nu*len
   const size_t p0 = (size_t)(L.ni + 1) * L.nj, p1 = (size_t)(L.nj + 1) * L.ni,
                p2 = (size_t)(L.ni + 1) * L.nk;
-  return 3 * std::max(p0, std::max(p1, p2)) * sizeof(double);
+  const size_t s0 = (size_t)L.ni * L.nj, s2 = (size_t)L.ni * L.nk;
+  return (3 * std::max(p0, std::max(p1, p2)) + std::max(s0, s2)) * sizeof(double);
 }
 
 void launch_lsm_residual(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,

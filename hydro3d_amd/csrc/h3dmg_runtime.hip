@@ -1277,6 +1277,15 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
   return H3DMG_OK;
 }
 
+// whether the whole-sweep TDMA kernel may use `budget` bytes of dynamic
+// LDS (process-wide attribute: set once, to the device's budget)
+bool tdma_sweep_ok(size_t budget) {
+  static std::once_flag once;
+  static bool ok = false;
+  std::call_once(once, [budget]() { ok = tdma_sweep_allow_lds(budget) == hipSuccess; });
+  return ok;
+}
+
 // whether the 16^3 GS kernel may use its dynamic LDS (the attribute is
 // process-wide: set once, to what the kernel needs)
 bool gs_big_ok() {
@@ -1365,10 +1374,11 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
           if (member(B)) {
             const LevelDev &L = h->levs[B.desc[g]];
             // cells per diagonal: ni*nj (j-lines), nj*ni (i-lines), ni*nk (k-lines)
-            small = small && std::max(L.ni, L.nj) <= NTHREADS &&
+            small = small && std::max(L.ni, L.nj) <= TDMA_SWEEP_THREADS &&
                     (long)L.ni * L.nj <= TDMA_SWEEP_CELLS && (long)L.ni * L.nk <= TDMA_SWEEP_CELLS;
             lds = std::max(lds, tdma_sweep_lds_bytes(L));
           }
+        small = small && lds <= h->lds_budget && tdma_sweep_ok(h->lds_budget);
         ts.tdma_lds = small ? lds : 0;
         if (small)
           for (const BlockInfo &B : h->blk)

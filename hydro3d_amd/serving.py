@@ -426,6 +426,7 @@ class PoissonServer:
         """At the V-cycle boundary: a new handle over the running blocks
         (state carried in their regions and counters) and the joiners."""
         S, running = lane["S"], lane["running"]
+        t0 = time.perf_counter()
         states = [S.block_state(m.b) for m in running]
         new = running + joiners
         try:
@@ -455,6 +456,8 @@ class PoissonServer:
             j.region.fresh = False
         S2.solve_begin([j.b for j in joiners])
         with self._lock:
+            # host and device time of the rebuild and the newcomers' start
+            self.stats["rebuild_us"] += int((time.perf_counter() - t0) * 1e6)
             self.stats["rebuilds"] += 1
             self.stats["joined"] += len(joiners)
             self.stats["joined_running"] += len(joiners) if running else 0

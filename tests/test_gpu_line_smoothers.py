@@ -81,20 +81,24 @@ def test_gpu_line_smoothers_variable_density_bitwise(gpu_lib, path, smoother, sc
 
 @pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
 @pytest.mark.parametrize("smoother,scheme", SMOOTHERS, ids=[s for s, _ in SMOOTHERS])
-@pytest.mark.parametrize("group", ["whole_sweeps", "diagonals"])
+@pytest.mark.parametrize("group", ["whole_sweeps", "wide_sweeps", "diagonals"])
 def test_gpu_line_smoothers_solve_bitwise_vs_oracle(gpu_lib, smoother, scheme, graph, group):
     """Unequal blocks in one handle (their hyperplanes and line diagonals
     share launches): six cycles of each are bitwise the oracle's solve of
-    that block alone.  TDMA levels of up to 2048 cells per diagonal run as
-    whole sweeps in one workgroup per block ("whole_sweeps"); a 128 x 40
-    finest level makes its group run diagonal by diagonal over workgroups of
-    lines ("diagonals")."""
+    that block alone.  TDMA levels of up to 4096 cells per diagonal run as
+    whole sweeps in one workgroup per block, the previous diagonal's solution
+    kept in LDS ("whole_sweeps"; "wide_sweeps": 64 x 64 and 64 x 48 x 64
+    diagonals, 130 KB of LDS); a 128 x 40 finest level makes its group run
+    diagonal by diagonal over workgroups of lines ("diagonals")."""
     from oracle import oracle as O
     rng = np.random.default_rng(17)
     if group == "whole_sweeps":
         specs = [dict(n=(32, 32, 32), h=(1 / 32,) * 3, bc=(1, 1, 1, 1, 0, 0), nlev=3),
                  dict(n=(24, 16, 40), h=(1 / 24, 1 / 16, 1 / 40), bc=(0, 0, 1, 1, 0, 0), nlev=3),
                  dict(n=(48, 32, 40), h=(1 / 48, 1 / 32, 1 / 40), bc=(1, 1, 0, 0, 1, 1), nlev=3)]
+    elif group == "wide_sweeps":
+        specs = [dict(n=(64, 64, 32), h=(1 / 64, 1 / 64, 1 / 32), bc=(1, 1, 0, 0, 1, 1), nlev=3),
+                 dict(n=(64, 48, 64), h=(1 / 64, 1 / 48, 1 / 64), bc=(0, 0, 1, 1, 0, 0), nlev=3)]
     else:
         specs = [dict(n=(128, 40, 16), h=(1 / 128, 1 / 40, 1 / 16), bc=(1, 1, 0, 0, 1, 1),
                       nlev=3),
