@@ -2956,6 +2956,237 @@ __device__ __forceinline__ void tdma_pass_in_group(const LevelDev &L, int src, i
   }
 }
 
+// One TDMA pass of a constant-coefficient level in one workgroup, pipelined
+// over the diagonals.  Diagonal d's right-hand sides need the solution of
+// diagonal d-1 only through its two updated cross terms, so phase A splits:
+//   A0 (everything that does not depend on diagonal d-1: the rhs, the two
+//      not-updated cross terms, ghost images, the lines' end ghosts) runs
+//      for diagonal d+1 on the waves that phase B does not use, while
+//   B (the Thomas recurrence of diagonal d, one thread per line) runs on
+//      the first waves;
+//   A1 (the updated cross terms from the previous diagonal's solution in
+//      LDS, and the last subtractions) runs for every cell between them.
+// The reference's left-to-right sum r - x1*X1 - x2*X2 - x3*X3 - x4*X4 is
+// kept: A0 forms t1 = r - x1*X1 and the product x3*X3, A1 finishes
+// ((t1 - x2*X2) - x3*X3) - x4*X4.  X2 and X4 are the updated neighbours
+// (pass 0: W on line (u-1, v) and B on (u, v-1); pass 1: S on (u-1, v), B
+// on (u, v-1); pass 2: S on (u, v-1), W on (u-1, v)); a neighbour that is a
+// ghost (its line coordinate 0) is an image before the sweep, taken by A0.
+// The Thomas tables of the pass (a(p), 1/(ap - alo*a(p-1)) per wall state)
+// sit in LDS for the whole pass.  LDS, doubles (tdma_sweep_lds_bytes):
+// t1, x3*X3, rs/c: 3*(nu+1)*len; tables 32*len; previous diagonal nu*len;
+// ghosts of the first line 2*len; line-end ghosts 2 x 2*nu.
+template <int PASS>
+__device__ __forceinline__ void tdma_pass_pipelined(const LevelDev &L, int src, int zero,
+                                                    double *buf, int tid) {
+  const TdmaGeom<PASS> G(L);
+  const int nu = G.nu, nv = G.nv, len = G.len, lwp = nu + 1;
+  const size_t n = (size_t)lwp * len;
+  double *const t1 = buf, *const p3 = buf + n, *const xs = buf + 2 * n;
+  double *const T = buf + 3 * n;
+  double *const prev = T + (size_t)32 * len;
+  double *const gA = prev + (size_t)nu * len;  // term 2's ghost values (line coordinate 0)
+  double *const gB = gA + len;                 // term 4's ghost values
+  double *const cg = gB + len;                 // [2][2*nu]: c(0) and phi(len) ghosts per line
+  constexpr bool U2 = PASS != 2;               // term 2 lies along u (else along v)
+  const int nb = 64 * ((nu + 63) / 64);        // phase-B threads (whole waves)
+  const int na = TDMA_SWEEP_THREADS - nb;      // A0 threads
+  const int dst = 1 - src;
+  const int ni = L.ni, nj = L.nj, nk = L.nk, H = L.H;
+  const bool px = L.per & 1, py = L.per & 2, pz = L.per & 4;
+  const gmem<char> *const s0 = (const gmem<char> *)L.phi[src][0];
+  const long long o_s1 = (const char *)L.phi[src][1] - (const char *)L.phi[src][0];
+  const long long o_d0 = (const char *)L.phi[dst][0] - (const char *)L.phi[src][0];
+  const long long o_d1 = (const char *)L.phi[dst][1] - (const char *)L.phi[src][0];
+  // as tdma_diagonal's: ghosts are images in src; interior neighbours
+  // updated ones and (after the first pass) not-yet-updated ones in dst
+  auto cross = [=](int x, int y, int z, bool updated) -> double {
+    const bool ghost = (unsigned)x >= (unsigned)ni || (unsigned)y >= (unsigned)nj ||
+                       (unsigned)z >= (unsigned)nk;
+    x = image(x, ni, px);
+    y = image(y, nj, py);
+    z = image(z, nk, pz);
+    const bool from_dst = !ghost && (updated || PASS > 0);
+    const bool odd = (x + y + z) & 1;
+    const long long o = from_dst ? (odd ? o_d1 : o_d0) : (odd ? o_s1 : 0);
+    const double val = *(const gmem<double> *)(s0 + o + 8 * (((long long)z * nj + y) * H + (x >> 1)));
+    return (zero && !from_dst) ? 0.0 : val;
+  };
+  const float inv_len = 1.0f / (float)len;  // q / len, exact while q < 2^12 * len
+  const int nd = nu + nv - 1;
+
+  // A0 of diagonal d on threads [t0, t0 + cnt): CPT cells a batch, every
+  // load of a batch issued before its first LDS store; then the "extra"
+  // items: the ghost terms of the lines whose term-2 or term-4 coordinate is
+  // 0 (one line each: items [0, 2*len)) and the lines' end ghosts c(0),
+  // phi(len) (items [2*len, 2*len + 2*lines)).
+  auto a0 = [=, &L](int d, int t0, int cnt) {
+    constexpr int CPT = 5;
+    const int ulo = max(0, d - nv + 1), uhi = min(nu, d + 1);
+    const int ncell = (uhi - ulo) * len;
+    // term 2's ghost line: u == 0 (passes 0, 1) or v == 0 (pass 2); term
+    // 4's: v == 0 (passes 0, 1) or u == 0 (pass 2)
+    const int ua = U2 ? 0 : d, ub = U2 ? d : 0;  // the line of each, if on d
+    const bool has_a = ua >= ulo && ua < uhi, has_b = ub >= ulo && ub < uhi;
+    const int nextra = 2 * len + 2 * (uhi - ulo);
+    double *const cgd = cg + (d & 1) * 2 * nu;
+    // item e: its value and its LDS index from buf (-1: nothing to store)
+    struct Item {
+      double v;
+      int at;
+    };
+    auto extra = [=, &L](int e) -> Item {
+      int i, j, k;
+      if (e < 2 * len) {
+        const bool second = e >= len;
+        const int p = second ? e - len : e;
+        if (second ? !has_b : !has_a) return Item{0.0, -1};
+        const int u = second ? ub : ua, v = d - u;
+        tdma_cell<PASS>(u, v, p, i, j, k);
+        const int at = (int)(((second ? gB : gA) + p) - buf);
+        if (PASS == 0)
+          return Item{second ? cross(i, j, k - 1, true) : cross(i - 1, j, k, true), at};
+        if (PASS == 1)
+          return Item{second ? cross(i, j, k - 1, true) : cross(i, j - 1, k, true), at};
+        return Item{second ? cross(i - 1, j, k, true) : cross(i, j - 1, k, true), at};
+      }
+      const int q = e - 2 * len, u = ulo + (q >> 1), v = d - u;
+      tdma_cell<PASS>(u, v, (q & 1) ? len : -1, i, j, k);
+      return Item{before_sweep(L, src, zero, i, j, k), (int)((cgd + 2 * u + (q & 1)) - buf)};
+    };
+    for (int q0 = 0; q0 < ncell; q0 += CPT * cnt) {
+      double tv[CPT], pv[CPT];
+      int slot[CPT];
+#pragma unroll
+      for (int it = 0; it < CPT; ++it) {
+        const int q = min(q0 + t0 + it * cnt, ncell - 1);
+        const int lq = (int)(((float)q + 0.5f) * inv_len);
+        const int p = q - lq * len, u = ulo + lq, v = d - u;
+        int i, j, k;
+        tdma_cell<PASS>(u, v, p, i, j, k);
+        const Coef<double> c = cell_coef<double>(L, i, j, k);
+        const double r = gload(L.rhs[(i + j + k) & 1], cidx(H, nj, i, j, k));
+        if (PASS == 0) {
+          tv[it] = r - c.ae * cross(i + 1, j, k, false);
+          pv[it] = c.at * cross(i, j, k + 1, false);
+        } else if (PASS == 1) {
+          tv[it] = r - c.an * cross(i, j + 1, k, false);
+          pv[it] = c.at * cross(i, j, k + 1, false);
+        } else {
+          tv[it] = r - c.an * cross(i, j + 1, k, false);
+          pv[it] = c.ae * cross(i + 1, j, k, false);
+        }
+        slot[it] = p * lwp + u;
+      }
+#pragma unroll
+      for (int it = 0; it < CPT; ++it)
+        if (q0 + t0 + it * cnt < ncell) {
+          t1[slot[it]] = tv[it];
+          p3[slot[it]] = pv[it];
+        }
+    }
+    for (int e = t0; e < nextra; e += cnt) {
+      const Item x = extra(e);
+      if (x.at >= 0) buf[x.at] = x.v;
+    }
+  };
+
+  // the pass's Thomas tables into LDS, and A0 of diagonal 0
+  for (int q = tid; q < 16 * len; q += TDMA_SWEEP_THREADS) {
+    const size_t t = tdma_table_offset<PASS>(L) + (size_t)2 * q;
+    T[2 * q] = gload(L.tdma, t);
+    T[2 * q + 1] = gload(L.tdma, t + 1);
+  }
+  a0(0, tid, TDMA_SWEEP_THREADS);
+  __syncthreads();
+  const double cax = PASS == 0 ? L.cy : PASS == 1 ? L.cx : L.cz;
+  const bool pax = PASS == 0 ? (L.per & 2) : PASS == 1 ? (L.per & 1) : (L.per & 4);
+  gmem<double> *const out0 = (gmem<double> *)L.phi[dst][0];
+  gmem<double> *const out1 = (gmem<double> *)L.phi[dst][1];
+  for (int d = 0; d < nd; ++d) {
+    const int ulo = max(0, d - nv + 1), uhi = min(nu, d + 1);
+    // A1: the updated cross terms and the last subtractions, every cell,
+    // four a batch (every LDS read of a batch before its first store)
+    const int ncell = (uhi - ulo) * len;
+    for (int q0 = 0; q0 < ncell; q0 += 4 * TDMA_SWEEP_THREADS) {
+      double rs[4];
+      int sl[4];
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int q = min(q0 + tid + it * TDMA_SWEEP_THREADS, ncell - 1);
+        const int lq = (int)(((float)q + 0.5f) * inv_len);
+        const int p = q - lq * len, u = ulo + lq, v = d - u;
+        int i, j, k;
+        tdma_cell<PASS>(u, v, p, i, j, k);
+        const Coef<double> c = cell_coef<double>(L, i, j, k);
+        const double x2 = PASS == 0 ? c.aw : c.as;
+        const double x4 = PASS == 2 ? c.aw : c.ab;
+        const bool in2 = U2 ? u > 0 : v > 0, in4 = U2 ? v > 0 : u > 0;
+        const double X2 = in2 ? prev[(U2 ? u - 1 : u) * len + p] : gA[p];
+        const double X4 = in4 ? prev[(U2 ? u : u - 1) * len + p] : gB[p];
+        sl[it] = p * lwp + u;
+        rs[it] = ((t1[sl[it]] - x2 * X2) - p3[sl[it]]) - x4 * X4;
+      }
+#pragma unroll
+      for (int it = 0; it < 4; ++it)
+        if (q0 + tid + it * TDMA_SWEEP_THREADS < ncell) xs[sl[it]] = rs[it];
+    }
+    __syncthreads();
+    if (tid < nb) {
+      // B: the reference's recurrence, one thread per line
+      const int u = tid, v = d - u;
+      if (u >= ulo && u < uhi) {
+        const double *const cgd = cg + (d & 1) * 2 * nu;
+        const int var = wall_state(u, nu, G.pu) | (wall_state(v, nv, G.pv) << 2);
+        const double *const Tv = T + (size_t)2 * var * len;
+        double cprev = cgd[2 * u];
+        for (int p0 = 0; p0 < len; p0 += TDMA_CHUNK) {
+          double rs[TDMA_CHUNK], apr[TDMA_CHUNK];
+#pragma unroll
+          for (int q = 0; q < TDMA_CHUNK; ++q) {
+            const int p = min(p0 + q, len - 1);
+            rs[q] = xs[p * lwp + u];
+            apr[q] = Tv[2 * p + 1];
+          }
+#pragma unroll
+          for (int q = 0; q < TDMA_CHUNK; ++q) {
+            const int p = p0 + q;
+            if (p >= len) break;
+            const double alo = (p == 0 && !pax) ? 0.0 : cax;
+            cprev = (rs[q] - alo * cprev) * apr[q];
+            xs[p * lwp + u] = cprev;
+          }
+        }
+        double next = cgd[2 * u + 1];
+        const int base =
+            PASS == 0 ? v * nj * H + (u >> 1) : PASS == 1 ? (v * nj + u) * H : v * H + (u >> 1);
+        const int step = PASS == 0 ? H : PASS == 1 ? 0 : nj * H;
+        for (int p1 = len - 1; p1 >= 0; p1 -= TDMA_CHUNK) {
+          double cv[TDMA_CHUNK], aq[TDMA_CHUNK];
+#pragma unroll
+          for (int q = 0; q < TDMA_CHUNK; ++q) {
+            const int p = max(p1 - q, 0);
+            cv[q] = xs[p * lwp + u];
+            aq[q] = Tv[2 * p];
+          }
+#pragma unroll
+          for (int q = 0; q < TDMA_CHUNK; ++q) {
+            const int p = p1 - q;
+            if (p < 0) break;
+            next = cv[q] - aq[q] * next;
+            const int e = PASS == 1 ? base + (p >> 1) : base + p * step;
+            (((u + v + p) & 1) ? out1 : out0)[e] = next;
+            prev[u * len + p] = next;
+          }
+        }
+      }
+    } else if (d + 1 < nd) {
+      a0(d + 1, tid - nb, na);
+    }
+    __syncthreads();  // prev and the next diagonal's A0 before A1 reads them
+  }
+}
+
 __global__ void __launch_bounds__(TDMA_SWEEP_THREADS)
     k_tdma_sweep(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
                  const int *__restrict__ active, int src, int zero) {
@@ -2967,6 +3198,21 @@ __global__ void __launch_bounds__(TDMA_SWEEP_THREADS)
   tdma_pass_in_group<0>(L, src, zero, line_lds, tid);
   tdma_pass_in_group<1>(L, src, zero, line_lds, tid);
   tdma_pass_in_group<2>(L, src, zero, line_lds, tid);
+}
+
+// the same sweep on a constant-coefficient level, passes pipelined over the
+// diagonals (a kernel of its own: its registers do not cost the other one)
+__global__ void __launch_bounds__(TDMA_SWEEP_THREADS)
+    k_tdma_sweep_pipe(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
+                      const int *__restrict__ active, int src, int zero) {
+  extern __shared__ double line_lds[];
+  const Tile t = tiles[blockIdx.x];
+  const LevelDev &L = levs[t.lev];
+  if (!active[L.blk] || L.lsm != nullptr || L.tdma == nullptr) return;
+  const int tid = (int)threadIdx.x + TILE_X * (int)threadIdx.y;
+  tdma_pass_pipelined<0>(L, src, zero, line_lds, tid);
+  tdma_pass_pipelined<1>(L, src, zero, line_lds, tid);
+  tdma_pass_pipelined<2>(L, src, zero, line_lds, tid);
 }
 
 }  // namespace
@@ -3026,14 +3272,18 @@ void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
 }
 
 void launch_tdma_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                       const int *active, int src, int zero, size_t lds_bytes) {
+                       const int *active, int src, int zero, size_t lds_bytes, bool pipelined) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_tdma_sweep, dim3(ntiles), dim3(TILE_X, TDMA_SWEEP_THREADS / TILE_X),
-                     lds_bytes, s, tiles, levs, active, src, zero);
+  hipLaunchKernelGGL(pipelined ? k_tdma_sweep_pipe : k_tdma_sweep, dim3(ntiles),
+                     dim3(TILE_X, TDMA_SWEEP_THREADS / TILE_X), lds_bytes, s, tiles, levs, active,
+                     src, zero);
 }
 
 hipError_t tdma_sweep_allow_lds(size_t lds_bytes) {
-  return hipFuncSetAttribute((const void *)k_tdma_sweep,
+  hipError_t e = hipFuncSetAttribute((const void *)k_tdma_sweep,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *)k_tdma_sweep_pipe,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
@@ -3044,12 +3294,19 @@ void launch_tdma_tables(hipStream_t s, const LevelDev *levs, int ndesc) {
 }
 
 size_t tdma_sweep_lds_bytes(const LevelDev &L) {
-  // per pass (nu lines of len cells): rs/c, a and 1/(ap - alo*a) for
-  // (nu+1)*len positions, and the previous diagonal's solution, nu*len
-  const size_t p0 = (size_t)(L.ni + 1) * L.nj, p1 = (size_t)(L.nj + 1) * L.ni,
-               p2 = (size_t)(L.ni + 1) * L.nk;
-  const size_t s0 = (size_t)L.ni * L.nj, s2 = (size_t)L.ni * L.nk;
-  return (3 * std::max(p0, std::max(p1, p2)) + std::max(s0, s2)) * sizeof(double);
+  // per pass (nu lines of len cells): rs/c, a and 1/(ap - alo*a) (or, for
+  // the pipelined passes, t1, x3*X3 and rs/c) for (nu+1)*len positions, the
+  // previous diagonal's solution, nu*len; pipelined passes also the tables
+  // (32*len), the first lines' ghosts (2*len) and the line-end ghosts (4*nu)
+  size_t most = 0;
+  const int geo[3][2] = {{L.ni, L.nj}, {L.nj, L.ni}, {L.ni, L.nk}};  // (nu, len) per pass
+  for (const auto &g : geo) {
+    const size_t nu = (size_t)g[0], len = (size_t)g[1];
+    size_t need = 3 * (nu + 1) * len + nu * len;
+    if (tdma_sweep_pipelined(L)) need += 32 * len + 2 * len + 4 * nu;
+    most = std::max(most, need);
+  }
+  return most * sizeof(double);
 }
 
 void launch_lsm_residual(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,

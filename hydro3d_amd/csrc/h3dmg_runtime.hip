@@ -101,6 +101,7 @@ struct TileSet {
   std::vector<int> diag_begin;       // GS sets: tiles of tile-diagonal D at [begin[D], begin[D+1])
   int gs_tile = GS_TILE;             // GS sets: tile edge (GS_TILE or GS_BIG_TILE)
   size_t tdma_lds = 0;  // TDMA sets: > 0 = whole sweeps in one workgroup per block (LDS bytes)
+  bool tdma_pipe = false;  // TDMA whole sweeps: the pipelined kernel (constant coefficients)
   int n() const { return (int)h.size(); }
 };
 
@@ -427,7 +428,7 @@ void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool 
                         h->d_levs, h->d_active, buf, zero ? 1 : 0, ts.gs_tile);
     else if (h->prm.smoother == H3DMG_SMOOTH_TDMA && ts.tdma_lds)  // small levels
       launch_tdma_sweep(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, buf, zero ? 1 : 0,
-                        ts.tdma_lds);
+                        ts.tdma_lds, ts.tdma_pipe);
     else if (h->prm.smoother == H3DMG_SMOOTH_TDMA)  // j-, i-, k-line diagonals
       for (int pass = 0; pass < 3; ++pass)
         for (int d = 0; d < ts.ndiag[pass]; ++d)
@@ -1374,12 +1375,15 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
           if (member(B)) {
             const LevelDev &L = h->levs[B.desc[g]];
             // cells per diagonal: ni*nj (j-lines), nj*ni (i-lines), ni*nk (k-lines)
-            small = small && std::max(L.ni, L.nj) <= TDMA_SWEEP_THREADS &&
+            // (the pipelined passes keep whole waves free for the next
+            // diagonal's loads: at most half the workgroup runs lines)
+            small = small && std::max(L.ni, L.nj) <= TDMA_SWEEP_THREADS / 2 &&
                     (long)L.ni * L.nj <= TDMA_SWEEP_CELLS && (long)L.ni * L.nk <= TDMA_SWEEP_CELLS;
             lds = std::max(lds, tdma_sweep_lds_bytes(L));
           }
         small = small && lds <= h->lds_budget && tdma_sweep_ok(h->lds_budget);
         ts.tdma_lds = small ? lds : 0;
+        ts.tdma_pipe = !h->lsm;
         if (small)
           for (const BlockInfo &B : h->blk)
             if (member(B)) ts.h.push_back(Tile{B.desc[g], 0, 0, 0, 1, 7});
