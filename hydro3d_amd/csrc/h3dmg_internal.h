@@ -94,9 +94,16 @@ struct RedPartial {
 // tiles are all `depth` (1, 2 or 4) planes deep: kinds 0 (march tile, y/z
 // interior), 3 (march tile touching a y/z face), or, on levels one cell wide
 // in x (depth 1 only), 1 (general boundary march) and 2 (x faces)
+// A launch whose tiles all belong to one descriptor (lev; lev2: its finer
+// level for a restriction, its coarser for a prolongation; blk its block):
+// the kernel then reads the tile, the descriptors and the block's flag at
+// once instead of one after the other.  lev = -1: tiles of several.
+struct OneDesc {
+  int lev = -1, lev2 = -1, blk = -1;
+};
 template <class T>
 void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
-                     const int *active, int colour, int src, int zero);
+                     const int *active, int colour, int src, int zero, OneDesc one = OneDesc{});
 // one full red-black sweep (both colour halves) of small levels in one
 // launch: tiles of FUSED_X x FUSED_Y x FUSED_Z cells (Tile::x0, y0, z0 in
 // cells), kind 4
@@ -110,12 +117,12 @@ void launch_rb_fused(hipStream_t s, const Tile *tiles, int ntiles, const LevelDe
 // blocks with neighbours)
 template <class Tf, class Tc>
 void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
-                     const int *active, int src_f);
+                     const int *active, int src_f, OneDesc one = OneDesc{});
 // gen != 0: coarse ghosts read through the neighbour table (blocks with
 // neighbours)
 template <class Tf, class Tc>
 void launch_prolong(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                    const int *active, int src_f, int src_c, int gen);
+                    const int *active, int src_f, int src_c, int gen, OneDesc one = OneDesc{});
 template <class T>
 void launch_fill_zero(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                       const int *active, int buf);

@@ -561,12 +561,39 @@ __device__ __forceinline__ void xface_tile(const Tile &t, const LevelDev *__rest
 // disjoint cells.  One instantiation per depth keeps each one's register
 // count to what its own tiles need.
 template <class T, int KB>
+__device__ __forceinline__ void rb_dispatch(const Tile &t, const LevelDev &L,
+                                            const LevelDev *__restrict__ levs, int colour, int src,
+                                            int zero);
+
+template <class T, int KB>
 __global__ void __launch_bounds__(NTHREADS)
     k_rb_sweep(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
-               const int *__restrict__ active, int colour, int src, int zero) {
-  const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
-  const LevelDev &L = levs[t.lev];
-  if (!active[L.blk]) return;
+               const int *__restrict__ active, int colour, int src, int zero, OneDesc one) {
+  const int bid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  if (one.lev >= 0) {
+    // every tile of the launch has descriptor one.lev: the tile, the
+    // descriptor and the block's flag are read together, not one after the
+    // other (small levels are bound by this latency)
+    const Tile t = tiles[bid];
+    const LevelDev &L = levs[one.lev];
+    const int on = active[one.blk];
+    const int nk = L.nk;
+    // the three reads in flight together: nothing may sink below the test
+    asm volatile("" ::"s"(t.x0), "s"(on), "s"(nk));
+    if (!on) return;
+    rb_dispatch<T, KB>(t, L, levs, colour, src, zero);
+  } else {
+    const Tile t = tiles[bid];
+    const LevelDev &L = levs[t.lev];
+    if (!active[L.blk]) return;
+    rb_dispatch<T, KB>(t, L, levs, colour, src, zero);
+  }
+}
+
+template <class T, int KB>
+__device__ __forceinline__ void rb_dispatch(const Tile &t, const LevelDev &L,
+                                            const LevelDev *__restrict__ levs, int colour, int src,
+                                            int zero) {
   // the interior neighbours are zeros in the red half of a zero-state sweep
   const bool zint = zero && colour == 0;
   if (t.kind == 0) {
@@ -936,9 +963,23 @@ __device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev
 template <class Tf, class Tc, int KBC>
 __global__ void __launch_bounds__(NTHREADS)
     k_restrict(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
-               const int *__restrict__ active, int src_f) {
+               const int *__restrict__ active, int src_f, OneDesc one) {
   const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
-  const LevelDev &C = levs[t.lev];
+  // one descriptor pair for the whole launch: no load waits for another
+  const LevelDev &C = levs[one.lev >= 0 ? one.lev : t.lev];
+  if (one.lev >= 0) {
+    const LevelDev &F1 = levs[one.lev2];
+    const int on = active[one.blk];
+    const int nk = F1.nk, nkc = levs[one.lev].nk;
+    // the reads in flight together: nothing may sink below the test
+    asm volatile("" ::"s"(t.x0), "s"(on), "s"(nk), "s"(nkc));
+    if (!on) return;
+    if (t.kind == 0)
+      restrict_tile_fast<Tf, Tc, KBC, false>(t, levs[one.lev], F1, src_f);
+    else
+      restrict_tile_fast<Tf, Tc, KBC, true>(t, levs[one.lev], F1, src_f);
+    return;
+  }
   if (!active[C.blk]) return;
   const LevelDev &F = levs[C.finer];
   if (t.kind == 0)
@@ -1002,11 +1043,19 @@ __global__ void __launch_bounds__(NTHREADS)
 template <class Tf, class Tc, bool GEN>
 __global__ void __launch_bounds__(NTHREADS)
     k_prolong(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
-              const int *__restrict__ active, int src_f, int src_c) {
+              const int *__restrict__ active, int src_f, int src_c, OneDesc one) {
   const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
-  const LevelDev &F = levs[t.lev];
-  if (!active[F.blk]) return;
-  const LevelDev &C = levs[F.coarser];
+  // one descriptor pair for the whole launch: no load waits for another
+  if (one.lev >= 0) {
+    const int on = active[one.blk];
+    const int nk = levs[one.lev].nk, nkc = levs[one.lev2].nk;
+    // the reads in flight together: nothing may sink below the test
+    asm volatile("" ::"s"(t.x0), "s"(on), "s"(nk), "s"(nkc));
+    if (!on) return;
+  }
+  const LevelDev &F = levs[one.lev >= 0 ? one.lev : t.lev];
+  if (one.lev < 0 && !active[F.blk]) return;
+  const LevelDev &C = levs[one.lev >= 0 ? one.lev2 : F.coarser];
   const int l = t.x0 + (int)threadIdx.x;
   const int m = t.y0 + (int)threadIdx.y;
   const int n = t.z0;
@@ -3220,10 +3269,11 @@ __global__ void __launch_bounds__(TDMA_SWEEP_THREADS)
 // ---- launchers -----------------------------------------------------------
 template <class T>
 void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
-                     const int *active, int colour, int src, int zero) {
+                     const int *active, int colour, int src, int zero, OneDesc one) {
   if (ntiles <= 0) return;
   auto k = depth == 4 ? k_rb_sweep<T, 4> : depth == 2 ? k_rb_sweep<T, 2> : k_rb_sweep<T, 1>;
-  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour, src, zero);
+  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour, src, zero,
+                     one);
 }
 
 void launch_lsm_dens_restrict(hipStream_t s, const double *fine, int fni, int fnj, int fnk,
@@ -3332,7 +3382,7 @@ void launch_rb_fused(hipStream_t s, const Tile *tiles, int ntiles, const LevelDe
 
 template <class Tf, class Tc>
 void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
-                     const int *active, int src_f) {
+                     const int *active, int src_f, OneDesc one) {
   if (ntiles <= 0) return;
   if (depth == 0) {  // natural tiles of the coarse level, blocks with neighbours
     hipLaunchKernelGGL((k_restrict_gen<Tf, Tc>), dim3(ntiles), tile_block(), 0, s, tiles, levs,
@@ -3342,19 +3392,19 @@ void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, co
   auto k = depth == 4   ? k_restrict<Tf, Tc, 4>
            : depth == 2 ? k_restrict<Tf, Tc, 2>
                         : k_restrict<Tf, Tc, 1>;
-  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, src_f);
+  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, src_f, one);
 }
 
 template <class Tf, class Tc>
 void launch_prolong(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
-                    const int *active, int src_f, int src_c, int gen) {
+                    const int *active, int src_f, int src_c, int gen, OneDesc one) {
   if (ntiles <= 0) return;
   if (gen)
     hipLaunchKernelGGL((k_prolong<Tf, Tc, true>), dim3(ntiles), tile_block(), 0, s, tiles, levs,
-                       active, src_f, src_c);
+                       active, src_f, src_c, one);
   else
     hipLaunchKernelGGL((k_prolong<Tf, Tc, false>), dim3(ntiles), tile_block(), 0, s, tiles, levs,
-                       active, src_f, src_c);
+                       active, src_f, src_c, one);
 }
 
 template <class T>
@@ -3529,21 +3579,21 @@ template void launch_rb_fused<double>(hipStream_t, const Tile *, int, const Leve
 template void launch_rb_fused<float>(hipStream_t, const Tile *, int, const LevelDev *, const int *,
                                      int, int);
 template void launch_rb_sweep<double>(hipStream_t, const Tile *, int, int, const LevelDev *,
-                                      const int *, int, int, int);
+                                      const int *, int, int, int, OneDesc);
 template void launch_rb_sweep<float>(hipStream_t, const Tile *, int, int, const LevelDev *,
-                                     const int *, int, int, int);
+                                     const int *, int, int, int, OneDesc);
 template void launch_restrict<double, double>(hipStream_t, const Tile *, int, int, const LevelDev *,
-                                              const int *, int);
+                                              const int *, int, OneDesc);
 template void launch_restrict<float, float>(hipStream_t, const Tile *, int, int, const LevelDev *,
-                                            const int *, int);
+                                            const int *, int, OneDesc);
 template void launch_restrict<float, double>(hipStream_t, const Tile *, int, int, const LevelDev *,
-                                             const int *, int);
+                                             const int *, int, OneDesc);
 template void launch_prolong<double, double>(hipStream_t, const Tile *, int, const LevelDev *,
-                                             const int *, int, int, int);
+                                             const int *, int, int, int, OneDesc);
 template void launch_prolong<float, float>(hipStream_t, const Tile *, int, const LevelDev *,
-                                           const int *, int, int, int);
+                                           const int *, int, int, int, OneDesc);
 template void launch_prolong<float, double>(hipStream_t, const Tile *, int, const LevelDev *,
-                                            const int *, int, int, int);
+                                            const int *, int, int, int, OneDesc);
 template void launch_fill_zero<double>(hipStream_t, const Tile *, int, const LevelDev *,
                                        const int *, int);
 template void launch_fill_zero<float>(hipStream_t, const Tile *, int, const LevelDev *,

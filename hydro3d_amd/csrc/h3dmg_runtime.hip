@@ -102,8 +102,23 @@ struct TileSet {
   int gs_tile = GS_TILE;             // GS sets: tile edge (GS_TILE or GS_BIG_TILE)
   size_t tdma_lds = 0;  // TDMA sets: > 0 = whole sweeps in one workgroup per block (LDS bytes)
   bool tdma_pipe = false;  // TDMA whole sweeps: the pipelined kernel (constant coefficients)
+  OneDesc one;             // every tile on one descriptor: passed to the kernels
   int n() const { return (int)h.size(); }
 };
+
+// ts.one: the descriptor of every tile of the set, if they share one, with
+// its finer (rel < 0, restrictions) or coarser (rel > 0, prolongations)
+// level and its block
+void set_one(TileSet &ts, const std::vector<LevelDev> &levs, int rel) {
+  ts.one = OneDesc{};
+  if (ts.h.empty()) return;
+  const int lev = ts.h[0].lev;
+  for (const Tile &t : ts.h)
+    if (t.lev != lev) return;
+  ts.one.lev = lev;
+  ts.one.blk = levs[lev].blk;
+  ts.one.lev2 = rel < 0 ? levs[lev].finer : rel > 0 ? levs[lev].coarser : -1;
+}
 
 // Append the tiles covering an (nx, ny, nz) index space of descriptor `lev`.
 void add_tiles(TileSet &ts, int lev, int nx, int ny, int nz) {
@@ -444,7 +459,7 @@ void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool 
     else
       for (int colour = 0; colour < 2; ++colour)
         launch_rb_sweep<T>(h->stream, ts.d, ts.n(), ts.depth, h->d_levs, h->d_active, colour,
-                           buf, zero ? 1 : 0);
+                           buf, zero ? 1 : 0, ts.one);
     buf ^= 1;
     zero = false;
   }
@@ -509,21 +524,21 @@ void enqueue_vcycle(h3dmg_handle *h) {
         if (f64)
           launch_restrict<double, double>(h->stream, h->restr_nc[g].d, h->restr_nc[g].n(),
                                           h->restr_nc[g].depth, h->d_levs, h->d_active,
-                                          buf_nc[g - 1]);
+                                          buf_nc[g - 1], h->restr_nc[g].one);
         else
           launch_restrict<float, float>(h->stream, h->restr_nc[g].d, h->restr_nc[g].n(),
                                         h->restr_nc[g].depth, h->d_levs, h->d_active,
-                                        buf_nc[g - 1]);
+                                        buf_nc[g - 1], h->restr_nc[g].one);
       }
       if (!h->lsm && h->restr_c[g].n()) {
         if (f64)
           launch_restrict<double, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(),
                                           h->restr_c[g].depth, h->d_levs, h->d_active,
-                                          buf_nc[g - 1]);
+                                          buf_nc[g - 1], h->restr_c[g].one);
         else
           launch_restrict<float, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(),
                                          h->restr_c[g].depth, h->d_levs, h->d_active,
-                                         buf_nc[g - 1]);
+                                         buf_nc[g - 1], h->restr_c[g].one);
       }
     }
     if (g == Lt) {
@@ -578,18 +593,22 @@ void enqueue_vcycle(h3dmg_handle *h) {
         }
         if (f64)
           launch_prolong<double, double>(h->stream, h->prol_nc[g].d, h->prol_nc[g].n(), h->d_levs,
-                                         h->d_active, buf_nc[g], buf_nc[g + 1], h->connected);
+                                         h->d_active, buf_nc[g], buf_nc[g + 1], h->connected,
+                                         h->prol_nc[g].one);
         else
           launch_prolong<float, float>(h->stream, h->prol_nc[g].d, h->prol_nc[g].n(), h->d_levs,
-                                       h->d_active, buf_nc[g], buf_nc[g + 1], h->connected);
+                                       h->d_active, buf_nc[g], buf_nc[g + 1], h->connected,
+                                       h->prol_nc[g].one);
       }
       if (h->prol_c[g].n()) {
         if (f64)
           launch_prolong<double, double>(h->stream, h->prol_c[g].d, h->prol_c[g].n(), h->d_levs,
-                                         h->d_active, buf_nc[g], buf_c[g + 1], h->connected);
+                                         h->d_active, buf_nc[g], buf_c[g + 1], h->connected,
+                                         h->prol_c[g].one);
         else
           launch_prolong<float, double>(h->stream, h->prol_c[g].d, h->prol_c[g].n(), h->d_levs,
-                                        h->d_active, buf_nc[g], buf_c[g + 1], h->connected);
+                                        h->d_active, buf_nc[g], buf_c[g + 1], h->connected,
+                                        h->prol_c[g].one);
       }
       bool z = false;
       relax_any(h, f64, h->relax_nc[g], nu2, buf_nc[g], z);
@@ -1072,6 +1091,12 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
   h->tile_begin[nblocks] = h->fine_tiles.n();
   if (fine_general) h->fine_tiles.depth = 0;  // every launch on the general path
 
+  for (int g = 1; g <= Lm; ++g) {
+    set_one(h->restr_nc[g], h->levs, -1);
+    set_one(h->restr_c[g], h->levs, -1);
+    set_one(h->prol_nc[g], h->levs, 1);
+    set_one(h->prol_c[g], h->levs, 1);
+  }
   std::vector<TileSet *> all;
   for (int g = 1; g <= Lm; ++g)
     for (auto *v : {&h->restr_nc, &h->restr_c, &h->prol_nc, &h->prol_c, &h->nat_nc, &h->nat_c,
@@ -1424,6 +1449,7 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
     for (const Tile &t : ts.h)
       if (t.kind != 2 && t.kind != 4 && t.z1 - t.z0 != ts.depth)
         return fail(H3DMG_ERR_ARG, "internal: tile depth");
+    set_one(ts, h->levs, 0);
     Tile *d = nullptr;
     if (!ts.h.empty()) {
       h3dmg_status st = dev_alloc(h, ts.h.size() * sizeof(Tile), (void **)&d);
