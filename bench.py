@@ -306,7 +306,7 @@ def served_pressure_error(reqs, results):
     return {"sample": len(idx), "max_by_tolerance": worst}
 
 
-def serving(nreq, fp64_prefix=1000):
+def serving(nreq, fp64_prefix=1000, burst_requests=2000):
     """The serving stream (seeded Poisson arrivals at 200/s, sizes 32..256,
     tolerances 1e-6/1e-8, 20% warm reuse): the continuous-batching mixed
     server over all `nreq` requests, against sequential FP64 solves (one at
@@ -335,8 +335,19 @@ def serving(nreq, fp64_prefix=1000):
         res, wall = SV.run_stream(srv, pre, keep=set(), progress=progress)
         out["sequential_fp64"] = SV.summarize(res, wall)
         out["sequential_fp64"]["prefix_requests"] = len(pre)
+    # under load: the stream's first requests arriving 5x faster, served
+    # with and without joining running batches (max_batch_blocks 1: each
+    # lane solves one request at a time)
+    burst = reqs[:min(burst_requests, nreq)]
+    burst = [dict(d, t=d["t"] / 5.0) for d in burst]
+    for name, mbb in (("burst_1000rps_batched", 32), ("burst_1000rps_one_per_lane", 1)):
+        with SV.PoissonServer(hbm_budget=32 << 30, lanes=3, max_batch_blocks=mbb) as srv:
+            srv.warmup()
+            res, wall = SV.run_stream(srv, burst, keep=set())
+            out[name] = SV.summarize(res, wall)
+            out[name]["server"] = {k: int(v) for k, v in srv.stats.items()}
     out["workload"] = (f"{nreq} requests of the seeded stream (seed 4, 200/s); FP64 baseline on "
-                       f"the first {len(pre)}")
+                       f"the first {len(pre)}; bursts: the first {len(burst)} at 1000/s")
     return out
 
 

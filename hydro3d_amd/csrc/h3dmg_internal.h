@@ -211,11 +211,11 @@ void launch_tdma_diag(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
 constexpr int TDMA_SWEEP_THREADS = 1024;
 constexpr int TDMA_SWEEP_CPT = 2;  // cells per thread in each load batch
 constexpr int TDMA_SWEEP_CELLS = 4096;
-// levels with constant coefficients run the pipelined whole-sweep kernel
-inline bool tdma_sweep_pipelined(const LevelDev &L) { return L.lsm == nullptr && L.tdma != nullptr; }
+// pipelined: constant coefficients (the pipelined whole-sweep kernel and
+// its larger LDS layout; the host decides, before the level's tables exist)
 void launch_tdma_sweep(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev *levs,
                        const int *active, int src, int zero, size_t lds_bytes, bool pipelined);
-size_t tdma_sweep_lds_bytes(const LevelDev &L);
+size_t tdma_sweep_lds_bytes(const LevelDev &L, bool pipelined);
 // permits `lds_bytes` of dynamic LDS for the whole-sweep TDMA kernel
 hipError_t tdma_sweep_allow_lds(size_t lds_bytes);
 // fill LevelDev::tdma of every descriptor that has one (32*(ni+nj+nk) doubles)

@@ -3343,7 +3343,7 @@ void launch_tdma_tables(hipStream_t s, const LevelDev *levs, int ndesc) {
                      levs, ndesc);
 }
 
-size_t tdma_sweep_lds_bytes(const LevelDev &L) {
+size_t tdma_sweep_lds_bytes(const LevelDev &L, bool pipelined) {
   // per pass (nu lines of len cells): rs/c, a and 1/(ap - alo*a) (or, for
   // the pipelined passes, t1, x3*X3 and rs/c) for (nu+1)*len positions, the
   // previous diagonal's solution, nu*len; pipelined passes also the tables
@@ -3353,7 +3353,7 @@ size_t tdma_sweep_lds_bytes(const LevelDev &L) {
   for (const auto &g : geo) {
     const size_t nu = (size_t)g[0], len = (size_t)g[1];
     size_t need = 3 * (nu + 1) * len + nu * len;
-    if (tdma_sweep_pipelined(L)) need += 32 * len + 2 * len + 4 * nu;
+    if (pipelined) need += 32 * len + 2 * len + 4 * nu;
     most = std::max(most, need);
   }
   return most * sizeof(double);

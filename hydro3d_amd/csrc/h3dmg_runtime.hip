@@ -349,6 +349,8 @@ struct h3dmg_handle {
   std::vector<int> outcome;
 };
 
+constexpr size_t POOL_STAGE_BYTES = (size_t)32 << 20;
+
 // One device allocation carved into pieces (first fit, 256-byte aligned),
 // and a pinned array of per-block statistics handed out in runs: the memory
 // of handles whose blocks join and leave between V-cycles.
@@ -360,6 +362,10 @@ struct h3dmg_pool {
   std::map<size_t, size_t> used_;  // offset -> size
   RedPartial *h_stats = nullptr;   // pinned, max_blocks entries
   std::vector<char> hs_used;
+  // pinned staging of the tables a pooled handle uploads (builds hold the
+  // device mutex, so one at a time; reset once the build's copies are done)
+  char *h_stage = nullptr;
+  size_t stage_bytes = 0, stage_used = 0;
   std::mutex mu;
 };
 
@@ -411,6 +417,32 @@ size_t pool_size_at(h3dmg_pool *pool, size_t off) {
   std::lock_guard<std::mutex> lock(pool->mu);
   auto u = pool->used_.find(off);
   return u == pool->used_.end() ? 0 : u->second;
+}
+
+// Host-to-device copy on the handle's stream.  A pooled handle copies
+// through the pool's pinned staging (a true asynchronous DMA, no pageable
+// bounce); when the staging is full it waits for the copies in flight and
+// starts over.  The caller holds the device mutex.
+hipError_t h2d(h3dmg_handle *h, void *dst, const void *src, size_t n) {
+  h3dmg_pool *pool = h->pool;
+  if (!pool || !pool->h_stage || n > pool->stage_bytes)
+    return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, h->stream);
+  if (pool->stage_used + n > pool->stage_bytes) {
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return e;
+    pool->stage_used = 0;
+  }
+  char *p = pool->h_stage + pool->stage_used;
+  std::memcpy(p, src, n);
+  pool->stage_used = align_up(pool->stage_used + n, 256);
+  return hipMemcpyAsync(dst, p, n, hipMemcpyHostToDevice, h->stream);
+}
+
+// the handle's stream has drained: its staged copies are done
+hipError_t drain(h3dmg_handle *h) {
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && h->pool) h->pool->stage_used = 0;
+  return e;
 }
 
 // device memory of a handle's separately held tables: the pool's or its own
@@ -466,9 +498,11 @@ void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool 
 }
 
 // (Re)build and upload the smoother tiles of level g for both block groups.
-h3dmg_status upload_relax(h3dmg_handle *h, int g);
+h3dmg_status upload_relax(h3dmg_handle *h, int g, bool sync = true);
+h3dmg_status gen_relax(h3dmg_handle *h, int g);
 // (Re)choose the tail level and upload its tile list.
-h3dmg_status upload_tail(h3dmg_handle *h);
+h3dmg_status upload_tail(h3dmg_handle *h, bool sync = true);
+h3dmg_status gen_tail(h3dmg_handle *h);
 
 void relax_any(h3dmg_handle *h, bool is64, const TileSet &ts, int nsweep, int &buf, bool &zero) {
   if (ts.n() == 0) {
@@ -1097,13 +1131,27 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
     set_one(h->prol_nc[g], h->levs, 1);
     set_one(h->prol_c[g], h->levs, 1);
   }
+  // the smoother and tail tables (later re-tuning gives them pieces of
+  // their own, place_owned)
+  h->march_kb.assign(Lm + 1, 0);
+  for (int g = 1; g <= Lm; ++g) {
+    h3dmg_status st = gen_relax(h, g);
+    if (st != H3DMG_OK) return st;
+  }
+  {
+    // the tail kernel runs the red-black smoother on independent blocks only
+    if (h->prm.smoother != H3DMG_SMOOTH_RBGS || h->connected) h->tail_req = Lm + 1;
+    h3dmg_status st = gen_tail(h);
+    if (st != H3DMG_OK) return st;
+  }
   std::vector<TileSet *> all;
   for (int g = 1; g <= Lm; ++g)
     for (auto *v : {&h->restr_nc, &h->restr_c, &h->prol_nc, &h->prol_c, &h->nat_nc, &h->nat_c,
-                    &h->coarse_list})
+                    &h->coarse_list, &h->relax_nc, &h->relax_c})
       all.push_back(&(*v)[g]);
   all.push_back(&h->fine_tiles);
   all.push_back(&h->update_tiles);
+  all.push_back(&h->tail_tiles);
   std::vector<size_t> toff(all.size());
   for (size_t q = 0; q < all.size(); ++q) {
     toff[q] = off;
@@ -1201,29 +1249,33 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
     h->fine[b] = F;
   }
   h->d_stage = stage ? (double *)(A + stage_off) : nullptr;
+  // Everything the global region holds that the host knows goes up as one
+  // image (the tables, descriptors, flags and ranges; the partials and
+  // statistics in between are written by the kernels before they are read)
+  const size_t img0 = all.empty() ? levs_off : toff[0];
+  std::vector<char> img(off - img0);
+  auto put = [&](size_t at, const void *src, size_t n) {
+    if (n) std::memcpy(img.data() + (at - img0), src, n);
+  };
   for (size_t q = 0; q < all.size(); ++q) {
     all[q]->d = (Tile *)(A + toff[q]);
-    if (!all[q]->h.empty())
-      HIP_TRY(hipMemcpyAsync(all[q]->d, all[q]->h.data(), all[q]->h.size() * sizeof(Tile), hipMemcpyHostToDevice, h->stream));
+    all[q]->owned = nullptr;
+    put(toff[q], all[q]->h.data(), all[q]->h.size() * sizeof(Tile));
   }
   h->d_levs = (LevelDev *)(A + levs_off);
-  HIP_TRY(hipMemcpyAsync(h->d_levs, h->levs.data(), h->levs.size() * sizeof(LevelDev), hipMemcpyHostToDevice, h->stream));
-  if (h->prm.smoother == H3DMG_SMOOTH_TDMA) {
-    launch_tdma_tables(h->stream, h->d_levs, (int)h->levs.size());
-    HIP_TRY(hipGetLastError());
-  }
+  put(levs_off, h->levs.data(), h->levs.size() * sizeof(LevelDev));
   h->d_fine = (FineDev *)(A + fine_off);
-  HIP_TRY(hipMemcpyAsync(h->d_fine, h->fine.data(), nblocks * sizeof(FineDev), hipMemcpyHostToDevice, h->stream));
+  put(fine_off, h->fine.data(), nblocks * sizeof(FineDev));
   h->active.assign(nblocks, 1);
   h->d_active = (int *)(A + act_off);
-  HIP_TRY(hipMemcpyAsync(h->d_active, h->active.data(), nblocks * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  put(act_off, h->active.data(), nblocks * sizeof(int));
   h->d_tile_begin = (int *)(A + tb_off);
-  HIP_TRY(hipMemcpyAsync(h->d_tile_begin, h->tile_begin.data(), (nblocks + 1) * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  put(tb_off, h->tile_begin.data(), (nblocks + 1) * sizeof(int));
   h->d_partials = (RedPartial *)(A + part_off);
   h->d_stats = (RedPartial *)(A + stats_off);
   h->d_proj = (RedPartial *)(A + proj_off);
   h->d_proj_begin = (int *)(A + pb_off);
-  HIP_TRY(hipMemcpyAsync(h->d_proj_begin, h->proj_begin.data(), (nblocks + 1) * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  put(pb_off, h->proj_begin.data(), (nblocks + 1) * sizeof(int));
   h->vel.assign(nblocks, {});
   if (h->pool) {
     // a run of the pool's pinned statistics entries
@@ -1247,18 +1299,25 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
     std::vector<double *> ptrs((size_t)nblocks * 8);
     size_t ed = 0;
     for (int b = 0; b < nblocks; ++b) {
-      // a block carried over into a pooled handle keeps its tables
+      // the block's eigen tables (Q and lambda per axis, consecutive in its
+      // region) in one copy; a block carried over into a pooled handle keeps
+      // its tables
       const bool upload = fresh == nullptr || fresh[b];
+      const size_t e0 = eoff[8 * b];
+      std::vector<char> eimg(upload ? eoff[8 * b + 6] - e0 : 0);
       for (int a = 0; a < 3; ++a) {
         ptrs[8 * b + a] = (double *)(R[b] + eoff[8 * b + a]);
         if (upload)
-          HIP_TRY(hipMemcpyAsync(ptrs[8 * b + a], edata[ed].data(), edata[ed].size() * 8, hipMemcpyHostToDevice, h->stream));
+          std::memcpy(eimg.data() + (eoff[8 * b + a] - e0), edata[ed].data(), edata[ed].size() * 8);
         ++ed;
         ptrs[8 * b + 3 + a] = (double *)(R[b] + eoff[8 * b + 3 + a]);
         if (upload)
-          HIP_TRY(hipMemcpyAsync(ptrs[8 * b + 3 + a], edata[ed].data(), edata[ed].size() * 8, hipMemcpyHostToDevice, h->stream));
+          std::memcpy(eimg.data() + (eoff[8 * b + 3 + a] - e0), edata[ed].data(),
+                      edata[ed].size() * 8);
         ++ed;
       }
+      if (upload) HIP_TRY(h2d(h, R[b] + e0, eimg.data(), eimg.size()));
+      if (upload && !h->pool) HIP_TRY(hipStreamSynchronize(h->stream));  // eimg goes out of scope
       ptrs[8 * b + 6] = (double *)(R[b] + eoff[8 * b + 6]);
       ptrs[8 * b + 7] = (double *)(R[b] + eoff[8 * b + 7]);
     }
@@ -1276,10 +1335,14 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
     h->d_eig_q = base;
     h->d_eig_l = base + 3 * nblocks;
     h->d_scratch = base + 6 * nblocks;
-    HIP_TRY(hipMemcpyAsync(h->d_eig_q, q3.data(), q3.size() * sizeof(double *), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->d_eig_l, l3.data(), l3.size() * sizeof(double *), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->d_scratch, s2.data(), s2.size() * sizeof(double *), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));  // the host tables above go out of scope
+    put(eptr_off, q3.data(), q3.size() * sizeof(double *));
+    put(eptr_off + 3 * nblocks * sizeof(double *), l3.data(), l3.size() * sizeof(double *));
+    put(eptr_off + 6 * nblocks * sizeof(double *), s2.data(), s2.size() * sizeof(double *));
+  }
+  HIP_TRY(h2d(h, A + img0, img.data(), img.size()));
+  if (h->prm.smoother == H3DMG_SMOOTH_TDMA) {  // after the descriptors are up
+    launch_tdma_tables(h->stream, h->d_levs, (int)h->levs.size());
+    HIP_TRY(hipGetLastError());
   }
   // p = 0, f = 0 (a block carried over into a pooled handle keeps its own)
   for (int b = 0; b < nblocks; ++b) {
@@ -1288,18 +1351,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
     const size_t half = (size_t)L.H * L.nj * L.nk * 8;
     for (int q = 0; q < 4; ++q) HIP_TRY(hipMemsetAsync(R[b] + foff[4 * b + q], 0, half, h->stream));
   }
-  h->march_kb.assign(Lm + 1, 0);
-  for (int g = 1; g <= Lm; ++g) {
-    h3dmg_status st = upload_relax(h, g);
-    if (st != H3DMG_OK) return st;
-  }
-  {
-    // the tail kernel runs the red-black smoother on independent blocks only
-    if (h->prm.smoother != H3DMG_SMOOTH_RBGS || h->connected) h->tail_req = Lm + 1;
-    h3dmg_status st = upload_tail(h);
-    if (st != H3DMG_OK) return st;
-  }
-  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(drain(h));  // img goes out of scope
   return H3DMG_OK;
 }
 
@@ -1321,7 +1373,8 @@ bool gs_big_ok() {
   return ok;
 }
 
-h3dmg_status upload_relax(h3dmg_handle *h, int g) {
+// (Re)build the smoother tiles of level g on the host (both block groups)
+h3dmg_status gen_relax(h3dmg_handle *h, int g) {
   for (int grp = 0; grp < 2; ++grp) {
     TileSet &ts = grp == 0 ? h->relax_nc[g] : h->relax_c[g];
     ts.h.clear();
@@ -1404,7 +1457,7 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
             // diagonal's loads: at most half the workgroup runs lines)
             small = small && std::max(L.ni, L.nj) <= TDMA_SWEEP_THREADS / 2 &&
                     (long)L.ni * L.nj <= TDMA_SWEEP_CELLS && (long)L.ni * L.nk <= TDMA_SWEEP_CELLS;
-            lds = std::max(lds, tdma_sweep_lds_bytes(L));
+            lds = std::max(lds, tdma_sweep_lds_bytes(L, !h->lsm));
           }
         small = small && lds <= h->lds_budget && tdma_sweep_ok(h->lds_budget);
         ts.tdma_lds = small ? lds : 0;
@@ -1450,19 +1503,32 @@ h3dmg_status upload_relax(h3dmg_handle *h, int g) {
       if (t.kind != 2 && t.kind != 4 && t.z1 - t.z0 != ts.depth)
         return fail(H3DMG_ERR_ARG, "internal: tile depth");
     set_one(ts, h->levs, 0);
-    Tile *d = nullptr;
-    if (!ts.h.empty()) {
-      h3dmg_status st = dev_alloc(h, ts.h.size() * sizeof(Tile), (void **)&d);
-      if (st) return st;
-      HIP_TRY(hipMemcpyAsync(d, ts.h.data(), ts.h.size() * sizeof(Tile), hipMemcpyHostToDevice,
-                             h->stream));
-    }
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    dev_free(h, ts.owned);
-    ts.owned = d;
-    ts.d = d;
   }
   return H3DMG_OK;
+}
+
+// Place a table set in a piece of its own (replacing the one it had) and
+// upload it.  sync = false: nothing is replaced and the caller drains the
+// stream (a fresh handle's build).
+h3dmg_status place_owned(h3dmg_handle *h, TileSet &ts, bool sync) {
+  Tile *d = nullptr;
+  if (!ts.h.empty()) {
+    h3dmg_status st = dev_alloc(h, ts.h.size() * sizeof(Tile), (void **)&d);
+    if (st) return st;
+    HIP_TRY(h2d(h, d, ts.h.data(), ts.h.size() * sizeof(Tile)));
+  }
+  if (sync || ts.owned) HIP_TRY(drain(h));
+  dev_free(h, ts.owned);
+  ts.owned = d;
+  ts.d = d;
+  return H3DMG_OK;
+}
+
+h3dmg_status upload_relax(h3dmg_handle *h, int g, bool sync) {
+  h3dmg_status st = gen_relax(h, g);
+  if (st) return st;
+  if ((st = place_owned(h, h->relax_nc[g], sync))) return st;
+  return place_owned(h, h->relax_c[g], sync);
 }
 
 // LDS the tail needs when it starts at level g (max over the blocks that
@@ -1486,7 +1552,7 @@ int auto_tail_level(const h3dmg_handle *h) {
   return h->Lmax + 1;
 }
 
-h3dmg_status upload_tail(h3dmg_handle *h) {
+h3dmg_status gen_tail(h3dmg_handle *h) {
   h->tail_lt = h->tail_req > 0 ? h->tail_req : auto_tail_level(h);
   h->tail_lds = h->tail_lt <= h->Lmax ? tail_lds_at(h, h->tail_lt) : 0;
   if (h->tail_lds > h->lds_budget) {
@@ -1512,18 +1578,13 @@ h3dmg_status upload_tail(h3dmg_handle *h) {
   if (h->tail_lt <= h->Lmax)
     for (int b = 0; b < h->nb; ++b)
       if (h->blk[b].L >= h->tail_lt) ts.h.push_back(Tile{h->blk[b].desc[h->tail_lt], b, 0, 0, 0, 0});
-  Tile *d = nullptr;
-  if (!ts.h.empty()) {
-    h3dmg_status st = dev_alloc(h, ts.h.size() * sizeof(Tile), (void **)&d);
-    if (st) return st;
-    HIP_TRY(hipMemcpyAsync(d, ts.h.data(), ts.h.size() * sizeof(Tile), hipMemcpyHostToDevice,
-                           h->stream));
-  }
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  dev_free(h, ts.owned);
-  ts.owned = d;
-  ts.d = d;
   return H3DMG_OK;
+}
+
+h3dmg_status upload_tail(h3dmg_handle *h, bool sync) {
+  h3dmg_status st = gen_tail(h);
+  if (st) return st;
+  return place_owned(h, h->tail_tiles, sync);
 }
 
 h3dmg_status check_block(const h3dmg_handle *h, int b) {
@@ -1863,6 +1924,12 @@ h3dmg_status h3dmg_pool_create(unsigned long long bytes, int max_blocks, h3dmg_p
   }
   p->hs_used.assign(max_blocks, 0);
   p->free_[0] = p->bytes;
+  // staging of the handles' table uploads (optional: without it they copy
+  // from pageable memory)
+  if (hipHostMalloc((void **)&p->h_stage, POOL_STAGE_BYTES, hipHostMallocDefault) == hipSuccess)
+    p->stage_bytes = POOL_STAGE_BYTES;
+  else
+    p->h_stage = nullptr;
   *out = p;
   return H3DMG_OK;
 }
@@ -1873,6 +1940,7 @@ void h3dmg_pool_destroy(h3dmg_pool *pool) {
   (void)hipDeviceSynchronize();
   (void)hipFree(pool->base);
   (void)hipHostFree(pool->h_stats);
+  if (pool->h_stage) (void)hipHostFree(pool->h_stage);
   delete pool;
 }
 

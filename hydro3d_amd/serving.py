@@ -443,6 +443,7 @@ class PoissonServer:
                 self._try_admit()
                 self._lock.notify_all()
             return S, running
+        t1 = time.perf_counter()
         if S is not None:
             S.close()
         lane["S"] = S2  # from here on, an error closes S2
@@ -450,14 +451,23 @@ class PoissonServer:
             m.b = i
         for i, m in enumerate(running):
             S2.set_block_state(i, states[i])
+        t2 = time.perf_counter()
         for j in joiners:
             S2.set_rhs(j.b, j.req.f)
             S2.set_block_tol(j.b, j.req.tol)
             j.region.fresh = False
+        t3 = time.perf_counter()
         S2.solve_begin([j.b for j in joiners])
+        t4 = time.perf_counter()
         with self._lock:
-            # host and device time of the rebuild and the newcomers' start
-            self.stats["rebuild_us"] += int((time.perf_counter() - t0) * 1e6)
+            # host and device time of the rebuild and the newcomers' start:
+            # the new handle, the old one's release, the right-hand sides,
+            # the newcomers' first residual
+            self.stats["rebuild_us"] += int((t4 - t0) * 1e6)
+            self.stats["rebuild_create_us"] += int((t1 - t0) * 1e6)
+            self.stats["rebuild_close_us"] += int((t2 - t1) * 1e6)
+            self.stats["rebuild_rhs_us"] += int((t3 - t2) * 1e6)
+            self.stats["rebuild_begin_us"] += int((t4 - t3) * 1e6)
             self.stats["rebuilds"] += 1
             self.stats["joined"] += len(joiners)
             self.stats["joined_running"] += len(joiners) if running else 0
