@@ -366,6 +366,7 @@ struct h3dmg_pool {
   // device mutex, so one at a time; reset once the build's copies are done)
   char *h_stage = nullptr;
   size_t stage_bytes = 0, stage_used = 0;
+  int live = 0;  // handles built in the pool and not yet destroyed
   std::mutex mu;
 };
 
@@ -1799,10 +1800,10 @@ static void destroy_locked(h3dmg_handle *h) {
   dev_free(h, h->tail_tiles.owned);
   if (h->pool) {
     if (h->glob_taken) (void)pool_give(h->pool, h->glob_off);
-    if (h->hs_slot >= 0) {
-      std::lock_guard<std::mutex> lock(h->pool->mu);
+    std::lock_guard<std::mutex> lock(h->pool->mu);
+    if (h->hs_slot >= 0)
       for (int b = 0; b < h->nb; ++b) h->pool->hs_used[h->hs_slot + b] = 0;
-    }
+    --h->pool->live;
     h->h_stats = nullptr;  // the pool's
   }
   if (h->h_stats) (void)hipHostFree(h->h_stats);
@@ -1878,6 +1879,8 @@ h3dmg_status create_locked(const h3dmg_block *blocks, int nblocks, const h3dmg_p
   if (pool) {
     h->pool = pool;
     h->blk_off.assign(offsets, offsets + nblocks);
+    std::lock_guard<std::mutex> lock(pool->mu);
+    ++pool->live;  // destroy_locked gives it back, on failure too
   }
   h3dmg_status st = build(h, blocks, nblocks, fresh);
   if (st != H3DMG_OK) {
@@ -1937,6 +1940,13 @@ h3dmg_status h3dmg_pool_create(unsigned long long bytes, int max_blocks, h3dmg_p
 void h3dmg_pool_destroy(h3dmg_pool *pool) {
   if (!pool) return;
   std::lock_guard<std::mutex> lock(g_device_mutex);
+  {
+    std::lock_guard<std::mutex> plock(pool->mu);
+    if (pool->live > 0) {  // its memory is still in use: keep it
+      fail(H3DMG_ERR_ARG, "pool destroyed with %d live handles: not freed", pool->live);
+      return;
+    }
+  }
   (void)hipDeviceSynchronize();
   (void)hipFree(pool->base);
   (void)hipHostFree(pool->h_stats);

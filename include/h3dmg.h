@@ -321,7 +321,8 @@ typedef struct h3dmg_pool h3dmg_pool;
 /* `bytes` of device memory and `max_blocks` pinned per-block statistics
  * entries (the blocks of all the pool's live handles together). */
 h3dmg_status h3dmg_pool_create(unsigned long long bytes, int max_blocks, h3dmg_pool **out);
-/* Waits for the device; no handle of the pool may be alive. */
+/* Waits for the device and frees the pool.  While handles built in it are
+ * alive it frees nothing (h3dmg_last_error says so): destroy them first. */
 void h3dmg_pool_destroy(h3dmg_pool *pool);
 /* A piece of at least `bytes` (256-byte aligned offset from the pool's start). */
 h3dmg_status h3dmg_pool_alloc(h3dmg_pool *pool, unsigned long long bytes,

@@ -210,4 +210,14 @@ def test_gpu_stepped_outcomes_and_pool_refusals(gpu_lib):
             MultigridSolver([a], pool=pool, offsets=[off], **kw)
         pool.free(off)
         assert pool.free_bytes == pool.nbytes
+    # a pool with a live handle is not freed under it
+    pool = DevicePool(1 << 28, 4)
+    off = pool.alloc(block_bytes(a, **kw))
+    S = MultigridSolver([a], pool=pool, offsets=[off], **kw)
+    pool.lib.h3dmg_pool_destroy(pool.p)
+    assert b"live handles" in pool.lib.h3dmg_last_error()
+    S.set_rhs(0, _rhs(a, 9))  # its memory is still there
+    S.close()
+    pool.free(off)
+    pool.close()
     torch.cuda.synchronize()
