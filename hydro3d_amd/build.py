@@ -30,6 +30,7 @@ HIP_FLAGS = [
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
+    "-mllvm", "-amdgpu-kernarg-preload-count=16",
 ]
 
 
@@ -51,6 +52,21 @@ def build_lib(verbose=True) -> str:
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
+
+
+PROBES = ["probe_grid_barrier"]
+
+
+def build_probes() -> list:
+    """Measurement programs beside the library (executables in the package
+    directory): probe_grid_barrier times a device-scope grid barrier against
+    a kernel boundary in a captured graph."""
+    out = []
+    for name in PROBES:
+        exe = os.path.join(PKG, name)
+        _run([HIPCC, *HIP_FLAGS, "-o", exe, os.path.join(CSRC, name + ".hip")])
+        out.append(exe)
+    return out
 
 
 def build_fortran() -> str | None:
@@ -83,6 +99,7 @@ def build_fortran() -> str | None:
 
 def main(argv=None):
     build_lib()
+    build_probes()
     build_fortran()
 
 

@@ -346,7 +346,7 @@ __device__ __forceinline__ void march_tile(const Tile &t, const LevelDev *__rest
 // relaxation, as in march_tile.
 template <class T, int KB, bool ZERO_INT, bool BND>
 __device__ __forceinline__ void march_tile_fast(const Tile &t, const LevelDev &L, int colour,
-                                                int src, int zero) {
+                                                int src, int zero, bool on) {
   typedef typename Vec2<T>::type v2;
   constexpr unsigned SZ = sizeof(T);
   const int lpr = L.lpr;
@@ -355,7 +355,7 @@ __device__ __forceinline__ void march_tile_fast(const Tile &t, const LevelDev &L
   const int m0 = t.x0 + 2 * rl;
   const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
   const int ni = L.ni, nj = L.nj, nk = L.nk, H = L.H;
-  const bool col_ok = m0 < H && j < nj;
+  const bool col_ok = on && m0 < H && j < nj;  // on: the block's flag gates the stores
   const int jc = min(j, nj - 1);
   const unsigned row = (unsigned)H * SZ, plane = row * (unsigned)nj;  // bytes
   // byte offsets of the level's arrays from its arena base (phi[0][0])
@@ -563,46 +563,55 @@ __device__ __forceinline__ void xface_tile(const Tile &t, const LevelDev *__rest
 template <class T, int KB>
 __device__ __forceinline__ void rb_dispatch(const Tile &t, const LevelDev &L,
                                             const LevelDev *__restrict__ levs, int colour, int src,
-                                            int zero);
+                                            int zero, bool on);
 
-template <class T, int KB>
+// In the one-descriptor kernels, one empty asm statement takes the tile,
+// the block's flag and every descriptor field the kernel reads: all of them
+// are loaded before it, together (scalar loads complete out of order, so any
+// earlier use would wait for every load issued before it).
+
+// Kernel arguments are scalars only (no aggregate) and the grid size is
+// passed in: every argument then arrives preloaded in SGPRs instead of being
+// read from the argument buffer.
+// ONE: every tile of the launch has descriptor one_lev of block one_blk.  The
+// tile, the descriptor and the block's flag are then read together, and the
+// flag masks the stores instead of ending inactive workgroups early, so that
+// the data loads wait for none of them but the tile and the descriptor
+// (small levels are bound by this chain of latencies).
+template <class T, int KB, bool ONE>
 __global__ void __launch_bounds__(NTHREADS)
     k_rb_sweep(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
-               const int *__restrict__ active, int colour, int src, int zero, OneDesc one) {
-  const int bid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  if (one.lev >= 0) {
-    // every tile of the launch has descriptor one.lev: the tile, the
-    // descriptor and the block's flag are read together, not one after the
-    // other (small levels are bound by this latency)
-    const Tile t = tiles[bid];
-    const LevelDev &L = levs[one.lev];
-    const int on = active[one.blk];
-    const int nk = L.nk;
-    // the three reads in flight together: nothing may sink below the test
-    asm volatile("" ::"s"(t.x0), "s"(on), "s"(nk));
-    if (!on) return;
-    rb_dispatch<T, KB>(t, L, levs, colour, src, zero);
+               const int *__restrict__ active, int colour, int src, int zero, int one_lev,
+               int one_blk, int ntiles) {
+  const Tile t = tiles[xcd_remap((int)blockIdx.x, ntiles)];
+  if (ONE) {
+    const LevelDev &L = levs[one_lev];
+    const int on = active[one_blk];
+    asm volatile("" ::"s"(t.x0), "s"(t.y0), "s"(t.z0), "s"(t.kind), "s"(on), "s"(L.ni), "s"(L.nj),
+                 "s"(L.nk), "s"(L.H), "s"(L.per), "s"(L.lpr), "s"(L.cx), "s"(L.cy), "s"(L.cz),
+                 "s"(L.phi[0][0]), "s"(L.phi[0][1]), "s"(L.phi[1][0]), "s"(L.phi[1][1]),
+                 "s"(L.rhs[0]), "s"(L.rhs[1]));
+    rb_dispatch<T, KB>(t, L, levs, colour, src, zero, on != 0);
   } else {
-    const Tile t = tiles[bid];
     const LevelDev &L = levs[t.lev];
     if (!active[L.blk]) return;
-    rb_dispatch<T, KB>(t, L, levs, colour, src, zero);
+    rb_dispatch<T, KB>(t, L, levs, colour, src, zero, true);
   }
 }
 
 template <class T, int KB>
 __device__ __forceinline__ void rb_dispatch(const Tile &t, const LevelDev &L,
                                             const LevelDev *__restrict__ levs, int colour, int src,
-                                            int zero) {
+                                            int zero, bool on) {
   // the interior neighbours are zeros in the red half of a zero-state sweep
   const bool zint = zero && colour == 0;
   if (t.kind == 0) {
-    zint ? march_tile_fast<T, KB, true, false>(t, L, colour, src, zero)
-         : march_tile_fast<T, KB, false, false>(t, L, colour, src, zero);
+    zint ? march_tile_fast<T, KB, true, false>(t, L, colour, src, zero, on)
+         : march_tile_fast<T, KB, false, false>(t, L, colour, src, zero, on);
   } else if (t.kind == 3) {
-    zint ? march_tile_fast<T, KB, true, true>(t, L, colour, src, zero)
-         : march_tile_fast<T, KB, false, true>(t, L, colour, src, zero);
-  } else if (KB == 1) {
+    zint ? march_tile_fast<T, KB, true, true>(t, L, colour, src, zero, on)
+         : march_tile_fast<T, KB, false, true>(t, L, colour, src, zero, on);
+  } else if (KB == 1 && on) {
     if (t.kind == 1)
       march_tile<T, false>(t, levs, colour, src, zero);
     else
@@ -793,7 +802,7 @@ __global__ void __launch_bounds__(NTHREADS)
 // descriptor.
 template <class Tf, class Tc, int KBC, bool BND>
 __device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev &C,
-                                                   const LevelDev &F, int src_f) {
+                                                   const LevelDev &F, int src_f, bool on) {
   constexpr unsigned SZ = sizeof(Tf);
   constexpr int NF = 2 * KBC;
   const int lpr = C.lpr1;
@@ -802,7 +811,7 @@ __device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev
   const int l = t.x0 + rl;
   const int jc = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
   const int ni = F.ni, nj = F.nj, nk = F.nk, H = F.H;
-  const bool ok = l < C.ni && jc < C.nj;
+  const bool ok = on && l < C.ni && jc < C.nj;  // on: the block's flag gates the stores
   const bool px = F.per & 1, py = F.per & 2, pz = F.per & 4;
   const unsigned row = (unsigned)H * SZ, plane = row * (unsigned)nj;  // bytes
   const int j0 = 2 * min(jc, C.nj - 1);  // even: a has colour (r + k) & 1 in row j0 + r
@@ -960,32 +969,35 @@ __device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev
 
 // One launch per coarse level: tiles of KBC coarse planes, kind 0 when every
 // fine row and plane the tile reads is inside the block, kind 3 otherwise.
-template <class Tf, class Tc, int KBC>
+// ONE, scalar arguments: as k_rb_sweep (one_lev the coarse descriptor,
+// one_lev2 its finer level).
+template <class Tf, class Tc, int KBC, bool ONE>
 __global__ void __launch_bounds__(NTHREADS)
     k_restrict(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
-               const int *__restrict__ active, int src_f, OneDesc one) {
-  const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
-  // one descriptor pair for the whole launch: no load waits for another
-  const LevelDev &C = levs[one.lev >= 0 ? one.lev : t.lev];
-  if (one.lev >= 0) {
-    const LevelDev &F1 = levs[one.lev2];
-    const int on = active[one.blk];
-    const int nk = F1.nk, nkc = levs[one.lev].nk;
-    // the reads in flight together: nothing may sink below the test
-    asm volatile("" ::"s"(t.x0), "s"(on), "s"(nk), "s"(nkc));
-    if (!on) return;
+               const int *__restrict__ active, int src_f, int one_lev, int one_lev2, int one_blk,
+               int ntiles) {
+  const Tile t = tiles[xcd_remap((int)blockIdx.x, ntiles)];
+  if (ONE) {
+    const LevelDev &C = levs[one_lev], &F = levs[one_lev2];
+    const int on = active[one_blk];
+    asm volatile("" ::"s"(t.x0), "s"(t.y0), "s"(t.z0), "s"(t.kind), "s"(on), "s"(C.ni), "s"(C.nj),
+                 "s"(C.H), "s"(C.lpr1), "s"(C.rhs[0]), "s"(C.rhs[1]), "s"(F.ni), "s"(F.nj),
+                 "s"(F.nk), "s"(F.H), "s"(F.per), "s"(F.cx), "s"(F.cy), "s"(F.cz),
+                 "s"(F.phi[0][0]), "s"(F.phi[0][1]), "s"(F.phi[1][0]), "s"(F.phi[1][1]),
+                 "s"(F.rhs[0]), "s"(F.rhs[1]));
     if (t.kind == 0)
-      restrict_tile_fast<Tf, Tc, KBC, false>(t, levs[one.lev], F1, src_f);
+      restrict_tile_fast<Tf, Tc, KBC, false>(t, C, F, src_f, on != 0);
     else
-      restrict_tile_fast<Tf, Tc, KBC, true>(t, levs[one.lev], F1, src_f);
+      restrict_tile_fast<Tf, Tc, KBC, true>(t, C, F, src_f, on != 0);
     return;
   }
+  const LevelDev &C = levs[t.lev];
   if (!active[C.blk]) return;
   const LevelDev &F = levs[C.finer];
   if (t.kind == 0)
-    restrict_tile_fast<Tf, Tc, KBC, false>(t, C, F, src_f);
+    restrict_tile_fast<Tf, Tc, KBC, false>(t, C, F, src_f, true);
   else
-    restrict_tile_fast<Tf, Tc, KBC, true>(t, C, F, src_f);
+    restrict_tile_fast<Tf, Tc, KBC, true>(t, C, F, src_f, true);
 }
 
 // Restriction for blocks with neighbours: one thread per coarse cell of
@@ -1040,22 +1052,23 @@ __global__ void __launch_bounds__(NTHREADS)
 // level is updated in place (each fine cell by exactly one thread).
 // GEN: blocks with neighbours -- the coarse ghost corners (edges and corners
 // included) are read through resolve_cell, possibly in another block.
-template <class Tf, class Tc, bool GEN>
+// ONE, scalar arguments: as k_rb_sweep (one_lev the fine descriptor,
+// one_lev2 its coarser level).
+template <class Tf, class Tc, bool GEN, bool ONE>
 __global__ void __launch_bounds__(NTHREADS)
     k_prolong(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
-              const int *__restrict__ active, int src_f, int src_c, OneDesc one) {
-  const Tile t = tiles[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
-  // one descriptor pair for the whole launch: no load waits for another
-  if (one.lev >= 0) {
-    const int on = active[one.blk];
-    const int nk = levs[one.lev].nk, nkc = levs[one.lev2].nk;
-    // the reads in flight together: nothing may sink below the test
-    asm volatile("" ::"s"(t.x0), "s"(on), "s"(nk), "s"(nkc));
-    if (!on) return;
-  }
-  const LevelDev &F = levs[one.lev >= 0 ? one.lev : t.lev];
-  if (one.lev < 0 && !active[F.blk]) return;
-  const LevelDev &C = levs[one.lev >= 0 ? one.lev2 : F.coarser];
+              const int *__restrict__ active, int src_f, int src_c, int one_lev, int one_lev2,
+              int one_blk, int ntiles) {
+  const Tile t = tiles[xcd_remap((int)blockIdx.x, ntiles)];
+  const LevelDev &F = levs[ONE ? one_lev : t.lev];
+  const int on = ONE ? active[one_blk] : 1;
+  if (!ONE && !active[F.blk]) return;
+  const LevelDev &C = levs[ONE ? one_lev2 : F.coarser];
+  if (ONE)
+    asm volatile("" ::"s"(t.x0), "s"(t.y0), "s"(t.z0), "s"(on), "s"(F.ni), "s"(F.nj), "s"(F.nk),
+                 "s"(F.H), "s"(F.phi[0][0]), "s"(F.phi[0][1]), "s"(F.phi[1][0]), "s"(F.phi[1][1]),
+                 "s"(C.ni), "s"(C.nj), "s"(C.nk), "s"(C.H), "s"(C.per), "s"(C.phi[0][0]),
+                 "s"(C.phi[0][1]), "s"(C.phi[1][0]), "s"(C.phi[1][1]));
   const int l = t.x0 + (int)threadIdx.x;
   const int m = t.y0 + (int)threadIdx.y;
   const int n = t.z0;
@@ -1110,7 +1123,8 @@ __global__ void __launch_bounds__(NTHREADS)
           s = (q == 0) ? w * v : s + w * v;
         }
         gmem<Tf> *fa = ((2 * l + e + j + k) & 1) ? F1 : F0;
-        fa[fq] = fa[fq] + (Tf)(s / Tc(64));
+        const Tf nv = fa[fq] + (Tf)(s / Tc(64));
+        if (on) fa[fq] = nv;
       }
     }
   }
@@ -3271,9 +3285,14 @@ template <class T>
 void launch_rb_sweep(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
                      const int *active, int colour, int src, int zero, OneDesc one) {
   if (ntiles <= 0) return;
-  auto k = depth == 4 ? k_rb_sweep<T, 4> : depth == 2 ? k_rb_sweep<T, 2> : k_rb_sweep<T, 1>;
+  auto k = one.lev >= 0 ? (depth == 4   ? k_rb_sweep<T, 4, true>
+                           : depth == 2 ? k_rb_sweep<T, 2, true>
+                                        : k_rb_sweep<T, 1, true>)
+                        : (depth == 4   ? k_rb_sweep<T, 4, false>
+                           : depth == 2 ? k_rb_sweep<T, 2, false>
+                                        : k_rb_sweep<T, 1, false>);
   hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, colour, src, zero,
-                     one);
+                     one.lev, one.blk, ntiles);
 }
 
 void launch_lsm_dens_restrict(hipStream_t s, const double *fine, int fni, int fnj, int fnk,
@@ -3389,10 +3408,14 @@ void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, co
                        active, src_f);
     return;
   }
-  auto k = depth == 4   ? k_restrict<Tf, Tc, 4>
-           : depth == 2 ? k_restrict<Tf, Tc, 2>
-                        : k_restrict<Tf, Tc, 1>;
-  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, src_f, one);
+  auto k = one.lev >= 0 ? (depth == 4   ? k_restrict<Tf, Tc, 4, true>
+                           : depth == 2 ? k_restrict<Tf, Tc, 2, true>
+                                        : k_restrict<Tf, Tc, 1, true>)
+                        : (depth == 4   ? k_restrict<Tf, Tc, 4, false>
+                           : depth == 2 ? k_restrict<Tf, Tc, 2, false>
+                                        : k_restrict<Tf, Tc, 1, false>);
+  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, src_f, one.lev,
+                     one.lev2, one.blk, ntiles);
 }
 
 template <class Tf, class Tc>
@@ -3400,11 +3423,14 @@ void launch_prolong(hipStream_t s, const Tile *tiles, int ntiles, const LevelDev
                     const int *active, int src_f, int src_c, int gen, OneDesc one) {
   if (ntiles <= 0) return;
   if (gen)
-    hipLaunchKernelGGL((k_prolong<Tf, Tc, true>), dim3(ntiles), tile_block(), 0, s, tiles, levs,
-                       active, src_f, src_c, one);
+    hipLaunchKernelGGL((k_prolong<Tf, Tc, true, false>), dim3(ntiles), tile_block(), 0, s, tiles,
+                       levs, active, src_f, src_c, -1, -1, -1, ntiles);
+  else if (one.lev >= 0)
+    hipLaunchKernelGGL((k_prolong<Tf, Tc, false, true>), dim3(ntiles), tile_block(), 0, s, tiles,
+                       levs, active, src_f, src_c, one.lev, one.lev2, one.blk, ntiles);
   else
-    hipLaunchKernelGGL((k_prolong<Tf, Tc, false>), dim3(ntiles), tile_block(), 0, s, tiles, levs,
-                       active, src_f, src_c, one);
+    hipLaunchKernelGGL((k_prolong<Tf, Tc, false, false>), dim3(ntiles), tile_block(), 0, s, tiles,
+                       levs, active, src_f, src_c, -1, -1, -1, ntiles);
 }
 
 template <class T>
