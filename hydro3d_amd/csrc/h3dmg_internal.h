@@ -240,6 +240,44 @@ int launch_calmas(hipStream_t s, const FineDev *fd, const double *u, const doubl
                   const double *w, double *su, int to_fine, double dt, double dx, double dy,
                   double dz, RedPartial *partials, size_t cells);
 
+// ---- the solve loop's stopping rule on the device ------------------------
+// h3dmg_solve keeps one cycle queued ahead of the host: after each cycle one
+// workgroup applies the loop's rule (per block REL_L2 against its tolerance,
+// or the reference's global rmax criterion; min_cycles; a non-finite norm
+// stops every block) to that cycle's statistics, writes the next cycle's
+// block flags, and copies the outcome to host memory.
+struct DecideBlock {  // per block, set when the solve starts
+  double fnorm;       // ||f||_2
+  double tol;         // its REL_L2 tolerance
+  double vol;         // dx*dy*dz
+  int enabled;
+  int pad;
+};
+struct DecideState {  // per block, carried across cycles
+  double rel;         // relative residual after the last cycle it ran
+  double rmax;        // max |r| * rfac * vol after that cycle
+};
+struct DecideCtl {
+  int it;        // cycles run
+  int all_done;  // no block active after cycle it
+  int bad;       // a non-finite residual norm after cycle it
+  int pad;
+  double worst;  // max rel over enabled blocks (REF: max rmax)
+};
+struct DecideParams {
+  int nb, ref, connected, min_cycles;
+  double tol, rfac;
+};
+// ring slot of one cycle in host memory: DecideCtl, then nb DecideState, nb
+// RedPartial and nb int (the flags for the next cycle); slot it & 1
+__host__ __device__ inline size_t decide_slot_bytes(int nb) {
+  return (sizeof(DecideCtl) + (size_t)nb * (sizeof(DecideState) + sizeof(RedPartial) + sizeof(int)) +
+          255) & ~(size_t)255;
+}
+// (prm in device memory: set when a solve starts, the captured graph fixed)
+void launch_decide(hipStream_t s, const DecideBlock *blocks, DecideState *state, DecideCtl *ctl,
+                   const RedPartial *stats, int *active, char *ring, const DecideParams *prm);
+
 // *out = max over n partials of RedPartial::amax (one workgroup)
 void launch_partials_max(hipStream_t s, const RedPartial *parts, int n, double *out);
 // workgroups launch_calmas uses for `cells` cells

@@ -1537,6 +1537,75 @@ __global__ void __launch_bounds__(FIN_THREADS)
   if (threadIdx.x == 0) out[b] = RedPartial{s_ss[0], s_am[0]};
 }
 
+// The solve loop's stopping rule after cycle ctl->it + 1, as h3dmg_solve's
+// host loop applies it (newsolv_mg.for:57-59 for the reference criterion),
+// in the same order and IEEE operations: per block ||r|| = sqrt(ss), rel =
+// ||r|| / ||f|| (one global rel for blocks joined by faces), rmax = amax *
+// rfac * vol; done when (REF) it >= max(min_cycles, 2) and the global rmax <
+// tol, else it >= min_cycles and rel <= the block's tolerance.  One thread:
+// a handful of blocks, serial in block order like the host's sums.
+__global__ void __launch_bounds__(64)
+    k_decide(const DecideBlock *__restrict__ blocks, DecideState *__restrict__ state,
+             DecideCtl *__restrict__ ctl, const RedPartial *__restrict__ stats,
+             int *__restrict__ active, char *__restrict__ ring,
+             const DecideParams *__restrict__ prm_in) {
+  if (threadIdx.x != 0) return;
+  const DecideParams prm = *prm_in;
+  const int nb = prm.nb;
+  const int it = ctl->it + 1;
+  double grel = 0.0;
+  if (prm.connected) {
+    double ss = 0.0, ff = 0.0;
+    for (int b = 0; b < nb; ++b)
+      if (blocks[b].enabled) {
+        ss += stats[b].ss;
+        ff += blocks[b].fnorm * blocks[b].fnorm;
+      }
+    grel = ff > 0 ? sqrt(ss) / sqrt(ff) : sqrt(ss);
+  }
+  double worst = 0.0, gmax = 0.0;
+  int bad = 0;
+  for (int b = 0; b < nb; ++b) {
+    const DecideBlock &B = blocks[b];
+    if (!B.enabled) continue;
+    const double nrm = sqrt(stats[b].ss);
+    if (active[b]) {
+      state[b].rel = prm.connected ? grel : B.fnorm > 0 ? nrm / B.fnorm : nrm;
+      state[b].rmax = stats[b].amax * prm.rfac * B.vol;
+    }
+    if (!isfinite(nrm)) bad = 1;
+    worst = worst < state[b].rel ? state[b].rel : worst;
+    gmax = gmax < state[b].rmax ? state[b].rmax : gmax;
+  }
+  int all_done = 1;
+  for (int b = 0; b < nb; ++b) {
+    const DecideBlock &B = blocks[b];
+    bool done;
+    if (!B.enabled)
+      done = true;
+    else if (prm.ref)
+      done = it >= max(prm.min_cycles, 2) && gmax < prm.tol;
+    else
+      done = it >= prm.min_cycles && state[b].rel <= B.tol;
+    // a non-finite norm ends the loop: a cycle queued behind this one does
+    // nothing
+    if (done || bad) active[b] = 0;
+    if (active[b]) all_done = 0;
+  }
+  const DecideCtl c{it, all_done, bad, 0, prm.ref ? gmax : worst};
+  *ctl = c;
+  char *slot = ring + (size_t)(it & 1) * decide_slot_bytes(nb);
+  *(DecideCtl *)slot = c;
+  DecideState *rs = (DecideState *)(slot + sizeof(DecideCtl));
+  RedPartial *rp = (RedPartial *)(rs + nb);
+  int *ra = (int *)(rp + nb);
+  for (int b = 0; b < nb; ++b) {
+    rs[b] = state[b];
+    rp[b] = stats[b];
+    ra[b] = active[b];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Exact FP64 solve of the coarsest level of one block (extension, no
 // reference counterpart): A = Dx (+) Dy (+) Dz with Dx the 1-D Neumann or
@@ -3462,6 +3531,12 @@ void launch_update(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_update<T>, dim3(ntiles), tile_block(), 0, s, tiles, fine, levs, active,
                      buf);
+}
+
+void launch_decide(hipStream_t s, const DecideBlock *blocks, DecideState *state, DecideCtl *ctl,
+                   const RedPartial *stats, int *active, char *ring, const DecideParams *prm) {
+  hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, s, blocks, state, ctl, stats, active, ring,
+                     prm);
 }
 
 void launch_finalize(hipStream_t s, const int *tile_begin, int nblocks, const RedPartial *partials,

@@ -331,6 +331,16 @@ struct h3dmg_handle {
 
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // h3dmg_solve's queued-ahead loop (made on its first graph-mode solve):
+  // the cycle graph with the device-side stopping rule, the rule's state
+  // (DecideBlock[nb], DecideState[nb], DecideCtl), the two-slot ring in
+  // pinned host memory it writes each cycle's outcome to, and an event per
+  // slot
+  hipGraph_t graph_dec = nullptr;
+  hipGraphExec_t gexec_dec = nullptr;
+  char *d_dec = nullptr;
+  char *h_ring = nullptr, *d_ring = nullptr;
+  hipEvent_t ev_dec[2] = {nullptr, nullptr};
   bool rhs_ready = false;
   int pp_buf = 0;  // level-1 buffer that holds pp after the last V-cycle
   unsigned long long bytes = 0;
@@ -1599,6 +1609,10 @@ void drop_graph(h3dmg_handle *h) {
   if (h->graph) (void)hipGraphDestroy(h->graph);
   h->gexec = nullptr;
   h->graph = nullptr;
+  if (h->gexec_dec) (void)hipGraphExecDestroy(h->gexec_dec);
+  if (h->graph_dec) (void)hipGraphDestroy(h->graph_dec);
+  h->gexec_dec = nullptr;
+  h->graph_dec = nullptr;
   if (h->vgexec) (void)hipGraphExecDestroy(h->vgexec);
   if (h->vgraph) (void)hipGraphDestroy(h->vgraph);
   h->vgexec = nullptr;
@@ -1798,6 +1812,10 @@ static void destroy_locked(h3dmg_handle *h) {
   for (auto *v : {&h->relax_nc, &h->relax_c})
     for (TileSet &ts : *v) dev_free(h, ts.owned);
   dev_free(h, h->tail_tiles.owned);
+  dev_free(h, h->d_dec);
+  if (h->h_ring) (void)hipHostFree(h->h_ring);
+  for (hipEvent_t e : h->ev_dec)
+    if (e) (void)hipEventDestroy(e);
   if (h->pool) {
     if (h->glob_taken) (void)pool_give(h->pool, h->glob_off);
     std::lock_guard<std::mutex> lock(h->pool->mu);
@@ -2202,6 +2220,158 @@ h3dmg_status h3dmg_get_pp(h3dmg_handle *h, int b, double *pp, int ld1, int ld2, 
   return stage_padded_to_host(h, b, pp, ld1, ld2, off);
 }
 
+namespace {
+
+// the queued-ahead loop's device state (d_dec: DecideParams, DecideCtl,
+// DecideBlock[nb], DecideState[nb], each part 256-byte aligned), ring, events
+// and cycle graph
+size_t dec_part(int part, int nb) {
+  const size_t a = 256, p0 = 0, p1 = p0 + ((sizeof(DecideParams) + a - 1) & ~(a - 1));
+  const size_t p2 = p1 + ((sizeof(DecideCtl) + a - 1) & ~(a - 1));
+  const size_t p3 = p2 + ((sizeof(DecideBlock) * nb + a - 1) & ~(a - 1));
+  const size_t p4 = p3 + ((sizeof(DecideState) * nb + a - 1) & ~(a - 1));
+  const size_t offs[5] = {p0, p1, p2, p3, p4};
+  return offs[part];
+}
+
+h3dmg_status decide_setup(h3dmg_handle *h) {
+  const int nb = h->nb;
+  std::lock_guard<std::mutex> lock(g_device_mutex);  // allocations, capture
+  if (!h->d_dec) {
+    void *p = nullptr;
+    if (h3dmg_status st = dev_alloc(h, dec_part(4, nb), &p)) return st;
+    h->d_dec = (char *)p;
+  }
+  if (!h->h_ring) {
+    HIP_TRY(hipHostMalloc((void **)&h->h_ring, 2 * decide_slot_bytes(nb),
+                          hipHostMallocMapped | hipHostMallocCoherent));
+    HIP_TRY(hipHostGetDevicePointer((void **)&h->d_ring, h->h_ring, 0));
+  }
+  for (hipEvent_t &e : h->ev_dec)
+    if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (!h->gexec_dec) {
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+    enqueue_vcycle(h);
+    enqueue_update_residual(h, true);
+    launch_decide(h->stream, (const DecideBlock *)(h->d_dec + dec_part(2, nb)),
+                  (DecideState *)(h->d_dec + dec_part(3, nb)),
+                  (DecideCtl *)(h->d_dec + dec_part(1, nb)), h->d_stats, h->d_active, h->d_ring,
+                  (const DecideParams *)(h->d_dec + dec_part(0, nb)));
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture(h->stream, &g));
+    h->graph_dec = g;
+    HIP_TRY(hipGraphInstantiate(&h->gexec_dec, h->graph_dec, nullptr, nullptr, 0));
+  }
+  return H3DMG_OK;
+}
+
+// Cycle n in flight: is it likely the last one?  Each running block's
+// relative residual (REF: the global rmax) after cycle n, extrapolated from
+// its last two values at the rate between them, within 1.5x of its
+// tolerance (a wrong "last" costs one wait for the host, a wrong "not last"
+// one cycle of launches that do nothing).
+bool likely_last(const h3dmg_handle *h, int n, const std::vector<double> &r1,
+                 const std::vector<double> &r0, double g1, double g0) {
+  const h3dmg_params &P = h->prm;
+  const bool ref = P.criterion == H3DMG_CRIT_REF_RMAX;
+  if (n < P.min_cycles || (ref && n < 2) || n < 3) return false;
+  if (ref) return g0 > 0 && g1 * (g1 / g0) <= 1.5 * P.tol;
+  for (int b = 0; b < h->nb; ++b) {
+    if (!h->active[b]) continue;
+    const double tol = h->blk[b].tol > 0 ? h->blk[b].tol : P.tol;
+    if (!(r0[b] > 0) || r1[b] * (r1[b] / r0[b]) > 1.5 * tol) return false;
+  }
+  return true;
+}
+
+// h3dmg_solve's cycles with the next cycle queued before the host reads the
+// last one's outcome: the stopping rule runs on the device (k_decide) and
+// writes the flags the queued cycle reads, so the GPU does not wait for the
+// host between cycles.  A cycle is queued ahead unless the residual history
+// says the one in flight is likely the last; one queued after the last runs
+// with every block inactive.  The host's bookkeeping (per block rel, rmax,
+// cycles; the history; h_stats) is the device's outcome of each cycle.
+h3dmg_status solve_queued(h3dmg_handle *h, double *hist, int hist_cap, int &it, double &worst,
+                          bool &all_done, bool &bad) {
+  const h3dmg_params &P = h->prm;
+  const int nb = h->nb;
+  if (h3dmg_status st = decide_setup(h)) return st;
+  // the rule's inputs and the state after the initial residual
+  std::vector<char> img(dec_part(4, nb), 0);
+  DecideParams &dp = *(DecideParams *)(img.data() + dec_part(0, nb));
+  dp.nb = nb;
+  dp.ref = P.criterion == H3DMG_CRIT_REF_RMAX;
+  dp.connected = h->connected ? 1 : 0;
+  dp.min_cycles = P.min_cycles;
+  dp.tol = P.tol;
+  dp.rfac = P.dt;
+  DecideBlock *db = (DecideBlock *)(img.data() + dec_part(2, nb));
+  DecideState *ds = (DecideState *)(img.data() + dec_part(3, nb));
+  for (int b = 0; b < nb; ++b) {
+    const BlockInfo &B = h->blk[b];
+    db[b].fnorm = B.fnorm;
+    db[b].tol = B.tol > 0 ? B.tol : P.tol;
+    db[b].vol = B.spec.dx * B.spec.dy * B.spec.dz;
+    db[b].enabled = B.enabled ? 1 : 0;
+    ds[b].rel = B.rel;
+    ds[b].rmax = B.rmax;
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_device_mutex);  // a pool's staging
+    HIP_TRY(h2d(h, h->d_dec, img.data(), img.size()));
+  }
+  if (h3dmg_status st = push_active(h)) return st;
+  std::vector<double> r1(nb), r0(nb, 0.0);
+  for (int b = 0; b < nb; ++b) r1[b] = h->blk[b].rel;
+  double g1 = worst, g0 = 0.0;
+  int queued = 0;
+  auto queue = [&]() -> h3dmg_status {
+    ++queued;
+    HIP_TRY(hipGraphLaunch(h->gexec_dec, h->stream));
+    HIP_TRY(hipEventRecord(h->ev_dec[queued & 1], h->stream));
+    return H3DMG_OK;
+  };
+  if (h3dmg_status st = queue()) return st;
+  const size_t slot = decide_slot_bytes(nb);
+  while (true) {
+    const int n = it + 1;  // the cycle whose outcome is read next
+    if (queued == n && queued < P.maxcy && !likely_last(h, n, r1, r0, g1, g0))
+      if (h3dmg_status st = queue()) return st;
+    HIP_TRY(hipEventSynchronize(h->ev_dec[n & 1]));
+    const char *sl = h->h_ring + (size_t)(n & 1) * slot;
+    const DecideCtl c = *(const DecideCtl *)sl;
+    const DecideState *rs = (const DecideState *)(sl + sizeof(DecideCtl));
+    const RedPartial *rp = (const RedPartial *)(rs + nb);
+    const int *ra = (const int *)(rp + nb);
+    if (c.it != n) return fail(H3DMG_ERR_DEVICE, "cycle %d outcome reads cycle %d", n, c.it);
+    it = n;
+    for (int b = 0; b < nb; ++b) {
+      BlockInfo &B = h->blk[b];
+      if (h->active[b]) {
+        r0[b] = r1[b];
+        B.rel = rs[b].rel;
+        B.rmax = rs[b].rmax;
+        B.cycles = it;
+        r1[b] = B.rel;
+      }
+      h->h_stats[b] = rp[b];
+      h->active[b] = ra[b];
+    }
+    worst = c.worst;
+    g0 = g1;
+    g1 = worst;
+    all_done = c.all_done != 0;
+    bad = c.bad != 0;
+    if (hist && it < hist_cap) hist[it] = worst;
+    if (all_done || bad || it >= P.maxcy) break;
+    if (queued == it)
+      if (h3dmg_status st = queue()) return st;
+  }
+  return H3DMG_OK;
+}
+
+}  // namespace
+
 h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycles,
                          double *final_res) {
   if (!h) return fail(H3DMG_ERR_ARG, "null handle");
@@ -2276,6 +2446,10 @@ h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycl
     all_done = std::none_of(h->active.begin(), h->active.end(), [](int a) { return a != 0; });
   }
   int it = 0;
+  if (P.use_graph && !all_done && P.maxcy > 0 && !bad) {
+    st = solve_queued(h, hist, hist_cap, it, worst, all_done, bad);
+    if (st) return st;
+  }
   while (!all_done && it < P.maxcy && !bad) {
     st = push_active(h);
     if (st) return st;
