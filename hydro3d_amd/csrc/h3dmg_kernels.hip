@@ -1769,35 +1769,38 @@ __device__ __forceinline__ T lds_value(const LevelDev &L, int buf, int x, int y,
   return a[cidx32(L.H, L.nj, x, y, z)];
 }
 
-// neighbour (x,y,z) of a cell of colour `colour` in a half-sweep reading
-// buffer `src`: interior neighbours from the other colour's array `nb_int`,
-// ghosts from their image in buffer src.  One load from a selected array,
-// then the zero-state select.
-template <class T>
-__device__ __forceinline__ T sweep_nb(const LevelDev &L, const lds<T> *nb_int, const lds<T> *g0,
-                                      const lds<T> *g1, bool zero_int, bool zero_ghost, int x,
-                                      int y, int z) {
-  const bool ghost =
-      (unsigned)x >= (unsigned)L.ni || (unsigned)y >= (unsigned)L.nj || (unsigned)z >= (unsigned)L.nk;
-  const int xi = image(x, L.ni, L.per & 1), yi = image(y, L.nj, L.per & 2),
-            zi = image(z, L.nk, L.per & 4);
-  const lds<T> *a = ghost ? (((xi + yi + zi) & 1) ? g1 : g0) : nb_int;
-  const T v = a[cidx32(L.H, L.nj, xi, yi, zi)];
-  return (ghost ? zero_ghost : zero_int) ? T(0) : v;
-}
-
 // One colour half-sweep over a whole level, cell-parallel; the loads of
-// TAIL_U cells per thread are issued before any of their stores.
+// TAIL_U cells per thread are issued before any of their stores.  A red cell
+// reads its black neighbours before the sweep (buffer src), a black cell its
+// red neighbours after it (buffer 1-src); ghosts read their image in buffer
+// src; with zero != 0 buffer src holds zeros.  Coefficients and arithmetic
+// are cell_coef's and the per-level kernels'.  Only the coordinate a
+// neighbour differs in can leave the level, so each neighbour resolves one
+// image and tests one ghost condition, and entries are split into (entry,
+// row, plane) by float reciprocals with one exact correction (n < 2^24).
 template <class T>
-__device__ __forceinline__ void tail_half_sweep(const LevelDev &L, int colour, int src, int zero) {
-  const int H = L.H, nj = L.nj, nk = L.nk, ni = L.ni;
-  const int n = H * nj * nk;
+__device__ __forceinline__ void tail_half_sweep(const LevelDev &Ls, int colour, int src, int zero) {
+  // the descriptor lives in LDS: its fields once, in registers
+  const int H = Ls.H, nj = Ls.nj, nk = Ls.nk, ni = Ls.ni, per = Ls.per;
+  const bool px = per & 1, py = per & 2, pz = per & 4;
+  const bool wx0 = wall_face(Ls, 0), wx1 = wall_face(Ls, 1), wy0 = wall_face(Ls, 2),
+             wy1 = wall_face(Ls, 3), wz0 = wall_face(Ls, 4), wz1 = wall_face(Ls, 5);
+  const T cx = (T)Ls.cx, cy = (T)Ls.cy, cz = (T)Ls.cz;
+  const int n = H * nj * nk, plane = H * nj;
   const lds<T> *nb_int =
-      colour == 0 ? (const lds<T> *)L.phi[src][1] : (const lds<T> *)L.phi[1 - src][0];
-  const lds<T> *g0 = (const lds<T> *)L.phi[src][0], *g1 = (const lds<T> *)L.phi[src][1];
-  const lds<T> *R = (const lds<T> *)L.rhs[colour];
-  lds<T> *D = (lds<T> *)L.phi[1 - src][colour];
+      colour == 0 ? (const lds<T> *)Ls.phi[src][1] : (const lds<T> *)Ls.phi[1 - src][0];
+  const lds<T> *g0 = (const lds<T> *)Ls.phi[src][0], *g1 = (const lds<T> *)Ls.phi[src][1];
+  const lds<T> *R = (const lds<T> *)Ls.rhs[colour];
+  lds<T> *D = (lds<T> *)Ls.phi[1 - src][colour];
   const bool zero_int = zero && colour == 0, zero_ghost = zero != 0;
+  const float invH = 1.0f / (float)H, invJ = 1.0f / (float)nj;
+  // a neighbour: interior from nb_int, a ghost from its image in buffer src
+  // (colour array by the image's parity); zeros in the zero state
+  auto nbv = [&](bool ghost, int par, int entry) -> T {
+    const lds<T> *a = ghost ? ((par & 1) ? g1 : g0) : nb_int;
+    const T v = a[entry];
+    return (ghost ? zero_ghost : zero_int) ? T(0) : v;
+  };
   for (int base = (int)threadIdx.x; base < n; base += TAIL_THREADS * TAIL_U) {
     T v[TAIL_U][7];
     int ci[TAIL_U], cj[TAIL_U], ck[TAIL_U];
@@ -1806,28 +1809,54 @@ __device__ __forceinline__ void tail_half_sweep(const LevelDev &L, int colour, i
     for (int u = 0; u < TAIL_U; ++u) {
       const int e0 = base + u * TAIL_THREADS;
       const int e = min(e0, n - 1);  // lanes past the end read a valid entry
-      const int m = e % H, r = e / H, j = r % nj, k = r / nj;
+      int r = (int)((float)e * invH);
+      int m = e - r * H;
+      if (m < 0) {
+        --r;
+        m += H;
+      } else if (m >= H) {
+        ++r;
+        m -= H;
+      }
+      int k = (int)((float)r * invJ);
+      int j = r - k * nj;
+      if (j < 0) {
+        --k;
+        j += nj;
+      } else if (j >= nj) {
+        ++k;
+        j -= nj;
+      }
       const int i0 = 2 * m + ((colour + j + k) & 1);
       const int i = min(i0, ni - 1);
       ok[u] = e0 < n && i0 < ni;
       ci[u] = i;
       cj[u] = j;
       ck[u] = k;
-      v[u][0] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i - 1, j, k);
-      v[u][1] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i + 1, j, k);
-      v[u][2] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i, j - 1, k);
-      v[u][3] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i, j + 1, k);
-      v[u][4] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i, j, k - 1);
-      v[u][5] = sweep_nb<T>(L, nb_int, g0, g1, zero_int, zero_ghost, i, j, k + 1);
+      const int rb = (k * nj + j) * H;  // the row's first entry
+      const int xw = image(i - 1, ni, px), xe = image(i + 1, ni, px);
+      const int ys = image(j - 1, nj, py), yn = image(j + 1, nj, py);
+      const int zb = image(k - 1, nk, pz), zt = image(k + 1, nk, pz);
+      v[u][0] = nbv(i == 0, xw + j + k, rb + (xw >> 1));
+      v[u][1] = nbv(i == ni - 1, xe + j + k, rb + (xe >> 1));
+      v[u][2] = nbv(j == 0, i + ys + k, rb + (ys - j) * H + (i >> 1));
+      v[u][3] = nbv(j == nj - 1, i + yn + k, rb + (yn - j) * H + (i >> 1));
+      v[u][4] = nbv(k == 0, i + j + zb, rb + (zb - k) * plane + (i >> 1));
+      v[u][5] = nbv(k == nk - 1, i + j + zt, rb + (zt - k) * plane + (i >> 1));
       v[u][6] = R[e];
     }
 #pragma unroll
     for (int u = 0; u < TAIL_U; ++u) {
       if (!ok[u]) continue;
-      const Coef<T> c = cell_coef<T>(L, ci[u], cj[u], ck[u]);
-      const T sum = c.aw * v[u][0] + c.ae * v[u][1] + c.as * v[u][2] + c.an * v[u][3] +
-                    c.ab * v[u][4] + c.at * v[u][5];
-      D[base + u * TAIL_THREADS] = div_ap(v[u][6] - sum, c.ap, T(1) / c.ap);
+      const int i = ci[u], j = cj[u], k = ck[u];
+      // cell_coef's coefficients: zero towards a wall
+      const T aw = (i == 0 && wx0) ? T(0) : cx, ae = (i == ni - 1 && wx1) ? T(0) : cx;
+      const T as = (j == 0 && wy0) ? T(0) : cy, an = (j == nj - 1 && wy1) ? T(0) : cy;
+      const T ab = (k == 0 && wz0) ? T(0) : cz, at = (k == nk - 1 && wz1) ? T(0) : cz;
+      const T ap = T(-1) * (aw + ae + as + an + ab + at);
+      const T sum = aw * v[u][0] + ae * v[u][1] + as * v[u][2] + an * v[u][3] + ab * v[u][4] +
+                    at * v[u][5];
+      D[base + u * TAIL_THREADS] = div_ap(v[u][6] - sum, ap, T(1) / ap);
     }
   }
   __syncthreads();
