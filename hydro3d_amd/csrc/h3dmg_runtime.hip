@@ -1555,8 +1555,18 @@ size_t tail_lds_at(const h3dmg_handle *h, int g) {
 
 // Automatic tail level: the first level from which every block's remaining
 // levels fit in one workgroup's LDS; none when no level does.
+// The tail's one workgroup per block is bound by its CU's VALU issue, so it
+// pays for the exact solves of the blocks' coarsest levels (all of them in
+// one launch, without their levels' global round trips), not for levels
+// above them, and not for a smoothed coarsest level: with the exact solve,
+// the first level from the shallowest block's coarsest down whose tail fits
+// in LDS; otherwise off.  Measured on MI355X (single 32^3 to 256^3 blocks
+// and batches of 2 to 48 blocks of mixed sizes, both precisions).
 int auto_tail_level(const h3dmg_handle *h) {
-  for (int g = 1; g <= h->Lmax; ++g) {
+  if (h->prm.coarse != H3DMG_COARSE_EXACT) return h->Lmax + 1;
+  int t0 = h->Lmax;
+  for (const BlockInfo &B : h->blk) t0 = std::min(t0, B.L);
+  for (int g = std::max(t0, 1); g <= h->Lmax; ++g) {
     const size_t need = tail_lds_at(h, g);
     if (need > 0 && need <= h->lds_budget) return g;
   }

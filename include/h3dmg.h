@@ -405,9 +405,10 @@ h3dmg_status h3dmg_set_fused(h3dmg_handle *h, long long max_cells);
 long long h3dmg_get_fused(const h3dmg_handle *h);
 /* Levels >= `level` of every block run as one workgroup per block in a
  * single launch with every level in LDS (the V-cycle tail); 0 restores the
- * automatic choice (the first level from which each block's remaining levels
- * fit in one workgroup's LDS), a level above every block's coarsest turns the
- * tail off; a level that does not fit returns H3DMG_ERR_ARG.  Results are
+ * automatic choice (with H3DMG_COARSE_EXACT the first level, from the
+ * shallowest block's coarsest down, whose remaining levels fit in one
+ * workgroup's LDS: the tail then runs the blocks' exact coarse solves;
+ * otherwise off), a level above every block's coarsest turns the tail off; a level that does not fit returns H3DMG_ERR_ARG.  Results are
  * identical either way.  Drops a captured graph.  h3dmg_get_tail returns the
  * level in use (nlevels_max + 1 when off).  Red-black smoother only: with GS
  * or TDMA the tail is off and a level that would turn it on returns

@@ -386,9 +386,9 @@ def test_gpu_tail_level_does_not_change_results(gpu_lib, prec):
             S.solve(raise_on_fail=True)
             cyc, _ = S.block_stats()
             results[tail] = (used, list(cyc), [S.get_p(b) for b in range(len(specs))])
-    # auto picks the first level whose remaining levels fit in LDS (2 here);
-    # 5 is above every coarsest level: off
-    assert results[0][0] == 2 and results[5][0] == 5
+    # auto: with the exact solve the shallowest coarsest level (3 here), off
+    # with a smoothed coarsest level; 5 is above every coarsest level: off
+    assert results[0][0] == (3 if prec == "mixed" else 5) and results[5][0] == 5
     assert {2, 3, 4} <= set(results), sorted(results)
     ref = results[5]
     for tail, (used, cyc, ps) in results.items():
