@@ -65,7 +65,7 @@ struct LevelDev {
 struct FineDev {
   int ni, nj, nk, H, per, blk;
   int nbr[6];  // neighbouring BLOCK across face f (the level-1 descriptors), or -1
-  int lpr;  // residual lanes per row (one colour-array entry per lane)
+  int lpr;  // residual lanes per row segment (two colour-array entries per lane)
   double cx, cy, cz;
   double *p[2];  // [colour]
   double *f[2];  // [colour]

@@ -1179,19 +1179,22 @@ __device__ __forceinline__ void block_reduce(double &ss, double &am) {
 // cells a = 2l (colour ca = (j+k)&1) and b = 2l+1.  Planes k-1..k+1 of both
 // entries stream through a register window, x-neighbours come from the lane's
 // other cell or the adjacent lanes (DPP shift), rows j-1, j+1 are loaded.
-// F.lpr lanes cover a row (64/F.lpr rows per wavefront); the tile walks the
-// planes [t.z0, t.z1).  Tile kind 0: every row/plane of the tile has its y/z
-// neighbours inside the block; kind 1: the general path with ghost images.
-// use_p = 0 gives r = f (the norm of the right-hand side).
+// A residual tile spans 2*F.lpr colour entries of a row (64/F.lpr rows per
+// wavefront; the fast kernel gives each lane two entries); this path walks
+// it as two halves of F.lpr entries, one entry per lane (x0: the half's
+// first entry), over the planes [t.z0, t.z1).  Tile kind 0: every row/plane
+// of the tile has its y/z neighbours inside the block; kind 1: the general
+// path with ghost images.  use_p = 0 gives r = f (the norm of the
+// right-hand side).
 template <class T, bool INNER>
-__device__ __forceinline__ void residual_tile(const Tile &t, const FineDev *__restrict__ fine,
+__device__ __forceinline__ void residual_half(const Tile &t, int x0, const FineDev *__restrict__ fine,
                                               const LevelDev &L, int use_p, double &ss,
                                               double &am) {
   const FineDev &F = fine[t.lev];
   const int lpr = F.lpr;
   const int lane = (int)threadIdx.x;
   const int rl = lane & (lpr - 1);
-  const int l = t.x0 + rl;
+  const int l = x0 + rl;
   const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
   const int ni = F.ni, nj = F.nj, nk = F.nk, H = F.H;
   const bool ok = 2 * l < ni && j < nj;
@@ -1299,11 +1302,20 @@ __device__ __forceinline__ void residual_tile(const Tile &t, const FineDev *__re
   }
 }
 
+template <class T, bool INNER>
+__device__ __forceinline__ void residual_tile(const Tile &t, const FineDev *__restrict__ fine,
+                                              const LevelDev &L, int use_p, double &ss,
+                                              double &am) {
+  const int lpr = fine[t.lev].lpr;
+  residual_half<T, INNER>(t, t.x0, fine, L, use_p, ss, am);
+  residual_half<T, INNER>(t, t.x0 + lpr, fine, L, use_p, ss, am);
+}
+
 // Finest-level residual tile of KB planes, software-pipelined: a rolling
 // window of planes k-1, k, k+1 of both colours, with the loads of plane k+1
 // (its f, rows j-1/j+1, x-edge values and window plane k+2) issued before
-// plane k is computed.  Every neighbour is addressed by the coordinates of
-// its image (a ghost row, plane or x position resolved once per lane or per
+// plane k is computed), two colour entries per lane.  Every neighbour is
+// addressed by the coordinates of its image (a ghost row, plane or x position resolved once per lane or per
 // plane), as a 32-bit byte offset from the block's first fine array, so
 // boundary tiles (BND) run the same code as interior ones; only their
 // coefficients differ (cell_coef's zeros towards walls).  The arithmetic of
@@ -1311,15 +1323,22 @@ __device__ __forceinline__ void residual_tile(const Tile &t, const FineDev *__re
 template <class T, int KB, bool BND>
 __device__ __forceinline__ void residual_tile_fast(const Tile &t, const FineDev &F,
                                                    const LevelDev &L, double &ss, double &am) {
+  typedef typename Vec2<double>::type d2;
+  typedef typename Vec2<T>::type t2;
   constexpr unsigned SZ = sizeof(double);
+  // a lane holds entries l0, l0+1 of both colour arrays of its row: cells
+  // x0 (a0), x0+1 (b0), x0+2 (a1), x0+3 (b1); a0/a1 share a colour array, as
+  // do b0/b1, so each array is one 16-byte load
   const int lpr = F.lpr;
   const int lane = (int)threadIdx.x;
   const int rl = lane & (lpr - 1);
-  const int l = t.x0 + rl;
+  const int l0 = t.x0 + 2 * rl;
   const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
   const int ni = F.ni, nj = F.nj, nk = F.nk, H = F.H;
-  const bool ok = 2 * l < ni && j < nj;
-  const bool b_ok = 2 * l + 1 < ni && j < nj;
+  const int x0 = 2 * l0;
+  const bool row_ok = j < nj;
+  const bool ok_a0 = row_ok && x0 < ni, ok_b0 = row_ok && x0 + 1 < ni;
+  const bool ok_a1 = row_ok && x0 + 2 < ni, ok_b1 = row_ok && x0 + 3 < ni;
   const int jc = min(j, nj - 1);
   const bool px = F.per & 1, py = F.per & 2, pz = F.per & 4;
   const unsigned row = (unsigned)H * SZ, plane = row * (unsigned)nj;  // bytes
@@ -1330,34 +1349,43 @@ __device__ __forceinline__ void residual_tile_fast(const Tile &t, const FineDev 
   const gmem<char> *Bs = (const gmem<char> *)base;
   gmem<T> *R[2] = {(gmem<T> *)L.rhs[0], (gmem<T> *)L.rhs[1]};
   auto ld = [&](unsigned off) -> double { return *(const gmem<double> *)(Bs + off); };
-  // p of the cell at (x, y, k) with x even/odd given by the parity argument:
-  // colour array (x + y + k) & 1
+  auto ld2 = [&](unsigned off) -> d2 { return *(const gmem<d2> *)(Bs + off); };
+  // p of the cells of parity `par` (x + y + k) in a row: colour array par & 1
   auto pcol = [&](int par) -> unsigned { return (par & 1) ? aP1 : 0u; };
-  const int ia = 2 * l, ib = ia + 1;
-  const unsigned ent = (unsigned)min(l, H - 1) * SZ;
+  // lanes past the row's last pair read that pair (nothing of theirs is stored)
+  const unsigned ent = (unsigned)min(l0, H - 2) * SZ;
   // rows of the y neighbours, ghosts resolved to their image row
   const int ys = BND ? image(jc - 1, nj, py) : jc - 1;
   const int yn = BND ? image(jc + 1, nj, py) : jc + 1;
   const unsigned o_c = (unsigned)jc * row + ent;
   const unsigned o_s = (unsigned)ys * row + ent, o_n = (unsigned)yn * row + ent;
-  // x neighbours not held by the adjacent lane: x-ghost image or the next
-  // tile's entry
-  const bool w_edge = ia == 0 || (rl == 0 && l > 0);
-  const bool e_edge = ib + 1 >= ni || (rl == lpr - 1 && l + 1 < H);
-  const int xw = image(ia - 1, ni, px), xe = image(ib + 1, ni, px);
-  const unsigned w_ent = (unsigned)jc * row + (w_edge ? (unsigned)(xw >> 1) * SZ : ent);
-  const unsigned e_ent = (unsigned)jc * row + (e_edge ? (unsigned)(xe >> 1) * SZ : ent);
-  const int w_par = w_edge ? (xw & 1) : 0, e_par = e_edge ? (xe & 1) : 0;
+  // x neighbours no lane of the row segment holds: cell x0-1 for the
+  // segment's first lane (the west ghost's image, or the previous tile's last
+  // entry); for the lane holding cell ni-1 (or lying past it) the east
+  // ghost's image, for the segment's last lane otherwise the next tile's
+  // first entry
+  const bool w_edge = rl == 0;
+  const bool e_last = x0 + 3 >= ni - 1;
+  const bool e_edge = e_last || rl == lpr - 1;
+  const int xw = image(x0 - 1, ni, px);
+  const int xe = e_last ? image(ni, ni, px) : x0 + 4;
+  const unsigned w_ent = (unsigned)jc * row + (unsigned)(xw >> 1) * SZ;
+  const unsigned e_ent = (unsigned)jc * row + (unsigned)(xe >> 1) * SZ;
+  const int w_par = xw & 1, e_par = xe & 1;
   const int x1_par = w_edge ? w_par : e_par;
   const unsigned x1_ent = w_edge ? w_ent : e_ent;
   const double cx = F.cx, cy = F.cy, cz = F.cz;
-  const double aw_a = (ia == 0 && !px) ? 0.0 : cx, ae_a = (ia == ni - 1 && !px) ? 0.0 : cx;
-  const double aw_b = cx, ae_b = (ib == ni - 1 && !px) ? 0.0 : cx;
+  const double aw_a0 = (x0 == 0 && !px) ? 0.0 : cx;
+  const double ae_a0 = (x0 == ni - 1 && !px) ? 0.0 : cx;
+  const double ae_b0 = (x0 + 1 == ni - 1 && !px) ? 0.0 : cx;
+  const double ae_a1 = (x0 + 2 == ni - 1 && !px) ? 0.0 : cx;
+  const double ae_b1 = (x0 + 3 == ni - 1 && !px) ? 0.0 : cx;
   const double as = (BND && !py && jc == 0) ? 0.0 : cy;
   const double an = (BND && !py && jc == nj - 1) ? 0.0 : cy;
 
   struct PlaneSet {
-    double fa, fb, sa, sb, na, nb, we, ee, ap1, bp1;  // ap1/bp1: window plane k+1
+    d2 fa, fb, sa, sb, na, nb, ap1, bp1;  // ap1/bp1: window plane k+1
+    double we, ee;
   };
   // the window plane after k: plane k+1 or its image
   auto next_plane = [&](int k) -> int { return BND ? image(k + 1, nk, pz) : k + 1; };
@@ -1365,16 +1393,16 @@ __device__ __forceinline__ void residual_tile_fast(const Tile &t, const FineDev 
     PlaneSet q;
     const unsigned o = (unsigned)k * plane;
     const int ca = (jc + k) & 1;
-    q.fa = ld((ca ? aF1 : aF0) + o + o_c);
-    q.fb = ld((ca ? aF0 : aF1) + o + o_c);
-    q.sa = ld(pcol(ys + k) + o + o_s);
-    q.sb = ld(pcol(ys + k + 1) + o + o_s);
-    q.na = ld(pcol(yn + k) + o + o_n);
-    q.nb = ld(pcol(yn + k + 1) + o + o_n);
+    q.fa = ld2((ca ? aF1 : aF0) + o + o_c);
+    q.fb = ld2((ca ? aF0 : aF1) + o + o_c);
+    q.sa = ld2(pcol(ys + k) + o + o_s);
+    q.sb = ld2(pcol(ys + k + 1) + o + o_s);
+    q.na = ld2(pcol(yn + k) + o + o_n);
+    q.nb = ld2(pcol(yn + k + 1) + o + o_n);
     const int kp = next_plane(k);
-    q.ap1 = ld(pcol(jc + kp) + (unsigned)kp * plane + o_c);
-    q.bp1 = ld(pcol(jc + kp + 1) + (unsigned)kp * plane + o_c);
-    // x-edge values: only the lanes at a row segment's ends use them
+    q.ap1 = ld2(pcol(jc + kp) + (unsigned)kp * plane + o_c);
+    q.bp1 = ld2(pcol(jc + kp + 1) + (unsigned)kp * plane + o_c);
+    // x-edge values: only the lanes at a row segment's ends load them
     double x1 = 0.0, x2 = 0.0;
     if (w_edge || e_edge) x1 = ld(pcol(x1_par + jc + k) + o + x1_ent);
     if (w_edge && e_edge) x2 = ld(pcol(e_par + jc + k) + o + e_ent);
@@ -1383,53 +1411,78 @@ __device__ __forceinline__ void residual_tile_fast(const Tile &t, const FineDev 
     return q;
   };
   const int k0 = t.z0;
-  // window: cells a/b of planes k0-1 and k0
-  double am1, bm1, a0, b0;
+  // window: the a and b pairs of planes k0-1 and k0
+  d2 am1, bm1, A, B;
   {
     const int km = BND ? image(k0 - 1, nk, pz) : k0 - 1;
-    am1 = ld(pcol(jc + km) + (unsigned)km * plane + o_c);
-    bm1 = ld(pcol(jc + km + 1) + (unsigned)km * plane + o_c);
-    a0 = ld(pcol(jc + k0) + (unsigned)k0 * plane + o_c);
-    b0 = ld(pcol(jc + k0 + 1) + (unsigned)k0 * plane + o_c);
+    am1 = ld2(pcol(jc + km) + (unsigned)km * plane + o_c);
+    bm1 = ld2(pcol(jc + km + 1) + (unsigned)km * plane + o_c);
+    A = ld2(pcol(jc + k0) + (unsigned)k0 * plane + o_c);
+    B = ld2(pcol(jc + k0 + 1) + (unsigned)k0 * plane + o_c);
   }
-  PlaneSet cur = load_set(k0);
 #pragma unroll
   for (int p = 0; p < KB; ++p) {
     const int k = k0 + p;
-    PlaneSet nxt;
-    if (p + 1 < KB) nxt = load_set(k + 1);
-    const double Pa = a0, Pb = b0;
-    // cell 2l-1 is lane-1's b, cell 2l+2 is lane+1's a
-    const double lf = wave_shift(Pb, 1), rt = wave_shift(Pa, 0);
-    const double Wa = w_edge ? cur.we : lf;
-    const double Ea = ib < ni ? Pb : cur.ee;
-    const double Wb = Pa;
-    const double Eb = e_edge ? cur.ee : rt;
+    const PlaneSet cur = load_set(k);
+    // cell x0-1 is lane-1's b1, cell x0+4 is lane+1's a0
+    const double lf = wave_shift(B.y, 1), rt = wave_shift(A.x, 0);
+    const double W_a0 = w_edge ? cur.we : lf;
+    const double E_a0 = x0 + 1 < ni ? B.x : cur.ee;
+    const double E_b0 = x0 + 2 < ni ? A.y : cur.ee;
+    const double E_a1 = x0 + 3 < ni ? B.y : cur.ee;
+    const double E_b1 = (x0 + 4 < ni && !e_edge) ? rt : cur.ee;
     const double ab = (BND && !pz && k == 0) ? 0.0 : cz;
     const double at = (BND && !pz && k == nk - 1) ? 0.0 : cz;
-    const double ap_a = -1.0 * (aw_a + ae_a + as + an + ab + at);
-    const double ap_b = -1.0 * (aw_b + ae_b + as + an + ab + at);
-    const double ra = cur.fa - (aw_a * Wa + ae_a * Ea + as * cur.sa + an * cur.na + ab * am1 +
-                                at * cur.ap1 + ap_a * Pa);
-    const double rb = cur.fb - (aw_b * Wb + ae_b * Eb + as * cur.sb + an * cur.nb + ab * bm1 +
-                                at * cur.bp1 + ap_b * Pb);
+    const double ap_a0 = -1.0 * (aw_a0 + ae_a0 + as + an + ab + at);
+    const double ap_b0 = -1.0 * (cx + ae_b0 + as + an + ab + at);
+    const double ap_a1 = -1.0 * (cx + ae_a1 + as + an + ab + at);
+    const double ap_b1 = -1.0 * (cx + ae_b1 + as + an + ab + at);
+    const double r_a0 = cur.fa.x - (aw_a0 * W_a0 + ae_a0 * E_a0 + as * cur.sa.x + an * cur.na.x +
+                                    ab * am1.x + at * cur.ap1.x + ap_a0 * A.x);
+    const double r_b0 = cur.fb.x - (cx * A.x + ae_b0 * E_b0 + as * cur.sb.x + an * cur.nb.x +
+                                    ab * bm1.x + at * cur.bp1.x + ap_b0 * B.x);
+    const double r_a1 = cur.fa.y - (cx * B.x + ae_a1 * E_a1 + as * cur.sa.y + an * cur.na.y +
+                                    ab * am1.y + at * cur.ap1.y + ap_a1 * A.y);
+    const double r_b1 = cur.fb.y - (cx * A.y + ae_b1 * E_b1 + as * cur.sb.y + an * cur.nb.y +
+                                    ab * bm1.y + at * cur.bp1.y + ap_b1 * B.y);
     const int ca = (jc + k) & 1, cb = ca ^ 1;
-    const unsigned q = ((unsigned)k * plane + o_c) / SZ;
-    if (ok) {
-      R[ca][q] = (T)ra;
-      ss += ra * ra;
-      am = fmax(am, fabs(ra));
+    const unsigned q = ((unsigned)k * plane + o_c) / SZ;  // even: a pair's first entry
+    if (ok_a1) {
+      t2 v;
+      v.x = (T)r_a0;
+      v.y = (T)r_a1;
+      *(gmem<t2> *)(R[ca] + q) = v;
+    } else if (ok_a0) {
+      R[ca][q] = (T)r_a0;
     }
-    if (b_ok) {
-      R[cb][q] = (T)rb;
-      ss += rb * rb;
-      am = fmax(am, fabs(rb));
+    if (ok_b1) {
+      t2 v;
+      v.x = (T)r_b0;
+      v.y = (T)r_b1;
+      *(gmem<t2> *)(R[cb] + q) = v;
+    } else if (ok_b0) {
+      R[cb][q] = (T)r_b0;
     }
-    am1 = a0;
-    bm1 = b0;
-    a0 = cur.ap1;
-    b0 = cur.bp1;
-    if (p + 1 < KB) cur = nxt;
+    if (ok_a0) {
+      ss += r_a0 * r_a0;
+      am = fmax(am, fabs(r_a0));
+    }
+    if (ok_b0) {
+      ss += r_b0 * r_b0;
+      am = fmax(am, fabs(r_b0));
+    }
+    if (ok_a1) {
+      ss += r_a1 * r_a1;
+      am = fmax(am, fabs(r_a1));
+    }
+    if (ok_b1) {
+      ss += r_b1 * r_b1;
+      am = fmax(am, fabs(r_b1));
+    }
+    am1 = A;
+    bm1 = B;
+    A = cur.ap1;
+    B = cur.bp1;
   }
 }
 
@@ -2212,10 +2265,12 @@ __global__ void __launch_bounds__(NTHREADS)
   if (active[F.blk]) {
     const int lpr = F.lpr;
     const int lane = (int)threadIdx.x;
-    const int l = t.x0 + (lane & (lpr - 1));
     const int j = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
+    // the tile's two halves of lpr entries, one entry per lane
+    for (int half = 0; half < 2; ++half)
     for (int k = t.z0; k < t.z1; ++k)
       for (int s = 0; s < 2; ++s) {
+        const int l = t.x0 + half * lpr + (lane & (lpr - 1));
         const int i = 2 * l + s;
         if (i >= F.ni || j >= F.nj) continue;
         const int c = (i + j + k) & 1;

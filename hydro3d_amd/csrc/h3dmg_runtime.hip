@@ -1097,9 +1097,11 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
   // of plane k+1 are in flight while plane k is computed), so depth costs
   // little occupancy and the plane window is re-read (kb+2)/kb times.
   h->tile_begin.assign(nblocks + 1, 0);
+  // a tile spans 2*lpr colour entries of TILE_Y*64/lpr rows (the residual
+  // gives each lane two entries of both colours)
   auto fine_xy = [&](const LevelDev &L, int &xs, int &ys) {
-    const int lpr = lanes_per_row(2 * L.H);
-    xs = lpr;
+    const int lpr = lanes_per_row(L.H);
+    xs = 2 * lpr;
     ys = TILE_Y * (64 / lpr);
     return ((L.H + xs - 1) / xs) * ((L.nj + ys - 1) / ys);
   };
@@ -1248,7 +1250,7 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
     F.H = L.H;
     F.per = L.per;
     for (int f = 0; f < 6; ++f) F.nbr[f] = L.nbr[f];  // level-1 descriptor d == block d
-    F.lpr = lanes_per_row(2 * L.H);
+    F.lpr = lanes_per_row(L.H);  // two colour entries per lane
     F.blk = b;
     F.cx = L.cx;
     F.cy = L.cy;
@@ -2456,7 +2458,8 @@ h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycl
     all_done = std::none_of(h->active.begin(), h->active.end(), [](int a) { return a != 0; });
   }
   int it = 0;
-  if (P.use_graph && !all_done && P.maxcy > 0 && !bad) {
+  // (a pooled handle allocates nothing: it keeps the host's loop)
+  if (P.use_graph && !h->pool && !all_done && P.maxcy > 0 && !bad) {
     st = solve_queued(h, hist, hist_cap, it, worst, all_done, bad);
     if (st) return st;
   }

@@ -168,7 +168,12 @@ h3dmg_status h3dmg_get_pp(h3dmg_handle *h, int b, double *pp, int ld1, int ld2, 
  * cycling) or maxcy.  hist (may be null) receives, per cycle k = 0..ncycles,
  * the largest relative residual over the blocks; ncycles and final_res may be
  * null.  Returns H3DMG_ERR_NOT_CONVERGED when maxcy is reached (p holds the
- * last iterate, as the reference goes on). */
+ * last iterate, as the reference goes on).  With use_graph (not pooled) the
+ * stopping rule runs on the device and the next cycle is queued before the
+ * host reads the last one's outcome: on return, one more cycle in which no
+ * block is active may still be running on the handle's stream (it changes
+ * nothing the API returns; later calls on the stream follow it).  The first
+ * such solve allocates a small pinned host buffer and device state. */
 h3dmg_status h3dmg_solve(h3dmg_handle *h, double *hist, int hist_cap, int *ncycles,
                          double *final_res);
 
