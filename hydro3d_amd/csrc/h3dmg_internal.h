@@ -38,7 +38,7 @@ struct LevelDev {
   int H;           // colour row pitch (elements), even, >= ceil(ni/2)
   int per;         // bit a set: axis a periodic (a = 0 x, 1 y, 2 z)
   int lpr;         // smoother lanes per row (power of two <= 64)
-  int lpr1;        // restriction lanes per coarse row (one fine entry per lane)
+  int lpr1;        // restriction lanes per coarse row segment (two coarse cells per lane)
   int blk;         // block index
   int finer;       // descriptor index of level g-1 (or -1)
   int coarser;     // descriptor index of level g+1 (or -1)

@@ -789,7 +789,194 @@ __global__ void __launch_bounds__(NTHREADS)
 // The coarse phi is not touched: the next relax of the coarse level runs with
 // zero = 1, which is the zeroing of mgsolver.for:655-662.
 //
-// Restriction tile of KBC coarse planes (2*KBC fine planes), software-
+// Restriction tile of an FP64 fine level, KBC coarse planes (2*KBC fine
+// planes): a rolling window
+// of fine planes k-1, k, k+1 of rows 2jc and 2jc+1.  A lane holds two coarse
+// cells l, l+1 of a coarse row: fine entries l, l+1 of both colours (cells
+// 2l .. 2l+3), each array one vector load.  Neighbours are addressed by the
+// coordinates of their image as 32-bit byte offsets from the fine level's
+// arena base, so tiles touching a y/z face (BND) run the same code; their
+// coefficients carry cell_coef's zeros towards walls.  Each coarse cell's
+// eight residuals arrive in the reference's summation order (a, b of row
+// 2jc, a, b of row 2jc+1; plane 2kc, then 2kc+1).  C.lpr1 lanes cover a
+// coarse row segment; the tile's `lev` is the coarse descriptor.
+template <class Tf, class Tc, int KBC, bool BND>
+__device__ __forceinline__ void restrict_tile_two(const Tile &t, const LevelDev &C,
+                                                  const LevelDev &F, int src_f, bool on) {
+  typedef typename Vec2<Tf>::type v2;
+  constexpr unsigned SZ = sizeof(Tf);
+  constexpr int NF = 2 * KBC;
+  const int lpr = C.lpr1;
+  const int lane = (int)threadIdx.x;
+  const int rl = lane & (lpr - 1);
+  const int l = t.x0 + 2 * rl;  // coarse cells l, l+1 = fine entries l, l+1
+  const int jc = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
+  const int ni = F.ni, nj = F.nj, nk = F.nk, H = F.H;
+  // on: the block's flag gates the stores
+  const bool ok0 = on && l < C.ni && jc < C.nj, ok1 = on && l + 1 < C.ni && jc < C.nj;
+  const bool px = F.per & 1, py = F.per & 2, pz = F.per & 4;
+  const unsigned row = (unsigned)H * SZ, plane = row * (unsigned)nj;  // bytes
+  const int j0 = 2 * min(jc, C.nj - 1);  // even: a has colour (r + k) & 1 in row j0 + r
+  // lanes past the row's last pair read that pair (nothing of theirs is stored)
+  const unsigned ent = (unsigned)min(l, H - 2) * SZ;
+  const char *base = (const char *)F.phi[0][0];
+  auto aoff = [&](const void *a) -> unsigned { return (unsigned)((const char *)a - base); };
+  const unsigned aP0 = aoff(F.phi[src_f][0]), dP = aoff(F.phi[src_f][1]) - aP0;
+  const unsigned aR0 = aoff(F.rhs[0]), dR = aoff(F.rhs[1]) - aR0;
+  const gmem<char> *Bs = (const gmem<char> *)base;
+  auto ld = [&](unsigned off) -> Tf { return *(const gmem<Tf> *)(Bs + off); };
+  auto ld2 = [&](unsigned off) -> v2 { return *(const gmem<v2> *)(Bs + off); };
+  // colour array (parity & 1) of phi and of rhs, computed (not selected)
+  auto pcol = [=](int par) -> unsigned { return aP0 + (unsigned)(par & 1) * dP; };
+  auto rcol = [=](int par) -> unsigned { return aR0 + (unsigned)(par & 1) * dR; };
+  const int x0 = 2 * l;  // fine cells x0 (a0), x0+1 (b0), x0+2 (a1), x0+3 (b1)
+  // rows j0-1 and j0+2, ghosts resolved to their image row
+  const int ys = BND ? image(j0 - 1, nj, py) : j0 - 1;
+  const int yn = BND ? image(j0 + 2, nj, py) : j0 + 2;
+  const unsigned o_r0 = (unsigned)j0 * row + ent, o_r1 = o_r0 + row;
+  const unsigned o_s = (unsigned)ys * row + ent, o_n = (unsigned)yn * row + ent;
+  // x neighbours no lane of the segment holds: fine cell x0-1 for the first
+  // lane (the west ghost's image or the previous tile's last entry); for the
+  // lane holding the last coarse cell (or lying past it) the east ghost's
+  // image, for the segment's last lane otherwise the next tile's first entry
+  const bool w_edge = rl == 0;
+  const bool e_last = l + 1 >= C.ni - 1;
+  const bool e_edge = e_last || rl == lpr - 1;
+  const int xw = image(x0 - 1, ni, px);
+  const int xe = e_last ? image(ni, ni, px) : x0 + 4;
+  const unsigned w_ent = (unsigned)(xw >> 1) * SZ, e_ent = (unsigned)(xe >> 1) * SZ;
+  const int w_par = xw & 1, e_par = xe & 1;
+  const int x1_par = w_edge ? w_par : e_par;
+  const unsigned x1_ent = w_edge ? w_ent : e_ent;
+  const Tf cx = (Tf)F.cx, cy = (Tf)F.cy, cz = (Tf)F.cz;
+  const Tf aw_a0 = (x0 == 0 && !px) ? Tf(0) : cx;
+  const Tf ae_a0 = (x0 == ni - 1 && !px) ? Tf(0) : cx;
+  const Tf ae_b0 = (x0 + 1 == ni - 1 && !px) ? Tf(0) : cx;
+  const Tf ae_a1 = (x0 + 2 == ni - 1 && !px) ? Tf(0) : cx;
+  const Tf ae_b1 = (x0 + 3 == ni - 1 && !px) ? Tf(0) : cx;
+  const Tf as0 = (BND && !py && j0 == 0) ? Tf(0) : cy;       // row j0 towards j0-1
+  const Tf an1 = (BND && !py && j0 + 1 == nj - 1) ? Tf(0) : cy;  // row j0+1 towards j0+2
+
+  struct PlaneSet {
+    v2 fa0, fb0, fa1, fb1;  // rhs of the a and b pairs in rows j0, j0+1
+    v2 sa, sb, na, nb;      // rows j0-1 and j0+2
+    v2 wa0, wb0, wa1, wb1;  // window plane k+1 of rows j0, j0+1
+    Tf we0, we1, ee0, ee1;  // x-edge values of rows j0, j0+1
+  };
+  auto next_plane = [&](int k) -> int { return BND ? image(k + 1, nk, pz) : k + 1; };
+  auto load_set = [&](int k) -> PlaneSet {
+    PlaneSet q;
+    const unsigned o = (unsigned)k * plane;
+    const int c0 = k & 1;  // colour of a in row j0 (j0 even)
+    q.fa0 = ld2(rcol(c0) + o + o_r0);
+    q.fb0 = ld2(rcol(c0 + 1) + o + o_r0);
+    q.fa1 = ld2(rcol(c0 + 1) + o + o_r1);
+    q.fb1 = ld2(rcol(c0) + o + o_r1);
+    q.sa = ld2(pcol(ys + k) + o + o_s);
+    q.sb = ld2(pcol(ys + k + 1) + o + o_s);
+    q.na = ld2(pcol(yn + k) + o + o_n);
+    q.nb = ld2(pcol(yn + k + 1) + o + o_n);
+    const int kp = next_plane(k);
+    const unsigned op = (unsigned)kp * plane;
+    q.wa0 = ld2(pcol(kp) + op + o_r0);
+    q.wb0 = ld2(pcol(kp + 1) + op + o_r0);
+    q.wa1 = ld2(pcol(kp + 1) + op + o_r1);
+    q.wb1 = ld2(pcol(kp) + op + o_r1);
+    // x-edge values: only the lanes at a row segment's ends load them
+    const unsigned r0 = (unsigned)j0 * row;
+    Tf x10 = Tf(0), x11 = Tf(0), x20 = Tf(0), x21 = Tf(0);
+    if (w_edge || e_edge) {
+      x10 = ld(pcol(x1_par + k) + o + r0 + x1_ent);
+      x11 = ld(pcol(x1_par + k + 1) + o + r0 + row + x1_ent);
+    }
+    if (w_edge && e_edge) {
+      x20 = ld(pcol(e_par + k) + o + r0 + e_ent);
+      x21 = ld(pcol(e_par + k + 1) + o + r0 + row + e_ent);
+    }
+    q.we0 = x10;
+    q.we1 = x11;
+    q.ee0 = w_edge ? x20 : x10;
+    q.ee1 = w_edge ? x21 : x11;
+    return q;
+  };
+  const int k0 = 2 * t.z0;
+  v2 ma0, mb0, ma1, mb1, ca0, cb0, ca1, cb1;  // window planes k-1 and k
+  {
+    const int km = BND ? image(k0 - 1, nk, pz) : k0 - 1;
+    const unsigned om = (unsigned)km * plane, oc = (unsigned)k0 * plane;
+    ma0 = ld2(pcol(km) + om + o_r0);
+    mb0 = ld2(pcol(km + 1) + om + o_r0);
+    ma1 = ld2(pcol(km + 1) + om + o_r1);
+    mb1 = ld2(pcol(km) + om + o_r1);
+    ca0 = ld2(pcol(k0) + oc + o_r0);
+    cb0 = ld2(pcol(k0 + 1) + oc + o_r0);
+    ca1 = ld2(pcol(k0 + 1) + oc + o_r1);
+    cb1 = ld2(pcol(k0) + oc + o_r1);
+  }
+  // the four cells of one fine row: A/B its a and b pairs, S/N the pairs of
+  // the rows below and above, M/W planes k-1 and k+1, F the rhs; adds the
+  // residuals of cells a0, b0 to acc0 and a1, b1 to acc1
+  auto row_cells = [&](const v2 &A, const v2 &B, const v2 &Sa, const v2 &Sb, const v2 &Na,
+                       const v2 &Nb, const v2 &Ma, const v2 &Mb, const v2 &Wa, const v2 &Wb,
+                       const v2 &Fa, const v2 &Fb, Tf we, Tf ee, Tf as, Tf an, Tf ab, Tf at,
+                       bool first, Tc &acc0, Tc &acc1) {
+    // cell x0-1 is lane-1's b1, cell x0+4 is lane+1's a0
+    const Tf lf = wave_shift(B.y, 1), rt = wave_shift(A.x, 0);
+    const Tf W_a0 = w_edge ? we : lf;
+    const Tf E_b0 = l + 1 < C.ni ? A.y : ee;
+    const Tf E_b1 = (l + 2 < C.ni && !e_edge) ? rt : ee;
+    const Tf ap_a0 = Tf(-1) * (aw_a0 + ae_a0 + as + an + ab + at);
+    const Tf ap_b0 = Tf(-1) * (cx + ae_b0 + as + an + ab + at);
+    const Tf ap_a1 = Tf(-1) * (cx + ae_a1 + as + an + ab + at);
+    const Tf ap_b1 = Tf(-1) * (cx + ae_b1 + as + an + ab + at);
+    const Tf r_a0 = Fa.x - (aw_a0 * W_a0 + ae_a0 * B.x + as * Sa.x + an * Na.x + ab * Ma.x +
+                            at * Wa.x + ap_a0 * A.x);
+    const Tf r_b0 = Fb.x - (cx * A.x + ae_b0 * E_b0 + as * Sb.x + an * Nb.x + ab * Mb.x +
+                            at * Wb.x + ap_b0 * B.x);
+    const Tf r_a1 = Fa.y - (cx * B.x + ae_a1 * B.y + as * Sa.y + an * Na.y + ab * Ma.y +
+                            at * Wa.y + ap_a1 * A.y);
+    const Tf r_b1 = Fb.y - (cx * A.y + ae_b1 * E_b1 + as * Sb.y + an * Nb.y + ab * Mb.y +
+                            at * Wb.y + ap_b1 * B.y);
+    acc0 = first ? (Tc)r_a0 : acc0 + (Tc)r_a0;
+    acc0 = acc0 + (Tc)r_b0;
+    acc1 = first ? (Tc)r_a1 : acc1 + (Tc)r_a1;
+    acc1 = acc1 + (Tc)r_b1;
+  };
+  Tc acc0 = Tc(0), acc1 = Tc(0);
+#pragma unroll
+  for (int p = 0; p < NF; ++p) {
+    const int k = k0 + p;
+    const PlaneSet cur = load_set(k);
+    const Tf ab = (BND && !pz && k == 0) ? Tf(0) : cz;
+    const Tf at = (BND && !pz && k == nk - 1) ? Tf(0) : cz;
+    // row j0: S from row j0-1, N = row j0+1 of the window
+    row_cells(ca0, cb0, cur.sa, cur.sb, ca1, cb1, ma0, mb0, cur.wa0, cur.wb0, cur.fa0, cur.fb0,
+              cur.we0, cur.ee0, as0, cy, ab, at, !(p & 1), acc0, acc1);
+    // row j0+1: S = row j0 of the window, N from row j0+2
+    row_cells(ca1, cb1, ca0, cb0, cur.na, cur.nb, ma1, mb1, cur.wa1, cur.wb1, cur.fa1, cur.fb1,
+              cur.we1, cur.ee1, cy, an1, ab, at, false, acc0, acc1);
+    if (p & 1) {
+      // coarse cells l, l+1 (l even): entry l/2 of the two colour arrays
+      const int kc = k >> 1;
+      const unsigned q =
+          ((unsigned)kc * (unsigned)C.nj + (unsigned)jc) * (unsigned)C.H + (unsigned)(l >> 1);
+      const int c0 = (l + jc + kc) & 1;
+      if (ok0) ((gmem<Tc> *)C.rhs[c0])[q] = Tc(0.125) * acc0;
+      if (ok1) ((gmem<Tc> *)C.rhs[c0 ^ 1])[q] = Tc(0.125) * acc1;
+    }
+    ma0 = ca0;
+    mb0 = cb0;
+    ma1 = ca1;
+    mb1 = cb1;
+    ca0 = cur.wa0;
+    cb0 = cur.wb0;
+    ca1 = cur.wa1;
+    cb1 = cur.wb1;
+  }
+}
+
+// Restriction tile of an FP32 fine level (mixed precision), KBC coarse planes
+// (2*KBC fine planes), software-
 // pipelined over the fine planes: a rolling window of fine planes k-1, k, k+1
 // of rows 2jc and 2jc+1, with the loads of plane k+1 (rhs of both rows, rows
 // 2jc-1 and 2jc+2, x-edge values and window plane k+2) issued before plane k
@@ -798,11 +985,12 @@ __global__ void __launch_bounds__(NTHREADS)
 // a y/z face (BND) run the same code; their coefficients carry cell_coef's
 // zeros towards walls.  The eight residuals arrive in the reference's
 // summation order (a, b of row 2jc, a, b of row 2jc+1; plane 2kc, then
-// 2kc+1).  C.lpr1 lanes cover a coarse row; the tile's `lev` is the coarse
-// descriptor.
+// 2kc+1).  One coarse cell per lane (FP32 fine entries: four-byte loads; two
+// cells per lane cost more occupancy than the loads they save here); C.lpr1
+// lanes cover a coarse row; the tile's `lev` is the coarse descriptor.
 template <class Tf, class Tc, int KBC, bool BND>
-__device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev &C,
-                                                   const LevelDev &F, int src_f, bool on) {
+__device__ __forceinline__ void restrict_tile_one(const Tile &t, const LevelDev &C,
+                                                  const LevelDev &F, int src_f, bool on) {
   constexpr unsigned SZ = sizeof(Tf);
   constexpr int NF = 2 * KBC;
   const int lpr = C.lpr1;
@@ -965,6 +1153,18 @@ __device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev
     cb1 = cur.wb1;
     if (p + 1 < NF) cur = nxt;
   }
+}
+
+// The restriction tile of the fine level's storage type: two coarse cells per
+// lane on an FP64 fine level (16-byte loads), one on an FP32 one; the tile
+// geometry follows (restriction tiles span 2*lpr1 or lpr1 coarse cells).
+template <class Tf, class Tc, int KBC, bool BND>
+__device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev &C,
+                                                   const LevelDev &F, int src_f, bool on) {
+  if constexpr (sizeof(Tf) == 8)
+    restrict_tile_two<Tf, Tc, KBC, BND>(t, C, F, src_f, on);
+  else
+    restrict_tile_one<Tf, Tc, KBC, BND>(t, C, F, src_f, on);
 }
 
 // One launch per coarse level: tiles of KBC coarse planes, kind 0 when every
@@ -2304,9 +2504,7 @@ __global__ void __launch_bounds__(NTHREADS)
   const LevelDev &F = levs[C.finer];
   const int lpr = C.lpr1;
   const int lane = (int)threadIdx.x;
-  const int l = t.x0 + (lane & (lpr - 1));
   const int jc = t.y0 + (int)threadIdx.y * (64 / lpr) + lane / lpr;
-  if (l >= C.ni || jc >= C.nj) return;
   auto res = [&](int i, int j, int k) -> double {
     const int c = (i + j + k) & 1;
     const size_t e = cidx(F.H, F.nj, i, j, k);
@@ -2318,6 +2516,10 @@ __global__ void __launch_bounds__(NTHREADS)
             cf[3] * nb(i, j + 1, k) + cf[4] * nb(i, j, k - 1) + cf[5] * nb(i, j, k + 1) +
             cf[6] * nb(i, j, k));
   };
+  // the tile's two halves of lpr coarse cells, one cell per lane
+  for (int half = 0; half < 2; ++half) {
+  const int l = t.x0 + half * lpr + (lane & (lpr - 1));
+  if (l >= C.ni || jc >= C.nj) continue;
   for (int kc = t.z0; kc < t.z1; ++kc) {
     const int i0 = 2 * l, j0 = 2 * jc, k0 = 2 * kc;
     double acc = res(i0, j0, k0);
@@ -2329,6 +2531,7 @@ __global__ void __launch_bounds__(NTHREADS)
     acc = acc + res(i0, j0 + 1, k0 + 1);
     acc = acc + res(i0 + 1, j0 + 1, k0 + 1);
     ((double *)C.rhs[(l + jc + kc) & 1])[cidx(C.H, C.nj, l, jc, kc)] = 0.125 * acc;
+  }
   }
 }
 

@@ -182,19 +182,20 @@ void add_tiles_march(TileSet &ts, int lev, const LevelDev &L, int kb, bool gener
   }
 }
 
-// Restriction tiles of coarse level descriptor `lev` (k_restrict): lpr1
-// coarse cells x 4*(64/lpr1) coarse rows x kb coarse planes; kind 0 when every
+// Restriction tiles of coarse level descriptor `lev` (k_restrict): cpl*lpr1
+// coarse cells (cpl per lane: 2 from an FP64 fine level, 1 from an FP32 one)
+// x 4*(64/lpr1) coarse rows x kb coarse planes; kind 0 when every
 // fine row and plane the tile reads is inside the block.  Deeper columns than
 // the smoother's (~2048 workgroups): a column re-reads two fine planes.
-int restrict_tiles_xy(const LevelDev &C) {
-  const int xs = C.lpr1, ys = TILE_Y * (64 / C.lpr1);
+int restrict_tiles_xy(const LevelDev &C, int cpl) {
+  const int xs = cpl * C.lpr1, ys = TILE_Y * (64 / C.lpr1);
   return ((C.ni + xs - 1) / xs) * ((C.nj + ys - 1) / ys);
 }
 
 // kb coarse planes per tile (kb divides C.nk); kind 3 for tiles touching a
 // y/z face
-void add_tiles_restrict(TileSet &ts, int lev, const LevelDev &C, int kb) {
-  const int xs = C.lpr1, ys = TILE_Y * (64 / C.lpr1);
+void add_tiles_restrict(TileSet &ts, int lev, const LevelDev &C, int kb, int cpl) {
+  const int xs = cpl * C.lpr1, ys = TILE_Y * (64 / C.lpr1);
   const int nx = C.ni, ny = C.nj, nz = C.nk;
   for (int z = 0; z < nz; z += kb) {
     const int z1 = z + kb;
@@ -208,11 +209,11 @@ void add_tiles_restrict(TileSet &ts, int lev, const LevelDev &C, int kb) {
 
 // One depth for a restriction launch over the coarse descriptors `lev`:
 // ~2048 workgroups, at most 4 coarse planes.
-int restrict_depth(const std::vector<LevelDev> &levs, const std::vector<int> &lev) {
+int restrict_depth(const std::vector<LevelDev> &levs, const std::vector<int> &lev, int cpl) {
   long cols = 0;
   std::vector<int> nks;
   for (int d : lev) {
-    cols += (long)restrict_tiles_xy(levs[d]) * levs[d].nk;
+    cols += (long)restrict_tiles_xy(levs[d], cpl) * levs[d].nk;
     nks.push_back(levs[d].nk);
   }
   return launch_depth(cols, 2048, 0, 4, nks);
@@ -936,7 +937,8 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
       L.nk = B.spec.nk >> (g - 1);
       L.H = (((L.ni + 1) / 2) + 1) & ~1;  // even: aligned 2-entry accesses
       L.lpr = lanes_per_row(L.H);
-      L.lpr1 = lanes_per_row(2 * L.ni);
+      // restriction lanes: two coarse cells per lane from an FP64 fine level
+      L.lpr1 = lanes_per_row(is_mixed(h) ? 2 * L.ni : L.ni);
       L.per = B.per;
       L.blk = b;
       const double s = (double)(1 << (g - 1));
@@ -1085,8 +1087,9 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
         for (int d : members) add_tiles(ts, d, h->levs[d].ni, h->levs[d].nj, h->levs[d].nk);
         continue;
       }
-      ts.depth = restrict_depth(h->levs, members);
-      for (int d : members) add_tiles_restrict(ts, d, h->levs[d], ts.depth);
+      const int cpl = is_mixed(h) ? 1 : 2;
+      ts.depth = restrict_depth(h->levs, members, cpl);
+      for (int d : members) add_tiles_restrict(ts, d, h->levs[d], ts.depth, cpl);
     }
   // finest level: residual tiles (one colour-array entry of both colours per
   // lane, lpr lanes per row, kb planes; kind 0 = y/z interior) and p += pp
