@@ -1269,29 +1269,35 @@ __global__ void __launch_bounds__(NTHREADS)
                  "s"(F.H), "s"(F.phi[0][0]), "s"(F.phi[0][1]), "s"(F.phi[1][0]), "s"(F.phi[1][1]),
                  "s"(C.ni), "s"(C.nj), "s"(C.nk), "s"(C.H), "s"(C.per), "s"(C.phi[0][0]),
                  "s"(C.phi[0][1]), "s"(C.phi[1][0]), "s"(C.phi[1][1]));
-  const int l = t.x0 + (int)threadIdx.x;
+  // a thread holds fine entries l, l+1 (l even): cells 2l .. 2l+3 of the
+  // 2x2 rows and planes around coarse row pair m, plane pair n
+  typedef typename Vec2<Tf>::type v2;
+  const int l = t.x0 + 2 * (int)threadIdx.x;
   const int m = t.y0 + (int)threadIdx.y;
   const int n = t.z0;
   if (2 * l >= F.ni || m > C.nj || n > C.nk) return;
+  const bool two = 2 * l + 2 < F.ni;  // entry l+1 holds cells (F.ni is even)
   const bool px = C.per & 1, py = C.per & 2, pz = C.per & 4;
   const gmem<Tc> *C0 = (const gmem<Tc> *)C.phi[src_c][0];
   const gmem<Tc> *C1 = (const gmem<Tc> *)C.phi[src_c][1];
-  // coarse corner coordinates (interior numbering, ghosts resolved)
-  int xc[3], yc[2], zc[2];
-  for (int a = 0; a < 3; ++a) xc[a] = image(l - 1 + a, C.ni, px);
+  // coarse corner coordinates x = l-1 .. l+2 (interior numbering, ghosts
+  // resolved; the last is read only when entry l+1 holds cells)
+  int xc[4], yc[2], zc[2];
+  for (int a = 0; a < 4; ++a) xc[a] = image(min(l - 1 + a, two ? l + 2 : l + 1), C.ni, px);
   for (int a = 0; a < 2; ++a) {
     yc[a] = image(m - 1 + a, C.nj, py);
     zc[a] = image(n - 1 + a, C.nk, pz);
   }
-  Tc cv[3][2][2];
+  Tc cv[4][2][2];
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         if (GEN) {
-          cv[a][b][c] = level_value<Tc>(levs, F.coarser, src_c, l - 1 + a, m - 1 + b, n - 1 + c);
+          cv[a][b][c] = level_value<Tc>(levs, F.coarser, src_c, min(l - 1 + a, two ? l + 2 : l + 1),
+                                        m - 1 + b, n - 1 + c);
         } else {
           const unsigned off = ((unsigned)zc[c] * (unsigned)C.nj + (unsigned)yc[b]) *
                                    (unsigned)C.H + (unsigned)(xc[a] >> 1);
@@ -1309,22 +1315,38 @@ __global__ void __launch_bounds__(NTHREADS)
       const int j = 2 * m - 1 + dj;
       if (j < 0 || j >= F.nj) continue;
       const unsigned fq = ((unsigned)k * (unsigned)F.nj + (unsigned)j) * (unsigned)F.H + (unsigned)l;
+      // the correction of cell 2(l+ee)+e: corners x = l+ee-1+e, l+ee+e with
+      // di = 1-e, summed in the corner order c, c+j, c+k, c+k+j, c+i, ...
+      Tc cr[2][2];  // [e][ee]
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int ee = 0; ee < 2; ++ee) {
+          const int di = 1 - e;
+          Tc s;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int oi = q >> 2, oj = q & 1, ok = (q >> 1) & 1;
+            const Tc w = (oi == di ? Tc(3) : Tc(1)) * (oj == dj ? Tc(3) : Tc(1)) *
+                         (ok == dk ? Tc(3) : Tc(1));
+            const Tc v = cv[ee + e + oi][oj][ok];
+            s = (q == 0) ? w * v : s + w * v;
+          }
+          cr[e][ee] = s;
+        }
+      // cells 2l+e and 2l+2+e share a colour array: entries l, l+1
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        // cell 2l (e = 0): corners x = l-1, l and di = 1; cell 2l+1: x = l, l+1, di = 0
-        const int di = 1 - e;
-        Tc s;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int oi = q >> 2, oj = q & 1, ok = (q >> 1) & 1;  // c, c+j, c+k, c+k+j, c+i, ...
-          const Tc w = (oi == di ? Tc(3) : Tc(1)) * (oj == dj ? Tc(3) : Tc(1)) *
-                       (ok == dk ? Tc(3) : Tc(1));
-          const Tc v = cv[e + oi][oj][ok];
-          s = (q == 0) ? w * v : s + w * v;
-        }
         gmem<Tf> *fa = ((2 * l + e + j + k) & 1) ? F1 : F0;
-        const Tf nv = fa[fq] + (Tf)(s / Tc(64));
-        if (on) fa[fq] = nv;
+        if (two) {
+          v2 v = *(gmem<v2> *)(fa + fq);
+          v.x = v.x + (Tf)(cr[e][0] / Tc(64));
+          v.y = v.y + (Tf)(cr[e][1] / Tc(64));
+          if (on) *(gmem<v2> *)(fa + fq) = v;
+        } else {
+          const Tf nv = fa[fq] + (Tf)(cr[e][0] / Tc(64));
+          if (on) fa[fq] = nv;
+        }
       }
     }
   }

@@ -220,13 +220,14 @@ int restrict_depth(const std::vector<LevelDev> &levs, const std::vector<int> &le
 }
 
 // Prolongation tiles of fine level descriptor `lev` (k_prolong): fine entries
-// x0.. (64 per tile) x coarse row pairs m = y0.. (TILE_Y per tile) x one
-// coarse plane pair n = z0, with m in [0, nj/2] and n in [0, nk/2].
+// x0.. (two per thread, 2*TILE_X per tile) x coarse row pairs m = y0..
+// (TILE_Y per tile) x one coarse plane pair n = z0, with m in [0, nj/2] and n
+// in [0, nk/2].
 void add_tiles_prolong(TileSet &ts, int lev, const LevelDev &F) {
   const int ncj = F.nj / 2, nck = F.nk / 2, nfx = F.ni / 2;
   for (int n = 0; n <= nck; ++n)
     for (int m = 0; m <= ncj; m += TILE_Y)
-      for (int x = 0; x < nfx; x += TILE_X) ts.h.push_back(Tile{lev, x, m, n, n + 1, 0});
+      for (int x = 0; x < nfx; x += 2 * TILE_X) ts.h.push_back(Tile{lev, x, m, n, n + 1, 0});
 }
 
 struct BlockInfo {
