@@ -114,10 +114,13 @@ void launch_rb_fused(hipStream_t s, const Tile *tiles, int ntiles, const LevelDe
 // restriction into the coarse levels of a tile list whose tiles are all
 // `depth` (1, 2 or 4) coarse planes deep
 // (depth 0: natural tiles of the coarse level and the general kernel, for
-// blocks with neighbours)
+// blocks with neighbours).  zbuf >= 0 (depth > 0 only): the coarse level is
+// relaxed next from zero; the restriction also writes the red half of that
+// first sweep (each red cell rhs/ap, its neighbours being zero) into buffer
+// zbuf, and the sweep starts with its black half.
 template <class Tf, class Tc>
 void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
-                     const int *active, int src_f, OneDesc one = OneDesc{});
+                     const int *active, int src_f, OneDesc one = OneDesc{}, int zbuf = -1);
 // gen != 0: coarse ghosts read through the neighbour table (blocks with
 // neighbours)
 template <class Tf, class Tc>

@@ -789,6 +789,18 @@ __global__ void __launch_bounds__(NTHREADS)
 // The coarse phi is not touched: the next relax of the coarse level runs with
 // zero = 1, which is the zeroing of mgsolver.for:655-662.
 //
+// The first half-sweep of a relaxation from zero, for a red coarse cell: every
+// neighbour is zero, so the sweep's sum is exactly zero and the cell becomes
+// div_ap(rhs - 0, ap, 1/ap) with cell_coef's ap, which the restriction writes
+// with the rhs (zbuf >= 0: into buffer zbuf), in the sweep's arithmetic.
+template <class Tc>
+__device__ __forceinline__ void zero_state_red(const LevelDev &C, int zbuf, int ic, int jc, int kc,
+                                               unsigned q, Tc rhs) {
+  if (zbuf < 0 || ((ic + jc + kc) & 1)) return;
+  const Coef<Tc> c = cell_coef<Tc>(C, ic, jc, kc);
+  ((gmem<Tc> *)C.phi[zbuf][0])[q] = div_ap(rhs - Tc(0), c.ap, Tc(1) / c.ap);
+}
+
 // Restriction tile of an FP64 fine level, KBC coarse planes (2*KBC fine
 // planes): a rolling window
 // of fine planes k-1, k, k+1 of rows 2jc and 2jc+1.  A lane holds two coarse
@@ -802,7 +814,8 @@ __global__ void __launch_bounds__(NTHREADS)
 // coarse row segment; the tile's `lev` is the coarse descriptor.
 template <class Tf, class Tc, int KBC, bool BND>
 __device__ __forceinline__ void restrict_tile_two(const Tile &t, const LevelDev &C,
-                                                  const LevelDev &F, int src_f, bool on) {
+                                                  const LevelDev &F, int src_f, bool on,
+                                                  int zbuf) {
   typedef typename Vec2<Tf>::type v2;
   constexpr unsigned SZ = sizeof(Tf);
   constexpr int NF = 2 * KBC;
@@ -961,8 +974,15 @@ __device__ __forceinline__ void restrict_tile_two(const Tile &t, const LevelDev 
       const unsigned q =
           ((unsigned)kc * (unsigned)C.nj + (unsigned)jc) * (unsigned)C.H + (unsigned)(l >> 1);
       const int c0 = (l + jc + kc) & 1;
-      if (ok0) ((gmem<Tc> *)C.rhs[c0])[q] = Tc(0.125) * acc0;
-      if (ok1) ((gmem<Tc> *)C.rhs[c0 ^ 1])[q] = Tc(0.125) * acc1;
+      const Tc v0 = Tc(0.125) * acc0, v1 = Tc(0.125) * acc1;
+      if (ok0) {
+        ((gmem<Tc> *)C.rhs[c0])[q] = v0;
+        zero_state_red<Tc>(C, zbuf, l, jc, kc, q, v0);
+      }
+      if (ok1) {
+        ((gmem<Tc> *)C.rhs[c0 ^ 1])[q] = v1;
+        zero_state_red<Tc>(C, zbuf, l + 1, jc, kc, q, v1);
+      }
     }
     ma0 = ca0;
     mb0 = cb0;
@@ -990,7 +1010,8 @@ __device__ __forceinline__ void restrict_tile_two(const Tile &t, const LevelDev 
 // lanes cover a coarse row; the tile's `lev` is the coarse descriptor.
 template <class Tf, class Tc, int KBC, bool BND>
 __device__ __forceinline__ void restrict_tile_one(const Tile &t, const LevelDev &C,
-                                                  const LevelDev &F, int src_f, bool on) {
+                                                  const LevelDev &F, int src_f, bool on,
+                                                  int zbuf) {
   constexpr unsigned SZ = sizeof(Tf);
   constexpr int NF = 2 * KBC;
   const int lpr = C.lpr1;
@@ -1139,8 +1160,11 @@ __device__ __forceinline__ void restrict_tile_one(const Tile &t, const LevelDev 
       const int kc = k >> 1;
       if (ok) {
         gmem<Tc> *D = (gmem<Tc> *)C.rhs[(l + jc + kc) & 1];
-        D[((unsigned)kc * (unsigned)C.nj + (unsigned)jc) * (unsigned)C.H + (unsigned)(l >> 1)] =
-            Tc(0.125) * acc;
+        const unsigned q =
+            ((unsigned)kc * (unsigned)C.nj + (unsigned)jc) * (unsigned)C.H + (unsigned)(l >> 1);
+        const Tc v = Tc(0.125) * acc;
+        D[q] = v;
+        zero_state_red<Tc>(C, zbuf, l, jc, kc, q, v);
       }
     }
     ma0 = ca0;
@@ -1160,11 +1184,12 @@ __device__ __forceinline__ void restrict_tile_one(const Tile &t, const LevelDev 
 // geometry follows (restriction tiles span 2*lpr1 or lpr1 coarse cells).
 template <class Tf, class Tc, int KBC, bool BND>
 __device__ __forceinline__ void restrict_tile_fast(const Tile &t, const LevelDev &C,
-                                                   const LevelDev &F, int src_f, bool on) {
+                                                   const LevelDev &F, int src_f, bool on,
+                                                   int zbuf) {
   if constexpr (sizeof(Tf) == 8)
-    restrict_tile_two<Tf, Tc, KBC, BND>(t, C, F, src_f, on);
+    restrict_tile_two<Tf, Tc, KBC, BND>(t, C, F, src_f, on, zbuf);
   else
-    restrict_tile_one<Tf, Tc, KBC, BND>(t, C, F, src_f, on);
+    restrict_tile_one<Tf, Tc, KBC, BND>(t, C, F, src_f, on, zbuf);
 }
 
 // One launch per coarse level: tiles of KBC coarse planes, kind 0 when every
@@ -1175,7 +1200,7 @@ template <class Tf, class Tc, int KBC, bool ONE>
 __global__ void __launch_bounds__(NTHREADS)
     k_restrict(const Tile *__restrict__ tiles, const LevelDev *__restrict__ levs,
                const int *__restrict__ active, int src_f, int one_lev, int one_lev2, int one_blk,
-               int ntiles) {
+               int ntiles, int zbuf) {
   const Tile t = tiles[xcd_remap((int)blockIdx.x, ntiles)];
   if (ONE) {
     const LevelDev &C = levs[one_lev], &F = levs[one_lev2];
@@ -1186,18 +1211,18 @@ __global__ void __launch_bounds__(NTHREADS)
                  "s"(F.phi[0][0]), "s"(F.phi[0][1]), "s"(F.phi[1][0]), "s"(F.phi[1][1]),
                  "s"(F.rhs[0]), "s"(F.rhs[1]));
     if (t.kind == 0)
-      restrict_tile_fast<Tf, Tc, KBC, false>(t, C, F, src_f, on != 0);
+      restrict_tile_fast<Tf, Tc, KBC, false>(t, C, F, src_f, on != 0, zbuf);
     else
-      restrict_tile_fast<Tf, Tc, KBC, true>(t, C, F, src_f, on != 0);
+      restrict_tile_fast<Tf, Tc, KBC, true>(t, C, F, src_f, on != 0, zbuf);
     return;
   }
   const LevelDev &C = levs[t.lev];
   if (!active[C.blk]) return;
   const LevelDev &F = levs[C.finer];
   if (t.kind == 0)
-    restrict_tile_fast<Tf, Tc, KBC, false>(t, C, F, src_f, true);
+    restrict_tile_fast<Tf, Tc, KBC, false>(t, C, F, src_f, true, zbuf);
   else
-    restrict_tile_fast<Tf, Tc, KBC, true>(t, C, F, src_f, true);
+    restrict_tile_fast<Tf, Tc, KBC, true>(t, C, F, src_f, true, zbuf);
 }
 
 // Restriction for blocks with neighbours: one thread per coarse cell of
@@ -3779,7 +3804,7 @@ void launch_rb_fused(hipStream_t s, const Tile *tiles, int ntiles, const LevelDe
 
 template <class Tf, class Tc>
 void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, const LevelDev *levs,
-                     const int *active, int src_f, OneDesc one) {
+                     const int *active, int src_f, OneDesc one, int zbuf) {
   if (ntiles <= 0) return;
   if (depth == 0) {  // natural tiles of the coarse level, blocks with neighbours
     hipLaunchKernelGGL((k_restrict_gen<Tf, Tc>), dim3(ntiles), tile_block(), 0, s, tiles, levs,
@@ -3793,7 +3818,7 @@ void launch_restrict(hipStream_t s, const Tile *tiles, int ntiles, int depth, co
                            : depth == 2 ? k_restrict<Tf, Tc, 2, false>
                                         : k_restrict<Tf, Tc, 1, false>);
   hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, levs, active, src_f, one.lev,
-                     one.lev2, one.blk, ntiles);
+                     one.lev2, one.blk, ntiles, zbuf);
 }
 
 template <class Tf, class Tc>
@@ -3993,11 +4018,11 @@ template void launch_rb_sweep<double>(hipStream_t, const Tile *, int, int, const
 template void launch_rb_sweep<float>(hipStream_t, const Tile *, int, int, const LevelDev *,
                                      const int *, int, int, int, OneDesc);
 template void launch_restrict<double, double>(hipStream_t, const Tile *, int, int, const LevelDev *,
-                                              const int *, int, OneDesc);
+                                              const int *, int, OneDesc, int);
 template void launch_restrict<float, float>(hipStream_t, const Tile *, int, int, const LevelDev *,
-                                            const int *, int, OneDesc);
+                                            const int *, int, OneDesc, int);
 template void launch_restrict<float, double>(hipStream_t, const Tile *, int, int, const LevelDev *,
-                                             const int *, int, OneDesc);
+                                             const int *, int, OneDesc, int);
 template void launch_prolong<double, double>(hipStream_t, const Tile *, int, const LevelDev *,
                                              const int *, int, int, int, OneDesc);
 template void launch_prolong<float, float>(hipStream_t, const Tile *, int, const LevelDev *,

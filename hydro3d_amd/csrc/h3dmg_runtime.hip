@@ -479,8 +479,11 @@ void dev_free(h3dmg_handle *h, void *p) {
     (void)hipFree(p);
 }
 
+// red_done: the red half of the first sweep (from zero) was written by the
+// restriction into this level (launch_restrict's zbuf)
 template <class T>
-void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool &zero) {
+void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool &zero,
+                 bool red_done = false) {
   for (int s = 0; s < nsweep; ++s) {
     if (h->prm.smoother == H3DMG_SMOOTH_GS)  // tile wavefront (every level FP64)
       for (int d = 0; d + 1 < (int)ts.diag_begin.size(); ++d)
@@ -502,7 +505,7 @@ void relax_group(h3dmg_handle *h, const TileSet &ts, int nsweep, int &buf, bool 
     else if (ts.fused)
       launch_rb_fused<T>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, buf, zero ? 1 : 0);
     else
-      for (int colour = 0; colour < 2; ++colour)
+      for (int colour = (s == 0 && red_done) ? 1 : 0; colour < 2; ++colour)
         launch_rb_sweep<T>(h->stream, ts.d, ts.n(), ts.depth, h->d_levs, h->d_active, colour,
                            buf, zero ? 1 : 0, ts.one);
     buf ^= 1;
@@ -517,7 +520,8 @@ h3dmg_status gen_relax(h3dmg_handle *h, int g);
 h3dmg_status upload_tail(h3dmg_handle *h, bool sync = true);
 h3dmg_status gen_tail(h3dmg_handle *h);
 
-void relax_any(h3dmg_handle *h, bool is64, const TileSet &ts, int nsweep, int &buf, bool &zero) {
+void relax_any(h3dmg_handle *h, bool is64, const TileSet &ts, int nsweep, int &buf, bool &zero,
+               bool red_done = false) {
   if (ts.n() == 0) {
     // keep the bookkeeping identical for empty groups
     if (nsweep > 0) {
@@ -527,9 +531,9 @@ void relax_any(h3dmg_handle *h, bool is64, const TileSet &ts, int nsweep, int &b
     return;
   }
   if (is64)
-    relax_group<double>(h, ts, nsweep, buf, zero);
+    relax_group<double>(h, ts, nsweep, buf, zero, red_done);
   else
-    relax_group<float>(h, ts, nsweep, buf, zero);
+    relax_group<float>(h, ts, nsweep, buf, zero, red_done);
 }
 
 void fill_any(h3dmg_handle *h, bool is64, const TileSet &ts, int buf) {
@@ -557,6 +561,12 @@ void enqueue_vcycle(h3dmg_handle *h) {
   const int Lt = h->tail_tiles.n() ? h->tail_lt : Lm + 1;
   const int gdown = std::min(Lm, Lt);
   for (int g = 1; g <= gdown; ++g) {
+    // the restriction into g writes the red half of g's first pre-smoothing
+    // sweep (from zero) when that sweep is a split red-black one
+    const bool red_by_restr = g >= 2 && g < Lt && g < Lm && h->prm.nu1 > 0 &&
+                              h->prm.smoother == H3DMG_SMOOTH_RBGS && !h->lsm && !h->connected &&
+                              h->restr_nc[g].n() && h->restr_nc[g].depth > 0 &&
+                              h->relax_nc[g].n() && !h->relax_nc[g].fused && z_nc[g];
     if (g >= 2) {
       // mgrestr(g): residual of level g-1 (non-coarsest there) into rhs_g
       const bool f64 = nc_is64(g - 1);
@@ -568,14 +578,15 @@ void enqueue_vcycle(h3dmg_handle *h) {
         for (const TileSet *ts : {&h->restr_nc[g], &h->restr_c[g]})
           launch_lsm_restrict(h->stream, ts->d, ts->n(), h->d_levs, h->d_active, buf_nc[g - 1]);
       } else if (h->restr_nc[g].n()) {
+        const int zbuf = red_by_restr ? 1 - buf_nc[g] : -1;
         if (f64)
           launch_restrict<double, double>(h->stream, h->restr_nc[g].d, h->restr_nc[g].n(),
                                           h->restr_nc[g].depth, h->d_levs, h->d_active,
-                                          buf_nc[g - 1], h->restr_nc[g].one);
+                                          buf_nc[g - 1], h->restr_nc[g].one, zbuf);
         else
           launch_restrict<float, float>(h->stream, h->restr_nc[g].d, h->restr_nc[g].n(),
                                         h->restr_nc[g].depth, h->d_levs, h->d_active,
-                                        buf_nc[g - 1], h->restr_nc[g].one);
+                                        buf_nc[g - 1], h->restr_nc[g].one, zbuf);
       }
       if (!h->lsm && h->restr_c[g].n()) {
         if (f64)
@@ -621,7 +632,7 @@ void enqueue_vcycle(h3dmg_handle *h) {
     // pre-smoothing of the blocks that go on coarsening
     if (g < Lm && h->relax_nc[g].n()) {
       bool z = z_nc[g];
-      relax_any(h, nc_is64(g), h->relax_nc[g], nu1, buf_nc[g], z);
+      relax_any(h, nc_is64(g), h->relax_nc[g], nu1, buf_nc[g], z, red_by_restr);
       z_nc[g] = z;
     }
   }
