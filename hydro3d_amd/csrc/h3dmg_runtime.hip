@@ -567,6 +567,12 @@ void enqueue_vcycle(h3dmg_handle *h) {
                               h->prm.smoother == H3DMG_SMOOTH_RBGS && !h->lsm && !h->connected &&
                               h->restr_nc[g].n() && h->restr_nc[g].depth > 0 &&
                               h->relax_nc[g].n() && !h->relax_nc[g].fused && z_nc[g];
+    // likewise for the blocks whose coarsest level g is smoothed (not solved)
+    const bool red_c_by_restr = g >= 2 && g < Lt && !(exact && g > 1) && h->prm.nu1 > 0 &&
+                                h->prm.smoother == H3DMG_SMOOTH_RBGS && !h->lsm &&
+                                !h->connected && h->restr_c[g].n() &&
+                                h->restr_c[g].depth > 0 && h->relax_c[g].n() &&
+                                !h->relax_c[g].fused;
     if (g >= 2) {
       // mgrestr(g): residual of level g-1 (non-coarsest there) into rhs_g
       const bool f64 = nc_is64(g - 1);
@@ -589,14 +595,15 @@ void enqueue_vcycle(h3dmg_handle *h) {
                                         buf_nc[g - 1], h->restr_nc[g].one, zbuf);
       }
       if (!h->lsm && h->restr_c[g].n()) {
+        const int zbuf = red_c_by_restr ? 1 : -1;  // the coarsest level's sweeps start in buffer 0
         if (f64)
           launch_restrict<double, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(),
                                           h->restr_c[g].depth, h->d_levs, h->d_active,
-                                          buf_nc[g - 1], h->restr_c[g].one);
+                                          buf_nc[g - 1], h->restr_c[g].one, zbuf);
         else
           launch_restrict<float, double>(h->stream, h->restr_c[g].d, h->restr_c[g].n(),
                                          h->restr_c[g].depth, h->d_levs, h->d_active,
-                                         buf_nc[g - 1], h->restr_c[g].one);
+                                         buf_nc[g - 1], h->restr_c[g].one, zbuf);
       }
     }
     if (g == Lt) {
@@ -619,7 +626,7 @@ void enqueue_vcycle(h3dmg_handle *h) {
         b = out;
         z = false;
       } else {
-        relax_any(h, true, h->relax_c[g], nu1, b, z);
+        relax_any(h, true, h->relax_c[g], nu1, b, z, red_c_by_restr);
         relax_any(h, true, h->relax_c[g], nu2, b, z);
         if (z) {  // nu1 = nu2 = 0
           fill_any(h, true, h->nat_c[g], b);
