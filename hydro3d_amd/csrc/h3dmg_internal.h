@@ -134,9 +134,12 @@ void launch_fill_zero(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
 // Tile::lev is the BLOCK index b; fine[b] and levs[b] (level-1 descriptors
 // come first) describe it.  use_p = 0 gives r = f (the norm of the
 // right-hand side).
+// zbuf >= 0 (the fast path, depth > 0 and use_p): also the red half of the
+// level-1 relaxation from zero (each red cell rhs/ap) into buffer zbuf
 template <class T>
 void launch_residual(hipStream_t s, const Tile *tiles, int ntiles, int depth, const FineDev *fine,
-                     const LevelDev *levs, const int *active, RedPartial *partials, int use_p);
+                     const LevelDev *levs, const int *active, RedPartial *partials, int use_p,
+                     int zbuf = -1);
 template <class T>
 void launch_update(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *fine,
                    const LevelDev *levs, const int *active, int buf);

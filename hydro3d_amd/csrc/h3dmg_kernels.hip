@@ -1569,7 +1569,8 @@ __device__ __forceinline__ void residual_tile(const Tile &t, const FineDev *__re
 // every cell is residual_tile's.
 template <class T, int KB, bool BND>
 __device__ __forceinline__ void residual_tile_fast(const Tile &t, const FineDev &F,
-                                                   const LevelDev &L, double &ss, double &am) {
+                                                   const LevelDev &L, double &ss, double &am,
+                                                   int zbuf) {
   typedef typename Vec2<double>::type d2;
   typedef typename Vec2<T>::type t2;
   constexpr unsigned SZ = sizeof(double);
@@ -1694,6 +1695,26 @@ __device__ __forceinline__ void residual_tile_fast(const Tile &t, const FineDev 
                                     ab * bm1.y + at * cur.bp1.y + ap_b1 * B.y);
     const int ca = (jc + k) & 1, cb = ca ^ 1;
     const unsigned q = ((unsigned)k * plane + o_c) / SZ;  // even: a pair's first entry
+    // (FP64 level 1 only: the host asks for it there, and the FP32 kernel
+    // stays free of it)
+    if (sizeof(T) == 8 && zbuf >= 0) {
+      // the level-1 relaxation from zero starts with the red cells: rhs/ap
+      // (zero_state_red's arithmetic) into buffer zbuf; the red pair is a0,
+      // a1 or b0, b1
+      const bool red_a = ca == 0;
+      const T r0 = (T)(red_a ? r_a0 : r_b0), r1 = (T)(red_a ? r_a1 : r_b1);
+      const int xr = x0 + (red_a ? 0 : 1);
+      const bool w0 = red_a ? ok_a0 : ok_b0, w1 = red_a ? ok_a1 : ok_b1;
+      const Coef<T> c0 = cell_coef<T>(L, xr, jc, k), c1 = cell_coef<T>(L, xr + 2, jc, k);
+      t2 v;
+      v.x = div_ap(r0 - T(0), c0.ap, T(1) / c0.ap);
+      v.y = div_ap(r1 - T(0), c1.ap, T(1) / c1.ap);
+      gmem<T> *P = (gmem<T> *)L.phi[zbuf][0];
+      if (w1)
+        *(gmem<t2> *)(P + q) = v;
+      else if (w0)
+        P[q] = v.x;
+    }
     if (ok_a1) {
       t2 v;
       v.x = (T)r_a0;
@@ -1742,7 +1763,7 @@ template <class T, int KB>
 __global__ void __launch_bounds__(NTHREADS)
     k_residual(const Tile *__restrict__ tiles, const FineDev *__restrict__ fine,
                const LevelDev *__restrict__ levs, const int *__restrict__ active,
-               RedPartial *__restrict__ partials) {
+               RedPartial *__restrict__ partials, int zbuf) {
   // XCD-contiguous tile ranges: rows j-1, j+1 of a tile are in the same L2
   const int ti = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const Tile t = tiles[ti];
@@ -1751,9 +1772,9 @@ __global__ void __launch_bounds__(NTHREADS)
   double ss = 0.0, am = 0.0;
   if (active[F.blk]) {
     if (t.kind == 0)
-      residual_tile_fast<T, KB, false>(t, F, L, ss, am);
+      residual_tile_fast<T, KB, false>(t, F, L, ss, am, zbuf);
     else
-      residual_tile_fast<T, KB, true>(t, F, L, ss, am);
+      residual_tile_fast<T, KB, true>(t, F, L, ss, am, zbuf);
   }
   block_reduce(ss, am);
   if (threadIdx.x == 0 && threadIdx.y == 0) partials[ti] = RedPartial{ss, am};
@@ -3845,7 +3866,8 @@ void launch_fill_zero(hipStream_t s, const Tile *tiles, int ntiles, const LevelD
 
 template <class T>
 void launch_residual(hipStream_t s, const Tile *tiles, int ntiles, int depth, const FineDev *fine,
-                     const LevelDev *levs, const int *active, RedPartial *partials, int use_p) {
+                     const LevelDev *levs, const int *active, RedPartial *partials, int use_p,
+                     int zbuf) {
   if (ntiles <= 0) return;
   if (!use_p || depth <= 0) {
     hipLaunchKernelGGL(k_residual_gen<T>, dim3(ntiles), tile_block(), 0, s, tiles, fine, levs,
@@ -3856,7 +3878,8 @@ void launch_residual(hipStream_t s, const Tile *tiles, int ntiles, int depth, co
            : depth == 4 ? k_residual<T, 4>
            : depth == 2 ? k_residual<T, 2>
                         : k_residual<T, 1>;
-  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, fine, levs, active, partials);
+  hipLaunchKernelGGL(k, dim3(ntiles), tile_block(), 0, s, tiles, fine, levs, active, partials,
+                     zbuf);
 }
 
 template <class T>
@@ -4034,9 +4057,9 @@ template void launch_fill_zero<double>(hipStream_t, const Tile *, int, const Lev
 template void launch_fill_zero<float>(hipStream_t, const Tile *, int, const LevelDev *,
                                       const int *, int);
 template void launch_residual<double>(hipStream_t, const Tile *, int, int, const FineDev *,
-                                      const LevelDev *, const int *, RedPartial *, int);
+                                      const LevelDev *, const int *, RedPartial *, int, int);
 template void launch_residual<float>(hipStream_t, const Tile *, int, int, const FineDev *,
-                                     const LevelDev *, const int *, RedPartial *, int);
+                                     const LevelDev *, const int *, RedPartial *, int, int);
 template void launch_update<double>(hipStream_t, const Tile *, int, const FineDev *,
                                     const LevelDev *, const int *, int);
 template void launch_update<float>(hipStream_t, const Tile *, int, const FineDev *,

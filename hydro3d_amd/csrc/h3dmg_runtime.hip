@@ -543,13 +543,28 @@ void fill_any(h3dmg_handle *h, bool is64, const TileSet &ts, int buf) {
     launch_fill_zero<float>(h->stream, ts.d, ts.n(), h->d_levs, h->d_active, buf);
 }
 
+// The residual's fast path (enqueue_update_residual) also writes the red half
+// of the level-1 relaxation from zero (each red cell rhs/ap, into buffer 1,
+// where the first half-sweep from buffer 0 puts it): an FP64 level 1 (in
+// mixed precision the FP32 residual slows by more than the half-sweep costs),
+// the red-black smoother, constant coefficients, no neighbours, nu1 > 0, and
+// nu1 + nu2 even (the correction pp then lies in buffer 0 and stays
+// untouched).
+bool red1_write(const h3dmg_handle *h) {
+  const h3dmg_params &P = h->prm;
+  return !is_mixed(h) && P.smoother == H3DMG_SMOOTH_RBGS && !h->lsm && !h->connected &&
+         P.nu1 > 0 && ((P.nu1 + P.nu2) & 1) == 0 && h->fine_tiles.depth > 0;
+}
+
 // Enqueue one V-cycle (mgkcyc, mgsolver.for:272-352) on the level-1 rhs that
-// is already in place.  Per level the schedule is relax(g, nu1) on the way
+// is already in place.  red1: that rhs came from the residual with the red
+// half of the first level-1 sweep (red1_write), so the sweep starts with its
+// black half.  Per level the schedule is relax(g, nu1) on the way
 // down and relax(g, nu2) after mgcorr on the way up; at a block's coarsest
 // level relax(L, nu1) + relax(L, nu2) (the reference) or the exact solve.
 // Sweep counts per level are nu1 + nu2 for every block, so the buffer that
 // holds a level at mgcorr time is the same for all blocks.
-void enqueue_vcycle(h3dmg_handle *h) {
+void enqueue_vcycle(h3dmg_handle *h, bool red1 = false) {
   const int nu1 = h->prm.nu1, nu2 = h->prm.nu2, Lm = h->Lmax;
   const bool mixed = is_mixed(h);
   const bool exact = h->prm.coarse == H3DMG_COARSE_EXACT;
@@ -563,10 +578,13 @@ void enqueue_vcycle(h3dmg_handle *h) {
   for (int g = 1; g <= gdown; ++g) {
     // the restriction into g writes the red half of g's first pre-smoothing
     // sweep (from zero) when that sweep is a split red-black one
-    const bool red_by_restr = g >= 2 && g < Lt && g < Lm && h->prm.nu1 > 0 &&
-                              h->prm.smoother == H3DMG_SMOOTH_RBGS && !h->lsm && !h->connected &&
-                              h->restr_nc[g].n() && h->restr_nc[g].depth > 0 &&
-                              h->relax_nc[g].n() && !h->relax_nc[g].fused && z_nc[g];
+    // (for level 1: the residual wrote it, red1)
+    const bool red_by_restr =
+        (g >= 2 ? h->prm.smoother == H3DMG_SMOOTH_RBGS && !h->lsm && !h->connected &&
+                      h->restr_nc[g].n() && h->restr_nc[g].depth > 0
+                : red1 && red1_write(h)) &&
+        g < Lt && g < Lm && h->prm.nu1 > 0 && h->relax_nc[g].n() && !h->relax_nc[g].fused &&
+        z_nc[g];
     // likewise for the blocks whose coarsest level g is smoothed (not solved)
     const bool red_c_by_restr = g >= 2 && g < Lt && !(exact && g > 1) && h->prm.nu1 > 0 &&
                                 h->prm.smoother == H3DMG_SMOOTH_RBGS && !h->lsm &&
@@ -698,10 +716,10 @@ void enqueue_update_residual(h3dmg_handle *h, bool update) {
                         h->d_partials);
   else if (f64)
     launch_residual<double>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
-                            h->d_partials, 1);
+                            h->d_partials, 1, red1_write(h) ? 1 : -1);
   else
     launch_residual<float>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
-                           h->d_partials, 1);
+                           h->d_partials, 1, red1_write(h) ? 1 : -1);
   launch_finalize(h->stream, h->d_tile_begin, h->nb, h->d_partials, h->d_stats);
 }
 
@@ -2285,7 +2303,8 @@ h3dmg_status decide_setup(h3dmg_handle *h) {
     if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   if (!h->gexec_dec) {
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
-    enqueue_vcycle(h);
+    // every level-1 rhs of h3dmg_solve comes from the residual: red1
+    enqueue_vcycle(h, true);
     enqueue_update_residual(h, true);
     launch_decide(h->stream, (const DecideBlock *)(h->d_dec + dec_part(2, nb)),
                   (DecideState *)(h->d_dec + dec_part(3, nb)),
