@@ -1128,9 +1128,9 @@ h3dmg_status build(h3dmg_handle *h, const h3dmg_block *blocks, int nblocks,
       ts.depth = restrict_depth(h->levs, members, cpl);
       for (int d : members) add_tiles_restrict(ts, d, h->levs[d], ts.depth, cpl);
     }
-  // finest level: residual tiles (one colour-array entry of both colours per
-  // lane, lpr lanes per row, kb planes; kind 0 = y/z interior) and p += pp
-  // tiles (1024 entries each)
+  // finest level: residual tiles (two colour-array entries of both colours
+  // per lane, lpr lanes per row segment, kb planes; kind 0 = y/z interior)
+  // and p += pp tiles (1024 entries each)
   bool fine_general = false;  // a block one cell wide in x: general tiles
   // One depth for every block (one launch): ~2048 workgroups over all
   // blocks, at most 8 planes -- the tiles are software-pipelined (the loads
