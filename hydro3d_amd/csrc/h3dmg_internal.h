@@ -280,9 +280,14 @@ __host__ __device__ inline size_t decide_slot_bytes(int nb) {
   return (sizeof(DecideCtl) + (size_t)nb * (sizeof(DecideState) + sizeof(RedPartial) + sizeof(int)) +
           255) & ~(size_t)255;
 }
-// (prm in device memory: set when a solve starts, the captured graph fixed)
-void launch_decide(hipStream_t s, const DecideBlock *blocks, DecideState *state, DecideCtl *ctl,
-                   const RedPartial *stats, int *active, char *ring, const DecideParams *prm);
+// launch_finalize, then the rule in the last of the nblocks workgroups to
+// finish (prm in device memory: set when a solve starts, the captured graph
+// fixed); *arrived counts the workgroups (zero before the first launch; the
+// last workgroup resets it)
+void launch_finalize_decide(hipStream_t s, const int *tile_begin, int nblocks,
+                            const RedPartial *partials, RedPartial *out, const DecideBlock *blocks,
+                            DecideState *state, DecideCtl *ctl, int *active, char *ring,
+                            const DecideParams *prm, unsigned *arrived);
 
 // *out = max over n partials of RedPartial::amax (one workgroup)
 void launch_partials_max(hipStream_t s, const RedPartial *parts, int n, double *out);

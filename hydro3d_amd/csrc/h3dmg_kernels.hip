@@ -1834,9 +1834,9 @@ __global__ void __launch_bounds__(NTHREADS)
 
 // per-block reduction of the partials of k_residual, fixed order
 constexpr int FIN_THREADS = 1024;
-__global__ void __launch_bounds__(FIN_THREADS)
-    k_finalize(const int *__restrict__ tile_begin, const RedPartial *__restrict__ partials,
-               RedPartial *__restrict__ out) {
+__device__ __forceinline__ void finalize_body(const int *__restrict__ tile_begin,
+                                              const RedPartial *__restrict__ partials,
+                                              RedPartial *__restrict__ out) {
   const int b = blockIdx.x;
   const int t0 = tile_begin[b], t1 = tile_begin[b + 1];
   double ss = 0.0, am = 0.0;
@@ -1858,6 +1858,12 @@ __global__ void __launch_bounds__(FIN_THREADS)
   if (threadIdx.x == 0) out[b] = RedPartial{s_ss[0], s_am[0]};
 }
 
+__global__ void __launch_bounds__(FIN_THREADS)
+    k_finalize(const int *__restrict__ tile_begin, const RedPartial *__restrict__ partials,
+               RedPartial *__restrict__ out) {
+  finalize_body(tile_begin, partials, out);
+}
+
 // The solve loop's stopping rule after cycle ctl->it + 1, as h3dmg_solve's
 // host loop applies it (newsolv_mg.for:57-59 for the reference criterion),
 // in the same order and IEEE operations: per block ||r|| = sqrt(ss), rel =
@@ -1865,12 +1871,10 @@ __global__ void __launch_bounds__(FIN_THREADS)
 // rfac * vol; done when (REF) it >= max(min_cycles, 2) and the global rmax <
 // tol, else it >= min_cycles and rel <= the block's tolerance.  One thread:
 // a handful of blocks, serial in block order like the host's sums.
-__global__ void __launch_bounds__(64)
-    k_decide(const DecideBlock *__restrict__ blocks, DecideState *__restrict__ state,
-             DecideCtl *__restrict__ ctl, const RedPartial *__restrict__ stats,
-             int *__restrict__ active, char *__restrict__ ring,
-             const DecideParams *__restrict__ prm_in) {
-  if (threadIdx.x != 0) return;
+__device__ void decide_body(const DecideBlock *__restrict__ blocks, DecideState *__restrict__ state,
+                            DecideCtl *__restrict__ ctl, const RedPartial *__restrict__ stats,
+                            int *__restrict__ active, char *__restrict__ ring,
+                            const DecideParams *__restrict__ prm_in) {
   const DecideParams prm = *prm_in;
   const int nb = prm.nb;
   const int it = ctl->it + 1;
@@ -1927,6 +1931,25 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// k_finalize followed by the stopping rule in one launch: each block's
+// workgroup writes its statistics, and the last workgroup to finish (one
+// counter, reset by that workgroup for the next launch) applies the rule to
+// all of them.
+__global__ void __launch_bounds__(FIN_THREADS)
+    k_finalize_decide(const int *__restrict__ tile_begin, const RedPartial *__restrict__ partials,
+                      RedPartial *__restrict__ out, const DecideBlock *__restrict__ blocks,
+                      DecideState *__restrict__ state, DecideCtl *__restrict__ ctl,
+                      int *__restrict__ active, char *__restrict__ ring,
+                      const DecideParams *__restrict__ prm, unsigned *__restrict__ arrived) {
+  finalize_body(tile_begin, partials, out);
+  if (threadIdx.x != 0) return;
+  __threadfence();  // this block's statistics before the count
+  const unsigned n = gridDim.x;
+  if (atomicAdd(arrived, 1u) != n - 1) return;
+  __threadfence();  // every block's statistics after it
+  *arrived = 0u;
+  decide_body(blocks, state, ctl, out, active, ring, prm);
+}
 // ---------------------------------------------------------------------------
 // Exact FP64 solve of the coarsest level of one block (extension, no
 // reference counterpart): A = Dx (+) Dy (+) Dz with Dx the 1-D Neumann or
@@ -3890,10 +3913,12 @@ void launch_update(hipStream_t s, const Tile *tiles, int ntiles, const FineDev *
                      buf);
 }
 
-void launch_decide(hipStream_t s, const DecideBlock *blocks, DecideState *state, DecideCtl *ctl,
-                   const RedPartial *stats, int *active, char *ring, const DecideParams *prm) {
-  hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, s, blocks, state, ctl, stats, active, ring,
-                     prm);
+void launch_finalize_decide(hipStream_t s, const int *tile_begin, int nblocks,
+                            const RedPartial *partials, RedPartial *out, const DecideBlock *blocks,
+                            DecideState *state, DecideCtl *ctl, int *active, char *ring,
+                            const DecideParams *prm, unsigned *arrived) {
+  hipLaunchKernelGGL(k_finalize_decide, dim3(nblocks), dim3(FIN_THREADS), 0, s, tile_begin,
+                     partials, out, blocks, state, ctl, active, ring, prm, arrived);
 }
 
 void launch_finalize(hipStream_t s, const int *tile_begin, int nblocks, const RedPartial *partials,

@@ -702,7 +702,8 @@ void enqueue_vcycle(h3dmg_handle *h, bool red1 = false) {
 }
 
 // p += pp, r = f - A p, per-block ||r||, max|r| (one outer iteration tail)
-void enqueue_update_residual(h3dmg_handle *h, bool update) {
+// finalize = false: the caller reduces the partials itself
+void enqueue_update_residual(h3dmg_handle *h, bool update, bool finalize = true) {
   const bool f64 = !is_mixed(h);
   const TileSet &ft = h->fine_tiles, &ut = h->update_tiles;
   if (update) {
@@ -720,7 +721,7 @@ void enqueue_update_residual(h3dmg_handle *h, bool update) {
   else
     launch_residual<float>(h->stream, ft.d, ft.n(), ft.depth, h->d_fine, h->d_levs, h->d_active,
                            h->d_partials, 1, red1_write(h) ? 1 : -1);
-  launch_finalize(h->stream, h->d_tile_begin, h->nb, h->d_partials, h->d_stats);
+  if (finalize) launch_finalize(h->stream, h->d_tile_begin, h->nb, h->d_partials, h->d_stats);
 }
 
 h3dmg_status read_stats(h3dmg_handle *h) {
@@ -2305,11 +2306,15 @@ h3dmg_status decide_setup(h3dmg_handle *h) {
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
     // every level-1 rhs of h3dmg_solve comes from the residual: red1
     enqueue_vcycle(h, true);
-    enqueue_update_residual(h, true);
-    launch_decide(h->stream, (const DecideBlock *)(h->d_dec + dec_part(2, nb)),
-                  (DecideState *)(h->d_dec + dec_part(3, nb)),
-                  (DecideCtl *)(h->d_dec + dec_part(1, nb)), h->d_stats, h->d_active, h->d_ring,
-                  (const DecideParams *)(h->d_dec + dec_part(0, nb)));
+    enqueue_update_residual(h, true, false);
+    // the partials' reduction and the stopping rule in one launch; its
+    // arrival counter shares the 256-byte slot of DecideCtl, a line apart
+    launch_finalize_decide(h->stream, h->d_tile_begin, nb, h->d_partials, h->d_stats,
+                           (const DecideBlock *)(h->d_dec + dec_part(2, nb)),
+                           (DecideState *)(h->d_dec + dec_part(3, nb)),
+                           (DecideCtl *)(h->d_dec + dec_part(1, nb)), h->d_active, h->d_ring,
+                           (const DecideParams *)(h->d_dec + dec_part(0, nb)),
+                           (unsigned *)(h->d_dec + dec_part(1, nb) + 128));
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamEndCapture(h->stream, &g));
     h->graph_dec = g;
@@ -2338,7 +2343,7 @@ bool likely_last(const h3dmg_handle *h, int n, const std::vector<double> &r1,
 }
 
 // h3dmg_solve's cycles with the next cycle queued before the host reads the
-// last one's outcome: the stopping rule runs on the device (k_decide) and
+// last one's outcome: the stopping rule runs on the device (k_finalize_decide) and
 // writes the flags the queued cycle reads, so the GPU does not wait for the
 // host between cycles.  A cycle is queued ahead unless the residual history
 // says the one in flight is likely the last; one queued after the last runs
